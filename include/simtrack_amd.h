@@ -1,0 +1,231 @@
+/*
+ * simtrack_amd.h -- C ABI of the MI355X tracking engine.
+ *
+ * This is the drop-in boundary for SimTrack's tracking hot path.  The
+ * reference has no FFI: its "plugin" interface is the abstract C++ class
+ * Element (simtrack.h:373-396, virtual Pass/sPass) and its operator is the
+ * free function Track(Line&, double x[], nturn, stable, lost_turn,
+ * lost_post) (simtrack.h:1360, simtrack.cpp:20791).  Here a Line is
+ * flattened into an array of st_element records (one per Element, same
+ * parameter names as the reference's SetP/GetP), uploaded once as a packed
+ * device table, and whole particle sets are tracked per call.
+ *
+ * Conventions (all identical to the reference):
+ *   - particle state x[6] = (x, px, y, py, z = -c dt, pt = dE/(p0 c))
+ *     (simtrack.cpp:11-21); spin runs use x[9] with (Sx, Sy, Ss) in x[6..8];
+ *     arrays are particle-major (AoS), np x 6 or np x 9 doubles.
+ *   - loss: before turn 0, |x| > 0.1 or |y| > 0.1 or stable == 0 marks the
+ *     particle lost at turn 0, element 0 (simtrack.cpp:20795-20796); after
+ *     every element, |x| > APx or |y| > APy or isnan(x) or isnan(y) stops it
+ *     with lost_turn = turn, lost_post = element index, x left at the loss
+ *     point (simtrack.cpp:20802-20803).  Surviving particles keep their
+ *     input lost_turn/lost_post values, as the reference leaves them.
+ *   - errors are negative return codes with a message from st_last_error();
+ *     nothing here exits the process (the reference calls exit()).
+ *
+ * Threading: calls on different contexts may run concurrently; calls on one
+ * context are serialised by the caller.  All calls block until the GPU work
+ * they issue has finished, except the *_device entry points, which are
+ * ordered on the context's stream and return after launching.
+ */
+#ifndef SIMTRACK_AMD_H
+#define SIMTRACK_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ST_ABI_VERSION 1
+
+/* ------------------------------------------------------------ status */
+enum st_status {
+    ST_OK = 0,
+    ST_ERR_ARG = -1,          /* bad argument (null pointer, size, index)       */
+    ST_ERR_TYPE = -2,         /* unknown or unsupported element type            */
+    ST_ERR_PARAM = -3,        /* element parameter out of the supported range   */
+    ST_ERR_DEVICE = -4,       /* HIP runtime error                              */
+    ST_ERR_NOMEM = -5,        /* host or device allocation failed               */
+    ST_ERR_STATE = -6         /* call not valid in the current state            */
+};
+
+/* ----------------------------------------------------- element types */
+/* Element classes of simtrack.h:397-1059 that the engine tracks. */
+enum st_element_type {
+    ST_DRIFT = 1,      /* DRIFT      cpp:8399  */
+    ST_SBEND = 2,      /* SBEND      cpp:8441  */
+    ST_GSBEND = 3,     /* GSBEND     cpp:8529  */
+    ST_QUAD = 4,       /* QUAD       cpp:8626  knl[1]=K1L knsl[1]=K1SL */
+    ST_SKEWQ = 5,      /* SKEWQ      cpp:8706  knsl[1]=K1SL */
+    ST_SEXT = 6,       /* SEXT       cpp:8773  knl[2]=K2L knsl[2]=K2SL */
+    ST_OCT = 7,        /* OCT        cpp:8853  knl[3]=K3L knsl[3]=K3SL */
+    ST_MULT = 8,       /* MULT       cpp:8933  */
+    ST_GMULT = 9,      /* GMULT      cpp:9046  */
+    ST_SBENDMULT = 10, /* SBENDMULT  cpp:9192  */
+    ST_SMULT = 11,     /* SMULT      cpp:9330  */
+    ST_GSBENDMULT = 12,/* GSBENDMULT cpp:9483  */
+    ST_KICKER = 13,    /* KICKER     cpp:9912  */
+    ST_HKICKER = 14,   /* HKICKER    cpp:9982  */
+    ST_VKICKER = 15,   /* VKICKER    cpp:10043 */
+    ST_MARKER = 16,    /* MARKER     cpp:10590 */
+    ST_BPM = 17,       /* BPM        cpp:10470 */
+    ST_HBPM = 18,      /* HBPM       cpp:10510 */
+    ST_VBPM = 19,      /* VBPM       cpp:10550 */
+    ST_RFCAV = 20,     /* RFCAV      cpp:10630 */
+    ST_CRABRF = 21,    /* CRABRF     cpp:10719 */
+    ST_CCMULT = 22,    /* CCMULT     cpp:10815 cc[0..7]=B1,A1,B2,A2,B3,A3,B4,A4 */
+    ST_LBT = 23,       /* LBT        cpp:10960 */
+    ST_ILBT = 24,      /* ILBT       cpp:11009 */
+    ST_BEAMBEAM = 25,  /* BEAMBEAM   cpp:11058 */
+    ST_CRBB = 26,      /* CRBB       cpp:11316 */
+    ST_MATRIX = 27,    /* MATRIX     cpp:9774  */
+    ST_ROTAT = 28,     /* ROTAT      cpp:11784 (spin rotator; orbit: drift)   */
+    ST_SNAKE = 29      /* SNAKE      cpp:11864 (spin rotator; orbit: drift)   */
+};
+
+/*
+ * One Element.  Field names follow the reference's member and SetP names;
+ * unused fields must be zero.  nint/norder carry the values GetP("Nint") and
+ * GetP("Norder") return (constructor defaults: Nint = int(L/slice)+1,
+ * simtrack.cpp:8451, 8635; Norder = highest non-zero order at construction,
+ * cpp:8944-8950, and SetP("KnL") does not refresh it).
+ */
+typedef struct st_element {
+    int32_t type;           /* enum st_element_type                          */
+    int32_t nint;           /* Nint                                          */
+    int32_t norder;         /* Norder (multipole families, 1..10)            */
+    int32_t nslice;         /* NSLICE (BEAMBEAM, CRBB)                       */
+    int32_t treatment;      /* TREATMENT (BEAMBEAM: 4 or 6)                  */
+    int32_t reserved[3];
+    double l;               /* L                                             */
+    double angle;           /* ANGLE                                         */
+    double e1, e2;          /* E1, E2                                        */
+    double k0l;             /* K0L (GSBEND, GSBENDMULT)                      */
+    double knl[11];         /* KNL[0..10]  (QUAD K1L in knl[1], ...)         */
+    double knsl[11];        /* KNSL[0..10]                                   */
+    double dx, dy, ds;      /* DX, DY, DS misalignment                       */
+    double dtheta, dphi, dpsi; /* DTHETA, DPHI, DPSI                         */
+    double apx, apy;        /* APx, APy (reference default 1.0)              */
+    double vrf, frf, phase0;/* RFCAV/CRABRF/CCMULT: VRF [MV], FRF [Hz], PHASE0 */
+    double hkick, vkick;    /* KICKER family                                 */
+    double theta;           /* LBT/ILBT/CRBB THETA                           */
+    double np, bbscale, sigmal;             /* beam-beam NP, BBSCALE, SIGMAL  */
+    double emitx, betax, alfax;             /* EMITX, BETAX, ALFAX            */
+    double emity, betay, alfay;             /* EMITY, BETAY, ALFAY            */
+    double cc[8];           /* CCMULT B1,A1,B2,A2,B3,A3,B4,A4                */
+    double rot_n[3];        /* ROTAT/SNAKE rotation axis n[3]                */
+    double rot_angle;       /* ROTAT/SNAKE angle                             */
+    double m66[36];         /* MATRIX M66 (row major)                        */
+    double xco_in[6], xco_out[6]; /* MATRIX XCO_IN, XCO_OUT                  */
+} st_element;
+
+/*
+ * Reference-particle state: the fields of GlobalVariables GP
+ * (simtrack.h:28-68) that the element maps read.  st_globals_set_energy
+ * fills them exactly as Set_RefPartEnergy (simtrack.cpp:87-108) does.
+ */
+typedef struct st_globals {
+    double gamma, beta;     /* GP.gamma, GP.beta                             */
+    double energy0, energy; /* GP.energy0, GP.energy [MeV]                   */
+    double cp, p, m0, m, q; /* GP.cp, GP.p, GP.m0, GP.m, GP.q                */
+    double brho;            /* GP.brho                                       */
+    double G;               /* GP.G (spin)                                   */
+    int32_t Q, A;           /* GP.Q, GP.A (int, as in the reference)         */
+    int32_t H_expand;       /* GP.H_expand                                   */
+    int32_t radiate;        /* GP.radiate                                    */
+    int32_t quad_fringe;    /* GP.quad_fringe                                */
+    int32_t reserved;
+    /* radiation runs (GP.radiate): seed of the per-particle counter-based
+       normal stream that replaces gsl_ran_gaussian(gsl_r2, 1.0) */
+    uint64_t rng_seed;
+} st_globals;
+
+/* GP defaults (simtrack.h:31-58) followed by Set_RefPartEnergy(gamma, E0, Q, A). */
+void st_globals_default(st_globals *gp);
+void st_globals_set_energy(st_globals *gp, double gamma, double E0, double Q, double A);
+
+/* --------------------------------------------------- library/context */
+typedef struct st_ctx st_ctx;
+typedef struct st_table st_table;
+
+int         st_abi_version(void);
+/* Message of the last failing call on this thread ("" if none). */
+const char *st_last_error(void);
+
+/* Create a context on HIP device `device` (own stream and buffers). */
+int st_ctx_create(int device, st_ctx **out);
+int st_ctx_destroy(st_ctx *ctx);
+
+/*
+ * Build the packed device table of a Line (replaces walking
+ * linename.Cell[j] in Track, simtrack.cpp:20800).  The globals are copied:
+ * later changes to *gp do not affect the table.  Element constants that the
+ * reference recomputes per call (Fkick*KNL/Nint, slice lengths, cos/sin of
+ * bend angles, beam-beam slice centroids, ...) are evaluated here once with
+ * the reference's own expressions, so results are unchanged bit for bit.
+ */
+int st_table_build(st_ctx *ctx, const st_globals *gp, const st_element *elems,
+                   int n_elems, st_table **out);
+int st_table_destroy(st_table *t);
+int st_table_size(const st_table *t, int *n_elems, int64_t *bytes);
+
+/*
+ * Track (simtrack.cpp:20791-20814) for np particles at once: x is np x 6
+ * (in/out), stable/lost_turn/lost_post are np ints (in/out).  Host arrays.
+ */
+int st_track(st_table *t, double *x, int np, int nturn,
+             int *stable, int *lost_turn, int *lost_post);
+
+/* Track_spin / Track_Spin (simtrack.cpp:20816, 21766): x is np x 9. */
+int st_track_spin(st_table *t, double *x, int np, int nturn,
+                  int *stable, int *lost_turn, int *lost_post);
+
+/*
+ * Track_tbt (simtrack.cpp:20841): also stores x at the start of every turn,
+ * x_tbt[(p*nturn + turn)*6 + k] (np x nturn x 6, caller-allocated; rows of
+ * turns after a loss are left untouched).  With bpm_index >= 0 the overload
+ * of cpp:20885 is followed instead: x is stored after element bpm_index.
+ * spin != 0 gives Track_tbt_spin (cpp:20863, 20909) with 9 columns.
+ */
+int st_track_tbt(st_table *t, double *x, int np, int nturn, double *x_tbt,
+                 int bpm_index, int spin,
+                 int *stable, int *lost_turn, int *lost_post);
+
+/*
+ * Track_Radial (simtrack.cpp:21641) and Example2's Track_Radial_Test
+ * (Example2/DA_cal.cpp:19-57): the on-momentum path length z after every
+ * element of one pass from x = 0 is subtracted from z after that element.
+ */
+int st_track_radial(st_table *t, double *x, int np, int nturn,
+                    int *stable, int *lost_turn, int *lost_post);
+
+/*
+ * Element::Pass / sPass of one element (simtrack.h:380-382) applied to np
+ * particles (x np x 6, or np x 9 with spin != 0).  turn is GP.turn.
+ */
+int st_element_pass(st_table *t, int elem_index, double *x, int np, int spin);
+
+/*
+ * Device-resident tracking of a turn range, for callers that keep particle
+ * state in HBM (device pointers from hipMalloc or a framework allocator):
+ * d_x is np x 6 (x 9 with spin), d_state is np ints holding stable, d_lost
+ * np x 2 ints (lost_turn, lost_post).  Turns turn0 .. turn0+nturn-1 are
+ * tracked; the 0.1 m pre-check is applied only when turn0 == 0, so a run
+ * split into ranges equals one call.  flags: ST_TRACK_SPIN, ST_TRACK_RADIAL.
+ * Returns after launching on the context stream; st_ctx_sync waits.
+ */
+#define ST_TRACK_SPIN 1
+#define ST_TRACK_RADIAL 2
+int st_track_device(st_table *t, double *d_x, int np, int turn0, int nturn,
+                    int *d_state, int *d_lost, int flags);
+int st_ctx_sync(st_ctx *ctx);
+
+/* Seconds spent in the last blocking call's GPU work (HIP events). */
+int st_last_gpu_seconds(st_ctx *ctx, double *seconds);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIMTRACK_AMD_H */

@@ -1,0 +1,14 @@
+#!/bin/sh
+# Build the CPU oracle (test infrastructure; see st_oracle.c).
+#   liboracle.so        project libm (st_libm.h): bit-comparable with the GPU
+#   liboracle_glibc.so  C library libm: what the reference itself calls
+set -e
+here=$(cd "$(dirname "$0")" && pwd)
+root=$(dirname "$here")
+mkdir -p "$here/build"
+CFLAGS="-std=gnu99 -O2 -ffp-contract=off -mfma -fopenmp -fPIC -shared -fvisibility=hidden -Wall -Wno-unused-function"
+INC="-I$root/include -I$root/simtrack_amd/csrc"
+gcc $CFLAGS $INC "$here/st_oracle.c" -o "$here/build/liboracle.so.tmp" -lm
+mv "$here/build/liboracle.so.tmp" "$here/build/liboracle.so"
+gcc $CFLAGS -DORACLE_GLIBC $INC "$here/st_oracle.c" -o "$here/build/liboracle_glibc.so.tmp" -lm
+mv "$here/build/liboracle_glibc.so.tmp" "$here/build/liboracle_glibc.so"

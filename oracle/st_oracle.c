@@ -1,0 +1,2088 @@
+/*
+ * st_oracle.c -- CPU oracle for the SimTrack tracking hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: the parity tests,
+ * the smoke check and the CPU baseline leg of bench.py load it; the
+ * product (simtrack_amd) never links it.
+ *
+ * It restates, in plain C99, the reference's element maps and tracking
+ * drivers operation for operation (same expression trees, same evaluation
+ * order, per-call recomputation of element constants exactly where the
+ * reference recomputes them).  Each function cites the simtrack.cpp lines
+ * it follows.  The reference itself cannot be built here: simtrack.h
+ * includes GSL headers (gsl_vector.h, gsl_eigen.h, gsl_cdf.h, ...) that
+ * the image does not have.
+ *
+ * Elementary functions: compiled with -DORACLE_GLIBC the oracle calls the
+ * C library (what the reference calls); by default it calls the project
+ * libm st_libm.h, the same implementation the GPU kernels use, so GPU and
+ * oracle can be compared bit for bit.  The two builds are compared with
+ * each other in the test suite.
+ *
+ * Build: oracle/build.sh (gcc -O2 -ffp-contract=off -mfma -fopenmp).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "simtrack_amd.h"
+
+#define ST_FN static inline
+#define ST_CONST_TABLE static const
+#define ST_FMA(a, b, c) fma((a), (b), (c))
+#define ST_RINT(x) nearbyint(x)
+#define ST_SQRT(x) sqrt(x)
+#include "st_libm.h"
+#include "st_rng.h"
+
+#ifdef ORACLE_GLIBC
+#define OSIN(x) sin(x)
+#define OCOS(x) cos(x)
+#define OTAN(x) tan(x)
+#define OASIN(x) asin(x)
+#define OEXP(x) exp(x)
+#define OPOW(x, n) pow((x), (double)(n))
+#else
+#define OSIN(x) st_sin(x)
+#define OCOS(x) st_cos(x)
+#define OTAN(x) st_tan(x)
+#define OASIN(x) st_asin(x)
+#define OEXP(x) st_exp(x)
+#define OPOW(x, n) st_powi((x), (n))
+#endif
+
+#define EXPORT __attribute__((visibility("default")))
+
+enum { x_ = 0, px_ = 1, y_ = 2, py_ = 3, z_ = 4, pt_ = 5 };
+
+/* --------------------------------------------------------- globals */
+/* simtrack.cpp:23-32 */
+static const double PI = 3.14159265358979323;
+static const double speed_light = 299792458.;
+static const double Cr = 8.84627632682e-5;
+static double Cf;
+static double Fdrift1, Fdrift2, Fkick1, Fkick2;
+static st_globals GP;
+static int g_init = 0;
+
+static void orc_init_constants(void)
+{
+    /* simtrack.cpp:25-32 (pow/sqrt of constants, evaluated once at start-up) */
+    Cf = (55. / 24. / sqrt(3.)) * 2.8179403e-15 * 1.054572e-34 / 9.1093822e-31;
+    Fdrift1 = 1.e0 / (2.e0 * (2.e0 - pow(2.e0, 1.0 / 3.0)));
+    Fdrift2 = 0.5e0 - Fdrift1;
+    Fkick1 = 2.e0 * Fdrift1;
+    Fkick2 = 1.e0 - 2e0 * Fkick1;
+    g_init = 1;
+}
+
+EXPORT void orc_set_globals(const st_globals *gp)
+{
+    if (!g_init)
+        orc_init_constants();
+    GP = *gp;
+}
+
+EXPORT void orc_get_yoshida(double out[4])
+{
+    if (!g_init)
+        orc_init_constants();
+    out[0] = Fdrift1;
+    out[1] = Fdrift2;
+    out[2] = Fkick1;
+    out[3] = Fkick2;
+}
+
+/* ---------------------------------------------------- misalignment */
+
+/* GtoL1 / LtoG1, simtrack.cpp:3443-3481 */
+static void GtoL1(double x[], double DX, double DY, double DT)
+{
+    double xtemp[6];
+    if (fabs(DX) + fabs(DY) > 1.0e-10) {
+        x[0] = x[0] - DX;
+        x[2] = x[2] - DY;
+    }
+    if (fabs(DT) > 1.0e-10) {
+        double cosT = OCOS(DT), sinT = OSIN(DT);
+        for (int i = 0; i < 6; i++) xtemp[i] = x[i];
+        x[0] = cosT * xtemp[0] + sinT * xtemp[2];
+        x[1] = cosT * xtemp[1] + sinT * xtemp[3];
+        x[2] = cosT * xtemp[2] - sinT * xtemp[0];
+        x[3] = cosT * xtemp[3] - sinT * xtemp[1];
+    }
+}
+
+static void LtoG1(double x[], double DX, double DY, double DT)
+{
+    double xtemp[6];
+    if (fabs(DT) > 1.0e-10) {
+        double cosT = OCOS(DT), sinT = OSIN(DT);
+        for (int i = 0; i < 6; i++) xtemp[i] = x[i];
+        x[0] = cosT * xtemp[0] - sinT * xtemp[2];
+        x[1] = cosT * xtemp[1] - sinT * xtemp[3];
+        x[2] = sinT * xtemp[0] + cosT * xtemp[2];
+        x[3] = sinT * xtemp[1] + cosT * xtemp[3];
+    }
+    if (fabs(DX) + fabs(DY) > 1.0e-10) {
+        x[0] = x[0] + DX;
+        x[2] = x[2] + DY;
+    }
+}
+
+/* Cal_W_Matrix, simtrack.cpp:3485-3503 */
+static void Cal_W_Matrix(double theta, double phi, double psi, double W[])
+{
+    double costhe = OCOS(theta), cosphi = OCOS(phi), cospsi = OCOS(psi);
+    double sinthe = OSIN(theta), sinphi = OSIN(phi), sinpsi = OSIN(psi);
+    W[0] = +costhe * cospsi - sinthe * sinphi * sinpsi;
+    W[1] = -costhe * sinpsi - sinthe * sinphi * cospsi;
+    W[2] = sinthe * cosphi;
+    W[3] = cosphi * sinpsi;
+    W[4] = cosphi * cospsi;
+    W[5] = sinphi;
+    W[6] = -sinthe * cospsi - costhe * sinphi * sinpsi;
+    W[7] = +sinthe * sinpsi - costhe * sinphi * cospsi;
+    W[8] = costhe * cosphi;
+}
+
+/* Cal_VE_WE_Matrx, simtrack.cpp:3505-3565 */
+static void Cal_VE_WE_Matrx(double length, double angle, double tilt, double VE[], double WE[])
+{
+    double temp, cospsi, costhe, sinpsi, sinthe;
+    if (fabs(angle) < 1.0e-12) {
+        for (int i = 0; i < 3; i++) VE[i] = 0.;
+        VE[2] = length;
+        for (int i = 0; i < 9; i++) WE[i] = 0.;
+        for (int i = 0; i < 3; i++) WE[i * 3 + i] = 1.0;
+    }
+    if (fabs(angle) > 1.0e-12) {
+        tilt = 0.;
+        cospsi = OCOS(tilt);
+        sinpsi = OSIN(tilt);
+        costhe = OCOS(angle);
+        sinthe = OSIN(angle);
+        temp = length * (costhe - 1.0) / angle;
+        VE[0] = temp * cospsi;
+        VE[1] = temp * sinpsi;
+        VE[2] = length * OSIN(angle) / angle;
+        WE[0 * 3 + 0] = costhe * cospsi * cospsi + sinpsi * sinpsi;
+        WE[1 * 3 + 0] = (costhe - 1.0) * cospsi * sinpsi;
+        WE[2 * 3 + 0] = sinthe * cospsi;
+        WE[0 * 3 + 1] = WE[1 * 3 + 0];
+        WE[1 * 3 + 1] = costhe * sinpsi * sinpsi + cospsi * cospsi;
+        WE[2 * 3 + 1] = sinthe * sinpsi;
+        WE[0 * 3 + 2] = -WE[2 * 3 + 0];
+        WE[1 * 3 + 2] = -WE[2 * 3 + 1];
+        WE[2 * 3 + 2] = costhe;
+    }
+}
+
+/* mat_mult (simtrack.cpp:1351-1362), mat_transpose (cpp:1406-1416) */
+static void mat_mult(const double a[], const double b[], double c[], int m, int n, int k)
+{
+    for (int i = 0; i <= m - 1; i++)
+        for (int j = 0; j <= k - 1; j++) {
+            int u = i * k + j;
+            c[u] = 0.0;
+            for (int l = 0; l <= n - 1; l++) c[u] = c[u] + a[i * n + l] * b[l * k + j];
+        }
+}
+
+static void mat_transpose(const double a[], double at[], int m, int n)
+{
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < m; j++) at[i * m + j] = a[j * n + i];
+}
+
+/* GtoL2, simtrack.cpp:3567-3626 (MAD-X tmali1) */
+static void GtoL2(double x[], double length, double angle, double dx, double dy, double ds,
+                  double theta, double phi, double psi)
+{
+    double s2, W[9], RM[36], xtemp[6];
+    (void)length;
+    (void)angle;
+    Cal_W_Matrix(theta, phi, psi, W);
+    s2 = (W[0 * 3 + 2] * dx + W[1 * 3 + 2] * dy + W[2 * 3 + 2] * ds) / W[2 * 3 + 2];
+    for (int i = 0; i < 36; i++) RM[i] = 0.;
+    for (int i = 0; i < 6; i++) RM[i * 6 + i] = 1.0;
+    RM[1 * 6 + 1] = W[0 * 3 + 0];
+    RM[1 * 6 + 3] = W[1 * 3 + 0];
+    RM[1 * 6 + 5] = W[2 * 3 + 0] / GP.beta;
+    RM[3 * 6 + 1] = W[0 * 3 + 1];
+    RM[3 * 6 + 3] = W[1 * 3 + 1];
+    RM[3 * 6 + 5] = W[2 * 3 + 1] / GP.beta;
+    RM[0 * 6 + 0] = W[1 * 3 + 1] / W[2 * 3 + 2];
+    RM[0 * 6 + 1] = RM[0 * 6 + 0] * s2;
+    RM[0 * 6 + 2] = -W[0 * 3 + 1] / W[2 * 3 + 2];
+    RM[0 * 6 + 3] = RM[0 * 6 + 2] * s2;
+    RM[2 * 6 + 0] = -W[1 * 3 + 0] / W[2 * 3 + 2];
+    RM[2 * 6 + 1] = RM[2 * 6 + 0] * s2;
+    RM[2 * 6 + 2] = W[0 * 3 + 0] / W[2 * 3 + 2];
+    RM[2 * 6 + 3] = RM[2 * 6 + 2] * s2;
+    RM[4 * 6 + 0] = W[0 * 3 + 2] / (W[2 * 3 + 2] * GP.beta);
+    RM[4 * 6 + 1] = RM[4 * 6 + 0] * s2;
+    RM[4 * 6 + 2] = W[1 * 3 + 2] / (W[2 * 3 + 2] * GP.beta);
+    RM[4 * 6 + 3] = RM[4 * 6 + 2] * s2;
+    RM[4 * 6 + 5] = -s2 / (GP.beta * GP.gamma * GP.beta * GP.gamma);
+    for (int i = 0; i < 6; i++) {
+        xtemp[i] = 0.;
+        for (int j = 0; j < 6; j++) xtemp[i] = xtemp[i] + RM[i * 6 + j] * x[j];
+    }
+    x[0] = xtemp[0] - (W[1 * 3 + 1] * dx - W[0 * 3 + 1] * dy) / W[2 * 3 + 2];
+    x[1] = xtemp[1] + W[2 * 3 + 0];
+    x[2] = xtemp[2] - (W[0 * 3 + 0] * dy - W[1 * 3 + 0] * dx) / W[2 * 3 + 2];
+    x[3] = xtemp[3] + W[2 * 3 + 1];
+    x[4] = xtemp[4] - s2 / GP.beta;
+    x[5] = xtemp[5];
+}
+
+/* LtoG2, simtrack.cpp:3628-3699 (MAD-X tmali2) */
+static void LtoG2(double x[], double length, double angle, double dx, double dy, double ds,
+                  double theta, double phi, double psi)
+{
+    double V[3], W[9], VE[3], WE[9], V1[3], M1[9], M2[9], RM[36], s2, xtemp[6];
+    double tilt = 0.;
+    Cal_W_Matrix(theta, phi, psi, W);
+    Cal_VE_WE_Matrx(length, angle, tilt, VE, WE);
+    V[0] = dx + W[0 * 3 + 0] * VE[0] + W[0 * 3 + 1] * VE[1] + W[0 * 3 + 2] * VE[2] - VE[0];
+    V[1] = dy + W[1 * 3 + 0] * VE[0] + W[1 * 3 + 1] * VE[1] + W[1 * 3 + 2] * VE[2] - VE[1];
+    V[2] = ds + W[2 * 3 + 0] * VE[0] + W[2 * 3 + 1] * VE[1] + W[2 * 3 + 2] * VE[2] - VE[2];
+    mat_transpose(WE, M1, 3, 3);
+    for (int i = 0; i < 3; i++) {
+        V1[i] = 0;
+        for (int j = 0; j < 3; j++) V1[i] = V1[i] + M1[i * 3 + j] * V[j];
+    }
+    for (int i = 0; i < 3; i++) V[i] = V1[i];
+    mat_mult(M1, W, M2, 3, 3, 3);
+    mat_mult(M2, WE, W, 3, 3, 3);
+    s2 = -(W[0 * 3 + 2] * V[0] + W[1 * 3 + 2] * V[1] + W[2 * 3 + 2] * V[2]) / W[2 * 3 + 2];
+    for (int i = 0; i < 36; i++) RM[i] = 0.;
+    for (int i = 0; i < 6; i++) RM[i * 6 + i] = 1.0;
+    RM[0 * 6 + 0] = W[0 * 3 + 0];
+    RM[2 * 6 + 0] = W[1 * 3 + 0];
+    RM[4 * 6 + 0] = W[2 * 3 + 0] / GP.beta;
+    RM[0 * 6 + 2] = W[0 * 3 + 1];
+    RM[2 * 6 + 2] = W[1 * 3 + 1];
+    RM[4 * 6 + 2] = W[2 * 3 + 1] / GP.beta;
+    RM[1 * 6 + 1] = W[1 * 3 + 1] / W[2 * 3 + 2];
+    RM[0 * 6 + 1] = RM[1 * 6 + 1] * s2;
+    RM[3 * 6 + 1] = -W[0 * 3 + 1] / W[2 * 3 + 2];
+    RM[2 * 6 + 1] = RM[3 * 6 + 1] * s2;
+    RM[1 * 6 + 3] = -W[1 * 3 + 0] / W[2 * 3 + 2];
+    RM[0 * 6 + 3] = RM[1 * 6 + 3] * s2;
+    RM[3 * 6 + 3] = W[0 * 3 + 0] / W[2 * 3 + 2];
+    RM[2 * 6 + 3] = RM[3 * 6 + 3] * s2;
+    RM[1 * 6 + 5] = W[0 * 3 + 2] / (W[2 * 3 + 2] * GP.beta);
+    RM[0 * 6 + 5] = RM[1 * 6 + 5] * s2;
+    RM[3 * 6 + 5] = W[1 * 3 + 2] / (W[2 * 3 + 2] * GP.beta);
+    RM[2 * 6 + 5] = RM[3 * 6 + 5] * s2;
+    RM[4 * 6 + 5] = -s2 / (GP.beta * GP.gamma * GP.beta * GP.gamma);
+    xtemp[0] = x[0] + (W[1 * 3 + 1] * V[0] - W[0 * 3 + 1] * V[1]) / W[2 * 3 + 2];
+    xtemp[1] = x[1] - W[2 * 3 + 0];
+    xtemp[2] = x[2] + (W[0 * 3 + 0] * V[1] - W[1 * 3 + 0] * V[0]) / W[2 * 3 + 2];
+    xtemp[3] = x[3] - W[2 * 3 + 1];
+    xtemp[4] = x[4] - s2 / GP.beta;
+    xtemp[5] = x[5];
+    for (int i = 0; i < 6; i++) {
+        x[i] = 0.;
+        for (int j = 0; j < 6; j++) x[i] = x[i] + RM[i * 6 + j] * xtemp[j];
+    }
+}
+
+/* GtoL / LtoG, simtrack.cpp:3701-3741 */
+static void GtoL(double x[], double length, double angle, double dx, double dy, double ds,
+                 double theta, double phi, double psi)
+{
+    double temp = fabs(dx) + fabs(dy) + fabs(ds) + fabs(theta) + fabs(phi) + fabs(psi);
+    if (temp < 1.0e-9) return;
+    if (fabs(theta) + fabs(phi) < 1.0e-9)
+        GtoL1(x, dx, dy, psi);
+    else
+        GtoL2(x, length, angle, dx, dy, ds, theta, phi, psi);
+}
+
+static void LtoG(double x[], double length, double angle, double dx, double dy, double ds,
+                 double theta, double phi, double psi)
+{
+    double temp = fabs(dx) + fabs(dy) + fabs(ds) + fabs(theta) + fabs(phi) + fabs(psi);
+    if (temp < 1.0e-9) return;
+    if (fabs(theta) + fabs(phi) < 1.0e-9)
+        LtoG1(x, dx, dy, psi);
+    else
+        LtoG2(x, length, angle, dx, dy, ds, theta, phi, psi);
+}
+
+/* -------------------------------------------------------- primitives */
+
+/* DRIFT_Pass, simtrack.cpp:3743-3766 */
+static void DRIFT_Pass(double x[], double L)
+{
+    if (GP.H_expand) {
+        double u = L / (1.0 + x[5]);
+        x[x_] = x[x_] + x[px_] * u;
+        x[y_] = x[y_] + x[py_] * u;
+        x[z_] = x[z_] - (x[px_] * x[px_] + x[py_] * x[py_]) * u / 2.0 / (1 + x[5]);
+    } else {
+        double pz, betaz, delta1, gamma1, beta1;
+        delta1 = sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0;
+        gamma1 = GP.gamma + sqrt(GP.gamma * GP.gamma - 1.) * x[pt_];
+        beta1 = sqrt(1.0 - 1.0 / gamma1 / gamma1);
+        pz = sqrt((1.0 + delta1) * (1.0 + delta1) - x[px_] * x[px_] - x[py_] * x[py_]);
+        betaz = pz * beta1 / (1.0 + delta1);
+        x[x_] = x[x_] + x[px_] * L / pz;
+        x[y_] = x[y_] + x[py_] * L / pz;
+        x[z_] = x[z_] - (L / betaz - L / GP.beta);
+    }
+}
+
+/* bend_kick_pass, simtrack.cpp:3790-3794 */
+static void bend_kick_pass(double x[], double L, double href)
+{
+    x[px_] = x[px_] + (href * x[pt_] - href * href * x[x_]) * L;
+    x[z_] = x[z_] - href * x[x_] * L;
+}
+
+/* general_bend_kick_pass, simtrack.cpp:3796-3800 */
+static void general_bend_kick_pass(double x[], double L, double href, double hreal)
+{
+    x[px_] = x[px_] + (-(hreal - href) + href * x[pt_] - href * hreal * x[x_]) * L;
+    x[z_] = x[z_] - href * x[x_] * L;
+}
+
+/* sbend_exact_pass, simtrack.cpp:3833-3858 */
+static void sbend_exact_pass(double x[], double L, double Angle, double cosAngle, double sinAngle)
+{
+    double href = Angle / L;
+    double x0[6], pz0, pz1, delta, gamma, beta, derivative, temp, arcsin1, arcsin2;
+    for (int i = 0; i < 6; i++) x0[i] = x[i];
+    delta = sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0;
+    gamma = GP.gamma + sqrt(GP.gamma * GP.gamma - 1) * x[pt_];
+    beta = sqrt(1.0 - 1.0 / gamma / gamma);
+    pz0 = sqrt((1. + delta) * (1. + delta) - x0[1] * x0[1] - x0[3] * x0[3]);
+    x[1] = x0[1] * cosAngle + (pz0 - 1. - href * x0[0]) * sinAngle;
+    pz1 = sqrt((1. + delta) * (1. + delta) - x[1] * x[1] - x[3] * x[3]);
+    derivative = -1.0 * x0[1] * sinAngle * href + (pz0 - 1. - x[0] * href) * cosAngle * href;
+    x[0] = pz1 / href - derivative / href / href - 1.0 / href;
+    temp = sqrt((1. + delta) * (1. + delta) - x0[3] * x0[3]);
+    arcsin1 = OASIN(x0[1] / temp);
+    arcsin2 = OASIN(x[1] / temp);
+    x[2] = x0[2] + x0[3] * L + x0[3] * (arcsin1 - arcsin2) / href;
+    temp = (1 + delta) * L + (1 + delta) * (arcsin1 - arcsin2) / href;
+    x[4] = x[4] - (temp / beta - L / GP.beta);
+}
+
+/* general_sbend_exact_pass, simtrack.cpp:3974-4002 */
+static void general_sbend_exact_pass(double x[], double L, double Angle, double K0L,
+                                     double cosAngle, double sinAngle)
+{
+    double rhoc = L / Angle;
+    double b0 = K0L / L;
+    double x0[6], pz0, pz1, delta, gamma, beta, derivative, temp, arcsin1, arcsin2;
+    for (int i = 0; i < 6; i++) x0[i] = x[i];
+    delta = sqrt(1.0 + 2 * x[5] / GP.beta + x[5] * x[5]) - 1.0;
+    gamma = GP.gamma + sqrt(GP.gamma * GP.gamma - 1) * x[5];
+    beta = sqrt(1.0 - 1.0 / gamma / gamma);
+    pz0 = sqrt((1. + delta) * (1. + delta) - x0[1] * x0[1] - x0[3] * x0[3]);
+    x[1] = x0[1] * cosAngle + (pz0 - b0 * rhoc - b0 * x0[0]) * sinAngle;
+    pz1 = sqrt((1. + delta) * (1. + delta) - x[1] * x[1] - x[3] * x[3]);
+    derivative = -x0[1] * sinAngle / rhoc + (pz0 - b0 * rhoc - b0 * x0[0]) * cosAngle / rhoc;
+    x[0] = pz1 / b0 - derivative * rhoc / b0 - rhoc;
+    temp = sqrt((1. + delta) * (1. + delta) - x0[3] * x0[3]);
+    arcsin1 = OASIN(x0[1] / temp);
+    arcsin2 = OASIN(x[1] / temp);
+    x[2] = x0[2] + x0[3] * L / b0 / rhoc + x0[3] * (arcsin1 - arcsin2) / b0;
+    temp = (1 + delta) * L / b0 / rhoc + (1 + delta) * (arcsin1 - arcsin2) / b0;
+    x[4] = x[4] - (temp / beta - L / GP.beta);
+}
+
+/* quad/sext/oct/mult kicks, simtrack.cpp:4046-4111 */
+static void quad_kick_pass(double x[], double k1l, double k1sl)
+{
+    double preal = x[0], pimag = x[2];
+    double Bx = k1sl * preal + k1l * pimag;
+    double By = k1l * preal - k1sl * pimag;
+    x[px_] = x[px_] - By;
+    x[py_] = x[py_] + Bx;
+}
+
+static void sext_kick_pass(double x[], double k2l, double k2sl)
+{
+    double preal = x[0] * x[0] - x[2] * x[2];
+    double pimag = 2. * x[0] * x[2];
+    double Bx = (k2sl * preal + k2l * pimag) / 2.;
+    double By = (k2l * preal - k2sl * pimag) / 2.;
+    x[px_] = x[px_] - By;
+    x[py_] = x[py_] + Bx;
+}
+
+static void oct_kick_pass(double x[], double k3l, double k3sl)
+{
+    double preal = x[0] * x[0] * x[0] - 3 * x[0] * x[2] * x[2];
+    double pimag = 3 * x[0] * x[0] * x[2] - x[2] * x[2] * x[2];
+    double Bx = (k3sl * preal + k3l * pimag) / 6.;
+    double By = (k3l * preal - k3sl * pimag) / 6.;
+    x[px_] = x[px_] - By;
+    x[py_] = x[py_] + Bx;
+}
+
+static void mult_kick_pass(double x[], int Norder, const double KNL[11], const double KNSL[11])
+{
+    int fac = 1;
+    double Xn, Yn, Xn0, Yn0, By, Bx;
+    By = KNL[0];
+    Bx = KNSL[0];
+    Xn = 1.;
+    Yn = 0.;
+    for (int i = 1; i < Norder + 1; i++) {
+        Xn0 = Xn;
+        Yn0 = Yn;
+        Xn = Xn0 * x[x_] - Yn0 * x[y_];
+        Yn = Xn0 * x[y_] + Yn0 * x[x_];
+        fac = fac * i;
+        if (KNL[i] != 0. || KNSL[i] != 0.) {
+            By = By + (KNL[i] * Xn - KNSL[i] * Yn) / fac;
+            Bx = Bx + (KNL[i] * Yn + KNSL[i] * Xn) / fac;
+        }
+    }
+    x[px_] = x[px_] - By;
+    x[py_] = x[py_] + Bx;
+}
+
+/* bend_mult_kick_pass, simtrack.cpp:4113-4137 */
+static void bend_mult_kick_pass(double x[], double L, double href, int Norder,
+                                const double KNL[11], const double KNSL[11])
+{
+    int fac = 1;
+    double Xn, Yn, Xn0, Yn0, By, Bx;
+    By = KNL[0];
+    Bx = KNSL[0];
+    Xn = 1.;
+    Yn = 0.;
+    for (int i = 1; i < Norder + 1; i++) {
+        Xn0 = Xn;
+        Yn0 = Yn;
+        Xn = Xn0 * x[x_] - Yn0 * x[y_];
+        Yn = Xn0 * x[y_] + Yn0 * x[x_];
+        fac = fac * i;
+        if (KNL[i] != 0. || KNSL[i] != 0.) {
+            By = By + (KNL[i] * Xn - KNSL[i] * Yn) / fac;
+            Bx = Bx + (KNL[i] * Yn + KNSL[i] * Xn) / fac;
+        }
+    }
+    x[px_] = x[px_] - By + (href * x[pt_] - href * href * x[x_]) * L;
+    x[py_] = x[py_] + Bx;
+    x[z_] = x[z_] - href * x[x_] * L;
+}
+
+/* general_bend_mult_kick_pass, simtrack.cpp:4139-4163 */
+static void general_bend_mult_kick_pass(double x[], double L, double href, double hreal, int Norder,
+                                        const double KNL[11], const double KNSL[11])
+{
+    int fac = 1;
+    double Xn, Yn, Xn0, Yn0, By, Bx;
+    By = KNL[0];
+    Bx = KNSL[0];
+    Xn = 1.;
+    Yn = 0.;
+    for (int i = 1; i < Norder + 1; i++) {
+        Xn0 = Xn;
+        Yn0 = Yn;
+        Xn = Xn0 * x[x_] - Yn0 * x[y_];
+        Yn = Xn0 * x[y_] + Yn0 * x[x_];
+        fac = fac * i;
+        if (KNL[i] != 0. || KNSL[i] != 0.) {
+            By = By + (KNL[i] * Xn - KNSL[i] * Yn) / fac;
+            Bx = Bx + (KNL[i] * Yn + KNSL[i] * Xn) / fac;
+        }
+    }
+    x[px_] = x[px_] - By + (-(hreal - href) + href * x[pt_] - href * hreal * x[x_]) * L;
+    x[py_] = x[py_] + Bx;
+    x[z_] = x[z_] - href * x[x_] * L;
+}
+
+/* cal_Bfield, simtrack.cpp:4169-4196 */
+static void cal_Bfield(const double x[], int Nint, int Norder, double Angle, const double KNL[11],
+                       const double KNSL[11], double BLbrho[])
+{
+    int fac = 1;
+    double Xn, Yn, Xn0, Yn0, By, Bx;
+    By = KNL[0];
+    Bx = KNSL[0];
+    Xn = 1.;
+    Yn = 0.;
+    for (int i = 1; i < Norder + 1; i++) {
+        Xn0 = Xn;
+        Yn0 = Yn;
+        Xn = Xn0 * x[x_] - Yn0 * x[y_];
+        Yn = Xn0 * x[y_] + Yn0 * x[x_];
+        fac = fac * i;
+        if (KNL[i] != 0. || KNSL[i] != 0.) {
+            By = By + (KNL[i] * Xn - KNSL[i] * Yn) / fac;
+            Bx = Bx + (KNL[i] * Yn + KNSL[i] * Xn) / fac;
+        }
+    }
+    BLbrho[0] = Bx / Nint;
+    BLbrho[1] = (Angle + By) / Nint;
+    BLbrho[2] = 0.;
+}
+
+/* vector_cross_product, simtrack.cpp (template near cal_Bfield) */
+static void vector_cross_product(const double a[], const double b[], double c[])
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = -1.0 * a[0] * b[2] + a[2] * b[0];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* ------------------------------------------------------- radiation */
+
+/* Per-particle normal stream replacing gsl_ran_gaussian(gsl_r2, 1.0). */
+typedef struct { uint64_t seed; uint32_t particle; uint32_t counter; } orc_rng;
+
+/* radiate, simtrack.cpp:4198-4234 */
+static void radiate(double x[], double L, double href, const double BLbrho[])
+{
+    double x0[6], delta1, gamma1, beta1, delta2, pz, xp, yp, temp1, temp2, ratio;
+    double u[3], BLper[3], BLper2;
+    for (int i = 0; i < 6; i++) x0[i] = x[i];
+    delta1 = sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0;
+    gamma1 = GP.gamma + sqrt(GP.gamma * GP.gamma - 1) * x[pt_];
+    beta1 = sqrt(1.0 - 1.0 / gamma1 / gamma1);
+    (void)beta1;
+    pz = sqrt((1 + delta1) * (1 + delta1) - x[px_] * x[px_] - x[py_] * x[py_]);
+    xp = (1 + x[x_] * href) * x[px_] / pz;
+    yp = (1 + x[x_] * href) * x[py_] / pz;
+    temp1 = sqrt((1 + x[x_] * href) * (1 + x[x_] * href) + xp * xp + yp * yp);
+    u[0] = xp / temp1;
+    u[1] = yp / temp1;
+    u[2] = (1 + x[x_] * href) / temp1;
+    vector_cross_product(BLbrho, u, BLper);
+    BLper2 = BLper[0] * BLper[0] + BLper[1] * BLper[1] + BLper[2] * BLper[2];
+    temp2 = Cr * (1 + delta1) * (1 + delta1) * OPOW(GP.energy / 1000., 3) / 2 / PI;
+    x[pt_] = x[pt_] - temp2 * BLper2 * temp1 / L;
+    delta2 = sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0;
+    ratio = (1 + delta2) / (1 + delta1);
+    x[px_] = x0[px_] * ratio;
+    x[py_] = x0[py_] * ratio;
+}
+
+/* radiate1, simtrack.cpp:4236-4288 */
+static void radiate1(double x[], double L, double href, const double BLbrho[], orc_rng *rng)
+{
+    double x0[6], delta1, gamma1, beta1, delta2, pz, xp, yp, temp1, temp2, ratio;
+    double u[3], BLper[3], BLper2, g1, dtdl, ptloss, ptfluct, randgaus;
+    for (int i = 0; i < 6; i++) x0[i] = x[i];
+    delta1 = sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0;
+    gamma1 = GP.gamma + sqrt(GP.gamma * GP.gamma - 1) * x[pt_];
+    beta1 = sqrt(1.0 - 1.0 / gamma1 / gamma1);
+    pz = sqrt((1 + delta1) * (1 + delta1) - x[px_] * x[px_] - x[py_] * x[py_]);
+    xp = (1 + x[x_] * href) * x[px_] / pz;
+    yp = (1 + x[x_] * href) * x[py_] / pz;
+    temp1 = sqrt((1 + x[x_] * href) * (1 + x[x_] * href) + xp * xp + yp * yp);
+    u[0] = xp / temp1;
+    u[1] = yp / temp1;
+    u[2] = (1 + x[x_] * href) / temp1;
+    vector_cross_product(BLbrho, u, BLper);
+    BLper2 = BLper[0] * BLper[0] + BLper[1] * BLper[1] + BLper[2] * BLper[2];
+    temp2 = Cr * (1 + delta1) * (1 + delta1) * OPOW(GP.energy / 1000., 3) / 2 / PI;
+    ptloss = temp2 * BLper2 * temp1 / L;
+    dtdl = temp1 / beta1 / speed_light;
+    g1 = sqrt(BLper2 / L / L) / (1 + delta1);
+    randgaus = st_rng_normal(rng->seed, rng->particle, rng->counter);
+    rng->counter++;
+    ptfluct = sqrt(Cf * gamma1 * gamma1 * gamma1 * gamma1 * gamma1 * g1 * g1 * g1 * dtdl * L) * randgaus;
+    x[pt_] = x[pt_] - ptloss + ptfluct;
+    delta2 = sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0;
+    ratio = (1.0 + delta2) / (1 + delta1);
+    x[px_] = x0[px_] * ratio;
+    x[py_] = x0[py_] * ratio;
+}
+
+/* ------------------------------------------------------------- spin */
+
+/* spin(), simtrack.cpp:4574-4630 */
+static void spin(double x[], double angle, double href, const double BLbrho[])
+{
+    double u[3], Bpar[3], udotB, temp;
+    double delta1 = sqrt(1.0 + 2. * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0;
+    double gamma1 = GP.gamma + sqrt(GP.gamma * GP.gamma - 1.) * x[pt_];
+    double Gr = GP.G * gamma1;
+    double F[3], Fabs, uF[3], theta, costheta, sintheta, rot[9], spin0[3], sp[3];
+    if (BLbrho[0] * BLbrho[0] + BLbrho[1] * BLbrho[1] + BLbrho[2] * BLbrho[2] != 0.) {
+        for (int i = 0; i < 3; i++) spin0[i] = x[6 + i];
+        temp = sqrt((1 + delta1) * (1 + delta1) - x[px_] * x[px_] - x[py_] * x[py_]);
+        u[0] = x[px_] / temp;
+        u[1] = x[py_] / temp;
+        u[2] = 1.0;
+        temp = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        for (int i = 0; i < 3; i++) u[i] = u[i] / temp;
+        temp = (1 + href * x[0]) * temp;
+        udotB = u[0] * BLbrho[0] + u[1] * BLbrho[1] + u[2] * BLbrho[2];
+        for (int i = 0; i < 3; i++) Bpar[i] = udotB * u[i];
+        for (int i = 0; i < 3; i++) F[i] = (1 + Gr) * BLbrho[i] + GP.G * (1 - gamma1) * Bpar[i];
+        for (int i = 0; i < 3; i++) F[i] = F[i] * temp / (1 + delta1);
+        F[1] = F[1] - angle;
+        Fabs = sqrt(F[0] * F[0] + F[1] * F[1] + F[2] * F[2]);
+        for (int i = 0; i < 3; i++) uF[i] = F[i] / Fabs;
+        theta = -Fabs;
+        costheta = OCOS(theta);
+        sintheta = OSIN(theta);
+        rot[0] = uF[0] * uF[0] * (1 - costheta) + costheta;
+        rot[1] = uF[0] * uF[1] * (1 - costheta) - uF[2] * sintheta;
+        rot[2] = uF[0] * uF[2] * (1 - costheta) + uF[1] * sintheta;
+        rot[3] = uF[0] * uF[1] * (1 - costheta) + uF[2] * sintheta;
+        rot[4] = uF[1] * uF[1] * (1 - costheta) + costheta;
+        rot[5] = uF[1] * uF[2] * (1 - costheta) - uF[0] * sintheta;
+        rot[6] = uF[0] * uF[2] * (1 - costheta) - uF[1] * sintheta;
+        rot[7] = uF[1] * uF[2] * (1 - costheta) + uF[0] * sintheta;
+        rot[8] = uF[2] * uF[2] * (1 - costheta) + costheta;
+        for (int i = 0; i < 3; i++) {
+            sp[i] = 0.;
+            for (int j = 0; j < 3; j++) sp[i] = sp[i] + rot[i * 3 + j] * spin0[j];
+        }
+        for (int i = 0; i < 3; i++) x[6 + i] = sp[i];
+    }
+}
+
+/* ------------------------------------------------- thick elements */
+
+static void edge(double x[], double E, double h)
+{
+    double edgefocus = OTAN(E) * h;
+    x[1] = x[1] + edgefocus * x[0];
+    x[3] = x[3] - edgefocus * x[2];
+}
+
+/* SBEND_Pass, simtrack.cpp:4762-4816 */
+static void SBEND_Pass(double x[], double L, int Nint, double Angle, double E1, double E2, orc_rng *rng)
+{
+    double href = Angle / L, Lint = L / Nint, cosAngle, sinAngle, BLbrho[3];
+    if (Angle == 0.) {
+        DRIFT_Pass(x, L);
+        return;
+    }
+    edge(x, E1, href);
+    if (GP.H_expand && !GP.radiate) {
+        for (int i = 0; i < Nint; i++) {
+            DRIFT_Pass(x, Fdrift1 * Lint);
+            bend_kick_pass(x, Fkick1 * Lint, href);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            bend_kick_pass(x, Fkick2 * Lint, href);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            bend_kick_pass(x, Fkick1 * Lint, href);
+            DRIFT_Pass(x, Fdrift1 * Lint);
+        }
+    }
+    if (GP.H_expand && GP.radiate) {
+        BLbrho[0] = 0.;
+        BLbrho[1] = Angle / Nint;
+        BLbrho[2] = 0.;
+        for (int i = 0; i < Nint; i++) {
+            DRIFT_Pass(x, Fdrift1 * Lint);
+            bend_kick_pass(x, Fkick1 * Lint, href);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            bend_kick_pass(x, Fkick2 * Lint, href);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            bend_kick_pass(x, Fkick1 * Lint, href);
+            DRIFT_Pass(x, Fdrift1 * Lint);
+            radiate1(x, Lint, href, BLbrho, rng);
+        }
+    }
+    if (!GP.H_expand && !GP.radiate)
+        sbend_exact_pass(x, L, Angle, OCOS(Angle), OSIN(Angle));
+    if (!GP.H_expand && GP.radiate) {
+        BLbrho[0] = 0.;
+        BLbrho[1] = Angle / Nint;
+        BLbrho[2] = 0.;
+        cosAngle = OCOS(Angle / Nint);
+        sinAngle = OSIN(Angle / Nint);
+        for (int i = 0; i < Nint; i++) {
+            sbend_exact_pass(x, Lint, Angle / Nint, cosAngle, sinAngle);
+            radiate1(x, Lint, href, BLbrho, rng);
+        }
+    }
+    edge(x, E2, href);
+}
+
+/* GSBEND_Pass, simtrack.cpp:4818-4880 (radiation path radiate2 not restated) */
+static int GSBEND_Pass(double x[], double L, int Nint, double Angle, double K0L, double E1, double E2)
+{
+    double href = Angle / L, hreal = K0L / L, Lint = L / Nint;
+    if (Angle == 0.) {
+        DRIFT_Pass(x, L);
+        return 0;
+    }
+    if (GP.radiate) return -1;
+    edge(x, E1, hreal);
+    if (GP.H_expand) {
+        for (int i = 0; i < Nint; i++) {
+            DRIFT_Pass(x, Fdrift1 * Lint);
+            general_bend_kick_pass(x, Fkick1 * Lint, href, hreal);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            general_bend_kick_pass(x, Fkick2 * Lint, href, hreal);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            general_bend_kick_pass(x, Fkick1 * Lint, href, hreal);
+            DRIFT_Pass(x, Fdrift1 * Lint);
+        }
+    } else {
+        general_sbend_exact_pass(x, L, Angle, K0L, OCOS(Angle), OSIN(Angle));
+    }
+    edge(x, E2, hreal);
+    return 0;
+}
+
+/* QUAD_Pass, simtrack.cpp:4882-4959 */
+static void QUAD_Pass(double x[], double L, int Nint, double k1l, double k1sl)
+{
+    if (k1l == 0. && k1sl == 0.) {
+        DRIFT_Pass(x, L);
+        return;
+    }
+    if (L == 0.) {
+        quad_kick_pass(x, k1l, k1sl);
+        return;
+    }
+    if (L != 0. && GP.quad_fringe) {
+        double x0[6], k = k1l / L;
+        for (int i = 0; i < 6; i++) x0[i] = x[i];
+        x[0] = x0[0] + k * (x0[0] * x0[0] * x0[0] + 3 * x0[0] * x0[2] * x0[2]) / 12 / (1 + x0[5]);
+        x[1] = x0[1] - k * ((x0[0] * x0[0] + x0[2] * x0[2]) * x0[1] - 2 * x0[0] * x0[2] * x0[3]) / 4 / (1 + x0[5]);
+        x[2] = x0[2] - k * (3 * x0[0] * x0[0] * x0[2] + x0[2] * x0[2] * x0[2]) / 12 / (1 + x0[5]);
+        x[3] = x0[3] + k * ((x0[0] * x0[0] + x0[2] * x0[2]) * x0[3] - 2 * x0[0] * x0[2] * x0[1]) / 4 / (1 + x0[5]);
+        x[4] = x0[4] + k * (x0[2] * x0[2] * x0[2] * x0[3] - x0[0] * x0[0] * x0[0] * x0[1] +
+                            3 * x0[0] * x0[0] * x0[2] * x0[3] - 3 * x0[2] * x0[2] * x0[0] * x0[1]) /
+                           12 / (1 + x0[5]) / (1 + x0[5]);
+    }
+    double Lint = L / Nint;
+    double k1l_kick1 = Fkick1 * k1l / Nint, k1sl_kick1 = Fkick1 * k1sl / Nint;
+    double k1l_kick2 = Fkick2 * k1l / Nint, k1sl_kick2 = Fkick2 * k1sl / Nint;
+    for (int i = 0; i < Nint; i++) {
+        DRIFT_Pass(x, Fdrift1 * Lint);
+        quad_kick_pass(x, k1l_kick1, k1sl_kick1);
+        DRIFT_Pass(x, Fdrift2 * Lint);
+        quad_kick_pass(x, k1l_kick2, k1sl_kick2);
+        DRIFT_Pass(x, Fdrift2 * Lint);
+        quad_kick_pass(x, k1l_kick1, k1sl_kick1);
+        DRIFT_Pass(x, Fdrift1 * Lint);
+    }
+    if (L != 0. && GP.quad_fringe) {
+        double x0[6], k = k1l / L;
+        for (int i = 0; i < 6; i++) x0[i] = x[i];
+        x[0] = x0[0] - k * (x0[0] * x0[0] * x0[0] + 3 * x0[0] * x0[2] * x0[2]) / 12 / (1 + x0[5]);
+        x[1] = x0[1] + k * ((x0[0] * x0[0] + x0[2] * x0[2]) * x0[1] - 2 * x0[0] * x0[2] * x0[3]) / 4 / (1 + x0[5]);
+        x[2] = x0[2] + k * (3 * x0[0] * x0[0] * x0[2] + x0[2] * x0[2] * x0[2]) / 12 / (1 + x0[5]);
+        x[3] = x0[3] - k * ((x0[0] * x0[0] + x0[2] * x0[2]) * x0[3] - 2 * x0[0] * x0[2] * x0[1]) / 4 / (1 + x0[5]);
+        x[4] = x0[4] - k * (x0[2] * x0[2] * x0[2] * x0[3] - x0[0] * x0[0] * x0[0] * x0[1] +
+                            3 * x0[0] * x0[0] * x0[2] * x0[3] - 3 * x0[2] * x0[2] * x0[0] * x0[1]) /
+                           12 / (1 + x0[5]) / (1 + x0[5]);
+    }
+}
+
+/* SEXT_Pass, simtrack.cpp:5004-5034 */
+static void SEXT_Pass(double x[], double L, int Nint, double k2l, double k2sl)
+{
+    if (k2l == 0. && k2sl == 0.) {
+        DRIFT_Pass(x, L);
+        return;
+    }
+    if (L == 0.) {
+        sext_kick_pass(x, k2l, k2sl);
+        return;
+    }
+    double Lint = L / Nint;
+    double k2l_kick1 = Fkick1 * k2l / Nint, k2sl_kick1 = Fkick1 * k2sl / Nint;
+    double k2l_kick2 = Fkick2 * k2l / Nint, k2sl_kick2 = Fkick2 * k2sl / Nint;
+    for (int i = 0; i < Nint; i++) {
+        DRIFT_Pass(x, Fdrift1 * Lint);
+        sext_kick_pass(x, k2l_kick1, k2sl_kick1);
+        DRIFT_Pass(x, Fdrift2 * Lint);
+        sext_kick_pass(x, k2l_kick2, k2sl_kick2);
+        DRIFT_Pass(x, Fdrift2 * Lint);
+        sext_kick_pass(x, k2l_kick1, k2sl_kick1);
+        DRIFT_Pass(x, Fdrift1 * Lint);
+    }
+}
+
+/* OCT_Pass, simtrack.cpp:5077-5107 */
+static void OCT_Pass(double x[], double L, int Nint, double k3l, double k3sl)
+{
+    if (k3l == 0. && k3sl == 0.) {
+        DRIFT_Pass(x, L);
+        return;
+    }
+    if (L == 0.) {
+        oct_kick_pass(x, k3l, k3sl);
+        return;
+    }
+    double Lint = L / Nint;
+    double k3l_kick1 = Fkick1 * k3l / Nint, k3sl_kick1 = Fkick1 * k3sl / Nint;
+    double k3l_kick2 = Fkick2 * k3l / Nint, k3sl_kick2 = Fkick2 * k3sl / Nint;
+    for (int i = 0; i < Nint; i++) {
+        DRIFT_Pass(x, Fdrift1 * Lint);
+        oct_kick_pass(x, k3l_kick1, k3sl_kick1);
+        DRIFT_Pass(x, Fdrift2 * Lint);
+        oct_kick_pass(x, k3l_kick2, k3sl_kick2);
+        DRIFT_Pass(x, Fdrift2 * Lint);
+        oct_kick_pass(x, k3l_kick1, k3sl_kick1);
+        DRIFT_Pass(x, Fdrift1 * Lint);
+    }
+}
+
+static void kick_coefs(const double KNL[11], const double KNSL[11], int Nint, double k1[11], double s1[11],
+                       double k2[11], double s2[11])
+{
+    for (int i = 0; i < 11; i++) {
+        k1[i] = Fkick1 * KNL[i] / Nint;
+        s1[i] = Fkick1 * KNSL[i] / Nint;
+    }
+    for (int i = 0; i < 11; i++) {
+        k2[i] = Fkick2 * KNL[i] / Nint;
+        s2[i] = Fkick2 * KNSL[i] / Nint;
+    }
+}
+
+/* MULT_Pass, simtrack.cpp:5152-5182 */
+static void MULT_Pass(double x[], double L, int Nint, int Norder, const double KNL[11], const double KNSL[11])
+{
+    double k1[11], s1[11], k2[11], s2[11];
+    if (L == 0.) {
+        mult_kick_pass(x, Norder, KNL, KNSL);
+        return;
+    }
+    double Lint = L / Nint;
+    kick_coefs(KNL, KNSL, Nint, k1, s1, k2, s2);
+    for (int i = 0; i < Nint; i++) {
+        DRIFT_Pass(x, Fdrift1 * Lint);
+        mult_kick_pass(x, Norder, k1, s1);
+        DRIFT_Pass(x, Fdrift2 * Lint);
+        mult_kick_pass(x, Norder, k2, s2);
+        DRIFT_Pass(x, Fdrift2 * Lint);
+        mult_kick_pass(x, Norder, k1, s1);
+        DRIFT_Pass(x, Fdrift1 * Lint);
+    }
+}
+
+/*
+ * GMULT_Pass (simtrack.cpp:5223-5346), SBENDMULT_Pass (cpp:5348-5470) and
+ * SMULT_Pass (cpp:5472-5595) have identical bodies.
+ */
+static void BENDMULT_Pass(double x[], double L, int Nint, int Norder, double Angle, double E1, double E2,
+                          const double KNL[11], const double KNSL[11])
+{
+    double k1[11], s1[11], k2[11], s2[11], x1[6], BLbrho[3];
+    double cosAngle, sinAngle, cosAngle1, sinAngle1;
+    if (L == 0.) {
+        mult_kick_pass(x, Norder, KNL, KNSL);
+        return;
+    }
+    double href = Angle / L, Lint = L / Nint;
+    edge(x, E1, href);
+    kick_coefs(KNL, KNSL, Nint, k1, s1, k2, s2);
+    if (GP.H_expand) {
+        for (int i = 0; i < Nint; i++) {
+            for (int j = 0; j < 6; j++) x1[j] = x[j];
+            DRIFT_Pass(x, Fdrift1 * Lint);
+            bend_mult_kick_pass(x, Fkick1 * Lint, href, Norder, k1, s1);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            bend_mult_kick_pass(x, Fkick2 * Lint, href, Norder, k2, s2);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            bend_mult_kick_pass(x, Fkick1 * Lint, href, Norder, k1, s1);
+            DRIFT_Pass(x, Fdrift1 * Lint);
+            if (GP.radiate) {
+                for (int j = 0; j < 6; j++) x1[j] = (x[j] + x1[j]) / 2.;
+                cal_Bfield(x1, Nint, Norder, Angle, KNL, KNSL, BLbrho);
+                radiate(x, Lint, href, BLbrho);
+            }
+        }
+    } else {
+        cosAngle1 = OCOS(Fdrift1 * Angle / Nint);
+        sinAngle1 = OSIN(Fdrift1 * Angle / Nint);
+        cosAngle = OCOS(Fdrift2 * Angle / Nint);
+        sinAngle = OSIN(Fdrift2 * Angle / Nint);
+        for (int i = 0; i < Nint; i++) {
+            for (int j = 0; j < 6; j++) x1[j] = x[j];
+            sbend_exact_pass(x, Fdrift1 * Lint, Fdrift1 * Angle / Nint, cosAngle1, sinAngle1);
+            mult_kick_pass(x, Norder, k1, s1);
+            sbend_exact_pass(x, Fdrift2 * Lint, Fdrift2 * Angle / Nint, cosAngle, sinAngle);
+            mult_kick_pass(x, Norder, k2, s2);
+            sbend_exact_pass(x, Fdrift2 * Lint, Fdrift2 * Angle / Nint, cosAngle, sinAngle);
+            mult_kick_pass(x, Norder, k1, s1);
+            sbend_exact_pass(x, Fdrift1 * Lint, Fdrift1 * Angle / Nint, cosAngle1, sinAngle1);
+            if (GP.radiate) {
+                for (int j = 0; j < 6; j++) x1[j] = (x[j] + x1[j]) / 2.;
+                cal_Bfield(x1, Nint, Norder, Angle, KNL, KNSL, BLbrho);
+                radiate(x, Lint, href, BLbrho);
+            }
+        }
+    }
+    edge(x, E2, href);
+}
+
+/* GSBENDMULT_Pass, simtrack.cpp:5597-5723 */
+static void GSBENDMULT_Pass(double x[], double L, int Nint, int Norder, double Angle, double K0L, double E1,
+                            double E2, const double KNL[11], const double KNSL[11])
+{
+    double k1[11], s1[11], k2[11], s2[11], BLbrho[3];
+    double cosAngle, sinAngle, cosAngle1, sinAngle1;
+    if (L == 0.) {
+        mult_kick_pass(x, Norder, KNL, KNSL);
+        return;
+    }
+    double href = Angle / L, hreal = K0L / L, Lint = L / Nint;
+    edge(x, E1, hreal);
+    kick_coefs(KNL, KNSL, Nint, k1, s1, k2, s2);
+    if (GP.H_expand) {
+        for (int i = 0; i < Nint; i++) {
+            DRIFT_Pass(x, Fdrift1 * Lint);
+            general_bend_mult_kick_pass(x, Fkick1 * Lint, href, hreal, Norder, k1, s1);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            general_bend_mult_kick_pass(x, Fkick2 * Lint, href, hreal, Norder, k2, s2);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            general_bend_mult_kick_pass(x, Fkick1 * Lint, href, hreal, Norder, k1, s1);
+            DRIFT_Pass(x, Fdrift1 * Lint);
+            if (GP.radiate) {
+                BLbrho[0] = 0.;
+                BLbrho[1] = K0L / Nint;
+                BLbrho[2] = 0.;
+                radiate(x, Lint, href, BLbrho);
+            }
+        }
+    } else {
+        cosAngle1 = OCOS(Fdrift1 * Angle / Nint);
+        sinAngle1 = OSIN(Fdrift1 * Angle / Nint);
+        cosAngle = OCOS(Fdrift2 * Angle / Nint);
+        sinAngle = OSIN(Fdrift2 * Angle / Nint);
+        for (int i = 0; i < Nint; i++) {
+            general_sbend_exact_pass(x, Fdrift1 * Lint, Fdrift1 * Angle / Nint, Fdrift1 * K0L / Nint, cosAngle1, sinAngle1);
+            mult_kick_pass(x, Norder, k1, s1);
+            general_sbend_exact_pass(x, Fdrift2 * Lint, Fdrift2 * Angle / Nint, Fdrift2 * K0L / Nint, cosAngle, sinAngle);
+            mult_kick_pass(x, Norder, k2, s2);
+            general_sbend_exact_pass(x, Fdrift2 * Lint, Fdrift2 * Angle / Nint, Fdrift2 * K0L / Nint, cosAngle, sinAngle);
+            mult_kick_pass(x, Norder, k1, s1);
+            general_sbend_exact_pass(x, Fdrift1 * Lint, Fdrift1 * Angle / Nint, Fdrift1 * K0L / Nint, cosAngle1, sinAngle1);
+            if (GP.radiate) {
+                BLbrho[0] = 0.;
+                BLbrho[1] = K0L / Nint;
+                BLbrho[2] = 0.;
+                radiate(x, Lint, href, BLbrho);
+            }
+        }
+    }
+    edge(x, E2, hreal);
+}
+
+/* ------------------------------------------------------ thin maps */
+
+/* MATRIX_Pass, simtrack.cpp:5873-5885 */
+static void MATRIX_Pass(double x[], const double XCO_IN[6], const double XCO_OUT[6], const double M66[36])
+{
+    double x1[6];
+    for (int i = 0; i < 6; i++) x[i] = x[i] - XCO_IN[i];
+    for (int i = 0; i < 6; i++) {
+        x1[i] = 0.0;
+        for (int j = 0; j < 6; j++) x1[i] = x1[i] + M66[i * 6 + j] * x[j];
+    }
+    for (int i = 0; i < 6; i++) x[i] = x1[i];
+    for (int i = 0; i < 6; i++) x[i] = x[i] + XCO_OUT[i];
+}
+
+/* KICK_Pass, simtrack.cpp:5887-5893 */
+static void KICK_Pass(double x[], double L, double HKICK, double VKICK)
+{
+    DRIFT_Pass(x, L / 2.0);
+    x[1] = x[1] + HKICK;
+    x[3] = x[3] + VKICK;
+    DRIFT_Pass(x, L / 2.0);
+}
+
+/* RFCAV_Pass, simtrack.cpp:6002-6012 */
+static void RFCAV_Pass(double x[], double L, double VRF, double FRF, double PHASE0)
+{
+    if (VRF == 0.) {
+        DRIFT_Pass(x, L);
+        return;
+    }
+    DRIFT_Pass(x, L / 2.);
+    x[pt_] = x[pt_] + fabs(VRF * GP.Q) * OSIN(-2.0 * PI * FRF * x[z_] / 2.99792458e8 + PI - PHASE0) / GP.beta / GP.energy;
+    DRIFT_Pass(x, L / 2.);
+}
+
+/* CRABRF_Pass, simtrack.cpp:6014-6025 */
+static void CRABRF_Pass(double x[], double L, double VRF, double FRF, double PHASE0)
+{
+    if (VRF == 0.) {
+        DRIFT_Pass(x, L);
+    } else {
+        DRIFT_Pass(x, L / 2.);
+        x[px_] = x[px_] - (VRF * GP.Q) * OSIN(2.0 * PI * FRF * x[z_] / 2.99792458e8 + PHASE0) / GP.beta / GP.energy;
+        x[pt_] = x[pt_] - (VRF * GP.Q) * OCOS(2.0 * PI * FRF * x[z_] / 2.99792458e8 + PHASE0) * x[0] *
+                              (2.0 * PI * FRF / 2.99792458e8) / GP.beta / GP.energy;
+        DRIFT_Pass(x, L / 2.);
+    }
+}
+
+/* CCMULT_Pass, simtrack.cpp:6027-6067 */
+static void CCMULT_Pass(double x[], double L, double VRF, double FRF, double PHASE0, double B2, double A2,
+                        double B3, double A3, double B4, double A4, double B5, double A5)
+{
+    if (VRF == 0.) {
+        DRIFT_Pass(x, L);
+    } else {
+        DRIFT_Pass(x, L / 2.);
+        double c = 2.99792458e8;
+        double scale = 2 * PI * FRF / c / GP.brho;
+        double sinangle = OSIN(2.0 * PI * FRF * x[z_] / 2.99792458e8 + PHASE0);
+        double cosangle = OCOS(2.0 * PI * FRF * x[z_] / 2.99792458e8 + PHASE0);
+        double x2 = x[x_] * x[x_], x3 = x2 * x[x_], x4 = x3 * x[x_], x5 = x4 * x[x_];
+        double y2 = x[y_] * x[y_], y3 = y2 * x[y_], y4 = y3 * x[y_], y5 = y4 * x[y_];
+        double xy = x[x_] * x[y_], x2y = x2 * x[y_], xy2 = x[x_] * y2, x2y2 = x[x_] * xy2;
+        double x3y = x[y_] * x3, xy3 = x[x_] * y3;
+        double x2y3 = x2y2 * x[y_], x3y2 = x2y2 * x[x_], x4y = x4 * x[y_], xy4 = y4 * x[x_];
+        x[px_] = x[px_] + (B2 * x[x_] + A2 * x[y_]) * cosangle / GP.brho;
+        x[py_] = x[py_] + (A2 * x[x_] - B2 * x[y_]) * cosangle / GP.brho;
+        x[pt_] = x[pt_] + (B2 * (x2 - y2) * sinangle + A2 * xy * sinangle) * scale;
+        x[px_] = x[px_] + (B3 * (x2 - y2) + 2 * A3 * xy) * cosangle / GP.brho;
+        x[py_] = x[py_] + (2 * B3 * xy + A3 * (x2 - y2)) * cosangle / GP.brho;
+        x[pt_] = x[pt_] + (B3 * (x3 - 3 * xy2) * sinangle + A3 * (3 * x2y - y3) * sinangle) * scale / 3.;
+        x[px_] = x[px_] + (B4 * (x3 - 3 * xy2) - A4 * (y3 - 3 * x2y)) * cosangle / GP.brho;
+        x[py_] = x[py_] + (B4 * (y3 - 3 * x2y) + A4 * (x3 - 3 * xy2)) * cosangle / GP.brho;
+        x[pt_] = x[pt_] + (B4 * (x4 - 6 * x2 * y2 + y4) * sinangle + A4 * (4 * x3y - 4 * xy3) * sinangle) * scale / 4.;
+        x[px_] = x[px_] + (B5 * (x4 - 6 * x2y2 + y4) + A5 * (4 * x3y - 4 * xy3)) * cosangle / GP.brho;
+        x[py_] = x[py_] + (B5 * (4 * xy3 - 4 * x3y) + A5 * (x4 - 6 * x2y2 + y4)) * cosangle / GP.brho;
+        x[pt_] = x[pt_] + (B5 * (x5 - 10 * x3y2 + 5 * xy4) * sinangle + A5 * (5 * x4y - 10 * x2y3 + y5) * sinangle) *
+                              scale / 5.;
+        DRIFT_Pass(x, L / 2.);
+    }
+}
+
+/* LBT_Pass, simtrack.cpp:6069-6093 */
+static void LBT_Pass(double x[], double theta)
+{
+    double sintheta = OSIN(theta), costheta = OCOS(theta), tantheta = sintheta / costheta;
+    double x0 = x[0], px0 = x[1], y0 = x[2], py0 = x[3], z0 = x[4], pz0 = x[5];
+    double ps0 = sqrt((1 + pz0) * (1 + pz0) - px0 * px0 - py0 * py0);
+    double h0 = (1 + pz0) - ps0;
+    double px_star = (px0 - tantheta * h0) / OCOS(theta);
+    double py_star = py0 / costheta;
+    double pz_star = pz0 - tantheta * px0 + tantheta * tantheta * h0;
+    double ps_star = sqrt((1 + pz_star) * (1 + pz_star) - px_star * px_star - py_star * py_star);
+    double h_star = (1 + pz_star) - ps_star;
+    double x_star = tantheta * z0 + (1 + (px_star / ps_star) * sintheta) * x0;
+    double y_star = y0 + sintheta * (py_star / ps_star) * x0;
+    double z_star = z0 / costheta + (-h_star / ps_star) * sintheta * x0;
+    x[0] = x_star;
+    x[1] = px_star;
+    x[2] = y_star;
+    x[3] = py_star;
+    x[4] = z_star;
+    x[5] = pz_star;
+}
+
+/* ILBT_Pass, simtrack.cpp:6095-6123 */
+static void ILBT_Pass(double x[], double theta)
+{
+    double sintheta = OSIN(theta), costheta = OCOS(theta);
+    double x_star = x[0], px_star = x[1], y_star = x[2], py_star = x[3], z_star = x[4], pz_star = x[5];
+    double ps_star = sqrt((1 + pz_star) * (1 + pz_star) - px_star * px_star - py_star * py_star);
+    double h_star = 1 + pz_star - ps_star;
+    double hx_star = px_star / ps_star;
+    double hy_star = py_star / ps_star;
+    double hz_star = -h_star / ps_star;
+    double px0 = (px_star + h_star * sintheta) * costheta;
+    double py0 = py_star * costheta;
+    double pz0 = pz_star + px_star * sintheta;
+    double x0 = (x_star - z_star * sintheta) / (1 + hx_star * sintheta - hz_star * sintheta * sintheta);
+    double z0 = (z_star - hz_star * sintheta * x0) * costheta;
+    double y0 = y_star - sintheta * hy_star * x0;
+    x[0] = x0;
+    x[1] = px0;
+    x[2] = y0;
+    x[3] = py0;
+    x[4] = z0;
+    x[5] = pz0;
+}
+
+/* ------------------------------------------------------ beam-beam */
+
+/* cerrf, simtrack.cpp:6258-6323 (Gautschi's algorithm for w(z)) */
+EXPORT void orc_cerrf(double xx, double yy, double *wx_out, double *wy_out)
+{
+    int n, nc, nu;
+    double x, y, q, h, xl, xh, yh, tx, ty, tn, sx, sy, saux, wx, wy;
+    double rx[33], ry[33];
+    double cc = 1.12837916709551, zero = 0, one = 1, two = 2, half = .5, xlim = 5.33, ylim = 4.29, fac1 = 3.2,
+           fac2 = 23, fac3 = 21;
+    x = fabs(xx);
+    y = fabs(yy);
+    if (y < ylim && x < xlim) {
+        q = (one - y / ylim) * sqrt(one - (x / xlim) * (x / xlim));
+        h = one / (fac1 * q);
+        nc = 7 + (int)(fac2 * q);
+        xl = OPOW(h, (1 - nc));
+        xh = y + half / h;
+        yh = x;
+        nu = 10 + (int)(fac3 * q);
+        rx[nu + 1] = zero;
+        ry[nu + 1] = zero;
+        for (n = nu; n >= 1; n--) {
+            tx = xh + n * rx[n + 1];
+            ty = yh - n * ry[n + 1];
+            tn = tx * tx + ty * ty;
+            rx[n] = half * tx / tn;
+            ry[n] = half * ty / tn;
+        }
+        sx = zero;
+        sy = zero;
+        for (n = nc; n >= 1; n--) {
+            saux = sx + xl;
+            sx = rx[n] * saux - ry[n] * sy;
+            sy = rx[n] * sy + ry[n] * saux;
+            xl = h * xl;
+        }
+        wx = cc * sx;
+        wy = cc * sy;
+    } else {
+        xh = y;
+        yh = x;
+        rx[1] = zero;
+        ry[1] = zero;
+        for (n = 9; n >= 1; n--) {
+            tx = xh + n * rx[1];
+            ty = yh - n * ry[1];
+            tn = tx * tx + ty * ty;
+            rx[1] = half * tx / tn;
+            ry[1] = half * ty / tn;
+        }
+        wx = cc * rx[1];
+        wy = cc * ry[1];
+    }
+    if (yy < zero) {
+        wx = two * OEXP(y * y - x * x) * OCOS(two * x * y) - wx;
+        wy = -two * OEXP(y * y - x * x) * OSIN(two * x * y) - wy;
+        if (xx > zero) wy = -wy;
+    } else {
+        if (xx < zero) wy = -wy;
+    }
+    *wx_out = wx;
+    *wy_out = wy;
+}
+
+/* BB4D (scalar form), simtrack.cpp:6325-6382 */
+EXPORT void orc_bb4d(double x, double y, double gamma, double N, double sigmax, double sigmay, double *Dpx,
+                     double *Dpy)
+{
+    double rp = 1.53469826769e-18;
+    double x1, y1, x2, y2;
+    int signx = (x >= 0.0) ? 1 : -1;
+    int signy = (y >= 0.0) ? 1 : -1;
+    if (x == 0. && y == 0.) {
+        *Dpx = 0.;
+        *Dpy = 0.;
+    } else {
+        if (fabs(sigmax - sigmay) / sigmax < 1.0e-6) {
+            double r2 = x * x + y * y;
+            double temp1 = 2 * N * rp / gamma / r2;
+            double temp2 = 1 - OEXP(-r2 / 2 / sigmax / sigmax);
+            *Dpx = temp1 * x * temp2;
+            *Dpy = temp1 * y * temp2;
+        } else if (sigmax > sigmay) {
+            double temp1 = (2. * N * rp / gamma) * sqrt(PI / 2. / (sigmax * sigmax - sigmay * sigmay));
+            double temp2 = OEXP(-x * x / 2. / sigmax / sigmax - y * y / 2. / sigmay / sigmay);
+            double temp3 = sqrt(2. * (sigmax * sigmax - sigmay * sigmay));
+            double w1_real, w1_imag, w2_real, w2_imag;
+            x1 = fabs(x) / temp3;
+            y1 = fabs(y) / temp3;
+            x2 = fabs(x) * sigmay / sigmax / temp3;
+            y2 = fabs(y) * sigmax / sigmay / temp3;
+            orc_cerrf(x1, y1, &w1_real, &w1_imag);
+            orc_cerrf(x2, y2, &w2_real, &w2_imag);
+            *Dpy = temp1 * (w1_real - temp2 * w2_real) * signy;
+            *Dpx = temp1 * (w1_imag - temp2 * w2_imag) * signx;
+        } else if (sigmax < sigmay) {
+            double temp1 = (2 * N * rp / gamma) * sqrt(PI / 2 / (sigmay * sigmay - sigmax * sigmax));
+            double temp2 = OEXP(-x * x / 2 / sigmax / sigmax - y * y / 2 / sigmay / sigmay);
+            double temp3 = sqrt(2 * (sigmay * sigmay - sigmax * sigmax));
+            double w1_real, w1_imag, w2_real, w2_imag;
+            x1 = fabs(x) / temp3;
+            y1 = fabs(y) / temp3;
+            x2 = fabs(x) * sigmay / sigmax / temp3;
+            y2 = fabs(y) * sigmax / sigmay / temp3;
+            orc_cerrf(y1, x1, &w1_real, &w1_imag);
+            orc_cerrf(y2, x2, &w2_real, &w2_imag);
+            *Dpy = temp1 * (w1_imag - temp2 * w2_imag) * signy;
+            *Dpx = temp1 * (w1_real - temp2 * w2_real) * signx;
+        }
+    }
+}
+
+/* gsl_cdf_ugaussian_Pinv: GSL not vendored; the project restatement of
+   GSL's AS 241 algorithm is used on both sides (st_libm.h). */
+EXPORT double orc_ugaussian_pinv(double p) { return st_ugaussian_pinv(p); }
+
+/* slice centroids of BB6D (cpp:6594-6620) and BB6D_Angle (cpp:6937-6962) */
+static void bb_slice_centers(int N_slice, double sigma_l, double z_star[])
+{
+    if (N_slice == 11) {
+        z_star[0] = 1.7996765362153655;
+        z_star[1] = 1.1049618149811977;
+        z_star[2] = 0.75072324235797372;
+        z_star[3] = 0.47407703558819131;
+        z_star[4] = 0.23041113693387344;
+        z_star[5] = 0.;
+        z_star[6] = -0.23041113693387344;
+        z_star[7] = -0.47407703558819131;
+        z_star[8] = -0.75072324235797372;
+        z_star[9] = -1.1049618149811977;
+        z_star[10] = -1.7996765362153655;
+        for (int i = 0; i < 11; i++) z_star[i] = z_star[i] * sigma_l;
+    } else {
+        double y[64];
+        for (int i = 0; i < N_slice - 1; i++) y[i] = st_ugaussian_pinv((i + 1) * 1.0 / N_slice);
+        z_star[0] = N_slice * (OEXP(-y[0] * y[0] / 2)) / sqrt(2.0 * 3.14159265);
+        z_star[N_slice - 1] = N_slice * (0 - OEXP(-y[0] * y[0] / 2)) / sqrt(2.0 * 3.14159265);
+        for (int i = 1; i < N_slice - 1; i++)
+            z_star[i] = N_slice * (OEXP(-y[i] * y[i] / 2) - OEXP(-y[i - 1] * y[i - 1] / 2)) / sqrt(2.0 * 3.14159265);
+        for (int i = 0; i < N_slice; i++) z_star[i] = z_star[i] * sigma_l;
+    }
+}
+
+EXPORT void orc_bb_slice_centers(int N_slice, double sigma_l, double *z_star)
+{
+    bb_slice_centers(N_slice, sigma_l, z_star);
+}
+
+/* BB6D, simtrack.cpp:6579-6678 */
+static void BB6D(double x[], double gamma, double Np, double bbscale, double sigma_l, int N_slice,
+                 double emitx_rms, double betax_star, double alfx_star, double emity_rms, double betay_star,
+                 double alfy_star)
+{
+    double rp = 1.534698e-18;
+    double x0[6], Np_slice[64], z_star[64];
+    double S, gx_star, gy_star, betax, betay, sigmax, sigmay, dsigmax2ds, dsigmay2ds, dUdsigmax2, dUdsigmay2;
+    double X, Y, Dpx, Dpy, temp1, temp2, temp3, temp4;
+    for (int i = 0; i < N_slice; i++) Np_slice[i] = 1.0 * Np / N_slice;
+    bb_slice_centers(N_slice, sigma_l, z_star);
+    for (int i = 0; i < N_slice; i++) {
+        for (int j = 0; j < 6; j++) x0[j] = x[j];
+        S = (x0[4] - z_star[i]) / 2.0;
+        gx_star = (1.0 + alfx_star * alfx_star) / betax_star;
+        gy_star = (1.0 + alfy_star * alfy_star) / betay_star;
+        betax = betax_star + 2 * S * alfx_star + S * S * gx_star;
+        betay = betay_star + 2 * S * alfy_star + S * S * gy_star;
+        sigmax = sqrt(emitx_rms * betax);
+        sigmay = sqrt(emity_rms * betay);
+        dsigmax2ds = 2.0 * emitx_rms * (alfx_star + S * gx_star);
+        dsigmay2ds = 2.0 * emity_rms * (alfy_star + S * gy_star);
+        X = x0[0] + x0[1] * S;
+        Y = x0[2] + x0[3] * S;
+        orc_bb4d(X, Y, gamma, Np_slice[i], sigmax, sigmay, &Dpx, &Dpy);
+        temp1 = 1.0 / 2. / (sigmax * sigmax - sigmay * sigmay);
+        temp2 = X * Dpx + Y * Dpy;
+        temp3 = 2. * Np_slice[i] * rp / gamma;
+        temp4 = OEXP(-X * X / 2. / sigmax / sigmax - Y * Y / 2. / sigmay / sigmay);
+        x[0] = x0[0] - S * Dpx * bbscale;
+        x[1] = x0[1] + Dpx * bbscale;
+        x[2] = x0[2] - S * Dpy * bbscale;
+        x[3] = x0[3] + Dpy * bbscale;
+        x[4] = x0[4];
+        if (fabs(sigmax - sigmay) / sigmax < 1.0e-6) {
+            x[5] = x0[5] + 0.5 * Dpx * bbscale * (x0[1] + 0.5 * Dpx * bbscale) +
+                   0.5 * Dpy * bbscale * (x0[3] + 0.5 * Dpy * bbscale) +
+                   (1.0 / sigmax / sigmax) * dsigmax2ds * (temp3 * bbscale / 4.) * temp4;
+        } else if (sigmax > sigmay) {
+            dUdsigmax2 = temp1 * bbscale * (temp2 + temp3 * (sigmay / sigmax * temp4 - 1.));
+            dUdsigmay2 = -temp1 * bbscale * (temp2 + temp3 * (sigmax / sigmay * temp4 - 1.));
+            x[5] = x0[5] + 0.5 * Dpx * bbscale * (x0[1] + 0.5 * Dpx * bbscale) +
+                   0.5 * Dpy * bbscale * (x0[3] + 0.5 * Dpy * bbscale) -
+                   (0.5 * dsigmax2ds * dUdsigmax2 + 0.5 * dsigmay2ds * dUdsigmay2);
+        } else {
+            dUdsigmay2 = -temp1 * bbscale * (temp2 + temp3 * (sigmax / sigmay * temp4 - 1.));
+            dUdsigmax2 = temp1 * bbscale * (temp2 + temp3 * (sigmay / sigmax * temp4 - 1.));
+            x[5] = x0[5] + 0.5 * Dpx * bbscale * (x0[1] + 0.5 * Dpx * bbscale) +
+                   0.5 * Dpy * bbscale * (x0[3] + 0.5 * Dpy * bbscale) -
+                   (0.5 * dsigmax2ds * dUdsigmax2 + 0.5 * dsigmay2ds * dUdsigmay2);
+        }
+    }
+}
+
+/* BB6D_Angle, simtrack.cpp:6917-7079 */
+static void BB6D_Angle(double x[], double gamma, double Np, double bbscale, double sigma_l, double theta,
+                       int N_slice, double emitx_rms, double betax_star, double alfx_star, double emity_rms,
+                       double betay_star, double alfy_star)
+{
+    double rp = 1.534698e-18;
+    double sintheta = OSIN(theta), costheta = OCOS(theta), tantheta = sintheta / costheta;
+    double Np_slice[64], Z0[64], Z0_star[64];
+    double S, gx_star, gy_star, betax, alfx, betay, alfy;
+    double sigmax, sigmay, dsigmax2ds, dsigmay2ds, dUdsigmax2, dUdsigmay2;
+    double X, Y, Dpx, Dpy, temp1, temp2, temp3, temp4;
+    double x0, y0, z0, px0, py0, pz0, ps0, h0;
+    double x_star, y_star, z_star, px_star, py_star, pz_star, hx_star, hy_star, hz_star, ps_star, h_star;
+    double x_star_0, y_star_0, z_star_0, px_star_0, py_star_0, pz_star_0;
+    for (int i = 0; i < N_slice; i++) Np_slice[i] = 1.0 * Np / N_slice;
+    bb_slice_centers(N_slice, sigma_l, Z0);
+    x0 = x[0];
+    px0 = x[1];
+    y0 = x[2];
+    py0 = x[3];
+    z0 = x[4];
+    pz0 = x[5];
+    ps0 = sqrt((1 + pz0) * (1 + pz0) - px0 * px0 - py0 * py0);
+    h0 = (1 + pz0) - ps0;
+    px_star = (px0 - tantheta * h0) / OCOS(theta);
+    py_star = py0 / costheta;
+    pz_star = pz0 - tantheta * px0 + tantheta * tantheta * h0;
+    ps_star = sqrt((1 + pz_star) * (1 + pz_star) - px_star * px_star - py_star * py_star);
+    h_star = (1 + pz_star) - ps_star;
+    x_star = tantheta * z0 + (1 + (px_star / ps_star) * sintheta) * x0;
+    y_star = y0 + sintheta * (py_star / ps_star) * x0;
+    z_star = z0 / costheta + (-h_star / ps_star) * sintheta * x0;
+    for (int i = 0; i < N_slice; i++) Z0_star[i] = Z0[i] / costheta;
+    gx_star = (1.0 + alfx_star * alfx_star) / betax_star;
+    gy_star = (1.0 + alfy_star * alfy_star) / betay_star;
+    gx_star = gx_star / costheta;
+    gy_star = gy_star / costheta;
+    betax_star = betax_star * costheta;
+    betay_star = betay_star * costheta;
+    emitx_rms = emitx_rms / costheta;
+    emity_rms = emity_rms / costheta;
+    for (int i = 0; i < N_slice; i++) {
+        x_star_0 = x_star;
+        px_star_0 = px_star;
+        y_star_0 = y_star;
+        py_star_0 = py_star;
+        z_star_0 = z_star;
+        pz_star_0 = pz_star;
+        S = (z_star_0 - Z0_star[i]) / 2.0;
+        betax = betax_star - 2 * S * alfx_star + S * S * gx_star;
+        alfx = (alfx_star - gx_star * S);
+        betay = betay_star - 2 * S * alfy_star + S * S * gy_star;
+        alfy = (alfy_star - gy_star * S);
+        sigmax = sqrt(emitx_rms * betax);
+        sigmay = sqrt(emity_rms * betay);
+        dsigmax2ds = -2.0 * alfx * (emitx_rms);
+        dsigmay2ds = -2.0 * alfy * (emity_rms);
+        X = x_star_0 + px_star_0 * S - Z0_star[i] * sintheta;
+        Y = y_star_0 + py_star_0 * S - 0;
+        orc_bb4d(X, Y, gamma, Np_slice[i], sigmax, sigmay, &Dpx, &Dpy);
+        temp1 = 1.0 / 2. / (sigmax * sigmax - sigmay * sigmay);
+        temp2 = X * Dpx + Y * Dpy;
+        temp3 = 2. * Np_slice[i] * rp / gamma;
+        temp4 = OEXP(-X * X / 2. / sigmax / sigmax - Y * Y / 2. / sigmay / sigmay);
+        x_star = x_star_0 - S * Dpx * bbscale;
+        px_star = px_star_0 + Dpx * bbscale;
+        y_star = y_star_0 - S * Dpy * bbscale;
+        py_star = py_star_0 + Dpy * bbscale;
+        z_star = z_star_0;
+        if (fabs(sigmax - sigmay) / sigmax < 1.0e-6) {
+            pz_star = pz_star_0 + 0.5 * Dpx * bbscale * (px_star_0 + 0.5 * Dpx * bbscale) +
+                      0.5 * Dpy * bbscale * (py_star_0 + 0.5 * Dpy * bbscale) +
+                      (1.0 / sigmax / sigmax) * dsigmax2ds * (temp3 * bbscale / 4.) * temp4;
+        } else if (sigmax > sigmay) {
+            dUdsigmax2 = temp1 * bbscale * (temp2 + temp3 * (sigmay / sigmax * temp4 - 1.));
+            dUdsigmay2 = -temp1 * bbscale * (temp2 + temp3 * (sigmax / sigmay * temp4 - 1.));
+            pz_star = pz_star_0 + 0.5 * Dpx * bbscale * (px_star_0 + 0.5 * Dpx * bbscale) +
+                      0.5 * Dpy * bbscale * (py_star_0 + 0.5 * Dpy * bbscale) -
+                      (0.5 * dsigmax2ds * dUdsigmax2 + 0.5 * dsigmay2ds * dUdsigmay2);
+        } else {
+            dUdsigmay2 = -temp1 * bbscale * (temp2 + temp3 * (sigmax / sigmay * temp4 - 1.));
+            dUdsigmax2 = temp1 * bbscale * (temp2 + temp3 * (sigmay / sigmax * temp4 - 1.));
+            pz_star = pz_star_0 + 0.5 * Dpx * bbscale * (px_star_0 + 0.5 * Dpx * bbscale) +
+                      0.5 * Dpy * bbscale * (py_star_0 + 0.5 * Dpy * bbscale) -
+                      (0.5 * dsigmax2ds * dUdsigmax2 + 0.5 * dsigmay2ds * dUdsigmay2);
+        }
+    }
+    ps_star = sqrt((1 + pz_star) * (1 + pz_star) - px_star * px_star - py_star * py_star);
+    h_star = 1 + pz_star - ps_star;
+    hx_star = px_star / ps_star;
+    hy_star = py_star / ps_star;
+    hz_star = -h_star / ps_star;
+    px0 = (px_star + h_star * sintheta) * costheta;
+    py0 = py_star * costheta;
+    pz0 = pz_star + px_star * sintheta;
+    x0 = (x_star - z_star * sintheta) / (1 + hx_star * sintheta - hz_star * sintheta * sintheta);
+    z0 = (z_star - hz_star * sintheta * x0) * costheta;
+    y0 = y_star - sintheta * hy_star * x0;
+    x[0] = x0;
+    x[1] = px0;
+    x[2] = y0;
+    x[3] = py0;
+    x[4] = z0;
+    x[5] = pz0;
+}
+
+/* BEAMBEAM_Pass, simtrack.cpp:7343-7356 */
+static void BEAMBEAM_Pass(double x[], int TREATMENT, double NP, double BBSCALE, double SIGMAL, int NSLICE,
+                          double EMITX, double BETAX, double ALFAX, double EMITY, double BETAY, double ALFAY)
+{
+    if (NP != 0.) {
+        if (TREATMENT == 6) {
+            BB6D(x, GP.gamma, NP, BBSCALE, SIGMAL, NSLICE, EMITX, BETAX, ALFAX, EMITY, BETAY, ALFAY);
+        } else {
+            double Dpx, Dpy;
+            orc_bb4d(x[0], x[2], GP.gamma, NP, sqrt(EMITX * BETAX), sqrt(EMITY * BETAY), &Dpx, &Dpy);
+            x[1] = x[1] + Dpx * BBSCALE;
+            x[3] = x[3] + Dpy * BBSCALE;
+        }
+    }
+}
+
+/* ------------------------------------------------------ spin passes */
+
+/* SBEND_sPass, simtrack.cpp:7486-7536 */
+static void SBEND_sPass(double x[], double L, int Nint, double Angle, double E1, double E2)
+{
+    double href = Angle / L, Lint = L / Nint, BLbrho[3], x0[9];
+    if (Angle == 0.) {
+        DRIFT_Pass(x, L);
+        return;
+    }
+    edge(x, E1, href);
+    BLbrho[0] = 0.;
+    BLbrho[1] = Angle / Nint;
+    BLbrho[2] = 0.;
+    if (GP.H_expand) {
+        for (int i = 0; i < Nint; i++) {
+            for (int j = 0; j < 9; j++) x0[j] = x[j];
+            DRIFT_Pass(x, Fdrift1 * Lint);
+            bend_kick_pass(x, Fkick1 * Lint, href);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            bend_kick_pass(x, Fkick2 * Lint, href);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            bend_kick_pass(x, Fkick1 * Lint, href);
+            DRIFT_Pass(x, Fdrift1 * Lint);
+            spin(x0, Angle / Nint, href, BLbrho);
+            x[6] = x0[6];
+            x[7] = x0[7];
+            x[8] = x0[8];
+            if (GP.radiate) radiate(x, Lint, href, BLbrho);
+        }
+    } else {
+        for (int i = 0; i < Nint; i++) {
+            for (int j = 0; j < 9; j++) x0[j] = x[j];
+            sbend_exact_pass(x, Lint, Angle / Nint, OCOS(Angle / Nint), OSIN(Angle / Nint));
+            spin(x0, Angle / Nint, href, BLbrho);
+            x[6] = x0[6];
+            x[7] = x0[7];
+            x[8] = x0[8];
+            if (GP.radiate) radiate(x, Lint, href, BLbrho);
+        }
+    }
+    edge(x, E2, href);
+}
+
+/* QUAD_sPass, simtrack.cpp:7538-7590 */
+static void QUAD_sPass(double x[], double L, int Nint, double k1l, double k1sl)
+{
+    double Lint = L / Nint, preal, pimag, BLbrho[3], x0[9], x1[9];
+    if (k1l == 0. && k1sl == 0.) {
+        DRIFT_Pass(x, L);
+        return;
+    }
+    if (L == 0.) {
+        for (int j = 0; j < 9; j++) x0[j] = x[j];
+        quad_kick_pass(x, k1l, k1sl);
+        for (int j = 0; j < 9; j++) x1[j] = (x[j] + x0[j]) / 2.;
+        preal = x1[0];
+        pimag = x1[2];
+        BLbrho[0] = k1sl * preal + k1l * pimag;
+        BLbrho[1] = k1l * preal - k1sl * pimag;
+        BLbrho[2] = 0.;
+        spin(x0, 0, 0, BLbrho);
+        x[6] = x0[6];
+        x[7] = x0[7];
+        x[8] = x0[8];
+    } else {
+        double k1l_kick1 = Fkick1 * k1l / Nint, k1sl_kick1 = Fkick1 * k1sl / Nint;
+        double k1l_kick2 = Fkick2 * k1l / Nint, k1sl_kick2 = Fkick2 * k1sl / Nint;
+        for (int i = 0; i < Nint; i++) {
+            for (int j = 0; j < 9; j++) x0[j] = x[j];
+            DRIFT_Pass(x, Fdrift1 * Lint);
+            quad_kick_pass(x, k1l_kick1, k1sl_kick1);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            quad_kick_pass(x, k1l_kick2, k1sl_kick2);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            quad_kick_pass(x, k1l_kick1, k1sl_kick1);
+            DRIFT_Pass(x, Fdrift1 * Lint);
+            for (int j = 0; j < 9; j++) x1[j] = (x[j] + x0[j]) / 2.;
+            preal = x1[0];
+            pimag = x1[2];
+            BLbrho[0] = (k1sl * preal + k1l * pimag) / Nint;
+            BLbrho[1] = (k1l * preal - k1sl * pimag) / Nint;
+            BLbrho[2] = 0.;
+            spin(x0, 0, 0, BLbrho);
+            x[6] = x0[6];
+            x[7] = x0[7];
+            x[8] = x0[8];
+        }
+    }
+}
+
+/* SEXT_sPass, simtrack.cpp:7592-7645 */
+static void SEXT_sPass(double x[], double L, int Nint, double k2l, double k2sl)
+{
+    double Lint = L / Nint, preal, pimag, BLbrho[3], x0[9], x1[9];
+    if (k2l == 0. && k2sl == 0.) {
+        DRIFT_Pass(x, L);
+        return;
+    }
+    if (L == 0.) {
+        for (int j = 0; j < 9; j++) x0[j] = x[j];
+        sext_kick_pass(x, k2l, k2sl);
+        for (int j = 0; j < 9; j++) x1[j] = (x[j] + x0[j]) / 2.;
+        preal = x1[0] * x1[0] - x1[2] * x1[2];
+        pimag = 2. * x1[0] * x1[2];
+        BLbrho[0] = (k2sl * preal + k2l * pimag) / 2.;
+        BLbrho[1] = (k2l * preal - k2sl * pimag) / 2.;
+        BLbrho[2] = 0.;
+        spin(x0, 0, 0, BLbrho);
+        x[6] = x0[6];
+        x[7] = x0[7];
+        x[8] = x0[8];
+    } else {
+        double k2l_kick1 = Fkick1 * k2l / Nint, k2sl_kick1 = Fkick1 * k2sl / Nint;
+        double k2l_kick2 = Fkick2 * k2l / Nint, k2sl_kick2 = Fkick2 * k2sl / Nint;
+        for (int i = 0; i < Nint; i++) {
+            for (int j = 0; j < 9; j++) x0[j] = x[j];
+            DRIFT_Pass(x, Fdrift1 * Lint);
+            sext_kick_pass(x, k2l_kick1, k2sl_kick1);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            sext_kick_pass(x, k2l_kick2, k2sl_kick2);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            sext_kick_pass(x, k2l_kick1, k2sl_kick1);
+            DRIFT_Pass(x, Fdrift1 * Lint);
+            for (int j = 0; j < 9; j++) x1[j] = (x[j] + x0[j]) / 2.;
+            preal = x1[0] * x1[0] - x1[2] * x1[2];
+            pimag = 2. * x1[0] * x1[2];
+            BLbrho[0] = (k2sl * preal + k2l * pimag) / 2. / Nint;
+            BLbrho[1] = (k2l * preal - k2sl * pimag) / 2. / Nint;
+            BLbrho[2] = 0.;
+            spin(x0, 0, 0, BLbrho);
+            x[6] = x0[6];
+            x[7] = x0[7];
+            x[8] = x0[8];
+        }
+    }
+}
+
+/* OCT_sPass, simtrack.cpp:7647-7700 */
+static void OCT_sPass(double x[], double L, int Nint, double k3l, double k3sl)
+{
+    double Lint = L / Nint, preal, pimag, BLbrho[3], x0[9], x1[9];
+    if (k3l == 0. && k3sl == 0.) {
+        DRIFT_Pass(x, L);
+        return;
+    }
+    if (L == 0.) {
+        for (int j = 0; j < 9; j++) x0[j] = x[j];
+        oct_kick_pass(x, k3l, k3sl);
+        for (int j = 0; j < 9; j++) x1[j] = (x[j] + x0[j]) / 2.;
+        preal = x1[0] * x1[0] * x1[0] - 3 * x1[0] * x1[2] * x1[2];
+        pimag = 3 * x1[0] * x1[0] * x1[2] - x1[2] * x1[2] * x1[2];
+        BLbrho[0] = (k3sl * preal + k3l * pimag) / 6 / Nint;
+        BLbrho[1] = (k3l * preal - k3sl * pimag) / 6 / Nint;
+        BLbrho[2] = 0.;
+        spin(x0, 0, 0, BLbrho);
+        x[6] = x0[6];
+        x[7] = x0[7];
+        x[8] = x0[8];
+    } else {
+        double k3l_kick1 = Fkick1 * k3l / Nint, k3sl_kick1 = Fkick1 * k3sl / Nint;
+        double k3l_kick2 = Fkick2 * k3l / Nint, k3sl_kick2 = Fkick2 * k3sl / Nint;
+        for (int i = 0; i < Nint; i++) {
+            for (int j = 0; j < 9; j++) x0[j] = x[j];
+            DRIFT_Pass(x, Fdrift1 * Lint);
+            oct_kick_pass(x, k3l_kick1, k3sl_kick1);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            oct_kick_pass(x, k3l_kick2, k3sl_kick2);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            oct_kick_pass(x, k3l_kick1, k3sl_kick1);
+            DRIFT_Pass(x, Fdrift1 * Lint);
+            for (int j = 0; j < 9; j++) x1[j] = (x[j] + x0[j]) / 2.;
+            preal = x1[0] * x1[0] * x1[0] - 3 * x1[0] * x1[2] * x1[2];
+            pimag = 3 * x1[0] * x1[0] * x1[2] - x1[2] * x1[2] * x1[2];
+            BLbrho[0] = (k3sl * preal + k3l * pimag) / 6 / Nint;
+            BLbrho[1] = (k3l * preal - k3sl * pimag) / 6 / Nint;
+            BLbrho[2] = 0.;
+            spin(x0, 0, 0, BLbrho);
+            x[6] = x0[6];
+            x[7] = x0[7];
+            x[8] = x0[8];
+        }
+    }
+}
+
+/* MULT_sPass, simtrack.cpp:7702-7740 */
+static void MULT_sPass(double x[], double L, int Nint, int Norder, const double KNL[11], const double KNSL[11])
+{
+    double Lint = L / Nint, BLbrho[3], x0[9], x1[9], k1[11], s1[11], k2[11], s2[11];
+    if (L == 0.) {
+        cal_Bfield(x, 1, Norder, 0, KNL, KNSL, BLbrho);
+        spin(x, 0, 0, BLbrho);
+        mult_kick_pass(x, Norder, KNL, KNSL);
+    } else {
+        kick_coefs(KNL, KNSL, Nint, k1, s1, k2, s2);
+        for (int i = 0; i < Nint; i++) {
+            for (int j = 0; j < 9; j++) x0[j] = x[j];
+            DRIFT_Pass(x, Fdrift1 * Lint);
+            mult_kick_pass(x, Norder, k1, s1);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            mult_kick_pass(x, Norder, k2, s2);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            mult_kick_pass(x, Norder, k1, s1);
+            DRIFT_Pass(x, Fdrift1 * Lint);
+            for (int j = 0; j < 9; j++) x1[j] = (x[j] + x0[j]) / 2.;
+            cal_Bfield(x1, Nint, Norder, 0, KNL, KNSL, BLbrho);
+            spin(x0, 0, 0, BLbrho);
+            x[6] = x0[6];
+            x[7] = x0[7];
+            x[8] = x0[8];
+        }
+    }
+}
+
+/*
+ * GMULT_sPass (simtrack.cpp:7742-7833), SBENDMULT_sPass (cpp:7835-7928),
+ * SMULT_sPass (cpp:7930-8022) and GSBENDMULT_sPass (cpp:8024-8114, K0L
+ * unused) have identical bodies.
+ */
+static void BENDMULT_sPass(double x[], double L, int Nint, int Norder, double Angle, double E1, double E2,
+                           const double KNL[11], const double KNSL[11])
+{
+    double href = Angle / L, Lint = L / Nint, BLbrho[3], x0[9], x1[9], k1[11], s1[11], k2[11], s2[11];
+    double cosAngle, sinAngle, cosAngle1, sinAngle1;
+    if (L == 0.) {
+        for (int j = 0; j < 9; j++) x0[j] = x[j];
+        mult_kick_pass(x, Norder, KNL, KNSL);
+        for (int j = 0; j < 9; j++) x1[j] = (x[j] + x0[j]) / 2.;
+        cal_Bfield(x1, 1, Norder, Angle, KNL, KNSL, BLbrho);
+        spin(x0, Angle, 0, BLbrho);
+        x[6] = x0[6];
+        x[7] = x0[7];
+        x[8] = x0[8];
+        return;
+    }
+    edge(x, E1, href);
+    kick_coefs(KNL, KNSL, Nint, k1, s1, k2, s2);
+    if (GP.H_expand) {
+        for (int i = 0; i < Nint; i++) {
+            for (int j = 0; j < 9; j++) x0[j] = x[j];
+            DRIFT_Pass(x, Fdrift1 * Lint);
+            bend_mult_kick_pass(x, Fkick1 * Lint, href, Norder, k1, s1);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            bend_mult_kick_pass(x, Fkick2 * Lint, href, Norder, k2, s2);
+            DRIFT_Pass(x, Fdrift2 * Lint);
+            bend_mult_kick_pass(x, Fkick1 * Lint, href, Norder, k1, s1);
+            DRIFT_Pass(x, Fdrift1 * Lint);
+            for (int j = 0; j < 9; j++) x1[j] = (x[j] + x0[j]) / 2.;
+            cal_Bfield(x1, Nint, Norder, Angle, KNL, KNSL, BLbrho);
+            spin(x0, Angle / Nint, href, BLbrho);
+            x[6] = x0[6];
+            x[7] = x0[7];
+            x[8] = x0[8];
+            if (GP.radiate) radiate(x, Lint, href, BLbrho);
+        }
+    } else {
+        cosAngle1 = OCOS(Fdrift1 * Angle / Nint);
+        sinAngle1 = OSIN(Fdrift1 * Angle / Nint);
+        cosAngle = OCOS(Fdrift2 * Angle / Nint);
+        sinAngle = OSIN(Fdrift2 * Angle / Nint);
+        for (int i = 0; i < Nint; i++) {
+            for (int j = 0; j < 9; j++) x0[j] = x[j];
+            sbend_exact_pass(x, Fdrift1 * Lint, Fdrift1 * Angle / Nint, cosAngle1, sinAngle1);
+            mult_kick_pass(x, Norder, k1, s1);
+            sbend_exact_pass(x, Fdrift2 * Lint, Fdrift2 * Angle / Nint, cosAngle, sinAngle);
+            mult_kick_pass(x, Norder, k2, s2);
+            sbend_exact_pass(x, Fdrift2 * Lint, Fdrift2 * Angle / Nint, cosAngle, sinAngle);
+            mult_kick_pass(x, Norder, k1, s1);
+            sbend_exact_pass(x, Fdrift1 * Lint, Fdrift1 * Angle / Nint, cosAngle1, sinAngle1);
+            for (int j = 0; j < 9; j++) x1[j] = (x[j] + x0[j]) / 2.;
+            cal_Bfield(x1, Nint, Norder, Angle, KNL, KNSL, BLbrho);
+            spin(x0, Angle / Nint, href, BLbrho);
+            x[6] = x0[6];
+            x[7] = x0[7];
+            x[8] = x0[8];
+            if (GP.radiate) radiate(x, Lint, href, BLbrho);
+        }
+    }
+    edge(x, E2, href);
+}
+
+/* KICK_sPass, simtrack.cpp:8193-8208 */
+static void KICK_sPass(double x[], double L, double HKICK, double VKICK)
+{
+    double BLbrho[3];
+    DRIFT_Pass(x, L / 2.0);
+    BLbrho[0] = VKICK;
+    BLbrho[1] = -HKICK;
+    BLbrho[2] = 0.;
+    spin(x, 0, 0, BLbrho);
+    x[1] = x[1] + HKICK;
+    x[3] = x[3] + VKICK;
+    DRIFT_Pass(x, L / 2.0);
+}
+
+/* ROTAT_sPass / SNAKE_sPass, simtrack.cpp:8286-8348 (identical bodies) */
+static void ROTAT_sPass(double x[], double L, const double n[3], double angle)
+{
+    double spin0[3], sp[3], F, uF[3], costheta = OCOS(-angle), sintheta = OSIN(-angle), rot[9];
+    DRIFT_Pass(x, L / 2.);
+    for (int i = 0; i < 3; i++) spin0[i] = x[6 + i];
+    F = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for (int i = 0; i < 3; i++) uF[i] = n[i] / F;
+    rot[0] = uF[0] * uF[0] * (1 - costheta) + costheta;
+    rot[1] = uF[0] * uF[1] * (1 - costheta) - uF[2] * sintheta;
+    rot[2] = uF[0] * uF[2] * (1 - costheta) + uF[1] * sintheta;
+    rot[3] = uF[0] * uF[1] * (1 - costheta) + uF[2] * sintheta;
+    rot[4] = uF[1] * uF[1] * (1 - costheta) + costheta;
+    rot[5] = uF[1] * uF[2] * (1 - costheta) - uF[0] * sintheta;
+    rot[6] = uF[0] * uF[2] * (1 - costheta) - uF[1] * sintheta;
+    rot[7] = uF[1] * uF[2] * (1 - costheta) + uF[0] * sintheta;
+    rot[8] = uF[2] * uF[2] * (1 - costheta) + costheta;
+    for (int i = 0; i < 3; i++) {
+        sp[i] = 0.;
+        for (int j = 0; j < 3; j++) sp[i] = sp[i] + rot[i * 3 + j] * spin0[j];
+    }
+    for (int i = 0; i < 3; i++) x[6 + i] = sp[i];
+    DRIFT_Pass(x, L / 2.);
+}
+
+/* ------------------------------------------------ Element::Pass/sPass */
+
+#define MIS(e) (e)->dx, (e)->dy, (e)->ds, (e)->dtheta, (e)->dphi, (e)->dpsi
+
+/* Element::Pass dispatch: the Pass() bodies of simtrack.cpp:8425-11470 */
+static int elem_pass(const st_element *e, double x[], orc_rng *rng)
+{
+    switch (e->type) {
+    case ST_DRIFT: case ST_MARKER: case ST_BPM: case ST_HBPM: case ST_VBPM:
+    case ST_ROTAT: case ST_SNAKE:
+        DRIFT_Pass(x, e->l);
+        return 0;
+    case ST_SBEND:
+        GtoL(x, e->l, e->angle, MIS(e));
+        SBEND_Pass(x, e->l, e->nint, e->angle, e->e1, e->e2, rng);
+        LtoG(x, e->l, e->angle, MIS(e));
+        return 0;
+    case ST_GSBEND: {
+        GtoL(x, e->l, e->angle, MIS(e));
+        int rc = GSBEND_Pass(x, e->l, e->nint, e->angle, e->k0l, e->e1, e->e2);
+        LtoG(x, e->l, e->angle, MIS(e));
+        return rc;
+    }
+    case ST_QUAD:
+        GtoL(x, e->l, 0., MIS(e));
+        QUAD_Pass(x, e->l, e->nint, e->knl[1], e->knsl[1]);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_SKEWQ:
+        GtoL(x, e->l, 0., MIS(e));
+        QUAD_Pass(x, e->l, e->nint, 0., e->knsl[1]);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_SEXT:
+        GtoL(x, e->l, 0., MIS(e));
+        SEXT_Pass(x, e->l, e->nint, e->knl[2], e->knsl[2]);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_OCT:
+        GtoL(x, e->l, 0., MIS(e));
+        OCT_Pass(x, e->l, e->nint, e->knl[3], e->knsl[3]);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_MULT:
+        GtoL(x, e->l, 0., MIS(e));
+        MULT_Pass(x, e->l, e->nint, e->norder, e->knl, e->knsl);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_GMULT: case ST_SBENDMULT: case ST_SMULT:
+        GtoL(x, e->l, e->angle, MIS(e));
+        BENDMULT_Pass(x, e->l, e->nint, e->norder, e->angle, e->e1, e->e2, e->knl, e->knsl);
+        LtoG(x, e->l, e->angle, MIS(e));
+        return 0;
+    case ST_GSBENDMULT:
+        GtoL(x, e->l, e->angle, MIS(e));
+        GSBENDMULT_Pass(x, e->l, e->nint, e->norder, e->angle, e->k0l, e->e1, e->e2, e->knl, e->knsl);
+        LtoG(x, e->l, e->angle, MIS(e));
+        return 0;
+    case ST_KICKER: case ST_HKICKER: case ST_VKICKER: {
+        double hk = e->type == ST_VKICKER ? 0. : e->hkick;
+        double vk = e->type == ST_HKICKER ? 0. : e->vkick;
+        GtoL(x, e->l, 0., MIS(e));
+        KICK_Pass(x, e->l, hk, vk);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    }
+    case ST_RFCAV:
+        GtoL(x, e->l, 0., MIS(e));
+        RFCAV_Pass(x, e->l, e->vrf, e->frf, e->phase0);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_CRABRF:
+        GtoL(x, e->l, 0., MIS(e));
+        CRABRF_Pass(x, e->l, e->vrf, e->frf, e->phase0);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_CCMULT:
+        GtoL(x, e->l, 0., MIS(e));
+        CCMULT_Pass(x, e->l, e->vrf, e->frf, e->phase0, e->cc[0], e->cc[1], e->cc[2], e->cc[3], e->cc[4], e->cc[5],
+                    e->cc[6], e->cc[7]);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_LBT:
+        GtoL(x, e->l, 0., MIS(e));
+        LBT_Pass(x, e->theta);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_ILBT:
+        GtoL(x, e->l, 0., MIS(e));
+        ILBT_Pass(x, e->theta);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_BEAMBEAM:
+        GtoL(x, e->l, 0., MIS(e));
+        if (e->np != 0.)
+            BEAMBEAM_Pass(x, e->treatment, e->np, e->bbscale, e->sigmal, e->nslice, e->emitx, e->betax, e->alfax,
+                          e->emity, e->betay, e->alfay);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_CRBB:
+        GtoL(x, e->l, 0., MIS(e));
+        BB6D_Angle(x, GP.gamma, e->np, e->bbscale, e->sigmal, e->theta, e->nslice, e->emitx, e->betax, e->alfax,
+                   e->emity, e->betay, e->alfay);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_MATRIX:
+        MATRIX_Pass(x, e->xco_in, e->xco_out, e->m66);
+        return 0;
+    default:
+        return -1;
+    }
+}
+
+/* Element::sPass dispatch: the sPass() bodies of simtrack.cpp:8425-11470 */
+static int elem_spass(const st_element *e, double x[])
+{
+    switch (e->type) {
+    case ST_DRIFT: case ST_MARKER: case ST_BPM: case ST_HBPM: case ST_VBPM:
+        DRIFT_Pass(x, e->l);
+        return 0;
+    case ST_ROTAT: case ST_SNAKE:
+        ROTAT_sPass(x, e->l, e->rot_n, e->rot_angle);
+        return 0;
+    case ST_SBEND: case ST_GSBEND:
+        GtoL(x, e->l, e->angle, MIS(e));
+        SBEND_sPass(x, e->l, e->nint, e->angle, e->e1, e->e2);
+        LtoG(x, e->l, e->angle, MIS(e));
+        return 0;
+    case ST_QUAD:
+        GtoL(x, e->l, 0., MIS(e));
+        QUAD_sPass(x, e->l, e->nint, e->knl[1], e->knsl[1]);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_SKEWQ:
+        GtoL(x, e->l, 0., MIS(e));
+        QUAD_sPass(x, e->l, e->nint, 0., e->knsl[1]);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_SEXT:
+        GtoL(x, e->l, 0., MIS(e));
+        SEXT_sPass(x, e->l, e->nint, e->knl[2], e->knsl[2]);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_OCT:
+        GtoL(x, e->l, 0., MIS(e));
+        OCT_sPass(x, e->l, e->nint, e->knl[3], e->knsl[3]);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_MULT:
+        GtoL(x, e->l, 0., MIS(e));
+        MULT_sPass(x, e->l, e->nint, e->norder, e->knl, e->knsl);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_GMULT: case ST_SBENDMULT: case ST_SMULT: case ST_GSBENDMULT:
+        GtoL(x, e->l, e->angle, MIS(e));
+        BENDMULT_sPass(x, e->l, e->nint, e->norder, e->angle, e->e1, e->e2, e->knl, e->knsl);
+        LtoG(x, e->l, e->angle, MIS(e));
+        return 0;
+    case ST_KICKER: case ST_HKICKER: case ST_VKICKER: {
+        double hk = e->type == ST_VKICKER ? 0. : e->hkick;
+        double vk = e->type == ST_HKICKER ? 0. : e->vkick;
+        GtoL(x, e->l, 0., MIS(e));
+        KICK_sPass(x, e->l, hk, vk);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    }
+    case ST_CRABRF:                         /* CRABRF::sPass is a drift, cpp:10808-10812 */
+        GtoL(x, e->l, 0., MIS(e));
+        DRIFT_Pass(x, e->l);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_LBT:                            /* no misalignment in sPass, cpp:11004-11006 */
+        LBT_Pass(x, e->theta);
+        return 0;
+    case ST_ILBT:
+        ILBT_Pass(x, e->theta);
+        return 0;
+    case ST_RFCAV: case ST_CCMULT: case ST_BEAMBEAM: case ST_CRBB: case ST_MATRIX:
+        return elem_pass(e, x, NULL);       /* sPass == Pass (orbit only) */
+    default:
+        return -1;
+    }
+}
+
+/* ------------------------------------------------------- drivers */
+
+EXPORT int orc_element_pass(const st_element *e, double *x, int spin_on)
+{
+    orc_rng rng = {GP.rng_seed, 0u, 0u};
+    return spin_on ? elem_spass(e, x) : elem_pass(e, x, &rng);
+}
+
+static int lost_check(const st_element *e, const double x[])
+{
+    return fabs(x[0]) > e->apx || fabs(x[2]) > e->apy || isnan(x[0]) || isnan(x[2]);
+}
+
+/*
+ * Track (simtrack.cpp:20791-20814), Track_spin (cpp:20816-20839),
+ * Track_Radial (cpp:21641-21680) for one particle.  mode bit 1: spin,
+ * bit 2: radial (on-momentum z subtraction with x04 from orc_radial_x04).
+ * x_tbt (nullable): Track_tbt storage (cpp:20841-20861) at the start of each
+ * turn, or after element bpm_index (cpp:20885-20907) when bpm_index >= 0.
+ */
+static int track_one(const st_element *line, int n, double *x, int nturn, int *stable, int *lost_turn,
+                     int *lost_post, int mode, const double *x04, double *x_tbt, int bpm_index, uint32_t pid)
+{
+    int spin_on = mode & 1, radial = mode & 2;
+    int ncol = spin_on ? 9 : 6;
+    orc_rng rng = {GP.rng_seed, pid, 0u};
+    if (fabs(x[0]) > 0.1 || fabs(x[2]) > 0.1 || *stable == 0) {
+        *stable = 0;
+        *lost_turn = 0;
+        *lost_post = 0;
+        return 0;
+    }
+    for (int turn = 0; turn < nturn; turn++) {
+        if (x_tbt && bpm_index < 0)
+            for (int k = 0; k < ncol; k++) x_tbt[(size_t)turn * ncol + k] = x[k];
+        for (int j = 0; j < n; j++) {
+            int rc = spin_on ? elem_spass(&line[j], x) : elem_pass(&line[j], x, &rng);
+            if (rc) return rc;
+            if (radial) x[4] = x[4] - x04[j];
+            if (x_tbt && j == bpm_index)
+                for (int k = 0; k < ncol; k++) x_tbt[(size_t)turn * ncol + k] = x[k];
+            if (lost_check(&line[j], x)) {
+                *stable = 0;
+                *lost_turn = turn;
+                *lost_post = j;
+                return 0;
+            }
+        }
+    }
+    return 0;
+}
+
+/* on-momentum z after every element (Track_Radial, cpp:21655-21661) */
+EXPORT int orc_radial_x04(const st_element *line, int n, double *x04)
+{
+    double x0[6];
+    orc_rng rng = {GP.rng_seed, 0u, 0u};
+    for (int k = 0; k < 6; k++) x0[k] = 0.;
+    for (int j = 0; j < n; j++) {
+        x0[4] = 0.;
+        x0[5] = 0.;
+        int rc = elem_pass(&line[j], x0, &rng);
+        if (rc) return rc;
+        x04[j] = x0[4];
+    }
+    return 0;
+}
+
+/*
+ * Batched driver over independent particles (the reference calls Track once
+ * per particle).  nthreads > 1 spreads particles over OpenMP threads; each
+ * particle's arithmetic is unchanged.
+ */
+EXPORT int orc_track(const st_element *line, int n, double *x, int np, int nturn, int *stable, int *lost_turn,
+                     int *lost_post, int mode, double *x_tbt, int bpm_index, int nthreads)
+{
+    double *x04 = NULL;
+    int ncol = (mode & 1) ? 9 : 6;
+    int err = 0;
+    if (mode & 2) {
+        x04 = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+        if (!x04) return -5;
+        err = orc_radial_x04(line, n, x04);
+        if (err) {
+            free(x04);
+            return err;
+        }
+    }
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1) reduction(| : err)
+    for (int p = 0; p < np; p++) {
+        double *tb = x_tbt ? x_tbt + (size_t)p * (size_t)nturn * ncol : NULL;
+        err |= track_one(line, n, x + (size_t)p * ncol, nturn, stable + p, lost_turn + p, lost_post + p, mode, x04,
+                         tb, bpm_index, (uint32_t)p);
+    }
+    free(x04);
+    return err ? -2 : 0;
+}
+
+/* FineTuneFinder, simtrack.cpp:2288-2319 */
+EXPORT void orc_fine_tune(int n, const double *x, double tune_lower, double tune_upper, double *peaktune)
+{
+    double tune, tune_step = 0.001, sumr, sumi, amp, peakamp;
+    do {
+        tune = tune_lower;
+        *peaktune = tune_lower;
+        peakamp = 0;
+        do {
+            sumr = 0.;
+            sumi = 0;
+            for (int i = 0; i < n; i++) {
+                sumr = sumr + x[i] * OCOS(-2 * PI * i * tune);
+                sumi = sumi + x[i] * OSIN(-2 * PI * i * tune);
+            }
+            amp = sumr * sumr + sumi * sumi;
+            if (amp > peakamp) {
+                *peaktune = tune;
+                peakamp = amp;
+            }
+            tune = tune + tune_step;
+        } while (tune < tune_upper);
+        tune_lower = *peaktune - tune_step;
+        tune_upper = *peaktune + tune_step;
+        tune_step = tune_step / 10.;
+    } while (tune_step > 1.0e-12);
+}
+
+/* ------------------------------------------- elementary functions */
+EXPORT double orc_sin(double x) { return OSIN(x); }
+EXPORT double orc_cos(double x) { return OCOS(x); }
+EXPORT double orc_tan(double x) { return OTAN(x); }
+EXPORT double orc_asin(double x) { return OASIN(x); }
+EXPORT double orc_exp(double x) { return OEXP(x); }
+EXPORT double orc_powi(double x, int n) { return OPOW(x, n); }
+EXPORT double orc_log(double x) { return st_log(x); }
+EXPORT double orc_rng_normal(uint64_t seed, uint32_t particle, uint32_t counter)
+{
+    return st_rng_normal(seed, particle, counter);
+}
+EXPORT int orc_libm_kind(void)
+{
+#ifdef ORACLE_GLIBC
+    return 1;
+#else
+    return 0;
+#endif
+}
+EXPORT void orc_philox(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out[4])
+{
+    st_u32x4 c;
+    for (int i = 0; i < 4; i++) c.v[i] = ctr[i];
+    c = st_philox4x32_10(c, k0, k1);
+    for (int i = 0; i < 4; i++) out[i] = c.v[i];
+}

@@ -1,0 +1,517 @@
+/*
+ * st_libm.h -- project-owned FP64 elementary functions for the tracking path.
+ *
+ * The reference calls the host C library for sin, cos, asin, exp and pow
+ * inside its element maps (simtrack.cpp:3854 asin in sbend_exact_pass,
+ * :6009 sin in RFCAV_Pass, :6021-6022 CRABRF_Pass, :6040-6041 CCMULT_Pass,
+ * :6273 pow and :6313-6314 exp/cos/sin in cerrf, :6345 and :6647 exp in
+ * BB4D/BB6D, :4612 cos/sin in spin()).  Device and host libraries differ by
+ * an ulp now and then, and one ulp is enough to change the loss turn of a
+ * chaotic particle.  This header therefore implements those functions once,
+ * from published methods, in a subset of C that both gcc (the CPU oracle)
+ * and hipcc (the gfx950 kernels) compile to the same IEEE operation
+ * sequence: only +, -, *, /, sqrt, explicit fma and integer arithmetic,
+ * with floating-point contraction disabled on both sides.
+ *
+ * Methods:
+ *   sin/cos  argument reduction x - k*pi/2 by Cody-Waite with a four-part
+ *            pi/2 and FMA for |x| < 2^20, Payne-Hanek with an exact
+ *            integer product against the bits of 2/pi beyond; kernels are
+ *            the fdlibm-style compensated forms with Taylor coefficients
+ *            (truncation < 2^-62 relative on |r| <= pi/4).
+ *   asin     Maclaurin series of asin for |x| <= 1/2 and the half-angle
+ *            identity asin(x) = pi/2 - 2 asin(sqrt((1-x)/2)) above, with a
+ *            fma-exact square-root residual.
+ *   exp      k = rint(x/ln2), r = x - k ln2 in double-double, Taylor to
+ *            t^14/14!, scaled by 2^k through the exponent field.
+ *   powi     h^n for integer n by binary powering in double-double
+ *            arithmetic (exact products via fma) and one final rounding.
+ * Accuracy (tested against mpmath at 50 digits): < 1 ulp for all four.
+ *
+ * The includer defines, before including:
+ *   ST_FN            function qualifiers ("static inline" for C,
+ *                    "__device__ __forceinline__" for HIP)
+ *   ST_CONST_TABLE   qualifiers for the constant tables
+ *   ST_FMA(a,b,c)    exactly rounded fused multiply-add
+ *   ST_RINT(x)       round to nearest integer, ties to even
+ *   ST_SQRT(x)       correctly rounded square root
+ */
+#ifndef ST_LIBM_H
+#define ST_LIBM_H
+
+#include <stdint.h>
+#include "st_libm_tables.h"
+
+#if !defined(ST_FN) || !defined(ST_CONST_TABLE) || !defined(ST_FMA) || \
+    !defined(ST_RINT) || !defined(ST_SQRT)
+#error "define ST_FN, ST_CONST_TABLE, ST_FMA, ST_RINT, ST_SQRT before including st_libm.h"
+#endif
+
+ST_CONST_TABLE double st_asin_coefs[ST_ASIN_NCOEF] = ST_ASIN_COEFS;
+ST_CONST_TABLE uint32_t st_two_over_pi_words[ST_TWO_OVER_PI_NWORDS] = ST_TWO_OVER_PI_WORDS;
+
+/* ---------------------------------------------------------------- bits */
+
+ST_FN uint64_t st_d2u(double d) { uint64_t u; __builtin_memcpy(&u, &d, 8); return u; }
+ST_FN double st_u2d(uint64_t u) { double d; __builtin_memcpy(&d, &u, 8); return d; }
+ST_FN double st_fabs(double x) { return st_u2d(st_d2u(x) & 0x7fffffffffffffffull); }
+ST_FN int st_isnan(double x) { return x != x; }
+
+/* 2^k for -1022 <= k <= 1023, exact */
+ST_FN double st_pow2i(int k) { return st_u2d((uint64_t)(k + 1023) << 52); }
+
+/* ------------------------------------------------------ double-double */
+
+typedef struct { double hi, lo; } st_dd;
+
+ST_FN st_dd st_two_sum(double a, double b)
+{
+    st_dd r;
+    double s = a + b;
+    double bb = s - a;
+    r.hi = s;
+    r.lo = (a - (s - bb)) + (b - bb);
+    return r;
+}
+
+ST_FN st_dd st_fast_two_sum(double a, double b) /* |a| >= |b| */
+{
+    st_dd r;
+    double s = a + b;
+    r.hi = s;
+    r.lo = b - (s - a);
+    return r;
+}
+
+ST_FN st_dd st_two_prod(double a, double b)
+{
+    st_dd r;
+    double p = a * b;
+    r.hi = p;
+    r.lo = ST_FMA(a, b, -p);
+    return r;
+}
+
+ST_FN st_dd st_dd_add(st_dd a, st_dd b)
+{
+    st_dd s = st_two_sum(a.hi, b.hi);
+    double e = s.lo + (a.lo + b.lo);
+    return st_fast_two_sum(s.hi, e);
+}
+
+ST_FN st_dd st_dd_mul(st_dd a, st_dd b)
+{
+    st_dd p = st_two_prod(a.hi, b.hi);
+    double e = p.lo + (a.hi * b.lo + a.lo * b.hi);
+    return st_fast_two_sum(p.hi, e);
+}
+
+/* ---------------------------------------------------- sin/cos kernels */
+
+/* sin(x+y), |x| <= pi/4, |y| <= ulp(x)/2 */
+ST_FN double st_ksin(double x, double y, int has_y)
+{
+    double z = x * x;
+    double v = z * x;
+    double r = ST_FMA(z, ST_SIN_8, ST_SIN_7);
+    r = ST_FMA(z, r, ST_SIN_6);
+    r = ST_FMA(z, r, ST_SIN_5);
+    r = ST_FMA(z, r, ST_SIN_4);
+    r = ST_FMA(z, r, ST_SIN_3);
+    r = ST_FMA(z, r, ST_SIN_2);
+    if (!has_y)
+        return x + v * ST_FMA(z, r, ST_SIN_1);
+    return x - ((z * (0.5 * y - v * r) - y) - v * ST_SIN_1);
+}
+
+/* cos(x+y), |x| <= pi/4 */
+ST_FN double st_kcos(double x, double y)
+{
+    double z = x * x;
+    double r = ST_FMA(z, ST_COS_8, ST_COS_7);
+    r = ST_FMA(z, r, ST_COS_6);
+    r = ST_FMA(z, r, ST_COS_5);
+    r = ST_FMA(z, r, ST_COS_4);
+    r = ST_FMA(z, r, ST_COS_3);
+    r = ST_FMA(z, r, ST_COS_2);
+    r = ST_FMA(z, r, ST_COS_1);
+    r = z * r;
+    double hz = 0.5 * z;
+    double w = 1.0 - hz;
+    return w + (((1.0 - w) - hz) + (z * r - x * y));
+}
+
+/* ------------------------------------------------ argument reduction */
+
+/* Payne-Hanek: |x| >= 2^20, finite.  Returns the quadrant q (mod 4) and
+   r = x - q*pi/2 as a double-double.  With x = m*2^e (m a 53-bit integer),
+   the bits of 2/pi of weight above 2^(2-e) only add multiples of 4 to
+   x*2/pi, so a 192-bit window of 2/pi starting at fraction bit e-1 gives
+   x*2/pi mod 4 to 2^-137 absolute. */
+ST_FN int st_rem_pio2_large(double ax, st_dd *r)
+{
+    uint64_t bits = st_d2u(ax);
+    int e = (int)((bits >> 52) & 0x7ff) - 1075;       /* ax = m * 2^e, e >= -32 */
+    uint64_t m = (bits & 0x000fffffffffffffull) | 0x0010000000000000ull;
+    /* fraction bit i (weight 2^-i) sits at table bit offset i-1+64 counted
+       from the msb of word 0 (two zero pad words cover i <= 0) */
+    int off = e + 62;                                 /* offset of bit e-1 */
+    int j0 = off >> 5, s = off & 31;
+    uint64_t wv[3];
+    for (int k = 0; k < 3; ++k) {
+        uint64_t hi = ((uint64_t)st_two_over_pi_words[j0 + 2 * k] << 32) |
+                      (uint64_t)st_two_over_pi_words[j0 + 2 * k + 1];
+        uint64_t nx = (uint64_t)st_two_over_pi_words[j0 + 2 * k + 2];
+        wv[k] = s ? ((hi << s) | (nx >> (32 - s))) : hi;
+    }
+    /* W = wv[0]*2^128 + wv[1]*2^64 + wv[2];  P = m*W mod 2^192 */
+    unsigned __int128 t2 = (unsigned __int128)m * wv[2];
+    unsigned __int128 t1 = (unsigned __int128)m * wv[1];
+    unsigned __int128 t0 = (unsigned __int128)m * wv[0];
+    uint64_t p0 = (uint64_t)t2;
+    unsigned __int128 c1 = (t2 >> 64) + (unsigned __int128)(uint64_t)t1;
+    uint64_t p1 = (uint64_t)c1;
+    uint64_t p2 = (uint64_t)(c1 >> 64) + (uint64_t)(t1 >> 64) + (uint64_t)t0;
+    /* x*2/pi mod 4 = P * 2^-190: the quadrant is bits 63..62 of p2 */
+    int q = (int)(p2 >> 62);
+    uint64_t F = (p2 << 2) | (p1 >> 62);              /* fraction bits 1..64   */
+    uint64_t G = (p1 << 2) | (p0 >> 62);              /* fraction bits 65..128 */
+    double sign = 1.0;
+    if (F >> 63) {                                    /* fraction >= 1/2: f - 1 */
+        q = (q + 1) & 3;
+        G = ~G + 1u;
+        F = ~F + (G == 0u ? 1u : 0u);
+        sign = -1.0;
+    }
+    /* |f| = F*2^-64 + G*2^-128 split into three exact doubles */
+    double fa = (double)(F >> 11) * 0x1p-53;
+    double fb = (double)(((F & 0x7ffu) << 42) | (G >> 22)) * 0x1p-106;
+    double fc = (double)(G & 0x3fffffu) * 0x1p-128;
+    st_dd f = st_two_sum(fa, fb);
+    f = st_fast_two_sum(f.hi, f.lo + fc);
+    st_dd pio2;
+    pio2.hi = ST_PIO2_1;
+    pio2.lo = ST_PIO2_2;
+    st_dd rr = st_dd_mul(f, pio2);
+    r->hi = sign * rr.hi;
+    r->lo = sign * rr.lo;
+    return q;
+}
+
+/* r = x - q*pi/2 with |r| <~ pi/4; returns q mod 4.  x finite. */
+ST_FN int st_rem_pio2(double x, st_dd *r)
+{
+    double ax = st_fabs(x);
+    if (ax < 0x1p20) {
+        double k = ST_RINT(x * ST_TWO_OVER_PI);
+        double r1 = ST_FMA(-k, ST_PIO2_1, x);            /* exact */
+        st_dd t = st_two_prod(k, ST_PIO2_2);
+        st_dd s = st_two_sum(r1, -t.hi);
+        s.lo = s.lo - t.lo;
+        st_dd u = st_two_prod(k, ST_PIO2_3);
+        s.lo = s.lo - u.hi;
+        s.lo = s.lo - (u.lo + k * ST_PIO2_4);
+        *r = st_fast_two_sum(s.hi, s.lo);
+        int64_t ki = (int64_t)k;
+        return (int)(ki & 3);
+    }
+    int q = st_rem_pio2_large(ax, r);
+    if (x < 0.0) {
+        r->hi = -r->hi;
+        r->lo = -r->lo;
+        q = (4 - q) & 3;
+    }
+    return q;
+}
+
+/* ------------------------------------------------------------ sin/cos */
+
+ST_FN void st_sincos(double x, double *s, double *c)
+{
+    double ax = st_fabs(x);
+    if (!(ax <= 0x1.fffffffffffffp+1023)) {          /* inf or nan */
+        *s = x - x;
+        *c = x - x;
+        return;
+    }
+    if (ax <= ST_PIO4) {
+        if (ax < 0x1p-27) {
+            *s = x;
+            *c = 1.0;
+            return;
+        }
+        *s = st_ksin(x, 0.0, 0);
+        *c = st_kcos(x, 0.0);
+        return;
+    }
+    st_dd r;
+    int q = st_rem_pio2(x, &r);
+    double sr = st_ksin(r.hi, r.lo, 1);
+    double cr = st_kcos(r.hi, r.lo);
+    switch (q) {
+    case 0: *s = sr;  *c = cr;  break;
+    case 1: *s = cr;  *c = -sr; break;
+    case 2: *s = -sr; *c = -cr; break;
+    default: *s = -cr; *c = sr; break;
+    }
+}
+
+ST_FN double st_sin(double x)
+{
+    double s, c;
+    st_sincos(x, &s, &c);
+    return s;
+}
+
+ST_FN double st_cos(double x)
+{
+    double s, c;
+    st_sincos(x, &s, &c);
+    return c;
+}
+
+/* --------------------------------------------------------------- asin */
+
+ST_FN double st_asin_poly(double t, int n)
+{
+    double p = st_asin_coefs[n - 1];
+    for (int i = n - 2; i >= 0; --i)
+        p = ST_FMA(p, t, st_asin_coefs[i]);
+    return p;
+}
+
+ST_FN double st_asin(double x)
+{
+    double ax = st_fabs(x);
+    if (!(ax <= 1.0)) {
+        if (st_isnan(x))
+            return x + x;
+        return (x - x) / (x - x);                    /* |x| > 1: nan */
+    }
+    if (ax < 0x1p-26)
+        return x;
+    if (ax <= 0.5) {
+        double t = x * x;
+        double p = st_asin_poly(t, ax < 0.125 ? 10 : ST_ASIN_NCOEF);
+        return ST_FMA(x * t, p, x);
+    }
+    /* asin(ax) = pi/2 - 2*asin(s), s = sqrt((1-ax)/2) <= 1/2 */
+    double t = (1.0 - ax) * 0.5;                     /* exact */
+    double s = ST_SQRT(t);
+    double slo = ST_FMA(-s, s, t) / (s + s);          /* sqrt(t) ~ s + slo */
+    double p = st_asin_poly(t, ST_ASIN_NCOEF);
+    st_dd r = st_two_sum(ST_PIO2_1, -(s + s));
+    double corr = ST_PIO2_2 - 2.0 * (slo + (s * t) * p);
+    double res = r.hi + (r.lo + corr);
+    return x < 0.0 ? -res : res;
+}
+
+/* ---------------------------------------------------------------- exp */
+
+ST_FN double st_exp(double x)
+{
+    if (st_isnan(x))
+        return x + x;
+    if (x > 709.782712893384)
+        return 0x1p1023 * 2.0;                        /* +inf */
+    if (x < -745.1332191019412)
+        return 0.0;
+    if (st_fabs(x) < 0x1p-54)
+        return 1.0 + x;
+    double k = ST_RINT(x * ST_INV_LN2);
+    double rhi = ST_FMA(-k, ST_LN2_HI, x);           /* exact */
+    double rlo = -k * ST_LN2_LO;
+    double t = rhi + rlo;
+    double tlo = (rhi - t) + rlo;
+    double p = ST_FMA(t, ST_EXP_14, ST_EXP_13);
+    p = ST_FMA(t, p, ST_EXP_12);
+    p = ST_FMA(t, p, ST_EXP_11);
+    p = ST_FMA(t, p, ST_EXP_10);
+    p = ST_FMA(t, p, ST_EXP_9);
+    p = ST_FMA(t, p, ST_EXP_8);
+    p = ST_FMA(t, p, ST_EXP_7);
+    p = ST_FMA(t, p, ST_EXP_6);
+    p = ST_FMA(t, p, ST_EXP_5);
+    p = ST_FMA(t, p, ST_EXP_4);
+    p = ST_FMA(t, p, ST_EXP_3);
+    p = ST_FMA(t, p, ST_EXP_2);
+    double e = t + (tlo + (t * t) * p);
+    double y = 1.0 + e;
+    int ki = (int)k;
+    if (ki > 1023)
+        return y * 0x1p1023 * st_pow2i(ki - 1023);
+    if (ki < -1021)
+        return (y * st_pow2i(ki + 1000)) * 0x1p-1000;
+    return y * st_pow2i(ki);
+}
+
+/* --------------------------------------------------------------- powi */
+
+/* h^n for integer n (the reference calls pow(h, 1-nc) with integer 1-nc). */
+ST_FN double st_powi(double h, int n)
+{
+    if (n == 0)
+        return 1.0;
+    unsigned int e = (unsigned int)(n < 0 ? -n : n);
+    st_dd base;
+    base.hi = h;
+    base.lo = 0.0;
+    st_dd acc;
+    acc.hi = 1.0;
+    acc.lo = 0.0;
+    while (e) {
+        if (e & 1u)
+            acc = st_dd_mul(acc, base);
+        e >>= 1;
+        if (e)
+            base = st_dd_mul(base, base);
+    }
+    double ah = st_fabs(acc.hi);
+    if (!(ah <= 0x1.fffffffffffffp+1023) || ah < 0x1p-969) {
+        /* overflow, underflow or nan: plain products decide */
+        double v = 1.0, b = h;
+        e = (unsigned int)(n < 0 ? -n : n);
+        while (e) {
+            if (e & 1u)
+                v *= b;
+            e >>= 1;
+            if (e)
+                b *= b;
+        }
+        return n < 0 ? 1.0 / v : v;
+    }
+    if (n > 0)
+        return acc.hi + acc.lo;
+    /* 1 / (acc.hi + acc.lo) */
+    double q = 1.0 / acc.hi;
+    double r = ST_FMA(-q, acc.hi, 1.0) - q * acc.lo;
+    return ST_FMA(r, q, q);
+}
+
+/* ---------------------------------------------------------------- log */
+
+ST_FN double st_log(double x)
+{
+    if (!(x > 0.0)) {
+        if (x == 0.0)
+            return -(0x1p1023 * 2.0);                 /* -inf */
+        return (x - x) / (x - x);                     /* x < 0 or nan: nan */
+    }
+    if (x > 0x1.fffffffffffffp+1023)
+        return x;                                     /* +inf */
+    uint64_t u = st_d2u(x);
+    int k = 0;
+    if (u < 0x0010000000000000ull) {                  /* subnormal */
+        x *= 0x1p54;
+        u = st_d2u(x);
+        k = -54;
+    }
+    k += (int)(u >> 52) - 1023;
+    double m = st_u2d((u & 0x000fffffffffffffull) | 0x3ff0000000000000ull);
+    if (m > 2.0 * ST_SQRT_HALF) {                     /* m in [sqrt(2)/2, sqrt(2)) */
+        m *= 0.5;
+        k += 1;
+    }
+    double f = m - 1.0;                               /* exact */
+    double s = f / (2.0 + f);
+    double z = s * s;
+    double R = ST_FMA(z, ST_LOG_14, ST_LOG_13);
+    R = ST_FMA(z, R, ST_LOG_12);
+    R = ST_FMA(z, R, ST_LOG_11);
+    R = ST_FMA(z, R, ST_LOG_10);
+    R = ST_FMA(z, R, ST_LOG_9);
+    R = ST_FMA(z, R, ST_LOG_8);
+    R = ST_FMA(z, R, ST_LOG_7);
+    R = ST_FMA(z, R, ST_LOG_6);
+    R = ST_FMA(z, R, ST_LOG_5);
+    R = ST_FMA(z, R, ST_LOG_4);
+    R = ST_FMA(z, R, ST_LOG_3);
+    R = ST_FMA(z, R, ST_LOG_2);
+    R = ST_FMA(z, R, ST_LOG_1);
+    R = z * R;
+    double hfsq = 0.5 * f * f;
+    /* log(1+f) = 2 atanh(s) = f - (hfsq - s*(hfsq + R)) */
+    if (k == 0)
+        return f - (hfsq - s * (hfsq + R));
+    double dk = (double)k;
+    double t = hfsq - (s * (hfsq + R) + dk * ST_LN2_LO);
+    st_dd p = st_two_prod(dk, ST_LN2_HI);             /* exact k*ln2_hi */
+    st_dd a = st_two_sum(p.hi, f);                    /* exact */
+    return a.hi + ((a.lo + p.lo) - t);
+}
+
+/* ---------------------------------------------------------------- tan */
+
+/* tan = sin/cos of one reduction (element edge angles, host-side constants) */
+ST_FN double st_tan(double x)
+{
+    double s, c;
+    st_sincos(x, &s, &c);
+    if (st_fabs(x) < 0x1p-27)
+        return x;
+    return s / c;
+}
+
+/* ------------------------------------------------ Gaussian quantile */
+
+/*
+ * Inverse of the standard normal CDF.  The reference obtains beam-beam
+ * slice quantiles from GSL's gsl_cdf_ugaussian_Pinv (simtrack.cpp:6614,
+ * 6958; GSL not vendored, version unpinned).  GSL implements Wichura's
+ * algorithm AS 241 (PPND16, Applied Statistics 37 (1988) 477-484):
+ * rational approximations in q = p - 1/2 for |q| <= 0.425, and in
+ * r = sqrt(-log(min(p,1-p))) for r <= 5 and beyond, with numerator and
+ * denominator each evaluated by Horner's rule (no fma) and one division.
+ */
+ST_FN double st_ratpoly8(const double *a, const double *b, double x)
+{
+    double u = a[7], v = b[7];
+    for (int i = 7; i > 0; --i)
+        u = x * u + a[i - 1];
+    for (int j = 7; j > 0; --j)
+        v = x * v + b[j - 1];
+    return u / v;
+}
+
+ST_FN double st_ugaussian_pinv(double P)
+{
+    const double a[8] = {3.387132872796366608, 133.14166789178437745,
+                         1971.5909503065514427, 13731.693765509461125,
+                         45921.953931549871457, 67265.770927008700853,
+                         33430.575583588128105, 2509.0809287301226727};
+    const double b[8] = {1.0, 42.313330701600911252, 687.1870074920579083,
+                         5394.1960214247511077, 21213.794301586595867,
+                         39307.89580009271061, 28729.085735721942674,
+                         5226.495278852545925};
+    const double c[8] = {1.42343711074968357734, 4.6303378461565452959,
+                         5.7694972214606914055, 3.64784832476320460504,
+                         1.27045825245236838258, 0.24178072517745061177,
+                         0.0227238449892691845833, 7.7454501427834140764e-4};
+    const double d[8] = {1.0, 2.05319162663775882187, 1.6763848301838038494,
+                         0.68976733498510000455, 0.14810397642748007459,
+                         0.0151986665636164571966, 5.475938084995344946e-4,
+                         1.05075007164441684324e-9};
+    const double e[8] = {6.6579046435011037772, 5.4637849111641143699,
+                         1.7848265399172913358, 0.29656057182850489123,
+                         0.026532189526576123093, 0.0012426609473880784386,
+                         2.71155556874348757815e-5, 2.01033439929228813265e-7};
+    const double f[8] = {1.0, 0.59983220655588793769, 0.13692988092273580531,
+                         0.0148753612908506148525, 7.868691311456132591e-4,
+                         1.8463183175100546818e-5, 1.4215117583164458887e-7,
+                         2.04426310338993978564e-15};
+    double dP = P - 0.5;
+    if (P == 1.0)
+        return 0x1p1023 * 2.0;
+    if (P == 0.0)
+        return -(0x1p1023 * 2.0);
+    if (st_fabs(dP) <= 0.425) {
+        double r = 0.180625 - dP * dP;
+        return dP * st_ratpoly8(a, b, r);
+    }
+    double pp = (P < 0.5) ? P : 1.0 - P;
+    double r = ST_SQRT(-st_log(pp));
+    double x = (r <= 5.0) ? st_ratpoly8(c, d, r - 1.6) : st_ratpoly8(e, f, r - 5.0);
+    return (P < 0.5) ? -x : x;
+}
+
+#endif /* ST_LIBM_H */
