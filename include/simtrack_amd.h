@@ -150,6 +150,8 @@ typedef struct st_ctx st_ctx;
 typedef struct st_table st_table;
 
 int         st_abi_version(void);
+/* sizeof(st_element), sizeof(st_globals): lets bindings check their layout */
+void        st_struct_sizes(int *element_bytes, int *globals_bytes);
 /* Message of the last failing call on this thread ("" if none). */
 const char *st_last_error(void);
 

@@ -1,0 +1,98 @@
+"""ctypes loader of the CPU oracle (test infrastructure only).
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's CPU baseline;
+never by the simtrack_amd package.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_libs = {}
+
+
+def build():
+    subprocess.run(["sh", os.path.join(HERE, "build.sh")], check=True)
+
+
+def load(glibc=False):
+    key = bool(glibc)
+    if key in _libs:
+        return _libs[key]
+    path = os.path.join(HERE, "build", "liboracle_glibc.so" if glibc else "liboracle.so")
+    if not os.path.exists(path):
+        build()
+    lib = ctypes.CDLL(path)
+    P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    lib.orc_set_globals.argtypes = [P]
+    lib.orc_get_yoshida.argtypes = [P]
+    lib.orc_element_pass.argtypes = [P, P, I, ctypes.c_uint32]
+    lib.orc_element_pass.restype = I
+    lib.orc_track.argtypes = [P, I, P, I, I, P, P, P, I, P, I, I]
+    lib.orc_track.restype = I
+    lib.orc_radial_x04.argtypes = [P, I, P]
+    lib.orc_draws_per_turn.argtypes = [P, I, I]
+    lib.orc_draws_per_turn.restype = ctypes.c_uint64
+    lib.orc_cerrf.argtypes = [D, D, P, P]
+    lib.orc_bb4d.argtypes = [D, D, D, D, D, D, P, P]
+    lib.orc_bb_slice_centers.argtypes = [I, D, P]
+    lib.orc_fine_tune.argtypes = [I, P, D, D, P]
+    for f in ("orc_sin", "orc_cos", "orc_tan", "orc_asin", "orc_exp", "orc_log", "orc_ugaussian_pinv"):
+        getattr(lib, f).argtypes = [D]
+        getattr(lib, f).restype = D
+    lib.orc_powi.argtypes = [D, I]
+    lib.orc_powi.restype = D
+    lib.orc_rng_normal.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]
+    lib.orc_rng_normal.restype = D
+    lib.orc_philox.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, P]
+    _libs[key] = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+class Oracle:
+    """Reference restatement on the CPU: Track/Track_spin/Track_tbt/Track_Radial."""
+
+    def __init__(self, gp, records, glibc=False, nthreads=None):
+        self.lib = load(glibc)
+        self.gp = gp
+        self.records = np.ascontiguousarray(records)
+        self.nthreads = nthreads or min(16, os.cpu_count() or 1)
+
+    def _set(self):
+        self.lib.orc_set_globals(ctypes.byref(self.gp))
+
+    def track(self, x, nturn, spin=False, radial=False, stable=None, lost_turn=None, lost_post=None,
+              tbt=False, bpm_index=-1, nthreads=None):
+        self._set()
+        ncol = 9 if spin else 6
+        x = np.ascontiguousarray(x, dtype=np.float64).copy()
+        n = x.shape[0]
+        stable = np.ones(n, np.int32) if stable is None else np.ascontiguousarray(stable, np.int32).copy()
+        lt = np.zeros(n, np.int32) if lost_turn is None else np.ascontiguousarray(lost_turn, np.int32).copy()
+        lp = np.zeros(n, np.int32) if lost_post is None else np.ascontiguousarray(lost_post, np.int32).copy()
+        xt = np.zeros((n, nturn, ncol)) if tbt else None
+        mode = (1 if spin else 0) | (2 if radial else 0)
+        rc = self.lib.orc_track(_p(self.records), len(self.records), _p(x), n, int(nturn), _p(stable), _p(lt),
+                                _p(lp), mode, _p(xt), int(bpm_index), int(nthreads or self.nthreads))
+        if rc != 0:
+            raise RuntimeError(f"oracle track failed ({rc})")
+        if tbt:
+            return x, xt, stable, lt, lp
+        return x, stable, lt, lp
+
+    def element_pass(self, index, x, spin=False):
+        self._set()
+        x = np.ascontiguousarray(x, dtype=np.float64).copy()
+        rec = self.records[index:index + 1]
+        for p in range(x.shape[0]):
+            row = x[p]
+            rc = self.lib.orc_element_pass(_p(rec), row.ctypes.data_as(ctypes.c_void_p), 1 if spin else 0, p)
+            if rc != 0:
+                raise RuntimeError(f"oracle pass failed ({rc})")
+        return x

@@ -539,7 +539,7 @@ static void vector_cross_product(const double a[], const double b[], double c[])
 /* ------------------------------------------------------- radiation */
 
 /* Per-particle normal stream replacing gsl_ran_gaussian(gsl_r2, 1.0). */
-typedef struct { uint64_t seed; uint32_t particle; uint32_t counter; } orc_rng;
+typedef struct { uint64_t seed; uint32_t particle; uint64_t counter; } orc_rng;
 
 /* radiate, simtrack.cpp:4198-4234 */
 static void radiate(double x[], double L, double href, const double BLbrho[])
@@ -1935,9 +1935,9 @@ static int elem_spass(const st_element *e, double x[])
 
 /* ------------------------------------------------------- drivers */
 
-EXPORT int orc_element_pass(const st_element *e, double *x, int spin_on)
+EXPORT int orc_element_pass(const st_element *e, double *x, int spin_on, uint32_t pid)
 {
-    orc_rng rng = {GP.rng_seed, 0u, 0u};
+    orc_rng rng = {GP.rng_seed, pid, 0u};
     return spin_on ? elem_spass(e, x) : elem_pass(e, x, &rng);
 }
 
@@ -1954,7 +1954,8 @@ static int lost_check(const st_element *e, const double x[])
  * turn, or after element bpm_index (cpp:20885-20907) when bpm_index >= 0.
  */
 static int track_one(const st_element *line, int n, double *x, int nturn, int *stable, int *lost_turn,
-                     int *lost_post, int mode, const double *x04, double *x_tbt, int bpm_index, uint32_t pid)
+                     int *lost_post, int mode, const double *x04, double *x_tbt, int bpm_index, uint32_t pid,
+                     uint64_t draws_per_turn)
 {
     int spin_on = mode & 1, radial = mode & 2;
     int ncol = spin_on ? 9 : 6;
@@ -1966,6 +1967,7 @@ static int track_one(const st_element *line, int n, double *x, int nturn, int *s
         return 0;
     }
     for (int turn = 0; turn < nturn; turn++) {
+        rng.counter = (uint64_t)turn * draws_per_turn;
         if (x_tbt && bpm_index < 0)
             for (int k = 0; k < ncol; k++) x_tbt[(size_t)turn * ncol + k] = x[k];
         for (int j = 0; j < n; j++) {
@@ -1983,6 +1985,20 @@ static int track_one(const st_element *line, int n, double *x, int nturn, int *s
         }
     }
     return 0;
+}
+
+/*
+ * Quantum-excitation draws per turn: radiate1 runs once per slice of every
+ * bending SBEND when GP.radiate (SBEND_Pass, cpp:4781-4813); spin passes
+ * radiate classically only.  Draw k of turn t has number t*D + k.
+ */
+EXPORT uint64_t orc_draws_per_turn(const st_element *line, int n, int spin_on)
+{
+    uint64_t d = 0;
+    if (!GP.radiate || spin_on) return 0;
+    for (int j = 0; j < n; j++)
+        if (line[j].type == ST_SBEND && line[j].angle != 0.) d += (uint64_t)(line[j].nint > 0 ? line[j].nint : 0);
+    return d;
 }
 
 /* on-momentum z after every element (Track_Radial, cpp:21655-21661) */
@@ -2012,6 +2028,7 @@ EXPORT int orc_track(const st_element *line, int n, double *x, int np, int nturn
     double *x04 = NULL;
     int ncol = (mode & 1) ? 9 : 6;
     int err = 0;
+    uint64_t draws = orc_draws_per_turn(line, n, mode & 1);
     if (mode & 2) {
         x04 = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
         if (!x04) return -5;
@@ -2025,7 +2042,7 @@ EXPORT int orc_track(const st_element *line, int n, double *x, int np, int nturn
     for (int p = 0; p < np; p++) {
         double *tb = x_tbt ? x_tbt + (size_t)p * (size_t)nturn * ncol : NULL;
         err |= track_one(line, n, x + (size_t)p * ncol, nturn, stable + p, lost_turn + p, lost_post + p, mode, x04,
-                         tb, bpm_index, (uint32_t)p);
+                         tb, bpm_index, (uint32_t)p, draws);
     }
     free(x04);
     return err ? -2 : 0;
@@ -2067,7 +2084,7 @@ EXPORT double orc_asin(double x) { return OASIN(x); }
 EXPORT double orc_exp(double x) { return OEXP(x); }
 EXPORT double orc_powi(double x, int n) { return OPOW(x, n); }
 EXPORT double orc_log(double x) { return st_log(x); }
-EXPORT double orc_rng_normal(uint64_t seed, uint32_t particle, uint32_t counter)
+EXPORT double orc_rng_normal(uint64_t seed, uint32_t particle, uint64_t counter)
 {
     return st_rng_normal(seed, particle, counter);
 }

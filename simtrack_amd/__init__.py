@@ -1,0 +1,12 @@
+"""MI355X-native engine for SimTrack's tracking hot path.
+
+The element-by-element FP64 tracking runs in hand-written HIP kernels for
+gfx950 (csrc/), behind the C ABI of include/simtrack_amd.h.  This package
+holds the build script, the ctypes binding of that ABI, the Element/Line
+construction API mirroring the reference's C++ classes, and synthetic
+workloads.
+"""
+from . import build  # noqa: F401
+from ._capi import (ELEMENT_DTYPE, TRACK_RADIAL, TRACK_SPIN, TYPES, Context, Globals,  # noqa: F401
+                    STError, Table, load)
+from .line import *  # noqa: F401,F403

@@ -1,0 +1,209 @@
+"""ctypes binding of the C ABI in include/simtrack_amd.h.
+
+Python is glue here: the tracking runs in the HIP library.  Loading fails
+loudly when the library is missing; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import build as _build
+
+ELEMENT_DTYPE = np.dtype([
+    ("type", "<i4"), ("nint", "<i4"), ("norder", "<i4"), ("nslice", "<i4"),
+    ("treatment", "<i4"), ("reserved", "<i4", (3,)),
+    ("l", "<f8"), ("angle", "<f8"), ("e1", "<f8"), ("e2", "<f8"), ("k0l", "<f8"),
+    ("knl", "<f8", (11,)), ("knsl", "<f8", (11,)),
+    ("dx", "<f8"), ("dy", "<f8"), ("ds", "<f8"),
+    ("dtheta", "<f8"), ("dphi", "<f8"), ("dpsi", "<f8"),
+    ("apx", "<f8"), ("apy", "<f8"),
+    ("vrf", "<f8"), ("frf", "<f8"), ("phase0", "<f8"),
+    ("hkick", "<f8"), ("vkick", "<f8"),
+    ("theta", "<f8"),
+    ("np", "<f8"), ("bbscale", "<f8"), ("sigmal", "<f8"),
+    ("emitx", "<f8"), ("betax", "<f8"), ("alfax", "<f8"),
+    ("emity", "<f8"), ("betay", "<f8"), ("alfay", "<f8"),
+    ("cc", "<f8", (8,)),
+    ("rot_n", "<f8", (3,)), ("rot_angle", "<f8"),
+    ("m66", "<f8", (36,)),
+    ("xco_in", "<f8", (6,)), ("xco_out", "<f8", (6,)),
+])
+
+
+class Globals(ctypes.Structure):
+    """st_globals: the GP fields the element maps read (simtrack.h:28-68)."""
+    _fields_ = [
+        ("gamma", ctypes.c_double), ("beta", ctypes.c_double),
+        ("energy0", ctypes.c_double), ("energy", ctypes.c_double),
+        ("cp", ctypes.c_double), ("p", ctypes.c_double), ("m0", ctypes.c_double),
+        ("m", ctypes.c_double), ("q", ctypes.c_double), ("brho", ctypes.c_double),
+        ("G", ctypes.c_double),
+        ("Q", ctypes.c_int32), ("A", ctypes.c_int32), ("H_expand", ctypes.c_int32),
+        ("radiate", ctypes.c_int32), ("quad_fringe", ctypes.c_int32),
+        ("reserved", ctypes.c_int32), ("rng_seed", ctypes.c_uint64),
+    ]
+
+
+# element type codes (enum st_element_type)
+TYPES = {
+    "DRIFT": 1, "SBEND": 2, "GSBEND": 3, "QUAD": 4, "SKEWQ": 5, "SEXT": 6, "OCT": 7,
+    "MULT": 8, "GMULT": 9, "SBENDMULT": 10, "SMULT": 11, "GSBENDMULT": 12,
+    "KICKER": 13, "HKICKER": 14, "VKICKER": 15, "MARKER": 16, "BPM": 17, "HBPM": 18,
+    "VBPM": 19, "RFCAV": 20, "CRABRF": 21, "CCMULT": 22, "LBT": 23, "ILBT": 24,
+    "BEAMBEAM": 25, "CRBB": 26, "MATRIX": 27, "ROTAT": 28, "SNAKE": 29,
+}
+TRACK_SPIN = 1
+TRACK_RADIAL = 2
+
+# every symbol declared in include/simtrack_amd.h
+EXPORTS = [
+    "st_globals_default", "st_globals_set_energy", "st_abi_version", "st_struct_sizes",
+    "st_last_error", "st_ctx_create", "st_ctx_destroy", "st_table_build", "st_table_destroy",
+    "st_table_size", "st_track", "st_track_spin", "st_track_tbt", "st_track_radial",
+    "st_element_pass", "st_track_device", "st_ctx_sync", "st_last_gpu_seconds",
+]
+
+_lib = None
+
+
+class STError(RuntimeError):
+    pass
+
+
+def load(path=None):
+    """Load libsimtrack_amd.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    path = path or _build.library_path()
+    if not os.path.exists(path):
+        raise STError(f"HIP library not built: {path} (run simtrack_amd/build.py)")
+    lib = ctypes.CDLL(path)
+    P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    lib.st_last_error.restype = ctypes.c_char_p
+    lib.st_abi_version.restype = I
+    lib.st_struct_sizes.argtypes = [ctypes.POINTER(I), ctypes.POINTER(I)]
+    lib.st_globals_default.argtypes = [ctypes.POINTER(Globals)]
+    lib.st_globals_set_energy.argtypes = [ctypes.POINTER(Globals), D, D, D, D]
+    lib.st_ctx_create.argtypes = [I, ctypes.POINTER(P)]
+    lib.st_ctx_destroy.argtypes = [P]
+    lib.st_ctx_sync.argtypes = [P]
+    lib.st_table_build.argtypes = [P, ctypes.POINTER(Globals), P, I, ctypes.POINTER(P)]
+    lib.st_table_destroy.argtypes = [P]
+    lib.st_table_size.argtypes = [P, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_int64)]
+    for f in ("st_track", "st_track_spin", "st_track_radial"):
+        getattr(lib, f).argtypes = [P, P, I, I, P, P, P]
+    lib.st_track_tbt.argtypes = [P, P, I, I, P, I, I, P, P, P]
+    lib.st_element_pass.argtypes = [P, I, P, I, I]
+    lib.st_track_device.argtypes = [P, P, I, I, I, P, P, I]
+    lib.st_last_gpu_seconds.argtypes = [P, ctypes.POINTER(D)]
+    es, gs = I(), I()
+    lib.st_struct_sizes(ctypes.byref(es), ctypes.byref(gs))
+    if es.value != ELEMENT_DTYPE.itemsize or gs.value != ctypes.sizeof(Globals):
+        raise STError(f"ABI layout mismatch: element {es.value} vs {ELEMENT_DTYPE.itemsize}, "
+                      f"globals {gs.value} vs {ctypes.sizeof(Globals)}")
+    _lib = lib
+    return lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = _lib.st_last_error().decode() if _lib is not None else ""
+        raise STError(f"{what} failed ({rc}): {msg}")
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+class Context:
+    """A device context (HIP stream and work buffers) on one GPU."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        self.h = ctypes.c_void_p()
+        check(self.lib.st_ctx_create(device, ctypes.byref(self.h)), "st_ctx_create")
+
+    def close(self):
+        if self.h:
+            self.lib.st_ctx_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def gpu_seconds(self):
+        v = ctypes.c_double()
+        check(self.lib.st_last_gpu_seconds(self.h, ctypes.byref(v)), "st_last_gpu_seconds")
+        return v.value
+
+    def sync(self):
+        check(self.lib.st_ctx_sync(self.h), "st_ctx_sync")
+
+
+class Table:
+    """Packed device table of a line (st_table_build)."""
+
+    def __init__(self, ctx, gp, records):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.records = np.ascontiguousarray(records, dtype=ELEMENT_DTYPE)
+        self.n = len(self.records)
+        self.h = ctypes.c_void_p()
+        check(self.lib.st_table_build(ctx.h, ctypes.byref(gp), _ptr(self.records), self.n,
+                                      ctypes.byref(self.h)), "st_table_build")
+
+    def close(self):
+        if self.h:
+            self.lib.st_table_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def track(self, x, nturn, stable=None, lost_turn=None, lost_post=None, spin=False, radial=False):
+        """Track (spin: Track_spin, radial: Track_Radial).  Arrays are updated in place."""
+        ncol = 9 if spin else 6
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert x.ndim == 2 and x.shape[1] == ncol
+        npart = x.shape[0]
+        stable = np.ones(npart, np.int32) if stable is None else np.ascontiguousarray(stable, np.int32)
+        lost_turn = np.zeros(npart, np.int32) if lost_turn is None else np.ascontiguousarray(lost_turn, np.int32)
+        lost_post = np.zeros(npart, np.int32) if lost_post is None else np.ascontiguousarray(lost_post, np.int32)
+        f = self.lib.st_track_spin if spin else (self.lib.st_track_radial if radial else self.lib.st_track)
+        assert not (spin and radial)
+        check(f(self.h, _ptr(x), npart, int(nturn), _ptr(stable), _ptr(lost_turn), _ptr(lost_post)), "st_track")
+        return x, stable, lost_turn, lost_post
+
+    def track_tbt(self, x, nturn, bpm_index=-1, spin=False, x_tbt=None):
+        ncol = 9 if spin else 6
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        npart = x.shape[0]
+        stable = np.ones(npart, np.int32)
+        lost_turn = np.zeros(npart, np.int32)
+        lost_post = np.zeros(npart, np.int32)
+        if x_tbt is None:
+            x_tbt = np.zeros((npart, nturn, ncol))
+        check(self.lib.st_track_tbt(self.h, _ptr(x), npart, int(nturn), _ptr(x_tbt), int(bpm_index),
+                                    1 if spin else 0, _ptr(stable), _ptr(lost_turn), _ptr(lost_post)),
+              "st_track_tbt")
+        return x, x_tbt, stable, lost_turn, lost_post
+
+    def element_pass(self, index, x, spin=False):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        check(self.lib.st_element_pass(self.h, int(index), _ptr(x), x.shape[0], 1 if spin else 0),
+              "st_element_pass")
+        return x
+
+    def track_device(self, d_x, npart, turn0, nturn, d_state, d_lost, flags=0):
+        """Device-resident tracking; d_* are integer device addresses (e.g. torch data_ptr())."""
+        check(self.lib.st_track_device(self.h, ctypes.c_void_p(d_x), int(npart), int(turn0), int(nturn),
+                                       ctypes.c_void_p(d_state), ctypes.c_void_p(d_lost), int(flags)),
+              "st_track_device")

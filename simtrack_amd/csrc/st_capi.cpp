@@ -1,0 +1,1301 @@
+/*
+ * st_capi.cpp -- host side of the C ABI (include/simtrack_amd.h): context
+ * and buffer management, lowering of st_element records to the packed device
+ * table, and the tracking drivers (turn blocks with survivor compaction).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "simtrack_amd.h"
+
+#define ST_FN static inline
+#define ST_CONST_TABLE static const
+#define ST_FMA(a, b, c) std::fma((a), (b), (c))
+#define ST_RINT(x) std::nearbyint(x)
+#define ST_SQRT(x) std::sqrt(x)
+#include "st_libm.h"
+#include "st_kernels.h"
+#include "st_table_format.h"
+
+/* ------------------------------------------------------------ errors */
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                       \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess) return fail(ST_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_));          \
+    } while (0)
+
+extern "C" const char *st_last_error(void) { return g_err.c_str(); }
+extern "C" int st_abi_version(void) { return ST_ABI_VERSION; }
+extern "C" void st_struct_sizes(int *element_bytes, int *globals_bytes)
+{
+    if (element_bytes) *element_bytes = (int)sizeof(st_element);
+    if (globals_bytes) *globals_bytes = (int)sizeof(st_globals);
+}
+
+/* ------------------------------------------------------ reference GP */
+
+/* 4th-order Yoshida coefficients, simtrack.cpp:29-32 (run-time pow, as at the
+   reference's static initialisation) */
+static double Fdrift1, Fdrift2, Fkick1, Fkick2;
+static void init_yoshida()
+{
+    static bool done = false;
+    if (done) return;
+    volatile double two = 2.e0, third = 1.0 / 3.0;
+    Fdrift1 = 1.e0 / (2.e0 * (2.e0 - std::pow(two, third)));
+    Fdrift2 = 0.5e0 - Fdrift1;
+    Fkick1 = 2.e0 * Fdrift1;
+    Fkick2 = 1.e0 - 2e0 * Fkick1;
+    done = true;
+}
+
+extern "C" void st_globals_default(st_globals *gp)
+{
+    /* GlobalVariables(), simtrack.h:31-58 */
+    std::memset(gp, 0, sizeof *gp);
+    gp->A = 1;
+    gp->Q = 1;
+    gp->m0 = 1.6726216e-27;
+    gp->m = 1.6726216e-27;
+    gp->q = 1.6021765e-19;
+    gp->gamma = 106.;
+    gp->beta = 0.999955;
+    gp->energy0 = 938.27201;
+    gp->energy = 99456.8;
+    gp->p = 1.;
+    gp->cp = 1.;
+    gp->brho = 331.738;
+    gp->G = 1.7928474;
+    gp->H_expand = 0;
+    gp->radiate = 0;
+    gp->quad_fringe = 0;
+    gp->rng_seed = 0;
+}
+
+extern "C" void st_globals_set_energy(st_globals *gp, double gamma, double E0, double Q, double A)
+{
+    /* Set_RefPartEnergy, simtrack.cpp:87-108 */
+    gp->Q = (int32_t)Q;
+    gp->A = (int32_t)A;
+    gp->m0 = E0 * 1.0e6 * 1.6021765e-19 / 299792458. / 299792458.;
+    gp->m = gamma * gp->m0;
+    gp->q = Q * 1.6021765e-19;
+    gp->gamma = gamma;
+    gp->beta = std::sqrt(1.0 - 1.0 / gamma / gamma);
+    gp->energy0 = E0;
+    gp->energy = gamma * gp->energy0;
+    gp->cp = gp->energy * gp->beta;
+    gp->p = gp->energy * gp->beta * 1.0e6 * 1.6021765e-19 / 299792458.;
+    gp->brho = 3.33564095198 * gp->beta * (gp->energy / 1000) / gp->Q;
+}
+
+/* ------------------------------------------------------- structures */
+
+struct st_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_gpu_s = 0.0;
+    /* grow-only work buffers */
+    double *d_x = nullptr;
+    size_t cap_x = 0;
+    int *d_state = nullptr, *d_lost = nullptr, *d_idx0 = nullptr, *d_idx1 = nullptr, *d_block = nullptr;
+    int *d_total = nullptr, *h_total = nullptr;
+    size_t cap_p = 0;
+    double *d_tbt = nullptr;
+    size_t cap_tbt = 0;
+    double *d_x04 = nullptr;
+    size_t cap_x04 = 0;
+};
+
+struct dev_table {
+    st_op *d_ops = nullptr;
+    double *d_par = nullptr;
+    std::vector<st_op> ops;
+    std::vector<double> par;
+};
+
+struct st_table {
+    st_ctx *ctx = nullptr;
+    st_globals gp;
+    st_kglobals kg;
+    int n = 0;
+    dev_table orbit, spin;
+    double *d_ap = nullptr;
+    int64_t draws_per_turn = 0;
+    int radial_ok = 1;
+};
+
+/* ---------------------------------------------------------- lowering */
+
+namespace {
+
+struct Builder {
+    const st_globals &gp;
+    bool spin;
+    dev_table &t;
+    std::string err;
+
+    Builder(const st_globals &g, bool s, dev_table &tab) : gp(g), spin(s), t(tab) {}
+
+    int put(double v)
+    {
+        t.par.push_back(v);
+        return (int)t.par.size() - 1;
+    }
+
+    st_op new_op(int kind)
+    {
+        st_op op;
+        std::memset(&op, 0, sizeof op);
+        op.kind = kind;
+        op.po = (int)t.par.size();
+        op.pm = -1;
+        return op;
+    }
+
+    /* Cal_W_Matrix, simtrack.cpp:3485-3503 */
+    static void w_matrix(double theta, double phi, double psi, double W[9])
+    {
+        double costhe = st_cos(theta), cosphi = st_cos(phi), cospsi = st_cos(psi);
+        double sinthe = st_sin(theta), sinphi = st_sin(phi), sinpsi = st_sin(psi);
+        W[0] = +costhe * cospsi - sinthe * sinphi * sinpsi;
+        W[1] = -costhe * sinpsi - sinthe * sinphi * cospsi;
+        W[2] = sinthe * cosphi;
+        W[3] = cosphi * sinpsi;
+        W[4] = cosphi * cospsi;
+        W[5] = sinphi;
+        W[6] = -sinthe * cospsi - costhe * sinphi * sinpsi;
+        W[7] = +sinthe * sinpsi - costhe * sinphi * cospsi;
+        W[8] = costhe * cosphi;
+    }
+
+    /* mat_mult, simtrack.cpp:1351-1362 */
+    static void mat_mult(const double *a, const double *b, double *c, int m, int n, int k)
+    {
+        for (int i = 0; i <= m - 1; i++)
+            for (int j = 0; j <= k - 1; j++) {
+                int u = i * k + j;
+                c[u] = 0.0;
+                for (int l = 0; l <= n - 1; l++) c[u] = c[u] + a[i * n + l] * b[l * k + j];
+            }
+    }
+
+    /*
+     * Misalignment block of GtoL/LtoG (simtrack.cpp:3701-3741) for an element of
+     * length L and bend angle `angle`: offset/roll fast path (GtoL1/LtoG1) or
+     * the general MAD-X frame change (GtoL2/LtoG2) with its element-constant
+     * matrices evaluated here.
+     */
+    void misalign(st_op &op, const st_element &e, double L, double angle)
+    {
+        double dx = e.dx, dy = e.dy, ds = e.ds, theta = e.dtheta, phi = e.dphi, psi = e.dpsi;
+        double temp = std::fabs(dx) + std::fabs(dy) + std::fabs(ds) + std::fabs(theta) + std::fabs(phi) +
+                      std::fabs(psi);
+        if (temp < 1.0e-9) return;
+        if (std::fabs(theta) + std::fabs(phi) < 1.0e-9) {
+            op.flags |= ST_F_MIS1;
+            if (std::fabs(dx) + std::fabs(dy) > 1.0e-10) op.flags |= ST_F_MIS1_OFS;
+            if (std::fabs(psi) > 1.0e-10) op.flags |= ST_F_MIS1_ROT;
+            op.pm = put(dx);
+            put(dy);
+            put(st_cos(psi));
+            put(st_sin(psi));
+            return;
+        }
+        op.flags |= ST_F_MIS2;
+        const double beta = gp.beta, gamma = gp.gamma;
+        double W[9], RM[36];
+        /* GtoL2, cpp:3567-3626 */
+        w_matrix(theta, phi, psi, W);
+        double s2 = (W[0 * 3 + 2] * dx + W[1 * 3 + 2] * dy + W[2 * 3 + 2] * ds) / W[2 * 3 + 2];
+        for (int i = 0; i < 36; i++) RM[i] = 0.;
+        for (int i = 0; i < 6; i++) RM[i * 6 + i] = 1.0;
+        RM[1 * 6 + 1] = W[0 * 3 + 0];
+        RM[1 * 6 + 3] = W[1 * 3 + 0];
+        RM[1 * 6 + 5] = W[2 * 3 + 0] / beta;
+        RM[3 * 6 + 1] = W[0 * 3 + 1];
+        RM[3 * 6 + 3] = W[1 * 3 + 1];
+        RM[3 * 6 + 5] = W[2 * 3 + 1] / beta;
+        RM[0 * 6 + 0] = W[1 * 3 + 1] / W[2 * 3 + 2];
+        RM[0 * 6 + 1] = RM[0 * 6 + 0] * s2;
+        RM[0 * 6 + 2] = -W[0 * 3 + 1] / W[2 * 3 + 2];
+        RM[0 * 6 + 3] = RM[0 * 6 + 2] * s2;
+        RM[2 * 6 + 0] = -W[1 * 3 + 0] / W[2 * 3 + 2];
+        RM[2 * 6 + 1] = RM[2 * 6 + 0] * s2;
+        RM[2 * 6 + 2] = W[0 * 3 + 0] / W[2 * 3 + 2];
+        RM[2 * 6 + 3] = RM[2 * 6 + 2] * s2;
+        RM[4 * 6 + 0] = W[0 * 3 + 2] / (W[2 * 3 + 2] * beta);
+        RM[4 * 6 + 1] = RM[4 * 6 + 0] * s2;
+        RM[4 * 6 + 2] = W[1 * 3 + 2] / (W[2 * 3 + 2] * beta);
+        RM[4 * 6 + 3] = RM[4 * 6 + 2] * s2;
+        RM[4 * 6 + 5] = -s2 / (beta * gamma * beta * gamma);
+        op.pm = (int)t.par.size();
+        for (int i = 0; i < 36; i++) put(RM[i]);
+        put(-((W[1 * 3 + 1] * dx - W[0 * 3 + 1] * dy) / W[2 * 3 + 2]));
+        put(W[2 * 3 + 0]);
+        put(-((W[0 * 3 + 0] * dy - W[1 * 3 + 0] * dx) / W[2 * 3 + 2]));
+        put(W[2 * 3 + 1]);
+        put(-(s2 / beta));
+        /* LtoG2, cpp:3628-3699, with Cal_VE_WE_Matrx (cpp:3505-3565); an
+           angle of exactly +-1e-12 (left uninitialised by the reference) is
+           treated as straight */
+        double VE[3], WE[9], V[3], V1[3], M1[9], M2[9];
+        w_matrix(theta, phi, psi, W);
+        if (!(std::fabs(angle) > 1.0e-12)) {
+            VE[0] = VE[1] = 0.;
+            VE[2] = L;
+            for (int i = 0; i < 9; i++) WE[i] = 0.;
+            for (int i = 0; i < 3; i++) WE[i * 3 + i] = 1.0;
+        } else {
+            double tilt = 0.;
+            double cospsi = st_cos(tilt), sinpsi = st_sin(tilt);
+            double costhe = st_cos(angle), sinthe = st_sin(angle);
+            double tt = L * (costhe - 1.0) / angle;
+            VE[0] = tt * cospsi;
+            VE[1] = tt * sinpsi;
+            VE[2] = L * st_sin(angle) / angle;
+            WE[0 * 3 + 0] = costhe * cospsi * cospsi + sinpsi * sinpsi;
+            WE[1 * 3 + 0] = (costhe - 1.0) * cospsi * sinpsi;
+            WE[2 * 3 + 0] = sinthe * cospsi;
+            WE[0 * 3 + 1] = WE[1 * 3 + 0];
+            WE[1 * 3 + 1] = costhe * sinpsi * sinpsi + cospsi * cospsi;
+            WE[2 * 3 + 1] = sinthe * sinpsi;
+            WE[0 * 3 + 2] = -WE[2 * 3 + 0];
+            WE[1 * 3 + 2] = -WE[2 * 3 + 1];
+            WE[2 * 3 + 2] = costhe;
+        }
+        V[0] = dx + W[0 * 3 + 0] * VE[0] + W[0 * 3 + 1] * VE[1] + W[0 * 3 + 2] * VE[2] - VE[0];
+        V[1] = dy + W[1 * 3 + 0] * VE[0] + W[1 * 3 + 1] * VE[1] + W[1 * 3 + 2] * VE[2] - VE[1];
+        V[2] = ds + W[2 * 3 + 0] * VE[0] + W[2 * 3 + 1] * VE[1] + W[2 * 3 + 2] * VE[2] - VE[2];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) M1[i * 3 + j] = WE[j * 3 + i];
+        for (int i = 0; i < 3; i++) {
+            V1[i] = 0;
+            for (int j = 0; j < 3; j++) V1[i] = V1[i] + M1[i * 3 + j] * V[j];
+        }
+        for (int i = 0; i < 3; i++) V[i] = V1[i];
+        mat_mult(M1, W, M2, 3, 3, 3);
+        mat_mult(M2, WE, W, 3, 3, 3);
+        s2 = -(W[0 * 3 + 2] * V[0] + W[1 * 3 + 2] * V[1] + W[2 * 3 + 2] * V[2]) / W[2 * 3 + 2];
+        for (int i = 0; i < 36; i++) RM[i] = 0.;
+        for (int i = 0; i < 6; i++) RM[i * 6 + i] = 1.0;
+        RM[0 * 6 + 0] = W[0 * 3 + 0];
+        RM[2 * 6 + 0] = W[1 * 3 + 0];
+        RM[4 * 6 + 0] = W[2 * 3 + 0] / beta;
+        RM[0 * 6 + 2] = W[0 * 3 + 1];
+        RM[2 * 6 + 2] = W[1 * 3 + 1];
+        RM[4 * 6 + 2] = W[2 * 3 + 1] / beta;
+        RM[1 * 6 + 1] = W[1 * 3 + 1] / W[2 * 3 + 2];
+        RM[0 * 6 + 1] = RM[1 * 6 + 1] * s2;
+        RM[3 * 6 + 1] = -W[0 * 3 + 1] / W[2 * 3 + 2];
+        RM[2 * 6 + 1] = RM[3 * 6 + 1] * s2;
+        RM[1 * 6 + 3] = -W[1 * 3 + 0] / W[2 * 3 + 2];
+        RM[0 * 6 + 3] = RM[1 * 6 + 3] * s2;
+        RM[3 * 6 + 3] = W[0 * 3 + 0] / W[2 * 3 + 2];
+        RM[2 * 6 + 3] = RM[3 * 6 + 3] * s2;
+        RM[1 * 6 + 5] = W[0 * 3 + 2] / (W[2 * 3 + 2] * beta);
+        RM[0 * 6 + 5] = RM[1 * 6 + 5] * s2;
+        RM[3 * 6 + 5] = W[1 * 3 + 2] / (W[2 * 3 + 2] * beta);
+        RM[2 * 6 + 5] = RM[3 * 6 + 5] * s2;
+        RM[4 * 6 + 5] = -s2 / (beta * gamma * beta * gamma);
+        for (int i = 0; i < 36; i++) put(RM[i]);
+        put((W[1 * 3 + 1] * V[0] - W[0 * 3 + 1] * V[1]) / W[2 * 3 + 2]);
+        put(-W[2 * 3 + 0]);
+        put((W[0 * 3 + 0] * V[1] - W[1 * 3 + 0] * V[0]) / W[2 * 3 + 2]);
+        put(-W[2 * 3 + 1]);
+        put(-(s2 / beta));
+    }
+
+    int drift_op(double L, const st_element *mis_e = nullptr, double mis_L = 0., double mis_angle = 0.)
+    {
+        st_op op = new_op(L == 0.0 && !std::signbit(L) ? K_DRIFT0 : K_DRIFT);
+        if (op.kind == K_DRIFT) {
+            put(L);
+            put(L / gp.beta);
+        }
+        if (mis_e) misalign(op, *mis_e, mis_L, mis_angle);
+        t.ops.push_back(op);
+        return 0;
+    }
+
+    static int order_mask(const double *kn, const double *ks, int norder)
+    {
+        int m = 0;
+        for (int i = 1; i < norder + 1; i++)
+            if (kn[i] != 0. || ks[i] != 0.) m |= 1 << i;
+        return m;
+    }
+
+    /* QUAD_Pass (cpp:4882) / SEXT_Pass (cpp:5004) / OCT_Pass (cpp:5077) and sPass twins */
+    int multipole_op(const st_element &e, int thick_kind, int thin_kind, double kn, double ks)
+    {
+        double L = e.l;
+        int Nint = e.nint;
+        if (kn == 0. && ks == 0.) return drift_op(L, &e, L, 0.);
+        if (L == 0.) {
+            st_op op = new_op(thin_kind);
+            put(kn);
+            put(ks);
+            put((double)Nint);
+            misalign(op, e, L, 0.);
+            t.ops.push_back(op);
+            return 0;
+        }
+        if (Nint < 1) return -1;
+        st_op op = new_op(thick_kind);
+        op.nint = Nint;
+        double Lint = L / Nint;
+        put(Fdrift1 * Lint);
+        put((Fdrift1 * Lint) / gp.beta);
+        put(Fdrift2 * Lint);
+        put((Fdrift2 * Lint) / gp.beta);
+        put(Fkick1 * kn / Nint);
+        put(Fkick1 * ks / Nint);
+        put(Fkick2 * kn / Nint);
+        put(Fkick2 * ks / Nint);
+        put(kn);
+        put(ks);
+        put((double)Nint);
+        put(kn / L);
+        if (!spin && thick_kind == K_QUAD && gp.quad_fringe) op.flags |= ST_F_FRINGE;
+        misalign(op, e, L, 0.);
+        t.ops.push_back(op);
+        return 0;
+    }
+
+    void kick_arrays(const double *KNL, const double *KNSL, int Nint, int norder, st_op &op)
+    {
+        double k1[11], s1[11], k2[11], s2[11];
+        for (int i = 0; i < 11; i++) {
+            k1[i] = Fkick1 * KNL[i] / Nint;
+            s1[i] = Fkick1 * KNSL[i] / Nint;
+            k2[i] = Fkick2 * KNL[i] / Nint;
+            s2[i] = Fkick2 * KNSL[i] / Nint;
+        }
+        for (int i = 0; i <= norder; i++) put(k1[i]);
+        for (int i = 0; i <= norder; i++) put(s1[i]);
+        for (int i = 0; i <= norder; i++) put(k2[i]);
+        for (int i = 0; i <= norder; i++) put(s2[i]);
+        op.mask1 = order_mask(k1, s1, norder);
+        op.mask2 = order_mask(k2, s2, norder);
+    }
+
+    void plain_arrays(const double *KNL, const double *KNSL, int norder)
+    {
+        for (int i = 0; i <= norder; i++) put(KNL[i]);
+        for (int i = 0; i <= norder; i++) put(KNSL[i]);
+    }
+
+    /* MULT_Pass (cpp:5152) / MULT_sPass (cpp:7702).  mis_angle: the ANGLE the
+       owning element passes to GtoL/LtoG (0 for MULT, ANGLE for the bend
+       multipoles, whose L = 0 branch is a plain mult_kick_pass). */
+    int mult_op(const st_element &e, double mis_angle = 0.)
+    {
+        double L = e.l;
+        int Nint = e.nint, norder = e.norder;
+        if (L == 0.) {
+            st_op op = new_op(K_MULT_THIN);
+            op.norder = norder;
+            op.mask1 = order_mask(e.knl, e.knsl, norder);
+            plain_arrays(e.knl, e.knsl, norder);
+            misalign(op, e, L, mis_angle);
+            t.ops.push_back(op);
+            return 0;
+        }
+        if (Nint < 1) return -1;
+        st_op op = new_op(K_MULT);
+        op.nint = Nint;
+        op.norder = norder;
+        double Lint = L / Nint;
+        put(Fdrift1 * Lint);
+        put((Fdrift1 * Lint) / gp.beta);
+        put(Fdrift2 * Lint);
+        put((Fdrift2 * Lint) / gp.beta);
+        put((double)Nint);
+        put((double)order_mask(e.knl, e.knsl, norder));
+        kick_arrays(e.knl, e.knsl, Nint, norder, op);
+        plain_arrays(e.knl, e.knsl, norder);
+        misalign(op, e, L, 0.);
+        t.ops.push_back(op);
+        return 0;
+    }
+
+    /* sbend_exact_pass piece (L, href, 1/href, cos, sin, L/beta) */
+    void exact_piece(double L, double A, double c, double s)
+    {
+        double href = A / L;
+        put(L);
+        put(href);
+        put(1.0 / href);
+        put(c);
+        put(s);
+        put(L / gp.beta);
+    }
+
+    /* general_sbend_exact_pass piece (L, rhoc, b0, b0*rhoc, cos, sin, L/beta) */
+    void general_piece(double L, double A, double K0L, double c, double s)
+    {
+        double rhoc = L / A, b0 = K0L / L;
+        put(L);
+        put(rhoc);
+        put(b0);
+        put(b0 * rhoc);
+        put(c);
+        put(s);
+        put(L / gp.beta);
+    }
+
+    void yoshida_pieces(double Lint)
+    {
+        put(Fdrift1 * Lint);
+        put((Fdrift1 * Lint) / gp.beta);
+        put(Fdrift2 * Lint);
+        put((Fdrift2 * Lint) / gp.beta);
+        put(Fkick1 * Lint);
+        put(Fkick2 * Lint);
+    }
+
+    /* SBEND_Pass (cpp:4762) / SBEND_sPass (cpp:7486) */
+    int sbend_op(const st_element &e)
+    {
+        double L = e.l, Angle = e.angle;
+        int Nint = e.nint;
+        if (Angle == 0.) return drift_op(L, &e, L, Angle);
+        double href = Angle / L;
+        bool hexp = gp.H_expand != 0, rad = gp.radiate != 0;
+        if (!spin && !hexp && !rad) {
+            st_op op = new_op(K_SBEND_EXACT);
+            put(st_tan(e.e1) * href);
+            put(st_tan(e.e2) * href);
+            put(L);
+            put(href);
+            put(1.0 / href);
+            put(st_cos(Angle));
+            put(st_sin(Angle));
+            put(L / gp.beta);
+            misalign(op, e, L, Angle);
+            t.ops.push_back(op);
+            return 0;
+        }
+        if (Nint < 1) return -1;
+        st_op op = new_op(K_SBEND_SLICED);
+        op.nint = Nint;
+        double Lint = L / Nint, An = Angle / Nint;
+        put(st_tan(e.e1) * href);
+        put(st_tan(e.e2) * href);
+        put(href);
+        put(Lint);
+        put(An);
+        put((double)Nint);
+        if (hexp) {
+            yoshida_pieces(Lint);
+        } else {
+            double hs = An / Lint;
+            put(hs);
+            put(1.0 / hs);
+            put(st_cos(An));
+            put(st_sin(An));
+            put(Lint / gp.beta);
+        }
+        if (rad) op.flags |= ST_F_RAD;
+        misalign(op, e, L, Angle);
+        t.ops.push_back(op);
+        return 0;
+    }
+
+    /* GSBEND_Pass (cpp:4818), without the radiate2 branch */
+    int gsbend_op(const st_element &e)
+    {
+        double L = e.l, Angle = e.angle, K0L = e.k0l;
+        int Nint = e.nint;
+        if (Angle == 0.) return drift_op(L, &e, L, Angle);
+        double href = Angle / L, hreal = K0L / L;
+        if (gp.H_expand) {
+            if (Nint < 1) return -1;
+            st_op op = new_op(K_GSBEND_YOSHIDA);
+            op.nint = Nint;
+            double Lint = L / Nint;
+            put(st_tan(e.e1) * hreal);
+            put(st_tan(e.e2) * hreal);
+            put(Fdrift1 * Lint);
+            put((Fdrift1 * Lint) / gp.beta);
+            put(Fdrift2 * Lint);
+            put((Fdrift2 * Lint) / gp.beta);
+            put(Fkick1 * Lint);
+            put(Fkick2 * Lint);
+            put(href);
+            put(hreal);
+            misalign(op, e, L, Angle);
+            t.ops.push_back(op);
+            return 0;
+        }
+        st_op op = new_op(K_GSBEND_EXACT);
+        put(st_tan(e.e1) * hreal);
+        put(st_tan(e.e2) * hreal);
+        general_piece(L, Angle, K0L, st_cos(Angle), st_sin(Angle));
+        misalign(op, e, L, Angle);
+        t.ops.push_back(op);
+        return 0;
+    }
+
+    /* GMULT/SBENDMULT/SMULT_Pass (cpp:5223-5595) and their sPass twins */
+    int bendmult_op(const st_element &e)
+    {
+        double L = e.l, Angle = e.angle;
+        int Nint = e.nint, norder = e.norder;
+        if (L == 0.) {
+            if (!spin) return mult_op(e, Angle);
+            st_op op = new_op(K_BENDMULT_THIN);
+            op.norder = norder;
+            op.mask1 = order_mask(e.knl, e.knsl, norder);
+            put(Angle);
+            plain_arrays(e.knl, e.knsl, norder);
+            misalign(op, e, L, Angle);
+            t.ops.push_back(op);
+            return 0;
+        }
+        if (Nint < 1) return -1;
+        st_op op = new_op(K_BENDMULT);
+        op.nint = Nint;
+        op.norder = norder;
+        double href = Angle / L, Lint = L / Nint;
+        put(st_tan(e.e1) * href);
+        put(st_tan(e.e2) * href);
+        put(href);
+        put(Lint);
+        put((double)Nint);
+        put(Angle);
+        put(Angle / Nint);
+        put((double)order_mask(e.knl, e.knsl, norder));
+        if (gp.H_expand) {
+            yoshida_pieces(Lint);
+            for (int i = 0; i < 6; i++) put(0.);
+        } else {
+            double A1 = Fdrift1 * Angle / Nint, A2 = Fdrift2 * Angle / Nint;
+            exact_piece(Fdrift1 * Lint, A1, st_cos(A1), st_sin(A1));
+            exact_piece(Fdrift2 * Lint, A2, st_cos(A2), st_sin(A2));
+        }
+        kick_arrays(e.knl, e.knsl, Nint, norder, op);
+        plain_arrays(e.knl, e.knsl, norder);
+        if (gp.radiate) op.flags |= ST_F_RAD;
+        misalign(op, e, L, Angle);
+        t.ops.push_back(op);
+        return 0;
+    }
+
+    /* GSBENDMULT_Pass (cpp:5597) */
+    int gsbendmult_op(const st_element &e)
+    {
+        double L = e.l, Angle = e.angle, K0L = e.k0l;
+        int Nint = e.nint, norder = e.norder;
+        if (spin) return bendmult_op(e); /* GSBENDMULT_sPass == SBENDMULT_sPass (cpp:8024) */
+        if (L == 0.) return mult_op(e, Angle);
+        if (Nint < 1) return -1;
+        st_op op = new_op(K_GSBENDMULT);
+        op.nint = Nint;
+        op.norder = norder;
+        double href = Angle / L, hreal = K0L / L, Lint = L / Nint;
+        put(st_tan(e.e1) * hreal);
+        put(st_tan(e.e2) * hreal);
+        put(href);
+        put(hreal);
+        put(Lint);
+        put((double)Nint);
+        put(K0L / Nint);
+        put(0.);
+        if (gp.H_expand) {
+            yoshida_pieces(Lint);
+            for (int i = 0; i < 8; i++) put(0.);
+        } else {
+            double A1 = Fdrift1 * Angle / Nint, A2 = Fdrift2 * Angle / Nint;
+            general_piece(Fdrift1 * Lint, A1, Fdrift1 * K0L / Nint, st_cos(A1), st_sin(A1));
+            general_piece(Fdrift2 * Lint, A2, Fdrift2 * K0L / Nint, st_cos(A2), st_sin(A2));
+        }
+        kick_arrays(e.knl, e.knsl, Nint, norder, op);
+        if (gp.radiate) op.flags |= ST_F_RAD;
+        misalign(op, e, L, Angle);
+        t.ops.push_back(op);
+        return 0;
+    }
+
+    /* slice centroids of BB6D / BB6D_Angle (cpp:6594-6620, 6937-6962) */
+    static void slice_centers(int N_slice, double sigma_l, double *z)
+    {
+        if (N_slice == 11) {
+            static const double tab[11] = {1.7996765362153655,   1.1049618149811977,   0.75072324235797372,
+                                           0.47407703558819131,  0.23041113693387344,  0.,
+                                           -0.23041113693387344, -0.47407703558819131, -0.75072324235797372,
+                                           -1.1049618149811977,  -1.7996765362153655};
+            for (int i = 0; i < 11; i++) z[i] = tab[i] * sigma_l;
+            return;
+        }
+        double y[64];
+        for (int i = 0; i < N_slice - 1; i++) y[i] = st_ugaussian_pinv((i + 1) * 1.0 / N_slice);
+        z[0] = N_slice * (st_exp(-y[0] * y[0] / 2)) / std::sqrt(2.0 * 3.14159265);
+        z[N_slice - 1] = N_slice * (0 - st_exp(-y[0] * y[0] / 2)) / std::sqrt(2.0 * 3.14159265);
+        for (int i = 1; i < N_slice - 1; i++)
+            z[i] = N_slice * (st_exp(-y[i] * y[i] / 2) - st_exp(-y[i - 1] * y[i - 1] / 2)) / std::sqrt(2.0 * 3.14159265);
+        for (int i = 0; i < N_slice; i++) z[i] = z[i] * sigma_l;
+    }
+
+    int lower(const st_element &e)
+    {
+        const double PI = 3.14159265358979323;
+        switch (e.type) {
+        case ST_DRIFT: case ST_MARKER: case ST_BPM: case ST_HBPM: case ST_VBPM:
+            return drift_op(e.l);
+        case ST_ROTAT: case ST_SNAKE: {
+            if (!spin) return drift_op(e.l);
+            /* ROTAT_sPass / SNAKE_sPass, cpp:8286-8348 */
+            st_op op = new_op(K_ROTAT);
+            double costheta = st_cos(-e.rot_angle), sintheta = st_sin(-e.rot_angle);
+            const double *n = e.rot_n;
+            double F = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            double uF[3] = {n[0] / F, n[1] / F, n[2] / F};
+            double rot[9];
+            rot[0] = uF[0] * uF[0] * (1 - costheta) + costheta;
+            rot[1] = uF[0] * uF[1] * (1 - costheta) - uF[2] * sintheta;
+            rot[2] = uF[0] * uF[2] * (1 - costheta) + uF[1] * sintheta;
+            rot[3] = uF[0] * uF[1] * (1 - costheta) + uF[2] * sintheta;
+            rot[4] = uF[1] * uF[1] * (1 - costheta) + costheta;
+            rot[5] = uF[1] * uF[2] * (1 - costheta) - uF[0] * sintheta;
+            rot[6] = uF[0] * uF[2] * (1 - costheta) - uF[1] * sintheta;
+            rot[7] = uF[1] * uF[2] * (1 - costheta) + uF[0] * sintheta;
+            rot[8] = uF[2] * uF[2] * (1 - costheta) + costheta;
+            put(e.l / 2.);
+            put((e.l / 2.) / gp.beta);
+            for (int i = 0; i < 9; i++) put(rot[i]);
+            t.ops.push_back(op);
+            return 0;
+        }
+        case ST_SBEND:
+            return sbend_op(e);
+        case ST_GSBEND:
+            if (spin) return sbend_op(e); /* GSBEND::sPass calls SBEND_sPass, cpp:8619-8623 */
+            if (gp.radiate) {
+                err = "GSBEND with GP.radiate (radiate2 photon sampling) is not supported";
+                return -2;
+            }
+            return gsbend_op(e);
+        case ST_QUAD:
+            return multipole_op(e, K_QUAD, K_QUAD_THIN, e.knl[1], e.knsl[1]);
+        case ST_SKEWQ:
+            return multipole_op(e, K_QUAD, K_QUAD_THIN, 0., e.knsl[1]);
+        case ST_SEXT:
+            return multipole_op(e, K_SEXT, K_SEXT_THIN, e.knl[2], e.knsl[2]);
+        case ST_OCT:
+            return multipole_op(e, K_OCT, K_OCT_THIN, e.knl[3], e.knsl[3]);
+        case ST_MULT:
+            return mult_op(e);
+        case ST_GMULT: case ST_SBENDMULT: case ST_SMULT:
+            return bendmult_op(e);
+        case ST_GSBENDMULT:
+            return gsbendmult_op(e);
+        case ST_KICKER: case ST_HKICKER: case ST_VKICKER: {
+            /* KICK_Pass cpp:5887, KICK_sPass cpp:8193 */
+            st_op op = new_op(K_KICK);
+            double hk = e.type == ST_VKICKER ? 0. : e.hkick;
+            double vk = e.type == ST_HKICKER ? 0. : e.vkick;
+            put(e.l / 2.0);
+            put((e.l / 2.0) / gp.beta);
+            put(hk);
+            put(vk);
+            misalign(op, e, e.l, 0.);
+            t.ops.push_back(op);
+            return 0;
+        }
+        case ST_RFCAV: {
+            /* RFCAV_Pass cpp:6002 (sPass identical, cpp:10712) */
+            if (e.vrf == 0.) return drift_op(e.l, &e, e.l, 0.);
+            st_op op = new_op(K_RFCAV);
+            put(e.l / 2.);
+            put((e.l / 2.) / gp.beta);
+            put(std::fabs(e.vrf * gp.Q));
+            put(-2.0 * PI * e.frf);
+            put(e.phase0);
+            misalign(op, e, e.l, 0.);
+            t.ops.push_back(op);
+            return 0;
+        }
+        case ST_CRABRF: {
+            /* CRABRF_Pass cpp:6014; CRABRF::sPass is a drift (cpp:10808-10812) */
+            if (e.vrf == 0. || spin) return drift_op(e.l, &e, e.l, 0.);
+            st_op op = new_op(K_CRABRF);
+            put(e.l / 2.);
+            put((e.l / 2.) / gp.beta);
+            put(e.vrf * gp.Q);
+            put(2.0 * PI * e.frf);
+            put(e.phase0);
+            put(2.0 * PI * e.frf / 2.99792458e8);
+            misalign(op, e, e.l, 0.);
+            t.ops.push_back(op);
+            return 0;
+        }
+        case ST_CCMULT: {
+            /* CCMULT_Pass cpp:6027 (B1..A4 of the element feed B2..A5, cpp:10943-10947) */
+            if (e.vrf == 0.) return drift_op(e.l, &e, e.l, 0.);
+            st_op op = new_op(K_CCMULT);
+            double c = 2.99792458e8;
+            put(e.l / 2.);
+            put((e.l / 2.) / gp.beta);
+            put(2.0 * PI * e.frf);
+            put(e.phase0);
+            put(2 * PI * e.frf / c / gp.brho);
+            put(gp.brho);
+            for (int i = 0; i < 8; i++) put(e.cc[i]);
+            misalign(op, e, e.l, 0.);
+            t.ops.push_back(op);
+            return 0;
+        }
+        case ST_LBT: case ST_ILBT: {
+            /* LBT_Pass cpp:6069 / ILBT_Pass cpp:6095; sPass without misalignment */
+            st_op op = new_op(e.type == ST_LBT ? K_LBT : K_ILBT);
+            double sn = st_sin(e.theta), cs = st_cos(e.theta);
+            put(sn);
+            put(cs);
+            if (e.type == ST_LBT) {
+                put(sn / cs);
+                put(st_cos(e.theta));
+            }
+            if (!spin) misalign(op, e, e.l, 0.);
+            t.ops.push_back(op);
+            return 0;
+        }
+        case ST_BEAMBEAM: {
+            /* BEAMBEAM::Pass cpp:11191 -> BEAMBEAM_Pass cpp:7343 */
+            st_op op = new_op(K_NOP);
+            if (e.np != 0.) {
+                if (e.treatment == 6) {
+                    if (e.nslice < 2 || e.nslice > 64) {
+                        err = "BEAMBEAM NSLICE must be in [2, 64]";
+                        return -3;
+                    }
+                    op.kind = K_BB6D;
+                    op.norder = e.nslice;
+                    double Nps = 1.0 * e.np / e.nslice;
+                    put(2 * Nps * 1.53469826769e-18 / gp.gamma);
+                    put(2. * Nps * 1.534698e-18 / gp.gamma);
+                    put(e.bbscale);
+                    put((1.0 + e.alfax * e.alfax) / e.betax);
+                    put((1.0 + e.alfay * e.alfay) / e.betay);
+                    put(e.betax);
+                    put(e.alfax);
+                    put(e.betay);
+                    put(e.alfay);
+                    put(e.emitx);
+                    put(e.emity);
+                    put(2.0 * e.emitx);
+                    put(2.0 * e.emity);
+                    double z[64];
+                    slice_centers(e.nslice, e.sigmal, z);
+                    for (int i = 0; i < e.nslice; i++) put(z[i]);
+                } else {
+                    op.kind = K_BB4D;
+                    put(2 * e.np * 1.53469826769e-18 / gp.gamma);
+                    put(std::sqrt(e.emitx * e.betax));
+                    put(std::sqrt(e.emity * e.betay));
+                    put(e.bbscale);
+                }
+            }
+            misalign(op, e, e.l, 0.);
+            t.ops.push_back(op);
+            return 0;
+        }
+        case ST_CRBB: {
+            /* CRBB::Pass cpp:11442 -> BB6D_Angle cpp:6917 */
+            if (e.nslice < 2 || e.nslice > 64) {
+                err = "CRBB NSLICE must be in [2, 64]";
+                return -3;
+            }
+            st_op op = new_op(K_BB6DA);
+            op.norder = e.nslice;
+            double sn = st_sin(e.theta), cs = st_cos(e.theta), tn = sn / cs;
+            double Nps = 1.0 * e.np / e.nslice;
+            put(2 * Nps * 1.53469826769e-18 / gp.gamma);
+            put(2. * Nps * 1.534698e-18 / gp.gamma);
+            put(e.bbscale);
+            put(sn);
+            put(cs);
+            put(tn);
+            put(st_cos(e.theta));
+            double gx = (1.0 + e.alfax * e.alfax) / e.betax;
+            double gy = (1.0 + e.alfay * e.alfay) / e.betay;
+            put(gx / cs);
+            put(gy / cs);
+            put(e.betax * cs);
+            put(e.alfax);
+            put(e.betay * cs);
+            put(e.alfay);
+            put(e.emitx / cs);
+            put(e.emity / cs);
+            double z[64];
+            slice_centers(e.nslice, e.sigmal, z);
+            for (int i = 0; i < e.nslice; i++) put(z[i] / cs);
+            misalign(op, e, e.l, 0.);
+            t.ops.push_back(op);
+            return 0;
+        }
+        case ST_MATRIX: {
+            st_op op = new_op(K_MATRIX);
+            for (int i = 0; i < 6; i++) put(e.xco_in[i]);
+            for (int i = 0; i < 36; i++) put(e.m66[i]);
+            for (int i = 0; i < 6; i++) put(e.xco_out[i]);
+            t.ops.push_back(op);
+            return 0;
+        }
+        default:
+            return -2;
+        }
+    }
+};
+
+} // namespace
+
+/* ----------------------------------------------------------- context */
+
+extern "C" int st_ctx_create(int device, st_ctx **out)
+{
+    if (!out) return fail(ST_ERR_ARG, "st_ctx_create: out is null");
+    *out = nullptr;
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(ST_ERR_ARG, "st_ctx_create: device %d not in [0, %d)", device, n);
+    HIPCHK(hipSetDevice(device));
+    st_ctx *c = new st_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_total, sizeof(int)) != hipSuccess) {
+        st_ctx_destroy(c);
+        return fail(ST_ERR_DEVICE, "st_ctx_create: stream/event/pinned allocation failed");
+    }
+    init_yoshida();
+    *out = c;
+    return ST_OK;
+}
+
+static void free_bufs(st_ctx *c)
+{
+    hipFree(c->d_x);
+    hipFree(c->d_state);
+    hipFree(c->d_lost);
+    hipFree(c->d_idx0);
+    hipFree(c->d_idx1);
+    hipFree(c->d_block);
+    hipFree(c->d_total);
+    hipFree(c->d_tbt);
+    hipFree(c->d_x04);
+    c->d_x = nullptr;
+    c->d_state = c->d_lost = c->d_idx0 = c->d_idx1 = c->d_block = c->d_total = nullptr;
+    c->d_tbt = c->d_x04 = nullptr;
+    c->cap_x = c->cap_p = c->cap_tbt = c->cap_x04 = 0;
+}
+
+extern "C" int st_ctx_destroy(st_ctx *c)
+{
+    if (!c) return ST_OK;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    free_bufs(c);
+    if (c->h_total) hipHostFree(c->h_total);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return ST_OK;
+}
+
+extern "C" int st_ctx_sync(st_ctx *c)
+{
+    if (!c) return fail(ST_ERR_ARG, "st_ctx_sync: null context");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return ST_OK;
+}
+
+extern "C" int st_last_gpu_seconds(st_ctx *c, double *s)
+{
+    if (!c || !s) return fail(ST_ERR_ARG, "st_last_gpu_seconds: null argument");
+    *s = c->last_gpu_s;
+    return ST_OK;
+}
+
+static int ensure_particles(st_ctx *c, size_t np, int ncol)
+{
+    size_t need_x = np * (size_t)ncol;
+    if (need_x > c->cap_x) {
+        hipFree(c->d_x);
+        c->d_x = nullptr;
+        c->cap_x = 0;
+        HIPCHK(hipMalloc(&c->d_x, need_x * sizeof(double)));
+        c->cap_x = need_x;
+    }
+    if (np > c->cap_p) {
+        hipFree(c->d_state);
+        hipFree(c->d_lost);
+        hipFree(c->d_idx0);
+        hipFree(c->d_idx1);
+        hipFree(c->d_block);
+        hipFree(c->d_total);
+        c->d_state = c->d_lost = c->d_idx0 = c->d_idx1 = c->d_block = c->d_total = nullptr;
+        c->cap_p = 0;
+        size_t nb = (np + 1023) / 1024 + 1;
+        HIPCHK(hipMalloc(&c->d_state, np * sizeof(int)));
+        HIPCHK(hipMalloc(&c->d_lost, 2 * np * sizeof(int)));
+        HIPCHK(hipMalloc(&c->d_idx0, np * sizeof(int)));
+        HIPCHK(hipMalloc(&c->d_idx1, np * sizeof(int)));
+        HIPCHK(hipMalloc(&c->d_block, nb * sizeof(int)));
+        HIPCHK(hipMalloc(&c->d_total, sizeof(int)));
+        c->cap_p = np;
+    }
+    return ST_OK;
+}
+
+/* ------------------------------------------------------------- table */
+
+static int upload(dev_table &t)
+{
+    size_t no = t.ops.size() ? t.ops.size() : 1, np = t.par.size() ? t.par.size() : 1;
+    HIPCHK(hipMalloc(&t.d_ops, no * sizeof(st_op)));
+    HIPCHK(hipMalloc(&t.d_par, np * sizeof(double)));
+    if (!t.ops.empty()) HIPCHK(hipMemcpy(t.d_ops, t.ops.data(), t.ops.size() * sizeof(st_op), hipMemcpyHostToDevice));
+    if (!t.par.empty()) HIPCHK(hipMemcpy(t.d_par, t.par.data(), t.par.size() * sizeof(double), hipMemcpyHostToDevice));
+    return ST_OK;
+}
+
+extern "C" int st_table_destroy(st_table *t)
+{
+    if (!t) return ST_OK;
+    if (t->ctx) hipSetDevice(t->ctx->device);
+    hipFree(t->orbit.d_ops);
+    hipFree(t->orbit.d_par);
+    hipFree(t->spin.d_ops);
+    hipFree(t->spin.d_par);
+    hipFree(t->d_ap);
+    delete t;
+    return ST_OK;
+}
+
+static int validate(const st_element &e, int j)
+{
+    switch (e.type) {
+    case ST_MULT: case ST_GMULT: case ST_SBENDMULT: case ST_SMULT: case ST_GSBENDMULT:
+        if (e.norder < 0 || e.norder > 10)
+            return fail(ST_ERR_PARAM, "element %d: Norder %d outside [0, 10] (KNL has 11 entries)", j, e.norder);
+        break;
+    default:
+        break;
+    }
+    if (e.l < 0.) return fail(ST_ERR_PARAM, "element %d: negative length", j);
+    return ST_OK;
+}
+
+extern "C" int st_table_build(st_ctx *ctx, const st_globals *gp, const st_element *elems, int n, st_table **out)
+{
+    if (!ctx || !gp || !out || (n > 0 && !elems) || n < 0) return fail(ST_ERR_ARG, "st_table_build: bad argument");
+    *out = nullptr;
+    HIPCHK(hipSetDevice(ctx->device));
+    init_yoshida();
+    st_table *t = new st_table();
+    t->ctx = ctx;
+    t->gp = *gp;
+    t->n = n;
+    st_kglobals &k = t->kg;
+    std::memset(&k, 0, sizeof k);
+    k.beta = gp->beta;
+    k.gamma = gp->gamma;
+    k.sg = std::sqrt(gp->gamma * gp->gamma - 1.);
+    k.G = gp->G;
+    k.energy = gp->energy;
+    k.brho = gp->brho;
+    k.bg2 = gp->beta * gp->gamma * gp->beta * gp->gamma;
+    k.e3 = st_powi(gp->energy / 1000., 3);
+    k.Cr = 8.84627632682e-5;
+    k.Cf = (55. / 24. / std::sqrt(3.)) * 2.8179403e-15 * 1.054572e-34 / 9.1093822e-31;
+    k.rng_seed = gp->rng_seed;
+    k.H_expand = gp->H_expand;
+    k.radiate = gp->radiate;
+    std::vector<double> ap(2 * (size_t)(n > 0 ? n : 1));
+    for (int pass = 0; pass < 2; ++pass) {
+        Builder b(*gp, pass == 1, pass == 1 ? t->spin : t->orbit);
+        for (int j = 0; j < n; ++j) {
+            int rc = validate(elems[j], j);
+            if (rc) {
+                st_table_destroy(t);
+                return rc;
+            }
+            rc = b.lower(elems[j]);
+            if (rc) {
+                std::string msg = b.err.empty() ? "unsupported element type or parameters" : b.err;
+                st_table_destroy(t);
+                return fail(rc == -2 ? ST_ERR_TYPE : ST_ERR_PARAM, "element %d (type %d): %s", j, elems[j].type,
+                            msg.c_str());
+            }
+        }
+    }
+    for (int j = 0; j < n; ++j) {
+        ap[2 * j] = elems[j].apx;
+        ap[2 * j + 1] = elems[j].apy;
+        if (gp->radiate && elems[j].type == ST_SBEND && elems[j].angle != 0.)
+            t->draws_per_turn += elems[j].nint > 0 ? elems[j].nint : 0;
+    }
+    int rc = upload(t->orbit);
+    if (!rc) rc = upload(t->spin);
+    if (!rc) {
+        hipError_t e = hipMalloc(&t->d_ap, ap.size() * sizeof(double));
+        if (e == hipSuccess) e = hipMemcpy(t->d_ap, ap.data(), ap.size() * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = fail(ST_ERR_DEVICE, "table upload: %s", hipGetErrorString(e));
+    }
+    if (rc) {
+        st_table_destroy(t);
+        return rc;
+    }
+    *out = t;
+    return ST_OK;
+}
+
+extern "C" int st_table_size(const st_table *t, int *n_elems, int64_t *bytes)
+{
+    if (!t) return fail(ST_ERR_ARG, "st_table_size: null table");
+    if (n_elems) *n_elems = t->n;
+    if (bytes)
+        *bytes = (int64_t)((t->orbit.ops.size() + t->spin.ops.size()) * sizeof(st_op) +
+                           (t->orbit.par.size() + t->spin.par.size()) * sizeof(double) + 2 * t->n * sizeof(double));
+    return ST_OK;
+}
+
+/* ---------------------------------------------------------- tracking */
+
+static st_track_args base_args(st_table *t, bool spin)
+{
+    st_track_args a;
+    std::memset(&a, 0, sizeof a);
+    dev_table &d = spin ? t->spin : t->orbit;
+    a.ops = d.d_ops;
+    a.par = d.d_par;
+    a.ap = t->d_ap;
+    a.nelem = t->n;
+    a.tbt_bpm = -1;
+    a.g = t->kg;
+    a.draws_per_turn = spin ? 0 : t->draws_per_turn;
+    return a;
+}
+
+/* turns per launch: long enough to amortise launch and compaction, short
+   enough that compaction removes lost particles early */
+static int turn_block(int n_elems, int np)
+{
+    (void)np;
+    long work = (long)n_elems > 0 ? (long)n_elems : 1;
+    long tb = 200000 / work;
+    if (tb < 1) tb = 1;
+    if (tb > 256) tb = 256;
+    return (int)tb;
+}
+
+/* device-side core: d_x/d_state/d_lost already on the device */
+static int track_device_core(st_table *t, double *d_x, int np, int turn0, int nturn, int *d_state, int *d_lost,
+                             bool spin, bool radial, double *d_tbt, int tbt_bpm, int tbt_nturn)
+{
+    st_ctx *c = t->ctx;
+    if (ensure_particles(c, (size_t)np, 1)) return ST_ERR_NOMEM;
+    st_track_args a = base_args(t, spin);
+    a.x = d_x;
+    a.state = d_state;
+    a.lost = d_lost;
+    a.xtbt = d_tbt;
+    a.tbt_bpm = tbt_bpm;
+    a.tbt_nturn = tbt_nturn;
+    a.tbt_turn0 = 0;
+    if (radial) {
+        if ((size_t)t->n > c->cap_x04) {
+            hipFree(c->d_x04);
+            c->d_x04 = nullptr;
+            c->cap_x04 = 0;
+            HIPCHK(hipMalloc(&c->d_x04, (size_t)(t->n > 0 ? t->n : 1) * sizeof(double)));
+            c->cap_x04 = (size_t)t->n;
+        }
+        st_track_args ar = base_args(t, false);
+        HIPCHK(st_launch_radial_x04(ar, c->d_x04, t->gp.H_expand != 0, c->stream));
+        a.x04 = c->d_x04;
+    }
+    const bool hexp = t->gp.H_expand != 0, rad = t->gp.radiate != 0;
+    int tb = turn_block(t->n, np);
+    int nact = np;
+    const int *idx = nullptr;
+    int *bufs[2] = {c->d_idx0, c->d_idx1};
+    int which = 0;
+    bool first = true;
+    for (int t0 = turn0; t0 < turn0 + nturn && nact > 0;) {
+        int nt = std::min(tb, turn0 + nturn - t0);
+        a.turn0 = t0;
+        a.nturn = nt;
+        a.precheck = (first && turn0 == 0) ? 1 : 0;
+        a.idx = idx;
+        a.nact = nact;
+        HIPCHK(st_launch_track(a, spin, hexp, rad, c->stream));
+        first = false;
+        t0 += nt;
+        if (t0 < turn0 + nturn) {
+            int *out = bufs[which];
+            HIPCHK(st_launch_compact(idx, nact, d_state, c->d_block, out, c->d_total, c->stream));
+            HIPCHK(hipMemcpyAsync(c->h_total, c->d_total, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            nact = *c->h_total;
+            idx = out;
+            which ^= 1;
+        }
+    }
+    if (nturn == 0 && turn0 == 0) {
+        /* the reference's pre-check runs even for nturn == 0 */
+        a.turn0 = 0;
+        a.nturn = 0;
+        a.precheck = 1;
+        a.idx = nullptr;
+        a.nact = np;
+        HIPCHK(st_launch_track(a, spin, hexp, rad, c->stream));
+    }
+    return ST_OK;
+}
+
+static int track_host(st_table *t, double *x, int np, int nturn, int *stable, int *lost_turn, int *lost_post,
+                      bool spin, bool radial, double *x_tbt, int tbt_bpm)
+{
+    if (!t || (np > 0 && (!x || !stable || !lost_turn || !lost_post)) || np < 0 || nturn < 0)
+        return fail(ST_ERR_ARG, "track: bad argument");
+    if (np == 0) return ST_OK;
+    st_ctx *c = t->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    const int ncol = spin ? 9 : 6;
+    int rc = ensure_particles(c, (size_t)np, ncol);
+    if (rc) return rc;
+    std::vector<int> lost(2 * (size_t)np);
+    for (int p = 0; p < np; ++p) {
+        lost[2 * p] = lost_turn[p];
+        lost[2 * p + 1] = lost_post[p];
+    }
+    double *d_tbt = nullptr;
+    size_t tbt_n = 0;
+    if (x_tbt) {
+        tbt_n = (size_t)np * (size_t)nturn * ncol;
+        if (tbt_n > c->cap_tbt) {
+            hipFree(c->d_tbt);
+            c->d_tbt = nullptr;
+            c->cap_tbt = 0;
+            HIPCHK(hipMalloc(&c->d_tbt, (tbt_n ? tbt_n : 1) * sizeof(double)));
+            c->cap_tbt = tbt_n;
+        }
+        d_tbt = c->d_tbt;
+        if (tbt_n) HIPCHK(hipMemcpyAsync(d_tbt, x_tbt, tbt_n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(hipMemcpyAsync(c->d_x, x, (size_t)np * ncol * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_state, stable, (size_t)np * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_lost, lost.data(), 2 * (size_t)np * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    rc = track_device_core(t, c->d_x, np, 0, nturn, c->d_state, c->d_lost, spin, radial, d_tbt, tbt_bpm, nturn);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipMemcpyAsync(x, c->d_x, (size_t)np * ncol * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(stable, c->d_state, (size_t)np * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(lost.data(), c->d_lost, 2 * (size_t)np * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (x_tbt && tbt_n)
+        HIPCHK(hipMemcpyAsync(x_tbt, d_tbt, tbt_n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_gpu_s = ms * 1e-3;
+    for (int p = 0; p < np; ++p) {
+        lost_turn[p] = lost[2 * p];
+        lost_post[p] = lost[2 * p + 1];
+    }
+    return ST_OK;
+}
+
+extern "C" int st_track(st_table *t, double *x, int np, int nturn, int *stable, int *lost_turn, int *lost_post)
+{
+    return track_host(t, x, np, nturn, stable, lost_turn, lost_post, false, false, nullptr, -1);
+}
+
+extern "C" int st_track_spin(st_table *t, double *x, int np, int nturn, int *stable, int *lost_turn, int *lost_post)
+{
+    return track_host(t, x, np, nturn, stable, lost_turn, lost_post, true, false, nullptr, -1);
+}
+
+extern "C" int st_track_tbt(st_table *t, double *x, int np, int nturn, double *x_tbt, int bpm_index, int spin,
+                            int *stable, int *lost_turn, int *lost_post)
+{
+    if (!x_tbt && np > 0) return fail(ST_ERR_ARG, "st_track_tbt: x_tbt is null");
+    if (t && (bpm_index >= t->n)) return fail(ST_ERR_ARG, "st_track_tbt: bpm_index %d >= %d", bpm_index, t->n);
+    return track_host(t, x, np, nturn, stable, lost_turn, lost_post, spin != 0, false, x_tbt,
+                      bpm_index < 0 ? -1 : bpm_index);
+}
+
+extern "C" int st_track_radial(st_table *t, double *x, int np, int nturn, int *stable, int *lost_turn,
+                               int *lost_post)
+{
+    return track_host(t, x, np, nturn, stable, lost_turn, lost_post, false, true, nullptr, -1);
+}
+
+extern "C" int st_track_device(st_table *t, double *d_x, int np, int turn0, int nturn, int *d_state, int *d_lost,
+                               int flags)
+{
+    if (!t || np < 0 || nturn < 0 || turn0 < 0 || (np > 0 && (!d_x || !d_state || !d_lost)))
+        return fail(ST_ERR_ARG, "st_track_device: bad argument");
+    if (np == 0) return ST_OK;
+    HIPCHK(hipSetDevice(t->ctx->device));
+    return track_device_core(t, d_x, np, turn0, nturn, d_state, d_lost, (flags & ST_TRACK_SPIN) != 0,
+                             (flags & ST_TRACK_RADIAL) != 0, nullptr, -1, 0);
+}
+
+extern "C" int st_element_pass(st_table *t, int elem_index, double *x, int np, int spin)
+{
+    if (!t || elem_index < 0 || elem_index >= t->n || np < 0 || (np > 0 && !x))
+        return fail(ST_ERR_ARG, "st_element_pass: bad argument");
+    if (np == 0) return ST_OK;
+    st_ctx *c = t->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    const int ncol = spin ? 9 : 6;
+    int rc = ensure_particles(c, (size_t)np, ncol);
+    if (rc) return rc;
+    std::vector<int> ones((size_t)np, 1);
+    HIPCHK(hipMemcpyAsync(c->d_x, x, (size_t)np * ncol * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_state, ones.data(), (size_t)np * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    st_track_args a = base_args(t, spin != 0);
+    a.ops += elem_index;
+    a.nelem = 1;
+    a.ap = nullptr;
+    a.x = c->d_x;
+    a.state = c->d_state;
+    a.lost = c->d_lost;
+    a.nact = np;
+    a.turn0 = 0;
+    a.nturn = 1;
+    a.draws_per_turn = 0;
+    HIPCHK(st_launch_track(a, spin != 0, t->gp.H_expand != 0, t->gp.radiate != 0, c->stream));
+    HIPCHK(hipMemcpyAsync(x, c->d_x, (size_t)np * ncol * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return ST_OK;
+}

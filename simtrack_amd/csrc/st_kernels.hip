@@ -1,0 +1,720 @@
+/*
+ * st_kernels.hip -- gfx950 kernels of the tracking engine.
+ *
+ * k_track: one thread per particle, state in VGPRs, a wavefront-uniform loop
+ * over the packed element table (scalar loads), loss masking per element
+ * with the reference's test, whole-wavefront exit once every lane is lost.
+ * A launch covers a block of turns; between blocks the host compacts the
+ * surviving particle indices (k_compact_*), so later launches only spend
+ * wavefronts on particles that are still alive.
+ *
+ * Compiled with -ffp-contract=off: no FMA contraction of the element maps
+ * (explicit fma() only inside st_libm.h, identical on the host).
+ */
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#define ST_FN __device__ __forceinline__
+#define ST_CONST_TABLE static __device__ __constant__ const
+#define ST_FMA(a, b, c) __builtin_fma((a), (b), (c))
+#define ST_RINT(x) __builtin_rint(x)
+#define ST_SQRT(x) __builtin_sqrt(x)
+#include "st_libm.h"
+#include "st_rng.h"
+#include "st_maps.h"
+#include "st_kernels.h"
+
+/* ----------------------------------------------------- element dispatch */
+
+/* quadrupole hard-edge fringe of QUAD_Pass, simtrack.cpp:4895-4911 (entry, sgn=+1)
+   and cpp:4942-4957 (exit, sgn=-1) */
+ST_DEV void quad_fringe(St &s, double k, bool entry)
+{
+    double *x = s.x;
+    double x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3], x4 = x[4], x5 = x[5];
+    if (entry) {
+        x[0] = x0 + k * (x0 * x0 * x0 + 3 * x0 * x2 * x2) / 12 / (1 + x5);
+        x[1] = x1 - k * ((x0 * x0 + x2 * x2) * x1 - 2 * x0 * x2 * x3) / 4 / (1 + x5);
+        x[2] = x2 - k * (3 * x0 * x0 * x2 + x2 * x2 * x2) / 12 / (1 + x5);
+        x[3] = x3 + k * ((x0 * x0 + x2 * x2) * x3 - 2 * x0 * x2 * x1) / 4 / (1 + x5);
+        x[4] = x4 + k * (x2 * x2 * x2 * x3 - x0 * x0 * x0 * x1 + 3 * x0 * x0 * x2 * x3 - 3 * x2 * x2 * x0 * x1) / 12 /
+                        (1 + x5) / (1 + x5);
+    } else {
+        x[0] = x0 - k * (x0 * x0 * x0 + 3 * x0 * x2 * x2) / 12 / (1 + x5);
+        x[1] = x1 + k * ((x0 * x0 + x2 * x2) * x1 - 2 * x0 * x2 * x3) / 4 / (1 + x5);
+        x[2] = x2 + k * (3 * x0 * x0 * x2 + x2 * x2 * x2) / 12 / (1 + x5);
+        x[3] = x3 - k * ((x0 * x0 + x2 * x2) * x3 - 2 * x0 * x2 * x1) / 4 / (1 + x5);
+        x[4] = x4 - k * (x2 * x2 * x2 * x3 - x0 * x0 * x0 * x1 + 3 * x0 * x0 * x2 * x3 - 3 * x2 * x2 * x0 * x1) / 12 /
+                        (1 + x5) / (1 + x5);
+    }
+}
+
+/* the thin kick of a QUAD / SEXT / OCT op */
+template <int KK>
+ST_DEV void thin_kick(St &s, double kn, double ks)
+{
+    if (KK == K_QUAD) quad_kick(s, kn, ks);
+    if (KK == K_SEXT) sext_kick(s, kn, ks);
+    if (KK == K_OCT) oct_kick(s, kn, ks);
+}
+
+/* midpoint field of the spin passes of QUAD/SEXT/OCT (cpp:7574-7579, 7631-7636, 7686-7691) */
+template <int KK>
+ST_DEV void thin_field(double xm, double ym, double kn, double ks, double &B0, double &B1)
+{
+    if (KK == K_QUAD) {
+        double preal = xm, pimag = ym;
+        B0 = ks * preal + kn * pimag;
+        B1 = kn * preal - ks * pimag;
+    }
+    if (KK == K_SEXT) {
+        double preal = xm * xm - ym * ym;
+        double pimag = 2. * xm * ym;
+        B0 = (ks * preal + kn * pimag) / 2.;
+        B1 = (kn * preal - ks * pimag) / 2.;
+    }
+    if (KK == K_OCT) {
+        double preal = xm * xm * xm - 3 * xm * ym * ym;
+        double pimag = 3 * xm * xm * ym - ym * ym * ym;
+        B0 = (ks * preal + kn * pimag) / 6;
+        B1 = (kn * preal - ks * pimag) / 6;
+    }
+}
+
+/*
+ * QUAD_Pass / SEXT_Pass / OCT_Pass thick body (cpp:4913-4935, 5017-5033,
+ * 5090-5106) and their spin passes QUAD_sPass / SEXT_sPass / OCT_sPass
+ * (cpp:7561-7589, 7618-7644, 7673-7699).
+ */
+template <int KK, bool SPIN, bool HEXP>
+ST_DEV void thick_multipole(St &s, Ptc &c, int nint, int flags, const double *__restrict__ P, const st_kglobals &g)
+{
+    double d1 = P[0], d1b = P[1], d2 = P[2], d2b = P[3];
+    double kn1 = P[4], ks1 = P[5], kn2 = P[6], ks2 = P[7];
+    if (!SPIN && KK == K_QUAD && (flags & ST_F_FRINGE)) quad_fringe(s, P[11], true);
+    for (int i = 0; i < nint; i++) {
+        double xe = s.x[0], pxe = s.x[1], ye = s.x[2], pye = s.x[3];
+        drift<HEXP>(s, d1, d1b, c);
+        thin_kick<KK>(s, kn1, ks1);
+        drift<HEXP>(s, d2, d2b, c);
+        thin_kick<KK>(s, kn2, ks2);
+        drift<HEXP>(s, d2, d2b, c);
+        thin_kick<KK>(s, kn1, ks1);
+        drift<HEXP>(s, d1, d1b, c);
+        if (SPIN) {
+            double xm = (s.x[0] + xe) / 2., ym = (s.x[2] + ye) / 2.;
+            double B0, B1;
+            thin_field<KK>(xm, ym, P[8], P[9], B0, B1);
+            B0 = B0 / P[10];
+            B1 = B1 / P[10];
+            spin_rotate(s, xe, pxe, pye, 0, 0, B0, B1, 0., c, g);
+        }
+    }
+    if (!SPIN && KK == K_QUAD && (flags & ST_F_FRINGE)) quad_fringe(s, P[11], false);
+}
+
+/* thin (L = 0) QUAD/SEXT/OCT and their spin passes (cpp:7546-7559, 7603-7616, 7658-7671) */
+template <int KK, bool SPIN>
+ST_DEV void thin_multipole(St &s, Ptc &c, const double *__restrict__ P, const st_kglobals &g)
+{
+    double xe = s.x[0], pxe = s.x[1], ye = s.x[2], pye = s.x[3];
+    thin_kick<KK>(s, P[0], P[1]);
+    if (SPIN) {
+        double xm = (s.x[0] + xe) / 2., ym = (s.x[2] + ye) / 2.;
+        double B0, B1;
+        thin_field<KK>(xm, ym, P[0], P[1], B0, B1);
+        if (KK == K_OCT) {          /* OCT_sPass divides by Nint in its L == 0 branch too */
+            B0 = B0 / P[2];
+            B1 = B1 / P[2];
+        }
+        spin_rotate(s, xe, pxe, pye, 0, 0, B0, B1, 0., c, g);
+    }
+}
+
+/* MULT_Pass (cpp:5152-5182) and MULT_sPass (cpp:7702-7740), thick */
+template <bool SPIN, bool HEXP>
+ST_DEV void thick_mult(St &s, Ptc &c, const st_op &op, const double *__restrict__ P, const st_kglobals &g)
+{
+    int n1 = op.norder + 1;
+    const double *kn1 = P + 6, *ks1 = kn1 + n1, *kn2 = ks1 + n1, *ks2 = kn2 + n1, *KNL = ks2 + n1, *KNSL = KNL + n1;
+    double d1 = P[0], d1b = P[1], d2 = P[2], d2b = P[3], nint_d = P[4];
+    int mask0 = (int)P[5];
+    for (int i = 0; i < op.nint; i++) {
+        double xe = s.x[0], pxe = s.x[1], ye = s.x[2], pye = s.x[3];
+        drift<HEXP>(s, d1, d1b, c);
+        mult_kick(s, op.norder, op.mask1, kn1, ks1);
+        drift<HEXP>(s, d2, d2b, c);
+        mult_kick(s, op.norder, op.mask2, kn2, ks2);
+        drift<HEXP>(s, d2, d2b, c);
+        mult_kick(s, op.norder, op.mask1, kn1, ks1);
+        drift<HEXP>(s, d1, d1b, c);
+        if (SPIN) {
+            St m;
+            m.x[0] = (s.x[0] + xe) / 2.;
+            m.x[2] = (s.x[2] + ye) / 2.;
+            double By, Bx;
+            mult_field(m, op.norder, mask0, KNL, KNSL, By, Bx);
+            spin_rotate(s, xe, pxe, pye, 0, 0, Bx / nint_d, (0. + By) / nint_d, 0., c, g);
+        }
+    }
+}
+
+/* thin MULT: mult_kick_pass; spin: cal_Bfield + spin on the entry state, then kick (cpp:7708-7713) */
+template <bool SPIN>
+ST_DEV void thin_mult(St &s, Ptc &c, const st_op &op, const double *__restrict__ P, const st_kglobals &g)
+{
+    int n1 = op.norder + 1;
+    const double *KNL = P, *KNSL = P + n1;
+    if (SPIN) {
+        double By, Bx;
+        mult_field(s, op.norder, op.mask1, KNL, KNSL, By, Bx);
+        spin_rotate(s, s.x[0], s.x[1], s.x[3], 0, 0, Bx / 1, (0. + By) / 1, 0., c, g);
+    }
+    mult_kick(s, op.norder, op.mask1, KNL, KNSL);
+}
+
+/*
+ * SBENDMULT_Pass / GMULT_Pass / SMULT_Pass (cpp:5348-5470; identical
+ * bodies) and their spin passes (cpp:7835-7928), L > 0.
+ */
+template <bool SPIN, bool HEXP, bool RAD>
+ST_DEV void bendmult(St &s, Ptc &c, const st_op &op, const double *__restrict__ P, const st_kglobals &g)
+{
+    int n1 = op.norder + 1;
+    const double *kn1 = P + ST_BM_ARR, *ks1 = kn1 + n1, *kn2 = ks1 + n1, *ks2 = kn2 + n1, *KNL = ks2 + n1,
+                 *KNSL = KNL + n1;
+    double href = P[2], Lint = P[3], nint_d = P[4], Angle = P[5], An = P[6];
+    int mask0 = (int)P[7];
+    edge(s, P[0]);
+    for (int i = 0; i < op.nint; i++) {
+        double xe = s.x[0], pxe = s.x[1], ye = s.x[2], pye = s.x[3];
+        if (HEXP) {
+            double d1 = P[8], d1b = P[9], d2 = P[10], d2b = P[11], kl1 = P[12], kl2 = P[13];
+            drift<HEXP>(s, d1, d1b, c);
+            bend_mult_kick(s, kl1, href, op.norder, op.mask1, kn1, ks1);
+            drift<HEXP>(s, d2, d2b, c);
+            bend_mult_kick(s, kl2, href, op.norder, op.mask2, kn2, ks2);
+            drift<HEXP>(s, d2, d2b, c);
+            bend_mult_kick(s, kl1, href, op.norder, op.mask1, kn1, ks1);
+            drift<HEXP>(s, d1, d1b, c);
+        } else {
+            sbend_exact(s, P[8], P[9], P[10], P[11], P[12], P[13], c);
+            mult_kick(s, op.norder, op.mask1, kn1, ks1);
+            sbend_exact(s, P[14], P[15], P[16], P[17], P[18], P[19], c);
+            mult_kick(s, op.norder, op.mask2, kn2, ks2);
+            sbend_exact(s, P[14], P[15], P[16], P[17], P[18], P[19], c);
+            mult_kick(s, op.norder, op.mask1, kn1, ks1);
+            sbend_exact(s, P[8], P[9], P[10], P[11], P[12], P[13], c);
+        }
+        if (SPIN || RAD) {
+            St m;
+            m.x[0] = (s.x[0] + xe) / 2.;
+            m.x[2] = (s.x[2] + ye) / 2.;
+            double By, Bx;
+            mult_field(m, op.norder, mask0, KNL, KNSL, By, Bx);
+            double B0 = Bx / nint_d, B1 = (Angle + By) / nint_d;
+            if (SPIN) spin_rotate(s, xe, pxe, pye, An, href, B0, B1, 0., c, g);
+            if (RAD) radiate(s, Lint, href, B0, B1, 0., c, g);
+        }
+    }
+    edge(s, P[1]);
+}
+
+/* L = 0 branch of the bend-multipole passes (cpp:5350-5352; spin cpp:7846-7854) */
+template <bool SPIN>
+ST_DEV void bendmult_thin(St &s, Ptc &c, const st_op &op, const double *__restrict__ P, const st_kglobals &g)
+{
+    int n1 = op.norder + 1;
+    const double *KNL = P + 1, *KNSL = KNL + n1;
+    double xe = s.x[0], pxe = s.x[1], ye = s.x[2], pye = s.x[3];
+    mult_kick(s, op.norder, op.mask1, KNL, KNSL);
+    if (SPIN) {
+        St m;
+        m.x[0] = (s.x[0] + xe) / 2.;
+        m.x[2] = (s.x[2] + ye) / 2.;
+        double By, Bx;
+        mult_field(m, op.norder, op.mask1, KNL, KNSL, By, Bx);
+        spin_rotate(s, xe, pxe, pye, P[0], 0, Bx / 1, (P[0] + By) / 1, 0., c, g);
+    }
+}
+
+/* GSBENDMULT_Pass, cpp:5597-5723 (orbit only; its sPass is the SBENDMULT one) */
+template <bool HEXP, bool RAD>
+ST_DEV void gsbendmult(St &s, Ptc &c, const st_op &op, const double *__restrict__ P, const st_kglobals &g)
+{
+    int n1 = op.norder + 1;
+    const double *kn1 = P + ST_GBM_ARR, *ks1 = kn1 + n1, *kn2 = ks1 + n1, *ks2 = kn2 + n1;
+    double href = P[2], hreal = P[3], Lint = P[4], BLy = P[6];
+    edge(s, P[0]);
+    for (int i = 0; i < op.nint; i++) {
+        if (HEXP) {
+            double d1 = P[8], d1b = P[9], d2 = P[10], d2b = P[11], kl1 = P[12], kl2 = P[13];
+            drift<HEXP>(s, d1, d1b, c);
+            general_bend_mult_kick(s, kl1, href, hreal, op.norder, op.mask1, kn1, ks1);
+            drift<HEXP>(s, d2, d2b, c);
+            general_bend_mult_kick(s, kl2, href, hreal, op.norder, op.mask2, kn2, ks2);
+            drift<HEXP>(s, d2, d2b, c);
+            general_bend_mult_kick(s, kl1, href, hreal, op.norder, op.mask1, kn1, ks1);
+            drift<HEXP>(s, d1, d1b, c);
+        } else {
+            general_sbend_exact(s, P + 8, c);
+            mult_kick(s, op.norder, op.mask1, kn1, ks1);
+            general_sbend_exact(s, P + 15, c);
+            mult_kick(s, op.norder, op.mask2, kn2, ks2);
+            general_sbend_exact(s, P + 15, c);
+            mult_kick(s, op.norder, op.mask1, kn1, ks1);
+            general_sbend_exact(s, P + 8, c);
+        }
+        if (RAD) radiate(s, Lint, href, 0., BLy, 0., c, g);
+    }
+    edge(s, P[1]);
+}
+
+/*
+ * SBEND_Pass with slices (H_expand or radiate, cpp:4781-4813) and
+ * SBEND_sPass (cpp:7486-7536).
+ */
+template <bool SPIN, bool HEXP, bool RAD>
+ST_DEV void sbend_sliced(St &s, Ptc &c, int nint, const double *__restrict__ P, const st_kglobals &g,
+                         uint32_t pid, uint64_t &rctr)
+{
+    double href = P[2], Lint = P[3], An = P[4];
+    edge(s, P[0]);
+    for (int i = 0; i < nint; i++) {
+        double xe = s.x[0], pxe = s.x[1], pye = s.x[3];
+        if (HEXP) {
+            double d1 = P[6], d1b = P[7], d2 = P[8], d2b = P[9], kl1 = P[10], kl2 = P[11];
+            drift<HEXP>(s, d1, d1b, c);
+            bend_kick(s, kl1, href);
+            drift<HEXP>(s, d2, d2b, c);
+            bend_kick(s, kl2, href);
+            drift<HEXP>(s, d2, d2b, c);
+            bend_kick(s, kl1, href);
+            drift<HEXP>(s, d1, d1b, c);
+        } else {
+            sbend_exact(s, Lint, P[6], P[7], P[8], P[9], P[10], c);
+        }
+        if (SPIN) spin_rotate(s, xe, pxe, pye, An, href, 0., An, 0., c, g);
+        if (RAD) {
+            if (SPIN)
+                radiate(s, Lint, href, 0., An, 0., c, g);
+            else
+                radiate1(s, Lint, href, 0., An, 0., c, g, pid, rctr);
+        }
+    }
+    edge(s, P[1]);
+}
+
+/* GSBEND_Pass without radiation, cpp:4818-4880 */
+template <bool HEXP>
+ST_DEV void gsbend(St &s, Ptc &c, int kind, int nint, const double *__restrict__ P)
+{
+    edge(s, P[0]);
+    if (kind == K_GSBEND_EXACT) {
+        general_sbend_exact(s, P + 2, c);
+    } else {
+        double d1 = P[2], d1b = P[3], d2 = P[4], d2b = P[5], kl1 = P[6], kl2 = P[7], href = P[8], hreal = P[9];
+        for (int i = 0; i < nint; i++) {
+            drift<HEXP>(s, d1, d1b, c);
+            general_bend_kick(s, kl1, href, hreal);
+            drift<HEXP>(s, d2, d2b, c);
+            general_bend_kick(s, kl2, href, hreal);
+            drift<HEXP>(s, d2, d2b, c);
+            general_bend_kick(s, kl1, href, hreal);
+            drift<HEXP>(s, d1, d1b, c);
+        }
+    }
+    edge(s, P[1]);
+}
+
+/* CCMULT_Pass body, cpp:6034-6064 */
+ST_DEV void ccmult_kick(St &s, const double *__restrict__ P, const st_kglobals &g)
+{
+    double *x = s.x;
+    double scale = P[4], brho = P[5];
+    double B2 = P[6], A2 = P[7], B3 = P[8], A3 = P[9], B4 = P[10], A4 = P[11], B5 = P[12], A5 = P[13];
+    double sinangle, cosangle;
+    st_sincos(P[2] * x[4] / 2.99792458e8 + P[3], &sinangle, &cosangle);
+    double x1 = x[0], y1 = x[2];
+    double x2 = x1 * x1, x3 = x2 * x1, x4 = x3 * x1, x5 = x4 * x1;
+    double y2 = y1 * y1, y3 = y2 * y1, y4 = y3 * y1, y5 = y4 * y1;
+    double xy = x1 * y1, x2y = x2 * y1, xy2 = x1 * y2, x2y2 = x1 * xy2;
+    double x3y = y1 * x3, xy3 = x1 * y3;
+    double x2y3 = x2y2 * y1, x3y2 = x2y2 * x1, x4y = x4 * y1, xy4 = y4 * x1;
+    (void)g;
+    x[1] = x[1] + (B2 * x1 + A2 * y1) * cosangle / brho;
+    x[3] = x[3] + (A2 * x1 - B2 * y1) * cosangle / brho;
+    x[5] = x[5] + (B2 * (x2 - y2) * sinangle + A2 * xy * sinangle) * scale;
+    x[1] = x[1] + (B3 * (x2 - y2) + 2 * A3 * xy) * cosangle / brho;
+    x[3] = x[3] + (2 * B3 * xy + A3 * (x2 - y2)) * cosangle / brho;
+    x[5] = x[5] + (B3 * (x3 - 3 * xy2) * sinangle + A3 * (3 * x2y - y3) * sinangle) * scale / 3.;
+    x[1] = x[1] + (B4 * (x3 - 3 * xy2) - A4 * (y3 - 3 * x2y)) * cosangle / brho;
+    x[3] = x[3] + (B4 * (y3 - 3 * x2y) + A4 * (x3 - 3 * xy2)) * cosangle / brho;
+    x[5] = x[5] + (B4 * (x4 - 6 * x2 * y2 + y4) * sinangle + A4 * (4 * x3y - 4 * xy3) * sinangle) * scale / 4.;
+    x[1] = x[1] + (B5 * (x4 - 6 * x2y2 + y4) + A5 * (4 * x3y - 4 * xy3)) * cosangle / brho;
+    x[3] = x[3] + (B5 * (4 * xy3 - 4 * x3y) + A5 * (x4 - 6 * x2y2 + y4)) * cosangle / brho;
+    x[5] = x[5] + (B5 * (x5 - 10 * x3y2 + 5 * xy4) * sinangle + A5 * (5 * x4y - 10 * x2y3 + y5) * sinangle) * scale / 5.;
+}
+
+/*
+ * Apply one op: Element::Pass (SPIN = false) or Element::sPass (SPIN = true),
+ * including the GtoL/LtoG frame changes.
+ */
+template <bool SPIN, bool HEXP, bool RAD>
+ST_DEV void apply_op(St &s, Ptc &c, const st_op &op, const double *__restrict__ par, const st_kglobals &g,
+                     uint32_t pid, uint64_t &rctr)
+{
+    const double *__restrict__ P = par + op.po;
+    if (op.pm >= 0) {
+        mis_in(s, op.flags, par + op.pm);
+        if (op.flags & ST_F_MIS2) ptc_update(c, s.x[5], g);
+    }
+    double *x = s.x;
+    switch (op.kind) {
+    case K_DRIFT:
+        drift<HEXP>(s, P[0], P[1], c);
+        break;
+    case K_DRIFT0:
+        drift0<HEXP>(s, c);
+        break;
+    case K_QUAD:
+        thick_multipole<K_QUAD, SPIN, HEXP>(s, c, op.nint, op.flags, P, g);
+        break;
+    case K_SEXT:
+        thick_multipole<K_SEXT, SPIN, HEXP>(s, c, op.nint, op.flags, P, g);
+        break;
+    case K_OCT:
+        thick_multipole<K_OCT, SPIN, HEXP>(s, c, op.nint, op.flags, P, g);
+        break;
+    case K_QUAD_THIN:
+        thin_multipole<K_QUAD, SPIN>(s, c, P, g);
+        break;
+    case K_SEXT_THIN:
+        thin_multipole<K_SEXT, SPIN>(s, c, P, g);
+        break;
+    case K_OCT_THIN:
+        thin_multipole<K_OCT, SPIN>(s, c, P, g);
+        break;
+    case K_MULT:
+        thick_mult<SPIN, HEXP>(s, c, op, P, g);
+        break;
+    case K_MULT_THIN:
+        thin_mult<SPIN>(s, c, op, P, g);
+        break;
+    case K_SBEND_EXACT:
+        edge(s, P[0]);
+        sbend_exact(s, P[2], P[3], P[4], P[5], P[6], P[7], c);
+        edge(s, P[1]);
+        break;
+    case K_SBEND_SLICED:
+        sbend_sliced<SPIN, HEXP, RAD>(s, c, op.nint, P, g, pid, rctr);
+        break;
+    case K_GSBEND_EXACT:
+    case K_GSBEND_YOSHIDA:
+        gsbend<HEXP>(s, c, op.kind, op.nint, P);
+        break;
+    case K_BENDMULT:
+        bendmult<SPIN, HEXP, RAD>(s, c, op, P, g);
+        break;
+    case K_BENDMULT_THIN:
+        bendmult_thin<SPIN>(s, c, op, P, g);
+        break;
+    case K_GSBENDMULT:
+        gsbendmult<HEXP, RAD>(s, c, op, P, g);
+        break;
+    case K_KICK: /* KICK_Pass cpp:5887-5893, KICK_sPass cpp:8193-8208 */
+        drift<HEXP>(s, P[0], P[1], c);
+        if (SPIN) spin_rotate(s, x[0], x[1], x[3], 0, 0, P[3], -P[2], 0., c, g);
+        x[1] = x[1] + P[2];
+        x[3] = x[3] + P[3];
+        drift<HEXP>(s, P[0], P[1], c);
+        break;
+    case K_RFCAV: /* RFCAV_Pass, cpp:6002-6012 */
+        drift<HEXP>(s, P[0], P[1], c);
+        x[5] = x[5] + P[2] * st_sin(P[3] * x[4] / 2.99792458e8 + 3.14159265358979323 - P[4]) / g.beta / g.energy;
+        ptc_update(c, x[5], g);
+        drift<HEXP>(s, P[0], P[1], c);
+        break;
+    case K_CRABRF: { /* CRABRF_Pass, cpp:6014-6025 */
+        drift<HEXP>(s, P[0], P[1], c);
+        double sn, cs;
+        st_sincos(P[3] * x[4] / 2.99792458e8 + P[4], &sn, &cs);
+        x[1] = x[1] - P[2] * sn / g.beta / g.energy;
+        x[5] = x[5] - P[2] * cs * x[0] * P[5] / g.beta / g.energy;
+        ptc_update(c, x[5], g);
+        drift<HEXP>(s, P[0], P[1], c);
+        break;
+    }
+    case K_CCMULT: /* CCMULT_Pass, cpp:6027-6067 */
+        drift<HEXP>(s, P[0], P[1], c);
+        ccmult_kick(s, P, g);
+        ptc_update(c, x[5], g);
+        drift<HEXP>(s, P[0], P[1], c);
+        break;
+    case K_LBT:
+        lbt(s, P);
+        ptc_update(c, x[5], g);
+        break;
+    case K_ILBT:
+        ilbt(s, P);
+        ptc_update(c, x[5], g);
+        break;
+    case K_BB4D: { /* BEAMBEAM_Pass TREATMENT != 6, cpp:7350-7353 */
+        double Dpx, Dpy;
+        bb4d(x[0], x[2], P[0], P[1], P[2], Dpx, Dpy);
+        x[1] = x[1] + Dpx * P[3];
+        x[3] = x[3] + Dpy * P[3];
+        break;
+    }
+    case K_BB6D:
+        bb6d(s, op.norder, P);
+        ptc_update(c, x[5], g);
+        break;
+    case K_BB6DA:
+        bb6d_angle(s, op.norder, P);
+        ptc_update(c, x[5], g);
+        break;
+    case K_MATRIX: { /* MATRIX_Pass, cpp:5873-5885 */
+        double t[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) t[i] = x[i] - P[i];
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            double a = 0.0;
+#pragma unroll
+            for (int j = 0; j < 6; j++) a = a + P[6 + i * 6 + j] * t[j];
+            x[i] = a;
+        }
+#pragma unroll
+        for (int i = 0; i < 6; i++) x[i] = x[i] + P[42 + i];
+        ptc_update(c, x[5], g);
+        break;
+    }
+    case K_ROTAT: { /* ROTAT_sPass / SNAKE_sPass, cpp:8286-8348 */
+        drift<HEXP>(s, P[0], P[1], c);
+        double s0 = x[6], s1 = x[7], s2 = x[8];
+        x[6] = ((0. + P[2] * s0) + P[3] * s1) + P[4] * s2;
+        x[7] = ((0. + P[5] * s0) + P[6] * s1) + P[7] * s2;
+        x[8] = ((0. + P[8] * s0) + P[9] * s1) + P[10] * s2;
+        drift<HEXP>(s, P[0], P[1], c);
+        break;
+    }
+    default:
+        break;
+    }
+    if (op.pm >= 0) {
+        mis_out(s, op.flags, par + op.pm);
+        if (op.flags & ST_F_MIS2) ptc_update(c, s.x[5], g);
+    }
+}
+
+/* ------------------------------------------------------ tracking kernel */
+
+template <bool SPIN, bool HEXP, bool RAD>
+__global__ __launch_bounds__(256) void k_track(st_track_args a)
+{
+    constexpr int NC = SPIN ? 9 : 6;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = t < a.nact;
+    const int p = in ? (a.idx ? a.idx[t] : t) : 0;
+    St s;
+#pragma unroll
+    for (int k = 0; k < 9; k++) s.x[k] = 0.;
+    bool alive = false;
+    if (in) {
+        const double *xp = a.x + (size_t)p * NC;
+#pragma unroll
+        for (int k = 0; k < NC; k++) s.x[k] = xp[k];
+        alive = a.state[p] != 0;
+        if (a.precheck) {
+            /* Track, simtrack.cpp:20795-20796 */
+            if (__builtin_fabs(s.x[0]) > 0.1 || __builtin_fabs(s.x[2]) > 0.1 || !alive) {
+                a.state[p] = 0;
+                a.lost[2 * p] = 0;
+                a.lost[2 * p + 1] = 0;
+                alive = false;
+            }
+        }
+    }
+    uint64_t rctr = 0;
+    Ptc c;
+    ptc_update(c, s.x[5], a.g);
+    int lt = 0, lp = 0;
+    bool died = false;
+    const int tend = a.turn0 + a.nturn;
+    for (int turn = a.turn0; turn < tend; ++turn) {
+        if (!__any(alive)) break;
+        if (RAD) rctr = (uint64_t)turn * (uint64_t)a.draws_per_turn;
+        if (a.xtbt && a.tbt_bpm < 0 && alive) {
+            double *o = a.xtbt + ((size_t)p * a.tbt_nturn + (size_t)(turn - a.tbt_turn0)) * NC;
+#pragma unroll
+            for (int k = 0; k < NC; k++) o[k] = s.x[k];
+        }
+        for (int e = 0; e < a.nelem; ++e) {
+            const st_op op = a.ops[e];
+            if (alive) {
+                apply_op<SPIN, HEXP, RAD>(s, c, op, a.par, a.g, (uint32_t)p, rctr);
+                if (a.x04) s.x[4] = s.x[4] - a.x04[e];
+                if (a.xtbt && e == a.tbt_bpm) {
+                    double *o = a.xtbt + ((size_t)p * a.tbt_nturn + (size_t)(turn - a.tbt_turn0)) * NC;
+#pragma unroll
+                    for (int k = 0; k < NC; k++) o[k] = s.x[k];
+                }
+                /* loss test of Track, simtrack.cpp:20802 (none for a bare Element::Pass) */
+                if (a.ap && (__builtin_fabs(s.x[0]) > a.ap[2 * e] || __builtin_fabs(s.x[2]) > a.ap[2 * e + 1] ||
+                             s.x[0] != s.x[0] || s.x[2] != s.x[2])) {
+                    alive = false;
+                    died = true;
+                    lt = turn;
+                    lp = e;
+                }
+            }
+        }
+    }
+    if (in) {
+        double *xp = a.x + (size_t)p * NC;
+#pragma unroll
+        for (int k = 0; k < NC; k++) xp[k] = s.x[k];
+        if (died) {
+            a.state[p] = 0;
+            a.lost[2 * p] = lt;
+            a.lost[2 * p + 1] = lp;
+        }
+    }
+}
+
+/* on-momentum z after each element: one particle from x = 0 (Track_Radial, cpp:21655-21661) */
+template <bool HEXP>
+__global__ void k_radial_x04(st_track_args a, double *x04)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    St s;
+#pragma unroll
+    for (int k = 0; k < 9; k++) s.x[k] = 0.;
+    Ptc c;
+    uint64_t rctr = 0;
+    for (int e = 0; e < a.nelem; ++e) {
+        s.x[4] = 0.;
+        s.x[5] = 0.;
+        ptc_update(c, s.x[5], a.g);
+        const st_op op = a.ops[e];
+        apply_op<false, HEXP, false>(s, c, op, a.par, a.g, 0u, rctr);
+        x04[e] = s.x[4];
+    }
+}
+
+/* --------------------------------------------------- survivor compaction */
+
+/* per-block count of alive particles among idx[0..n) */
+__global__ __launch_bounds__(1024) void k_count_alive(const int *__restrict__ idx, int n, const int *__restrict__ state,
+                                                      int *__restrict__ block_count)
+{
+    __shared__ int wsum[16];
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int f = (t < n) ? (state[idx ? idx[t] : t] != 0) : 0;
+    unsigned long long b = __ballot(f);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wsum[w] = __popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += wsum[i];
+        block_count[blockIdx.x] = s;
+    }
+}
+
+/* exclusive scan of block counts (one block, loops for any count); total -> out_total */
+__global__ __launch_bounds__(1024) void k_scan_blocks(int *__restrict__ block_count, int nb, int *__restrict__ out_total)
+{
+    __shared__ int tmp[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nb; base += 1024) {
+        int i = base + threadIdx.x;
+        int v = (i < nb) ? block_count[i] : 0;
+        tmp[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int add = (threadIdx.x >= off) ? tmp[threadIdx.x - off] : 0;
+            __syncthreads();
+            tmp[threadIdx.x] += add;
+            __syncthreads();
+        }
+        int incl = tmp[threadIdx.x];
+        if (i < nb) block_count[i] = carry + incl - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out_total = carry;
+}
+
+/* order-preserving scatter of alive indices */
+__global__ __launch_bounds__(1024) void k_compact(const int *__restrict__ idx, int n, const int *__restrict__ state,
+                                                  const int *__restrict__ block_off, int *__restrict__ idx_out)
+{
+    __shared__ int woff[16];
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int pid = (t < n) ? (idx ? idx[t] : t) : 0;
+    int f = (t < n) ? (state[pid] != 0) : 0;
+    unsigned long long b = __ballot(f);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) woff[w] = __popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
+            int v = woff[i];
+            woff[i] = s;
+            s += v;
+        }
+    }
+    __syncthreads();
+    if (f) {
+        unsigned long long below = (lane == 0) ? 0ull : (b & ((~0ull) >> (64 - lane)));
+        int pos = block_off[blockIdx.x] + woff[w] + __popcll(below);
+        idx_out[pos] = pid;
+    }
+}
+
+/* ------------------------------------------------------------- launchers */
+
+#define ST_LAUNCH(S, H, R) \
+    hipLaunchKernelGGL((k_track<S, H, R>), dim3(grid), dim3(256), 0, stream, a)
+
+hipError_t st_launch_track(const st_track_args &a, bool spin, bool hexp, bool rad, hipStream_t stream)
+{
+    if (a.nact <= 0) return hipSuccess;
+    unsigned grid = (unsigned)((a.nact + 255) / 256);
+    if (!spin && !hexp && !rad) ST_LAUNCH(false, false, false);
+    else if (!spin && !hexp && rad) ST_LAUNCH(false, false, true);
+    else if (!spin && hexp && !rad) ST_LAUNCH(false, true, false);
+    else if (!spin && hexp && rad) ST_LAUNCH(false, true, true);
+    else if (spin && !hexp && !rad) ST_LAUNCH(true, false, false);
+    else if (spin && !hexp && rad) ST_LAUNCH(true, false, true);
+    else if (spin && hexp && !rad) ST_LAUNCH(true, true, false);
+    else ST_LAUNCH(true, true, true);
+    return hipGetLastError();
+}
+
+hipError_t st_launch_radial_x04(const st_track_args &a, double *x04, bool hexp, hipStream_t stream)
+{
+    if (hexp)
+        hipLaunchKernelGGL((k_radial_x04<true>), dim3(1), dim3(64), 0, stream, a, x04);
+    else
+        hipLaunchKernelGGL((k_radial_x04<false>), dim3(1), dim3(64), 0, stream, a, x04);
+    return hipGetLastError();
+}
+
+hipError_t st_launch_compact(const int *idx, int n, const int *state, int *block_buf, int *idx_out, int *d_total,
+                             hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    int nb = (n + 1023) / 1024;
+    hipLaunchKernelGGL(k_count_alive, dim3(nb), dim3(1024), 0, stream, idx, n, state, block_buf);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, stream, block_buf, nb, d_total);
+    hipLaunchKernelGGL(k_compact, dim3(nb), dim3(1024), 0, stream, idx, n, state, block_buf, idx_out);
+    return hipGetLastError();
+}

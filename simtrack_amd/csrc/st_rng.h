@@ -11,8 +11,11 @@
  * independent of scheduling.
  *
  * Generator: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random
- * numbers: as easy as 1, 2, 3", SC'11), counter (counter, particle,
- * 0x53544e44, 0), key (seed low, seed high).  Normal deviate: Box-Muller
+ * numbers: as easy as 1, 2, 3", SC'11), counter (draw low 32 bits, particle,
+ * 0x53544e44, draw high 32 bits), key (seed low, seed high).  The tracking
+ * drivers number the draws of a particle as turn * D + k, D the number of
+ * draws per turn and k the draw within the turn, so a run split into turn
+ * ranges draws the same numbers as one call.  Normal deviate: Box-Muller
  * z = sqrt(-2 log u1) cos(2 pi u2) with u1, u2 53-bit uniforms in (0,1),
  * evaluated with st_libm.h so host and device agree bit for bit.
  *
@@ -57,13 +60,13 @@ ST_FN double st_u01_53(uint32_t a, uint32_t b)
     return ((double)k + 0.5) * 0x1p-53;
 }
 
-ST_FN double st_rng_normal(uint64_t seed, uint32_t particle, uint32_t counter)
+ST_FN double st_rng_normal(uint64_t seed, uint32_t particle, uint64_t draw)
 {
     st_u32x4 c;
-    c.v[0] = counter;
+    c.v[0] = (uint32_t)draw;
     c.v[1] = particle;
     c.v[2] = 0x53544e44u;
-    c.v[3] = 0u;
+    c.v[3] = (uint32_t)(draw >> 32);
     st_u32x4 r = st_philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
     double u1 = st_u01_53(r.v[0], r.v[1]);
     double u2 = st_u01_53(r.v[2], r.v[3]);

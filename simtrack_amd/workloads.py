@@ -1,0 +1,354 @@
+"""Synthetic lattices and particle sets for tests and benchmarks.
+
+Everything is generated from explicit seeds with numpy's PCG64, so the CPU
+oracle and the GPU engine receive identical inputs.
+"""
+import math
+
+import numpy as np
+
+from . import line as L
+
+PROTON_E0 = 938.27201
+
+
+def proton_globals(total_energy_mev=100000.0, **flags):
+    """Set_RefPartEnergy(E/E0, 938.27201, 1, 1): a proton of total energy E."""
+    return L.make_globals(total_energy_mev / PROTON_E0, PROTON_E0, 1, 1, **flags)
+
+
+def electron_globals(total_energy_mev=18000.0, **flags):
+    e0 = 0.51099895
+    return L.make_globals(total_energy_mev / e0, e0, -1, 1, **flags)
+
+
+def _rng(seed):
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def all_types_line(seed=7, misalign=True, with_bb=True, with_rf=True, with_gsbend=True):
+    """One of every element type the engine tracks, moderate random strengths,
+    thick and thin variants, offset/roll and pitched misalignments."""
+    r = _rng(seed)
+    u = lambda a, b: float(r.uniform(a, b))  # noqa: E731
+    knl = [0.] * 11
+    knsl = [0.] * 11
+    for j in range(1, 9):
+        knl[j] = u(-1, 1) * 10.0 ** (-1 - 0.5 * j) * math.factorial(j)
+        knsl[j] = u(-1, 1) * 10.0 ** (-2 - 0.5 * j) * math.factorial(j)
+    knl[4] = 0.  # exercise the zero-order mask
+    knsl[4] = 0.
+    els = [
+        L.DRIFT("D1", 1.3), L.MARKER("M0", 0.), L.BPM("B0", 0.),
+        L.QUAD("QF", 1.1, 0.05, 0.002), L.QUAD("QT", 0., 0.03, -0.001), L.SKEWQ("SQ", 0.4, 0.01),
+        L.SEXT("SF", 0.5, 0.3, 0.02), L.SEXT("ST", 0., -0.2, 0.01),
+        L.OCT("OF", 0.3, 2.0, 0.5), L.OCT("OT", 0., -1.0, 0.2),
+        L.MULT("MU", 0.9, knl, knsl), L.MULT("MT", 0., knl, knsl),
+        L.SBEND("BE", 3.0, 0.02, 0.003, -0.002), L.SBEND("BN", 2.0, -0.015, 0., 0.),
+        L.SBENDMULT("SBM", 2.5, 0.018, 0.001, 0.001, knl, knsl),
+        L.GMULT("GM", 1.5, -0.012, 0., 0., knl, knsl), L.SMULT("SM", 1.2, 0.01, 0., 0., knl, knsl),
+        L.GSBENDMULT("GBM", 1.9, 0.014, 0.0139, 0., 0., knl, knsl),
+        L.SBENDMULT("SBT", 0., 0.01, 0., 0., knl, knsl),
+        L.KICKER("K", 0.4, 2e-5, -1e-5), L.HKICKER("HK", 0., 3e-5), L.VKICKER("VK", 0.2, -2e-5),
+        L.ROTAT("ROT", 0.5, [0.3, 0.9, 0.1], 0.2), L.SNAKE("SNK", 0.1, [1.0, 0., 0.2], math.pi / 2),
+        L.MATRIX("MAT", 0., np.zeros(6), np.zeros(6), np.eye(6) + 1e-3 * np.triu(np.ones((6, 6)), 1)),
+    ]
+    if with_gsbend:
+        els.insert(17, L.GSBEND("GB", 2.2, 0.016, 0.0161, 0.001, 0.0))
+    if with_rf:
+        els += [L.RFCAV("RF", 0.6, 3.0, 28.15e6, 0.05), L.RFCAV("RF0", 0.3, 0., 28e6, 0.),
+                L.CRABRF("CR", 0.4, 2.0, 197e6, 0.1),
+                L.CCMULT("CCM", 0., 25.8, 197e6, 0.0, 2.8e-3, -9.8e-7, 2.2e-3, -3.9e-5, 0.59, 2.3e-3, 0.1, 0.01)]
+    if with_bb:
+        eps = 2.5e-9
+        els += [L.LBT("LBT", 0.0125),
+                L.BEAMBEAM("BB6", 6, 1.72e11, -1.0, 0.02, 5, 20e-9, 0.45, 0., 1.3e-9, 0.056, 0.),
+                L.ILBT("ILBT", 0.0125),
+                L.BEAMBEAM("BB6R", 6, 2e11, 1.0, 0.3, 5, eps, 0.5, 0., eps, 0.5, 0.),
+                L.BEAMBEAM("BB6E", 6, 2e11, 1.0, 0.1, 11, 3 * eps, 0.6, 0.1, eps, 0.3, -0.05),
+                L.BEAMBEAM("BB4", 4, 1.5e11, 1.0, 0.1, 1, 4 * eps, 0.5, 0., eps, 0.5, 0.),
+                L.CRBB("CRB", 1.5e11, 0.01, 1.0, 0.05, 5, 2 * eps, 0.4, 0.02, eps, 0.2, 0.)]
+    for e in els:
+        if hasattr(e, "Nint") and e.L > 0:
+            e.Nint = max(1, min(e.Nint, 5))
+    if misalign:
+        for i, e in enumerate(els):
+            if e.TYPE in ("DRIFT", "MARKER", "BPM", "MATRIX", "ROTAT", "SNAKE"):
+                continue
+            k = i % 4
+            if k == 1:
+                e.DX, e.DY = u(-1e-4, 1e-4), u(-1e-4, 1e-4)
+            elif k == 2:
+                e.DX, e.DPSI = u(-1e-4, 1e-4), u(-1e-3, 1e-3)
+            elif k == 3:
+                e.DX, e.DY, e.DS = u(-1e-4, 1e-4), u(-1e-4, 1e-4), u(-1e-3, 1e-3)
+                e.DTHETA, e.DPHI, e.DPSI = u(-1e-4, 1e-4), u(-1e-4, 1e-4), u(-1e-3, 1e-3)
+    return L.Line(els)
+
+
+def random_particles(n, seed=1, scale=1e-3, spin=False, pt_scale=1e-4):
+    r = _rng(seed)
+    x = np.zeros((n, 9 if spin else 6))
+    x[:, 0] = r.normal(0, scale, n)
+    x[:, 1] = r.normal(0, scale * 0.05, n)
+    x[:, 2] = r.normal(0, scale, n)
+    x[:, 3] = r.normal(0, scale * 0.05, n)
+    x[:, 4] = r.normal(0, 0.05, n)
+    x[:, 5] = r.normal(0, pt_scale, n)
+    if spin:
+        v = r.normal(size=(n, 3))
+        v[:, 1] += 3.0
+        v /= np.linalg.norm(v, axis=1)[:, None]
+        x[:, 6:9] = v
+    return x
+
+
+# ------------------------------------------------------- DA ring (config 0)
+#
+# 48 FODO arc cells of 76 m (two 32 m sector dipoles, thick quadrupoles and
+# sextupoles as MULT with b3..b10 errors, a corrector, BPMs), two 91 m low-beta
+# straights with a matched quadruplet on each side of the IP (beta* = 1 m),
+# one 28 MHz RF cavity at h = 360.  999 elements, C = 3830 m.
+
+ARC_KQ = 0.03355          # arc quadrupole K1L (QF +, QD -), ~90 deg cells
+ARC_ANGLE = 2 * math.pi / 96
+IR_LQ = (2.0, 3.0, 3.0, 2.0)
+IR_GAP = (1.5, 1.5, 1.5, 1.0)
+IR_DRIFT = 30.0
+BETA_STAR = 1.0
+EMIT_N = 2.5e-6           # normalised rms emittance [m]
+R_ARC, R_IR = 0.04, 0.06  # multipole reference radii [m]
+
+
+def _m_quad(L, k):
+    def m(kk):
+        if kk > 0:
+            s = math.sqrt(kk)
+            return np.array([[math.cos(s * L), math.sin(s * L) / s], [-s * math.sin(s * L), math.cos(s * L)]])
+        s = math.sqrt(-kk)
+        return np.array([[math.cosh(s * L), math.sinh(s * L) / s], [s * math.sinh(s * L), math.cosh(s * L)]])
+    return m(k), m(-k)
+
+
+def _m_drift(L):
+    m = np.array([[1., L], [0., 1.]])
+    return m, m
+
+
+def _m_bend(L, th):
+    h = th / L
+    return np.array([[math.cos(th), math.sin(th) / h], [-h * math.sin(th), math.cos(th)]]), _m_drift(L)[1]
+
+
+def _arc_cell_linear():
+    d = _m_drift
+    return [_m_quad(0.75, ARC_KQ / 1.5), d(0.75), d(0.5), d(0.75), _m_bend(32., ARC_ANGLE), d(0.75), d(0.5),
+            d(0.75), _m_quad(1.5, -ARC_KQ / 1.5), d(0.75), d(0.5), d(0.75), _m_bend(32., ARC_ANGLE), d(0.75),
+            d(0.5), d(0.75), _m_quad(0.75, ARC_KQ / 1.5)]
+
+
+def _ir_half_linear(k):
+    seq = [_m_drift(IR_DRIFT)]
+    for i in range(4):
+        seq += [_m_quad(IR_LQ[i], k[i] / IR_LQ[i]), _m_drift(IR_GAP[i])]
+    return seq
+
+
+def _product(seq):
+    mx, my = np.eye(2), np.eye(2)
+    for a, b in seq:
+        mx, my = a @ mx, b @ my
+    return mx, my
+
+
+def _periodic(m):
+    mu = math.acos((m[0, 0] + m[1, 1]) / 2)
+    if m[0, 1] < 0:
+        mu = 2 * math.pi - mu
+    return m[0, 1] / math.sin(mu), (m[0, 0] - m[1, 1]) / (2 * math.sin(mu)), mu / (2 * math.pi)
+
+
+def _transport(m, b, a):
+    c, s, cp, sp = m[0, 0], m[0, 1], m[1, 0], m[1, 1]
+    g = (1 + a * a) / b
+    return c * c * b - 2 * c * s * a + s * s * g, -c * cp * b + (c * sp + s * cp) * a - s * sp * g
+
+
+def match_ir():
+    """Quadruplet K1L matching the IP (beta*, alpha = 0) onto the arc cell's
+    periodic Twiss in both planes (linear thick-element model)."""
+    from scipy.optimize import least_squares
+    mx, my = _product(_arc_cell_linear())
+    bx0, ax0, _ = _periodic(mx)
+    by0, ay0, _ = _periodic(my)
+
+    def res(k):
+        mx, my = _product(_ir_half_linear(k))
+        bx, ax = _transport(mx, BETA_STAR, 0.)
+        by, ay = _transport(my, BETA_STAR, 0.)
+        return [math.log(bx / bx0) if bx > 0 else 10., ax - ax0, math.log(by / by0) if by > 0 else 10., ay - ay0]
+
+    sol = least_squares(res, [-0.21, 0.27, -0.05, -0.42], xtol=1e-15, ftol=1e-15, gtol=1e-15)
+    return [float(v) for v in sol.x]
+
+
+# result of match_ir() (checked by the test suite)
+IR_K1L = (-0.21269946112305568, 0.27000198780176277, -0.049269261431258135, -0.4178277891509443)
+
+
+def ring_linear_tunes(k=IR_K1L):
+    h = _ir_half_linear(k)
+    full = h + _arc_cell_linear() * 24 + h[::-1] + h + _arc_cell_linear() * 24 + h[::-1]
+    mx, my = _product(full)
+    return _periodic(mx), _periodic(my)
+
+
+def _errors(r, main, order_main, R, orders):
+    """b/a errors of relative size 1e-4 of the main field at radius R:
+    KNL[j] R^j / j! = 1e-4 * (KNL[n] R^n / n!) * N(0,1)."""
+    kn = [0.] * 11
+    ks = [0.] * 11
+    kn[order_main] = main
+    ref = abs(main) * R ** order_main / math.factorial(order_main)
+    for j in orders:
+        kn[j] = 1e-4 * ref * float(r.normal()) * math.factorial(j) / R ** j
+        ks[j] = 1e-4 * ref * float(r.normal()) * math.factorial(j) / R ** j
+    return kn, ks
+
+
+def da_ring(seed=2026, k2f=0.10, k2d=-0.18, vrf=3.0, ip_elements=None, nint=4, aperture=0.5):
+    """The ~1000-element DA ring.  ip_elements: two element lists placed at
+    IP1 (start of the line) and IP2; default: zero-length markers."""
+    r = _rng(seed)
+
+    def mult(name, L, main, order, R, orders):
+        kn, ks = _errors(r, main, order, R, orders)
+        m = L_.MULT(name, L, kn, ks)
+        m.Nint = nint
+        return m
+
+    L_ = L
+    quad_err = range(2, 10)
+    sext_err = range(3, 10)
+
+    def cell(i):
+        return [
+            mult(f"QFH{i}a", 0.75, ARC_KQ / 2, 1, R_ARC, quad_err), L.DRIFT("D", 0.75),
+            mult(f"SF{i}", 0.5, k2f, 2, R_ARC, sext_err), L.DRIFT("D", 0.75), L.SBEND(f"B{i}a", 32., ARC_ANGLE),
+            L.DRIFT("D", 0.75), L.KICKER(f"C{i}", 0.5, 0., 0.), L.DRIFT("D", 0.75), L.BPM(f"BPM{i}a", 0.),
+            mult(f"QD{i}", 1.5, -ARC_KQ, 1, R_ARC, quad_err), L.DRIFT("D", 0.75),
+            mult(f"SD{i}", 0.5, k2d, 2, R_ARC, sext_err), L.DRIFT("D", 0.75), L.SBEND(f"B{i}b", 32., ARC_ANGLE),
+            L.DRIFT("D", 0.75), L.BPM(f"BPM{i}b", 0.), L.DRIFT("D", 0.5), L.MARKER(f"MK{i}", 0.),
+            L.DRIFT("D", 0.75), mult(f"QFH{i}b", 0.75, ARC_KQ / 2, 1, R_ARC, quad_err)]
+
+    def ir_out(tag):
+        seq = [L.DRIFT("DIR", IR_DRIFT)]
+        for q in range(4):
+            seq += [mult(f"Q{q + 1}{tag}", IR_LQ[q], IR_K1L[q], 1, R_IR, quad_err), L.DRIFT("DQ", IR_GAP[q])]
+        return seq
+
+    def ir_in(tag):
+        seq = []
+        for q in reversed(range(4)):
+            seq += [L.DRIFT("DQ", IR_GAP[q]), mult(f"Q{q + 1}{tag}", IR_LQ[q], IR_K1L[q], 1, R_IR, quad_err)]
+        return seq + [L.DRIFT("DIR", IR_DRIFT)]
+
+    ip1, ip2 = ip_elements or ([L.MARKER("IP1", 0.)], [L.MARKER("IP2", 0.)])
+    cells = []
+    cells += list(ip1) + ir_out("1R")
+    for i in range(24):
+        cells += cell(i)
+    cells += ir_in("2L") + list(ip2) + ir_out("2R")
+    for i in range(24, 48):
+        cells += cell(i)
+    cells += ir_in("1L")
+    # one RF cavity at h = 360 in the first arc cell
+    length = sum(e.L for e in cells)
+    g_beta = None
+    ln = L.Line(cells)
+    ln.info = dict(length=length)
+    for e in ln.Cell:
+        e.APx = e.APy = aperture
+    k = [i for i, e in enumerate(ln.Cell) if e.NAME == "MK0"][0]
+    ln.Cell[k] = L.RFCAV("RF28", 0., vrf, 0., 0.)     # FRF set by set_rf_frequency
+    ln.Cell[k].APx = ln.Cell[k].APy = aperture
+    ln.Update()
+    del g_beta
+    return ln
+
+
+def set_rf_frequency(ln, gp, harmonic=360):
+    """FRF = h * beta c / C for every RFCAV (Ex1:43 sets FRF = frev * h)."""
+    frev = 299792458. * gp.beta / ln.Length
+    for e in ln.Cell:
+        if e.TYPE == "RFCAV":
+            e.FRF = harmonic * frev
+
+
+def beam_sizes(gp, emit_n=EMIT_N, beta=BETA_STAR):
+    emit = emit_n / (gp.beta * gp.gamma)
+    return emit, math.sqrt(emit * beta)
+
+
+def da_grid(gp, n_amp=205, angles_deg=(15., 30., 45., 60., 75.), amax_sigma=20., pt=3e-4):
+    """DA scan grid (Track_DA1 angles, cpp:21455): x = (A cos t sx, 0, A sin t sy, 0, 0, pt),
+    A_k = (k+1) * amax/n_amp sigma.  x[5] carries deltap0 unconverted, as the
+    reference's DA drivers do (cpp:21415)."""
+    _, sig = beam_sizes(gp)
+    amps = (np.arange(n_amp) + 1) * amax_sigma / n_amp
+    rows = []
+    for t in angles_deg:
+        th = math.radians(t)
+        for a in amps:
+            rows.append((a * math.cos(th) * sig, 0., a * math.sin(th) * sig, 0., 0., pt))
+    return np.array(rows), amps, np.array(angles_deg)
+
+
+def config0(seed=2026):
+    """Config 0: the DA ring, 100 GeV protons, 5 x 205 grid over 0-20 sigma."""
+    gp = proton_globals(100000.0)
+    ln = da_ring(seed=seed)
+    set_rf_frequency(ln, gp)
+    x, amps, angles = da_grid(gp)
+    return gp, ln, x
+
+
+def bb_intensity(gp, xi=0.012, emit_n=EMIT_N):
+    """Bunch intensity giving head-on beam-beam parameter xi for round beams:
+    xi = N r_p / (4 pi gamma eps)."""
+    emit, _ = beam_sizes(gp, emit_n)
+    return 4 * math.pi * gp.gamma * emit * xi / 1.534698e-18
+
+
+def config1(n_particles=65536, seed=2027, lattice_seed=2026, nslice=5, sigma_z=0.3, sigma_pt=2.4e-4):
+    """Config 1: config-0 ring with 6-D weak-strong beam-beam at both IPs
+    (round strong beam sigma = 0.15 mm, sigma_z = 0.3 m, 5 slices, xi = 0.012
+    per IP), spin on; Gaussian particles truncated at 6 sigma."""
+    gp = proton_globals(100000.0)
+    emit, sig = beam_sizes(gp)
+    npb = bb_intensity(gp)
+
+    def bb(name):
+        return [L.BEAMBEAM(name, 6, npb, 1.0, sigma_z, nslice, emit, BETA_STAR, 0., emit, BETA_STAR, 0.)]
+
+    ln = da_ring(seed=lattice_seed, ip_elements=(bb("BB1"), bb("BB2")))
+    set_rf_frequency(ln, gp)
+    r = _rng(seed)
+
+    def tg(n, s):
+        v = r.normal(0., 1., n)
+        bad = np.abs(v) > 6.
+        while bad.any():
+            v[bad] = r.normal(0., 1., bad.sum())
+            bad = np.abs(v) > 6.
+        return v * s
+
+    x = np.zeros((n_particles, 9))
+    x[:, 0] = tg(n_particles, sig)
+    x[:, 1] = tg(n_particles, sig / BETA_STAR)
+    x[:, 2] = tg(n_particles, sig)
+    x[:, 3] = tg(n_particles, sig / BETA_STAR)
+    x[:, 4] = tg(n_particles, sigma_z)
+    x[:, 5] = tg(n_particles, sigma_pt)
+    x[:, 7] = 1.0          # spin along the vertical guide field
+    return gp, ln, x
