@@ -226,6 +226,16 @@ int st_ctx_sync(st_ctx *ctx);
 /* Seconds spent in the last blocking call's GPU work (HIP events). */
 int st_last_gpu_seconds(st_ctx *ctx, double *seconds);
 
+/*
+ * Per-launch timing of the tracking kernel: when enabled, every tracking
+ * launch on this context is bracketed by HIP events on the context stream.
+ * st_ctx_kernel_stats returns (after st_ctx_sync) the summed kernel time,
+ * the number of launches and the sum over launches of launched particles x
+ * turns, then clears the counters.
+ */
+int st_ctx_set_timing(st_ctx *ctx, int enable);
+int st_ctx_kernel_stats(st_ctx *ctx, double *kernel_seconds, int64_t *launches, int64_t *launched_particle_turns);
+
 #ifdef __cplusplus
 }
 #endif

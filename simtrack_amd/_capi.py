@@ -62,6 +62,7 @@ EXPORTS = [
     "st_last_error", "st_ctx_create", "st_ctx_destroy", "st_table_build", "st_table_destroy",
     "st_table_size", "st_track", "st_track_spin", "st_track_tbt", "st_track_radial",
     "st_element_pass", "st_track_device", "st_ctx_sync", "st_last_gpu_seconds",
+    "st_ctx_set_timing", "st_ctx_kernel_stats",
 ]
 
 _lib = None
@@ -98,6 +99,9 @@ def load(path=None):
     lib.st_element_pass.argtypes = [P, I, P, I, I]
     lib.st_track_device.argtypes = [P, P, I, I, I, P, P, I]
     lib.st_last_gpu_seconds.argtypes = [P, ctypes.POINTER(D)]
+    lib.st_ctx_set_timing.argtypes = [P, I]
+    lib.st_ctx_kernel_stats.argtypes = [P, ctypes.POINTER(D), ctypes.POINTER(ctypes.c_int64),
+                                        ctypes.POINTER(ctypes.c_int64)]
     es, gs = I(), I()
     lib.st_struct_sizes(ctypes.byref(es), ctypes.byref(gs))
     if es.value != ELEMENT_DTYPE.itemsize or gs.value != ctypes.sizeof(Globals):
@@ -143,6 +147,16 @@ class Context:
 
     def sync(self):
         check(self.lib.st_ctx_sync(self.h), "st_ctx_sync")
+
+    def set_timing(self, on=True):
+        check(self.lib.st_ctx_set_timing(self.h, 1 if on else 0), "st_ctx_set_timing")
+
+    def kernel_stats(self):
+        """(summed kernel seconds, launches, launched particle-turns) since the last call."""
+        s, n, w = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
+        check(self.lib.st_ctx_kernel_stats(self.h, ctypes.byref(s), ctypes.byref(n), ctypes.byref(w)),
+              "st_ctx_kernel_stats")
+        return s.value, n.value, w.value
 
 
 class Table:

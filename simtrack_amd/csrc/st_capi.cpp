@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "simtrack_amd.h"
@@ -128,6 +129,13 @@ struct st_ctx {
     size_t cap_tbt = 0;
     double *d_x04 = nullptr;
     size_t cap_x04 = 0;
+    /* per-launch kernel timing */
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, int>> pending;  /* indices into ev_pool */
+    std::vector<int64_t> pending_work;
+    double k_seconds = 0.0;
+    int64_t k_launches = 0, k_work = 0;
 };
 
 struct dev_table {
@@ -897,6 +905,58 @@ extern "C" int st_ctx_create(int device, st_ctx **out)
     return ST_OK;
 }
 
+/* fold finished timed launches into the counters (stream must be idle) */
+static int drain_timing(st_ctx *c)
+{
+    for (size_t i = 0; i < c->pending.size(); ++i) {
+        float ms = 0.f;
+        hipError_t e = hipEventElapsedTime(&ms, c->ev_pool[c->pending[i].first], c->ev_pool[c->pending[i].second]);
+        if (e != hipSuccess) return fail(ST_ERR_DEVICE, "kernel timing: %s", hipGetErrorString(e));
+        c->k_seconds += ms * 1e-3;
+        c->k_launches += 1;
+        c->k_work += c->pending_work[i];
+    }
+    c->pending.clear();
+    c->pending_work.clear();
+    return ST_OK;
+}
+
+static int timed_event(st_ctx *c, int *idx)
+{
+    size_t used = 2 * c->pending.size();
+    if (used + 1 >= c->ev_pool.size()) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        c->ev_pool.push_back(e);
+        HIPCHK(hipEventCreate(&e));
+        c->ev_pool.push_back(e);
+    }
+    *idx = (int)used;
+    return ST_OK;
+}
+
+extern "C" int st_ctx_set_timing(st_ctx *c, int enable)
+{
+    if (!c) return fail(ST_ERR_ARG, "st_ctx_set_timing: null context");
+    c->timing = enable != 0;
+    return ST_OK;
+}
+
+extern "C" int st_ctx_kernel_stats(st_ctx *c, double *sec, int64_t *launches, int64_t *work)
+{
+    if (!c) return fail(ST_ERR_ARG, "st_ctx_kernel_stats: null context");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int rc = drain_timing(c);
+    if (rc) return rc;
+    if (sec) *sec = c->k_seconds;
+    if (launches) *launches = c->k_launches;
+    if (work) *work = c->k_work;
+    c->k_seconds = 0.0;
+    c->k_launches = c->k_work = 0;
+    return ST_OK;
+}
+
 static void free_bufs(st_ctx *c)
 {
     hipFree(c->d_x);
@@ -921,6 +981,7 @@ extern "C" int st_ctx_destroy(st_ctx *c)
     if (c->stream) hipStreamSynchronize(c->stream);
     free_bufs(c);
     if (c->h_total) hipHostFree(c->h_total);
+    for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -1156,7 +1217,18 @@ static int track_device_core(st_table *t, double *d_x, int np, int turn0, int nt
         a.precheck = (first && turn0 == 0) ? 1 : 0;
         a.idx = idx;
         a.nact = nact;
+        int ev = -1;
+        if (c->timing) {
+            int rc2 = timed_event(c, &ev);
+            if (rc2) return rc2;
+            HIPCHK(hipEventRecord(c->ev_pool[ev], c->stream));
+        }
         HIPCHK(st_launch_track(a, spin, hexp, rad, c->stream));
+        if (c->timing) {
+            HIPCHK(hipEventRecord(c->ev_pool[ev + 1], c->stream));
+            c->pending.push_back({ev, ev + 1});
+            c->pending_work.push_back((int64_t)nact * nt);
+        }
         first = false;
         t0 += nt;
         if (t0 < turn0 + nturn) {
@@ -1164,6 +1236,10 @@ static int track_device_core(st_table *t, double *d_x, int np, int turn0, int nt
             HIPCHK(st_launch_compact(idx, nact, d_state, c->d_block, out, c->d_total, c->stream));
             HIPCHK(hipMemcpyAsync(c->h_total, c->d_total, sizeof(int), hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
+            if (c->timing) {
+                int rc2 = drain_timing(c);
+                if (rc2) return rc2;
+            }
             nact = *c->h_total;
             idx = out;
             which ^= 1;
