@@ -68,12 +68,13 @@ def all_types_line(seed=7, misalign=True, with_bb=True, with_rf=True, with_gsben
                 L.BEAMBEAM("BB6E", 6, 2e11, 1.0, 0.1, 11, 3 * eps, 0.6, 0.1, eps, 0.3, -0.05),
                 L.BEAMBEAM("BB4", 4, 1.5e11, 1.0, 0.1, 1, 4 * eps, 0.5, 0., eps, 0.5, 0.),
                 L.CRBB("CRB", 1.5e11, 0.01, 1.0, 0.05, 5, 2 * eps, 0.4, 0.02, eps, 0.2, 0.)]
+    els += [L.HBPM("HB", 0.2), L.VBPM("VB", 0.)]
     for e in els:
         if hasattr(e, "Nint") and e.L > 0:
             e.Nint = max(1, min(e.Nint, 5))
     if misalign:
         for i, e in enumerate(els):
-            if e.TYPE in ("DRIFT", "MARKER", "BPM", "MATRIX", "ROTAT", "SNAKE"):
+            if e.TYPE in ("DRIFT", "MARKER", "BPM", "HBPM", "VBPM", "MATRIX", "ROTAT", "SNAKE"):
                 continue
             k = i % 4
             if k == 1:

@@ -1,0 +1,145 @@
+"""Pins the project elementary functions and RNG (st_libm.h, st_rng.h) that both
+the oracle and the kernels use, against mpmath, scipy, glibc and published
+known-answer vectors."""
+import ctypes
+import math
+
+import mpmath
+import numpy as np
+import pytest
+from scipy.stats import norm
+
+import pyoracle
+
+mpmath.mp.dps = 60
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return pyoracle.load(glibc=False)
+
+
+@pytest.fixture(scope="module")
+def glibc_lib():
+    return pyoracle.load(glibc=True)
+
+
+def ulp_err(got, exact):
+    exact = mpmath.mpf(exact)
+    if exact == 0:
+        return 0.0 if got == 0 else math.inf
+    return float(abs(mpmath.mpf(got) - exact) / mpmath.mpf(math.ulp(float(exact))))
+
+
+RNG = np.random.default_rng(20261203)
+CASES = {
+    "orc_sin": (mpmath.sin, np.concatenate([RNG.uniform(-4, 4, 600), RNG.uniform(-3e5, 3e5, 300),
+                                            10 ** RNG.uniform(6, 300, 300) * RNG.choice([-1, 1], 300),
+                                            RNG.uniform(-1e-6, 1e-6, 50), [0.0, 1e-300, math.pi / 4]])),
+    "orc_cos": (mpmath.cos, np.concatenate([RNG.uniform(-4, 4, 600), RNG.uniform(-3e5, 3e5, 300),
+                                            10 ** RNG.uniform(6, 300, 300), [0.0, math.pi / 2]])),
+    "orc_asin": (mpmath.asin, np.concatenate([RNG.uniform(-1, 1, 800), RNG.uniform(-0.01, 0.01, 200),
+                                              1 - 10 ** RNG.uniform(-16, -1, 100), [0.5, -0.5, 0.125, 1.0, -1.0]])),
+    "orc_exp": (mpmath.exp, np.concatenate([RNG.uniform(-700, 700, 600), RNG.uniform(-1, 1, 400),
+                                            RNG.uniform(-745, -708, 60), [0.0, 1e-20, -1e-20]])),
+    "orc_log": (mpmath.log, np.concatenate([10 ** RNG.uniform(-300, 300, 600), RNG.uniform(0.5, 2, 400),
+                                            [5e-324, 1e-310, 1.0, 2.0]])),
+    "orc_tan": (mpmath.tan, np.concatenate([RNG.uniform(-1.5, 1.5, 500), [1e-9, 0.0]])),
+}
+BOUND = {"orc_tan": 2.0}   # tan = sin/cos: two roundings plus the quotient
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_accuracy_vs_mpmath(lib, name):
+    f = getattr(lib, name)
+    ref, xs = CASES[name]
+    worst = 0.0
+    for x in xs:
+        x = float(x)
+        worst = max(worst, ulp_err(f(x), ref(mpmath.mpf(x))))
+    assert worst < BOUND.get(name, 1.0), f"{name}: {worst} ulp"
+
+
+def test_powi(lib):
+    worst = 0.0
+    for h in RNG.uniform(0.3, 30., 300):
+        for n in range(-30, 31, 3):
+            worst = max(worst, ulp_err(lib.orc_powi(float(h), n), mpmath.mpf(float(h)) ** n))
+    assert worst <= 0.5001
+
+
+def test_reduction_hard_cases(lib):
+    """Arguments closest to multiples of pi/2 (Payne-Hanek path) and integers k*pi/2."""
+    hard = [6381956970095103.0 * 2.0 ** 797, 5209563731060755.0 * 2.0 ** 237, 1.0e22, 2 ** 1000,
+            8248.251512623145, 355.0, 103993.0]
+    for x in hard:
+        assert ulp_err(lib.orc_sin(x), mpmath.sin(mpmath.mpf(x))) < 1.0
+        assert ulp_err(lib.orc_cos(x), mpmath.cos(mpmath.mpf(x))) < 1.0
+
+
+def test_special_values(lib):
+    assert math.isnan(lib.orc_sin(math.inf)) and math.isnan(lib.orc_cos(math.nan))
+    assert math.isnan(lib.orc_asin(1.0000001)) and math.isnan(lib.orc_log(-1.0))
+    assert lib.orc_exp(1000.) == math.inf and lib.orc_exp(-1000.) == 0.0
+    assert lib.orc_log(0.0) == -math.inf and lib.orc_asin(-0.0) == 0.0
+
+
+def test_agrees_with_glibc(lib, glibc_lib):
+    """The project libm and the C library agree to within one ulp (both < 1 ulp
+    from the exact value); identical results in the large majority of cases."""
+    xs = RNG.uniform(-3, 3, 5000)
+    same = 0
+    for name in ("orc_sin", "orc_cos", "orc_exp"):
+        a, b = getattr(lib, name), getattr(glibc_lib, name)
+        for x in xs:
+            u, v = a(float(x)), b(float(x))
+            assert abs(np.float64(u).view(np.int64) - np.float64(v).view(np.int64)) <= 1
+            same += u == v
+    assert same > 0.9 * 3 * len(xs)
+
+
+def test_yoshida_constants():
+    """Fdrift/Fkick (simtrack.cpp:29-32): consistency of the 4th-order
+    integrator and the correctly rounded value of Fdrift1."""
+    lib = pyoracle.load()
+    out = (ctypes.c_double * 4)()
+    lib.orc_get_yoshida(out)
+    f1, f2, k1, k2 = list(out)
+    assert 2 * f1 + 2 * f2 == 1.0 and abs(2 * k1 + k2 - 1.0) < 1e-16
+    exact = 1 / (2 * (2 - mpmath.cbrt(2)))
+    assert ulp_err(f1, exact) <= 1.0
+    # 4th order: sum of cubes of kick weights vanishes
+    assert abs(2 * k1 ** 3 + k2 ** 3) < 1e-15
+
+
+PHILOX_KAT = [  # Random123 known-answer vectors for philox4x32_10
+    ((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff,) * 4, (0xffffffff, 0xffffffff), (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,expect", PHILOX_KAT)
+def test_philox_kat(lib, ctr, key, expect):
+    out = (ctypes.c_uint32 * 4)()
+    lib.orc_philox((ctypes.c_uint32 * 4)(*ctr), key[0], key[1], out)
+    assert tuple(out) == expect
+
+
+def test_normal_stream(lib):
+    z = np.array([lib.orc_rng_normal(7, p, c) for p in range(300) for c in range(100)])
+    assert abs(z.mean()) < 0.02 and abs(z.std() - 1) < 0.02
+    # a pure function of (seed, particle, draw)
+    assert lib.orc_rng_normal(7, 3, 5) == lib.orc_rng_normal(7, 3, 5) != lib.orc_rng_normal(8, 3, 5)
+    # Kolmogorov-Smirnov against N(0,1)
+    from scipy.stats import kstest
+    assert kstest(z, "norm").pvalue > 1e-3
+
+
+def test_gaussian_quantile(lib):
+    """GSL's gsl_cdf_ugaussian_Pinv is AS 241; pinned against scipy's ndtri."""
+    ps = np.concatenate([np.linspace(1e-6, 1 - 1e-6, 4001), [0.2, 0.4, 0.6, 0.8], 10.0 ** -np.arange(2, 300, 13)])
+    for p in ps:
+        got, ref = lib.orc_ugaussian_pinv(float(p)), norm.ppf(p)
+        assert abs(got - ref) <= 2e-15 * max(abs(ref), 1.0)

@@ -1,0 +1,445 @@
+"""Pins the CPU oracle's element maps with checks that do not retype its
+formulas: kinematics in physical units, mpmath/scipy references, numerical
+quadrature of the Gaussian beam field, symplecticity of every map, the order
+of the Yoshida integrator, closed-form spin precession and synchrotron
+radiation loss, and the loss bookkeeping of Track."""
+import ctypes
+import math
+
+import mpmath
+import numpy as np
+import pytest
+from scipy import integrate
+from scipy.special import wofz
+from scipy.stats import norm
+
+import pyoracle
+import simtrack_amd as st
+from simtrack_amd import workloads as W
+
+GP = W.proton_globals(100000.0)
+
+
+def one(el, x, gp=GP, spin=False):
+    """Element::Pass / sPass of a single element on each row of x."""
+    rec = st.Line([el]).records()
+    return pyoracle.Oracle(gp, rec).element_pass(0, np.atleast_2d(x).astype(float), spin=spin)
+
+
+# ------------------------------------------------------------------ drift
+
+def test_drift_matches_relativistic_kinematics():
+    """Straight-line motion in physical units: E = E0 + pt p0c, P = sqrt(E^2-m^2),
+    x1 = x0 + L Px/Pz, z1 = z0 - (c t - L/beta0), t = L E/(Pz c)."""
+    mpmath.mp.dps = 40
+    g = GP
+    r = np.random.default_rng(1)
+    x = np.zeros((50, 6))
+    x[:, 0], x[:, 2] = r.normal(0, 1e-3, 50), r.normal(0, 1e-3, 50)
+    x[:, 1], x[:, 3] = r.normal(0, 1e-3, 50), r.normal(0, 1e-3, 50)
+    x[:, 4], x[:, 5] = r.normal(0, 0.1, 50), r.normal(0, 1e-3, 50)
+    L = 7.3
+    out = one(st.DRIFT("D", L), x)
+    E0 = mpmath.mpf(g.energy)
+    m = mpmath.mpf(g.energy0)
+    p0 = mpmath.sqrt(E0 ** 2 - m ** 2)
+    beta0 = p0 / E0
+    for i in range(len(x)):
+        E = E0 + mpmath.mpf(x[i, 5]) * p0
+        P = mpmath.sqrt(E ** 2 - m ** 2)
+        Px, Py = mpmath.mpf(x[i, 1]) * p0, mpmath.mpf(x[i, 3]) * p0
+        Pz = mpmath.sqrt(P ** 2 - Px ** 2 - Py ** 2)
+        t = L * E / Pz                      # c = 1
+        ex = [x[i, 0] + L * Px / Pz, x[i, 1], x[i, 2] + L * Py / Pz, x[i, 3], x[i, 4] - (t - L / beta0), x[i, 5]]
+        for k in range(6):
+            assert abs(out[i, k] - float(ex[k])) <= 1e-12 * max(abs(float(ex[k])), 1e-3), (i, k)
+
+
+def test_zero_length_drift_is_identity():
+    x = W.random_particles(20, seed=5)
+    assert np.array_equal(one(st.DRIFT("D", 0.), x), x)
+
+
+# ------------------------------------------------------------------ kicks
+
+def test_thin_multipole_is_complex_polynomial():
+    """mult_kick_pass: By + i Bx = sum (KNL_n + i KNSL_n) (x + i y)^n / n! (mpmath)."""
+    mpmath.mp.dps = 40
+    r = np.random.default_rng(2)
+    kn = list(r.normal(0, 1, 11) * 10.0 ** -np.arange(11))
+    ks = list(r.normal(0, 1, 11) * 10.0 ** -np.arange(11))
+    kn[5] = ks[5] = 0.
+    x = W.random_particles(30, seed=3, scale=5e-3)
+    out = one(st.MULT("M", 0., kn, ks), x)
+    for i in range(len(x)):
+        z = mpmath.mpc(x[i, 0], x[i, 2])
+        B = sum((mpmath.mpf(kn[n]) + 1j * mpmath.mpf(ks[n])) * z ** n / math.factorial(n) for n in range(11))
+        assert abs(out[i, 1] - float(x[i, 1] - B.real)) < 1e-16 + 1e-14 * abs(float(B.real))
+        assert abs(out[i, 3] - float(x[i, 3] + B.imag)) < 1e-16 + 1e-14 * abs(float(B.imag))
+
+
+@pytest.mark.parametrize("order,cls", [(1, st.QUAD), (2, st.SEXT), (3, st.OCT)])
+def test_dedicated_kicks_equal_generic_multipole(order, cls):
+    """quad/sext/oct_kick_pass (explicit polynomials) vs mult_kick_pass (complex powers)."""
+    x = W.random_particles(40, seed=4, scale=3e-3)
+    kn = [0.] * 11
+    ks = [0.] * 11
+    kn[order], ks[order] = 0.37, -0.11
+    a = one(cls("K", 0., 0.37, -0.11), x)
+    b = one(st.MULT("M", 0., kn, ks), x)
+    assert np.allclose(a, b, rtol=1e-14, atol=1e-18)
+
+
+# ---------------------------------------------------------- symplecticity
+
+def jacobian(el, x0, gp=GP, h=1e-7):
+    J = np.zeros((6, 6))
+    for k in range(6):
+        xp = np.tile(x0, (2, 1))
+        xp[0, k] += h
+        xp[1, k] -= h
+        o = one(el, xp, gp)
+        J[:, k] = (o[0] - o[1]) / (2 * h)
+    return J
+
+
+S6 = np.kron(np.eye(3), np.array([[0., 1.], [-1., 0.]]))
+
+_KN = [0., 0.04, 0.3, -2., 5., 0., 30., 0., 0., 0., 0.]
+_KS = [0., 0.001, -0.1, 1., 0., 0., 0., 0., 0., 0., 0.]
+SYMPLECTIC_CASES = {
+    "drift": st.DRIFT("D", 3.0),
+    "quad": st.QUAD("Q", 1.2, 0.08, 0.01),
+    "sext": st.SEXT("S", 0.5, 2.0, 0.3),
+    "mult": st.MULT("M", 0.9, _KN, _KS),
+    "sbend_exact": st.SBEND("B", 4.0, 0.05, 0.01, -0.02),
+    "sbendmult": st.SBENDMULT("BM", 2.0, 0.03, 0., 0., _KN, _KS),
+    "gsbend": st.GSBEND("G", 2.0, 0.03, 0.0302, 0., 0.),
+    "kicker": st.KICKER("K", 0.4, 1e-4, -2e-4),
+    "rfcav": st.RFCAV("RF", 0.5, 5.0, 28e6, 0.3),
+    "crabrf": st.CRABRF("CR", 0.3, 3.0, 197e6, 0.2),
+    "lbt": st.LBT("LBT", 0.0125),
+    "ilbt": st.ILBT("ILBT", 0.0125),
+    "bb6d_round": st.BEAMBEAM("BB", 6, 2e11, 1.0, 0.2, 5, 2.5e-8, 0.8, 0.1, 2.5e-8, 0.8, 0.1),
+    "bb6d_flat": st.BEAMBEAM("BB", 6, 2e11, -1.0, 0.05, 7, 2e-8, 0.45, 0.0, 1.3e-9, 0.056, 0.0),
+    "crbb": st.CRBB("CRB", 2e11, 0.0125, 1.0, 0.05, 5, 2e-8, 0.45, 0.0, 1.3e-9, 0.056, 0.0),
+}
+
+
+@pytest.mark.parametrize("name", list(SYMPLECTIC_CASES))
+def test_maps_are_symplectic(name):
+    """J^T S J = S.  Hirata's boost alone is conformal: J^T S J = S / cos(theta)
+    for LBT and S cos(theta) for ILBT, so boost-kick-inverse boost (CRBB) is
+    symplectic."""
+    el = SYMPLECTIC_CASES[name]
+    x0 = np.array([3e-4, -2e-5, -1e-4, 1e-5, 0.05, 2e-4])
+    J = jacobian(el, x0)
+    factor = {"lbt": 1 / math.cos(0.0125), "ilbt": math.cos(0.0125)}.get(name, 1.0)
+    assert np.max(np.abs(J.T @ S6 @ J - factor * S6)) < 2e-6 * max(1.0, np.max(np.abs(J)) ** 2)
+
+
+def test_misaligned_frame_changes_round_trip():
+    """GtoL/LtoG of a zero-strength element cancel (offset, roll and pitch)."""
+    x = W.random_particles(20, seed=6, scale=1e-3)
+    for setter in ({"DX": 1e-3, "DY": -2e-3, "DPSI": 0.01}, {"DX": 1e-3, "DTHETA": 1e-3, "DPHI": -2e-3, "DS": 0.01}):
+        q = st.QUAD("Q", 0., 0., 0.)      # k = 0: QUAD_Pass is a drift of length 0
+        q.KN = 1e-300                      # non-zero: forces GtoL/quad kick/LtoG
+        for k, v in setter.items():
+            setattr(q, k, v)
+        out = one(q, x)
+        assert np.allclose(out, x, atol=1e-13, rtol=1e-12)
+
+
+# --------------------------------------------------------- integrator
+
+def test_yoshida_fourth_order_convergence():
+    """Error of the thick quadrupole map vs slice count falls as Nint^-4."""
+    x = np.array([[2e-3, 1e-4, -1e-3, 2e-4, 0., 1e-3]])
+    errs = []
+    q = st.SEXT("S", 1.0, 30.0, 5.0)
+    q.Nint = 256
+    ref = one(q, x)
+    for n in (2, 4, 8, 16):
+        q.Nint = n
+        errs.append(np.max(np.abs(one(q, x) - ref)))
+    ratios = [errs[i] / errs[i + 1] for i in range(3)]
+    assert all(12 < r_ < 20 for r_ in ratios), ratios
+
+
+def test_quadrupole_linear_limit():
+    """Small amplitude through a thick quadrupole = the textbook cos/cosh matrix."""
+    L, k1l = 1.5, 0.06
+    k = k1l / L
+    q = st.QUAD("Q", L, k1l, 0.)
+    q.Nint = 64
+    x = np.array([[1e-7, 2e-9, 1e-7, -1e-9, 0., 0.]])
+    out = one(q, x)[0]
+    s = math.sqrt(k)
+    mx = np.array([[math.cos(s * L), math.sin(s * L) / s], [-s * math.sin(s * L), math.cos(s * L)]])
+    my = np.array([[math.cosh(s * L), math.sinh(s * L) / s], [s * math.sinh(s * L), math.cosh(s * L)]])
+    assert np.allclose(out[[0, 1]], mx @ x[0, [0, 1]], rtol=1e-9, atol=1e-20)
+    assert np.allclose(out[[2, 3]], my @ x[0, [2, 3]], rtol=1e-9, atol=1e-20)
+
+
+# ------------------------------------------------------------ exact bend
+
+def sbend_geometric(x, L, angle, g):
+    """Independent geometric solution of the sector bend (the circle
+    construction of sbend_exact_pass_v0, simtrack.cpp:3861-3938), angle > 0."""
+    pt = x[5]
+    delta1 = math.sqrt(1.0 + 2 * pt / g.beta + pt * pt) - 1.0
+    gamma1 = g.gamma + math.sqrt(g.gamma ** 2 - 1) * pt
+    beta1 = math.sqrt(1.0 - 1.0 / gamma1 ** 2)
+    pz0 = math.sqrt((1 + delta1) ** 2 - x[1] ** 2 - x[3] ** 2)
+    xp0, yp0 = x[1] / pz0, x[3] / pz0
+    t1 = math.sqrt(xp0 ** 2 + yp0 ** 2 + 1.0)
+    vx0, vy0, vz0 = beta1 * xp0 / t1, beta1 * yp0 / t1, beta1 / t1
+    vt = math.hypot(vx0, vz0)
+    rho = L / angle
+    rho1 = (1. + delta1) * rho * math.sqrt(xp0 ** 2 + 1) / t1
+    s, c = math.sin(angle), math.cos(angle)
+    posx0, posy0 = (rho + x[0]) * c, (rho + x[0]) * s
+    theta = math.atan(vx0 / vz0) + angle
+    o1x, o1y = posx0 + rho1 * math.cos(math.pi + theta), posy0 + rho1 * math.sin(math.pi + theta)
+    posx1 = math.sqrt(rho1 ** 2 - o1y ** 2) + o1x
+    chord = math.hypot(posx1 - posx0, posy0)
+    ang1 = 2. * math.asin(chord / rho1 / 2.)
+    dt1 = ang1 * rho1 / vt
+    out = np.array(x, dtype=float)
+    out[0] = posx1 - rho
+    out[2] = x[2] + vy0 * dt1
+    out[4] = x[4] - (dt1 - L / g.beta)
+    th = math.asin(o1y / rho1)
+    t_, t2 = math.sin(th), math.cos(th)
+    vz1 = vt * t2
+    xp1, yp1 = -t_ / t2, vy0 / vz1
+    tt = (1 + delta1) ** 2 / (1. + 1. / (xp1 ** 2 + yp1 ** 2))
+    out[1] = xp1 * math.sqrt((1 + delta1) ** 2 - tt)
+    out[3] = yp1 * math.sqrt((1 + delta1) ** 2 - tt)
+    return out
+
+
+def test_exact_bend_matches_geometric_solution():
+    r = np.random.default_rng(9)
+    L, ang = 6.0, 0.08
+    for _ in range(30):
+        x = np.array([r.normal(0, 2e-3), r.normal(0, 1e-4), r.normal(0, 1e-3), r.normal(0, 1e-4),
+                      r.normal(0, 0.01), r.normal(0, 1e-3)])
+        out = one(st.SBEND("B", L, ang), x)[0]
+        ref = sbend_geometric(x, L, ang, GP)
+        assert np.allclose(out, ref, rtol=1e-9, atol=2e-13)
+
+
+def test_reference_particle_through_bend():
+    out = one(st.SBEND("B", 32., 2 * math.pi / 96), np.zeros((1, 6)))[0]
+    assert np.all(np.abs(out[:5]) < 1e-15) and out[5] == 0.
+
+
+# ------------------------------------------------------------------- RF
+
+def test_rf_kick_sign_and_amplitude():
+    """Zero-length cavity: pt changes by |V Q| sin(-k z + pi - phi0) / (beta E0)
+    and a particle ahead (z > 0) above transition gains energy for phi0 = 0."""
+    V, f = 3.0, 28e6
+    x = np.zeros((3, 6))
+    x[:, 4] = [0.0, 0.5, -0.5]
+    out = one(st.RFCAV("RF", 0., V, f, 0.), x)
+    k = 2 * math.pi * f / 299792458.
+    expect = V * np.sin(k * x[:, 4]) / GP.beta / GP.energy
+    # z = 0: the phase is pi in floating point, sin(pi) = 1.2e-16, so the kick is ~4e-21
+    assert abs(out[0, 5]) < 1e-19 and out[1, 5] > 0 > out[2, 5]
+    assert np.allclose(out[:, 5], expect, rtol=1e-12, atol=1e-19)
+
+
+# ------------------------------------------------------------ boosts
+
+def test_boost_round_trip():
+    x = W.random_particles(50, seed=12, scale=1e-3)
+    x[:, 5] *= 10
+    lib = pyoracle.load()
+    rec = st.Line([st.LBT("L", 0.0125), st.ILBT("I", 0.0125)]).records()
+    orc = pyoracle.Oracle(GP, rec)
+    y = orc.element_pass(1, orc.element_pass(0, x))
+    assert np.allclose(y, x, rtol=1e-12, atol=1e-17)
+    assert lib is not None
+    z = one(st.LBT("L", 0.0), x)
+    assert np.allclose(z, x, rtol=0, atol=1e-18)
+
+
+# ------------------------------------------------------------ beam-beam
+
+def test_cerrf_vs_scipy_wofz():
+    """Gautschi's algorithm with the reference's parameters (xlim 5.33, ylim
+    4.29, 3.2/23/21 terms; a 9-term continued fraction outside) is accurate
+    to about 2e-10 relative over the plane, including y < 0 by reflection."""
+    lib = pyoracle.load()
+    wx, wy = ctypes.c_double(), ctypes.c_double()
+    worst = 0.
+    for xx in np.linspace(-7, 7, 57):
+        for yy in np.concatenate([np.linspace(-3, 7, 41), [0.0, 1e-8]]):
+            lib.orc_cerrf(float(xx), float(yy), ctypes.byref(wx), ctypes.byref(wy))
+            ref = wofz(complex(xx, yy))
+            worst = max(worst, abs(complex(wx.value, wy.value) - ref) / abs(ref))
+    assert worst < 3e-10
+
+
+def bb_kick_quadrature(x, y, sx, sy, N, gamma):
+    """Field of a 2-D Gaussian by the integral representation
+    K_x = (2 N r_p / gamma) x int_0^inf exp(-x^2/(2sx^2+q) - y^2/(2sy^2+q))
+          / ((2sx^2+q)^(3/2) (2sy^2+q)^(1/2)) dq   (and x <-> y)."""
+    c = 2 * N * 1.53469826769e-18 / gamma
+
+    def integral(a, b, sa, sb):
+        # q = 2 sa^2 v: an O(1) integrand on v in [0, inf)
+        A, B = 2 * sa * sa, 2 * sb * sb
+
+        def f(v):
+            return math.exp(-a * a / (A * (1 + v)) - b * b / (B + A * v)) / (
+                (1 + v) ** 1.5 * (B / A + v) ** 0.5)
+        return integrate.quad(f, 0, np.inf, epsabs=0, epsrel=1e-12, limit=400)[0] / A
+
+    return c * x * integral(x, y, sx, sy), c * y * integral(y, x, sy, sx)
+
+
+@pytest.mark.parametrize("sx,sy", [(1.5e-4, 1.5e-4), (3e-4, 1e-4), (1e-4, 4e-4)])
+def test_bb4d_against_field_quadrature(sx, sy):
+    lib = pyoracle.load()
+    dpx, dpy = ctypes.c_double(), ctypes.c_double()
+    N, gamma = 2e11, GP.gamma
+    for x, y in [(1e-4, 2e-4), (-3e-4, 1e-4), (5e-4, -5e-4), (-1e-5, -2e-4), (1e-3, 0.), (0., 2e-5)]:
+        lib.orc_bb4d(x, y, gamma, N, sx, sy, ctypes.byref(dpx), ctypes.byref(dpy))
+        kx, ky = bb_kick_quadrature(x, y, sx, sy, N, gamma)
+        assert abs(dpx.value - kx) <= 1e-9 * max(abs(kx), abs(ky)), (x, y)
+        assert abs(dpy.value - ky) <= 1e-9 * max(abs(kx), abs(ky)), (x, y)
+
+
+def test_bb6d_slice_centroids():
+    """z*_i = sigma_l * N (phi(y_i) - phi(y_{i-1})) with y_i = Phi^-1((i+1)/N): the
+    centroid of equal-charge slices, head (z > 0) first.  The reference uses
+    sqrt(2 * 3.14159265) (relative 1.8e-9 from sqrt(2 pi))."""
+    lib = pyoracle.load()
+    for n in (2, 3, 5, 8):
+        z = (ctypes.c_double * n)()
+        lib.orc_bb_slice_centers(n, 0.3, z)
+        edges = norm.ppf(np.arange(n + 1) / n)
+        cent = np.array([(norm.pdf(edges[i]) - norm.pdf(edges[i + 1])) * n for i in range(n)])  # tail first
+        assert np.allclose(list(z), 0.3 * cent[::-1], rtol=2e-8, atol=1e-15)
+    # 11 slices: the reference's tabulated values are the centroids computed
+    # with the same sqrt(2 * 3.14159265), i.e. exact * sqrt(pi / 3.14159265)
+    z11 = (ctypes.c_double * 11)()
+    lib.orc_bb_slice_centers(11, 1.0, z11)
+    edges = norm.ppf(np.arange(12) / 11)
+    cent = np.array([(norm.pdf(edges[i]) - norm.pdf(edges[i + 1])) * 11 for i in range(11)])
+    assert np.allclose(list(z11), cent[::-1] * math.sqrt(math.pi / 3.14159265), rtol=1e-13, atol=1e-15)
+
+
+def test_bb6d_short_bunch_equals_bb4d():
+    """sigma_z -> 0: the slices collapse to the IP and the kick is BB4D's."""
+    N, emit, beta = 2e11, 2.3e-8, 1.0
+    bb = st.BEAMBEAM("BB", 6, N, 1.0, 1e-12, 5, emit, beta, 0., emit, beta, 0.)
+    x = np.array([[2e-4, 0., -1e-4, 0., 0., 0.]])
+    out = one(bb, x)[0]
+    lib = pyoracle.load()
+    dpx, dpy = ctypes.c_double(), ctypes.c_double()
+    s = math.sqrt(emit * beta)
+    lib.orc_bb4d(2e-4, -1e-4, GP.gamma, N, s, s, ctypes.byref(dpx), ctypes.byref(dpy))
+    assert np.allclose([out[1], out[3]], [dpx.value, dpy.value], rtol=1e-8)
+
+
+# ------------------------------------------------------------------ spin
+
+def test_spin_precession_in_dipoles_is_g_gamma():
+    """On the reference orbit through bends of total angle 2 pi the spin turns
+    by G gamma 2 pi about the vertical (Thomas-BMT); |S| is preserved."""
+    els = []
+    for i in range(16):
+        els += [st.SBEND("B", 2.0, 2 * math.pi / 16), st.DRIFT("D", 1.0)]
+    rec = st.Line(els).records()
+    orc = pyoracle.Oracle(GP, rec)
+    x = np.zeros((1, 9))
+    x[0, 8] = 1.0                     # longitudinal spin
+    out, s, _, _ = orc.track(x, 1, spin=True)
+    ggamma = GP.G * GP.gamma
+    phi = 2 * math.pi * ggamma
+    assert abs(out[0, 8] - math.cos(phi)) < 1e-9 and abs(abs(out[0, 6]) - abs(math.sin(phi))) < 1e-9
+    assert abs(out[0, 7]) < 1e-14
+
+
+def test_spin_norm_preserved_everywhere():
+    ln = W.all_types_line(seed=4)
+    rec = ln.records()
+    orc = pyoracle.Oracle(GP, rec)
+    x = W.random_particles(64, seed=8, spin=True, scale=5e-4)
+    for j in range(len(rec)):
+        out = orc.element_pass(j, x, spin=True)
+        nrm = np.linalg.norm(out[:, 6:9], axis=1)
+        assert np.all(np.abs(nrm - 1) < 1e-13), ln.Cell[j].TYPE
+
+
+# ------------------------------------------------------------- radiation
+
+def test_classical_radiation_loss_per_turn():
+    """Mean energy loss per turn of an on-axis electron in an isomagnetic ring:
+    U0 [GeV] = C_gamma E^4 [GeV] / rho, C_gamma = 8.846e-5 m/GeV^3."""
+    g = W.electron_globals(10000.0, radiate=1)
+    rho = 100.0
+    nb = 64
+    ang = 2 * math.pi / nb
+    els = [st.SBENDMULT("B", rho * ang, ang, 0., 0., [0.] * 11, [0.] * 11) for _ in range(nb)]
+    rec = st.Line(els).records()
+    orc = pyoracle.Oracle(g, rec)
+    out, s, _, _ = orc.track(np.zeros((1, 6)), 1)
+    u0 = 8.84627632682e-5 * (g.energy / 1000.) ** 4 / rho * 1000.   # MeV
+    assert abs(-out[0, 5] * g.energy * g.beta - u0) < 2e-3 * u0
+
+
+# -------------------------------------------------------------- tune
+
+def test_fine_tune_finder_locates_dft_peak():
+    lib = pyoracle.load()
+    n, q = 1024, 0.6789012345
+    i = np.arange(n)
+    x = np.cos(2 * np.pi * q * i + 0.3) + 0.05 * np.cos(2 * np.pi * 0.31 * i)
+    peak = ctypes.c_double()
+    lib.orc_fine_tune(n, x.ctypes.data_as(ctypes.c_void_p), 0.66, 0.70, ctypes.byref(peak))
+
+    def amp(t):
+        e = np.exp(-2j * np.pi * i * t)
+        return -abs(np.dot(x, e)) ** 2
+
+    from scipy.optimize import minimize_scalar
+    ref = minimize_scalar(amp, bracket=(q - 1e-3, q, q + 1e-3), tol=1e-14).x
+    assert abs(peak.value - ref) < 1e-9 and abs(peak.value - q) < 1e-5
+
+
+# --------------------------------------------------------- loss semantics
+
+def test_track_loss_records():
+    ln = W.da_ring(seed=1)
+    g = W.proton_globals(100000.0)
+    W.set_rf_frequency(ln, g)
+    orc = pyoracle.Oracle(g, ln.records())
+    x = np.zeros((5, 6))
+    x[0, 0] = 0.1000001
+    x[1, 2] = -0.2
+    x[2, 0] = 1e-3
+    x[3, 1] = 3.0                 # sqrt of a negative number in the first drift: NaN
+    x[4, 0] = 1e-5
+    stable = np.array([1, 1, 0, 1, 1], np.int32)
+    out, s, lt, lp = orc.track(x, 3, stable=stable, lost_turn=np.full(5, 4, np.int32),
+                               lost_post=np.full(5, 6, np.int32))
+    assert list(s) == [0, 0, 0, 0, 1]
+    assert list(lt[:3]) == [0, 0, 0] and list(lp[:3]) == [0, 0, 0]
+    # element 0 is the IP marker: MARKER::Pass is DRIFT_Pass(x, 0), whose
+    # sqrt((1+delta)^2 - px^2) is already NaN there
+    assert lt[3] == 0 and lp[3] == 0 and np.isnan(out[3, 0])
+    assert lt[4] == 4 and lp[4] == 6                                  # untouched for survivors
+
+
+def test_radial_on_momentum_particle_has_zero_z():
+    ln = W.da_ring(seed=1)
+    g = W.proton_globals(100000.0)
+    W.set_rf_frequency(ln, g)
+    orc = pyoracle.Oracle(g, ln.records())
+    out, s, _, _ = orc.track(np.zeros((1, 6)), 3, radial=True)
+    assert s[0] == 1 and out[0, 4] == 0.0
