@@ -44,6 +44,9 @@ def load(glibc=False):
         getattr(lib, f).restype = D
     lib.orc_powi.argtypes = [D, I]
     lib.orc_powi.restype = D
+    if not glibc:
+        lib.orc_asin_small.restype = D
+        lib.orc_asin_small.argtypes = [D]
     lib.orc_rng_normal.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]
     lib.orc_rng_normal.restype = D
     lib.orc_philox.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, P]

@@ -2082,6 +2082,12 @@ EXPORT double orc_cos(double x) { return OCOS(x); }
 EXPORT double orc_tan(double x) { return OTAN(x); }
 EXPORT double orc_asin(double x) { return OASIN(x); }
 EXPORT double orc_exp(double x) { return OEXP(x); }
+#ifndef ORACLE_GLIBC
+/* the branch-free common paths the GPU kernels evaluate first */
+EXPORT double orc_asin_small(double x) { return st_asin_small(x); }
+EXPORT void orc_sincos_small(double x, double *s, double *c) { st_sincos_small(x, s, c); }
+EXPORT void orc_sincos(double x, double *s, double *c) { st_sincos(x, s, c); }
+#endif
 EXPORT double orc_powi(double x, int n) { return OPOW(x, n); }
 EXPORT double orc_log(double x) { return st_log(x); }
 EXPORT double orc_rng_normal(uint64_t seed, uint32_t particle, uint64_t counter)

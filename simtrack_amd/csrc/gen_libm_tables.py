@@ -87,8 +87,8 @@ def main(out):
     nas = 27
     w(f"#define ST_ASIN_NCOEF {nas}")
     coefs = [rn(Fraction(comb(2 * n, n), 4 ** n * (2 * n + 1))) for n in range(1, nas + 1)]
-    w("#define ST_ASIN_COEFS \\")
-    w("  { " + ", \\\n    ".join(hexd(c) for c in coefs) + " }")
+    for i, c in enumerate(coefs):
+        w(f"#define ST_ASIN_C{i} {hexd(c)}")
     w("")
     # 2/pi bits, 32 per word, most significant first; words 0 and 1 are zero pads.
     nbits = 32 * 40

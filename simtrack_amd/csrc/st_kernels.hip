@@ -16,10 +16,12 @@
 #include <stdint.h>
 
 #define ST_FN __device__ __forceinline__
+#define ST_FN_COLD __device__ __noinline__
 #define ST_CONST_TABLE static __device__ __constant__ const
 #define ST_FMA(a, b, c) __builtin_fma((a), (b), (c))
 #define ST_RINT(x) __builtin_rint(x)
-#define ST_SQRT(x) __builtin_sqrt(x)
+#include "st_fastdiv.h"
+#define ST_SQRT(x) st_fsqrt(x)
 #include "st_libm.h"
 #include "st_rng.h"
 #include "st_maps.h"
@@ -88,7 +90,7 @@ ST_DEV void thin_field(double xm, double ym, double kn, double ks, double &B0, d
  * (cpp:7561-7589, 7618-7644, 7673-7699).
  */
 template <int KK, bool SPIN, bool HEXP>
-ST_DEV void thick_multipole(St &s, Ptc &c, int nint, int flags, const double *__restrict__ P, const st_kglobals &g)
+ST_DEV void thick_multipole(St &s, Ptc &c, int nint, int flags, st_pp P, const st_kglobals &g)
 {
     double d1 = P[0], d1b = P[1], d2 = P[2], d2b = P[3];
     double kn1 = P[4], ks1 = P[5], kn2 = P[6], ks2 = P[7];
@@ -116,7 +118,7 @@ ST_DEV void thick_multipole(St &s, Ptc &c, int nint, int flags, const double *__
 
 /* thin (L = 0) QUAD/SEXT/OCT and their spin passes (cpp:7546-7559, 7603-7616, 7658-7671) */
 template <int KK, bool SPIN>
-ST_DEV void thin_multipole(St &s, Ptc &c, const double *__restrict__ P, const st_kglobals &g)
+ST_DEV void thin_multipole(St &s, Ptc &c, st_pp P, const st_kglobals &g)
 {
     double xe = s.x[0], pxe = s.x[1], ye = s.x[2], pye = s.x[3];
     thin_kick<KK>(s, P[0], P[1]);
@@ -134,10 +136,10 @@ ST_DEV void thin_multipole(St &s, Ptc &c, const double *__restrict__ P, const st
 
 /* MULT_Pass (cpp:5152-5182) and MULT_sPass (cpp:7702-7740), thick */
 template <bool SPIN, bool HEXP>
-ST_DEV void thick_mult(St &s, Ptc &c, const st_op &op, const double *__restrict__ P, const st_kglobals &g)
+ST_DEV void thick_mult(St &s, Ptc &c, const st_op &op, st_pp P, const st_kglobals &g)
 {
     int n1 = op.norder + 1;
-    const double *kn1 = P + 6, *ks1 = kn1 + n1, *kn2 = ks1 + n1, *ks2 = kn2 + n1, *KNL = ks2 + n1, *KNSL = KNL + n1;
+    st_pp kn1 = P + 6, ks1 = kn1 + n1, kn2 = ks1 + n1, ks2 = kn2 + n1, KNL = ks2 + n1, KNSL = KNL + n1;
     double d1 = P[0], d1b = P[1], d2 = P[2], d2b = P[3], nint_d = P[4];
     int mask0 = (int)P[5];
     for (int i = 0; i < op.nint; i++) {
@@ -162,10 +164,10 @@ ST_DEV void thick_mult(St &s, Ptc &c, const st_op &op, const double *__restrict_
 
 /* thin MULT: mult_kick_pass; spin: cal_Bfield + spin on the entry state, then kick (cpp:7708-7713) */
 template <bool SPIN>
-ST_DEV void thin_mult(St &s, Ptc &c, const st_op &op, const double *__restrict__ P, const st_kglobals &g)
+ST_DEV void thin_mult(St &s, Ptc &c, const st_op &op, st_pp P, const st_kglobals &g)
 {
     int n1 = op.norder + 1;
-    const double *KNL = P, *KNSL = P + n1;
+    st_pp KNL = P, KNSL = P + n1;
     if (SPIN) {
         double By, Bx;
         mult_field(s, op.norder, op.mask1, KNL, KNSL, By, Bx);
@@ -179,13 +181,13 @@ ST_DEV void thin_mult(St &s, Ptc &c, const st_op &op, const double *__restrict__
  * bodies) and their spin passes (cpp:7835-7928), L > 0.
  */
 template <bool SPIN, bool HEXP, bool RAD>
-ST_DEV void bendmult(St &s, Ptc &c, const st_op &op, const double *__restrict__ P, const st_kglobals &g)
+ST_DEV void bendmult(St &s, Ptc &c, const st_op &op, st_pp P, const st_kglobals &g)
 {
     int n1 = op.norder + 1;
-    const double *kn1 = P + ST_BM_ARR, *ks1 = kn1 + n1, *kn2 = ks1 + n1, *ks2 = kn2 + n1, *KNL = ks2 + n1,
-                 *KNSL = KNL + n1;
+    st_pp kn1 = P + ST_BM_ARR, ks1 = kn1 + n1, kn2 = ks1 + n1, ks2 = kn2 + n1, KNL = ks2 + n1, KNSL = KNL + n1;
     double href = P[2], Lint = P[3], nint_d = P[4], Angle = P[5], An = P[6];
     int mask0 = (int)P[7];
+    StRcp rh1 = st_rcp_prep(P[9]), rh2 = st_rcp_prep(P[15]), rn = st_rcp_prep(nint_d);
     edge(s, P[0]);
     for (int i = 0; i < op.nint; i++) {
         double xe = s.x[0], pxe = s.x[1], ye = s.x[2], pye = s.x[3];
@@ -199,13 +201,13 @@ ST_DEV void bendmult(St &s, Ptc &c, const st_op &op, const double *__restrict__ 
             bend_mult_kick(s, kl1, href, op.norder, op.mask1, kn1, ks1);
             drift<HEXP>(s, d1, d1b, c);
         } else {
-            sbend_exact(s, P[8], P[9], P[10], P[11], P[12], P[13], c);
+            sbend_exact(s, P[8], P[9], P[10], P[11], P[12], P[13], c, rh1);
             mult_kick(s, op.norder, op.mask1, kn1, ks1);
-            sbend_exact(s, P[14], P[15], P[16], P[17], P[18], P[19], c);
+            sbend_exact(s, P[14], P[15], P[16], P[17], P[18], P[19], c, rh2);
             mult_kick(s, op.norder, op.mask2, kn2, ks2);
-            sbend_exact(s, P[14], P[15], P[16], P[17], P[18], P[19], c);
+            sbend_exact(s, P[14], P[15], P[16], P[17], P[18], P[19], c, rh2);
             mult_kick(s, op.norder, op.mask1, kn1, ks1);
-            sbend_exact(s, P[8], P[9], P[10], P[11], P[12], P[13], c);
+            sbend_exact(s, P[8], P[9], P[10], P[11], P[12], P[13], c, rh1);
         }
         if (SPIN || RAD) {
             St m;
@@ -213,7 +215,7 @@ ST_DEV void bendmult(St &s, Ptc &c, const st_op &op, const double *__restrict__ 
             m.x[2] = (s.x[2] + ye) / 2.;
             double By, Bx;
             mult_field(m, op.norder, mask0, KNL, KNSL, By, Bx);
-            double B0 = Bx / nint_d, B1 = (Angle + By) / nint_d;
+            double B0 = st_qdiv(Bx, rn), B1 = st_qdiv(Angle + By, rn);
             if (SPIN) spin_rotate(s, xe, pxe, pye, An, href, B0, B1, 0., c, g);
             if (RAD) radiate(s, Lint, href, B0, B1, 0., c, g);
         }
@@ -223,10 +225,10 @@ ST_DEV void bendmult(St &s, Ptc &c, const st_op &op, const double *__restrict__ 
 
 /* L = 0 branch of the bend-multipole passes (cpp:5350-5352; spin cpp:7846-7854) */
 template <bool SPIN>
-ST_DEV void bendmult_thin(St &s, Ptc &c, const st_op &op, const double *__restrict__ P, const st_kglobals &g)
+ST_DEV void bendmult_thin(St &s, Ptc &c, const st_op &op, st_pp P, const st_kglobals &g)
 {
     int n1 = op.norder + 1;
-    const double *KNL = P + 1, *KNSL = KNL + n1;
+    st_pp KNL = P + 1, KNSL = KNL + n1;
     double xe = s.x[0], pxe = s.x[1], ye = s.x[2], pye = s.x[3];
     mult_kick(s, op.norder, op.mask1, KNL, KNSL);
     if (SPIN) {
@@ -241,10 +243,10 @@ ST_DEV void bendmult_thin(St &s, Ptc &c, const st_op &op, const double *__restri
 
 /* GSBENDMULT_Pass, cpp:5597-5723 (orbit only; its sPass is the SBENDMULT one) */
 template <bool HEXP, bool RAD>
-ST_DEV void gsbendmult(St &s, Ptc &c, const st_op &op, const double *__restrict__ P, const st_kglobals &g)
+ST_DEV void gsbendmult(St &s, Ptc &c, const st_op &op, st_pp P, const st_kglobals &g)
 {
     int n1 = op.norder + 1;
-    const double *kn1 = P + ST_GBM_ARR, *ks1 = kn1 + n1, *kn2 = ks1 + n1, *ks2 = kn2 + n1;
+    st_pp kn1 = P + ST_GBM_ARR, ks1 = kn1 + n1, kn2 = ks1 + n1, ks2 = kn2 + n1;
     double href = P[2], hreal = P[3], Lint = P[4], BLy = P[6];
     edge(s, P[0]);
     for (int i = 0; i < op.nint; i++) {
@@ -272,15 +274,87 @@ ST_DEV void gsbendmult(St &s, Ptc &c, const st_op &op, const double *__restrict_
 }
 
 /*
+ * Exact-body slices of SBEND_sPass (cpp:7486-7536) without radiation:
+ * Nint times sbend_exact_pass (cpp:3833-3858) followed by spin() on the
+ * slice's entry state.  Across slices of one element x[3] and pt do not
+ * change, so quantities the reference recomputes per slice from identical
+ * operands are carried instead (same values, same bits): sqrt((1+d)^2-py^2)
+ * once per element, the entry pz of slice i+1 is the exit pz of slice i
+ * (also spin()'s temp), and asin(px_entry/temp) of slice i+1 is the second
+ * asin of slice i.
+ */
+template <bool SPIN, class A>
+ST_DEV void sbend_slice_t(A &ar, St &s, double &pz0, double &as1, double x3, const StRcp &rt, const StRcp &rh,
+                          double L, double hs, double ih, double cosA, double sinA, double Lb, double An, double href,
+                          const Ptc &c, const st_kglobals &g)
+{
+    double *x = s.x;
+    const double x0 = x[0], x1 = x[1];
+    double nx1 = x1 * cosA + (pz0 - 1. - hs * x0) * sinA;
+    double pz1 = ar.sqrt(c.opd2 - nx1 * nx1 - x3 * x3);
+    double derivative = -1.0 * x1 * sinA * hs + (pz0 - 1. - x0 * hs) * cosA * hs;
+    x[1] = nx1;
+    x[0] = ar.div(pz1, rh) - ar.div(ar.div(derivative, rh), rh) - ih;
+    double as2 = ar.asin(ar.div(nx1, rt));
+    x[2] = x[2] + x3 * L + ar.div(x3 * (as1 - as2), rh);
+    double t = c.opd * L + ar.div(c.opd * (as1 - as2), rh);
+    x[4] = x[4] - (ar.div(t, c.rb1) - Lb);
+    if (SPIN) spin_rotate_t(ar, s, x0, x1, x3, pz0, An, href, 0., An, 0., c, g);
+    pz0 = pz1;
+    as1 = as2;
+}
+
+template <bool SPIN>
+ST_DEV void sbend_slices_exact(St &s, Ptc &c, int nint, double L, double hs, double ih, double cosA, double sinA,
+                               double Lb, double An, double href, const st_kglobals &g)
+{
+    double *x = s.x;
+    StRcp rh = st_rcp_prep(hs);
+    const double x3 = x[3];
+    const double temp = __builtin_sqrt(c.opd2 - x3 * x3);
+    StRcp rt = st_rcp_prep(temp);
+    double pz0 = __builtin_sqrt(c.opd2 - x[1] * x[1] - x3 * x3);
+    double as1 = st_asin(x[1] / temp);
+    /* spin() runs only for a non-zero field; B = (0, An, 0) here */
+    const bool rot = SPIN && (0. * 0. + An * An + 0. * 0. != 0.);
+    for (int i = 0; i < nint; i++) {
+        St s0 = s;
+        double pz00 = pz0, as10 = as1;
+        ArFast af;
+        if (rot)
+            sbend_slice_t<true>(af, s, pz0, as1, x3, rt, rh, L, hs, ih, cosA, sinA, Lb, An, href, c, g);
+        else
+            sbend_slice_t<false>(af, s, pz0, as1, x3, rt, rh, L, hs, ih, cosA, sinA, Lb, An, href, c, g);
+        if (__builtin_expect(af.bad, 0)) {
+            ST_FD_COLD_PATH();
+            s = s0;
+            pz0 = pz00;
+            as1 = as10;
+            ArExact ae;
+            if (rot)
+                sbend_slice_t<true>(ae, s, pz0, as1, x3, rt, rh, L, hs, ih, cosA, sinA, Lb, An, href, c, g);
+            else
+                sbend_slice_t<false>(ae, s, pz0, as1, x3, rt, rh, L, hs, ih, cosA, sinA, Lb, An, href, c, g);
+        }
+    }
+}
+
+/*
  * SBEND_Pass with slices (H_expand or radiate, cpp:4781-4813) and
  * SBEND_sPass (cpp:7486-7536).
  */
 template <bool SPIN, bool HEXP, bool RAD>
-ST_DEV void sbend_sliced(St &s, Ptc &c, int nint, const double *__restrict__ P, const st_kglobals &g,
+ST_DEV void sbend_sliced(St &s, Ptc &c, int nint, st_pp P, const st_kglobals &g,
                          uint32_t pid, uint64_t &rctr)
 {
     double href = P[2], Lint = P[3], An = P[4];
     edge(s, P[0]);
+    if (!HEXP && !RAD) {
+        sbend_slices_exact<SPIN>(s, c, nint, Lint, P[6], P[7], P[8], P[9], P[10], An, href, g);
+        edge(s, P[1]);
+        return;
+    }
+    StRcp rh = st_rcp_prep(P[6]);
     for (int i = 0; i < nint; i++) {
         double xe = s.x[0], pxe = s.x[1], pye = s.x[3];
         if (HEXP) {
@@ -293,7 +367,7 @@ ST_DEV void sbend_sliced(St &s, Ptc &c, int nint, const double *__restrict__ P, 
             bend_kick(s, kl1, href);
             drift<HEXP>(s, d1, d1b, c);
         } else {
-            sbend_exact(s, Lint, P[6], P[7], P[8], P[9], P[10], c);
+            sbend_exact(s, Lint, P[6], P[7], P[8], P[9], P[10], c, rh);
         }
         if (SPIN) spin_rotate(s, xe, pxe, pye, An, href, 0., An, 0., c, g);
         if (RAD) {
@@ -308,7 +382,7 @@ ST_DEV void sbend_sliced(St &s, Ptc &c, int nint, const double *__restrict__ P, 
 
 /* GSBEND_Pass without radiation, cpp:4818-4880 */
 template <bool HEXP>
-ST_DEV void gsbend(St &s, Ptc &c, int kind, int nint, const double *__restrict__ P)
+ST_DEV void gsbend(St &s, Ptc &c, int kind, int nint, st_pp P)
 {
     edge(s, P[0]);
     if (kind == K_GSBEND_EXACT) {
@@ -329,7 +403,7 @@ ST_DEV void gsbend(St &s, Ptc &c, int kind, int nint, const double *__restrict__
 }
 
 /* CCMULT_Pass body, cpp:6034-6064 */
-ST_DEV void ccmult_kick(St &s, const double *__restrict__ P, const st_kglobals &g)
+ST_DEV void ccmult_kick(St &s, st_pp P, const st_kglobals &g)
 {
     double *x = s.x;
     double scale = P[4], brho = P[5];
@@ -362,12 +436,12 @@ ST_DEV void ccmult_kick(St &s, const double *__restrict__ P, const st_kglobals &
  * including the GtoL/LtoG frame changes.
  */
 template <bool SPIN, bool HEXP, bool RAD>
-ST_DEV void apply_op(St &s, Ptc &c, const st_op &op, const double *__restrict__ par, const st_kglobals &g,
+ST_DEV void apply_op(St &s, Ptc &c, const st_op &op, st_pp par, const st_kglobals &g,
                      uint32_t pid, uint64_t &rctr)
 {
-    const double *__restrict__ P = par + op.po;
+    st_pp P = st_launder(par + op.po);
     if (op.pm >= 0) {
-        mis_in(s, op.flags, par + op.pm);
+        mis_in(s, op.flags, st_launder(par + op.pm));
         if (op.flags & ST_F_MIS2) ptc_update(c, s.x[5], g);
     }
     double *x = s.x;
@@ -404,7 +478,7 @@ ST_DEV void apply_op(St &s, Ptc &c, const st_op &op, const double *__restrict__ 
         break;
     case K_SBEND_EXACT:
         edge(s, P[0]);
-        sbend_exact(s, P[2], P[3], P[4], P[5], P[6], P[7], c);
+        sbend_exact(s, P[2], P[3], P[4], P[5], P[6], P[7], c, st_rcp_prep(P[3]));
         edge(s, P[1]);
         break;
     case K_SBEND_SLICED:
@@ -504,7 +578,7 @@ ST_DEV void apply_op(St &s, Ptc &c, const st_op &op, const double *__restrict__ 
         break;
     }
     if (op.pm >= 0) {
-        mis_out(s, op.flags, par + op.pm);
+        mis_out(s, op.flags, st_launder(par + op.pm));
         if (op.flags & ST_F_MIS2) ptc_update(c, s.x[5], g);
     }
 }
@@ -540,6 +614,10 @@ __global__ __launch_bounds__(256) void k_track(st_track_args a)
     uint64_t rctr = 0;
     Ptc c;
     ptc_update(c, s.x[5], a.g);
+    const st_opp ops = (st_opp)a.ops;
+    const st_pp par = (st_pp)a.par;
+    const st_pp ap = (st_pp)a.ap;
+    const st_pp x04 = (st_pp)a.x04;
     int lt = 0, lp = 0;
     bool died = false;
     const int tend = a.turn0 + a.nturn;
@@ -552,17 +630,17 @@ __global__ __launch_bounds__(256) void k_track(st_track_args a)
             for (int k = 0; k < NC; k++) o[k] = s.x[k];
         }
         for (int e = 0; e < a.nelem; ++e) {
-            const st_op op = a.ops[e];
+            const st_op op = ops[e];
             if (alive) {
-                apply_op<SPIN, HEXP, RAD>(s, c, op, a.par, a.g, (uint32_t)p, rctr);
-                if (a.x04) s.x[4] = s.x[4] - a.x04[e];
+                apply_op<SPIN, HEXP, RAD>(s, c, op, par, a.g, (uint32_t)p, rctr);
+                if (x04) s.x[4] = s.x[4] - x04[e];
                 if (a.xtbt && e == a.tbt_bpm) {
                     double *o = a.xtbt + ((size_t)p * a.tbt_nturn + (size_t)(turn - a.tbt_turn0)) * NC;
 #pragma unroll
                     for (int k = 0; k < NC; k++) o[k] = s.x[k];
                 }
                 /* loss test of Track, simtrack.cpp:20802 (none for a bare Element::Pass) */
-                if (a.ap && (__builtin_fabs(s.x[0]) > a.ap[2 * e] || __builtin_fabs(s.x[2]) > a.ap[2 * e + 1] ||
+                if (ap && (__builtin_fabs(s.x[0]) > ap[2 * e] || __builtin_fabs(s.x[2]) > ap[2 * e + 1] ||
                              s.x[0] != s.x[0] || s.x[2] != s.x[2])) {
                     alive = false;
                     died = true;
@@ -598,8 +676,8 @@ __global__ void k_radial_x04(st_track_args a, double *x04)
         s.x[4] = 0.;
         s.x[5] = 0.;
         ptc_update(c, s.x[5], a.g);
-        const st_op op = a.ops[e];
-        apply_op<false, HEXP, false>(s, c, op, a.par, a.g, 0u, rctr);
+        const st_op op = ((st_opp)a.ops)[e];
+        apply_op<false, HEXP, false>(s, c, op, (st_pp)a.par, a.g, 0u, rctr);
         x04[e] = s.x[4];
     }
 }

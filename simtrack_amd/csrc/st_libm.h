@@ -46,8 +46,12 @@
     !defined(ST_RINT) || !defined(ST_SQRT)
 #error "define ST_FN, ST_CONST_TABLE, ST_FMA, ST_RINT, ST_SQRT before including st_libm.h"
 #endif
+/* functions on rare paths (huge trigonometric arguments); a GPU build makes
+   them out-of-line so the hot code stays small */
+#ifndef ST_FN_COLD
+#define ST_FN_COLD ST_FN
+#endif
 
-ST_CONST_TABLE double st_asin_coefs[ST_ASIN_NCOEF] = ST_ASIN_COEFS;
 ST_CONST_TABLE uint32_t st_two_over_pi_words[ST_TWO_OVER_PI_NWORDS] = ST_TWO_OVER_PI_WORDS;
 
 /* ---------------------------------------------------------------- bits */
@@ -148,7 +152,7 @@ ST_FN double st_kcos(double x, double y)
    the bits of 2/pi of weight above 2^(2-e) only add multiples of 4 to
    x*2/pi, so a 192-bit window of 2/pi starting at fraction bit e-1 gives
    x*2/pi mod 4 to 2^-137 absolute. */
-ST_FN int st_rem_pio2_large(double ax, st_dd *r)
+ST_FN_COLD int st_rem_pio2_large(double ax, st_dd *r)
 {
     uint64_t bits = st_d2u(ax);
     int e = (int)((bits >> 52) & 0x7ff) - 1075;       /* ax = m * 2^e, e >= -32 */
@@ -256,6 +260,18 @@ ST_FN void st_sincos(double x, double *s, double *c)
     }
 }
 
+/* The |x| <= pi/4 path of st_sincos without its tiny-argument branch: for
+   0 < |x| < 2^-27 the kernels already return sin = x and cos = 1 exactly
+   (the cubic term is below a quarter ulp of x, 1 - x^2/2 rounds to 1); at
+   x = -0 the sine kernel gives +0, so zero is passed through by a select.
+   For every |x| <= pi/4, including subnormals, this equals st_sincos. */
+ST_FN void st_sincos_small(double x, double *s, double *c)
+{
+    double sr = st_ksin(x, 0.0, 0);
+    *s = x == 0.0 ? x : sr;
+    *c = st_kcos(x, 0.0);
+}
+
 ST_FN double st_sin(double x)
 {
     double s, c;
@@ -272,12 +288,40 @@ ST_FN double st_cos(double x)
 
 /* --------------------------------------------------------------- asin */
 
-ST_FN double st_asin_poly(double t, int n)
+/* Horner evaluation c0 + t*(c1 + t*(...)) of the first 10 and all 27
+   series coefficients, written out so the coefficients are immediate
+   operands rather than table loads. */
+#define ST_ASIN_STEP(p, t, i) p = ST_FMA(p, t, ST_ASIN_C##i)
+ST_FN double st_asin_poly10(double t)
 {
-    double p = st_asin_coefs[n - 1];
-    for (int i = n - 2; i >= 0; --i)
-        p = ST_FMA(p, t, st_asin_coefs[i]);
+    double p = ST_ASIN_C9;
+    ST_ASIN_STEP(p, t, 8); ST_ASIN_STEP(p, t, 7); ST_ASIN_STEP(p, t, 6); ST_ASIN_STEP(p, t, 5);
+    ST_ASIN_STEP(p, t, 4); ST_ASIN_STEP(p, t, 3); ST_ASIN_STEP(p, t, 2); ST_ASIN_STEP(p, t, 1);
+    ST_ASIN_STEP(p, t, 0);
     return p;
+}
+
+ST_FN double st_asin_poly27(double t)
+{
+    double p = ST_ASIN_C26;
+    ST_ASIN_STEP(p, t, 25); ST_ASIN_STEP(p, t, 24); ST_ASIN_STEP(p, t, 23); ST_ASIN_STEP(p, t, 22);
+    ST_ASIN_STEP(p, t, 21); ST_ASIN_STEP(p, t, 20); ST_ASIN_STEP(p, t, 19); ST_ASIN_STEP(p, t, 18);
+    ST_ASIN_STEP(p, t, 17); ST_ASIN_STEP(p, t, 16); ST_ASIN_STEP(p, t, 15); ST_ASIN_STEP(p, t, 14);
+    ST_ASIN_STEP(p, t, 13); ST_ASIN_STEP(p, t, 12); ST_ASIN_STEP(p, t, 11); ST_ASIN_STEP(p, t, 10);
+    ST_ASIN_STEP(p, t, 9); ST_ASIN_STEP(p, t, 8); ST_ASIN_STEP(p, t, 7); ST_ASIN_STEP(p, t, 6);
+    ST_ASIN_STEP(p, t, 5); ST_ASIN_STEP(p, t, 4); ST_ASIN_STEP(p, t, 3); ST_ASIN_STEP(p, t, 2);
+    ST_ASIN_STEP(p, t, 1); ST_ASIN_STEP(p, t, 0);
+    return p;
+}
+
+/* The |x| < 0.125 path of st_asin without its tiny-argument branch: for
+   |x| < 2^-26, x*t*p is below 0.17 * 2^-52 |x|, less than half an ulp of x
+   even at a binade boundary, so fma(x*t, p, x) = x; for every |x| < 0.125,
+   including +-0 and subnormals, this equals st_asin. */
+ST_FN double st_asin_small(double x)
+{
+    double t = x * x;
+    return ST_FMA(x * t, st_asin_poly10(t), x);
 }
 
 ST_FN double st_asin(double x)
@@ -292,14 +336,14 @@ ST_FN double st_asin(double x)
         return x;
     if (ax <= 0.5) {
         double t = x * x;
-        double p = st_asin_poly(t, ax < 0.125 ? 10 : ST_ASIN_NCOEF);
+        double p = ax < 0.125 ? st_asin_poly10(t) : st_asin_poly27(t);
         return ST_FMA(x * t, p, x);
     }
     /* asin(ax) = pi/2 - 2*asin(s), s = sqrt((1-ax)/2) <= 1/2 */
     double t = (1.0 - ax) * 0.5;                     /* exact */
     double s = ST_SQRT(t);
     double slo = ST_FMA(-s, s, t) / (s + s);          /* sqrt(t) ~ s + slo */
-    double p = st_asin_poly(t, ST_ASIN_NCOEF);
+    double p = st_asin_poly27(t);
     st_dd r = st_two_sum(ST_PIO2_1, -(s + s));
     double corr = ST_PIO2_2 - 2.0 * (slo + (s * t) * p);
     double res = r.hi + (r.lo + corr);

@@ -61,34 +61,33 @@
 
 /* asin(x) = x + x*t*sum_{n>=1} c_n t^(n-1), t = x^2, c_n = C(2n,n)/(4^n (2n+1)) */
 #define ST_ASIN_NCOEF 27
-#define ST_ASIN_COEFS \
-  { 0x1.5555555555555p-3, \
-    0x1.3333333333333p-4, \
-    0x1.6db6db6db6db7p-5, \
-    0x1.f1c71c71c71c7p-6, \
-    0x1.6e8ba2e8ba2e9p-6, \
-    0x1.1c4ec4ec4ec4fp-6, \
-    0x1.c99999999999ap-7, \
-    0x1.7a87878787878p-7, \
-    0x1.3fde50d79435ep-7, \
-    0x1.12ef3cf3cf3cfp-7, \
-    0x1.df3bd37a6f4dfp-8, \
-    0x1.a6863d70a3d71p-8, \
-    0x1.782dda12f684cp-8, \
-    0x1.51ba308d3dcb1p-8, \
-    0x1.31683bdef7bdfp-8, \
-    0x1.15ee9d45d1746p-8, \
-    0x1.fcaf8fb6db6dbp-9, \
-    0x1.d3d2a8e0dd67dp-9, \
-    0x1.b026f57b13b14p-9, \
-    0x1.90cb77f60c7cep-9, \
-    0x1.750de64d7d05fp-9, \
-    0x1.5c5f56efaaaabp-9, \
-    0x1.464c0950f7d47p-9, \
-    0x1.3275586c5f2f0p-9, \
-    0x1.208d3570ae5a6p-9, \
-    0x1.1052bc5fa960ap-9, \
-    0x1.018f963c229bfp-9 }
+#define ST_ASIN_C0 0x1.5555555555555p-3
+#define ST_ASIN_C1 0x1.3333333333333p-4
+#define ST_ASIN_C2 0x1.6db6db6db6db7p-5
+#define ST_ASIN_C3 0x1.f1c71c71c71c7p-6
+#define ST_ASIN_C4 0x1.6e8ba2e8ba2e9p-6
+#define ST_ASIN_C5 0x1.1c4ec4ec4ec4fp-6
+#define ST_ASIN_C6 0x1.c99999999999ap-7
+#define ST_ASIN_C7 0x1.7a87878787878p-7
+#define ST_ASIN_C8 0x1.3fde50d79435ep-7
+#define ST_ASIN_C9 0x1.12ef3cf3cf3cfp-7
+#define ST_ASIN_C10 0x1.df3bd37a6f4dfp-8
+#define ST_ASIN_C11 0x1.a6863d70a3d71p-8
+#define ST_ASIN_C12 0x1.782dda12f684cp-8
+#define ST_ASIN_C13 0x1.51ba308d3dcb1p-8
+#define ST_ASIN_C14 0x1.31683bdef7bdfp-8
+#define ST_ASIN_C15 0x1.15ee9d45d1746p-8
+#define ST_ASIN_C16 0x1.fcaf8fb6db6dbp-9
+#define ST_ASIN_C17 0x1.d3d2a8e0dd67dp-9
+#define ST_ASIN_C18 0x1.b026f57b13b14p-9
+#define ST_ASIN_C19 0x1.90cb77f60c7cep-9
+#define ST_ASIN_C20 0x1.750de64d7d05fp-9
+#define ST_ASIN_C21 0x1.5c5f56efaaaabp-9
+#define ST_ASIN_C22 0x1.464c0950f7d47p-9
+#define ST_ASIN_C23 0x1.3275586c5f2f0p-9
+#define ST_ASIN_C24 0x1.208d3570ae5a6p-9
+#define ST_ASIN_C25 0x1.1052bc5fa960ap-9
+#define ST_ASIN_C26 0x1.018f963c229bfp-9
 
 #define ST_TWO_OVER_PI_NWORDS 42
 /* word j (j >= 2) holds bits 32(j-2)+1 .. 32(j-2)+32 of the binary fraction 2/pi */

@@ -20,10 +20,33 @@
 #define ST_MAPS_H
 
 #include "st_table_format.h"
+#include "st_fastdiv.h"
 
 #define ST_DEV __device__ __forceinline__
 
 typedef double st_real;
+
+/* Element payload pointer.  The packed table is read-only for the whole
+   launch, so it is addressed in the constant address space: reads at
+   wave-uniform offsets compile to scalar loads into SGPRs (one per wave, no
+   VGPRs, scalar-cache latency) instead of per-lane vector loads. */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define ST_KCONST const __attribute__((address_space(4)))
+#else
+#define ST_KCONST const /* host pass: kernel bodies are parsed only */
+#endif
+typedef ST_KCONST double *st_pp;
+typedef ST_KCONST st_op *st_opp;
+
+/* The element's payload base pointer, made opaque to the optimiser.  Every
+   P[k] then becomes a scalar load at an immediate offset from one SGPR pair;
+   without it the compiler hoists par + 8k for every k used by any element
+   type out of the element loop, and those loop-invariant addresses spill. */
+ST_DEV st_pp st_launder(st_pp p)
+{
+    __asm__ volatile("" : "+s"(p));
+    return p;
+}
 
 /* particle state: x[0..5] orbit, x[6..8] spin */
 struct St {
@@ -37,35 +60,117 @@ struct Ptc {
     double opd2;   /* (1 + delta1)*(1 + delta1)                              */
     double g1;     /* gamma1 = gamma + sqrt(gamma^2 - 1) pt      (cpp:3756)  */
     double b1;     /* beta1 = sqrt(1 - 1/gamma1/gamma1)          (cpp:3757)  */
+    StRcp ropd;    /* 1 + delta1 and beta1 prepared as divisors              */
+    StRcp rb1;
 };
 
 ST_DEV void ptc_update(Ptc &c, double pt, const st_kglobals &g)
 {
-    c.d1 = __builtin_sqrt(1.0 + 2 * pt / g.beta + pt * pt) - 1.0;
+    c.d1 = st_fsqrt(1.0 + 2 * pt / g.beta + pt * pt) - 1.0;
     c.g1 = g.gamma + g.sg * pt;
-    c.b1 = __builtin_sqrt(1.0 - 1.0 / c.g1 / c.g1);
+    c.b1 = st_fsqrt(1.0 - 1.0 / c.g1 / c.g1);
     c.opd = 1.0 + c.d1;
     c.opd2 = c.opd * c.opd;
+    c.ropd = st_rcp_prep(c.opd);
+    c.rb1 = st_rcp_prep(c.b1);
 }
+
+/* ------------------------------------------------ arithmetic policies */
+
+/*
+ * A map step is written once, templated on how it divides, takes square
+ * roots, asin and sin/cos.  ArFast evaluates the common case without a
+ * single branch (shared reciprocals, guard-free sqrt, the small-argument
+ * libm paths) and records in `bad` whether any precondition of those
+ * shortcuts failed; ArExact uses the operators and the full functions.
+ * Where every precondition holds the two give identical bits, so a caller
+ * runs ArFast, and replays the step with ArExact from the saved input state
+ * in the (rare) lanes where bad is set.  A step is thereby one basic block
+ * that the scheduler can interleave freely -- the point on a GPU running one
+ * wavefront per SIMD, whose speed is set by instruction-level parallelism.
+ */
+struct ArFast {
+    bool bad = false;
+    ST_DEV double div(double a, const StRcp &r)
+    {
+        double q0 = a * r.y;
+        double rr = __builtin_fma(-r.b, q0, a);
+        double q = __builtin_fma(rr, r.y, q0);
+        double aq = __builtin_fabs(q);
+        bad = bad | !(r.ok && aq > 0x1p-900 && aq < 0x1p700);
+        return q;
+    }
+    ST_DEV double sqrt(double x)
+    {
+        double y = __builtin_amdgcn_rsq(x);
+        double g = x * y;
+        double h = y * 0.5;
+        double r = __builtin_fma(-h, g, 0.5);
+        g = __builtin_fma(g, r, g);
+        h = __builtin_fma(h, r, h);
+        double d = __builtin_fma(-g, g, x);
+        g = __builtin_fma(d, h, g);
+        d = __builtin_fma(-g, g, x);
+        g = __builtin_fma(d, h, g);
+        bad = bad | !(x >= 0x1p-767 && x < __builtin_inf());
+        return g;
+    }
+    ST_DEV double asin(double x)
+    {
+        bad = bad | !(__builtin_fabs(x) < 0.125);
+        return st_asin_small(x);
+    }
+    ST_DEV void sincos(double x, double *sn, double *cs)
+    {
+        bad = bad | !(__builtin_fabs(x) <= ST_PIO4);
+        st_sincos_small(x, sn, cs);
+    }
+};
+
+struct ArExact {
+    bool bad = false;
+    ST_DEV double div(double a, const StRcp &r) { return a / r.b; }
+    ST_DEV double sqrt(double x) { return __builtin_sqrt(x); }
+    ST_DEV double asin(double x) { return st_asin(x); }
+    ST_DEV void sincos(double x, double *sn, double *cs) { st_sincos(x, sn, cs); }
+};
 
 /* ------------------------------------------------------------ drift */
 
 /* DRIFT_Pass, simtrack.cpp:3743-3766.  Lb = L/GP.beta. */
-template <bool HEXP>
-ST_DEV void drift(St &s, double L, double Lb, const Ptc &c)
+template <bool HEXP, class A>
+ST_DEV void drift_t(A &ar, St &s, double L, double Lb, const Ptc &c)
 {
     double *x = s.x;
     if (HEXP) {
-        double u = L / (1.0 + x[5]);
+        StRcp r5 = st_rcp_prep(1.0 + x[5]);
+        double u = ar.div(L, r5);
         x[0] = x[0] + x[1] * u;
         x[2] = x[2] + x[3] * u;
-        x[4] = x[4] - (x[1] * x[1] + x[3] * x[3]) * u / 2.0 / (1 + x[5]);
+        x[4] = x[4] - ar.div((x[1] * x[1] + x[3] * x[3]) * u / 2.0, r5);
     } else {
-        double pz = __builtin_sqrt(c.opd2 - x[1] * x[1] - x[3] * x[3]);
-        double betaz = pz * c.b1 / c.opd;
-        x[0] = x[0] + x[1] * L / pz;
-        x[2] = x[2] + x[3] * L / pz;
+        double pz = ar.sqrt(c.opd2 - x[1] * x[1] - x[3] * x[3]);
+        StRcp rz = st_rcp_prep(pz);
+        double betaz = ar.div(pz * c.b1, c.ropd);
+        x[0] = x[0] + ar.div(x[1] * L, rz);
+        x[2] = x[2] + ar.div(x[3] * L, rz);
         x[4] = x[4] - (L / betaz - Lb);
+    }
+}
+
+template <bool HEXP>
+ST_DEV void drift(St &s, double L, double Lb, const Ptc &c)
+{
+    double x0 = s.x[0], x2 = s.x[2], x4 = s.x[4];
+    ArFast af;
+    drift_t<HEXP>(af, s, L, Lb, c);
+    if (__builtin_expect(af.bad, 0)) {
+        ST_FD_COLD_PATH();
+        s.x[0] = x0;
+        s.x[2] = x2;
+        s.x[4] = x4;
+        ArExact ae;
+        drift_t<HEXP>(ae, s, L, Lb, c);
     }
 }
 
@@ -96,7 +201,7 @@ ST_DEV void drift0(St &s, const Ptc &c)
 /* ------------------------------------------------------ misalignment */
 
 /* GtoL1 (cpp:3443-3461) and GtoL2 (cpp:3567-3626) */
-ST_DEV void mis_in(St &s, int flags, const double *__restrict__ M)
+ST_DEV void mis_in(St &s, int flags, st_pp M)
 {
     double *x = s.x;
     if (flags & ST_F_MIS1) {
@@ -128,7 +233,7 @@ ST_DEV void mis_in(St &s, int flags, const double *__restrict__ M)
 }
 
 /* LtoG1 (cpp:3463-3481) and LtoG2 (cpp:3628-3699) */
-ST_DEV void mis_out(St &s, int flags, const double *__restrict__ M)
+ST_DEV void mis_out(St &s, int flags, st_pp M)
 {
     double *x = s.x;
     if (flags & ST_F_MIS1) {
@@ -145,7 +250,7 @@ ST_DEV void mis_out(St &s, int flags, const double *__restrict__ M)
             x[2] = x[2] + M[1];
         }
     } else if (flags & ST_F_MIS2) {
-        const double *R = M + 41;
+        st_pp R = M + 41;
         double xt[6];
 #pragma unroll
         for (int i = 0; i < 5; i++) xt[i] = x[i] + R[36 + i];
@@ -206,8 +311,8 @@ __device__ __constant__ const double st_fact[11] = {1., 1., 2., 6., 24., 120., 7
  * (simtrack.cpp:4086-4111, 4113-4137, 4169-4196): By + i Bx accumulated over
  * orders 1..Norder whose coefficients are not both zero (mask bit i).
  */
-ST_DEV void mult_field(const St &s, int norder, int mask, const double *__restrict__ kn,
-                       const double *__restrict__ ks, double &By, double &Bx)
+ST_DEV void mult_field(const St &s, int norder, int mask, st_pp kn,
+                       st_pp ks, double &By, double &Bx)
 {
     const double *x = s.x;
     double Xn = 1., Yn = 0.;
@@ -225,7 +330,7 @@ ST_DEV void mult_field(const St &s, int norder, int mask, const double *__restri
     }
 }
 
-ST_DEV void mult_kick(St &s, int norder, int mask, const double *__restrict__ kn, const double *__restrict__ ks)
+ST_DEV void mult_kick(St &s, int norder, int mask, st_pp kn, st_pp ks)
 {
     double By, Bx;
     mult_field(s, norder, mask, kn, ks, By, Bx);
@@ -250,8 +355,8 @@ ST_DEV void general_bend_kick(St &s, double L, double href, double hreal)
 }
 
 /* bend_mult_kick_pass, simtrack.cpp:4113-4137 */
-ST_DEV void bend_mult_kick(St &s, double L, double href, int norder, int mask, const double *__restrict__ kn,
-                           const double *__restrict__ ks)
+ST_DEV void bend_mult_kick(St &s, double L, double href, int norder, int mask, st_pp kn,
+                           st_pp ks)
 {
     double By, Bx;
     mult_field(s, norder, mask, kn, ks, By, Bx);
@@ -263,7 +368,7 @@ ST_DEV void bend_mult_kick(St &s, double L, double href, int norder, int mask, c
 
 /* general_bend_mult_kick_pass, simtrack.cpp:4139-4163 */
 ST_DEV void general_bend_mult_kick(St &s, double L, double href, double hreal, int norder, int mask,
-                                   const double *__restrict__ kn, const double *__restrict__ ks)
+                                   st_pp kn, st_pp ks)
 {
     double By, Bx;
     mult_field(s, norder, mask, kn, ks, By, Bx);
@@ -275,39 +380,60 @@ ST_DEV void general_bend_mult_kick(St &s, double L, double href, double hreal, i
 
 /* ------------------------------------------------------------ bends */
 
-/* sbend_exact_pass, simtrack.cpp:3833-3858.  ih = 1.0/href, Lb = L/GP.beta */
-ST_DEV void sbend_exact(St &s, double L, double href, double ih, double cosA, double sinA, double Lb, const Ptc &c)
+/* sbend_exact_pass, simtrack.cpp:3833-3858.  ih = 1.0/href, Lb = L/GP.beta,
+   rh = href prepared as a divisor */
+template <class A>
+ST_DEV void sbend_exact_t(A &ar, St &s, double L, double href, double ih, double cosA, double sinA, double Lb,
+                          const Ptc &c, const StRcp &rh)
 {
     double *x = s.x;
     double x0 = x[0], x1 = x[1], x3 = x[3];
-    double pz0 = __builtin_sqrt(c.opd2 - x1 * x1 - x3 * x3);
+    double pz0 = ar.sqrt(c.opd2 - x1 * x1 - x3 * x3);
     double nx1 = x1 * cosA + (pz0 - 1. - href * x0) * sinA;
-    double pz1 = __builtin_sqrt(c.opd2 - nx1 * nx1 - x3 * x3);
+    double pz1 = ar.sqrt(c.opd2 - nx1 * nx1 - x3 * x3);
     double derivative = -1.0 * x1 * sinA * href + (pz0 - 1. - x0 * href) * cosA * href;
     x[1] = nx1;
-    x[0] = pz1 / href - derivative / href / href - ih;
-    double temp = __builtin_sqrt(c.opd2 - x3 * x3);
-    double arcsin1 = st_asin(x1 / temp);
-    double arcsin2 = st_asin(nx1 / temp);
-    x[2] = x[2] + x3 * L + x3 * (arcsin1 - arcsin2) / href;
-    temp = c.opd * L + c.opd * (arcsin1 - arcsin2) / href;
-    x[4] = x[4] - (temp / c.b1 - Lb);
+    x[0] = ar.div(pz1, rh) - ar.div(ar.div(derivative, rh), rh) - ih;
+    double temp = ar.sqrt(c.opd2 - x3 * x3);
+    StRcp rt = st_rcp_prep(temp);
+    double arcsin1 = ar.asin(ar.div(x1, rt));
+    double arcsin2 = ar.asin(ar.div(nx1, rt));
+    x[2] = x[2] + x3 * L + ar.div(x3 * (arcsin1 - arcsin2), rh);
+    temp = c.opd * L + ar.div(c.opd * (arcsin1 - arcsin2), rh);
+    x[4] = x[4] - (ar.div(temp, c.rb1) - Lb);
+}
+
+ST_DEV void sbend_exact(St &s, double L, double href, double ih, double cosA, double sinA, double Lb, const Ptc &c,
+                        const StRcp &rh)
+{
+    double x0 = s.x[0], x1 = s.x[1], x2 = s.x[2], x4 = s.x[4];
+    ArFast af;
+    sbend_exact_t(af, s, L, href, ih, cosA, sinA, Lb, c, rh);
+    if (__builtin_expect(af.bad, 0)) {
+        ST_FD_COLD_PATH();
+        s.x[0] = x0;
+        s.x[1] = x1;
+        s.x[2] = x2;
+        s.x[4] = x4;
+        ArExact ae;
+        sbend_exact_t(ae, s, L, href, ih, cosA, sinA, Lb, c, rh);
+    }
 }
 
 /* general_sbend_exact_pass, simtrack.cpp:3974-4002.
    P: L, rhoc, b0, b0*rhoc, cosA, sinA, Lb */
-ST_DEV void general_sbend_exact(St &s, const double *__restrict__ P, const Ptc &c)
+ST_DEV void general_sbend_exact(St &s, st_pp P, const Ptc &c)
 {
     double *x = s.x;
     double L = P[0], rhoc = P[1], b0 = P[2], b0r = P[3], cosA = P[4], sinA = P[5], Lb = P[6];
     double x0 = x[0], x1 = x[1], x3 = x[3];
-    double pz0 = __builtin_sqrt(c.opd2 - x1 * x1 - x3 * x3);
+    double pz0 = st_fsqrt(c.opd2 - x1 * x1 - x3 * x3);
     double nx1 = x1 * cosA + (pz0 - b0r - b0 * x0) * sinA;
-    double pz1 = __builtin_sqrt(c.opd2 - nx1 * nx1 - x3 * x3);
+    double pz1 = st_fsqrt(c.opd2 - nx1 * nx1 - x3 * x3);
     double derivative = -x1 * sinA / rhoc + (pz0 - b0r - b0 * x0) * cosA / rhoc;
     x[1] = nx1;
     x[0] = pz1 / b0 - derivative * rhoc / b0 - rhoc;
-    double temp = __builtin_sqrt(c.opd2 - x3 * x3);
+    double temp = st_fsqrt(c.opd2 - x3 * x3);
     double arcsin1 = st_asin(x1 / temp);
     double arcsin2 = st_asin(nx1 / temp);
     x[2] = x[2] + x3 * L / b0 / rhoc + x3 * (arcsin1 - arcsin2) / b0;
@@ -332,18 +458,20 @@ ST_DEV void edge(St &s, double ef)
  * entry x, px, py are passed explicitly; pt is unchanged inside a step, so
  * the entry delta1/gamma1 are those of the cache c.
  */
-ST_DEV void spin_rotate(St &s, double xe, double pxe, double pye, double angle, double href, double B0, double B1,
-                        double B2, const Ptc &c, const st_kglobals &g)
+template <class A>
+ST_DEV void spin_rotate_t(A &ar, St &s, double xe, double pxe, double pye, double pze, double angle, double href,
+                          double B0, double B1, double B2, const Ptc &c, const st_kglobals &g)
 {
     double *x = s.x;
-    if (B0 * B0 + B1 * B1 + B2 * B2 != 0.) {
+    {
         double Gr = g.G * c.g1;
-        double temp = __builtin_sqrt(c.opd2 - pxe * pxe - pye * pye);
-        double u0 = pxe / temp, u1 = pye / temp, u2 = 1.0;
-        temp = __builtin_sqrt(u0 * u0 + u1 * u1 + u2 * u2);
-        u0 = u0 / temp;
-        u1 = u1 / temp;
-        u2 = u2 / temp;
+        StRcp rt = st_rcp_prep(pze);
+        double u0 = ar.div(pxe, rt), u1 = ar.div(pye, rt), u2 = 1.0;
+        double temp = ar.sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+        StRcp rn = st_rcp_prep(temp);
+        u0 = ar.div(u0, rn);
+        u1 = ar.div(u1, rn);
+        u2 = ar.div(u2, rn);
         temp = (1 + href * xe) * temp;
         double udotB = u0 * B0 + u1 * B1 + u2 * B2;
         double Bp0 = udotB * u0, Bp1 = udotB * u1, Bp2 = udotB * u2;
@@ -351,14 +479,15 @@ ST_DEV void spin_rotate(St &s, double xe, double pxe, double pye, double angle, 
         double F0 = (1 + Gr) * B0 + k * Bp0;
         double F1 = (1 + Gr) * B1 + k * Bp1;
         double F2 = (1 + Gr) * B2 + k * Bp2;
-        F0 = F0 * temp / c.opd;
-        F1 = F1 * temp / c.opd;
-        F2 = F2 * temp / c.opd;
+        F0 = ar.div(F0 * temp, c.ropd);
+        F1 = ar.div(F1 * temp, c.ropd);
+        F2 = ar.div(F2 * temp, c.ropd);
         F1 = F1 - angle;
-        double Fabs = __builtin_sqrt(F0 * F0 + F1 * F1 + F2 * F2);
-        double a = F0 / Fabs, b = F1 / Fabs, d = F2 / Fabs;
+        double Fabs = ar.sqrt(F0 * F0 + F1 * F1 + F2 * F2);
+        StRcp rf = st_rcp_prep(Fabs);
+        double a = ar.div(F0, rf), b = ar.div(F1, rf), d = ar.div(F2, rf);
         double sn, cs;
-        st_sincos(-Fabs, &sn, &cs);
+        ar.sincos(-Fabs, &sn, &cs);
         double omc = 1 - cs;
         double r0 = a * a * omc + cs, r1 = a * b * omc - d * sn, r2 = a * d * omc + b * sn;
         double r3 = a * b * omc + d * sn, r4 = b * b * omc + cs, r5 = b * d * omc - a * sn;
@@ -370,6 +499,29 @@ ST_DEV void spin_rotate(St &s, double xe, double pxe, double pye, double angle, 
     }
 }
 
+/* spin() on the entry state (the reference's temp = sqrt((1+delta1)^2 -
+   px^2 - py^2) is the entry pz, evaluated only for a non-zero field), fast
+   pass with an exact replay of the rotation where needed */
+ST_DEV void spin_rotate(St &s, double xe, double pxe, double pye, double angle, double href, double B0, double B1,
+                        double B2, const Ptc &c, const st_kglobals &g)
+{
+    if (B0 * B0 + B1 * B1 + B2 * B2 != 0.) {
+        double s6 = s.x[6], s7 = s.x[7], s8 = s.x[8];
+        ArFast af;
+        double pze = af.sqrt(c.opd2 - pxe * pxe - pye * pye);
+        spin_rotate_t(af, s, xe, pxe, pye, pze, angle, href, B0, B1, B2, c, g);
+        if (__builtin_expect(af.bad, 0)) {
+            ST_FD_COLD_PATH();
+            s.x[6] = s6;
+            s.x[7] = s7;
+            s.x[8] = s8;
+            ArExact ae;
+            pze = ae.sqrt(c.opd2 - pxe * pxe - pye * pye);
+            spin_rotate_t(ae, s, xe, pxe, pye, pze, angle, href, B0, B1, B2, c, g);
+        }
+    }
+}
+
 /* ------------------------------------------------------- radiation */
 
 /* vector_cross_product(BLbrho, u) and the energy loss of radiate/radiate1 */
@@ -377,10 +529,10 @@ ST_DEV void rad_common(const St &s, double href, double B0, double B1, double B2
                        double &BLper2)
 {
     const double *x = s.x;
-    double pz = __builtin_sqrt(c.opd2 - x[1] * x[1] - x[3] * x[3]);
+    double pz = st_fsqrt(c.opd2 - x[1] * x[1] - x[3] * x[3]);
     double xp = (1 + x[0] * href) * x[1] / pz;
     double yp = (1 + x[0] * href) * x[3] / pz;
-    temp1 = __builtin_sqrt((1 + x[0] * href) * (1 + x[0] * href) + xp * xp + yp * yp);
+    temp1 = st_fsqrt((1 + x[0] * href) * (1 + x[0] * href) + xp * xp + yp * yp);
     double u0 = xp / temp1, u1 = yp / temp1, u2 = (1 + x[0] * href) / temp1;
     double c0 = B1 * u2 - B2 * u1;
     double c1 = -1.0 * B0 * u2 + B2 * u0;
@@ -397,7 +549,7 @@ ST_DEV void radiate(St &s, double L, double href, double B0, double B1, double B
     rad_common(s, href, B0, B1, B2, c, temp1, BLper2);
     double temp2 = g.Cr * c.opd * c.opd * g.e3 / 2 / 3.14159265358979323;
     x[5] = x[5] - temp2 * BLper2 * temp1 / L;
-    double delta2 = __builtin_sqrt(1.0 + 2 * x[5] / g.beta + x[5] * x[5]) - 1.0;
+    double delta2 = st_fsqrt(1.0 + 2 * x[5] / g.beta + x[5] * x[5]) - 1.0;
     double ratio = (1 + delta2) / c.opd;
     x[1] = px0 * ratio;
     x[3] = py0 * ratio;
@@ -415,13 +567,13 @@ ST_DEV void radiate1(St &s, double L, double href, double B0, double B1, double 
     double temp2 = g.Cr * c.opd * c.opd * g.e3 / 2 / 3.14159265358979323;
     double ptloss = temp2 * BLper2 * temp1 / L;
     double dtdl = temp1 / c.b1 / 299792458.;
-    double g1 = __builtin_sqrt(BLper2 / L / L) / c.opd;
+    double g1 = st_fsqrt(BLper2 / L / L) / c.opd;
     double randgaus = st_rng_normal(g.rng_seed, pid, ctr);
     ctr++;
     double gm = c.g1;
-    double ptfluct = __builtin_sqrt(g.Cf * gm * gm * gm * gm * gm * g1 * g1 * g1 * dtdl * L) * randgaus;
+    double ptfluct = st_fsqrt(g.Cf * gm * gm * gm * gm * gm * g1 * g1 * g1 * dtdl * L) * randgaus;
     x[5] = x[5] - ptloss + ptfluct;
-    double delta2 = __builtin_sqrt(1.0 + 2 * x[5] / g.beta + x[5] * x[5]) - 1.0;
+    double delta2 = st_fsqrt(1.0 + 2 * x[5] / g.beta + x[5] * x[5]) - 1.0;
     double ratio = (1.0 + delta2) / c.opd;
     x[1] = px0 * ratio;
     x[3] = py0 * ratio;
@@ -443,7 +595,7 @@ ST_DEV void cerrf(double xx, double yy, double &wx, double &wy)
                  fac1 = 3.2, fac2 = 23, fac3 = 21;
     double x = __builtin_fabs(xx), y = __builtin_fabs(yy);
     if (y < ylim && x < xlim) {
-        double q = (one - y / ylim) * __builtin_sqrt(one - (x / xlim) * (x / xlim));
+        double q = (one - y / ylim) * st_fsqrt(one - (x / xlim) * (x / xlim));
         double h = one / (fac1 * q);
         int nc = 7 + (int)(fac2 * q);
         double xl = st_powi(h, 1 - nc);
@@ -512,9 +664,9 @@ ST_DEV void bb4d(double x, double y, double c2Nrg, double sigmax, double sigmay,
         Dpx = temp1 * x * temp2;
         Dpy = temp1 * y * temp2;
     } else if (sigmax > sigmay) {
-        double temp1 = c2Nrg * __builtin_sqrt(PI / 2. / (sigmax * sigmax - sigmay * sigmay));
+        double temp1 = c2Nrg * st_fsqrt(PI / 2. / (sigmax * sigmax - sigmay * sigmay));
         double temp2 = st_exp(-x * x / 2. / sigmax / sigmax - y * y / 2. / sigmay / sigmay);
-        double temp3 = __builtin_sqrt(2. * (sigmax * sigmax - sigmay * sigmay));
+        double temp3 = st_fsqrt(2. * (sigmax * sigmax - sigmay * sigmay));
         double w1r, w1i, w2r, w2i;
         double x1 = __builtin_fabs(x) / temp3;
         double y1 = __builtin_fabs(y) / temp3;
@@ -525,9 +677,9 @@ ST_DEV void bb4d(double x, double y, double c2Nrg, double sigmax, double sigmay,
         Dpy = temp1 * (w1r - temp2 * w2r) * signy;
         Dpx = temp1 * (w1i - temp2 * w2i) * signx;
     } else if (sigmax < sigmay) {
-        double temp1 = c2Nrg * __builtin_sqrt(PI / 2 / (sigmay * sigmay - sigmax * sigmax));
+        double temp1 = c2Nrg * st_fsqrt(PI / 2 / (sigmay * sigmay - sigmax * sigmax));
         double temp2 = st_exp(-x * x / 2 / sigmax / sigmax - y * y / 2 / sigmay / sigmay);
-        double temp3 = __builtin_sqrt(2 * (sigmay * sigmay - sigmax * sigmax));
+        double temp3 = st_fsqrt(2 * (sigmay * sigmay - sigmax * sigmax));
         double w1r, w1i, w2r, w2i;
         double x1 = __builtin_fabs(x) / temp3;
         double y1 = __builtin_fabs(y) / temp3;
@@ -583,7 +735,7 @@ ST_DEV void bb_slice_kick(double *x, double S, double sigmax, double sigmay, dou
  * BB6D, simtrack.cpp:6579-6678.  P: c2Nrg, temp3, bbscale, gx*, gy*, betax*,
  * alfx*, betay*, alfy*, emitx, emity, 2*emitx, 2*emity, z*[nslice]
  */
-ST_DEV void bb6d(St &s, int nslice, const double *__restrict__ P)
+ST_DEV void bb6d(St &s, int nslice, st_pp P)
 {
     double *x = s.x;
     double c2Nrg = P[0], temp3 = P[1], bbscale = P[2], gx = P[3], gy = P[4];
@@ -592,8 +744,8 @@ ST_DEV void bb6d(St &s, int nslice, const double *__restrict__ P)
         double S = (x[4] - P[13 + i]) / 2.0;
         double betax = bx + 2 * S * ax + S * S * gx;
         double betay = by + 2 * S * ay + S * S * gy;
-        double sigmax = __builtin_sqrt(ex * betax);
-        double sigmay = __builtin_sqrt(ey * betay);
+        double sigmax = st_fsqrt(ex * betax);
+        double sigmay = st_fsqrt(ey * betay);
         double dsx2 = ex2 * (ax + S * gx);
         double dsy2 = ey2 * (ay + S * gy);
         double X = x[0] + x[1] * S;
@@ -608,19 +760,19 @@ ST_DEV void bb6d(St &s, int nslice, const double *__restrict__ P)
  * alfx*, betay*, alfy*, emitx, emity (all in the boosted frame),
  * Z0*[nslice] (= Z0/cos).
  */
-ST_DEV void bb6d_angle(St &s, int nslice, const double *__restrict__ P)
+ST_DEV void bb6d_angle(St &s, int nslice, st_pp P)
 {
     double *x = s.x;
     double c2Nrg = P[0], temp3 = P[1], bbscale = P[2], sintheta = P[3], costheta = P[4], tantheta = P[5],
            cos_theta = P[6];
     double gx = P[7], gy = P[8], bx = P[9], ax = P[10], by = P[11], ay = P[12], ex = P[13], ey = P[14];
     double x0 = x[0], px0 = x[1], y0 = x[2], py0 = x[3], z0 = x[4], pz0 = x[5];
-    double ps0 = __builtin_sqrt((1 + pz0) * (1 + pz0) - px0 * px0 - py0 * py0);
+    double ps0 = st_fsqrt((1 + pz0) * (1 + pz0) - px0 * px0 - py0 * py0);
     double h0 = (1 + pz0) - ps0;
     double px_star = (px0 - tantheta * h0) / cos_theta;
     double py_star = py0 / costheta;
     double pz_star = pz0 - tantheta * px0 + tantheta * tantheta * h0;
-    double ps_star = __builtin_sqrt((1 + pz_star) * (1 + pz_star) - px_star * px_star - py_star * py_star);
+    double ps_star = st_fsqrt((1 + pz_star) * (1 + pz_star) - px_star * px_star - py_star * py_star);
     double h_star = (1 + pz_star) - ps_star;
     double x_star = tantheta * z0 + (1 + (px_star / ps_star) * sintheta) * x0;
     double y_star = y0 + sintheta * (py_star / ps_star) * x0;
@@ -633,8 +785,8 @@ ST_DEV void bb6d_angle(St &s, int nslice, const double *__restrict__ P)
         double alfx = (ax - gx * S);
         double betay = by - 2 * S * ay + S * S * gy;
         double alfy = (ay - gy * S);
-        double sigmax = __builtin_sqrt(ex * betax);
-        double sigmay = __builtin_sqrt(ey * betay);
+        double sigmax = st_fsqrt(ex * betax);
+        double sigmay = st_fsqrt(ey * betay);
         double dsx2 = -2.0 * alfx * (ex);
         double dsy2 = -2.0 * alfy * (ey);
         double X = w[0] + w[1] * S - Z0s * sintheta;
@@ -647,7 +799,7 @@ ST_DEV void bb6d_angle(St &s, int nslice, const double *__restrict__ P)
     py_star = w[3];
     z_star = w[4];
     pz_star = w[5];
-    ps_star = __builtin_sqrt((1 + pz_star) * (1 + pz_star) - px_star * px_star - py_star * py_star);
+    ps_star = st_fsqrt((1 + pz_star) * (1 + pz_star) - px_star * px_star - py_star * py_star);
     h_star = 1 + pz_star - ps_star;
     double hx_star = px_star / ps_star;
     double hy_star = py_star / ps_star;
@@ -667,17 +819,17 @@ ST_DEV void bb6d_angle(St &s, int nslice, const double *__restrict__ P)
 }
 
 /* LBT_Pass, simtrack.cpp:6069-6093.  P: sin, cos, tan, cos(theta) */
-ST_DEV void lbt(St &s, const double *__restrict__ P)
+ST_DEV void lbt(St &s, st_pp P)
 {
     double *x = s.x;
     double sintheta = P[0], costheta = P[1], tantheta = P[2], cos_theta = P[3];
     double x0 = x[0], px0 = x[1], y0 = x[2], py0 = x[3], z0 = x[4], pz0 = x[5];
-    double ps0 = __builtin_sqrt((1 + pz0) * (1 + pz0) - px0 * px0 - py0 * py0);
+    double ps0 = st_fsqrt((1 + pz0) * (1 + pz0) - px0 * px0 - py0 * py0);
     double h0 = (1 + pz0) - ps0;
     double px_star = (px0 - tantheta * h0) / cos_theta;
     double py_star = py0 / costheta;
     double pz_star = pz0 - tantheta * px0 + tantheta * tantheta * h0;
-    double ps_star = __builtin_sqrt((1 + pz_star) * (1 + pz_star) - px_star * px_star - py_star * py_star);
+    double ps_star = st_fsqrt((1 + pz_star) * (1 + pz_star) - px_star * px_star - py_star * py_star);
     double h_star = (1 + pz_star) - ps_star;
     x[0] = tantheta * z0 + (1 + (px_star / ps_star) * sintheta) * x0;
     x[1] = px_star;
@@ -688,12 +840,12 @@ ST_DEV void lbt(St &s, const double *__restrict__ P)
 }
 
 /* ILBT_Pass, simtrack.cpp:6095-6123.  P: sin, cos */
-ST_DEV void ilbt(St &s, const double *__restrict__ P)
+ST_DEV void ilbt(St &s, st_pp P)
 {
     double *x = s.x;
     double sintheta = P[0], costheta = P[1];
     double x_star = x[0], px_star = x[1], y_star = x[2], py_star = x[3], z_star = x[4], pz_star = x[5];
-    double ps_star = __builtin_sqrt((1 + pz_star) * (1 + pz_star) - px_star * px_star - py_star * py_star);
+    double ps_star = st_fsqrt((1 + pz_star) * (1 + pz_star) - px_star * px_star - py_star * py_star);
     double h_star = 1 + pz_star - ps_star;
     double hx_star = px_star / ps_star;
     double hy_star = py_star / ps_star;

@@ -143,3 +143,40 @@ def test_gaussian_quantile(lib):
     for p in ps:
         got, ref = lib.orc_ugaussian_pinv(float(p)), norm.ppf(p)
         assert abs(got - ref) <= 2e-15 * max(abs(ref), 1.0)
+
+
+def _small_inputs(limit, rng):
+    """Dense, adversarial arguments below `limit`: uniform, log-spaced down to
+    subnormals, signed zeros, and the neighbourhood of the tiny-argument
+    thresholds and of `limit` itself."""
+    xs = [rng.uniform(-limit, limit, 20000),
+          10.0 ** rng.uniform(-323, np.log10(limit), 20000) * rng.choice([-1.0, 1.0], 20000),
+          np.array([0.0, -0.0, 5e-324, -5e-324, 2.2250738585072014e-308])]
+    for thr in (2.0 ** -27, 2.0 ** -26, limit):
+        v = np.float64(thr)
+        xs.append(np.array([np.nextafter(v, 0.0), np.nextafter(np.nextafter(v, 0.0), 0.0)] if thr == limit else
+                           [v, np.nextafter(v, 0.0), np.nextafter(v, 1.0)]))
+    x = np.concatenate(xs)
+    x = x[np.abs(x) < limit] if limit == 0.125 else x[np.abs(x) <= limit]
+    return np.concatenate([x, -x])
+
+
+def test_branch_free_asin_equals_asin(lib):
+    """st_asin_small (the GPU kernels' first pass) equals st_asin bit for bit
+    for every |x| < 0.125."""
+    xs = _small_inputs(0.125, np.random.default_rng(5))
+    for x in xs:
+        a, b = lib.orc_asin_small(float(x)), lib.orc_asin(float(x))
+        assert np.float64(a).view(np.int64) == np.float64(b).view(np.int64), x
+
+
+def test_branch_free_sincos_equals_sincos(lib):
+    """st_sincos_small equals st_sincos bit for bit for every |x| <= pi/4."""
+    pio4 = float.fromhex("0x1.921fb54442d18p-1")
+    xs = _small_inputs(pio4, np.random.default_rng(6))
+    s1, c1, s2, c2 = (ctypes.c_double() for _ in range(4))
+    for x in xs:
+        lib.orc_sincos_small(ctypes.c_double(x), ctypes.byref(s1), ctypes.byref(c1))
+        lib.orc_sincos(ctypes.c_double(x), ctypes.byref(s2), ctypes.byref(c2))
+        for u, v in ((s1.value, s2.value), (c1.value, c2.value)):
+            assert np.float64(u).view(np.int64) == np.float64(v).view(np.int64), x
