@@ -55,7 +55,7 @@ def main():
     ap.add_argument("--turns-per-step", type=int, default=20)
     ap.add_argument("--particles", type=int, default=65536)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=str, default="512x32", help="particles x turns for the CPU leg")
+    ap.add_argument("--cpu-sample", type=str, default="1024x64", help="particles x turns for the CPU leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the tracking "

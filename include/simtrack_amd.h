@@ -236,6 +236,47 @@ int st_last_gpu_seconds(st_ctx *ctx, double *seconds);
 int st_ctx_set_timing(st_ctx *ctx, int enable);
 int st_ctx_kernel_stats(st_ctx *ctx, double *kernel_seconds, int64_t *launches, int64_t *launched_particle_turns);
 
+/* ------------------------------------------------- dynamic aperture */
+
+/*
+ * Angle sets and bisection ranges of the reference's bisection drivers:
+ * ST_DA_TRACK_DA     Track_DA        (simtrack.cpp:21389): 10 angles
+ *                    (90/11)(i+1) deg, sigma in [0, 50];
+ * ST_DA_TRACK_DA1    Track_DA1       (cpp:21443): 5 angles 15(i+1) deg, [0, 50];
+ * ST_DA_TRACK_DA_RADIAL Track_DA_Radial (cpp:21682): 10 angles, [0, 15].
+ * angles (radians) needs room for 10.
+ */
+#define ST_DA_TRACK_DA 0
+#define ST_DA_TRACK_DA1 1
+#define ST_DA_TRACK_DA_RADIAL 2
+int st_da_angles(int variant, double *angles, int *nangle, double *sigma_max);
+
+/*
+ * The twin-particle bisection of Track_DA / Track_DA1 / Track_DA_Radial
+ * (cpp:21389-21441, 21443-21495, 21682-21734) for nangle angles at once:
+ * particles x = (s cos(a) sigmax0, 0, s sin(a) sigmay0, 0, 0, deltap0),
+ * twins at s and s + 0.05 must both survive nturn turns (Track, or
+ * Track_Radial if radial != 0); the interval [0, sigma_max] is halved until
+ * it is <= 0.1.  sigma_out[i] is the last midpoint tested -- the number the
+ * reference writes as "angle sigma".  Replaces the per-particle Track calls
+ * (cpp:21417, 21426) with one batched launch over the whole bisection tree.
+ */
+int st_da_bisect(st_table *t, int nturn, double deltap0, double sigmax0, double sigmay0,
+                 const double *angles, int nangle, double sigma_max, int radial, double *sigma_out);
+
+/*
+ * Track_DA_Uniform (cpp:21574; sigma0 = 1, dsigma = 0.2, n = 150),
+ * Track_DA_Radial_Uniform (cpp:21736; radial) and
+ * Track_DA_Radial_Uniform_Test (Example2/DA_cal.cpp:61; sigma0 = 3, n = 100,
+ * radial): one particle at sigma_i = sigma0 + i*dsigma on the given angle.
+ * sigma_out/stable_out get all n results; *n_scanned is the number of lines
+ * the reference writes (up to and including the first loss, after which it
+ * calls exit(0); this library returns instead).
+ */
+int st_da_uniform(st_table *t, int nturn, double deltap0, double angle, double sigmax0, double sigmay0,
+                  double sigma0, double dsigma, int n, int radial, double *sigma_out, int *stable_out,
+                  int *n_scanned);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,7 @@ construction API mirroring the reference's C++ classes, and synthetic
 workloads.
 """
 from . import build  # noqa: F401
-from ._capi import (ELEMENT_DTYPE, TRACK_RADIAL, TRACK_SPIN, TYPES, Context, Globals,  # noqa: F401
+from ._capi import (ELEMENT_DTYPE, ST_DA_TRACK_DA, ST_DA_TRACK_DA1, ST_DA_TRACK_DA_RADIAL,  # noqa: F401
+                    TRACK_RADIAL, TRACK_SPIN, TYPES, Context, Globals,
                     STError, Table, load)
 from .line import *  # noqa: F401,F403

@@ -62,8 +62,9 @@ EXPORTS = [
     "st_last_error", "st_ctx_create", "st_ctx_destroy", "st_table_build", "st_table_destroy",
     "st_table_size", "st_track", "st_track_spin", "st_track_tbt", "st_track_radial",
     "st_element_pass", "st_track_device", "st_ctx_sync", "st_last_gpu_seconds",
-    "st_ctx_set_timing", "st_ctx_kernel_stats",
+    "st_ctx_set_timing", "st_ctx_kernel_stats", "st_da_angles", "st_da_bisect", "st_da_uniform",
 ]
+ST_DA_TRACK_DA, ST_DA_TRACK_DA1, ST_DA_TRACK_DA_RADIAL = 0, 1, 2
 
 _lib = None
 
@@ -102,6 +103,9 @@ def load(path=None):
     lib.st_ctx_set_timing.argtypes = [P, I]
     lib.st_ctx_kernel_stats.argtypes = [P, ctypes.POINTER(D), ctypes.POINTER(ctypes.c_int64),
                                         ctypes.POINTER(ctypes.c_int64)]
+    lib.st_da_angles.argtypes = [I, P, ctypes.POINTER(I), ctypes.POINTER(D)]
+    lib.st_da_bisect.argtypes = [P, I, D, D, D, P, I, D, I, P]
+    lib.st_da_uniform.argtypes = [P, I, D, D, D, D, D, D, I, I, P, P, ctypes.POINTER(I)]
     es, gs = I(), I()
     lib.st_struct_sizes(ctypes.byref(es), ctypes.byref(gs))
     if es.value != ELEMENT_DTYPE.itemsize or gs.value != ctypes.sizeof(Globals):

@@ -51,7 +51,8 @@ def build(verbose=False):
     hc = hipcc()
     objs = []
     for src, extra in (("st_kernels.hip", ["--offload-arch=" + ARCH]),
-                       ("st_capi.cpp", ["--offload-arch=" + ARCH, "-x", "hip"])):
+                       ("st_capi.cpp", ["--offload-arch=" + ARCH, "-x", "hip"]),
+                       ("st_da.cpp", [])):
         obj = os.path.join(objdir, src.rsplit(".", 1)[0] + ".o")
         _run([hc, *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
         objs.append(obj)

@@ -48,6 +48,13 @@ static int fail(int code, const char *fmt, ...)
     } while (0)
 
 extern "C" const char *st_last_error(void) { return g_err.c_str(); }
+
+/* error reporting for the library's other translation units (not exported
+   in the public header) */
+extern "C" __attribute__((visibility("hidden"))) int st_internal_fail(int code, const char *msg)
+{
+    return fail(code, "%s", msg);
+}
 extern "C" int st_abi_version(void) { return ST_ABI_VERSION; }
 extern "C" void st_struct_sizes(int *element_bytes, int *globals_bytes)
 {
