@@ -1,0 +1,956 @@
+/*
+ * simtrack_amd.hpp -- SimTrack's C++ tracking API on the MI355X engine.
+ *
+ * Header-only C++17 layer over the C ABI (simtrack_amd.h) that keeps the
+ * reference's names and signatures, so that a SimTrack driver (e.g.
+ * Example2/DA_cal.cpp) switches engines by changing its include:
+ *
+ *   GlobalVariables GP, Set_RefPartEnergy, Set_Spin_G   simtrack.h:28-72, simtrack.cpp:87-115
+ *   Element and the element classes below              simtrack.h:373-959
+ *   Line                                               simtrack.h:1067-1100, simtrack.cpp:12341-12431
+ *   Track, Track_spin, Track_Spin, Track_tbt(_spin),   simtrack.h:1360-1390
+ *   Track_Radial, Track_DA, Track_DA1, Track_DA_Radial,
+ *   Track_DA_Uniform, Track_DA_Radial_Uniform
+ *
+ * Element classes: DRIFT, MARKER, BPM, HBPM, VBPM, SBEND, GSBEND, QUAD,
+ * SKEWQ, SEXT, OCT, MULT, GMULT, SBENDMULT, SMULT, GSBENDMULT, KICKER,
+ * HKICKER, VKICKER, RFCAV, CRABRF, CCMULT, LBT, ILBT, BEAMBEAM, CRBB,
+ * MATRIX, ROTAT, SNAKE, with the reference's constructor arguments, Nint and
+ * Norder defaults (simtrack.cpp:8451, 8635, 8943-8950) and SetP/GetP names.
+ *
+ * Differences, all deliberate:
+ *  - errors throw simtrack_amd::Error instead of printing and calling exit();
+ *  - the per-particle Track* functions lower the Line to a device table on
+ *    first use and reuse it while the lowered records are unchanged;
+ *  - every Track* also has a batched overload taking np particles;
+ *  - Track_DA* run the whole scan as one batch (st_da_bisect/st_da_uniform)
+ *    and write the reference's files; the uniform scans return where the
+ *    reference calls exit(0).
+ *
+ * records_from_line() lowers any Line-like object whose Cell[] elements have
+ * TYPE, L, DX..DPSI, APx, APy and GetP() -- this header's Line or the
+ * reference's own.
+ */
+#ifndef SIMTRACK_AMD_HPP
+#define SIMTRACK_AMD_HPP
+
+#include <cmath>
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "simtrack_amd.h"
+
+namespace simtrack_amd {
+
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+inline void check(int rc, const char *what)
+{
+    if (rc != 0) throw Error(std::string(what) + ": " + st_last_error());
+}
+
+/* slice lengths, simtrack.cpp:45-47 */
+constexpr double BLslice = 0.25, QLslice = 0.05, GLslice = 0.05;
+
+/* ------------------------------------------------------------- globals */
+
+/* GlobalVariables GP (simtrack.h:28-68): the fields the maps read */
+struct GlobalVariables : st_globals {
+    GlobalVariables() { st_globals_default(this); }
+};
+inline GlobalVariables GP;
+
+inline void Set_RefPartEnergy(double gamma, double E0, double Q, double A)
+{
+    st_globals_set_energy(&GP, gamma, E0, Q, A);
+}
+
+/* Set_Spin_G, simtrack.cpp:110-115 (GP.Gr is not used by the maps) */
+inline void Set_Spin_G(double g) { GP.G = g; }
+
+/* ------------------------------------------------------------- elements */
+
+class Element {
+public:
+    explicit Element(std::string name) : NAME(std::move(name)) {}
+    virtual ~Element() = default;
+
+    virtual void SetP(const char *name, double value)
+    {
+        const Param *p = find(name);
+        if (!p) throw Error(TYPE + " does not have a parameter of " + name);
+        if (p->is_int)
+            *reinterpret_cast<int32_t *>(reinterpret_cast<char *>(&rec) + p->off) = (int32_t)value;
+        else
+            *reinterpret_cast<double *>(reinterpret_cast<char *>(&rec) + p->off) = value;
+        after_setp(name, value);
+    }
+    virtual double GetP(const char *name)
+    {
+        const Param *p = find(name);
+        if (!p) throw Error(TYPE + " does not have a parameter of " + name);
+        if (p->is_int) return *reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(&rec) + p->off);
+        return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(&rec) + p->off);
+    }
+
+    /* the C-ABI record of this element */
+    st_element record() const
+    {
+        st_element r = rec;
+        r.l = L;
+        r.dx = DX;
+        r.dy = DY;
+        r.ds = DS;
+        r.dtheta = DTHETA;
+        r.dphi = DPHI;
+        r.dpsi = DPSI;
+        r.apx = APx;
+        r.apy = APy;
+        return r;
+    }
+
+    std::string NAME, TYPE, GROUP;
+    double S = 0., L = 0.;
+    double DX = 0., DY = 0., DS = 0.;
+    double DTHETA = 0., DPHI = 0., DPSI = 0.;
+    double APx = 1., APy = 1.; /* Element::Element, simtrack.cpp:8394-8395 */
+
+protected:
+    struct Param {
+        std::string name;
+        size_t off;
+        bool is_int;
+    };
+    st_element rec{};
+    std::vector<Param> params;
+
+    void init(const char *type, int code, double l)
+    {
+        TYPE = type;
+        rec.type = code;
+        if (l < 0.) throw Error(TYPE + " length should be non-negative");
+        L = l;
+    }
+    void bind(const char *name, double &field) { params.push_back({name, off(&field), false}); }
+    void bind_int(const char *name, int32_t &field) { params.push_back({name, off(&field), true}); }
+    void bind_kn()
+    {
+        for (int i = 0; i < 11; ++i) {
+            bind(("K" + std::to_string(i) + "L").c_str(), rec.knl[i]);
+            bind(("K" + std::to_string(i) + "SL").c_str(), rec.knsl[i]);
+        }
+    }
+    virtual void after_setp(const char *, double) {}
+
+    /* Norder at construction: highest order with KNL or KNSL != 0, else 1
+       (simtrack.cpp:8943-8950); SetP("KnL") does not refresh it */
+    void init_norder()
+    {
+        rec.norder = 1;
+        for (int i = 0; i < 10; ++i)
+            if (rec.knl[10 - i] != 0. || rec.knsl[10 - i] != 0.) {
+                rec.norder = 10 - i;
+                break;
+            }
+    }
+
+private:
+    size_t off(const void *field) const
+    {
+        return (size_t)(reinterpret_cast<const char *>(field) - reinterpret_cast<const char *>(&rec));
+    }
+    const Param *find(const char *name) const
+    {
+        for (const Param &p : params)
+            if (p.name == name) return &p;
+        return nullptr;
+    }
+};
+
+#define SIMTRACK_AMD_SIMPLE(CLS, CODE)                                                                              \
+    class CLS : public Element {                                                                                    \
+    public:                                                                                                         \
+        CLS(std::string name, double l) : Element(std::move(name)) { init(#CLS, CODE, l); }                         \
+    };
+SIMTRACK_AMD_SIMPLE(DRIFT, ST_DRIFT)
+SIMTRACK_AMD_SIMPLE(MARKER, ST_MARKER)
+SIMTRACK_AMD_SIMPLE(BPM, ST_BPM)
+SIMTRACK_AMD_SIMPLE(HBPM, ST_HBPM)
+SIMTRACK_AMD_SIMPLE(VBPM, ST_VBPM)
+#undef SIMTRACK_AMD_SIMPLE
+
+class SBEND : public Element {
+public:
+    SBEND(std::string name, double l, double angle, double e1, double e2) : Element(std::move(name))
+    {
+        init("SBEND", ST_SBEND, l);
+        if (l <= 0.) throw Error("SBEND length should be positive");
+        rec.angle = angle;
+        rec.e1 = e1;
+        rec.e2 = e2;
+        rec.nint = int(l / BLslice) + 1; /* simtrack.cpp:8451 */
+        bind("ANGLE", rec.angle);
+        bind("E1", rec.e1);
+        bind("E2", rec.e2);
+        bind_int("Nint", rec.nint);
+    }
+};
+
+class GSBEND : public Element {
+public:
+    GSBEND(std::string name, double l, double angle, double k0l, double e1, double e2) : Element(std::move(name))
+    {
+        init("GSBEND", ST_GSBEND, l);
+        if (l <= 0.) throw Error("GSBEND length should be positive");
+        rec.angle = angle;
+        rec.k0l = k0l;
+        rec.e1 = e1;
+        rec.e2 = e2;
+        rec.nint = int(l / BLslice) + 1; /* simtrack.cpp:8540 */
+        bind("ANGLE", rec.angle);
+        bind("K0L", rec.k0l);
+        bind("E1", rec.e1);
+        bind("E2", rec.e2);
+        bind_int("Nint", rec.nint);
+    }
+};
+
+/* QUAD / SKEWQ / SEXT / OCT: one order, Nint = int(L/QLslice)+1 */
+class Multipole1 : public Element {
+protected:
+    Multipole1(std::string name, const char *type, int code, int order, double l, double kn, double ks,
+               const char *kn_name, const char *ks_name)
+        : Element(std::move(name))
+    {
+        init(type, code, l);
+        rec.knl[order] = kn;
+        rec.knsl[order] = ks;
+        rec.nint = int(l / QLslice) + 1;
+        rec.norder = order;
+        if (kn_name) bind(kn_name, rec.knl[order]);
+        bind(ks_name, rec.knsl[order]);
+        bind_int("Nint", rec.nint);
+    }
+};
+
+class QUAD : public Multipole1 {
+public:
+    QUAD(std::string name, double l, double k1l, double k1sl)
+        : Multipole1(std::move(name), "QUAD", ST_QUAD, 1, l, k1l, k1sl, "K1L", "K1SL") {}
+};
+class SKEWQ : public Multipole1 {
+public:
+    SKEWQ(std::string name, double l, double k1sl)
+        : Multipole1(std::move(name), "SKEWQ", ST_SKEWQ, 1, l, 0., k1sl, nullptr, "K1SL") {}
+};
+class SEXT : public Multipole1 {
+public:
+    SEXT(std::string name, double l, double k2l, double k2sl)
+        : Multipole1(std::move(name), "SEXT", ST_SEXT, 2, l, k2l, k2sl, "K2L", "K2SL") {}
+};
+class OCT : public Multipole1 {
+public:
+    OCT(std::string name, double l, double k3l, double k3sl)
+        : Multipole1(std::move(name), "OCT", ST_OCT, 3, l, k3l, k3sl, "K3L", "K3SL") {}
+};
+
+class MULT : public Element {
+public:
+    MULT(std::string name, double l, const double knl[11], const double knsl[11]) : Element(std::move(name))
+    {
+        init("MULT", ST_MULT, l);
+        setup(knl, knsl, QLslice);
+    }
+
+protected:
+    MULT(std::string name, const char *type, int code, double l) : Element(std::move(name)) { init(type, code, l); }
+    void setup(const double knl[11], const double knsl[11], double slice)
+    {
+        for (int i = 0; i < 11; ++i) {
+            rec.knl[i] = knl[i];
+            rec.knsl[i] = knsl[i];
+        }
+        rec.nint = int(L / slice) + 1;
+        init_norder();
+        bind_kn();
+        bind_int("Norder", rec.norder);
+        bind_int("Nint", rec.nint);
+    }
+};
+
+/* GMULT / SBENDMULT / SMULT: bend body with multipoles */
+class BendMult : public MULT {
+protected:
+    BendMult(std::string name, const char *type, int code, double l, double angle, double e1, double e2,
+             const double knl[11], const double knsl[11], double slice, const double *k0l = nullptr)
+        : MULT(std::move(name), type, code, l)
+    {
+        rec.angle = angle;
+        rec.e1 = e1;
+        rec.e2 = e2;
+        bind("ANGLE", rec.angle);
+        if (k0l) {
+            /* GSBENDMULT: "K0L" names the bend's K0L before KNL[0], as in
+               its SetP/GetP chain (simtrack.cpp:9519, 9575) */
+            rec.k0l = *k0l;
+            bind("K0L", rec.k0l);
+        }
+        bind("E1", rec.e1);
+        bind("E2", rec.e2);
+        setup(knl, knsl, slice);
+    }
+};
+
+class GMULT : public BendMult {
+public:
+    GMULT(std::string name, double l, double angle, double e1, double e2, const double knl[11], const double knsl[11])
+        : BendMult(std::move(name), "GMULT", ST_GMULT, l, angle, e1, e2, knl, knsl, BLslice) {}
+};
+class SBENDMULT : public BendMult {
+public:
+    SBENDMULT(std::string name, double l, double angle, double e1, double e2, const double knl[11],
+              const double knsl[11])
+        : BendMult(std::move(name), "SBENDMULT", ST_SBENDMULT, l, angle, e1, e2, knl, knsl, BLslice) {}
+};
+class SMULT : public BendMult {
+public:
+    SMULT(std::string name, double l, double angle, double e1, double e2, const double knl[11], const double knsl[11])
+        : BendMult(std::move(name), "SMULT", ST_SMULT, l, angle, e1, e2, knl, knsl, GLslice) {}
+};
+class GSBENDMULT : public BendMult {
+public:
+    GSBENDMULT(std::string name, double l, double angle, double k0l, double e1, double e2, const double knl[11],
+               const double knsl[11])
+        : BendMult(std::move(name), "GSBENDMULT", ST_GSBENDMULT, l, angle, e1, e2, knl, knsl, BLslice, &k0l)
+    {
+    }
+};
+
+class KICKER : public Element {
+public:
+    KICKER(std::string name, double l, double hkick, double vkick) : Element(std::move(name))
+    {
+        init("KICKER", ST_KICKER, l);
+        set(hkick, vkick, true, true);
+    }
+
+protected:
+    KICKER(std::string name, const char *type, int code, double l) : Element(std::move(name)) { init(type, code, l); }
+    void set(double h, double v, bool bh, bool bv)
+    {
+        rec.hkick = h;
+        rec.vkick = v;
+        if (bh) bind("HKICK", rec.hkick);
+        if (bv) bind("VKICK", rec.vkick);
+    }
+};
+class HKICKER : public KICKER {
+public:
+    HKICKER(std::string name, double l, double hkick) : KICKER(std::move(name), "HKICKER", ST_HKICKER, l)
+    {
+        set(hkick, 0., true, false);
+    }
+};
+class VKICKER : public KICKER {
+public:
+    VKICKER(std::string name, double l, double vkick) : KICKER(std::move(name), "VKICKER", ST_VKICKER, l)
+    {
+        set(0., vkick, false, true);
+    }
+};
+
+class RFCAV : public Element {
+public:
+    RFCAV(std::string name, double l, double vrf, double frf, double phase0) : Element(std::move(name))
+    {
+        init("RFCAV", ST_RFCAV, l);
+        set(vrf, frf, phase0);
+    }
+
+protected:
+    RFCAV(std::string name, const char *type, int code, double l) : Element(std::move(name)) { init(type, code, l); }
+    void set(double vrf, double frf, double phase0)
+    {
+        rec.vrf = vrf;
+        rec.frf = frf;
+        rec.phase0 = phase0;
+        bind("VRF", rec.vrf);
+        bind("FRF", rec.frf);
+        bind("PHASE0", rec.phase0);
+    }
+};
+class CRABRF : public RFCAV {
+public:
+    CRABRF(std::string name, double l, double vrf, double frf, double phase0)
+        : RFCAV(std::move(name), "CRABRF", ST_CRABRF, l)
+    {
+        set(vrf, frf, phase0);
+    }
+};
+/* B1..A4 feed CCMULT_Pass's B2..A5 (simtrack.cpp:10943-10947) */
+class CCMULT : public RFCAV {
+public:
+    CCMULT(std::string name, double l, double vrf, double frf, double phase0, double b1, double a1, double b2,
+           double a2, double b3, double a3, double b4, double a4)
+        : RFCAV(std::move(name), "CCMULT", ST_CCMULT, l)
+    {
+        set(vrf, frf, phase0);
+        const double v[8] = {b1, a1, b2, a2, b3, a3, b4, a4};
+        const char *n[8] = {"B1", "A1", "B2", "A2", "B3", "A3", "B4", "A4"};
+        for (int i = 0; i < 8; ++i) {
+            rec.cc[i] = v[i];
+            bind(n[i], rec.cc[i]);
+        }
+    }
+};
+
+class LBT : public Element {
+public:
+    LBT(std::string name, double theta) : Element(std::move(name))
+    {
+        init("LBT", ST_LBT, 0.);
+        rec.theta = theta;
+        bind("THETA", rec.theta);
+    }
+
+protected:
+    LBT(std::string name, const char *type, int code, double theta) : Element(std::move(name))
+    {
+        init(type, code, 0.);
+        rec.theta = theta;
+        bind("THETA", rec.theta);
+    }
+};
+class ILBT : public LBT {
+public:
+    ILBT(std::string name, double theta) : LBT(std::move(name), "ILBT", ST_ILBT, theta) {}
+};
+
+class BEAMBEAM : public Element {
+public:
+    BEAMBEAM(std::string name, int treatment, double np, double bbscale, double sigmal, int nslice, double emitx,
+             double betax, double alfax, double emity, double betay, double alfay)
+        : Element(std::move(name))
+    {
+        init("BEAMBEAM", ST_BEAMBEAM, 0.);
+        if (treatment != 4 && treatment != 6) throw Error("beambeam treatment only can be 4 or 6");
+        rec.treatment = treatment;
+        bind_int("TREATMENT", rec.treatment);
+        set(np, bbscale, sigmal, nslice, emitx, betax, alfax, emity, betay, alfay);
+    }
+
+protected:
+    BEAMBEAM(std::string name, const char *type, int code) : Element(std::move(name)) { init(type, code, 0.); }
+    void set(double np, double bbscale, double sigmal, int nslice, double emitx, double betax, double alfax,
+             double emity, double betay, double alfay)
+    {
+        rec.np = np;
+        rec.bbscale = bbscale;
+        rec.sigmal = sigmal;
+        rec.nslice = nslice;
+        rec.emitx = emitx;
+        rec.betax = betax;
+        rec.alfax = alfax;
+        rec.emity = emity;
+        rec.betay = betay;
+        rec.alfay = alfay;
+        bind("NP", rec.np);
+        bind("BBSCALE", rec.bbscale);
+        bind("SIGMAL", rec.sigmal);
+        bind_int("NSLICE", rec.nslice);
+        bind("EMITX", rec.emitx);
+        bind("EMITY", rec.emity);
+        bind("BETAX", rec.betax);
+        bind("ALFAX", rec.alfax);
+        bind("BETAY", rec.betay);
+        bind("ALFAY", rec.alfay);
+    }
+    void after_setp(const char *name, double value) override
+    {
+        if (std::strcmp(name, "TREATMENT") == 0 && (int)value != 4 && (int)value != 6)
+            throw Error("beambeam treatment only can be 4 or 6");
+    }
+};
+class CRBB : public BEAMBEAM {
+public:
+    CRBB(std::string name, double np, double theta, double bbscale, double sigmal, int nslice, double emitx,
+         double betax, double alfax, double emity, double betay, double alfay)
+        : BEAMBEAM(std::move(name), "CRBB", ST_CRBB)
+    {
+        rec.treatment = 6;
+        rec.theta = theta;
+        bind("THETA", rec.theta);
+        set(np, bbscale, sigmal, nslice, emitx, betax, alfax, emity, betay, alfay);
+    }
+};
+
+class MATRIX : public Element {
+public:
+    MATRIX(std::string name, double l, const double xco_in[6], const double xco_out[6], const double m66[36])
+        : Element(std::move(name))
+    {
+        init("MATRIX", ST_MATRIX, l);
+        for (int i = 0; i < 6; ++i) {
+            rec.xco_in[i] = xco_in[i];
+            rec.xco_out[i] = xco_out[i];
+        }
+        for (int i = 0; i < 36; ++i) rec.m66[i] = m66[i];
+    }
+};
+
+class ROTAT : public Element {
+public:
+    ROTAT(std::string name, double l, const double n[3], double angle) : Element(std::move(name))
+    {
+        init("ROTAT", ST_ROTAT, l);
+        set(n, angle);
+    }
+
+protected:
+    ROTAT(std::string name, const char *type, int code, double l) : Element(std::move(name)) { init(type, code, l); }
+    void set(const double n[3], double angle)
+    {
+        for (int i = 0; i < 3; ++i) rec.rot_n[i] = n[i];
+        rec.rot_angle = angle;
+        bind("NX", rec.rot_n[0]);
+        bind("NY", rec.rot_n[1]);
+        bind("NS", rec.rot_n[2]);
+        bind("ANGLE", rec.rot_angle);
+    }
+};
+class SNAKE : public ROTAT {
+public:
+    SNAKE(std::string name, double l, const double n[3], double angle)
+        : ROTAT(std::move(name), "SNAKE", ST_SNAKE, l)
+    {
+        set(n, angle);
+    }
+};
+
+/* ----------------------------------------------------------------- line */
+
+/* Line (simtrack.h:1067-1100): owns its elements, as the reference's does
+   (Delete/Replace/Empty delete them, simtrack.cpp:12381-12431) */
+class Line {
+public:
+    Line() = default;
+    Line(const Line &) = delete;
+    Line &operator=(const Line &) = delete;
+    ~Line() { Empty(); }
+
+    void Update()
+    {
+        double s = 0.;
+        for (Element *e : Cell) {
+            s = e->L + s;
+            e->S = s;
+        }
+        Ncell = (long)Cell.size();
+        Length = s;
+        frev0 = (299792458. * GP.beta) / Length;
+    }
+    void Append(Element *x)
+    {
+        Cell.push_back(x);
+        Update();
+    }
+    void Delete(int i)
+    {
+        if (i < 0 || i >= (int)Cell.size()) return;
+        delete Cell[i];
+        Cell.erase(Cell.begin() + i);
+        Update();
+    }
+    void Insert(int i, Element *x)
+    {
+        if (i < 0 || i >= (int)Cell.size()) return;
+        Cell.insert(Cell.begin() + i, x);
+        Update();
+    }
+    void Replace(int i, Element *x)
+    {
+        if (i < 0 || i >= (int)Cell.size()) return;
+        delete Cell[i];
+        Cell[i] = x;
+        Update();
+    }
+    void Empty()
+    {
+        for (Element *e : Cell) delete e;
+        Cell.clear();
+        Ncell = 0;
+        Length = 0.;
+    }
+
+    std::vector<Element *> Cell;
+    double Length = 0.;
+    long Ncell = 0;
+    double frev0 = 0.;
+};
+
+/* --------------------------------------------------------- lowering */
+
+/* st_element code of a reference TYPE string, -1 if the engine lacks it */
+inline int type_code(const std::string &t)
+{
+    static const char *names[] = {"", "DRIFT", "SBEND", "GSBEND", "QUAD", "SKEWQ", "SEXT", "OCT", "MULT", "GMULT",
+                                  "SBENDMULT", "SMULT", "GSBENDMULT", "KICKER", "HKICKER", "VKICKER", "MARKER",
+                                  "BPM", "HBPM", "VBPM", "RFCAV", "CRABRF", "CCMULT", "LBT", "ILBT", "BEAMBEAM",
+                                  "CRBB", "MATRIX", "ROTAT", "SNAKE"};
+    for (int i = 1; i < (int)(sizeof(names) / sizeof(names[0])); ++i)
+        if (t == names[i]) return i;
+    return -1;
+}
+
+/* The record of one element of any Line-like type, read through the public
+   fields and GetP() only (simtrack.h:379-395 and each class's GetP names):
+   what a reference-side driver can see.  MATRIX keeps M66/XCO private in
+   the reference and has no GetP, so its matrix is read only from this
+   header's own MATRIX. */
+template <class E>
+st_element record_of(E &e)
+{
+    st_element r{};
+    if constexpr (std::is_base_of_v<Element, E>) {
+        r = e.record();
+        return r;
+    } else {
+        int code = type_code(e.TYPE);
+        if (code < 0) throw Error("element " + e.NAME + ": type " + e.TYPE + " is not supported by the engine");
+        r.type = code;
+        r.l = e.L;
+        r.dx = e.DX;
+        r.dy = e.DY;
+        r.ds = e.DS;
+        r.dtheta = e.DTHETA;
+        r.dphi = e.DPHI;
+        r.dpsi = e.DPSI;
+        r.apx = e.APx;
+        r.apy = e.APy;
+        auto P = [&](const char *n) { return e.GetP(n); };
+        auto kn = [&]() {
+            for (int i = 0; i < 11; ++i) {
+                r.knl[i] = P(("K" + std::to_string(i) + "L").c_str());
+                r.knsl[i] = P(("K" + std::to_string(i) + "SL").c_str());
+            }
+        };
+        switch (code) {
+        case ST_SBEND:
+        case ST_GSBEND:
+            r.angle = P("ANGLE");
+            r.e1 = P("E1");
+            r.e2 = P("E2");
+            r.nint = (int)P("Nint");
+            if (code == ST_GSBEND) r.k0l = P("K0L");
+            break;
+        case ST_QUAD:
+            r.knl[1] = P("K1L");
+            r.knsl[1] = P("K1SL");
+            r.nint = (int)P("Nint");
+            r.norder = 1;
+            break;
+        case ST_SKEWQ:
+            r.knsl[1] = P("K1SL");
+            r.nint = (int)P("Nint");
+            r.norder = 1;
+            break;
+        case ST_SEXT:
+            r.knl[2] = P("K2L");
+            r.knsl[2] = P("K2SL");
+            r.nint = (int)P("Nint");
+            r.norder = 2;
+            break;
+        case ST_OCT:
+            r.knl[3] = P("K3L");
+            r.knsl[3] = P("K3SL");
+            r.nint = (int)P("Nint");
+            r.norder = 3;
+            break;
+        case ST_MULT:
+        case ST_GMULT:
+        case ST_SBENDMULT:
+        case ST_SMULT:
+        case ST_GSBENDMULT:
+            kn();
+            /* GSBENDMULT's "K0L" is the bend's K0L; its KNL[0] has no name */
+            if (code == ST_GSBENDMULT) r.knl[0] = 0.;
+            r.norder = (int)P("Norder"); /* the constructor's, not recomputed */
+            r.nint = (int)P("Nint");
+            if (code != ST_MULT) {
+                r.angle = P("ANGLE");
+                r.e1 = P("E1");
+                r.e2 = P("E2");
+            }
+            if (code == ST_GSBENDMULT) r.k0l = P("K0L");
+            break;
+        case ST_KICKER:
+            r.hkick = P("HKICK");
+            r.vkick = P("VKICK");
+            break;
+        case ST_HKICKER:
+            r.hkick = P("HKICK");
+            break;
+        case ST_VKICKER:
+            r.vkick = P("VKICK");
+            break;
+        case ST_RFCAV:
+        case ST_CRABRF:
+        case ST_CCMULT:
+            r.vrf = P("VRF");
+            r.frf = P("FRF");
+            r.phase0 = P("PHASE0");
+            if (code == ST_CCMULT) {
+                const char *n[8] = {"B1", "A1", "B2", "A2", "B3", "A3", "B4", "A4"};
+                for (int i = 0; i < 8; ++i) r.cc[i] = P(n[i]);
+            }
+            break;
+        case ST_LBT:
+        case ST_ILBT:
+            r.theta = P("THETA");
+            break;
+        case ST_BEAMBEAM:
+        case ST_CRBB:
+            r.treatment = code == ST_CRBB ? 6 : (int)P("TREATMENT");
+            if (code == ST_CRBB) r.theta = P("THETA");
+            r.np = P("NP");
+            r.bbscale = P("BBSCALE");
+            r.sigmal = P("SIGMAL");
+            r.nslice = (int)P("NSLICE");
+            r.emitx = P("EMITX");
+            r.emity = P("EMITY");
+            r.betax = P("BETAX");
+            r.alfax = P("ALFAX");
+            r.betay = P("BETAY");
+            r.alfay = P("ALFAY");
+            break;
+        case ST_ROTAT:
+        case ST_SNAKE:
+            r.rot_n[0] = P("NX");
+            r.rot_n[1] = P("NY");
+            r.rot_n[2] = P("NS");
+            r.rot_angle = P("ANGLE");
+            break;
+        case ST_MATRIX:
+            throw Error("element " + e.NAME + ": the reference's MATRIX exposes no parameters to read");
+        default:
+            break;
+        }
+        return r;
+    }
+}
+
+template <class LineT>
+std::vector<st_element> records_from_line(LineT &line)
+{
+    std::vector<st_element> out;
+    out.reserve(line.Cell.size());
+    for (auto *e : line.Cell) out.push_back(record_of(*e));
+    return out;
+}
+
+/* -------------------------------------------------------------- engine */
+
+/* the process's device context on HIP device 0 (one per process, as the
+   reference's GP is) */
+inline st_ctx *default_context()
+{
+    struct Holder {
+        st_ctx *c = nullptr;
+        Holder() { check(st_ctx_create(0, &c), "st_ctx_create"); }
+        ~Holder()
+        {
+            if (c) st_ctx_destroy(c);
+        }
+    };
+    static Holder h;
+    return h.c;
+}
+
+/* A Line lowered to a device table, for repeated tracking */
+class Tracker {
+public:
+    template <class LineT>
+    explicit Tracker(LineT &line, const st_globals &gp = GP, st_ctx *ctx = default_context())
+        : ctx_(ctx), recs_(records_from_line(line)), gp_(gp)
+    {
+        check(st_table_build(ctx_, &gp_, recs_.data(), (int)recs_.size(), &t_), "st_table_build");
+    }
+    Tracker(const Tracker &) = delete;
+    Tracker &operator=(const Tracker &) = delete;
+    ~Tracker()
+    {
+        if (t_) st_table_destroy(t_);
+    }
+    st_table *table() const { return t_; }
+    bool same(const std::vector<st_element> &r, const st_globals &gp) const
+    {
+        return r.size() == recs_.size() && std::memcmp(r.data(), recs_.data(), r.size() * sizeof(st_element)) == 0 &&
+               std::memcmp(&gp, &gp_, sizeof(st_globals)) == 0;
+    }
+
+    void Track(double *x, int np, int nturn, int *stable, int *lost_turn, int *lost_post)
+    {
+        check(st_track(t_, x, np, nturn, stable, lost_turn, lost_post), "Track");
+    }
+    void Track_spin(double *x, int np, int nturn, int *stable, int *lost_turn, int *lost_post)
+    {
+        check(st_track_spin(t_, x, np, nturn, stable, lost_turn, lost_post), "Track_spin");
+    }
+    void Track_Radial(double *x, int np, int nturn, int *stable, int *lost_turn, int *lost_post)
+    {
+        check(st_track_radial(t_, x, np, nturn, stable, lost_turn, lost_post), "Track_Radial");
+    }
+    void Track_tbt(double *x, int np, int nturn, double *x_tbt, int bpm_index, bool spin, int *stable,
+                   int *lost_turn, int *lost_post)
+    {
+        check(st_track_tbt(t_, x, np, nturn, x_tbt, bpm_index, spin ? 1 : 0, stable, lost_turn, lost_post),
+              "Track_tbt");
+    }
+
+private:
+    st_ctx *ctx_;
+    std::vector<st_element> recs_;
+    st_globals gp_;
+    st_table *t_ = nullptr;
+};
+
+/* the table of a Line for the per-call API: rebuilt only when the lowered
+   records or GP change */
+template <class LineT>
+Tracker &tracker_for(LineT &line)
+{
+    static std::unique_ptr<Tracker> cached;
+    static const void *owner = nullptr;
+    std::vector<st_element> r = records_from_line(line);
+    if (!cached || owner != (const void *)&line || !cached->same(r, GP)) {
+        cached.reset();
+        cached = std::make_unique<Tracker>(line);
+        owner = (const void *)&line;
+    }
+    return *cached;
+}
+
+/* ------------------------------------------------- Track* (reference API) */
+
+inline void Track(Line &l, double x[], int nturn, int &stable, int &lost_turn, int &lost_post)
+{
+    tracker_for(l).Track(x, 1, nturn, &stable, &lost_turn, &lost_post);
+}
+inline void Track_spin(Line &l, double x[], int nturn, int &stable, int &lost_turn, int &lost_post)
+{
+    tracker_for(l).Track_spin(x, 1, nturn, &stable, &lost_turn, &lost_post);
+}
+inline void Track_Spin(Line &l, double x[], int nturn, int &stable, int &lost_turn, int &lost_post)
+{
+    Track_spin(l, x, nturn, stable, lost_turn, lost_post);
+}
+inline void Track_Radial(Line &l, double x[], int nturn, int &stable, int &lost_turn, int &lost_post)
+{
+    tracker_for(l).Track_Radial(x, 1, nturn, &stable, &lost_turn, &lost_post);
+}
+inline void Track_tbt(Line &l, double x[], int nturn, double x_tbt[], int &stable, int &lost_turn, int &lost_post)
+{
+    tracker_for(l).Track_tbt(x, 1, nturn, x_tbt, -1, false, &stable, &lost_turn, &lost_post);
+}
+inline void Track_tbt(Line &l, double x[], int nturn, double x_tbt[], int &bpm_index, int &stable, int &lost_turn,
+                      int &lost_post)
+{
+    tracker_for(l).Track_tbt(x, 1, nturn, x_tbt, bpm_index, false, &stable, &lost_turn, &lost_post);
+}
+inline void Track_tbt_spin(Line &l, double x[], int nturn, double x_tbt[], int &stable, int &lost_turn,
+                           int &lost_post)
+{
+    tracker_for(l).Track_tbt(x, 1, nturn, x_tbt, -1, true, &stable, &lost_turn, &lost_post);
+}
+inline void Track_tbt_spin(Line &l, double x[], int nturn, double x_tbt[], int &bpm_index, int &stable,
+                           int &lost_turn, int &lost_post)
+{
+    tracker_for(l).Track_tbt(x, 1, nturn, x_tbt, bpm_index, true, &stable, &lost_turn, &lost_post);
+}
+
+/* batched forms: np particles, x is np x 6 (np x 9 with spin) */
+inline void Track(Line &l, double *x, int np, int nturn, int *stable, int *lost_turn, int *lost_post)
+{
+    tracker_for(l).Track(x, np, nturn, stable, lost_turn, lost_post);
+}
+inline void Track_spin(Line &l, double *x, int np, int nturn, int *stable, int *lost_turn, int *lost_post)
+{
+    tracker_for(l).Track_spin(x, np, nturn, stable, lost_turn, lost_post);
+}
+inline void Track_Radial(Line &l, double *x, int np, int nturn, int *stable, int *lost_turn, int *lost_post)
+{
+    tracker_for(l).Track_Radial(x, np, nturn, stable, lost_turn, lost_post);
+}
+
+/* --------------------------------------------------------- DA drivers */
+
+namespace detail {
+inline void da_bisect_file(Line &l, int variant, int nturn, double deltap0, double sigmax0, double sigmay0,
+                           const char *filename)
+{
+    double angles[10], smax;
+    int n;
+    check(st_da_angles(variant, angles, &n, &smax), "st_da_angles");
+    std::vector<double> sig((size_t)n);
+    check(st_da_bisect(tracker_for(l).table(), nturn, deltap0, sigmax0, sigmay0, angles, n, smax,
+                       variant == ST_DA_TRACK_DA_RADIAL ? 1 : 0, sig.data()),
+          "st_da_bisect");
+    FILE *f = std::fopen(filename, "w");
+    if (!f) throw Error(std::string("cannot open ") + filename);
+    for (int i = 0; i < n; ++i) {
+        double deg = variant == ST_DA_TRACK_DA1 ? 15. * (i + 1) : (90 / 11.) * (i + 1);
+        std::fprintf(f, "%f %f \n", deg, sig[i]);
+    }
+    std::fclose(f);
+}
+
+inline void da_uniform_file(Line &l, int nturn, double deltap0, double angle, double sigmax0, double sigmay0,
+                            const char *filename, double sigma0, int n, bool radial)
+{
+    std::vector<double> sig((size_t)n);
+    std::vector<int> st((size_t)n);
+    int m = 0;
+    check(st_da_uniform(tracker_for(l).table(), nturn, deltap0, angle, sigmax0, sigmay0, sigma0, 0.2, n,
+                        radial ? 1 : 0, sig.data(), st.data(), &m),
+          "st_da_uniform");
+    FILE *f = std::fopen(filename, "w");
+    if (!f) throw Error(std::string("cannot open ") + filename);
+    for (int i = 0; i < m; ++i) std::fprintf(f, "%f %f %f \n", angle, sig[i], 1.0 * st[i]);
+    std::fclose(f);
+}
+} // namespace detail
+
+inline void Track_DA(Line &l, int nturn, double deltap0, double sigmax0, double sigmay0, const char *filename)
+{
+    detail::da_bisect_file(l, ST_DA_TRACK_DA, nturn, deltap0, sigmax0, sigmay0, filename);
+}
+inline void Track_DA1(Line &l, int nturn, double deltap0, double sigmax0, double sigmay0, const char *filename)
+{
+    detail::da_bisect_file(l, ST_DA_TRACK_DA1, nturn, deltap0, sigmax0, sigmay0, filename);
+}
+inline void Track_DA_Radial(Line &l, int nturn, double deltap0, double sigmax0, double sigmay0, const char *filename)
+{
+    detail::da_bisect_file(l, ST_DA_TRACK_DA_RADIAL, nturn, deltap0, sigmax0, sigmay0, filename);
+}
+inline void Track_DA_Uniform(Line &l, int nturn, double deltap0, double angle, double sigmax0, double sigmay0,
+                             const char *filename)
+{
+    detail::da_uniform_file(l, nturn, deltap0, angle, sigmax0, sigmay0, filename, 1., 150, false);
+}
+inline void Track_DA_Radial_Uniform(Line &l, int nturn, double deltap0, double angle, double sigmax0,
+                                    double sigmay0, const char *filename)
+{
+    detail::da_uniform_file(l, nturn, deltap0, angle, sigmax0, sigmay0, filename, 1.0, 150, true);
+}
+
+} // namespace simtrack_amd
+
+#endif /* SIMTRACK_AMD_HPP */

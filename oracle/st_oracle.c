@@ -151,7 +151,10 @@ static void Cal_W_Matrix(double theta, double phi, double psi, double W[])
 static void Cal_VE_WE_Matrx(double length, double angle, double tilt, double VE[], double WE[])
 {
     double temp, cospsi, costhe, sinpsi, sinthe;
-    if (fabs(angle) < 1.0e-12) {
+    /* the reference tests < 1e-12 and > 1e-12, leaving VE/WE unset at
+       |angle| == 1e-12 exactly; that case is taken as straight here and in
+       the table builder */
+    if (fabs(angle) <= 1.0e-12) {
         for (int i = 0; i < 3; i++) VE[i] = 0.;
         VE[2] = length;
         for (int i = 0; i < 9; i++) WE[i] = 0.;

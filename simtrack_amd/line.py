@@ -219,10 +219,12 @@ class OCT(_Multipole):
 
 
 def _kn_params(extra):
+    """KnL/KnSL names after the class's own: a name the class already uses
+    keeps its meaning (GSBENDMULT's "K0L" is the bend's K0L, simtrack.cpp:9575)."""
     p = dict(extra)
     for i in range(11):
-        p[f"K{i}L"] = ("KNL", i)
-        p[f"K{i}SL"] = ("KNSL", i)
+        p.setdefault(f"K{i}L", ("KNL", i))
+        p.setdefault(f"K{i}SL", ("KNSL", i))
     return p
 
 
