@@ -52,7 +52,7 @@ int main(int argc, char **argv)
     const int np = 24, nturn = 200;
     std::vector<double> x0(np * 6, 0.), xs(np * 9, 0.);
     for (int p = 0; p < np; ++p) {
-        double a = 1e-3 * (p + 1), t = 0.2 + 0.05 * p;
+        double a = 2e-3 * (p + 1), t = 0.2 + 0.05 * p;
         x0[p * 6 + 0] = a * std::cos(t);
         x0[p * 6 + 2] = 0.6 * a * std::sin(t);
         x0[p * 6 + 5] = 2e-4;

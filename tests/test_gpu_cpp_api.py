@@ -78,7 +78,7 @@ def test_cpp_api_tracks_like_the_oracle(tmp_path):
     orc = pyoracle.Oracle(g, rec)
     x0 = np.zeros((NP, 6))
     for p in range(NP):
-        a, t = 1e-3 * (p + 1), 0.2 + 0.05 * p
+        a, t = 2e-3 * (p + 1), 0.2 + 0.05 * p
         x0[p] = [a * math.cos(t), 0., 0.6 * a * math.sin(t), 0., 0., 2e-4]
     xo, so, lto, lpo = orc.track(x0.copy(), NTURN)
     assert np.array_equal(s1, so) and np.array_equal(t1, lto) and np.array_equal(p1, lpo)
