@@ -277,6 +277,28 @@ int st_da_uniform(st_table *t, int nturn, double deltap0, double angle, double s
                   double sigma0, double dsigma, int n, int radial, double *sigma_out, int *stable_out,
                   int *n_scanned);
 
+/* ------------------------------------------------------ tune finder */
+
+/*
+ * FineTuneFinder (simtrack.cpp:2288-2319) for nsig signals of n samples:
+ * the peak of sum_i x[i] exp(-2 pi i tune i) over [tune_lower, tune_upper),
+ * scanned in steps of 1e-3 and refined by factors of 10 down to 1e-12.
+ * With px != null the x/px form (cpp:2321-2352) is used:
+ * sum_i (x[i] + i px[i]) exp(-2 pi i tune i).  Host arrays are nsig x n
+ * (signal-major).  The range may span at most 4096 steps of 1e-3, and the
+ * samples (x and px) plus one level's candidates must fit in 60 KiB.
+ */
+int st_fine_tune(st_ctx *ctx, const double *x, const double *px, int n, int nsig, double tune_lower,
+                 double tune_upper, double *tune);
+
+/*
+ * Device form on strided data, e.g. turn-by-turn records: sample i of
+ * signal j is d_x[j*stride_sig + i*stride] (and d_px likewise, or null).
+ * Launches on the context stream; st_ctx_sync waits.
+ */
+int st_fine_tune_device(st_ctx *ctx, const double *d_x, const double *d_px, int64_t stride_sig, int64_t stride,
+                        int n, int nsig, double tune_lower, double tune_upper, double *d_tune);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,6 +39,7 @@ def load(glibc=False):
     lib.orc_bb4d.argtypes = [D, D, D, D, D, D, P, P]
     lib.orc_bb_slice_centers.argtypes = [I, D, P]
     lib.orc_fine_tune.argtypes = [I, P, D, D, P]
+    lib.orc_fine_tune_xpx.argtypes = [I, P, P, D, D, P]
     for f in ("orc_sin", "orc_cos", "orc_tan", "orc_asin", "orc_exp", "orc_log", "orc_ugaussian_pinv"):
         getattr(lib, f).argtypes = [D]
         getattr(lib, f).restype = D

@@ -2079,6 +2079,35 @@ EXPORT void orc_fine_tune(int n, const double *x, double tune_lower, double tune
     } while (tune_step > 1.0e-12);
 }
 
+/* FineTuneFinder, x/px form, simtrack.cpp:2321-2352 */
+EXPORT void orc_fine_tune_xpx(int n, const double *x, const double *px, double tune_lower, double tune_upper,
+                              double *peaktune)
+{
+    double tune, tune_step = 0.001, sumr, sumi, amp, peakamp;
+    do {
+        tune = tune_lower;
+        *peaktune = tune_lower;
+        peakamp = 0;
+        do {
+            sumr = 0;
+            sumi = 0.;
+            for (int i = 0; i < n; i++) {
+                sumr = sumr + x[i] * OCOS(-2 * PI * i * tune) - px[i] * OSIN(-2 * PI * i * tune);
+                sumi = sumi + x[i] * OSIN(-2 * PI * i * tune) + px[i] * OCOS(-2 * PI * i * tune);
+            }
+            amp = sumr * sumr + sumi * sumi;
+            if (amp > peakamp) {
+                *peaktune = tune;
+                peakamp = amp;
+            }
+            tune = tune + tune_step;
+        } while (tune < tune_upper);
+        tune_lower = *peaktune - tune_step;
+        tune_upper = *peaktune + tune_step;
+        tune_step = tune_step / 10.;
+    } while (tune_step > 1.0e-12);
+}
+
 /* ------------------------------------------- elementary functions */
 EXPORT double orc_sin(double x) { return OSIN(x); }
 EXPORT double orc_cos(double x) { return OCOS(x); }

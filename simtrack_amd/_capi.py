@@ -63,6 +63,7 @@ EXPORTS = [
     "st_table_size", "st_track", "st_track_spin", "st_track_tbt", "st_track_radial",
     "st_element_pass", "st_track_device", "st_ctx_sync", "st_last_gpu_seconds",
     "st_ctx_set_timing", "st_ctx_kernel_stats", "st_da_angles", "st_da_bisect", "st_da_uniform",
+    "st_fine_tune", "st_fine_tune_device",
 ]
 ST_DA_TRACK_DA, ST_DA_TRACK_DA1, ST_DA_TRACK_DA_RADIAL = 0, 1, 2
 
@@ -106,6 +107,8 @@ def load(path=None):
     lib.st_da_angles.argtypes = [I, P, ctypes.POINTER(I), ctypes.POINTER(D)]
     lib.st_da_bisect.argtypes = [P, I, D, D, D, P, I, D, I, P]
     lib.st_da_uniform.argtypes = [P, I, D, D, D, D, D, D, I, I, P, P, ctypes.POINTER(I)]
+    lib.st_fine_tune.argtypes = [P, P, P, I, I, D, D, P]
+    lib.st_fine_tune_device.argtypes = [P, P, P, ctypes.c_int64, ctypes.c_int64, I, I, D, D, P]
     es, gs = I(), I()
     lib.st_struct_sizes(ctypes.byref(es), ctypes.byref(gs))
     if es.value != ELEMENT_DTYPE.itemsize or gs.value != ctypes.sizeof(Globals):
@@ -154,6 +157,17 @@ class Context:
 
     def set_timing(self, on=True):
         check(self.lib.st_ctx_set_timing(self.h, 1 if on else 0), "st_ctx_set_timing")
+
+    def fine_tune(self, x, lower, upper, px=None):
+        """FineTuneFinder (simtrack.cpp:2288, x/px form cpp:2321) of each row of x."""
+        x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float64)
+        if px is not None:
+            px = np.ascontiguousarray(np.atleast_2d(px), dtype=np.float64)
+            assert px.shape == x.shape
+        out = np.zeros(x.shape[0])
+        check(self.lib.st_fine_tune(self.h, _ptr(x), _ptr(px), x.shape[1], x.shape[0], float(lower), float(upper),
+                                    _ptr(out)), "st_fine_tune")
+        return out
 
     def kernel_stats(self):
         """(summed kernel seconds, launches, launched particle-turns) since the last call."""

@@ -23,6 +23,7 @@
 #define ST_SQRT(x) std::sqrt(x)
 #include "st_libm.h"
 #include "st_kernels.h"
+#include "st_tune.h"
 #include "st_table_format.h"
 
 /* ------------------------------------------------------------ errors */
@@ -1380,5 +1381,75 @@ extern "C" int st_element_pass(st_table *t, int elem_index, double *x, int np, i
     HIPCHK(st_launch_track(a, spin != 0, t->gp.H_expand != 0, t->gp.radiate != 0, c->stream));
     HIPCHK(hipMemcpyAsync(x, c->d_x, (size_t)np * ncol * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    return ST_OK;
+}
+
+/* ------------------------------------------------------ tune finder */
+
+static int tune_check(const st_ctx *c, int n, int nsig, double lo, double hi, bool px, int *max_cand)
+{
+    if (!c || n <= 0 || nsig < 0 || !(hi > lo) || !std::isfinite(lo) || !std::isfinite(hi))
+        return fail(ST_ERR_ARG, "st_fine_tune: bad argument");
+    double lvl0 = std::ceil((hi - lo) / 0.001) + 2.0;
+    if (lvl0 > 4096.) return fail(ST_ERR_ARG, "st_fine_tune: tune range too wide (more than 4096 steps of 1e-3)");
+    int mc = std::max((int)lvl0, 24);
+    size_t lds = sizeof(double) * ((size_t)n * (px ? 2 : 1) + (size_t)mc);
+    if (lds > 60 * 1024)
+        return fail(ST_ERR_ARG, "st_fine_tune: %d samples%s and %d candidates exceed the 60 KiB LDS budget", n,
+                    px ? " (x and px)" : "", mc);
+    *max_cand = mc;
+    return ST_OK;
+}
+
+extern "C" int st_fine_tune_device(st_ctx *c, const double *d_x, const double *d_px, int64_t stride_sig,
+                                   int64_t stride, int n, int nsig, double tune_lower, double tune_upper,
+                                   double *d_tune)
+{
+    int mc = 0;
+    int rc = tune_check(c, n, nsig, tune_lower, tune_upper, d_px != nullptr, &mc);
+    if (rc) return rc;
+    if (nsig == 0) return ST_OK;
+    if (!d_x || !d_tune || stride < 1 || stride_sig < 0) return fail(ST_ERR_ARG, "st_fine_tune_device: bad argument");
+    HIPCHK(hipSetDevice(c->device));
+    st_tune_args a;
+    a.x = d_x;
+    a.px = d_px;
+    a.stride_sig = stride_sig;
+    a.stride = stride;
+    a.n = n;
+    a.nsig = nsig;
+    a.max_cand = mc;
+    a.lower = tune_lower;
+    a.upper = tune_upper;
+    a.tune = d_tune;
+    HIPCHK(st_launch_fine_tune(a, c->stream));
+    return ST_OK;
+}
+
+extern "C" int st_fine_tune(st_ctx *c, const double *x, const double *px, int n, int nsig, double tune_lower,
+                            double tune_upper, double *tune)
+{
+    int mc = 0;
+    int rc = tune_check(c, n, nsig, tune_lower, tune_upper, px != nullptr, &mc);
+    if (rc) return rc;
+    if (nsig == 0) return ST_OK;
+    if (!x || !tune) return fail(ST_ERR_ARG, "st_fine_tune: bad argument");
+    HIPCHK(hipSetDevice(c->device));
+    size_t cnt = (size_t)n * (size_t)nsig;
+    double *d = nullptr, *dt = nullptr;
+    HIPCHK(hipMalloc(&d, cnt * (px ? 2 : 1) * sizeof(double)));
+    hipError_t e = hipMalloc(&dt, (size_t)nsig * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpyAsync(d, x, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && px)
+        e = hipMemcpyAsync(d + cnt, px, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        rc = st_fine_tune_device(c, d, px ? d + cnt : nullptr, n, 1, n, nsig, tune_lower, tune_upper, dt);
+        if (rc == ST_OK) e = hipMemcpyAsync(tune, dt, (size_t)nsig * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (rc == ST_OK && e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    hipFree(d);
+    if (dt) hipFree(dt);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(ST_ERR_DEVICE, "st_fine_tune: %s", hipGetErrorString(e));
     return ST_OK;
 }
