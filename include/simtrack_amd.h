@@ -277,6 +277,39 @@ int st_da_uniform(st_table *t, int nturn, double deltap0, double angle, double s
                   double sigma0, double dsigma, int n, int radial, double *sigma_out, int *stable_out,
                   int *n_scanned);
 
+/* ------------------------------------------ many lattices, one launch */
+
+/*
+ * A batch of tables (lattice seeds of uneven element counts) tracked by one
+ * persistent launch per turn block: work items of up to 64 particles of one
+ * table are claimed from a global counter, longest table first; survivors
+ * are regrouped between turn blocks.  Results equal st_track on each table
+ * separately.  All tables must share the context and the globals.  The
+ * reference has no counterpart: its drivers call Track per particle and
+ * per seed (simtrack.cpp:20791, Example2/DA_cal.cpp:81).
+ */
+typedef struct st_batch st_batch;
+typedef struct st_batch_config {
+    int wg_waves;        /* wavefronts per workgroup: 1, 2 or 4                 */
+    int waves_per_simd;  /* resident wavefronts per SIMD of the persistent grid */
+    int turn_block;      /* turns between regroupings of the survivors          */
+} st_batch_config;
+
+int st_batch_create(st_ctx *ctx, st_table *const *tables, int n_tables, st_batch **out);
+int st_batch_destroy(st_batch *b);
+int st_batch_default_config(st_batch_config *cfg);
+/* particles of table k are x rows offsets[k] .. offsets[k+1]-1 (n_tables+1
+   offsets, offsets[0] = 0); flags: ST_TRACK_SPIN; cfg null: defaults */
+int st_batch_track(st_batch *b, double *x, const int *offsets, int nturn, int *stable, int *lost_turn,
+                   int *lost_post, int flags, const st_batch_config *cfg);
+int st_batch_track_device(st_batch *b, double *d_x, const int *offsets, int turn0, int nturn, int *d_state,
+                          int *d_lost, int flags, const st_batch_config *cfg);
+/* times every candidate configuration on calib_turns turns of x (host,
+   fresh particles) and returns the fastest; seconds_out (room for 32) and
+   n_candidates may be null */
+int st_batch_autotune(st_batch *b, const double *x, const int *offsets, int calib_turns, int flags,
+                      st_batch_config *best, double *seconds_out, int *n_candidates);
+
 /* ------------------------------------------------------ tune finder */
 
 /*

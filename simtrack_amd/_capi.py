@@ -63,8 +63,18 @@ EXPORTS = [
     "st_table_size", "st_track", "st_track_spin", "st_track_tbt", "st_track_radial",
     "st_element_pass", "st_track_device", "st_ctx_sync", "st_last_gpu_seconds",
     "st_ctx_set_timing", "st_ctx_kernel_stats", "st_da_angles", "st_da_bisect", "st_da_uniform",
-    "st_fine_tune", "st_fine_tune_device",
+    "st_fine_tune", "st_fine_tune_device", "st_batch_create", "st_batch_destroy", "st_batch_default_config",
+    "st_batch_track", "st_batch_track_device", "st_batch_autotune",
 ]
+
+
+class BatchConfig(ctypes.Structure):
+    """st_batch_config: launch shape of the persistent multi-table kernel."""
+    _fields_ = [("wg_waves", ctypes.c_int), ("waves_per_simd", ctypes.c_int), ("turn_block", ctypes.c_int)]
+
+    def __repr__(self):
+        return f"BatchConfig(wg_waves={self.wg_waves}, waves_per_simd={self.waves_per_simd}, " \
+               f"turn_block={self.turn_block})"
 ST_DA_TRACK_DA, ST_DA_TRACK_DA1, ST_DA_TRACK_DA_RADIAL = 0, 1, 2
 
 _lib = None
@@ -109,6 +119,13 @@ def load(path=None):
     lib.st_da_uniform.argtypes = [P, I, D, D, D, D, D, D, I, I, P, P, ctypes.POINTER(I)]
     lib.st_fine_tune.argtypes = [P, P, P, I, I, D, D, P]
     lib.st_fine_tune_device.argtypes = [P, P, P, ctypes.c_int64, ctypes.c_int64, I, I, D, D, P]
+    BC = ctypes.POINTER(BatchConfig)
+    lib.st_batch_create.argtypes = [P, P, I, ctypes.POINTER(P)]
+    lib.st_batch_destroy.argtypes = [P]
+    lib.st_batch_default_config.argtypes = [BC]
+    lib.st_batch_track.argtypes = [P, P, P, I, P, P, P, I, BC]
+    lib.st_batch_track_device.argtypes = [P, P, P, I, I, P, P, I, BC]
+    lib.st_batch_autotune.argtypes = [P, P, P, I, I, BC, P, ctypes.POINTER(I)]
     es, gs = I(), I()
     lib.st_struct_sizes(ctypes.byref(es), ctypes.byref(gs))
     if es.value != ELEMENT_DTYPE.itemsize or gs.value != ctypes.sizeof(Globals):
@@ -239,3 +256,68 @@ class Table:
         check(self.lib.st_track_device(self.h, ctypes.c_void_p(d_x), int(npart), int(turn0), int(nturn),
                                        ctypes.c_void_p(d_state), ctypes.c_void_p(d_lost), int(flags)),
               "st_track_device")
+
+
+class Batch:
+    """Tables of several lattices (seeds) tracked together by one persistent
+    launch per turn block (st_batch_*)."""
+
+    def __init__(self, ctx, tables):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.tables = list(tables)          # keeps the tables alive
+        arr = (ctypes.c_void_p * len(self.tables))(*[t.h.value for t in self.tables])
+        self.h = ctypes.c_void_p()
+        check(self.lib.st_batch_create(ctx.h, arr, len(self.tables), ctypes.byref(self.h)), "st_batch_create")
+
+    def close(self):
+        if self.h:
+            self.lib.st_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def default_config():
+        cfg = BatchConfig()
+        load().st_batch_default_config(ctypes.byref(cfg))
+        return cfg
+
+    def track(self, x, offsets, nturn, spin=False, cfg=None, stable=None, lost_turn=None, lost_post=None):
+        """Track rows offsets[k]..offsets[k+1]-1 of x on table k; arrays updated in place."""
+        ncol = 9 if spin else 6
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert x.ndim == 2 and x.shape[1] == ncol
+        offs = np.ascontiguousarray(offsets, dtype=np.int32)
+        assert len(offs) == len(self.tables) + 1 and offs[-1] == x.shape[0]
+        n = x.shape[0]
+        stable = np.ones(n, np.int32) if stable is None else np.ascontiguousarray(stable, np.int32)
+        lost_turn = np.zeros(n, np.int32) if lost_turn is None else np.ascontiguousarray(lost_turn, np.int32)
+        lost_post = np.zeros(n, np.int32) if lost_post is None else np.ascontiguousarray(lost_post, np.int32)
+        check(self.lib.st_batch_track(self.h, _ptr(x), _ptr(offs), int(nturn), _ptr(stable), _ptr(lost_turn),
+                                      _ptr(lost_post), TRACK_SPIN if spin else 0,
+                                      ctypes.byref(cfg) if cfg is not None else None), "st_batch_track")
+        return x, stable, lost_turn, lost_post
+
+    def track_device(self, d_x, offsets, turn0, nturn, d_state, d_lost, flags=0, cfg=None):
+        offs = np.ascontiguousarray(offsets, dtype=np.int32)
+        check(self.lib.st_batch_track_device(self.h, ctypes.c_void_p(d_x), _ptr(offs), int(turn0), int(nturn),
+                                             ctypes.c_void_p(d_state), ctypes.c_void_p(d_lost), int(flags),
+                                             ctypes.byref(cfg) if cfg is not None else None),
+              "st_batch_track_device")
+
+    def autotune(self, x, offsets, calib_turns, spin=False):
+        """(best BatchConfig, [(config, seconds), ...]) over the candidate shapes."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        offs = np.ascontiguousarray(offsets, dtype=np.int32)
+        best = BatchConfig()
+        secs = np.zeros(32)
+        n = ctypes.c_int()
+        check(self.lib.st_batch_autotune(self.h, _ptr(x), _ptr(offs), int(calib_turns), TRACK_SPIN if spin else 0,
+                                         ctypes.byref(best), _ptr(secs), ctypes.byref(n)), "st_batch_autotune")
+        cands = [(w, o, t) for w in (1, 2, 4) for o in (1, 2) for t in (16, 64, 256)]
+        return best, list(zip(cands, secs[:n.value]))

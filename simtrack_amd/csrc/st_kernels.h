@@ -31,7 +31,34 @@ struct st_track_args {
     st_kglobals g;
 };
 
+/* one table (lattice seed) of a multi-table launch */
+struct st_seed_desc {
+    const st_op *ops;
+    const double *par;
+    const double *ap;
+    int nelem;
+    int reserved;
+    int64_t draws_per_turn;
+};
+
+/* arguments of the persistent multi-table launch */
+struct st_batch_args {
+    const st_seed_desc *seeds;
+    const int *items;       /* 3 ints per item: table, start in idx, count (<= 64) */
+    int nitems;
+    int *counter;           /* next item to claim (zero at launch)            */
+    const int *idx;         /* particle indices                               */
+    double *x;
+    int *state;
+    int *lost;
+    int turn0, nturn;
+    int precheck;
+    st_kglobals g;
+};
+
 hipError_t st_launch_track(const st_track_args &a, bool spin, bool hexp, bool rad, hipStream_t stream);
+hipError_t st_launch_track_batch(const st_batch_args &b, int grid, int block, bool spin, bool hexp, bool rad,
+                                 hipStream_t stream);
 hipError_t st_launch_radial_x04(const st_track_args &a, double *x04, bool hexp, hipStream_t stream);
 hipError_t st_launch_compact(const int *idx, int n, const int *state, int *block_buf, int *idx_out, int *d_total,
                              hipStream_t stream);

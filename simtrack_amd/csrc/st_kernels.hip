@@ -585,13 +585,17 @@ ST_DEV void apply_op(St &s, Ptc &c, const st_op &op, st_pp par, const st_kglobal
 
 /* ------------------------------------------------------ tracking kernel */
 
+/*
+ * The turn loop of one wavefront: particle p (lane active if `in`) tracked
+ * through turns [a.turn0, a.turn0 + a.nturn) of the table (ops, par, ap,
+ * nelem), with Track's pre-check, loss test and loss records.  Used by the
+ * one-table kernel and by the persistent multi-table kernel.
+ */
 template <bool SPIN, bool HEXP, bool RAD>
-__global__ __launch_bounds__(256) void k_track(st_track_args a)
+ST_DEV void track_turns(const st_track_args &a, const st_opp ops, const st_pp par, const st_pp ap, const st_pp x04,
+                        const int nelem, const int64_t draws_per_turn, const int p, const bool in)
 {
     constexpr int NC = SPIN ? 9 : 6;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in = t < a.nact;
-    const int p = in ? (a.idx ? a.idx[t] : t) : 0;
     St s;
 #pragma unroll
     for (int k = 0; k < 9; k++) s.x[k] = 0.;
@@ -614,22 +618,18 @@ __global__ __launch_bounds__(256) void k_track(st_track_args a)
     uint64_t rctr = 0;
     Ptc c;
     ptc_update(c, s.x[5], a.g);
-    const st_opp ops = (st_opp)a.ops;
-    const st_pp par = (st_pp)a.par;
-    const st_pp ap = (st_pp)a.ap;
-    const st_pp x04 = (st_pp)a.x04;
     int lt = 0, lp = 0;
     bool died = false;
     const int tend = a.turn0 + a.nturn;
     for (int turn = a.turn0; turn < tend; ++turn) {
         if (!__any(alive)) break;
-        if (RAD) rctr = (uint64_t)turn * (uint64_t)a.draws_per_turn;
+        if (RAD) rctr = (uint64_t)turn * (uint64_t)draws_per_turn;
         if (a.xtbt && a.tbt_bpm < 0 && alive) {
             double *o = a.xtbt + ((size_t)p * a.tbt_nturn + (size_t)(turn - a.tbt_turn0)) * NC;
 #pragma unroll
             for (int k = 0; k < NC; k++) o[k] = s.x[k];
         }
-        for (int e = 0; e < a.nelem; ++e) {
+        for (int e = 0; e < nelem; ++e) {
             const st_op op = ops[e];
             if (alive) {
                 apply_op<SPIN, HEXP, RAD>(s, c, op, par, a.g, (uint32_t)p, rctr);
@@ -641,7 +641,7 @@ __global__ __launch_bounds__(256) void k_track(st_track_args a)
                 }
                 /* loss test of Track, simtrack.cpp:20802 (none for a bare Element::Pass) */
                 if (ap && (__builtin_fabs(s.x[0]) > ap[2 * e] || __builtin_fabs(s.x[2]) > ap[2 * e + 1] ||
-                             s.x[0] != s.x[0] || s.x[2] != s.x[2])) {
+                           s.x[0] != s.x[0] || s.x[2] != s.x[2])) {
                     alive = false;
                     died = true;
                     lt = turn;
@@ -659,6 +659,53 @@ __global__ __launch_bounds__(256) void k_track(st_track_args a)
             a.lost[2 * p] = lt;
             a.lost[2 * p + 1] = lp;
         }
+    }
+}
+
+template <bool SPIN, bool HEXP, bool RAD>
+__global__ __launch_bounds__(256) void k_track(st_track_args a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = t < a.nact;
+    const int p = in ? (a.idx ? a.idx[t] : t) : 0;
+    track_turns<SPIN, HEXP, RAD>(a, (st_opp)a.ops, (st_pp)a.par, (st_pp)a.ap, (st_pp)a.x04, a.nelem,
+                                 a.draws_per_turn, p, in);
+}
+
+/*
+ * Persistent tracking of many tables (lattice seeds) in one launch.  Work
+ * items are (table, up to 64 particles); every wavefront takes the next item
+ * from a global counter until none is left, so each wavefront always exits
+ * after at most nitems claims.  The host orders items longest table first.
+ */
+template <bool SPIN, bool HEXP, bool RAD>
+__global__ __launch_bounds__(256) void k_track_batch(st_batch_args b)
+{
+    const int lane = threadIdx.x & 63;
+    st_track_args a;
+    a.x = b.x;
+    a.state = b.state;
+    a.lost = b.lost;
+    a.turn0 = b.turn0;
+    a.nturn = b.nturn;
+    a.precheck = b.precheck;
+    a.xtbt = nullptr;
+    a.tbt_bpm = -1;
+    a.tbt_nturn = 0;
+    a.tbt_turn0 = 0;
+    a.g = b.g;
+    const ST_KCONST st_seed_desc *seeds = (const ST_KCONST st_seed_desc *)b.seeds;
+    const ST_KCONST int *items = (const ST_KCONST int *)b.items;
+    for (;;) {
+        int item = 0;
+        if (lane == 0) item = atomicAdd(b.counter, 1);
+        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0));
+        if (item >= b.nitems) break;
+        const int sd = items[3 * item], start = items[3 * item + 1], cnt = items[3 * item + 2];
+        const bool in = lane < cnt;
+        const int p = in ? b.idx[start + lane] : 0;
+        track_turns<SPIN, HEXP, RAD>(a, (st_opp)seeds[sd].ops, (st_pp)seeds[sd].par, (st_pp)seeds[sd].ap, nullptr,
+                                     seeds[sd].nelem, seeds[sd].draws_per_turn, p, in);
     }
 }
 
@@ -783,6 +830,24 @@ hipError_t st_launch_radial_x04(const st_track_args &a, double *x04, bool hexp, 
         hipLaunchKernelGGL((k_radial_x04<true>), dim3(1), dim3(64), 0, stream, a, x04);
     else
         hipLaunchKernelGGL((k_radial_x04<false>), dim3(1), dim3(64), 0, stream, a, x04);
+    return hipGetLastError();
+}
+
+#define ST_LAUNCH_B(S, H, R)                                                                                   \
+    hipLaunchKernelGGL((k_track_batch<S, H, R>), dim3(grid), dim3(block), 0, stream, b)
+
+hipError_t st_launch_track_batch(const st_batch_args &b, int grid, int block, bool spin, bool hexp, bool rad,
+                                 hipStream_t stream)
+{
+    if (b.nitems <= 0 || grid <= 0) return hipSuccess;
+    if (!spin && !hexp && !rad) ST_LAUNCH_B(false, false, false);
+    else if (!spin && !hexp && rad) ST_LAUNCH_B(false, false, true);
+    else if (!spin && hexp && !rad) ST_LAUNCH_B(false, true, false);
+    else if (!spin && hexp && rad) ST_LAUNCH_B(false, true, true);
+    else if (spin && !hexp && !rad) ST_LAUNCH_B(true, false, false);
+    else if (spin && !hexp && rad) ST_LAUNCH_B(true, false, true);
+    else if (spin && hexp && !rad) ST_LAUNCH_B(true, true, false);
+    else ST_LAUNCH_B(true, true, true);
     return hipGetLastError();
 }
 

@@ -353,3 +353,89 @@ def config1(n_particles=65536, seed=2027, lattice_seed=2026, nslice=5, sigma_z=0
     x[:, 5] = tg(n_particles, sigma_pt)
     x[:, 7] = 1.0          # spin along the vertical guide field
     return gp, ln, x
+
+
+# ------------------------------------------------ config 2: many seeds
+
+def cc_voltage(theta, freq_cc, beta_star, beta_cc, energy_mev):
+    """Cal_CC_Voltage, simtrack.cpp:13570-13577: crab voltage [MV] for half
+    crossing angle theta with ideal 90-degree phase advance."""
+    c = 2.99792458e8
+    energy = energy_mev * 1e6
+    return c * energy * (2 * theta) / math.sqrt(beta_star * beta_cc) / 4 / 3.14159265358979323 / freq_cc / 1e6
+
+
+def arc_beta():
+    """Periodic horizontal beta at the arc-cell boundary (linear model)."""
+    mx, _ = _product(_arc_cell_linear())
+    return _periodic(mx)[0]
+
+
+def crab_ring(lattice_seed, n_elements, gp, theta=0.0125, f_cc=197e6, nslice=5, sigma_z=0.3, rng_seed=None):
+    """One config-2 lattice: the DA ring (errors from lattice_seed) with a
+    crossing angle at IP1 (LBT, 6-D beam-beam, ILBT, as Example2/DA_cal.cpp:
+    777-805), crab cavities at 197 MHz on both sides of IP1 at the arc ends
+    of its insertion, head-on 6-D beam-beam at IP2, and zero-length extra
+    correctors (KICKER) and thin multipole-error slices (MULT, b3..b6)
+    inserted at random positions until the line has n_elements."""
+    emit, sig = beam_sizes(gp)
+    npb = bb_intensity(gp)
+    ip1 = [L.LBT("LBT1", theta),
+           L.BEAMBEAM("BB1", 6, npb, 1.0, sigma_z, nslice, emit, BETA_STAR, 0., emit, BETA_STAR, 0.),
+           L.ILBT("ILBT1", theta)]
+    ip2 = [L.BEAMBEAM("BB2", 6, npb, 1.0, sigma_z, nslice, emit, BETA_STAR, 0., emit, BETA_STAR, 0.)]
+    ln = da_ring(seed=lattice_seed, ip_elements=(ip1, ip2))
+    set_rf_frequency(ln, gp)
+    v = cc_voltage(theta, f_cc, BETA_STAR, arc_beta(), gp.energy)
+    cells = list(ln.Cell)
+    # the first arc cell after IP1's insertion, and the last before it
+    i_out = [i for i, e in enumerate(cells) if e.NAME == "QFH0a"][0]
+    i_in = [i for i, e in enumerate(cells) if e.NAME == "QFH47b"][0] + 1
+    # the insertion turns the phase by ~pi between the two cavities, so equal
+    # voltages close the crab bump (opposite ones double it and destroy the DA)
+    cells.insert(i_in, L.CRABRF("CC1L", 0., v, f_cc, 0.))
+    cells.insert(i_out, L.CRABRF("CC1R", 0., v, f_cc, 0.))
+    r = _rng(lattice_seed if rng_seed is None else rng_seed)
+    n_extra = max(0, n_elements - len(cells))
+    lo = i_out + 1
+    for k in range(n_extra):
+        pos = int(r.integers(lo, len(cells) - 60))
+        if k % 2 == 0:
+            e = L.KICKER(f"XC{k}", 0., float(r.normal(0, 2e-7)), float(r.normal(0, 2e-7)))
+        else:
+            kn = [0.] * 11
+            ks = [0.] * 11
+            for j in range(2, 6):
+                kn[j] = float(r.normal(0, 1e-4)) * math.factorial(j) / 0.04 ** (j - 1) * 1e-2
+                ks[j] = float(r.normal(0, 1e-4)) * math.factorial(j) / 0.04 ** (j - 1) * 1e-2
+            e = L.MULT(f"XM{k}", 0., kn, ks)
+        cells.insert(pos, e)
+    for e in cells:
+        e.APx = e.APy = 0.5
+    return L.Line(cells)
+
+
+def config2(n_seeds=64, n_per_seed=4096, seed=2028, min_elements=1000, max_elements=1800, n_angles=8,
+            amax_sigma=20., pt=3e-4):
+    """Config 2: n_seeds lattices of uneven length (uniform in [min, max]
+    elements), each with its own n_per_seed DA-grid particles (n_angles
+    angles in (0, 90) degrees x amplitudes up to amax_sigma).  Returns
+    (globals, [Line], x with all seeds' particles stacked, offsets)."""
+    gp = proton_globals(100000.0)
+    r = _rng(seed)
+    sizes = r.integers(min_elements, max_elements + 1, n_seeds)
+    lines = [crab_ring(2026 + s, int(sizes[s]), gp) for s in range(n_seeds)]
+    _, sig = beam_sizes(gp)
+    n_amp = n_per_seed // n_angles
+    amps = (np.arange(n_amp) + 1) * amax_sigma / n_amp
+    angles = (np.arange(n_angles) + 1) * 90. / (n_angles + 1)
+    one = np.zeros((n_angles * n_amp, 6))
+    k = 0
+    for t in angles:
+        th = math.radians(t)
+        for a in amps:
+            one[k] = (a * math.cos(th) * sig, 0., a * math.sin(th) * sig, 0., 0., pt)
+            k += 1
+    x = np.concatenate([one] * n_seeds)
+    offsets = np.arange(n_seeds + 1, dtype=np.int32) * len(one)
+    return gp, lines, x, offsets
