@@ -56,11 +56,18 @@ def main():
     ap.add_argument("--particles", type=int, default=65536)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=str, default="1024x64", help="particles x turns for the CPU leg")
+    ap.add_argument("--workload", choices=("config1", "config2"), default="config1",
+                    help="config1: the headline workload; config2: 64 uneven lattices in one persistent launch")
+    ap.add_argument("--seeds", type=int, default=64, help="config2: lattices")
+    ap.add_argument("--per-seed", type=int, default=4096, help="config2: particles per lattice")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the tracking "
               f"path shards by particles or seeds with no collective, see the sharding notes", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "config2":
+        return bench_config2(args)
 
     import numpy as np
     import torch
@@ -150,6 +157,94 @@ def main():
                      "f_turn_flops": f_turn, "kernel": "k_track<spin>",
                      "avg_launch_ms": 1e3 * k_sec / max(k_launches, 1), "launches": k_launches},
         "cpu_baseline": baseline,
+    }
+    print(json.dumps(out))
+
+
+def bench_config2(args):
+    """64 lattices of 1,000-1,800 elements (crab crossing + 6-D beam-beam),
+    4,096 DA-grid particles each, tracked by the persistent multi-lattice
+    launch with the autotuned shape; one launch per lattice (the plain
+    per-table path) timed beside it on the same turns."""
+    import numpy as np
+    import torch
+
+    import simtrack_amd as st
+    from simtrack_amd import flops
+    from simtrack_amd import workloads as W
+
+    gp, lines, x, offs = W.config2(n_seeds=args.seeds, n_per_seed=args.per_seed)
+    ctx = st.Context(0)
+    tabs = [st.Table(ctx, gp, ln.records()) for ln in lines]
+    f_seed = np.array([flops.flops_per_turn(ln.records(), gp) for ln in lines], dtype=float)
+    batch = st.Batch(ctx, tabs)
+    best, table = batch.autotune(x, offs, 32)
+    dev = torch.device("cuda:0")
+    T = args.turns_per_step
+
+    def fresh():
+        d_x = torch.tensor(x, dtype=torch.float64, device=dev).contiguous()
+        d_s = torch.ones(len(x), dtype=torch.int32, device=dev)
+        d_l = torch.zeros(2 * len(x), dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+        return d_x, d_s, d_l
+
+    def run(batched, d_x, d_s, d_l, turn0, nsteps):
+        t = turn0
+        for _ in range(nsteps):
+            if batched:
+                batch.track_device(d_x.data_ptr(), offs, t, T, d_s.data_ptr(), d_l.data_ptr(), 0, best)
+            else:
+                for k, tb in enumerate(tabs):
+                    o = int(offs[k])
+                    tb.track_device(d_x.data_ptr() + 8 * 6 * o, int(offs[k + 1] - o), t, T, d_s.data_ptr() + 4 * o,
+                                    d_l.data_ptr() + 8 * o, 0)
+            t += T
+        ctx.sync()
+        torch.cuda.synchronize()
+        return t
+
+    results = {}
+    for mode in ("batched", "per_lattice"):
+        d_x, d_s, d_l = fresh()
+        t = run(mode == "batched", d_x, d_s, d_l, 0, args.warmup)
+        s0 = d_s.cpu().numpy().copy()
+        ctx.set_timing(True)
+        ctx.kernel_stats()
+        t0 = time.perf_counter()
+        t_end = run(mode == "batched", d_x, d_s, d_l, t, args.steps)
+        wall = time.perf_counter() - t0
+        k_sec, k_launch, _ = ctx.kernel_stats()
+        ctx.set_timing(False)
+        s1 = d_s.cpu().numpy()
+        lost = d_l.cpu().numpy().reshape(-1, 2)
+        per = np.where(s1 != 0, t_end - t, np.clip(lost[:, 0] - t, 0, t_end - t))
+        per = np.where(s0 == 0, 0, per)
+        flops_done = sum(float(per[offs[k]:offs[k + 1]].sum()) * f_seed[k] for k in range(len(tabs)))
+        results[mode] = dict(pturns=int(per.sum()), wall=wall, k_sec=k_sec, launches=k_launch, flops=flops_done)
+    r = results["batched"]
+    value = r["pturns"] / r["wall"]
+    achieved = r["flops"] / r["k_sec"] if r["k_sec"] > 0 else float("nan")
+    peak = flops.PEAK_FP64_VECTOR
+    out = {
+        "metric": "FP64 particle-turns/s", "value": value, "unit": "particle-turns/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * r["wall"] / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded lattice errors, extra elements and DA grids)",
+        "config": {"workload": f"config2: {len(tabs)} lattices of {min(len(l) for l in lines)}-"
+                               f"{max(len(l) for l in lines)} elements, crab crossing + 6-D beam-beam, "
+                               f"{args.per_seed} DA-grid particles each", "turns_per_step": T,
+                   "autotuned": {"wg_waves": best.wg_waves, "waves_per_simd": best.waves_per_simd,
+                                 "turn_block": best.turn_block}},
+        "roofline": {"bound": "mfma", "pipe": "fp64 VALU", "achieved": achieved / 1e12, "peak": peak / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None, "kernel": "k_track_batch",
+                     "launches": r["launches"]},
+        "per_lattice_launches": {"value": results["per_lattice"]["pturns"] / results["per_lattice"]["wall"],
+                                 "launches": results["per_lattice"]["launches"],
+                                 "speedup_of_batched": (value / (results["per_lattice"]["pturns"] /
+                                                                 results["per_lattice"]["wall"]))},
+        "autotune_seconds": {f"{c[0]}x{c[1]}x{c[2]}": round(t_, 5) for c, t_ in table},
+        "cpu_baseline": None,
     }
     print(json.dumps(out))
 
