@@ -310,6 +310,32 @@ int st_batch_track_device(st_batch *b, double *d_x, const int *offsets, int turn
 int st_batch_autotune(st_batch *b, const double *x, const int *offsets, int calib_turns, int flags,
                       st_batch_config *best, double *seconds_out, int *n_candidates);
 
+/* ------------------------------------------------ survival surrogate */
+
+/*
+ * A multilayer perceptron n_in -> H -> H -> H -> 1 (ReLU hidden layers,
+ * sigmoid output) predicting long-term survival from features the tracker
+ * generates.  BF16 weights and activations on the matrix cores with FP32
+ * accumulation; FP32 master weights, binary cross-entropy loss and Adam
+ * (beta1 0.9, beta2 0.999, eps 1e-8).  No counterpart in the reference.
+ * Parameters are flat FP32: W0 (H x n_in), W1, W2 (H x H), W3 (1 x H), then
+ * b0, b1, b2 (H), b3 (1); weights row-major (out x in).  d_* arguments are
+ * device pointers; calls are ordered on the context stream.
+ */
+typedef struct st_mlp st_mlp;
+int st_mlp_create(st_ctx *ctx, int n_in, int n_hidden, uint64_t seed, st_mlp **out);
+int st_mlp_destroy(st_mlp *m);
+int st_mlp_num_params(const st_mlp *m, int64_t *n);
+int st_mlp_get_params(st_mlp *m, float *params, int64_t n);
+int st_mlp_set_params(st_mlp *m, const float *params, int64_t n);
+/* probabilities of survival for n feature rows (n x n_in FP32) */
+int st_mlp_forward(st_mlp *m, const float *d_x, int n, float *d_prob);
+/* loss and gradients (host, same layout as the parameters) of a batch with
+   labels d_y (1 = survived), without updating */
+int st_mlp_gradients(st_mlp *m, const float *d_x, const float *d_y, int n, float *grads, float *loss);
+/* one Adam step on a batch; *loss (may be null) gets the batch's mean loss */
+int st_mlp_train_step(st_mlp *m, const float *d_x, const float *d_y, int n, float lr, float *loss);
+
 /* ------------------------------------------------------ tune finder */
 
 /*

@@ -64,7 +64,9 @@ EXPORTS = [
     "st_element_pass", "st_track_device", "st_ctx_sync", "st_last_gpu_seconds",
     "st_ctx_set_timing", "st_ctx_kernel_stats", "st_da_angles", "st_da_bisect", "st_da_uniform",
     "st_fine_tune", "st_fine_tune_device", "st_batch_create", "st_batch_destroy", "st_batch_default_config",
-    "st_batch_track", "st_batch_track_device", "st_batch_autotune",
+    "st_batch_track", "st_batch_track_device", "st_batch_autotune", "st_mlp_create", "st_mlp_destroy",
+    "st_mlp_num_params", "st_mlp_get_params", "st_mlp_set_params", "st_mlp_forward", "st_mlp_gradients",
+    "st_mlp_train_step",
 ]
 
 
@@ -126,6 +128,15 @@ def load(path=None):
     lib.st_batch_track.argtypes = [P, P, P, I, P, P, P, I, BC]
     lib.st_batch_track_device.argtypes = [P, P, P, I, I, P, P, I, BC]
     lib.st_batch_autotune.argtypes = [P, P, P, I, I, BC, P, ctypes.POINTER(I)]
+    F = ctypes.c_float
+    lib.st_mlp_create.argtypes = [P, I, I, ctypes.c_uint64, ctypes.POINTER(P)]
+    lib.st_mlp_destroy.argtypes = [P]
+    lib.st_mlp_num_params.argtypes = [P, ctypes.POINTER(ctypes.c_int64)]
+    lib.st_mlp_get_params.argtypes = [P, P, ctypes.c_int64]
+    lib.st_mlp_set_params.argtypes = [P, P, ctypes.c_int64]
+    lib.st_mlp_forward.argtypes = [P, P, I, P]
+    lib.st_mlp_gradients.argtypes = [P, P, P, I, P, ctypes.POINTER(F)]
+    lib.st_mlp_train_step.argtypes = [P, P, P, I, F, ctypes.POINTER(F)]
     es, gs = I(), I()
     lib.st_struct_sizes(ctypes.byref(es), ctypes.byref(gs))
     if es.value != ELEMENT_DTYPE.itemsize or gs.value != ctypes.sizeof(Globals):
@@ -321,3 +332,69 @@ class Batch:
                                          ctypes.byref(best), _ptr(secs), ctypes.byref(n)), "st_batch_autotune")
         cands = [(w, o, t) for w in (1, 2, 4) for o in (1, 2) for t in (16, 64, 256)]
         return best, list(zip(cands, secs[:n.value]))
+
+
+class Mlp:
+    """The survival surrogate network (st_mlp_*): n_in -> H -> H -> H -> 1.
+    Device arguments are integer addresses (e.g. torch tensor data_ptr())."""
+
+    def __init__(self, ctx, n_in, n_hidden=256, seed=1):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.n_in, self.n_hidden = n_in, n_hidden
+        self.h = ctypes.c_void_p()
+        check(self.lib.st_mlp_create(ctx.h, n_in, n_hidden, seed, ctypes.byref(self.h)), "st_mlp_create")
+        n = ctypes.c_int64()
+        check(self.lib.st_mlp_num_params(self.h, ctypes.byref(n)), "st_mlp_num_params")
+        self.nparam = n.value
+
+    def close(self):
+        if self.h:
+            self.lib.st_mlp_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(self.nparam, np.float32)
+        check(self.lib.st_mlp_get_params(self.h, _ptr(p), self.nparam), "st_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparam
+        check(self.lib.st_mlp_set_params(self.h, _ptr(p), self.nparam), "st_mlp_set_params")
+
+    def unpack(self, p):
+        """[(W, b)] per layer from a flat parameter vector."""
+        H, n = self.n_hidden, self.n_in
+        shapes = [(H, n), (H, H), (H, H), (1, H)]
+        out, o = [], 0
+        Ws = []
+        for sh in shapes:
+            Ws.append(p[o:o + sh[0] * sh[1]].reshape(sh))
+            o += sh[0] * sh[1]
+        for sh, W in zip(shapes, Ws):
+            out.append((W, p[o:o + sh[0]]))
+            o += sh[0]
+        return out
+
+    def forward(self, d_x, n, d_prob):
+        check(self.lib.st_mlp_forward(self.h, ctypes.c_void_p(d_x), int(n), ctypes.c_void_p(d_prob)), "st_mlp_forward")
+
+    def gradients(self, d_x, d_y, n):
+        g = np.zeros(self.nparam, np.float32)
+        loss = ctypes.c_float()
+        check(self.lib.st_mlp_gradients(self.h, ctypes.c_void_p(d_x), ctypes.c_void_p(d_y), int(n), _ptr(g),
+                                        ctypes.byref(loss)), "st_mlp_gradients")
+        return g, loss.value
+
+    def train_step(self, d_x, d_y, n, lr):
+        loss = ctypes.c_float()
+        check(self.lib.st_mlp_train_step(self.h, ctypes.c_void_p(d_x), ctypes.c_void_p(d_y), int(n), float(lr),
+                                         ctypes.byref(loss)), "st_mlp_train_step")
+        return loss.value

@@ -57,6 +57,10 @@ extern "C" __attribute__((visibility("hidden"))) int st_internal_fail(int code, 
 {
     return fail(code, "%s", msg);
 }
+
+struct st_ctx;
+/* the context's stream and device, for the library's other translation units */
+extern "C" __attribute__((visibility("hidden"))) hipStream_t st_internal_stream(st_ctx *c, int *device);
 extern "C" int st_abi_version(void) { return ST_ABI_VERSION; }
 extern "C" void st_struct_sizes(int *element_bytes, int *globals_bytes)
 {
@@ -1746,4 +1750,10 @@ extern "C" int st_batch_autotune(st_batch *b, const double *x, const int *offset
             }
     if (n_candidates) *n_candidates = k;
     return ST_OK;
+}
+
+extern "C" __attribute__((visibility("hidden"))) hipStream_t st_internal_stream(st_ctx *c, int *device)
+{
+    if (device) *device = c ? c->device : 0;
+    return c ? c->stream : nullptr;
 }
