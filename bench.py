@@ -56,10 +56,12 @@ def main():
     ap.add_argument("--particles", type=int, default=65536)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=str, default="1024x64", help="particles x turns for the CPU leg")
-    ap.add_argument("--workload", choices=("config1", "config2"), default="config1",
+    ap.add_argument("--workload", choices=("config1", "config2", "config3"), default="config1",
                     help="config1: the headline workload; config2: 64 uneven lattices in one persistent launch")
     ap.add_argument("--seeds", type=int, default=64, help="config2: lattices")
-    ap.add_argument("--per-seed", type=int, default=4096, help="config2: particles per lattice")
+    ap.add_argument("--per-seed", type=int, default=4096, help="config2/3: particles per lattice")
+    ap.add_argument("--train-seeds", type=int, default=16, help="config3: lattices for the training set")
+    ap.add_argument("--n-long", type=int, default=30000, help="config3: survival horizon in turns")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the tracking "
@@ -68,6 +70,8 @@ def main():
 
     if args.workload == "config2":
         return bench_config2(args)
+    if args.workload == "config3":
+        return bench_config3(args)
 
     import numpy as np
     import torch
@@ -245,6 +249,73 @@ def bench_config2(args):
                                                                  results["per_lattice"]["wall"]))},
         "autotune_seconds": {f"{c[0]}x{c[1]}x{c[2]}": round(t_, 5) for c, t_ in table},
         "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
+def bench_config3(args):
+    """Surrogate + fallback DA scan vs full tracking: the network is trained
+    on lattices the tracker generates, then both scans run on other seeds;
+    DA boundaries compared per angle and lattice."""
+    import numpy as np
+    import torch
+
+    import simtrack_amd as st
+    from simtrack_amd import surrogate as S
+    from simtrack_amd import workloads as W
+
+    ctx = st.Context(0)
+    gp = W.proton_globals(100000.0)
+    twiss = (W.BETA_STAR, 0., W.BETA_STAR, 0.)
+    n_ang = 8
+    _, _, x1, _ = W.config2(n_seeds=1, n_per_seed=args.per_seed, n_angles=n_ang)
+    rng = np.random.default_rng(7)
+
+    def lattices(seeds):
+        lines = [W.crab_ring(s, int(rng.integers(1000, 1801)), gp) for s in seeds]
+        tabs = [st.Table(ctx, gp, ln.records()) for ln in lines]
+        x = np.concatenate([x1] * len(seeds))
+        offs = np.arange(len(seeds) + 1, dtype=np.int32) * len(x1)
+        return tabs, x, offs
+
+    t0 = time.perf_counter()
+    tabs, x, offs = lattices(range(3000, 3000 + args.train_seeds))
+    batch = st.Batch(ctx, tabs)
+    d_x, d_s, d_l, d_f = S.features(ctx, batch, x, offs, *twiss)
+    alive = d_s != 0
+    S.track_to(ctx, tabs, d_x, d_s, d_l, offs, alive, 2 * S.WINDOW, args.n_long)
+    log = []
+    mlp, std = S.train(ctx, d_f[alive], d_s[alive] != 0, epochs=30, batch=2048, log=log)
+    t_train = time.perf_counter() - t0
+    del d_x, d_s, d_l, d_f, batch
+
+    etabs, ex, eoffs = lattices(range(2026, 2026 + args.seeds))
+    ref, t_full, _, _ = S.full(ctx, etabs, ex, eoffs, args.n_long)
+    surv, info = S.scan(ctx, etabs, ex, eoffs, args.n_long, mlp, std, twiss)
+    amps = (np.arange(len(x1) // n_ang) + 1) * 20. / (len(x1) // n_ang)
+    err_net, err_plain = [], []
+    for k in range(len(etabs)):
+        sl = slice(eoffs[k], eoffs[k + 1])
+        d_ref = S.da_boundary(ref[sl], amps, n_ang)
+        err_net.append(np.abs(S.da_boundary(surv[sl], amps, n_ang) - d_ref))
+        err_plain.append(np.abs(S.da_boundary(info["plain"][sl], amps, n_ang) - d_ref))
+    err_net, err_plain = np.array(err_net), np.array(err_plain)
+    speedup = t_full / info["wall"]
+    out = {
+        "metric": "DA-scan wall time, surrogate+fallback vs full track", "value": speedup, "unit": "x (speedup)",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "f64 tracking, bf16/fp32 network", "data": "synthetic",
+        "config": {"workload": f"config3: {args.seeds} evaluation lattices x {len(x1)} DA-grid particles, "
+                               f"survival to {args.n_long} turns; network trained on {args.train_seeds} other "
+                               f"lattices from their first 1,024 turns",
+                   "mlp": "12-256-256-256-1 BF16 MFMA"},
+        "full_track_s": t_full, "surrogate_scan_s": info["wall"], "features_s": info["t_features"],
+        "fallback_fraction": info["n_fallback"] / max(info["n_alive_1k"], 1),
+        "da_error_sigma": {"max": float(err_net.max()), "mean": float(err_net.mean()),
+                           "lattices_within_0.25": int((err_net.max(axis=1) <= 0.25).sum())},
+        "plain_method_da_error_sigma": {"max": float(err_plain.max()), "mean": float(err_plain.mean()),
+                                        "lattices_within_0.25": int((err_plain.max(axis=1) <= 0.25).sum())},
+        "survival_accuracy": {"surrogate": float((surv == ref).mean()), "plain": float((info["plain"] == ref).mean())},
+        "train_s": t_train, "train_loss_first_last": [log[0], log[-1]],
     }
     print(json.dumps(out))
 

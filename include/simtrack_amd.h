@@ -304,6 +304,10 @@ int st_batch_track(st_batch *b, double *x, const int *offsets, int nturn, int *s
                    int *lost_post, int flags, const st_batch_config *cfg);
 int st_batch_track_device(st_batch *b, double *d_x, const int *offsets, int turn0, int nturn, int *d_state,
                           int *d_lost, int flags, const st_batch_config *cfg);
+/* as st_batch_track_device, also storing x at the start of every turn:
+   d_tbt[(p*nturn + (turn - turn0))*ncol + k] (rows after a loss untouched) */
+int st_batch_track_tbt_device(st_batch *b, double *d_x, const int *offsets, int turn0, int nturn, double *d_tbt,
+                              int *d_state, int *d_lost, int flags, const st_batch_config *cfg);
 /* times every candidate configuration on calib_turns turns of x (host,
    fresh particles) and returns the fastest; seconds_out (room for 32) and
    n_candidates may be null */
@@ -335,6 +339,16 @@ int st_mlp_forward(st_mlp *m, const float *d_x, int n, float *d_prob);
 int st_mlp_gradients(st_mlp *m, const float *d_x, const float *d_y, int n, float *grads, float *loss);
 /* one Adam step on a batch; *loss (may be null) gets the batch's mean loss */
 int st_mlp_train_step(st_mlp *m, const float *d_x, const float *d_y, int n, float lr, float *loss);
+
+/*
+ * The surrogate's 12 FP32 features per particle: Courant-Snyder actions
+ * Jx, Jy and angle cos/sin in both planes of the initial coordinates d_x0
+ * (n rows of ncol doubles) with the Twiss parameters of the start point,
+ * the four tunes d_tunes[n][4] = (nux, nuy of turn window 1, nux, nuy of
+ * window 2) and log10 |tune change| per plane.
+ */
+int st_surrogate_features(st_ctx *ctx, const double *d_x0, int ncol, const double *d_tunes, int n, double beta_x,
+                          double alpha_x, double beta_y, double alpha_y, float *d_feat);
 
 /* ------------------------------------------------------ tune finder */
 

@@ -66,7 +66,7 @@ EXPORTS = [
     "st_fine_tune", "st_fine_tune_device", "st_batch_create", "st_batch_destroy", "st_batch_default_config",
     "st_batch_track", "st_batch_track_device", "st_batch_autotune", "st_mlp_create", "st_mlp_destroy",
     "st_mlp_num_params", "st_mlp_get_params", "st_mlp_set_params", "st_mlp_forward", "st_mlp_gradients",
-    "st_mlp_train_step",
+    "st_mlp_train_step", "st_batch_track_tbt_device", "st_surrogate_features",
 ]
 
 
@@ -128,6 +128,8 @@ def load(path=None):
     lib.st_batch_track.argtypes = [P, P, P, I, P, P, P, I, BC]
     lib.st_batch_track_device.argtypes = [P, P, P, I, I, P, P, I, BC]
     lib.st_batch_autotune.argtypes = [P, P, P, I, I, BC, P, ctypes.POINTER(I)]
+    lib.st_batch_track_tbt_device.argtypes = [P, P, P, I, I, P, P, P, I, BC]
+    lib.st_surrogate_features.argtypes = [P, P, I, P, I, D, D, D, D, P]
     F = ctypes.c_float
     lib.st_mlp_create.argtypes = [P, I, I, ctypes.c_uint64, ctypes.POINTER(P)]
     lib.st_mlp_destroy.argtypes = [P]
@@ -185,6 +187,16 @@ class Context:
 
     def set_timing(self, on=True):
         check(self.lib.st_ctx_set_timing(self.h, 1 if on else 0), "st_ctx_set_timing")
+
+    def fine_tune_device(self, d_x, stride_sig, stride, n, nsig, lower, upper, d_tune, d_px=None):
+        check(self.lib.st_fine_tune_device(self.h, ctypes.c_void_p(d_x), ctypes.c_void_p(d_px) if d_px else None,
+                                           int(stride_sig), int(stride), int(n), int(nsig), float(lower), float(upper),
+                                           ctypes.c_void_p(d_tune)), "st_fine_tune_device")
+
+    def surrogate_features(self, d_x0, ncol, d_tunes, n, beta_x, alpha_x, beta_y, alpha_y, d_feat):
+        check(self.lib.st_surrogate_features(self.h, ctypes.c_void_p(d_x0), int(ncol), ctypes.c_void_p(d_tunes), int(n),
+                                             float(beta_x), float(alpha_x), float(beta_y), float(alpha_y),
+                                             ctypes.c_void_p(d_feat)), "st_surrogate_features")
 
     def fine_tune(self, x, lower, upper, px=None):
         """FineTuneFinder (simtrack.cpp:2288, x/px form cpp:2321) of each row of x."""
@@ -320,6 +332,14 @@ class Batch:
                                              ctypes.c_void_p(d_state), ctypes.c_void_p(d_lost), int(flags),
                                              ctypes.byref(cfg) if cfg is not None else None),
               "st_batch_track_device")
+
+    def track_tbt_device(self, d_x, offsets, turn0, nturn, d_tbt, d_state, d_lost, flags=0, cfg=None):
+        offs = np.ascontiguousarray(offsets, dtype=np.int32)
+        check(self.lib.st_batch_track_tbt_device(self.h, ctypes.c_void_p(d_x), _ptr(offs), int(turn0), int(nturn),
+                                                 ctypes.c_void_p(d_tbt), ctypes.c_void_p(d_state),
+                                                 ctypes.c_void_p(d_lost), int(flags),
+                                                 ctypes.byref(cfg) if cfg is not None else None),
+              "st_batch_track_tbt_device")
 
     def autotune(self, x, offsets, calib_turns, spin=False):
         """(best BatchConfig, [(config, seconds), ...]) over the candidate shapes."""

@@ -1553,7 +1553,7 @@ static void build_items(const st_batch *b, const std::vector<int> &state, const 
 }
 
 static int batch_core(st_batch *b, double *d_x, const int *offsets, int turn0, int nturn, int *d_state, int *d_lost,
-                      bool spin, const st_batch_config &cfg)
+                      bool spin, const st_batch_config &cfg, double *d_tbt = nullptr)
 {
     st_ctx *c = b->c;
     const int ntab = (int)b->tabs.size();
@@ -1604,6 +1604,9 @@ static int batch_core(st_batch *b, double *d_x, const int *offsets, int turn0, i
         a.turn0 = t0;
         a.nturn = nt;
         a.precheck = (first && turn0 == 0) ? 1 : 0;
+        a.xtbt = d_tbt;
+        a.tbt_nturn = nturn;
+        a.tbt_turn0 = turn0;
         a.g = b->kg;
         int ev = -1;
         if (c->timing) {
@@ -1658,6 +1661,21 @@ extern "C" int st_batch_track_device(st_batch *b, double *d_x, const int *offset
     st_batch_default_config(&dc);
     HIPCHK(hipSetDevice(b->c->device));
     return batch_core(b, d_x, offsets, turn0, nturn, d_state, d_lost, (flags & ST_TRACK_SPIN) != 0, cfg ? *cfg : dc);
+}
+
+extern "C" int st_batch_track_tbt_device(st_batch *b, double *d_x, const int *offsets, int turn0, int nturn,
+                                         double *d_tbt, int *d_state, int *d_lost, int flags,
+                                         const st_batch_config *cfg)
+{
+    int rc = batch_check(b, offsets);
+    if (rc) return rc;
+    if (!d_x || !d_tbt || !d_state || !d_lost || turn0 < 0 || nturn < 0)
+        return fail(ST_ERR_ARG, "st_batch_track_tbt_device: bad argument");
+    st_batch_config dc;
+    st_batch_default_config(&dc);
+    HIPCHK(hipSetDevice(b->c->device));
+    return batch_core(b, d_x, offsets, turn0, nturn, d_state, d_lost, (flags & ST_TRACK_SPIN) != 0, cfg ? *cfg : dc,
+                      d_tbt);
 }
 
 extern "C" int st_batch_track(st_batch *b, double *x, const int *offsets, int nturn, int *stable, int *lost_turn,
@@ -1756,4 +1774,15 @@ extern "C" __attribute__((visibility("hidden"))) hipStream_t st_internal_stream(
 {
     if (device) *device = c ? c->device : 0;
     return c ? c->stream : nullptr;
+}
+
+extern "C" int st_surrogate_features(st_ctx *c, const double *d_x0, int ncol, const double *d_tunes, int n,
+                                     double beta_x, double alpha_x, double beta_y, double alpha_y, float *d_feat)
+{
+    if (!c || n < 0 || (n > 0 && (!d_x0 || !d_tunes || !d_feat)) || ncol < 4 || !(beta_x > 0.) || !(beta_y > 0.))
+        return fail(ST_ERR_ARG, "st_surrogate_features: bad argument");
+    if (n == 0) return ST_OK;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(st_launch_features(d_x0, ncol, d_tunes, n, beta_x, alpha_x, beta_y, alpha_y, d_feat, c->stream));
+    return ST_OK;
 }

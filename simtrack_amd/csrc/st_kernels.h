@@ -53,6 +53,8 @@ struct st_batch_args {
     int *lost;
     int turn0, nturn;
     int precheck;
+    double *xtbt;           /* turn-by-turn store at turn start, or null      */
+    int tbt_nturn, tbt_turn0;
     st_kglobals g;
 };
 

@@ -689,10 +689,10 @@ __global__ __launch_bounds__(256) void k_track_batch(st_batch_args b)
     a.turn0 = b.turn0;
     a.nturn = b.nturn;
     a.precheck = b.precheck;
-    a.xtbt = nullptr;
+    a.xtbt = b.xtbt;
     a.tbt_bpm = -1;
-    a.tbt_nturn = 0;
-    a.tbt_turn0 = 0;
+    a.tbt_nturn = b.tbt_nturn;
+    a.tbt_turn0 = b.tbt_turn0;
     a.g = b.g;
     const ST_KCONST st_seed_desc *seeds = (const ST_KCONST st_seed_desc *)b.seeds;
     const ST_KCONST int *items = (const ST_KCONST int *)b.items;

@@ -20,5 +20,7 @@ struct st_tune_args {
 };
 
 hipError_t st_launch_fine_tune(const st_tune_args &a, hipStream_t stream);
+hipError_t st_launch_features(const double *x0, int ncol, const double *tunes, int n, double bx, double ax,
+                              double by, double ay, float *feat, hipStream_t stream);
 
 #endif

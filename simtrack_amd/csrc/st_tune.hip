@@ -126,7 +126,49 @@ __global__ __launch_bounds__(64) void k_fine_tune(st_tune_args a)
     if (lane == 0) a.tune[j] = peak;
 }
 
+/*
+ * Surrogate features of one particle (FP32): Courant-Snyder actions Jx, Jy
+ * and angles (cos, sin) of the initial coordinates with the Twiss
+ * parameters of the start point, the tunes of two turn-by-turn windows
+ * (nux1, nuy1, nux2, nuy2) and log10 |tune change| of both planes.
+ */
+__global__ void k_features(const double *x0, int ncol, const double *tunes, int n, double bx, double ax, double by,
+                           double ay, float *feat)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *x = x0 + (int64_t)i * ncol;
+    const double *q = tunes + (int64_t)i * 4;
+    double sbx = sqrt(bx), sby = sqrt(by);
+    double X = x[0] / sbx, PX = (ax * x[0] + bx * x[1]) / sbx;
+    double Y = x[2] / sby, PY = (ay * x[2] + by * x[3]) / sby;
+    double jx = 0.5 * (X * X + PX * PX), jy = 0.5 * (Y * Y + PY * PY);
+    double fx = atan2(-PX, X), fy = atan2(-PY, Y);
+    float *o = feat + (int64_t)i * 12;
+    o[0] = (float)jx;
+    o[1] = (float)jy;
+    o[2] = (float)cos(fx);
+    o[3] = (float)sin(fx);
+    o[4] = (float)cos(fy);
+    o[5] = (float)sin(fy);
+    o[6] = (float)q[0];
+    o[7] = (float)q[1];
+    o[8] = (float)q[2];
+    o[9] = (float)q[3];
+    o[10] = (float)log10(fabs(q[2] - q[0]) + 1e-12);
+    o[11] = (float)log10(fabs(q[3] - q[1]) + 1e-12);
+}
+
 } // namespace
+
+hipError_t st_launch_features(const double *x0, int ncol, const double *tunes, int n, double bx, double ax,
+                              double by, double ay, float *feat, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_features, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, x0, ncol, tunes, n, bx,
+                       ax, by, ay, feat);
+    return hipGetLastError();
+}
 
 hipError_t st_launch_fine_tune(const st_tune_args &a, hipStream_t stream)
 {

@@ -1,0 +1,177 @@
+"""Surrogate-assisted dynamic-aperture scan (config 3).
+
+The tracker itself produces the training data and the features:
+  1. every grid particle of every lattice is tracked for 2 x 512 turns with
+     turn-by-turn capture (st_batch_track_tbt_device);
+  2. FineTuneFinder (st_fine_tune_device) gives the horizontal and vertical
+     tune of each 512-turn window; with the initial actions and angles they
+     form 12 FP32 features (st_surrogate_features);
+  3. the MLP (st_mlp_*, BF16 MFMA) predicts survival to the long horizon;
+  4. particles with predicted probability between 0.1 and 0.9 fall back to
+     full FP64 tracking from where step 1 left them, to the long horizon.
+A particle lost within the first 1,024 turns is lost; above 0.9 it is taken
+as surviving, below 0.1 as lost.  Full tracking of every particle to the
+long horizon is the reference, and judges the surrogate's errors.
+
+The DA boundary of one angle of one lattice is the amplitude of the first
+particle (by increasing amplitude) that does not survive.
+"""
+import time
+
+import numpy as np
+
+from ._capi import Batch, Mlp
+
+WINDOW = 512
+TUNE_RANGE = (0.01, 0.49)
+P_LOW, P_HIGH = 0.1, 0.9
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def features(ctx, batch, x, offsets, beta_x, alpha_x, beta_y, alpha_y, cfg=None):
+    """Track 2*WINDOW turns with turn-by-turn capture and build the features.
+    Returns device tensors (x after 2*WINDOW turns, state, lost, features)."""
+    torch = _torch()
+    dev = torch.device("cuda:0")
+    n = len(x)
+    d_x0 = torch.tensor(x, dtype=torch.float64, device=dev).contiguous()
+    d_x = d_x0.clone()
+    d_s = torch.ones(n, dtype=torch.int32, device=dev)
+    d_l = torch.zeros(2 * n, dtype=torch.int32, device=dev)
+    nt = 2 * WINDOW
+    d_tbt = torch.zeros((n, nt, 6), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    batch.track_tbt_device(d_x.data_ptr(), offsets, 0, nt, d_tbt.data_ptr(), d_s.data_ptr(), d_l.data_ptr(), 0, cfg)
+    d_q = torch.zeros((4, n), dtype=torch.float64, device=dev)
+    base = d_tbt.data_ptr()
+    for k, (col, win) in enumerate(((0, 0), (2, 0), (0, 1), (2, 1))):
+        ctx.fine_tune_device(base + 8 * (win * WINDOW * 6 + col), nt * 6, 6, WINDOW, n, TUNE_RANGE[0], TUNE_RANGE[1],
+                             d_q[k].data_ptr())
+    ctx.sync()
+    d_tunes = d_q.t().contiguous()
+    d_f = torch.zeros((n, 12), dtype=torch.float32, device=dev)
+    ctx.surrogate_features(d_x0.data_ptr(), 6, d_tunes.data_ptr(), n, beta_x, alpha_x, beta_y, alpha_y, d_f.data_ptr())
+    ctx.sync()
+    del d_tbt
+    return d_x, d_s, d_l, d_f
+
+
+def track_to(ctx, tabs, d_x, d_s, d_l, offsets, sel, turn0, turn1, cfg=None):
+    """Continue the particles of boolean device mask `sel` from turn0 to turn1
+    (batched over lattices); results written back into d_x, d_s, d_l."""
+    torch = _torch()
+    idx = torch.nonzero(sel).flatten()
+    if len(idx) == 0 or turn1 <= turn0:
+        return 0
+    seed_of = torch.bucketize(idx, torch.as_tensor(offsets[1:], device=idx.device), right=True)
+    counts = torch.bincount(seed_of, minlength=len(tabs)).cpu().numpy()
+    keep = [k for k in range(len(tabs)) if counts[k] > 0]
+    sub_offs = np.concatenate([[0], np.cumsum(counts[keep])]).astype(np.int32)
+    b = Batch(ctx, [tabs[k] for k in keep])
+    xs = d_x[idx].contiguous()
+    ss = d_s[idx].contiguous()
+    ls = d_l.view(-1, 2)[idx].contiguous()
+    torch.cuda.synchronize()
+    b.track_device(xs.data_ptr(), sub_offs, turn0, turn1 - turn0, ss.data_ptr(), ls.data_ptr(), 0, cfg)
+    ctx.sync()
+    d_x[idx] = xs
+    d_s[idx] = ss
+    d_l.view(-1, 2)[idx] = ls
+    torch.cuda.synchronize()
+    return len(idx)
+
+
+class Standardizer:
+    def __init__(self, f):
+        self.mean = f.mean(dim=0)
+        self.std = f.std(dim=0).clamp_min(1e-6)
+
+    def __call__(self, f):
+        return ((f - self.mean) / self.std).contiguous()
+
+
+def train(ctx, feats, labels, epochs=30, batch=2048, lr=1e-3, hidden=256, seed=1, log=None):
+    """Train the MLP on (features, labels) device tensors; returns (mlp, standardizer)."""
+    torch = _torch()
+    std = Standardizer(feats)
+    fz = std(feats)
+    y = labels.float().contiguous()
+    m = Mlp(ctx, 12, hidden, seed=seed)
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    n = len(fz)
+    for ep in range(epochs):
+        perm = torch.randperm(n, generator=g).to(fz.device)
+        fsh, ysh = fz[perm].contiguous(), y[perm].contiguous()
+        torch.cuda.synchronize()
+        tot = 0.
+        nb = 0
+        for s in range(0, n - batch + 1, batch):
+            tot += m.train_step(fsh[s:s + batch].data_ptr(), ysh[s:s + batch].data_ptr(), batch,
+                                lr * (0.5 ** (ep // 10)))
+            nb += 1
+        if log is not None:
+            log.append(tot / max(nb, 1))
+    ctx.sync()
+    return m, std
+
+
+def predict(ctx, mlp, std, feats):
+    torch = _torch()
+    fz = std(feats)
+    p = torch.zeros(len(fz), dtype=torch.float32, device=fz.device)
+    torch.cuda.synchronize()
+    mlp.forward(fz.data_ptr(), len(fz), p.data_ptr())
+    ctx.sync()
+    return p
+
+
+def da_boundary(survive, amps, n_angles):
+    """Per-angle DA boundary: amplitude of the first non-surviving particle
+    (grid rows are angle-major, amplitudes increasing); amps[-1] + step if
+    all survive."""
+    s = np.asarray(survive).reshape(n_angles, -1)
+    out = np.empty(n_angles)
+    for a in range(n_angles):
+        lost = np.nonzero(s[a] == 0)[0]
+        out[a] = amps[lost[0]] if len(lost) else amps[-1] + (amps[1] - amps[0])
+    return out
+
+
+def scan(ctx, tabs, x, offsets, n_long, mlp, std, twiss, cfg=None):
+    """The surrogate-plus-fallback scan.  Returns (survive, info)."""
+    torch = _torch()
+    t0 = time.perf_counter()
+    batch = Batch(ctx, tabs)
+    d_x, d_s, d_l, d_f = features(ctx, batch, x, offsets, *twiss, cfg=cfg)
+    t_feat = time.perf_counter() - t0
+    p = predict(ctx, mlp, std, d_f)
+    alive = d_s != 0
+    fallback = alive & (p >= P_LOW) & (p <= P_HIGH)
+    decided_ok = alive & (p > P_HIGH)
+    n_fb = track_to(ctx, tabs, d_x, d_s, d_l, offsets, fallback, 2 * WINDOW, n_long, cfg)
+    surv = torch.where(fallback, d_s != 0, decided_ok)
+    surv = surv.cpu().numpy().astype(np.int32)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return surv, dict(wall=wall, t_features=t_feat, n_fallback=int(n_fb), n_alive_1k=int(alive.sum().item()),
+                      plain=alive.cpu().numpy().astype(np.int32), prob=p.cpu().numpy())
+
+
+def full(ctx, tabs, x, offsets, n_long, cfg=None):
+    """Reference: every particle tracked to n_long.  Returns (survive, wall)."""
+    torch = _torch()
+    t0 = time.perf_counter()
+    batch = Batch(ctx, tabs)
+    dev = torch.device("cuda:0")
+    d_x = torch.tensor(x, dtype=torch.float64, device=dev).contiguous()
+    d_s = torch.ones(len(x), dtype=torch.int32, device=dev)
+    d_l = torch.zeros(2 * len(x), dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    batch.track_device(d_x.data_ptr(), offsets, 0, n_long, d_s.data_ptr(), d_l.data_ptr(), 0, cfg)
+    ctx.sync()
+    s = d_s.cpu().numpy().astype(np.int32)
+    return s, time.perf_counter() - t0, d_x, d_l
