@@ -55,6 +55,9 @@ def main():
     ap.add_argument("--turns-per-step", type=int, default=20)
     ap.add_argument("--particles", type=int, default=65536)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the run, copy this many MiB device-to-device (a kernel of known traffic for "
+                         "calibrating the HBM byte counters)")
     ap.add_argument("--cpu-sample", type=str, default="1024x64", help="particles x turns for the CPU leg")
     ap.add_argument("--workload", choices=("config1", "config2", "config3"), default="config1",
                     help="config1: the headline workload; config2: 64 uneven lattices in one persistent launch")
@@ -131,6 +134,15 @@ def main():
     kernel_rate = pturns / k_sec if k_sec > 0 else float("nan")
     achieved = f_turn * kernel_rate
     peak = flops.PEAK_FP64_VECTOR
+
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     baseline = None
     if not args.no_cpu_baseline:
@@ -267,6 +279,9 @@ def bench_config3(args):
     ctx = st.Context(0)
     gp = W.proton_globals(100000.0)
     twiss = (W.BETA_STAR, 0., W.BETA_STAR, 0.)
+
+    def progress(msg):
+        print(f"[config3] {msg}", file=sys.stderr, flush=True)
     n_ang = 8
     _, _, x1, _ = W.config2(n_seeds=1, n_per_seed=args.per_seed, n_angles=n_ang)
     rng = np.random.default_rng(7)
@@ -282,16 +297,18 @@ def bench_config3(args):
     tabs, x, offs = lattices(range(3000, 3000 + args.train_seeds))
     batch = st.Batch(ctx, tabs)
     d_x, d_s, d_l, d_f = S.features(ctx, batch, x, offs, *twiss)
+    progress("training set: features done")
     alive = d_s != 0
-    S.track_to(ctx, tabs, d_x, d_s, d_l, offs, alive, 2 * S.WINDOW, args.n_long)
+    S.track_to(ctx, tabs, d_x, d_s, d_l, offs, alive, 2 * S.WINDOW, args.n_long, progress=progress)
     log = []
     mlp, std = S.train(ctx, d_f[alive], d_s[alive] != 0, epochs=30, batch=2048, log=log)
+    progress(f"trained: loss {log[0]:.4f} -> {log[-1]:.4f}")
     t_train = time.perf_counter() - t0
     del d_x, d_s, d_l, d_f, batch
 
     etabs, ex, eoffs = lattices(range(2026, 2026 + args.seeds))
-    ref, t_full, _, _ = S.full(ctx, etabs, ex, eoffs, args.n_long)
-    surv, info = S.scan(ctx, etabs, ex, eoffs, args.n_long, mlp, std, twiss)
+    ref, t_full, _, _ = S.full(ctx, etabs, ex, eoffs, args.n_long, progress=progress)
+    surv, info = S.scan(ctx, etabs, ex, eoffs, args.n_long, mlp, std, twiss, progress=progress)
     amps = (np.arange(len(x1) // n_ang) + 1) * 20. / (len(x1) // n_ang)
     err_net, err_plain = [], []
     for k in range(len(etabs)):

@@ -60,7 +60,22 @@ def features(ctx, batch, x, offsets, beta_x, alpha_x, beta_y, alpha_y, cfg=None)
     return d_x, d_s, d_l, d_f
 
 
-def track_to(ctx, tabs, d_x, d_s, d_l, offsets, sel, turn0, turn1, cfg=None):
+CHUNK = 2000
+
+
+def _chunked(run, turn0, turn1, progress, what):
+    """run(t, nt) over [turn0, turn1) in chunks (turn ranges compose exactly),
+    reporting progress after each."""
+    t = turn0
+    while t < turn1:
+        nt = min(CHUNK, turn1 - t)
+        run(t, nt)
+        t += nt
+        if progress:
+            progress(f"{what}: turn {t}/{turn1}")
+
+
+def track_to(ctx, tabs, d_x, d_s, d_l, offsets, sel, turn0, turn1, cfg=None, progress=None):
     """Continue the particles of boolean device mask `sel` from turn0 to turn1
     (batched over lattices); results written back into d_x, d_s, d_l."""
     torch = _torch()
@@ -76,8 +91,8 @@ def track_to(ctx, tabs, d_x, d_s, d_l, offsets, sel, turn0, turn1, cfg=None):
     ss = d_s[idx].contiguous()
     ls = d_l.view(-1, 2)[idx].contiguous()
     torch.cuda.synchronize()
-    b.track_device(xs.data_ptr(), sub_offs, turn0, turn1 - turn0, ss.data_ptr(), ls.data_ptr(), 0, cfg)
-    ctx.sync()
+    _chunked(lambda t, nt: (b.track_device(xs.data_ptr(), sub_offs, t, nt, ss.data_ptr(), ls.data_ptr(), 0, cfg),
+                            ctx.sync()), turn0, turn1, progress, f"tracking {len(idx)} particles")
     d_x[idx] = xs
     d_s[idx] = ss
     d_l.view(-1, 2)[idx] = ls
@@ -141,7 +156,7 @@ def da_boundary(survive, amps, n_angles):
     return out
 
 
-def scan(ctx, tabs, x, offsets, n_long, mlp, std, twiss, cfg=None):
+def scan(ctx, tabs, x, offsets, n_long, mlp, std, twiss, cfg=None, progress=None):
     """The surrogate-plus-fallback scan.  Returns (survive, info)."""
     torch = _torch()
     t0 = time.perf_counter()
@@ -152,7 +167,9 @@ def scan(ctx, tabs, x, offsets, n_long, mlp, std, twiss, cfg=None):
     alive = d_s != 0
     fallback = alive & (p >= P_LOW) & (p <= P_HIGH)
     decided_ok = alive & (p > P_HIGH)
-    n_fb = track_to(ctx, tabs, d_x, d_s, d_l, offsets, fallback, 2 * WINDOW, n_long, cfg)
+    if progress:
+        progress(f"surrogate: features and predictions in {time.perf_counter() - t0:.1f} s")
+    n_fb = track_to(ctx, tabs, d_x, d_s, d_l, offsets, fallback, 2 * WINDOW, n_long, cfg, progress)
     surv = torch.where(fallback, d_s != 0, decided_ok)
     surv = surv.cpu().numpy().astype(np.int32)
     torch.cuda.synchronize()
@@ -161,7 +178,7 @@ def scan(ctx, tabs, x, offsets, n_long, mlp, std, twiss, cfg=None):
                       plain=alive.cpu().numpy().astype(np.int32), prob=p.cpu().numpy())
 
 
-def full(ctx, tabs, x, offsets, n_long, cfg=None):
+def full(ctx, tabs, x, offsets, n_long, cfg=None, progress=None):
     """Reference: every particle tracked to n_long.  Returns (survive, wall)."""
     torch = _torch()
     t0 = time.perf_counter()
@@ -171,7 +188,7 @@ def full(ctx, tabs, x, offsets, n_long, cfg=None):
     d_s = torch.ones(len(x), dtype=torch.int32, device=dev)
     d_l = torch.zeros(2 * len(x), dtype=torch.int32, device=dev)
     torch.cuda.synchronize()
-    batch.track_device(d_x.data_ptr(), offsets, 0, n_long, d_s.data_ptr(), d_l.data_ptr(), 0, cfg)
-    ctx.sync()
+    _chunked(lambda t, nt: (batch.track_device(d_x.data_ptr(), offsets, t, nt, d_s.data_ptr(), d_l.data_ptr(), 0, cfg),
+                            ctx.sync()), 0, n_long, progress, "full tracking")
     s = d_s.cpu().numpy().astype(np.int32)
     return s, time.perf_counter() - t0, d_x, d_l
