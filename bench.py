@@ -134,6 +134,10 @@ def main():
     kernel_rate = pturns / k_sec if k_sec > 0 else float("nan")
     achieved = f_turn * kernel_rate
     peak = flops.PEAK_FP64_VECTOR
+    # HBM bytes per launch of this shape, measured with rocprofv3 FETCH_SIZE and
+    # WRITE_SIZE against a copy of known size (profiles/k_track_spin_config1_traffic.txt);
+    # the state is read and written once per launch, independent of the turn count
+    traffic = 13.77e6 if args.particles == 65536 else None
 
     if args.copy_probe_mib > 0:
         n = args.copy_probe_mib * (1 << 20) // 8
@@ -169,7 +173,8 @@ def main():
         # compute-bound: the roof is MI355X's FP64 peak (78.6 TFLOP/s, the same figure for
         # the vector and matrix pipes); the kernel runs on the FP64 vector pipe
         "roofline": {"bound": "mfma", "pipe": "fp64 VALU", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": traffic,
+                     "algorithmic_bytes": 2 * 9 * 8 * args.particles,
                      "f_turn_flops": f_turn, "kernel": "k_track<spin>",
                      "avg_launch_ms": 1e3 * k_sec / max(k_launches, 1), "launches": k_launches},
         "cpu_baseline": baseline,
