@@ -58,50 +58,70 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (a kernel of known traffic for "
                          "calibrating the HBM byte counters)")
-    ap.add_argument("--cpu-sample", type=str, default="1024x64", help="particles x turns for the CPU leg")
-    ap.add_argument("--workload", choices=("config1", "config2", "config3"), default="config1",
-                    help="config1: the headline workload; config2: 64 uneven lattices in one persistent launch")
+    ap.add_argument("--cpu-sample", type=str, default=None,
+                    help="particles x turns for the CPU leg (default 1024x64; config4: 2048x24)")
+    ap.add_argument("--workload", choices=("config1", "config2", "config3", "config4"), default="config1",
+                    help="config1: the headline workload; config2: 64 uneven lattices in one persistent launch; "
+                         "config3: surrogate DA scan; config4: radiating electron ring, 1,048,576 particles")
     ap.add_argument("--seeds", type=int, default=64, help="config2: lattices")
     ap.add_argument("--per-seed", type=int, default=4096, help="config2/3: particles per lattice")
     ap.add_argument("--train-seeds", type=int, default=16, help="config3: lattices for the training set")
     ap.add_argument("--n-long", type=int, default=30000, help="config3: survival horizon in turns")
+    ap.add_argument("--orbit", action="store_true", help="config4: orbit tracking with quantum excitation "
+                    "instead of spin tracking")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the tracking "
               f"path shards by particles or seeds with no collective, see the sharding notes", file=sys.stderr)
         sys.exit(2)
 
+    if args.cpu_sample is None:
+        args.cpu_sample = "2048x24" if args.workload == "config4" else "1024x64"
     if args.workload == "config2":
         return bench_config2(args)
     if args.workload == "config3":
         return bench_config3(args)
 
+    if args.workload == "config4":
+        return bench_config4(args)
+
+    from simtrack_amd import workloads as W
+    gp, ln, x = W.config1(n_particles=args.particles)
+    # HBM bytes per launch of this shape, measured with rocprofv3 FETCH_SIZE and
+    # WRITE_SIZE against a copy of known size (profiles/k_track_spin_config1_traffic.txt);
+    # the state is read and written once per launch, independent of the turn count
+    traffic = 13.77e6 if args.particles == 65536 else None
+    config = {"workload": "config1: 998-element DA ring + 6-D beam-beam at 2 IPs + spin, 100 GeV p",
+              "particles": args.particles, "turns_per_step": args.turns_per_step, "elements": int(len(ln.Cell)),
+              "spin": True, "beam_beam_slices": 5}
+    bench_tracking(args, gp, ln.records(), x, True, config, traffic,
+                   "synthetic (seeded lattice errors and Gaussian particles)")
+
+
+def bench_tracking(args, gp, rec, x, spin, config, traffic, data):
+    """Time args.steps steps of args.turns_per_step turns of x (resident in HBM) and print the JSON line."""
     import numpy as np
     import torch
 
     import simtrack_amd as st
     from simtrack_amd import flops
-    from simtrack_amd import workloads as W
 
-    gp, ln, x = W.config1(n_particles=args.particles)
-    rec = ln.records()
-    spin = True
-    ncol = 9
+    npart = len(x)
+    flags = st.TRACK_SPIN if spin else 0
     f_turn = flops.flops_per_turn(rec, gp, spin=spin)
 
     ctx = st.Context(0)
     tab = st.Table(ctx, gp, rec)
     dev = torch.device("cuda:0")
     d_x = torch.tensor(x, dtype=torch.float64, device=dev).contiguous()
-    d_state = torch.ones(args.particles, dtype=torch.int32, device=dev)
-    d_lost = torch.zeros(2 * args.particles, dtype=torch.int32, device=dev)
+    d_state = torch.ones(npart, dtype=torch.int32, device=dev)
+    d_lost = torch.zeros(2 * npart, dtype=torch.int32, device=dev)
     torch.cuda.synchronize()
 
     T = args.turns_per_step
     turn = 0
     for _ in range(args.warmup):
-        tab.track_device(d_x.data_ptr(), args.particles, turn, T, d_state.data_ptr(), d_lost.data_ptr(),
-                         st.TRACK_SPIN)
+        tab.track_device(d_x.data_ptr(), npart, turn, T, d_state.data_ptr(), d_lost.data_ptr(), flags)
         turn += T
     ctx.sync()
     torch.cuda.synchronize()
@@ -116,8 +136,7 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        tab.track_device(d_x.data_ptr(), args.particles, turn, T, d_state.data_ptr(), d_lost.data_ptr(),
-                         st.TRACK_SPIN)
+        tab.track_device(d_x.data_ptr(), npart, turn, T, d_state.data_ptr(), d_lost.data_ptr(), flags)
         turn += T
     ctx.sync()
     torch.cuda.synchronize()
@@ -134,10 +153,6 @@ def main():
     kernel_rate = pturns / k_sec if k_sec > 0 else float("nan")
     achieved = f_turn * kernel_rate
     peak = flops.PEAK_FP64_VECTOR
-    # HBM bytes per launch of this shape, measured with rocprofv3 FETCH_SIZE and
-    # WRITE_SIZE against a copy of known size (profiles/k_track_spin_config1_traffic.txt);
-    # the state is read and written once per launch, independent of the turn count
-    traffic = 13.77e6 if args.particles == 65536 else None
 
     if args.copy_probe_mib > 0:
         n = args.copy_probe_mib * (1 << 20) // 8
@@ -154,6 +169,8 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         baseline = cpu_baseline(gp, rec, x, spin, sp, stn, threads)
 
+    ncol = 9 if spin else 6
+    kname = "k_track<spin>" if spin else "k_track"
     out = {
         "metric": "FP64 particle-turns/s",
         "value": value,
@@ -166,20 +183,35 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded lattice errors and Gaussian particles)",
-        "config": {"workload": "config1: 998-element DA ring + 6-D beam-beam at 2 IPs + spin, 100 GeV p",
-                   "particles": args.particles, "turns_per_step": T, "elements": int(len(rec)),
-                   "spin": True, "beam_beam_slices": 5},
+        "data": data,
+        "config": config,
         # compute-bound: the roof is MI355X's FP64 peak (78.6 TFLOP/s, the same figure for
         # the vector and matrix pipes); the kernel runs on the FP64 vector pipe
-        "roofline": {"bound": "mfma", "pipe": "fp64 VALU", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": traffic,
-                     "algorithmic_bytes": 2 * 9 * 8 * args.particles,
-                     "f_turn_flops": f_turn, "kernel": "k_track<spin>",
+        "roofline": {"bound": "mfma", "pipe": "fp64 VALU", "achieved": achieved / 1e12, "peak": peak / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
+                     "algorithmic_bytes": 2 * ncol * 8 * npart,
+                     "f_turn_flops": f_turn, "kernel": kname,
                      "avg_launch_ms": 1e3 * k_sec / max(k_launches, 1), "launches": k_launches},
         "cpu_baseline": baseline,
     }
     print(json.dumps(out))
+
+
+def bench_config4(args):
+    """Config 4: the ~3,500-element 18 GeV electron ring with radiation and
+    spin, 1,048,576 electrons matched to the radiation equilibrium."""
+    from simtrack_amd import workloads as W
+    npart = args.particles if args.particles != 65536 else 1048576
+    spin = not args.orbit
+    gp, ln, x = W.config4(n_particles=npart, spin=spin)
+    gp.rng_seed = 2029
+    what = ("spin (Track_spin: classical radiation, as SBEND_sPass)" if spin
+            else "orbit only (Track: radiation damping + quantum excitation)")
+    config = {"workload": "config4: 18 GeV electron ring, synthetic FODO arcs + 2 straights, H_expand, "
+                          "synchrotron radiation, " + what, "particles": npart,
+              "turns_per_step": args.turns_per_step, "elements": int(len(ln.Cell)), "spin": spin, "radiate": True}
+    bench_tracking(args, gp, ln.records(), x, spin, config, None,
+                   "synthetic (Gaussian electrons matched to the radiation equilibrium)")
 
 
 def bench_config2(args):

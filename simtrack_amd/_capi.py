@@ -86,6 +86,18 @@ class STError(RuntimeError):
     pass
 
 
+def _load_torch_first():
+    """PyTorch ships its own HIP runtime (libamdhip64, libhsa-runtime64 under
+    torch/lib).  When this library, linked against the system ROCm, is loaded
+    into a process before torch, torch's device initialisation later fails
+    with "No HIP GPUs are available"; loading torch first avoids that.  Torch
+    is optional: without it nothing is loaded."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load(path=None):
     """Load libsimtrack_amd.so (raises if it is not built)."""
     global _lib
@@ -94,6 +106,7 @@ def load(path=None):
     path = path or _build.library_path()
     if not os.path.exists(path):
         raise STError(f"HIP library not built: {path} (run simtrack_amd/build.py)")
+    _load_torch_first()
     lib = ctypes.CDLL(path)
     P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     lib.st_last_error.restype = ctypes.c_char_p

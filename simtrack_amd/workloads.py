@@ -439,3 +439,195 @@ def config2(n_seeds=64, n_per_seed=4096, seed=2028, min_elements=1000, max_eleme
     x = np.concatenate([one] * n_seeds)
     offsets = np.arange(n_seeds + 1, dtype=np.int32) * len(one)
     return gp, lines, x, offsets
+
+
+# ------------------------------------------------- electron ring (config 4)
+#
+# FODO arc cells of two sector dipoles, thick quadrupoles and sextupoles,
+# with horizontal and vertical BPMs, and two dispersion-carrying straights of
+# bend-free cells that hold the RF cavities and the IP markers.  Every bend
+# radiates (GP.radiate) and the maps use the expanded Hamiltonian
+# (GP.H_expand), the only bend branch with quantum excitation besides the
+# exact one (SBEND_Pass, simtrack.cpp:4720-4746).
+
+ELECTRON_G = 0.00115965218128      # anomalous magnetic moment a_e (CODATA 2018)
+E_LQ, E_LS = 0.6, 0.25             # quadrupole and sextupole lengths [m]
+
+
+def _e_half_drifts(lb):
+    """Drifts of a half cell: quad-sext, sext-bend, bend-BPM, BPM-quad."""
+    return 0.3, 0.3, 0.25, 0.3
+
+
+def _e_cell(tag, lb, angle, kq, k2f, k2d, quad_nint, sext_nint, rf=None):
+    """One arc cell (17 elements).  rf: an RFCAV that replaces the VBPM."""
+    d1, d2, d3, d4 = _e_half_drifts(lb)
+
+    def q(name, length, k1):
+        e = L.QUAD(name, length, k1 * length, 0.)
+        e.Nint = quad_nint
+        return e
+
+    def s(name, k2l):
+        e = L.SEXT(name, E_LS, k2l, 0.)
+        e.Nint = sext_nint
+        return e
+    return [q(f"QFA{tag}", E_LQ / 2, kq), L.DRIFT("D1", d1), s(f"SF{tag}", k2f), L.DRIFT("D2", d2),
+            L.SBEND(f"BA{tag}", lb, angle), L.DRIFT("D3", d3), L.HBPM(f"HBPM{tag}", 0.), L.DRIFT("D4", d4),
+            q(f"QD{tag}", E_LQ, -kq), L.DRIFT("D1", d1), s(f"SD{tag}", k2d), L.DRIFT("D2", d2),
+            L.SBEND(f"BB{tag}", lb, angle), L.DRIFT("D3", d3),
+            rf if rf is not None else L.VBPM(f"VBPM{tag}", 0.), L.DRIFT("D4", d4), q(f"QFB{tag}", E_LQ / 2, kq)]
+
+
+def _e_straight_cell(tag, lb, kq, quad_nint, centre, kq_scale=1.0):
+    """A bend-free cell of the arc cell's length (9 elements); centre is
+    the element placed in its middle (an RFCAV or a MARKER)."""
+    d1, d2, d3, d4 = _e_half_drifts(lb)
+    half = d1 + E_LS + d2 + lb + d3 + d4
+
+    def q(name, length, k1):
+        e = L.QUAD(name, length, k1 * length, 0.)
+        e.Nint = quad_nint
+        return e
+    kq = kq * kq_scale
+    return [q(f"QFA{tag}", E_LQ / 2, kq), L.DRIFT("DS", half / 2), centre, L.DRIFT("DS", half / 2),
+            q(f"QD{tag}", E_LQ, -kq), L.DRIFT("DS", half / 2), L.MARKER(f"MS{tag}", 0.), L.DRIFT("DS", half / 2),
+            q(f"QFB{tag}", E_LQ / 2, kq)]
+
+
+def _e_cell_kq(lb, angle, mu_deg):
+    """Quadrupole gradient giving the arc cell a horizontal phase advance of mu_deg."""
+    from scipy.optimize import brentq
+    from . import optics as O
+
+    def f(kq):
+        cell = L.Line(_e_cell("X", lb, angle, kq, 0., 0., 4, 2))
+        m = np.eye(3)
+        for e in cell.Cell:
+            m = O.element_matrices(e)[0] @ m
+        return math.degrees(math.acos(max(-1., min(1., (m[0, 0] + m[1, 1]) / 2)))) - mu_deg
+    return brentq(f, 0.05, 3.0, xtol=1e-14)
+
+
+def _e_sextupoles(lb, angle, kq, xi_target):
+    """Arc sextupole K2L (SF, SD) that set both chromaticities of an arc
+    cell to xi_target / n_cells each (thin-lens sums over the cell)."""
+    from . import optics as O
+    cell = L.Line(_e_cell("X", lb, angle, kq, 0., 0., 4, 2))
+    opt = O.Optics(cell)
+    tx = np.array([[opt.beta_x, -opt.alpha_x], [-opt.alpha_x, (1 + opt.alpha_x ** 2) / opt.beta_x]])
+    ty = np.array([[opt.beta_y, -opt.alpha_y], [-opt.alpha_y, (1 + opt.alpha_y ** 2) / opt.beta_y]])
+    eta = np.array([opt.D, opt.Dp, 1.])
+    xi_x = xi_y = 0.
+    at = {}
+    for e in cell.Cell:
+        mx, my = O.element_matrices(e)
+        b0x, b0y, d0 = tx[0, 0], ty[0, 0], eta[0]
+        eta = mx @ eta
+        tx = mx[:2, :2] @ tx @ mx[:2, :2].T
+        ty = my @ ty @ my.T
+        bx, by, d = (b0x + tx[0, 0]) / 2, (b0y + ty[0, 0]) / 2, (d0 + eta[0]) / 2
+        if e.TYPE == "QUAD":
+            xi_x -= bx * e.KN / (4 * math.pi)
+            xi_y += by * e.KN / (4 * math.pi)
+        if e.TYPE == "SEXT":
+            at[e.NAME[:2]] = (bx, by, d)
+    (bfx, bfy, df), (bdx, bdy, dd) = at["SF"], at["SD"]
+    # xi_x + (bfx df k2f + bdx dd k2d)/4pi = target, xi_y - (bfy df k2f + bdy dd k2d)/4pi = target
+    a = np.array([[bfx * df, bdx * dd], [-bfy * df, -bdy * dd]]) / (4 * math.pi)
+    return np.linalg.solve(a, [xi_target - xi_x, xi_target - xi_y])
+
+
+def electron_globals_ring(total_energy_mev=18000.0, spin=False):
+    """18 GeV electrons with radiation on and the expanded Hamiltonian."""
+    gp = electron_globals(total_energy_mev, radiate=1, H_expand=1)
+    gp.G = ELECTRON_G
+    return gp
+
+
+def electron_ring(gp, n_arc_cells=200, n_straight_cells=6, bend_length=3.8, mu_deg=86., chromaticity=1.0,
+                  harmonic=4800, overvoltage=1.6, rf_in_arcs=False, quad_nint=4, sext_nint=2, aperture=0.1,
+                  straight_kq_scale=1.0):
+    """An electron ring of 17-element FODO arc cells (two SBENDs each) split
+    into two arcs by straights of n_straight_cells bend-free cells.  The RF
+    cavities sit in the straights (one per straight cell), or, with
+    rf_in_arcs, one per arc cell in place of its VBPM, which keeps the
+    radiation sawtooth small.  Total voltage = overvoltage x U0 (the loss of
+    Cal_SRLoss_U0rad, simtrack.cpp:20691-20725); PHASE0 = 0 puts the stable
+    phase where the RF restores U0.  chromaticity: target of both planes
+    (None switches the sextupoles off)."""
+    from . import optics as O
+    angle = 2 * math.pi / (2 * n_arc_cells)
+    kq = _e_cell_kq(bend_length, angle, mu_deg)
+    if chromaticity is None:
+        k2f = k2d = 0.
+    else:
+        k2f, k2d = _e_sextupoles(bend_length, angle, kq, chromaticity / (n_arc_cells + n_straight_cells * 0.5))
+    rho = bend_length / angle
+    u0 = O.CR * (gp.energy / 1000.) ** 4 / rho * 1000.        # MeV per turn
+
+    def cav(tag):
+        return L.RFCAV(f"RF{tag}", 0., 0., 0., 0.)             # VRF set below
+    half = n_arc_cells // 2
+    ns = max(n_straight_cells // 2, 1)
+    cells = []
+    for arc in range(2):
+        for j in range(ns):
+            centre = L.MARKER(f"IP{arc + 1}", 0.) if j == 0 else (L.MARKER(f"MK{arc}{j}", 0.) if rf_in_arcs
+                                                                  else cav(f"S{arc}{j}"))
+            cells += _e_straight_cell(f"S{arc}{j}", bend_length, kq, quad_nint, centre, straight_kq_scale)
+        for i in range(half):
+            k = arc * half + i
+            cells += _e_cell(k, bend_length, angle, kq, k2f, k2d, quad_nint, sext_nint,
+                             rf=cav(f"A{k}") if rf_in_arcs else None)
+    if not rf_in_arcs and ns == 1:
+        cells[1:1] = [cav("S0")]
+    ln = L.Line(cells)
+    cavities = [e for e in ln.Cell if e.TYPE == "RFCAV"]
+    vcav = overvoltage * u0 / len(cavities)
+    for e in cavities:
+        e.VRF = vcav
+    for e in ln.Cell:
+        e.APx = e.APy = aperture
+    ln.Update()
+    set_rf_frequency(ln, gp, harmonic)
+    ln.info = dict(kq=kq, k2f=k2f, k2d=k2d, rho=rho, u0_mev=u0, n_cav=len(cavities), vcav_mv=vcav)
+    return ln
+
+
+def config4(n_particles=1048576, seed=2029, spin=True):
+    """Config 4: the 18 GeV electron ring (~3,500 elements, radiation, spin)
+    with n_particles Gaussian electrons matched to its radiation
+    equilibrium, spins vertical.  Returns (globals, line, x)."""
+    from . import optics as O
+    gp = electron_globals_ring(18000.0)
+    ln = electron_ring(gp)
+    opt = O.Optics(ln)
+    eq = opt.equilibrium(gp)
+    sz = _bunch_length(opt, gp, ln, eq["sigma_delta"])
+    zs, _ = rf_synchronous(opt, gp, ln)
+    x = opt.matched_beam(n_particles, eq["eps_x"], eq["eps_x"] * 0.1, eq["sigma_delta"], sz, seed=seed, spin=spin)
+    x[:, 4] += zs
+    return gp, ln, x
+
+
+def rf_synchronous(opt, gp, ln):
+    """(z_s, Qs): the synchronous z where the RF restores U0, and the
+    linear synchrotron tune.  RFCAV_Pass (simtrack.cpp:6002-6012) adds
+    |VRF Q| sin(-k z + pi - PHASE0) / beta0 / E0 to pt with k = 2 pi FRF / c;
+    with PHASE0 = 0 that is VRF sin(k z), so sin(k z_s) = U0 / V."""
+    u0 = opt.equilibrium(gp)["U0_MeV"]
+    cav = [e for e in ln.Cell if e.TYPE == "RFCAV"]
+    v = sum(e.VRF for e in cav)
+    k = 2 * math.pi * cav[0].FRF / 299792458.
+    zs = math.asin(u0 / v) / k
+    a = v * k * math.cos(k * zs) / gp.energy
+    b = opt.alpha_c * opt.length
+    qs = math.acos(1 - a * b / 2) / (2 * math.pi)
+    return zs, qs
+
+
+def _bunch_length(opt, gp, ln, sigma_delta):
+    """Equilibrium rms bunch length of the linearised RF: alpha_c C sigma_delta / (2 pi Qs)."""
+    _, qs = rf_synchronous(opt, gp, ln)
+    return opt.alpha_c * opt.length * sigma_delta / (2 * math.pi * qs)
