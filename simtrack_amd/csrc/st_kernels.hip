@@ -325,7 +325,7 @@ ST_DEV void sbend_slices_exact(St &s, Ptc &c, int nint, double L, double hs, dou
             sbend_slice_t<true>(af, s, pz0, as1, x3, rt, rh, L, hs, ih, cosA, sinA, Lb, An, href, c, g);
         else
             sbend_slice_t<false>(af, s, pz0, as1, x3, rt, rh, L, hs, ih, cosA, sinA, Lb, An, href, c, g);
-        if (__builtin_expect(af.bad, 0)) {
+        if (__builtin_expect(af.bad(), 0)) {
             ST_FD_COLD_PATH();
             s = s0;
             pz0 = pz00;

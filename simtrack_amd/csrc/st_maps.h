@@ -81,23 +81,37 @@ ST_DEV void ptc_update(Ptc &c, double pt, const st_kglobals &g)
  * A map step is written once, templated on how it divides, takes square
  * roots, asin and sin/cos.  ArFast evaluates the common case without a
  * single branch (shared reciprocals, guard-free sqrt, the small-argument
- * libm paths) and records in `bad` whether any precondition of those
- * shortcuts failed; ArExact uses the operators and the full functions.
- * Where every precondition holds the two give identical bits, so a caller
- * runs ArFast, and replays the step with ArExact from the saved input state
- * in the (rare) lanes where bad is set.  A step is thereby one basic block
- * that the scheduler can interleave freely -- the point on a GPU running one
+ * libm paths) and records whether every precondition of those shortcuts
+ * held; ArExact uses the operators and the full functions.  Where every
+ * precondition holds the two give identical bits, so a caller runs ArFast,
+ * and replays the step with ArExact from the saved input state in the
+ * (rare) lanes where bad() is true.  A step is thereby one basic block that
+ * the scheduler can interleave freely -- the point on a GPU running one
  * wavefront per SIMD, whose speed is set by instruction-level parallelism.
+ *
+ * The preconditions are tracked cheaply: the smallest and the largest of
+ * every |a * y| (the first product of a division) and every sqrt argument,
+ * two FP64 min/max per operation, tested once per step.  The division needs
+ * 2^-60 < |b| < 2^60 (StRcp::ok) and 2^-900 < |a/b| < 2^700; sqrt needs
+ * 2^-767 <= x < inf.  One window [2^-767, 2^700) for both is stricter than
+ * either (|a * y| is within 2 ulp of a/b), so it only sends a few more lanes
+ * to the exact replay.  Zero, infinite and negative values fall outside it.
+ * A NaN never reaches lo or hi (min/max return the other operand); NaN in
+ * gives NaN out on both paths, and NaN sign and payload carry no meaning.
  */
 struct ArFast {
-    bool bad = false;
+    double lo = __builtin_inf(), hi = 0.;
+    bool ok = true;     /* every divisor in range, every asin/sincos argument small */
+    ST_DEV bool bad() const { return !(ok && lo >= 0x1p-767 && hi < 0x1p700); }
     ST_DEV double div(double a, const StRcp &r)
     {
         double q0 = a * r.y;
         double rr = __builtin_fma(-r.b, q0, a);
         double q = __builtin_fma(rr, r.y, q0);
-        double aq = __builtin_fabs(q);
-        bad = bad | !(r.ok && aq > 0x1p-900 && aq < 0x1p700);
+        double aq = __builtin_fabs(q0);
+        lo = __builtin_fmin(lo, aq);
+        hi = __builtin_fmax(hi, aq);
+        ok = ok & r.ok;
         return q;
     }
     ST_DEV double sqrt(double x)
@@ -112,23 +126,23 @@ struct ArFast {
         g = __builtin_fma(d, h, g);
         d = __builtin_fma(-g, g, x);
         g = __builtin_fma(d, h, g);
-        bad = bad | !(x >= 0x1p-767 && x < __builtin_inf());
+        lo = __builtin_fmin(lo, x);
+        hi = __builtin_fmax(hi, x);
         return g;
     }
     ST_DEV double asin(double x)
     {
-        bad = bad | !(__builtin_fabs(x) < 0.125);
+        ok = ok & (__builtin_fabs(x) < 0.125);
         return st_asin_small(x);
     }
     ST_DEV void sincos(double x, double *sn, double *cs)
     {
-        bad = bad | !(__builtin_fabs(x) <= ST_PIO4);
+        ok = ok & (__builtin_fabs(x) <= ST_PIO4);
         st_sincos_small(x, sn, cs);
     }
 };
 
 struct ArExact {
-    bool bad = false;
     ST_DEV double div(double a, const StRcp &r) { return a / r.b; }
     ST_DEV double sqrt(double x) { return __builtin_sqrt(x); }
     ST_DEV double asin(double x) { return st_asin(x); }
@@ -164,7 +178,7 @@ ST_DEV void drift(St &s, double L, double Lb, const Ptc &c)
     double x0 = s.x[0], x2 = s.x[2], x4 = s.x[4];
     ArFast af;
     drift_t<HEXP>(af, s, L, Lb, c);
-    if (__builtin_expect(af.bad, 0)) {
+    if (__builtin_expect(af.bad(), 0)) {
         ST_FD_COLD_PATH();
         s.x[0] = x0;
         s.x[2] = x2;
@@ -409,7 +423,7 @@ ST_DEV void sbend_exact(St &s, double L, double href, double ih, double cosA, do
     double x0 = s.x[0], x1 = s.x[1], x2 = s.x[2], x4 = s.x[4];
     ArFast af;
     sbend_exact_t(af, s, L, href, ih, cosA, sinA, Lb, c, rh);
-    if (__builtin_expect(af.bad, 0)) {
+    if (__builtin_expect(af.bad(), 0)) {
         ST_FD_COLD_PATH();
         s.x[0] = x0;
         s.x[1] = x1;
@@ -510,7 +524,7 @@ ST_DEV void spin_rotate(St &s, double xe, double pxe, double pye, double angle, 
         ArFast af;
         double pze = af.sqrt(c.opd2 - pxe * pxe - pye * pye);
         spin_rotate_t(af, s, xe, pxe, pye, pze, angle, href, B0, B1, B2, c, g);
-        if (__builtin_expect(af.bad, 0)) {
+        if (__builtin_expect(af.bad(), 0)) {
             ST_FD_COLD_PATH();
             s.x[6] = s6;
             s.x[7] = s7;
