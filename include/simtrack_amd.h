@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define ST_ABI_VERSION 1
+#define ST_ABI_VERSION 2
 
 /* ------------------------------------------------------------ status */
 enum st_status {
@@ -81,7 +81,9 @@ enum st_element_type {
     ST_CRBB = 26,      /* CRBB       cpp:11316 */
     ST_MATRIX = 27,    /* MATRIX     cpp:9774  */
     ST_ROTAT = 28,     /* ROTAT      cpp:11784 (spin rotator; orbit: drift)   */
-    ST_SNAKE = 29      /* SNAKE      cpp:11864 (spin rotator; orbit: drift)   */
+    ST_SNAKE = 29,     /* SNAKE      cpp:11864 (spin rotator; orbit: drift)   */
+    ST_PATCH = 30,     /* PATCH      cpp:12262 dx,dy,ds = DX,DY,DS, theta = THETA, L = DS */
+    ST_SOLEN = 31      /* SOLEN      cpp:9645  ks = KS (KS = 0 only: a misaligned drift) */
 };
 
 /*
@@ -118,6 +120,7 @@ typedef struct st_element {
     double rot_angle;       /* ROTAT/SNAKE angle                             */
     double m66[36];         /* MATRIX M66 (row major)                        */
     double xco_in[6], xco_out[6]; /* MATRIX XCO_IN, XCO_OUT                  */
+    double ks;              /* SOLEN KS                                      */
 } st_element;
 
 /*

@@ -11,11 +11,12 @@
  *   Track, Track_spin, Track_Spin, Track_tbt(_spin),   simtrack.h:1360-1390
  *   Track_Radial, Track_DA, Track_DA1, Track_DA_Radial,
  *   Track_DA_Uniform, Track_DA_Radial_Uniform
+ *   Read_MADX_Lattice, Read_BMAD_Lattice               simtrack.cpp:13591, 14054
  *
  * Element classes: DRIFT, MARKER, BPM, HBPM, VBPM, SBEND, GSBEND, QUAD,
  * SKEWQ, SEXT, OCT, MULT, GMULT, SBENDMULT, SMULT, GSBENDMULT, KICKER,
  * HKICKER, VKICKER, RFCAV, CRABRF, CCMULT, LBT, ILBT, BEAMBEAM, CRBB,
- * MATRIX, ROTAT, SNAKE, with the reference's constructor arguments, Nint and
+ * MATRIX, ROTAT, SNAKE, PATCH, SOLEN, with the reference's constructor arguments, Nint and
  * Norder defaults (simtrack.cpp:8451, 8635, 8943-8950) and SetP/GetP names.
  *
  * Differences, all deliberate:
@@ -38,7 +39,9 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <memory>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -65,6 +68,8 @@ constexpr double BLslice = 0.25, QLslice = 0.05, GLslice = 0.05;
 /* GlobalVariables GP (simtrack.h:28-68): the fields the maps read */
 struct GlobalVariables : st_globals {
     GlobalVariables() { st_globals_default(this); }
+    int harm = 360;             /* GP.harm (simtrack.h:51), read by drivers         */
+    double circumference = 0.;  /* GP.circumference, set by the lattice readers     */
 };
 inline GlobalVariables GP;
 
@@ -535,6 +540,53 @@ public:
     }
 };
 
+/* PATCH (simtrack.cpp:12262-12334): TRANS_Pass by DX, DY, DS (held in the
+   Element's own DX, DY, DS, which are not misalignments here, L = DS), then
+   a frame rotation about y by THETA */
+class PATCH : public Element {
+public:
+    PATCH(std::string name, double dx, double dy, double ds, double theta) : Element(std::move(name))
+    {
+        TYPE = "PATCH";
+        rec.type = ST_PATCH;
+        L = ds;
+        DX = dx;
+        DY = dy;
+        DS = ds;
+        rec.theta = theta;
+        bind("THETA", rec.theta);
+    }
+    void SetP(const char *name, double value) override
+    {
+        if (!std::strcmp(name, "DX")) DX = value;
+        else if (!std::strcmp(name, "DY")) DY = value;
+        else if (!std::strcmp(name, "DS")) DS = value;
+        else Element::SetP(name, value);
+    }
+    double GetP(const char *name) override
+    {
+        if (!std::strcmp(name, "DX")) return DX;
+        if (!std::strcmp(name, "DY")) return DY;
+        if (!std::strcmp(name, "DS")) return DS;
+        return Element::GetP(name);
+    }
+};
+
+/* SOLEN (simtrack.cpp:9645-9704), KS = Bz / brho, Nint = 1.  The engine
+   tracks KS = 0 (then SOLEN_Pass is DRIFT_Pass inside GtoL/LtoG); building a
+   table with a non-zero KS fails. */
+class SOLEN : public Element {
+public:
+    SOLEN(std::string name, double l, double ks) : Element(std::move(name))
+    {
+        if (!(l > 0.)) throw Error("Solenoid: L must be positive.");
+        init("SOLEN", ST_SOLEN, l);
+        rec.ks = ks;
+        rec.nint = 1;
+        bind("KS", rec.ks);
+    }
+};
+
 /* ----------------------------------------------------------------- line */
 
 /* Line (simtrack.h:1067-1100): owns its elements, as the reference's does
@@ -596,6 +648,161 @@ public:
     double frev0 = 0.;
 };
 
+/* ------------------------------------------------------- lattice files */
+
+namespace detail {
+inline bool read_row(const std::string &line, std::string &name, std::string &type, double *v, int n)
+{
+    std::istringstream ss(line);
+    if (!(ss >> name >> type)) return false;
+    for (int i = 0; i < n; ++i)
+        if (!(ss >> v[i])) return false;
+    return true;
+}
+} // namespace detail
+
+/* Read_MADX_Lattice (simtrack.cpp:13591-13796): a MAD-X TWISS table with the
+   columns NAME KEYWORD S L ANGLE E1 E2 TILT K0L K0SL ... K10L K10SL KS,
+   keyword by keyword as the reference maps them (kickers get zero kicks;
+   SOLENOID with KS = 0 or L = 0, and unknown keywords, become DRIFTs). */
+inline void Read_MADX_Lattice(Line &line, const char *filename)
+{
+    std::ifstream f(filename);
+    if (!f) throw Error(std::string("error in opening the file: ") + filename);
+    std::vector<std::string> rows;
+    for (std::string r; std::getline(f, r);) rows.push_back(r);
+    size_t nskip = rows.size();
+    for (size_t i = 0; i < rows.size() && i < 100; ++i)
+        if (rows[i].find("KEYWORD") != std::string::npos) {
+            nskip = i + 2;
+            break;
+        }
+    if (nskip >= rows.size()) throw Error(std::string(filename) + ": no KEYWORD header");
+    for (size_t i = nskip; i < rows.size(); ++i) {
+        std::string name, type;
+        double v[29];
+        if (rows[i].find_first_not_of(" \t\r") == std::string::npos) continue;
+        if (!detail::read_row(rows[i], name, type, v, 29)) throw Error(std::string(filename) + ": short row");
+        name = name.substr(1, name.size() - 2);
+        type = type.substr(1, type.size() - 2);
+        double l = v[1], angle = v[2], e1 = v[3], e2 = v[4], tilt = v[5], knl[11], knsl[11], sk = v[28];
+        for (int k = 0; k < 11; ++k) {
+            knl[k] = v[6 + 2 * k];
+            knsl[k] = v[7 + 2 * k];
+        }
+        Element *e;
+        if (type == "DRIFT") e = new DRIFT(name, l);
+        else if (type == "SBEND") {
+            if (knl[0] != 0. && knl[2] == 0.) e = new GSBEND(name, l, angle, knl[0], e1, e2);
+            else if (knl[1] == 0. && knl[2] == 0.) e = new SBEND(name, l, angle, e1, e2);
+            else e = new GMULT(name, l, angle, e1, e2, knl, knsl);
+            e->DPSI = tilt;
+        } else if (type == "RBEND") {
+            if (knl[1] == 0. && knl[2] == 0.) e = new SBEND(name, l, angle, angle / 2, angle / 2);
+            else e = new GMULT(name, l, angle, e1, e2, knl, knsl);
+        } else if (type == "QUADRUPOLE") {
+            e = new QUAD(name, l, knl[1], knsl[1]);
+            e->DPSI = tilt;
+        } else if (type == "SEXTUPOLE") {
+            e = new SEXT(name, l, knl[2], knsl[2]);
+            e->DPSI = tilt;
+        } else if (type == "OCTUPOLE") {
+            e = new OCT(name, l, knl[3], knsl[3]);
+            e->DPSI = tilt;
+        } else if (type == "HKICKER") e = new HKICKER(name, l, 0.);
+        else if (type == "VKICKER") e = new VKICKER(name, l, 0.);
+        else if (type == "TKICKER") e = new KICKER(name, l, 0., 0.);
+        else if (type == "MULTIPOLE") {
+            e = new MULT(name, l, knl, knsl);
+            e->DPSI = tilt;
+        } else if (type == "VMONITOR") e = new VBPM(name, l);
+        else if (type == "MONITOR") e = new BPM(name, l);
+        else if (type == "MARKER") e = new MARKER(name, l);
+        else if (type == "SOLENOID") {
+            if (sk != 0. && l != 0.) e = new SOLEN(name, l, sk);
+            else e = new DRIFT(name, l);
+        } else if (type == "RFCAVITY") e = new RFCAV(name, l, 0., 0., 0.);
+        else if (type == "BEAMBEAM") {
+            if (l != 0.) throw Error("Beam-Beam element should have zero length!");
+            e = new BEAMBEAM(name, 6, 0., 1, 0.4545, 11, 2.5e-06, 2.5e-06, 0.53, 0.0, 0.53, 0.0);
+        } else if (type == "ELENS") throw Error(name + ": ELENS is not an element of this engine");
+        else e = new DRIFT(name, l);
+        line.Cell.push_back(e);
+    }
+    line.Update();
+    GP.circumference = line.Cell.empty() ? 0. : line.Cell.back()->S;
+}
+
+/* Read_BMAD_Lattice (simtrack.cpp:14054-14237): a Bmad element table with
+   the columns name key s l angle actual_angle e1 e2 tilt x_offset z_offset
+   x_pitch k0l k0sl ... k10l k10sl ks hkick vkick.  Only the first line is
+   skipped, as in the reference: the column-name and units lines become
+   zero-length DRIFTs (the reference gives them the length of an unset
+   variable).  Keys are matched with the reference's capitalisation. */
+inline void Read_BMAD_Lattice(Line &line, const char *filename)
+{
+    std::ifstream f(filename);
+    if (!f) throw Error(std::string("error in opening the file: ") + filename);
+    std::string row;
+    std::getline(f, row);
+    while (std::getline(f, row)) {
+        std::string name, type;
+        double v[35];
+        if (row.find_first_not_of(" \t\r") == std::string::npos) continue;
+        if (!detail::read_row(row, name, type, v, 35)) {
+            std::istringstream ss(row);
+            ss >> name;
+            line.Cell.push_back(new DRIFT(name, 0.));
+            continue;
+        }
+        double l = v[1], angle = v[2], e1 = v[4], e2 = v[5], xoff = v[7], zoff = v[8], xpitch = v[9];
+        double knl[11], knsl[11], sk = v[32], hkick = v[33], vkick = v[34];
+        for (int k = 0; k < 11; ++k) {
+            knl[k] = v[10 + 2 * k];
+            knsl[k] = v[11 + 2 * k];
+        }
+        Element *e;
+        if (type == "Drift") e = new DRIFT(name, l);
+        else if (type == "Sbend") {
+            if (std::fabs(angle) < 1.0e-9) e = new MULT(name, l, knl, knsl);
+            else if (std::fabs(angle - knl[0]) > 1.0e-6) e = new GSBEND(name, l, angle, knl[0], e1, e2);
+            else {
+                knl[0] = 0.;
+                if (knl[2] == 0. && knl[1] == 0.) e = new SBEND(name, l, angle, e1, e2);
+                else e = new SBENDMULT(name, l, angle, e1, e2, knl, knsl);
+            }
+        } else if (type == "RBEND") {
+            if (knl[1] == 0. && knl[2] == 0.) e = new SBEND(name, l, angle, angle / 2., angle / 2.);
+            else e = new SBENDMULT(name, l, angle, angle / 2., angle / 2., knl, knsl);
+        } else if (type == "Quadrupole") e = new QUAD(name, l, knl[1], knsl[1]);
+        else if (type == "Sextupole") e = new SEXT(name, l, knl[2], knsl[2]);
+        else if (type == "OCTUPOLE") e = new OCT(name, l, knl[3], knsl[3]);
+        else if (type == "HKICKER") e = new HKICKER(name, l, hkick);
+        else if (type == "VKICKER") e = new VKICKER(name, l, vkick);
+        else if (type == "Kicker") e = new KICKER(name, l, hkick, vkick);
+        else if (type == "Multipole") {
+            if (knl[0] == 0) e = new MULT(name, l, knl, knsl);
+            else e = new SBEND(name + "_tb", 1.0e-6, knl[0], 0, 0);
+        } else if (type == "HMONITOR") e = new HBPM(name, l);
+        else if (type == "VMONITOR") e = new VBPM(name, l);
+        else if (type == "Monitor") e = new BPM(name, l);
+        else if (type == "MARKER") e = new MARKER(name, l);
+        else if (type == "Solenoid") {
+            if (l != 0.) e = new SOLEN(name, l, sk);
+            else e = new DRIFT(name, l);
+        } else if (type == "RFcavity") e = new RFCAV(name, l, 0., 0., 0.);
+        else if (type == "BEAMBEAM") {
+            if (l != 0.) throw Error("Beam-Beam element should have zero length!");
+            e = new BEAMBEAM(name, 6, 0., 1, 0.4545, 11, 2.5e-06, 2.5e-06, 0.53, 0.0, 0.53, 0.0);
+        } else if (type == "ELENS") throw Error(name + ": ELENS is not an element of this engine");
+        else if (type == "Patch") e = new PATCH(name, xoff, 0.0, zoff, -xpitch);
+        else e = new DRIFT(name, l);
+        line.Cell.push_back(e);
+    }
+    line.Update();
+    GP.circumference = line.Cell.empty() ? 0. : line.Cell.back()->S;
+}
+
 /* --------------------------------------------------------- lowering */
 
 /* st_element code of a reference TYPE string, -1 if the engine lacks it */
@@ -604,7 +811,7 @@ inline int type_code(const std::string &t)
     static const char *names[] = {"", "DRIFT", "SBEND", "GSBEND", "QUAD", "SKEWQ", "SEXT", "OCT", "MULT", "GMULT",
                                   "SBENDMULT", "SMULT", "GSBENDMULT", "KICKER", "HKICKER", "VKICKER", "MARKER",
                                   "BPM", "HBPM", "VBPM", "RFCAV", "CRABRF", "CCMULT", "LBT", "ILBT", "BEAMBEAM",
-                                  "CRBB", "MATRIX", "ROTAT", "SNAKE"};
+                                  "CRBB", "MATRIX", "ROTAT", "SNAKE", "PATCH", "SOLEN"};
     for (int i = 1; i < (int)(sizeof(names) / sizeof(names[0])); ++i)
         if (t == names[i]) return i;
     return -1;
@@ -737,6 +944,13 @@ st_element record_of(E &e)
             r.rot_n[1] = P("NY");
             r.rot_n[2] = P("NS");
             r.rot_angle = P("ANGLE");
+            break;
+        case ST_PATCH:
+            r.theta = P("THETA");   /* DX, DY, DS are the public fields read above */
+            break;
+        case ST_SOLEN:
+            r.ks = P("KS");
+            r.nint = 1;
             break;
         case ST_MATRIX:
             throw Error("element " + e.NAME + ": the reference's MATRIX exposes no parameters to read");

@@ -340,6 +340,30 @@ static void DRIFT_Pass(double x[], double L)
     }
 }
 
+/* PATCH_Pass, simtrack.cpp:6248-6256: TRANS_Pass (cpp:6140-6151), then
+   YROTAT_Pass (cpp:6192-6219, the frame y rotation of Forest's book) */
+static void PATCH_Pass(double x[], double DX, double DY, double DS, double THETA)
+{
+    x[0] = x[0] - DX;
+    x[2] = x[2] - DY;
+    x[4] = x[4] - DS;
+    if (fabs(THETA) < 1.0e-9) return;
+    double xtemp[6], temp, pz;
+    double delta1 = sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0;
+    double gamma1 = GP.gamma + sqrt(GP.gamma * GP.gamma - 1.) * x[pt_];
+    double beta1 = sqrt(1.0 - 1.0 / gamma1 / gamma1);
+    double cosT = OCOS(THETA), sinT = OSIN(THETA), tanT = OTAN(THETA);
+    for (int i = 0; i < 6; i++) xtemp[i] = x[i];
+    pz = sqrt((1 + delta1) * (1 + delta1) - xtemp[1] * xtemp[1] - xtemp[3] * xtemp[3]);
+    temp = 1 - xtemp[1] * tanT / pz;
+    x[0] = xtemp[0] / cosT / temp;
+    x[1] = xtemp[1] * cosT + pz * sinT;
+    x[2] = xtemp[2] + xtemp[3] * xtemp[0] * tanT / pz / temp;
+    x[3] = xtemp[3];
+    x[4] = xtemp[4] - (1 + delta1) * xtemp[0] * tanT / pz / temp / beta1;
+    x[5] = xtemp[5];
+}
+
 /* bend_kick_pass, simtrack.cpp:3790-3794 */
 static void bend_kick_pass(double x[], double L, double href)
 {
@@ -1860,6 +1884,15 @@ static int elem_pass(const st_element *e, double x[], orc_rng *rng)
     case ST_MATRIX:
         MATRIX_Pass(x, e->xco_in, e->xco_out, e->m66);
         return 0;
+    case ST_PATCH:                          /* PATCH::Pass, cpp:12323-12325 */
+        PATCH_Pass(x, e->dx, e->dy, e->ds, e->theta);
+        return 0;
+    case ST_SOLEN:                          /* SOLEN::Pass, cpp:9688-9692; KS = 0: DRIFT_Pass, cpp:5811-5814 */
+        if (e->ks != 0.) return -1;
+        GtoL(x, e->l, 0., MIS(e));
+        DRIFT_Pass(x, e->l);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
     default:
         return -1;
     }
@@ -1931,6 +1964,9 @@ static int elem_spass(const st_element *e, double x[])
         return 0;
     case ST_RFCAV: case ST_CCMULT: case ST_BEAMBEAM: case ST_CRBB: case ST_MATRIX:
         return elem_pass(e, x, NULL);       /* sPass == Pass (orbit only) */
+    case ST_PATCH:                          /* PATCH::sPass, cpp:12331-12333: the orbit map */
+    case ST_SOLEN:                          /* SOLEN::sPass, cpp:9700-9704; KS = 0: DRIFT_Pass, cpp:8125-8128 */
+        return elem_pass(e, x, NULL);
     default:
         return -1;
     }

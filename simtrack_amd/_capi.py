@@ -28,6 +28,7 @@ ELEMENT_DTYPE = np.dtype([
     ("rot_n", "<f8", (3,)), ("rot_angle", "<f8"),
     ("m66", "<f8", (36,)),
     ("xco_in", "<f8", (6,)), ("xco_out", "<f8", (6,)),
+    ("ks", "<f8"),
 ])
 
 
@@ -51,7 +52,7 @@ TYPES = {
     "MULT": 8, "GMULT": 9, "SBENDMULT": 10, "SMULT": 11, "GSBENDMULT": 12,
     "KICKER": 13, "HKICKER": 14, "VKICKER": 15, "MARKER": 16, "BPM": 17, "HBPM": 18,
     "VBPM": 19, "RFCAV": 20, "CRABRF": 21, "CCMULT": 22, "LBT": 23, "ILBT": 24,
-    "BEAMBEAM": 25, "CRBB": 26, "MATRIX": 27, "ROTAT": 28, "SNAKE": 29,
+    "BEAMBEAM": 25, "CRBB": 26, "MATRIX": 27, "ROTAT": 28, "SNAKE": 29, "PATCH": 30, "SOLEN": 31,
 }
 TRACK_SPIN = 1
 TRACK_RADIAL = 2

@@ -879,6 +879,29 @@ struct Builder {
             t.ops.push_back(op);
             return 0;
         }
+        case ST_PATCH: {
+            /* PATCH::Pass / sPass = PATCH_Pass (cpp:12323-12333, 6248-6256):
+               TRANS_Pass by DX, DY, DS, then YROTAT_Pass by THETA (skipped
+               for |THETA| < 1e-9, cpp:6195); no GtoL/LtoG */
+            st_op op = new_op(K_PATCH);
+            put(e.dx);
+            put(e.dy);
+            put(e.ds);
+            put(std::fabs(e.theta) < 1.0e-9 ? 0. : 1.);
+            put(st_cos(e.theta));
+            put(st_sin(e.theta));
+            put(st_tan(e.theta));
+            t.ops.push_back(op);
+            return 0;
+        }
+        case ST_SOLEN:
+            /* SOLEN::Pass / sPass (cpp:9688-9704): GtoL, SOLEN_Pass, LtoG; with
+               KS = 0 SOLEN_Pass and SOLEN_sPass are DRIFT_Pass (cpp:5811, 8125) */
+            if (e.ks != 0.) {
+                err = "SOLEN with KS != 0 is not supported (only KS = 0, a drift)";
+                return -2;
+            }
+            return drift_op(e.l, &e, e.l, 0.);
         case ST_MATRIX: {
             st_op op = new_op(K_MATRIX);
             for (int i = 0; i < 6; i++) put(e.xco_in[i]);

@@ -574,6 +574,21 @@ ST_DEV void apply_op(St &s, Ptc &c, const st_op &op, st_pp par, const st_kglobal
         drift<HEXP>(s, P[0], P[1], c);
         break;
     }
+    case K_PATCH: /* PATCH_Pass, cpp:6248-6256: TRANS_Pass (cpp:6140) then YROTAT_Pass (cpp:6192) */
+        x[0] = x[0] - P[0];
+        x[2] = x[2] - P[1];
+        x[4] = x[4] - P[2];
+        if (P[3] != 0.) {
+            const double cosT = P[4], sinT = P[5], tanT = P[6];
+            const double x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3], x4 = x[4];
+            double pz = st_fsqrt(c.opd2 - x1 * x1 - x3 * x3);
+            double temp = 1 - x1 * tanT / pz;
+            x[0] = x0 / cosT / temp;
+            x[1] = x1 * cosT + pz * sinT;
+            x[2] = x2 + x3 * x0 * tanT / pz / temp;
+            x[4] = x4 - c.opd * x0 * tanT / pz / temp / c.b1;
+        }
+        break;
     default:
         break;
     }

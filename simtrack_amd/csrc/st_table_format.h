@@ -57,6 +57,7 @@ enum st_kind {
     K_BB6DA,          /* see bb6d_angle()                                         */
     K_MATRIX,         /* 0..5 XCO_IN, 6..41 M66, 42..47 XCO_OUT                   */
     K_ROTAT,          /* 0 L/2, 1 Lb, 2..10 rotation matrix                       */
+    K_PATCH,          /* 0 DX,1 DY,2 DS, 3 rotate (1/0), 4 cos,5 sin,6 tan of THETA */
     K_COUNT
 };
 

@@ -142,6 +142,11 @@ def element_ops(r, gp, spin=False):
         return 60 + int(r["nslice"]) * (97 + 2 * 150 + 20) + PT_UPDATE + mis
     if name == "MATRIX":
         return 6 + 72 + 6 + PT_UPDATE
+    if name == "PATCH":
+        # TRANS_Pass 3; YROTAT_Pass: pz 5 (sqrt), temp 3, x 2, px 3, y 5, z 7
+        return 3 + (0 if abs(r["theta"]) < 1e-9 else 25)
+    if name == "SOLEN":
+        return drift_l(L) + mis
     return 0
 
 

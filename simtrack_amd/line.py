@@ -440,6 +440,40 @@ class SNAKE(ROTAT):
     TYPE = "SNAKE"
 
 
+class PATCH(Element):
+    """PATCH (simtrack.cpp:12262-12334): a frame translation by DX, DY, DS
+    and a rotation about y by THETA.  As in the reference the offsets live in
+    the element's DX, DY, DS (which are not misalignments here) and L = DS."""
+    TYPE = "PATCH"
+    _params = {"DX": "DX", "DY": "DY", "DS": "DS", "THETA": "THETA"}
+
+    def __init__(self, name, dx, dy, ds, theta):
+        super().__init__(name)
+        self.L = float(ds)
+        self.DX, self.DY, self.DS, self.THETA = float(dx), float(dy), float(ds), float(theta)
+
+    def _fill(self, r):
+        r["theta"] = self.THETA
+
+
+class SOLEN(Element):
+    """SOLEN (simtrack.cpp:9645-9704), KS = Bz / brho.  The engine tracks
+    KS = 0 (SOLEN_Pass is then DRIFT_Pass inside GtoL/LtoG); a table with a
+    non-zero KS is rejected when it is built."""
+    TYPE = "SOLEN"
+    _params = {"KS": "KS"}
+
+    def __init__(self, name, l, ks):
+        super().__init__(name)
+        _check_len(l, strict=True)
+        self.L, self.KS = float(l), float(ks)
+        self.Nint = 1
+
+    def _fill(self, r):
+        r["ks"] = self.KS
+        r["nint"] = self.Nint
+
+
 # -------------------------------------------------------------------- line
 
 class Line:

@@ -69,12 +69,14 @@ def all_types_line(seed=7, misalign=True, with_bb=True, with_rf=True, with_gsben
                 L.BEAMBEAM("BB4", 4, 1.5e11, 1.0, 0.1, 1, 4 * eps, 0.5, 0., eps, 0.5, 0.),
                 L.CRBB("CRB", 1.5e11, 0.01, 1.0, 0.05, 5, 2 * eps, 0.4, 0.02, eps, 0.2, 0.)]
     els += [L.HBPM("HB", 0.2), L.VBPM("VB", 0.)]
+    els += [L.PATCH("PT", -0.012, 0.001, 3e-4, 0.021), L.PATCH("PT0", 0.003, 0., -1e-4, 0.),
+            L.SOLEN("SOL", 1.1, 0.)]
     for e in els:
         if hasattr(e, "Nint") and e.L > 0:
             e.Nint = max(1, min(e.Nint, 5))
     if misalign:
         for i, e in enumerate(els):
-            if e.TYPE in ("DRIFT", "MARKER", "BPM", "HBPM", "VBPM", "MATRIX", "ROTAT", "SNAKE"):
+            if e.TYPE in ("DRIFT", "MARKER", "BPM", "HBPM", "VBPM", "MATRIX", "ROTAT", "SNAKE", "PATCH"):
                 continue
             k = i % 4
             if k == 1:
@@ -631,3 +633,32 @@ def _bunch_length(opt, gp, ln, sigma_delta):
     """Equilibrium rms bunch length of the linearised RF: alpha_c C sigma_delta / (2 pi Qs)."""
     _, qs = rf_synchronous(opt, gp, ln)
     return opt.alpha_c * opt.length * sigma_delta / (2 * math.pi * qs)
+
+
+# ----------------------------------------- the reference's example lattices
+
+def rhic_example(path, rf_volts_mv=4.0):
+    """The RHIC set-up of the reference's first example driver: Au ions at
+    gamma = 100 (Set_RefPartEnergy(100, 938.27201*197, 79, 197)), the MAD-X
+    table read by Read_MADX_Lattice, and the first RFCAV set to VRF = 4 MV,
+    FRF = 78250.42279 * harm with GP.harm = 360.  Returns (globals, line)."""
+    from . import lattice_io
+    gp = L.make_globals(100., 938.27201 * 197, 79, 197)
+    ln = lattice_io.read_madx_twiss(path)
+    rf = [e for e in ln.Cell if e.TYPE == "RFCAV"][0]
+    rf.VRF, rf.FRF = rf_volts_mv, 78250.42279 * 360
+    return gp, ln
+
+
+def hsr_example(path):
+    """The EIC hadron ring of the reference's second example driver:
+    275 GeV protons, the Bmad table read by Read_BMAD_Lattice, and its two
+    cavities set as the driver's (6 MV at h = 2520 and 20 MV at h = 7560 of
+    c / length).  Returns (globals, line)."""
+    from . import lattice_io
+    gp = proton_globals(275000.)
+    ln = lattice_io.read_bmad_table(path)
+    cav = [e for e in ln.Cell if e.TYPE == "RFCAV"]
+    for e, (v, h) in zip(cav, ((6., 2520), (20., 7560))):
+        e.VRF, e.FRF = v, h * 299792458. / ln.Length
+    return gp, ln
