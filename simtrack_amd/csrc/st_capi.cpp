@@ -1106,7 +1106,8 @@ static int validate(const st_element &e, int j)
     default:
         break;
     }
-    if (e.l < 0.) return fail(ST_ERR_PARAM, "element %d: negative length", j);
+    /* a PATCH's L is its DS, which may be negative (cpp:12267); no map uses it */
+    if (e.l < 0. && e.type != ST_PATCH) return fail(ST_ERR_PARAM, "element %d: negative length", j);
     return ST_OK;
 }
 
