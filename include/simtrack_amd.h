@@ -217,11 +217,14 @@ int st_element_pass(st_table *t, int elem_index, double *x, int np, int spin);
  * d_x is np x 6 (x 9 with spin), d_state is np ints holding stable, d_lost
  * np x 2 ints (lost_turn, lost_post).  Turns turn0 .. turn0+nturn-1 are
  * tracked; the 0.1 m pre-check is applied only when turn0 == 0, so a run
- * split into ranges equals one call.  flags: ST_TRACK_SPIN, ST_TRACK_RADIAL.
+ * split into ranges equals one call.  flags: ST_TRACK_SPIN, ST_TRACK_RADIAL,
+ * ST_TRACK_BARE (no pre-check and no loss test: Element::Pass turn after
+ * turn, as Cal_Orbit_Num, simtrack.cpp:14970-14982, passes particles).
  * Returns after launching on the context stream; st_ctx_sync waits.
  */
 #define ST_TRACK_SPIN 1
 #define ST_TRACK_RADIAL 2
+#define ST_TRACK_BARE 4
 int st_track_device(st_table *t, double *d_x, int np, int turn0, int nturn,
                     int *d_state, int *d_lost, int flags);
 int st_ctx_sync(st_ctx *ctx);
@@ -374,6 +377,42 @@ int st_fine_tune(st_ctx *ctx, const double *x, const double *px, int n, int nsig
  */
 int st_fine_tune_device(st_ctx *ctx, const double *d_x, const double *d_px, int64_t stride_sig, int64_t stride,
                         int n, int nsig, double tune_lower, double tune_upper, double *d_tune);
+
+/* ------------------------------------------- closed orbit and TBT scans */
+
+/*
+ * Cal_Orbit_Num (simtrack.cpp:14947-15030): the 4-D closed orbit at
+ * deltap (x[5]) of the line `rec` with its RF, crab and CCMULT voltages set
+ * to zero, by the reference's Newton iteration (finite differences 1e-9,
+ * tolerance 1e-12, at most 20 iterations, its mat_inv).  x_co: the fixed
+ * point (x[4] = 0, x[5] = deltap); x_end (may be null): x_co passed once
+ * more through the line, the Cell[Ncell-1]->X the drivers read.
+ * iterations (may be null): iterations used.  ST_ERR_PARAM when the
+ * reference would report failure.
+ */
+int st_closed_orbit(st_ctx *ctx, const st_globals *gp, const st_element *rec, int n, double deltap,
+                    double x_co[6], double x_end[6], int *iterations);
+
+/*
+ * Track_tbt_FMA (simtrack.cpp:21179-21241) with its grid and turns as
+ * arguments (the reference: nturn 2048, sigma_step 0.1, sigma_max 6):
+ * rows of 8 doubles (i*step, j*step, tunex1, tunex2, tuney1, tuney2,
+ * tunex1-tunex2, tuney1-tuney2) for the stable grid particles, in the
+ * reference's order; x_end[0..3] is the closed orbit added to the grid.
+ * *nrows receives the row count (rows may be null to ask for it).
+ */
+int st_fma_scan(st_table *t, double deltap0, double sigmax0, double sigmay0, const double x_end[4],
+                double tune_low, double tune_high, int nturn, double sigma_step, double sigma_max,
+                double *rows, int max_rows, int *nrows);
+
+/*
+ * Track_tbt_Lyapunov (simtrack.cpp:21243-21303) with its grid and turns
+ * as arguments (the reference: nturn 2048, sigma_step 0.05, sigma_max 6):
+ * rows of 3 doubles (i*step, j*step, mean of the last 25 log(d/1e-6)/(k+1))
+ * for the grid points whose twin particles both survive.
+ */
+int st_lyapunov_scan(st_table *t, double deltap0, double sigmax0, double sigmay0, int nturn, double sigma_step,
+                     double sigma_max, double *rows, int max_rows, int *nrows);
 
 #ifdef __cplusplus
 }

@@ -39,6 +39,9 @@ def load(glibc=False):
     lib.orc_bb4d.argtypes = [D, D, D, D, D, D, P, P]
     lib.orc_bb_slice_centers.argtypes = [I, D, P]
     lib.orc_fine_tune.argtypes = [I, P, D, D, P]
+    lib.orc_closed_orbit.argtypes = [P, I, D, P, P, ctypes.POINTER(I)]
+    lib.orc_fma_scan.argtypes = [P, I, D, D, D, P, D, D, I, D, D, P, I]
+    lib.orc_lyapunov_scan.argtypes = [P, I, D, D, D, I, D, D, P, I]
     lib.orc_fine_tune_xpx.argtypes = [I, P, P, D, D, P]
     for f in ("orc_sin", "orc_cos", "orc_tan", "orc_asin", "orc_exp", "orc_log", "orc_ugaussian_pinv"):
         getattr(lib, f).argtypes = [D]
@@ -100,3 +103,35 @@ class Oracle:
             if rc != 0:
                 raise RuntimeError(f"oracle pass failed ({rc})")
         return x
+
+    def closed_orbit(self, deltap):
+        """Cal_Orbit_Num (simtrack.cpp:14947-15030): (x_co, x_end, iterations)."""
+        self._set()
+        xc, xe, it = np.zeros(6), np.zeros(6), ctypes.c_int()
+        rc = self.lib.orc_closed_orbit(_p(self.records), len(self.records), deltap, _p(xc), _p(xe), ctypes.byref(it))
+        if rc != 0:
+            raise RuntimeError(f"oracle closed orbit failed ({rc})")
+        return xc, xe, it.value
+
+    def fma_scan(self, deltap0, sigmax0, sigmay0, x_end, tune_low, tune_high, nturn, sigma_step, sigma_max):
+        """Track_tbt_FMA (simtrack.cpp:21179-21241) one particle at a time: rows of 8."""
+        self._set()
+        cap = (int(sigma_max / sigma_step) + 2) ** 2
+        rows = np.zeros((cap, 8))
+        xe = np.ascontiguousarray(x_end[:4], dtype=np.float64)
+        m = self.lib.orc_fma_scan(_p(self.records), len(self.records), deltap0, sigmax0, sigmay0, _p(xe), tune_low,
+                                  tune_high, nturn, sigma_step, sigma_max, _p(rows), cap)
+        if m < 0:
+            raise RuntimeError(f"oracle fma scan failed ({m})")
+        return rows[:m]
+
+    def lyapunov_scan(self, deltap0, sigmax0, sigmay0, nturn, sigma_step, sigma_max):
+        """Track_tbt_Lyapunov (simtrack.cpp:21243-21303) one pair at a time: rows of 3."""
+        self._set()
+        cap = (int(sigma_max / sigma_step) + 2) ** 2
+        rows = np.zeros((cap, 3))
+        m = self.lib.orc_lyapunov_scan(_p(self.records), len(self.records), deltap0, sigmax0, sigmay0, nturn,
+                                       sigma_step, sigma_max, _p(rows), cap)
+        if m < 0:
+            raise RuntimeError(f"oracle lyapunov scan failed ({m})")
+        return rows[:m]

@@ -2177,3 +2177,271 @@ EXPORT void orc_philox(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t
     c = st_philox4x32_10(c, k0, k1);
     for (int i = 0; i < 4; i++) out[i] = c.v[i];
 }
+
+/* ------------------------------------------------- closed orbit, TBT scans */
+
+/* mat_inv, simtrack.cpp:1440-1496 */
+static int mat_inv(double a[], int n)
+{
+    int is[16], js[16], i, j, k, l, u, v;
+    double d, p;
+    for (k = 0; k <= n - 1; k++) {
+        d = 0.0;
+        for (i = k; i <= n - 1; i++)
+            for (j = k; j <= n - 1; j++) {
+                l = i * n + j;
+                p = fabs(a[l]);
+                if (p > d) {
+                    d = p;
+                    is[k] = i;
+                    js[k] = j;
+                }
+            }
+        if (d + 1.0 == 1.0) return 0;
+        if (is[k] != k)
+            for (j = 0; j <= n - 1; j++) {
+                u = k * n + j;
+                v = is[k] * n + j;
+                p = a[u];
+                a[u] = a[v];
+                a[v] = p;
+            }
+        if (js[k] != k)
+            for (i = 0; i <= n - 1; i++) {
+                u = i * n + k;
+                v = i * n + js[k];
+                p = a[u];
+                a[u] = a[v];
+                a[v] = p;
+            }
+        l = k * n + k;
+        a[l] = 1.0 / a[l];
+        for (j = 0; j <= n - 1; j++)
+            if (j != k) {
+                u = k * n + j;
+                a[u] = a[u] * a[l];
+            }
+        for (i = 0; i <= n - 1; i++)
+            if (i != k)
+                for (j = 0; j <= n - 1; j++)
+                    if (j != k) {
+                        u = i * n + j;
+                        a[u] = a[u] - a[i * n + k] * a[k * n + j];
+                    }
+        for (i = 0; i <= n - 1; i++)
+            if (i != k) {
+                u = i * n + k;
+                a[u] = -a[u] * a[l];
+            }
+    }
+    for (k = n - 1; k >= 0; k--) {
+        if (js[k] != k)
+            for (j = 0; j <= n - 1; j++) {
+                u = k * n + j;
+                v = js[k] * n + j;
+                p = a[u];
+                a[u] = a[v];
+                a[v] = p;
+            }
+        if (is[k] != k)
+            for (i = 0; i <= n - 1; i++) {
+                u = i * n + k;
+                v = i * n + is[k];
+                p = a[u];
+                a[u] = a[v];
+                a[v] = p;
+            }
+    }
+    return 1;
+}
+
+/* one bare turn (Element::Pass of every element, no loss test) of particle pid */
+static int pass_turn(const st_element *line, int n, double x[], uint32_t pid, uint64_t draws)
+{
+    orc_rng rng = {GP.rng_seed, pid, 0u};
+    (void)draws;
+    for (int j = 0; j < n; j++) {
+        int rc = elem_pass(&line[j], x, &rng);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+/* Cal_Orbit_Num, simtrack.cpp:14947-15030, on a copy of the line with the
+   RF, crab and CCMULT voltages zeroed.  Returns 0, or -1 where the
+   reference reports failure.  x_end: x_co passed once more. */
+EXPORT int orc_closed_orbit(const st_element *line0, int n, double deltap, double x_co[6], double x_end[6],
+                            int *iterations)
+{
+    st_element *line = (st_element *)malloc(sizeof(st_element) * (size_t)(n > 0 ? n : 1));
+    if (!line) return -5;
+    memcpy(line, line0, sizeof(st_element) * (size_t)n);
+    for (int i = 0; i < n; i++)
+        if (line[i].type == ST_RFCAV || line[i].type == ST_CRABRF || line[i].type == ST_CCMULT) line[i].vrf = 0.;
+    int i, j, k, iter = 0, rc = 0;
+    double x0[6], x01[6], x1[6], dx[4], d = 1.0e-09, mat[4][4], chi, codeps = 1e-12;
+    int Max_iter = 20;
+    for (i = 0; i < 6; i++) x0[i] = 0.;
+    do {
+        iter++;
+        x0[4] = 0;
+        x0[5] = deltap;
+        for (j = 0; j < 6; j++) x01[j] = x0[j];
+        if ((rc = pass_turn(line, n, x01, 0, 0))) goto out;
+        for (k = 0; k < 4; k++) {
+            for (j = 0; j < 6; j++) x1[j] = x0[j];
+            x1[k] = x1[k] + d;
+            if ((rc = pass_turn(line, n, x1, (uint32_t)(k + 1), 0))) goto out;
+            for (j = 0; j < 4; j++) mat[j][k] = (x1[j] - x01[j]) / d;
+        }
+        for (j = 0; j < 6; j++) x1[j] = x0[j];
+        if ((rc = pass_turn(line, n, x1, 0, 0))) goto out;
+        for (i = 0; i < 4; i++) dx[i] = x0[i] - x1[i];
+        chi = 0;
+        for (i = 0; i < 4; i++) chi += dx[i] * dx[i];
+        chi = sqrt(chi / 4.);
+        if (chi > codeps) {
+            for (i = 0; i < 4; i++) mat[i][i] = mat[i][i] - 1.0000001;
+            if (mat_inv(&mat[0][0], 4) == 0) {
+                rc = -1;
+                goto out;
+            }
+            for (i = 0; i < 4; i++)
+                for (j = 0; j < 4; j++) x0[i] += mat[i][j] * dx[j];
+        }
+    } while (chi > codeps && iter < Max_iter);
+    if (iterations) *iterations = iter;
+    if (iter == Max_iter - 1) {
+        rc = -1;
+        goto out;
+    }
+    x0[4] = 0.000;
+    x0[5] = deltap;
+    for (i = 0; i < 6; i++) x_co[i] = x0[i];
+    if (x_end) {
+        for (i = 0; i < 6; i++) x_end[i] = x0[i];
+        rc = pass_turn(line, n, x_end, 0, 0);
+    }
+out:
+    free(line);
+    return rc;
+}
+
+/* Track_tbt_FMA, simtrack.cpp:21179-21241, one particle after another;
+   rows as st_fma_scan.  Returns the number of rows, or a negative code. */
+EXPORT int orc_fma_scan(const st_element *line, int n, double deltap0, double sigmax0, double sigmay0,
+                        const double x_end[4], double tune_low, double tune_high, int nturn, double sigma_step,
+                        double sigma_max, double *rows, int max_rows)
+{
+    int nsigma = (int)((sigma_max - 0) * 1.0 / sigma_step) + 1, m = 0, w = nturn / 2, q = 0;
+    double *x_tbt = (double *)malloc(sizeof(double) * (size_t)nturn * 6);
+    double *xtemp = (double *)malloc(sizeof(double) * (size_t)w);
+    uint64_t draws = orc_draws_per_turn(line, n, 0);
+    if (!x_tbt || !xtemp) return -5;
+    for (int i = 0; i < nsigma; i++)
+        for (int j = 0; j < nsigma; j++) {
+            if (!(sqrt(1.0 * i * i + 1.0 * j * j) <= (sigma_max / sigma_step))) continue;
+            double x[6];
+            x[0] = i * sigma_step * sigmax0 + 0.;
+            x[1] = 0.;
+            x[2] = j * sigma_step * sigmay0;
+            x[3] = 0.;
+            x[4] = 0.;
+            x[5] = deltap0;
+            x[0] = x[0] + x_end[0];
+            x[1] = x[1] + x_end[1];
+            x[2] = x[2] + x_end[2];
+            x[3] = x[3] + x_end[3];
+            if (x[0] < 1.0e-09) x[0] = x[0] + 1.0e-09;
+            if (x[2] < 1.0e-09) x[2] = x[2] + 1.0e-09;
+            int stable = 1, lt = 0, lp = 0;
+            /* the GPU scan numbers the grid particles 0, 1, ...; radiation draws depend on it */
+            int rc = track_one(line, n, x, nturn, &stable, &lt, &lp, 0, NULL, x_tbt, -1, (uint32_t)q++, draws);
+            if (rc) return rc;
+            if (stable != 1) continue;
+            double t[4];
+            for (int k = 0; k < w; k++) xtemp[k] = x_tbt[k * 6 + 0];
+            orc_fine_tune(w, xtemp, tune_low, tune_high, &t[0]);
+            for (int k = 0; k < w; k++) xtemp[k] = x_tbt[k * 6 + 2];
+            orc_fine_tune(w, xtemp, tune_low, tune_high, &t[2]);
+            for (int k = 0; k < w; k++) xtemp[k] = x_tbt[(k + w - 1) * 6 + 0];
+            orc_fine_tune(w, xtemp, tune_low, tune_high, &t[1]);
+            for (int k = 0; k < w; k++) xtemp[k] = x_tbt[(k + w - 1) * 6 + 2];
+            orc_fine_tune(w, xtemp, tune_low, tune_high, &t[3]);
+            if (m < max_rows) {
+                double *o = rows + (size_t)m * 8;
+                o[0] = i * sigma_step;
+                o[1] = j * sigma_step;
+                o[2] = t[0];
+                o[3] = t[1];
+                o[4] = t[2];
+                o[5] = t[3];
+                o[6] = t[0] - t[1];
+                o[7] = t[2] - t[3];
+            }
+            m++;
+        }
+    free(x_tbt);
+    free(xtemp);
+    return m;
+}
+
+/* Track_tbt_Lyapunov, simtrack.cpp:21243-21303, one pair after another;
+   rows as st_lyapunov_scan.  Returns the number of rows. */
+EXPORT int orc_lyapunov_scan(const st_element *line, int n, double deltap0, double sigmax0, double sigmay0, int nturn,
+                             double sigma_step, double sigma_max, double *rows, int max_rows)
+{
+    int nsigma = (int)((sigma_max - 0) * 1.0 / sigma_step), m = 0, q = 0;
+    double distance0 = 1.0e-06;
+    double *x1_tbt = (double *)malloc(sizeof(double) * (size_t)nturn * 6);
+    double *x2_tbt = (double *)malloc(sizeof(double) * (size_t)nturn * 6);
+    double *Lay = (double *)malloc(sizeof(double) * (size_t)nturn);
+    uint64_t draws = orc_draws_per_turn(line, n, 0);
+    if (!x1_tbt || !x2_tbt || !Lay) return -5;
+    for (int i = 0; i < nsigma; i++)
+        for (int j = 0; j < nsigma; j++) {
+            if (!(sqrt(1.0 * i * i + 1.0 * j * j) <= (sigma_max / sigma_step))) continue;
+            double x1[6], x2[6];
+            x1[0] = i * sigma_step * sigmax0;
+            x1[1] = 0.;
+            x1[2] = j * sigma_step * sigmay0;
+            x1[3] = 0.;
+            x1[4] = 0.;
+            x1[5] = deltap0;
+            if (x1[0] == 0.) x1[0] = x1[0] + 1.0e-09;
+            if (x1[2] == 0.) x1[2] = x1[2] + 1.0e-09;
+            x2[0] = x1[0] + 0.707e-06;
+            x2[1] = 0.;
+            x2[2] = x1[2] + 0.707e-06;
+            x2[3] = 0.;
+            x2[4] = 0.;
+            x2[5] = deltap0;
+            int s1 = 1, s2 = 1, lt = 0, lp = 0;
+            /* the GPU scan numbers the pair's particles 2q and 2q+1; radiation draws depend on it */
+            int rc = track_one(line, n, x1, nturn, &s1, &lt, &lp, 0, NULL, x1_tbt, -1, (uint32_t)(2 * q), draws);
+            if (!rc) rc = track_one(line, n, x2, nturn, &s2, &lt, &lp, 0, NULL, x2_tbt, -1, (uint32_t)(2 * q + 1), draws);
+            q++;
+            if (rc) return rc;
+            if (s1 * s2 != 1) continue;
+            for (int k = 0; k < nturn; k++) {
+                double dd = sqrt((x1_tbt[k * 6 + 0] - x2_tbt[k * 6 + 0]) * (x1_tbt[k * 6 + 0] - x2_tbt[k * 6 + 0]) +
+                                 (x1_tbt[k * 6 + 1] - x2_tbt[k * 6 + 1]) * (x1_tbt[k * 6 + 1] - x2_tbt[k * 6 + 1]) +
+                                 (x1_tbt[k * 6 + 2] - x2_tbt[k * 6 + 2]) * (x1_tbt[k * 6 + 2] - x2_tbt[k * 6 + 2]) +
+                                 (x1_tbt[k * 6 + 3] - x2_tbt[k * 6 + 3]) * (x1_tbt[k * 6 + 3] - x2_tbt[k * 6 + 3]));
+                Lay[k] = st_log(dd / distance0) / (k + 1);
+            }
+            double sum = 0.0;
+            for (int k = 0; k < 25; k++) sum = sum + Lay[nturn - 1 - k];
+            if (m < max_rows) {
+                double *o = rows + (size_t)m * 3;
+                o[0] = i * sigma_step;
+                o[1] = j * sigma_step;
+                o[2] = sum / 25;
+            }
+            m++;
+        }
+    free(x1_tbt);
+    free(x2_tbt);
+    free(Lay);
+    return m;
+}

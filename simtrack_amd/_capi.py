@@ -56,6 +56,7 @@ TYPES = {
 }
 TRACK_SPIN = 1
 TRACK_RADIAL = 2
+TRACK_BARE = 4
 
 # every symbol declared in include/simtrack_amd.h
 EXPORTS = [
@@ -67,7 +68,8 @@ EXPORTS = [
     "st_fine_tune", "st_fine_tune_device", "st_batch_create", "st_batch_destroy", "st_batch_default_config",
     "st_batch_track", "st_batch_track_device", "st_batch_autotune", "st_mlp_create", "st_mlp_destroy",
     "st_mlp_num_params", "st_mlp_get_params", "st_mlp_set_params", "st_mlp_forward", "st_mlp_gradients",
-    "st_mlp_train_step", "st_batch_track_tbt_device", "st_surrogate_features",
+    "st_mlp_train_step", "st_batch_track_tbt_device", "st_surrogate_features", "st_closed_orbit", "st_fma_scan",
+    "st_lyapunov_scan",
 ]
 
 
@@ -144,6 +146,9 @@ def load(path=None):
     lib.st_batch_autotune.argtypes = [P, P, P, I, I, BC, P, ctypes.POINTER(I)]
     lib.st_batch_track_tbt_device.argtypes = [P, P, P, I, I, P, P, P, I, BC]
     lib.st_surrogate_features.argtypes = [P, P, I, P, I, D, D, D, D, P]
+    lib.st_closed_orbit.argtypes = [P, ctypes.POINTER(Globals), P, I, D, P, P, ctypes.POINTER(I)]
+    lib.st_fma_scan.argtypes = [P, D, D, D, P, D, D, I, D, D, P, I, ctypes.POINTER(I)]
+    lib.st_lyapunov_scan.argtypes = [P, D, D, D, I, D, D, P, I, ctypes.POINTER(I)]
     F = ctypes.c_float
     lib.st_mlp_create.argtypes = [P, I, I, ctypes.c_uint64, ctypes.POINTER(P)]
     lib.st_mlp_destroy.argtypes = [P]

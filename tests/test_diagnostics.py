@@ -1,0 +1,118 @@
+"""Closed orbit and turn-by-turn scans on the oracle (CPU), pinned by linear
+optics: the closed orbit of a ring with a dipole kick equals the linear
+response (I - M)^-1 of the one-turn matrix, and its momentum dependence is
+the dispersion MAD-X reports; small-amplitude FMA tunes are the linear
+tunes; a linear lattice has no positive Lyapunov exponent.  The writers
+reproduce the reference's file lines."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle
+import simtrack_amd as st
+from simtrack_amd import diagnostics as D
+from simtrack_amd import lattice_io as LI
+from simtrack_amd import optics as O
+from simtrack_amd import workloads as W
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+RHIC = os.path.join(HERE, "golden", "lattices", "rhic_2012_madx_twiss.tfs.gz")
+
+
+def _fodo(kick=0.):
+    """24 thick-quad FODO cells with sector bends, no sextupoles; one
+    horizontal kicker after the first cell."""
+    gp = W.proton_globals(100000.)
+    cells = []
+    ang = 2 * math.pi / 48
+    for c in range(24):
+        cells += [st.QUAD("QF", 1.0, 0.025, 0.), st.DRIFT("D", 2.0), st.SBEND("B", 10.0, ang),
+                  st.DRIFT("D", 2.0), st.QUAD("QD", 1.0, -0.025, 0.), st.DRIFT("D", 2.0),
+                  st.SBEND("B", 10.0, ang), st.DRIFT("D", 2.0)]
+        if c == 0:
+            cells.append(st.HKICKER("HK", 0., kick))
+    for e in cells:
+        if e.TYPE == "QUAD":
+            e.Nint = 8
+    return gp, st.Line(cells)
+
+
+def test_closed_orbit_of_a_kick_is_the_linear_response():
+    theta = 2e-6
+    gp, ln = _fodo(theta)
+    orc = pyoracle.Oracle(gp, ln.records(), nthreads=1)
+    xc, xe, it = orc.closed_orbit(0.)
+    assert 1 <= it < 19
+    # fixed point: one more turn returns it
+    assert np.abs(xe[:4] - xc[:4]).max() < 1e-12
+    # linear response: x_co = (I - M)^-1 M_(kick -> end) (0, theta)
+    k = [i for i, e in enumerate(ln.Cell) if e.TYPE == "HKICKER"][0]
+    m_after = np.eye(3)
+    for e in ln.Cell[k + 1:]:
+        m_after = O.element_matrices(e)[0] @ m_after
+    opt = O.Optics(ln)
+    rhs = (m_after @ np.array([0., theta, 0.]))[:2]
+    lin = np.linalg.solve(np.eye(2) - opt.mx[:2, :2], rhs)
+    assert np.allclose(xc[:2], lin, rtol=2e-3, atol=1e-12), (xc[:2], lin)
+    assert abs(xc[2]) < 1e-15 and abs(xc[3]) < 1e-15
+
+
+def test_closed_orbit_momentum_dependence_is_the_dispersion():
+    gp, ln = W.rhic_example(RHIC)
+    orc = pyoracle.Oracle(gp, ln.records(), nthreads=1)
+    opt = O.Optics(ln)
+    d = 1e-5                       # x[5] is pt; at gamma = 100, pt = beta * delta to 1e-4
+    xp, _, _ = orc.closed_orbit(d)
+    xm, _, _ = orc.closed_orbit(-d)
+    # the symmetric difference removes the second-order dispersion (~5 m here)
+    assert abs((xp[0] - xm[0]) / (2 * d / gp.beta) / opt.D - 1) < 1e-4
+    assert abs((xp[1] - xm[1]) / (2 * d / gp.beta) / opt.Dp - 1) < 1e-4
+    xc0, _, _ = orc.closed_orbit(0.)
+    assert np.abs(xc0[:4]).max() < 1e-12
+
+
+def test_fma_tunes_at_small_amplitude_are_the_linear_tunes():
+    gp, ln = _fodo()
+    opt = O.Optics(ln)
+    orc = pyoracle.Oracle(gp, ln.records(), nthreads=1)
+    eps, _ = W.beam_sizes(gp)
+    sx, sy = math.sqrt(eps * opt.beta_x), math.sqrt(eps * opt.beta_y)
+    qx, qy = opt.tune_x, opt.tune_y
+    lo, hi = min(qx, qy) - 0.02, max(qx, qy) + 0.02
+    rows = orc.fma_scan(0., sx, sy, np.zeros(4), lo, hi, 256, 1.0, 2.0)
+    # grid i, j in 0..2 with i^2 + j^2 <= 4: 6 points, all stable in a linear ring
+    assert [tuple(r[:2]) for r in rows] == [(0, 0), (0, 1), (0, 2), (1, 0), (1, 1), (2, 0)]
+    r = rows[4]
+    assert abs(r[2] - qx) < 2e-3 and abs(r[4] - qy) < 2e-3
+    assert abs(r[6]) < 1e-4 and abs(r[7]) < 1e-4        # no tune drift in a linear ring
+
+
+def test_lyapunov_of_a_linear_ring_is_not_positive():
+    gp, ln = _fodo()
+    orc = pyoracle.Oracle(gp, ln.records(), nthreads=1)
+    eps, _ = W.beam_sizes(gp)
+    rows = orc.lyapunov_scan(0., math.sqrt(eps * 20), math.sqrt(eps * 20), 200, 2.0, 6.0)
+    assert len(rows) == 9                       # nsigma = int(6/2) = 3: i, j in 0..2, all within radius 3
+    # bounded twin distance: log(d/d0)/k over the last turns stays tiny
+    assert np.abs(rows[:, 2]).max() < 0.01
+
+
+def test_writers_reproduce_the_reference_lines(tmp_path, monkeypatch):
+    rows8 = np.array([[0., 0.1, 0.23, 0.2301, 0.22, 0.2202, -1e-4, -2e-4],
+                      [0., 0.2, 0.23, 0.2301, 0.22, 0.2202, -1e-4, -2e-4]])
+    monkeypatch.setattr(D, "fma_scan", lambda *a, **k: rows8)
+    f = tmp_path / "FMA-output.dat"
+    D.Track_tbt_FMA(None, 0., 1., 1., np.zeros(4), filename=str(f))
+    lines = f.read_text().split("\n")
+    assert lines[0] == ("0 0.1 " + " ".join(f"{v:.18e}".rjust(25) for v in rows8[0, 2:6]) + "  "
+                        + " ".join(f"{v:.18e}".rjust(25) for v in rows8[0, 6:]))
+    assert lines[1].startswith("0.000000000000000000e+00 2.000000000000000111e-01  2.300000000000000100e-01")
+    rows3 = np.array([[0.05, 0., 1.5e-3], [0.1, 0.05, -2e-4]])
+    monkeypatch.setattr(D, "lyapunov_scan", lambda *a, **k: rows3)
+    g = tmp_path / "Lmap-output.dat"
+    D.Track_tbt_Lyapunov(None, 0., 1., 1., filename=str(g))
+    lines = g.read_text().split("\n")
+    assert lines[0] == "0.05  0" + f"{1.5e-3:.18e}".rjust(25)
+    assert lines[1] == "1.000000000000000056e-01  5.000000000000000278e-02" + f"{-2e-4:.18e}".rjust(25)
