@@ -12,6 +12,7 @@
  *   Track_Radial, Track_DA, Track_DA1, Track_DA_Radial,
  *   Track_DA_Uniform, Track_DA_Radial_Uniform
  *   Read_MADX_Lattice, Read_BMAD_Lattice               simtrack.cpp:13591, 14054
+ *   Cal_Orbit_Num, Track_tbt_FMA, Track_tbt_Lyapunov   simtrack.cpp:14947, 21179, 21243
  *
  * Element classes: DRIFT, MARKER, BPM, HBPM, VBPM, SBEND, GSBEND, QUAD,
  * SKEWQ, SEXT, OCT, MULT, GMULT, SBENDMULT, SMULT, GSBENDMULT, KICKER,
@@ -127,6 +128,7 @@ public:
     double DX = 0., DY = 0., DS = 0.;
     double DTHETA = 0., DPHI = 0., DPSI = 0.;
     double APx = 1., APy = 1.; /* Element::Element, simtrack.cpp:8394-8395 */
+    double X[6] = {0., 0., 0., 0., 0., 0.}; /* closed orbit after the element (Cal_Orbit_Num) */
 
 protected:
     struct Param {
@@ -1163,6 +1165,102 @@ inline void Track_DA_Radial_Uniform(Line &l, int nturn, double deltap0, double a
                                     double sigmay0, const char *filename)
 {
     detail::da_uniform_file(l, nturn, deltap0, angle, sigmax0, sigmay0, filename, 1.0, 150, true);
+}
+
+/* ----------------------------------------- closed orbit, TBT diagnostics */
+
+/* the reference's global tune search window, simtrack.cpp:27 */
+inline double tune_low = 0.660, tune_high = 0.70;
+
+/* Cal_Orbit_Num (simtrack.cpp:14947-15030): the closed orbit at deltap with
+   RF off; fills every element's X with the orbit after it, as the
+   reference does (cpp:15015-15018). */
+inline void Cal_Orbit_Num(Line &l, double deltap)
+{
+    std::vector<st_element> rec = records_from_line(l);
+    double xc[6], xe[6];
+    int it = 0;
+    st_ctx *c = default_context();
+    check(st_closed_orbit(c, &GP, rec.data(), (int)rec.size(), deltap, xc, xe, &it), "Cal_Orbit_Num");
+    for (auto &r : rec)
+        if (r.type == ST_RFCAV || r.type == ST_CRABRF || r.type == ST_CCMULT) r.vrf = 0.;
+    st_table *t = nullptr;
+    check(st_table_build(c, &GP, rec.data(), (int)rec.size(), &t), "st_table_build");
+    double x[6];
+    for (int k = 0; k < 6; ++k) x[k] = xc[k];
+    for (size_t j = 0; j < rec.size(); ++j) {
+        int rc = st_element_pass(t, (int)j, x, 1, 0);
+        if (rc) {
+            st_table_destroy(t);
+            check(rc, "Cal_Orbit_Num");
+        }
+        for (int k = 0; k < 6; ++k) l.Cell[j]->X[k] = x[k];
+    }
+    st_table_destroy(t);
+}
+
+namespace detail {
+/* the reference's fstream keeps scientific/precision(18) after the first
+   line it writes, so only that line has the default format (%g) */
+inline std::string amp(double v, bool first)
+{
+    char b[64];
+    std::snprintf(b, sizeof b, first ? "%g" : "%.18e", v);
+    return b;
+}
+inline std::string sci25(double v)
+{
+    char b[64];
+    std::snprintf(b, sizeof b, "%25.18e", v);
+    return b;
+}
+} // namespace detail
+
+/* Track_tbt_FMA (simtrack.cpp:21179-21241): Cal_Orbit_Num, then the tunes of
+   two 1024-turn windows of every stable particle of the 0-6 sigma grid,
+   written to ./FMA-output.dat. */
+inline void Track_tbt_FMA(Line &l, double deltap0, double sigmax0, double sigmay0)
+{
+    Cal_Orbit_Num(l, deltap0);
+    const double *xe = l.Cell.back()->X;
+    Tracker &tr = tracker_for(l);
+    int n = 0;
+    check(st_fma_scan(tr.table(), deltap0, sigmax0, sigmay0, xe, tune_low, tune_high, 2048, 0.1, 6., nullptr, 0, &n),
+          "Track_tbt_FMA");
+    std::vector<double> rows((size_t)(n > 0 ? n : 1) * 8);
+    check(st_fma_scan(tr.table(), deltap0, sigmax0, sigmay0, xe, tune_low, tune_high, 2048, 0.1, 6., rows.data(), n,
+                      &n),
+          "Track_tbt_FMA");
+    FILE *f = std::fopen("./FMA-output.dat", "w");
+    if (!f) throw Error("cannot open ./FMA-output.dat");
+    for (int m = 0; m < n; ++m) {
+        const double *r = &rows[(size_t)m * 8];
+        std::fprintf(f, "%s %s %s %s %s %s  %s %s\n", detail::amp(r[0], m == 0).c_str(),
+                     detail::amp(r[1], m == 0).c_str(), detail::sci25(r[2]).c_str(), detail::sci25(r[3]).c_str(),
+                     detail::sci25(r[4]).c_str(), detail::sci25(r[5]).c_str(), detail::sci25(r[6]).c_str(),
+                     detail::sci25(r[7]).c_str());
+    }
+    std::fclose(f);
+}
+
+/* Track_tbt_Lyapunov (simtrack.cpp:21243-21303): ./Lmap-output.dat */
+inline void Track_tbt_Lyapunov(Line &l, double deltap0, double sigmax0, double sigmay0)
+{
+    Tracker &tr = tracker_for(l);
+    int n = 0;
+    check(st_lyapunov_scan(tr.table(), deltap0, sigmax0, sigmay0, 2048, 0.05, 6., nullptr, 0, &n),
+          "Track_tbt_Lyapunov");
+    std::vector<double> rows((size_t)(n > 0 ? n : 1) * 3);
+    check(st_lyapunov_scan(tr.table(), deltap0, sigmax0, sigmay0, 2048, 0.05, 6., rows.data(), n, &n),
+          "Track_tbt_Lyapunov");
+    FILE *f = std::fopen("./Lmap-output.dat", "w");
+    if (!f) throw Error("cannot open ./Lmap-output.dat");
+    for (int m = 0; m < n; ++m) {
+        const double *r = &rows[(size_t)m * 3];
+        std::fprintf(f, "%s  %s%s\n", detail::amp(r[0], m == 0).c_str(), detail::amp(r[1], m == 0).c_str(),
+                     detail::sci25(r[2]).c_str());
+    }
+    std::fclose(f);
 }
 
 } // namespace simtrack_amd
