@@ -67,6 +67,7 @@ def main():
     ap.add_argument("--per-seed", type=int, default=4096, help="config2/3: particles per lattice")
     ap.add_argument("--train-seeds", type=int, default=16, help="config3: lattices for the training set")
     ap.add_argument("--n-long", type=int, default=30000, help="config3: survival horizon in turns")
+    ap.add_argument("--epochs", type=int, default=60, help="config3: training epochs")
     ap.add_argument("--orbit", action="store_true", help="config4: orbit tracking with quantum excitation "
                     "instead of spin tracking")
     args = ap.parse_args()
@@ -338,7 +339,7 @@ def bench_config3(args):
     alive = d_s != 0
     S.track_to(ctx, tabs, d_x, d_s, d_l, offs, alive, 2 * S.WINDOW, args.n_long, progress=progress)
     log = []
-    mlp, std = S.train(ctx, d_f[alive], d_s[alive] != 0, epochs=30, batch=2048, log=log)
+    mlp, std = S.train(ctx, d_f[alive], d_s[alive] != 0, epochs=args.epochs, batch=2048, log=log)
     progress(f"trained: loss {log[0]:.4f} -> {log[-1]:.4f}")
     t_train = time.perf_counter() - t0
     del d_x, d_s, d_l, d_f, batch
@@ -369,7 +370,8 @@ def bench_config3(args):
         "plain_method_da_error_sigma": {"max": float(err_plain.max()), "mean": float(err_plain.mean()),
                                         "lattices_within_0.25": int((err_plain.max(axis=1) <= 0.25).sum())},
         "survival_accuracy": {"surrogate": float((surv == ref).mean()), "plain": float((info["plain"] == ref).mean())},
-        "train_s": t_train, "train_loss_first_last": [log[0], log[-1]],
+        "train_s": t_train, "train_loss_first_last": [log[0], log[-1]], "train_lattices": args.train_seeds,
+        "features": "tunes of Hann-windowed 512-turn windows (FineTuneFinder on w*x)",
     }
     print(json.dumps(out))
 

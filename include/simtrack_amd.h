@@ -377,6 +377,12 @@ int st_fine_tune(st_ctx *ctx, const double *x, const double *px, int n, int nsig
  */
 int st_fine_tune_device(st_ctx *ctx, const double *d_x, const double *d_px, int64_t stride_sig, int64_t stride,
                         int n, int nsig, double tune_lower, double tune_upper, double *d_tune);
+/* st_fine_tune_device on w[i] * x[i] (and w[i] * px[i]): d_window holds n
+   device weights, e.g. a Hann window for NAFF-like tunes.  No counterpart
+   in the reference (its FineTuneFinder is unwindowed). */
+int st_fine_tune_window_device(st_ctx *ctx, const double *d_x, const double *d_px, int64_t stride_sig, int64_t stride,
+                               int n, int nsig, double tune_lower, double tune_upper, const double *d_window,
+                               double *d_tune);
 
 /* ------------------------------------------- closed orbit and TBT scans */
 

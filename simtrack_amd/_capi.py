@@ -69,7 +69,7 @@ EXPORTS = [
     "st_batch_track", "st_batch_track_device", "st_batch_autotune", "st_mlp_create", "st_mlp_destroy",
     "st_mlp_num_params", "st_mlp_get_params", "st_mlp_set_params", "st_mlp_forward", "st_mlp_gradients",
     "st_mlp_train_step", "st_batch_track_tbt_device", "st_surrogate_features", "st_closed_orbit", "st_fma_scan",
-    "st_lyapunov_scan",
+    "st_lyapunov_scan", "st_fine_tune_window_device",
 ]
 
 
@@ -137,6 +137,7 @@ def load(path=None):
     lib.st_da_uniform.argtypes = [P, I, D, D, D, D, D, D, I, I, P, P, ctypes.POINTER(I)]
     lib.st_fine_tune.argtypes = [P, P, P, I, I, D, D, P]
     lib.st_fine_tune_device.argtypes = [P, P, P, ctypes.c_int64, ctypes.c_int64, I, I, D, D, P]
+    lib.st_fine_tune_window_device.argtypes = [P, P, P, ctypes.c_int64, ctypes.c_int64, I, I, D, D, P, P]
     BC = ctypes.POINTER(BatchConfig)
     lib.st_batch_create.argtypes = [P, P, I, ctypes.POINTER(P)]
     lib.st_batch_destroy.argtypes = [P]
@@ -207,7 +208,13 @@ class Context:
     def set_timing(self, on=True):
         check(self.lib.st_ctx_set_timing(self.h, 1 if on else 0), "st_ctx_set_timing")
 
-    def fine_tune_device(self, d_x, stride_sig, stride, n, nsig, lower, upper, d_tune, d_px=None):
+    def fine_tune_device(self, d_x, stride_sig, stride, n, nsig, lower, upper, d_tune, d_px=None, d_window=None):
+        if d_window:
+            check(self.lib.st_fine_tune_window_device(
+                self.h, ctypes.c_void_p(d_x), ctypes.c_void_p(d_px) if d_px else None, int(stride_sig), int(stride),
+                int(n), int(nsig), float(lower), float(upper), ctypes.c_void_p(d_window), ctypes.c_void_p(d_tune)),
+                "st_fine_tune_window_device")
+            return
         check(self.lib.st_fine_tune_device(self.h, ctypes.c_void_p(d_x), ctypes.c_void_p(d_px) if d_px else None,
                                            int(stride_sig), int(stride), int(n), int(nsig), float(lower), float(upper),
                                            ctypes.c_void_p(d_tune)), "st_fine_tune_device")

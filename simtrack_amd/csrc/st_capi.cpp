@@ -1432,9 +1432,21 @@ static int tune_check(const st_ctx *c, int n, int nsig, double lo, double hi, bo
     return ST_OK;
 }
 
+extern "C" int st_fine_tune_window_device(st_ctx *c, const double *d_x, const double *d_px, int64_t stride_sig,
+                                          int64_t stride, int n, int nsig, double tune_lower, double tune_upper,
+                                          const double *d_window, double *d_tune);
+
 extern "C" int st_fine_tune_device(st_ctx *c, const double *d_x, const double *d_px, int64_t stride_sig,
                                    int64_t stride, int n, int nsig, double tune_lower, double tune_upper,
                                    double *d_tune)
+{
+    return st_fine_tune_window_device(c, d_x, d_px, stride_sig, stride, n, nsig, tune_lower, tune_upper, nullptr,
+                                      d_tune);
+}
+
+extern "C" int st_fine_tune_window_device(st_ctx *c, const double *d_x, const double *d_px, int64_t stride_sig,
+                                          int64_t stride, int n, int nsig, double tune_lower, double tune_upper,
+                                          const double *d_window, double *d_tune)
 {
     int mc = 0;
     int rc = tune_check(c, n, nsig, tune_lower, tune_upper, d_px != nullptr, &mc);
@@ -1453,6 +1465,7 @@ extern "C" int st_fine_tune_device(st_ctx *c, const double *d_x, const double *d
     a.lower = tune_lower;
     a.upper = tune_upper;
     a.tune = d_tune;
+    a.window = d_window;
     HIPCHK(st_launch_fine_tune(a, c->stream));
     return ST_OK;
 }

@@ -17,6 +17,7 @@ struct st_tune_args {
     int max_cand;       /* LDS slots for one level's candidate tunes         */
     double lower, upper;
     double *tune;       /* nsig results                                      */
+    const double *window; /* n weights applied to every signal, or null     */
 };
 
 hipError_t st_launch_fine_tune(const st_tune_args &a, hipStream_t stream);

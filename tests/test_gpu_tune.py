@@ -79,3 +79,27 @@ def test_tunes_of_tracked_particles(ctx):
         lib.orc_fine_tune(512, r.ctypes.data_as(ctypes.c_void_p), 0.001, 0.499, ctypes.byref(out))
         assert got[j] == out.value
     assert np.ptp(got) < 0.05 and 0.001 < got.min()
+
+
+def test_windowed_fine_tune_matches_oracle_on_windowed_samples(ctx):
+    """st_fine_tune_window_device on x equals the oracle's FineTuneFinder on
+    w * x (the product formed in numpy, the same IEEE multiply)."""
+    import torch
+    from simtrack_amd.surrogate import hann
+    lib = pyoracle.load()
+    n, nsig, lo, hi = 512, 24, 0.05, 0.45
+    x, px, q = signals(nsig, n, 11)
+    w = hann(n)
+    dev = torch.device("cuda:0")
+    d_x = torch.tensor(x, dtype=torch.float64, device=dev)
+    d_w = torch.tensor(w, dtype=torch.float64, device=dev)
+    d_t = torch.zeros(nsig, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    ctx.fine_tune_device(d_x.data_ptr(), n, 1, n, nsig, lo, hi, d_t.data_ptr(), d_window=d_w.data_ptr())
+    ctx.sync()
+    got = d_t.cpu().numpy()
+    for j in range(nsig):
+        xw = np.ascontiguousarray(w * x[j])
+        ref = ctypes.c_double()
+        lib.orc_fine_tune(n, xw.ctypes.data_as(ctypes.c_void_p), lo, hi, ctypes.byref(ref))
+        assert got[j] == ref.value, (j, got[j], ref.value)

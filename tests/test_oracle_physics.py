@@ -443,3 +443,21 @@ def test_radial_on_momentum_particle_has_zero_z():
     orc = pyoracle.Oracle(g, ln.records())
     out, s, _, _ = orc.track(np.zeros((1, 6)), 3, radial=True)
     assert s[0] == 1 and out[0, 4] == 0.0
+
+
+def test_hann_window_sharpens_the_tune_of_a_nonlinear_signal():
+    """The surrogate's features search the DFT peak of Hann-windowed samples:
+    on a signal with a harmonic and an off-peak component the windowed peak
+    is two orders closer to the true tune than the unwindowed one."""
+    from simtrack_amd.surrogate import hann
+    lib = pyoracle.load()
+    n, q = 512, 0.2372819
+    i = np.arange(n)
+    x = np.cos(2 * np.pi * q * i + 0.4) + 0.3 * np.cos(2 * np.pi * 3 * q * i) + 0.2 * np.cos(2 * np.pi * 0.31 * i)
+    errs = []
+    for w in (np.ones(n), hann(n)):
+        xw = np.ascontiguousarray(w * x)
+        peak = ctypes.c_double()
+        lib.orc_fine_tune(n, xw.ctypes.data_as(ctypes.c_void_p), 0.2, 0.3, ctypes.byref(peak))
+        errs.append(abs(peak.value - q))
+    assert errs[1] < 1e-7 and errs[1] < 0.01 * errs[0], errs
