@@ -60,9 +60,10 @@ def main():
                          "calibrating the HBM byte counters)")
     ap.add_argument("--cpu-sample", type=str, default=None,
                     help="particles x turns for the CPU leg (default 1024x64; config4: 2048x24)")
-    ap.add_argument("--workload", choices=("config1", "config2", "config3", "config4"), default="config1",
+    ap.add_argument("--workload", choices=("config1", "config2", "config3", "config4", "rhic"), default="config1",
                     help="config1: the headline workload; config2: 64 uneven lattices in one persistent launch; "
-                         "config3: surrogate DA scan; config4: radiating electron ring, 1,048,576 particles")
+                         "config3: surrogate DA scan; config4: radiating electron ring, 1,048,576 particles; "
+                         "rhic: the reference's RHIC example lattice")
     ap.add_argument("--seeds", type=int, default=64, help="config2: lattices")
     ap.add_argument("--per-seed", type=int, default=4096, help="config2/3: particles per lattice")
     ap.add_argument("--train-seeds", type=int, default=16, help="config3: lattices for the training set")
@@ -85,6 +86,8 @@ def main():
 
     if args.workload == "config4":
         return bench_config4(args)
+    if args.workload == "rhic":
+        return bench_rhic(args)
 
     from simtrack_amd import workloads as W
     gp, ln, x = W.config1(n_particles=args.particles)
@@ -196,6 +199,34 @@ def bench_tracking(args, gp, rec, x, spin, config, traffic, data):
         "cpu_baseline": baseline,
     }
     print(json.dumps(out))
+
+
+def bench_rhic(args):
+    """The reference's first example: RHIC from its MAD-X table (6,858
+    elements), Au ions at gamma = 100 with the 4 MV cavity, orbit tracking of
+    Gaussian particles at the beam size."""
+    import numpy as np
+
+    from simtrack_amd import optics as O
+    from simtrack_amd import workloads as W
+    path = os.path.join(ROOT, "tests", "golden", "lattices", "rhic_2012_madx_twiss.tfs.gz")
+    gp, ln = W.rhic_example(path)
+    opt = O.Optics(ln)
+    eps = 2.5e-6 * 0.65 / gp.gamma          # the example's sigma^2 / beta (Ex1: sqrt(2.5e-6*0.65/gamma))
+    r = np.random.default_rng(2030)
+    n = args.particles
+    x = np.zeros((n, 6))
+    u = r.standard_normal((n, 4))
+    x[:, 0] = np.sqrt(eps * opt.beta_x) * u[:, 0]
+    x[:, 1] = np.sqrt(eps / opt.beta_x) * (u[:, 1] - opt.alpha_x * u[:, 0])
+    x[:, 2] = np.sqrt(eps * opt.beta_y) * u[:, 2]
+    x[:, 3] = np.sqrt(eps / opt.beta_y) * (u[:, 3] - opt.alpha_y * u[:, 2])
+    x[:, 5] = 0.0015                         # the example's deltap
+    config = {"workload": "rhic: the reference's Example1 set-up (MAD-X table, 6,858 elements, Au at gamma 100, "
+                          "4 MV RF), Gaussian particles at the beam size, deltap 1.5e-3",
+              "particles": n, "turns_per_step": args.turns_per_step, "elements": int(len(ln.Cell)), "spin": False}
+    bench_tracking(args, gp, ln.records(), x, False, config, None,
+                   "the reference's RHIC lattice table; synthetic Gaussian particles")
 
 
 def bench_config4(args):
