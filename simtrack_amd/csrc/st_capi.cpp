@@ -165,6 +165,7 @@ struct st_table {
     int n = 0;
     dev_table orbit, spin;
     double *d_ap = nullptr;
+    st_kglobals *d_kg = nullptr; /* kg in device memory, read by the kernels */
     int64_t draws_per_turn = 0;
     int radial_ok = 1;
 };
@@ -1092,6 +1093,7 @@ extern "C" int st_table_destroy(st_table *t)
     hipFree(t->spin.d_ops);
     hipFree(t->spin.d_par);
     hipFree(t->d_ap);
+    hipFree(t->d_kg);
     delete t;
     return ST_OK;
 }
@@ -1165,6 +1167,8 @@ extern "C" int st_table_build(st_ctx *ctx, const st_globals *gp, const st_elemen
     if (!rc) {
         hipError_t e = hipMalloc(&t->d_ap, ap.size() * sizeof(double));
         if (e == hipSuccess) e = hipMemcpy(t->d_ap, ap.data(), ap.size() * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc(&t->d_kg, sizeof(st_kglobals));
+        if (e == hipSuccess) e = hipMemcpy(t->d_kg, &t->kg, sizeof(st_kglobals), hipMemcpyHostToDevice);
         if (e != hipSuccess) rc = fail(ST_ERR_DEVICE, "table upload: %s", hipGetErrorString(e));
     }
     if (rc) {
@@ -1198,6 +1202,7 @@ static st_track_args base_args(st_table *t, bool spin)
     a.nelem = t->n;
     a.tbt_bpm = -1;
     a.g = t->kg;
+    a.gdev = t->d_kg;
     a.draws_per_turn = spin ? 0 : t->draws_per_turn;
     return a;
 }
@@ -1961,6 +1966,7 @@ static int batch_core(st_batch *b, double *d_x, const int *offsets, int turn0, i
         a.tbt_nturn = nturn;
         a.tbt_turn0 = turn0;
         a.g = b->kg;
+        a.gdev = b->tabs[0]->d_kg;
         int ev = -1;
         if (c->timing) {
             rc = timed_event(c, &ev);

@@ -29,6 +29,7 @@ struct st_track_args {
     int tbt_nturn;          /* rows per particle in xtbt                      */
     int tbt_turn0;          /* turn of row 0                                  */
     st_kglobals g;
+    const st_kglobals *gdev; /* the same globals in device memory             */
 };
 
 /* one table (lattice seed) of a multi-table launch */
@@ -56,6 +57,7 @@ struct st_batch_args {
     double *xtbt;           /* turn-by-turn store at turn start, or null      */
     int tbt_nturn, tbt_turn0;
     st_kglobals g;
+    const st_kglobals *gdev; /* the same globals in device memory             */
 };
 
 hipError_t st_launch_track(const st_track_args &a, bool spin, bool hexp, bool rad, hipStream_t stream);

@@ -90,7 +90,7 @@ ST_DEV void thin_field(double xm, double ym, double kn, double ks, double &B0, d
  * (cpp:7561-7589, 7618-7644, 7673-7699).
  */
 template <int KK, bool SPIN, bool HEXP>
-ST_DEV void thick_multipole(St &s, Ptc &c, int nint, int flags, st_pp P, const st_kglobals &g)
+ST_DEV void thick_multipole(St &s, Ptc &c, int nint, int flags, st_pp P, st_gref g)
 {
     double d1 = P[0], d1b = P[1], d2 = P[2], d2b = P[3];
     double kn1 = P[4], ks1 = P[5], kn2 = P[6], ks2 = P[7];
@@ -118,7 +118,7 @@ ST_DEV void thick_multipole(St &s, Ptc &c, int nint, int flags, st_pp P, const s
 
 /* thin (L = 0) QUAD/SEXT/OCT and their spin passes (cpp:7546-7559, 7603-7616, 7658-7671) */
 template <int KK, bool SPIN>
-ST_DEV void thin_multipole(St &s, Ptc &c, st_pp P, const st_kglobals &g)
+ST_DEV void thin_multipole(St &s, Ptc &c, st_pp P, st_gref g)
 {
     double xe = s.x[0], pxe = s.x[1], ye = s.x[2], pye = s.x[3];
     thin_kick<KK>(s, P[0], P[1]);
@@ -136,7 +136,7 @@ ST_DEV void thin_multipole(St &s, Ptc &c, st_pp P, const st_kglobals &g)
 
 /* MULT_Pass (cpp:5152-5182) and MULT_sPass (cpp:7702-7740), thick */
 template <bool SPIN, bool HEXP>
-ST_DEV void thick_mult(St &s, Ptc &c, const st_op &op, st_pp P, const st_kglobals &g)
+ST_DEV void thick_mult(St &s, Ptc &c, const st_op &op, st_pp P, st_gref g)
 {
     int n1 = op.norder + 1;
     st_pp kn1 = P + 6, ks1 = kn1 + n1, kn2 = ks1 + n1, ks2 = kn2 + n1, KNL = ks2 + n1, KNSL = KNL + n1;
@@ -164,7 +164,7 @@ ST_DEV void thick_mult(St &s, Ptc &c, const st_op &op, st_pp P, const st_kglobal
 
 /* thin MULT: mult_kick_pass; spin: cal_Bfield + spin on the entry state, then kick (cpp:7708-7713) */
 template <bool SPIN>
-ST_DEV void thin_mult(St &s, Ptc &c, const st_op &op, st_pp P, const st_kglobals &g)
+ST_DEV void thin_mult(St &s, Ptc &c, const st_op &op, st_pp P, st_gref g)
 {
     int n1 = op.norder + 1;
     st_pp KNL = P, KNSL = P + n1;
@@ -181,7 +181,7 @@ ST_DEV void thin_mult(St &s, Ptc &c, const st_op &op, st_pp P, const st_kglobals
  * bodies) and their spin passes (cpp:7835-7928), L > 0.
  */
 template <bool SPIN, bool HEXP, bool RAD>
-ST_DEV void bendmult(St &s, Ptc &c, const st_op &op, st_pp P, const st_kglobals &g)
+ST_DEV void bendmult(St &s, Ptc &c, const st_op &op, st_pp P, st_gref g)
 {
     int n1 = op.norder + 1;
     st_pp kn1 = P + ST_BM_ARR, ks1 = kn1 + n1, kn2 = ks1 + n1, ks2 = kn2 + n1, KNL = ks2 + n1, KNSL = KNL + n1;
@@ -225,7 +225,7 @@ ST_DEV void bendmult(St &s, Ptc &c, const st_op &op, st_pp P, const st_kglobals 
 
 /* L = 0 branch of the bend-multipole passes (cpp:5350-5352; spin cpp:7846-7854) */
 template <bool SPIN>
-ST_DEV void bendmult_thin(St &s, Ptc &c, const st_op &op, st_pp P, const st_kglobals &g)
+ST_DEV void bendmult_thin(St &s, Ptc &c, const st_op &op, st_pp P, st_gref g)
 {
     int n1 = op.norder + 1;
     st_pp KNL = P + 1, KNSL = KNL + n1;
@@ -243,7 +243,7 @@ ST_DEV void bendmult_thin(St &s, Ptc &c, const st_op &op, st_pp P, const st_kglo
 
 /* GSBENDMULT_Pass, cpp:5597-5723 (orbit only; its sPass is the SBENDMULT one) */
 template <bool HEXP, bool RAD>
-ST_DEV void gsbendmult(St &s, Ptc &c, const st_op &op, st_pp P, const st_kglobals &g)
+ST_DEV void gsbendmult(St &s, Ptc &c, const st_op &op, st_pp P, st_gref g)
 {
     int n1 = op.norder + 1;
     st_pp kn1 = P + ST_GBM_ARR, ks1 = kn1 + n1, kn2 = ks1 + n1, ks2 = kn2 + n1;
@@ -286,7 +286,7 @@ ST_DEV void gsbendmult(St &s, Ptc &c, const st_op &op, st_pp P, const st_kglobal
 template <bool SPIN, class A>
 ST_DEV void sbend_slice_t(A &ar, St &s, double &pz0, double &as1, double x3, const StRcp &rt, const StRcp &rh,
                           double L, double hs, double ih, double cosA, double sinA, double Lb, double An, double href,
-                          const Ptc &c, const st_kglobals &g)
+                          const Ptc &c, st_gref g)
 {
     double *x = s.x;
     const double x0 = x[0], x1 = x[1];
@@ -306,7 +306,7 @@ ST_DEV void sbend_slice_t(A &ar, St &s, double &pz0, double &as1, double x3, con
 
 template <bool SPIN>
 ST_DEV void sbend_slices_exact(St &s, Ptc &c, int nint, double L, double hs, double ih, double cosA, double sinA,
-                               double Lb, double An, double href, const st_kglobals &g)
+                               double Lb, double An, double href, st_gref g)
 {
     double *x = s.x;
     StRcp rh = st_rcp_prep(hs);
@@ -344,7 +344,7 @@ ST_DEV void sbend_slices_exact(St &s, Ptc &c, int nint, double L, double hs, dou
  * SBEND_sPass (cpp:7486-7536).
  */
 template <bool SPIN, bool HEXP, bool RAD>
-ST_DEV void sbend_sliced(St &s, Ptc &c, int nint, st_pp P, const st_kglobals &g,
+ST_DEV void sbend_sliced(St &s, Ptc &c, int nint, st_pp P, st_gref g,
                          uint32_t pid, uint64_t &rctr)
 {
     double href = P[2], Lint = P[3], An = P[4];
@@ -403,7 +403,7 @@ ST_DEV void gsbend(St &s, Ptc &c, int kind, int nint, st_pp P)
 }
 
 /* CCMULT_Pass body, cpp:6034-6064 */
-ST_DEV void ccmult_kick(St &s, st_pp P, const st_kglobals &g)
+ST_DEV void ccmult_kick(St &s, st_pp P, st_gref g)
 {
     double *x = s.x;
     double scale = P[4], brho = P[5];
@@ -436,7 +436,7 @@ ST_DEV void ccmult_kick(St &s, st_pp P, const st_kglobals &g)
  * including the GtoL/LtoG frame changes.
  */
 template <bool SPIN, bool HEXP, bool RAD>
-ST_DEV void apply_op(St &s, Ptc &c, const st_op &op, st_pp par, const st_kglobals &g,
+ST_DEV void apply_op(St &s, Ptc &c, const st_op &op, st_pp par, st_gref g,
                      uint32_t pid, uint64_t &rctr)
 {
     st_pp P = st_launder(par + op.po);
@@ -614,27 +614,29 @@ ST_DEV void track_turns(const st_track_args &a, const st_opp ops, const st_pp pa
     St s;
 #pragma unroll
     for (int k = 0; k < 9; k++) s.x[k] = 0.;
-    bool alive = false;
+    /* per-lane flags as integers: kept in VGPRs, they need no lane-mask
+       bookkeeping at every element */
+    int alive = 0;
     if (in) {
         const double *xp = a.x + (size_t)p * NC;
 #pragma unroll
         for (int k = 0; k < NC; k++) s.x[k] = xp[k];
-        alive = a.state[p] != 0;
+        alive = a.state[p] != 0 ? 1 : 0;
         if (a.precheck) {
             /* Track, simtrack.cpp:20795-20796 */
             if (__builtin_fabs(s.x[0]) > 0.1 || __builtin_fabs(s.x[2]) > 0.1 || !alive) {
                 a.state[p] = 0;
                 a.lost[2 * p] = 0;
                 a.lost[2 * p + 1] = 0;
-                alive = false;
+                alive = 0;
             }
         }
     }
     uint64_t rctr = 0;
     Ptc c;
-    ptc_update(c, s.x[5], a.g);
+    ptc_update(c, s.x[5], *st_launder_g((st_gp)a.gdev));
     int lt = 0, lp = 0;
-    bool died = false;
+    int died = 0;
     const int tend = a.turn0 + a.nturn;
     for (int turn = a.turn0; turn < tend; ++turn) {
         if (!__any(alive)) break;
@@ -647,7 +649,7 @@ ST_DEV void track_turns(const st_track_args &a, const st_opp ops, const st_pp pa
         for (int e = 0; e < nelem; ++e) {
             const st_op op = ops[e];
             if (alive) {
-                apply_op<SPIN, HEXP, RAD>(s, c, op, par, a.g, (uint32_t)p, rctr);
+                apply_op<SPIN, HEXP, RAD>(s, c, op, par, *st_launder_g((st_gp)a.gdev), (uint32_t)p, rctr);
                 if (x04) s.x[4] = s.x[4] - x04[e];
                 if (a.xtbt && e == a.tbt_bpm) {
                     double *o = a.xtbt + ((size_t)p * a.tbt_nturn + (size_t)(turn - a.tbt_turn0)) * NC;
@@ -657,8 +659,8 @@ ST_DEV void track_turns(const st_track_args &a, const st_opp ops, const st_pp pa
                 /* loss test of Track, simtrack.cpp:20802 (none for a bare Element::Pass) */
                 if (ap && (__builtin_fabs(s.x[0]) > ap[2 * e] || __builtin_fabs(s.x[2]) > ap[2 * e + 1] ||
                            s.x[0] != s.x[0] || s.x[2] != s.x[2])) {
-                    alive = false;
-                    died = true;
+                    alive = 0;
+                    died = 1;
                     lt = turn;
                     lp = e;
                 }
@@ -709,6 +711,7 @@ __global__ __launch_bounds__(256) void k_track_batch(st_batch_args b)
     a.tbt_nturn = b.tbt_nturn;
     a.tbt_turn0 = b.tbt_turn0;
     a.g = b.g;
+    a.gdev = b.gdev;
     const ST_KCONST st_seed_desc *seeds = (const ST_KCONST st_seed_desc *)b.seeds;
     const ST_KCONST int *items = (const ST_KCONST int *)b.items;
     for (;;) {
@@ -737,9 +740,9 @@ __global__ void k_radial_x04(st_track_args a, double *x04)
     for (int e = 0; e < a.nelem; ++e) {
         s.x[4] = 0.;
         s.x[5] = 0.;
-        ptc_update(c, s.x[5], a.g);
+        ptc_update(c, s.x[5], *st_launder_g((st_gp)a.gdev));
         const st_op op = ((st_opp)a.ops)[e];
-        apply_op<false, HEXP, false>(s, c, op, (st_pp)a.par, a.g, 0u, rctr);
+        apply_op<false, HEXP, false>(s, c, op, (st_pp)a.par, *st_launder_g((st_gp)a.gdev), 0u, rctr);
         x04[e] = s.x[4];
     }
 }

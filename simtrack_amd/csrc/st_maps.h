@@ -37,12 +37,26 @@ typedef double st_real;
 #endif
 typedef ST_KCONST double *st_pp;
 typedef ST_KCONST st_op *st_opp;
+/* the reference-particle globals, read through a constant-address reference
+   (scalar loads where a map uses them, instead of SGPRs held for the whole
+   element loop) */
+typedef ST_KCONST st_kglobals &st_gref;
+typedef ST_KCONST st_kglobals *st_gp;
 
 /* The element's payload base pointer, made opaque to the optimiser.  Every
    P[k] then becomes a scalar load at an immediate offset from one SGPR pair;
    without it the compiler hoists par + 8k for every k used by any element
    type out of the element loop, and those loop-invariant addresses spill. */
 ST_DEV st_pp st_launder(st_pp p)
+{
+    __asm__ volatile("" : "+s"(p));
+    return p;
+}
+
+/* the same for the globals: laundered once per element, their fields are
+   loaded where the element's map reads them and not kept live across the
+   element loop (where they spilled SGPRs into VGPR lanes) */
+ST_DEV st_gp st_launder_g(st_gp p)
 {
     __asm__ volatile("" : "+s"(p));
     return p;
@@ -64,7 +78,7 @@ struct Ptc {
     StRcp rb1;
 };
 
-ST_DEV void ptc_update(Ptc &c, double pt, const st_kglobals &g)
+ST_DEV void ptc_update(Ptc &c, double pt, st_gref g)
 {
     c.d1 = st_fsqrt(1.0 + 2 * pt / g.beta + pt * pt) - 1.0;
     c.g1 = g.gamma + g.sg * pt;
@@ -474,7 +488,7 @@ ST_DEV void edge(St &s, double ef)
  */
 template <class A>
 ST_DEV void spin_rotate_t(A &ar, St &s, double xe, double pxe, double pye, double pze, double angle, double href,
-                          double B0, double B1, double B2, const Ptc &c, const st_kglobals &g)
+                          double B0, double B1, double B2, const Ptc &c, st_gref g)
 {
     double *x = s.x;
     {
@@ -517,7 +531,7 @@ ST_DEV void spin_rotate_t(A &ar, St &s, double xe, double pxe, double pye, doubl
    px^2 - py^2) is the entry pz, evaluated only for a non-zero field), fast
    pass with an exact replay of the rotation where needed */
 ST_DEV void spin_rotate(St &s, double xe, double pxe, double pye, double angle, double href, double B0, double B1,
-                        double B2, const Ptc &c, const st_kglobals &g)
+                        double B2, const Ptc &c, st_gref g)
 {
     if (B0 * B0 + B1 * B1 + B2 * B2 != 0.) {
         double s6 = s.x[6], s7 = s.x[7], s8 = s.x[8];
@@ -555,7 +569,7 @@ ST_DEV void rad_common(const St &s, double href, double B0, double B1, double B2
 }
 
 /* radiate, simtrack.cpp:4198-4234 (classical energy loss) */
-ST_DEV void radiate(St &s, double L, double href, double B0, double B1, double B2, Ptc &c, const st_kglobals &g)
+ST_DEV void radiate(St &s, double L, double href, double B0, double B1, double B2, Ptc &c, st_gref g)
 {
     double *x = s.x;
     double px0 = x[1], py0 = x[3];
@@ -571,7 +585,7 @@ ST_DEV void radiate(St &s, double L, double href, double B0, double B1, double B
 }
 
 /* radiate1, simtrack.cpp:4236-4288 (loss + Gaussian quantum excitation) */
-ST_DEV void radiate1(St &s, double L, double href, double B0, double B1, double B2, Ptc &c, const st_kglobals &g,
+ST_DEV void radiate1(St &s, double L, double href, double B0, double B1, double B2, Ptc &c, st_gref g,
                      uint32_t pid, uint64_t &ctr)
 {
     double *x = s.x;
