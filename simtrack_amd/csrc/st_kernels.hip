@@ -649,7 +649,18 @@ ST_DEV void track_turns(const st_track_args &a, const st_opp ops, const st_pp pa
         for (int e = 0; e < nelem; ++e) {
             const st_op op = ops[e];
             if (alive) {
-                apply_op<SPIN, HEXP, RAD>(s, c, op, par, *st_launder_g((st_gp)a.gdev), (uint32_t)p, rctr);
+                /* drifts and zero-length drifts without misalignment are most
+                   of a lattice's elements: dispatched ahead of the full switch
+                   (orbit kernels only: in the spin kernels the extra path
+                   costs the spin-tracked bends more than it saves) */
+                if (!SPIN && op.kind == K_DRIFT && op.pm < 0) {
+                    st_pp P = st_launder(par + op.po);
+                    drift<HEXP>(s, P[0], P[1], c);
+                } else if (!SPIN && op.kind == K_DRIFT0 && op.pm < 0) {
+                    drift0<HEXP>(s, c);
+                } else {
+                    apply_op<SPIN, HEXP, RAD>(s, c, op, par, *st_launder_g((st_gp)a.gdev), (uint32_t)p, rctr);
+                }
                 if (x04) s.x[4] = s.x[4] - x04[e];
                 if (a.xtbt && e == a.tbt_bpm) {
                     double *o = a.xtbt + ((size_t)p * a.tbt_nturn + (size_t)(turn - a.tbt_turn0)) * NC;
