@@ -420,6 +420,15 @@ int st_fma_scan(st_table *t, double deltap0, double sigmax0, double sigmay0, con
 int st_lyapunov_scan(st_table *t, double deltap0, double sigmax0, double sigmay0, int nturn, double sigma_step,
                      double sigma_max, double *rows, int max_rows, int *nrows);
 
+/*
+ * Track_tbt_ActionDiff (simtrack.cpp:21305-21383) with the start Twiss
+ * (betx, alfx, bety, alfy) as arguments: rows (nstep x 6) of (turns, count,
+ * mean 2Jx, rms 2Jx, mean 2Jy, rms 2Jy); *seed is the reference's LCG state
+ * (its global `seed`, initially 1234), advanced as the reference does.
+ */
+int st_action_diff(st_table *t, double betx, double alfx, double bety, double alfy, double nsigma, double phase,
+                   double emitx0, double emity0, int nturn, int nstep, int npart, double *seed, double *rows);
+
 #ifdef __cplusplus
 }
 #endif

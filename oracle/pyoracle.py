@@ -42,6 +42,7 @@ def load(glibc=False):
     lib.orc_closed_orbit.argtypes = [P, I, D, P, P, ctypes.POINTER(I)]
     lib.orc_fma_scan.argtypes = [P, I, D, D, D, P, D, D, I, D, D, P, I]
     lib.orc_lyapunov_scan.argtypes = [P, I, D, D, D, I, D, D, P, I]
+    lib.orc_action_diff.argtypes = [P, I, D, D, D, D, D, D, D, D, I, I, I, P, P]
     lib.orc_fine_tune_xpx.argtypes = [I, P, P, D, D, P]
     for f in ("orc_sin", "orc_cos", "orc_tan", "orc_asin", "orc_exp", "orc_log", "orc_ugaussian_pinv"):
         getattr(lib, f).argtypes = [D]
@@ -135,3 +136,14 @@ class Oracle:
         if m < 0:
             raise RuntimeError(f"oracle lyapunov scan failed ({m})")
         return rows[:m]
+
+    def action_diff(self, betx, alfx, bety, alfy, nsigma, phase, emitx0, emity0, nturn, nstep, npart, seed=1234.):
+        """Track_tbt_ActionDiff (simtrack.cpp:21305-21383): (rows, seed after)."""
+        self._set()
+        rows = np.zeros((nstep, 6))
+        sd = np.array([seed], dtype=np.float64)
+        rc = self.lib.orc_action_diff(_p(self.records), len(self.records), betx, alfx, bety, alfy, nsigma, phase,
+                                      emitx0, emity0, nturn, nstep, npart, _p(sd), _p(rows))
+        if rc != 0:
+            raise RuntimeError(f"oracle action diffusion failed ({rc})")
+        return rows, float(sd[0])

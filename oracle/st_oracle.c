@@ -2445,3 +2445,123 @@ EXPORT int orc_lyapunov_scan(const st_element *line, int n, double deltap0, doub
     free(Lay);
     return m;
 }
+
+/* rnd, simtrack.cpp:283-291 */
+static double rnd(double *r)
+{
+    int m;
+    double s, u, v, p;
+    s = 65536.0;
+    u = 2053.0;
+    v = 13849.0;
+    m = (int)(*r / s);
+    *r = *r - m * s;
+    *r = u * *r + v;
+    m = (int)(*r / s);
+    *r = *r - m * s;
+    p = *r / s;
+    return (p);
+}
+
+/* Cal_Twiss_Emit, simtrack.cpp:771-815 (the emittance) */
+static double Cal_Twiss_Emit(const double x[], const double px[], double n)
+{
+    int i;
+    double sum, xmean, pxmean, sigma_xx, sigma_xpx, sigma_pxpx;
+    sum = 0.;
+    for (i = 0; i < n; i++) sum = sum + x[i];
+    xmean = sum / n;
+    sum = 0.;
+    for (i = 0; i < n; i++) sum = sum + (x[i] - xmean) * (x[i] - xmean);
+    sigma_xx = sum / n;
+    sum = 0.;
+    for (i = 0; i < n; i++) sum = sum + px[i];
+    pxmean = sum / n;
+    sum = 0.;
+    for (i = 0; i < n; i++) sum = sum + (px[i] - pxmean) * (px[i] - pxmean);
+    sigma_pxpx = sum / n;
+    sum = 0.;
+    for (i = 0; i < n; i++) sum = sum + (x[i] - xmean) * (px[i] - pxmean);
+    sigma_xpx = sum / n;
+    return sqrt(sigma_xx * sigma_pxpx - sigma_xpx * sigma_xpx);
+}
+
+/* Cal_Mean_RMS, simtrack.cpp:708-722 */
+static void Cal_Mean_RMS(const double x[], int n, double *mean, double *rms)
+{
+    int i;
+    *mean = 0;
+    for (i = 0; i < n; i++) *mean += x[i];
+    *mean = *mean * 1.0 / n;
+    *rms = 0;
+    for (i = 0; i < n; i++) *rms += (x[i] - *mean) * (x[i] - *mean);
+    *rms = sqrt(*rms * 1.0 / (n - 1));
+}
+
+/* Track_tbt_ActionDiff, simtrack.cpp:21305-21383, one particle after
+   another, start Twiss as arguments; rows as st_action_diff */
+EXPORT int orc_action_diff(const st_element *line, int n, double betx, double alfx, double bety, double alfy,
+                           double nsigma, double phase, double emitx0, double emity0, int nturn, int nstep, int npart,
+                           double *seed, double *rows)
+{
+    const double PI = 3.14159265358979323;
+    double nsigmax = nsigma * OCOS(phase), nsigmay = nsigma * OSIN(phase);
+    double TwoJx0 = (nsigmax * nsigmax) * emitx0, TwoJy0 = (nsigmay * nsigmay) * emity0;
+    double *x = (double *)malloc(sizeof(double) * (size_t)npart * 6);
+    double *jx = (double *)calloc((size_t)npart, sizeof(double));
+    double *jy = (double *)calloc((size_t)npart, sizeof(double));
+    double *x_tbt = (double *)malloc(sizeof(double) * (size_t)nturn * 6);
+    double *tx = (double *)malloc(sizeof(double) * (size_t)nturn);
+    double *tp = (double *)malloc(sizeof(double) * (size_t)nturn);
+    uint64_t draws = orc_draws_per_turn(line, n, 0);
+    int rc = 0;
+    if (!x || !jx || !jy || !x_tbt || !tx || !tp) return -5;
+    for (int i = 0; i < npart; i++) {
+        double phix0 = rnd(seed) * 360;
+        double phiy0 = rnd(seed) * 360;
+        double xn = sqrt(TwoJx0) * OCOS(phix0 * PI / 180), pxn = -sqrt(TwoJx0) * OSIN(phix0 * PI / 180);
+        x[i * 6 + 0] = xn * sqrt(betx);
+        x[i * 6 + 1] = -alfx * xn / sqrt(betx) + pxn / sqrt(betx);
+        double yn = sqrt(TwoJy0) * OCOS(phiy0 * PI / 180), pyn = -sqrt(TwoJy0) * OSIN(phiy0 * PI / 180);
+        x[i * 6 + 2] = yn * sqrt(bety);
+        x[i * 6 + 3] = -alfy * yn / sqrt(bety) + pyn / sqrt(bety);
+        x[i * 6 + 4] = 0.;
+        x[i * 6 + 5] = 0.;
+    }
+    for (int j = 0; j < nstep && !rc; j++) {
+        int ncount = 0;
+        for (int i = 0; i < npart; i++) {
+            int stable = 1, lt = 0, lp = 0;
+            double *xt = x + (size_t)i * 6;
+            if (xt[0] == 0.) xt[0] = xt[0] + 1.0e-09;
+            if (xt[2] == 0.) xt[2] = xt[2] + 1.0e-09;
+            rc = track_one(line, n, xt, nturn, &stable, &lt, &lp, 0, NULL, x_tbt, -1, (uint32_t)i, draws);
+            if (rc) break;
+            if (stable == 1) {
+                ncount++;
+                for (int k = 0; k < nturn; k++) {
+                    tx[k] = x_tbt[k * 6 + 0];
+                    tp[k] = x_tbt[k * 6 + 1];
+                }
+                jx[i] = Cal_Twiss_Emit(tx, tp, nturn);
+                for (int k = 0; k < nturn; k++) {
+                    tx[k] = x_tbt[k * 6 + 2];
+                    tp[k] = x_tbt[k * 6 + 3];
+                }
+                jy[i] = Cal_Twiss_Emit(tx, tp, nturn);
+            }
+        }
+        double *o = rows + (size_t)j * 6;
+        o[0] = (double)((j + 1) * nturn);
+        o[1] = ncount;
+        Cal_Mean_RMS(jx, ncount, &o[2], &o[3]);
+        Cal_Mean_RMS(jy, ncount, &o[4], &o[5]);
+    }
+    free(x);
+    free(jx);
+    free(jy);
+    free(x_tbt);
+    free(tx);
+    free(tp);
+    return rc;
+}

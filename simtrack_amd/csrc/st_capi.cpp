@@ -1817,6 +1817,115 @@ extern "C" int st_lyapunov_scan(st_table *t, double deltap0, double sigmax0, dou
     return ST_OK;
 }
 
+/* rnd, simtrack.cpp:283-291: the reference's LCG; r is its state (global `seed`) */
+static double rnd_ref(double &r)
+{
+    int m;
+    double s, u, v, p;
+    s = 65536.0;
+    u = 2053.0;
+    v = 13849.0;
+    m = int(r / s);
+    r = r - m * s;
+    r = u * r + v;
+    m = (int)(r / s);
+    r = r - m * s;
+    p = r / s;
+    return (p);
+}
+
+/* Cal_Twiss_Emit (cpp:771-815): rms emittance of n (x, px) samples */
+static double twiss_emit_ref(const double *x, const double *px, int stride, double n)
+{
+    double sum = 0., xmean, pxmean, sxx, sxp, spp;
+    for (int i = 0; i < n; i++) sum = sum + x[(size_t)i * stride];
+    xmean = sum / n;
+    sum = 0.;
+    for (int i = 0; i < n; i++) sum = sum + (x[(size_t)i * stride] - xmean) * (x[(size_t)i * stride] - xmean);
+    sxx = sum / n;
+    sum = 0.;
+    for (int i = 0; i < n; i++) sum = sum + px[(size_t)i * stride];
+    pxmean = sum / n;
+    sum = 0.;
+    for (int i = 0; i < n; i++) sum = sum + (px[(size_t)i * stride] - pxmean) * (px[(size_t)i * stride] - pxmean);
+    spp = sum / n;
+    sum = 0.;
+    for (int i = 0; i < n; i++) sum = sum + (x[(size_t)i * stride] - xmean) * (px[(size_t)i * stride] - pxmean);
+    sxp = sum / n;
+    return std::sqrt(sxx * spp - sxp * sxp);
+}
+
+/* Cal_Mean_RMS (cpp:708-722) */
+static void mean_rms_ref(const double *x, int n, double &mean, double &rms)
+{
+    mean = 0;
+    for (int i = 0; i < n; i++) mean += x[i];
+    mean = mean * 1.0 / n;
+    rms = 0;
+    for (int i = 0; i < n; i++) rms += (x[i] - mean) * (x[i] - mean);
+    rms = std::sqrt(rms * 1.0 / (n - 1));
+}
+
+/* Track_tbt_ActionDiff (simtrack.cpp:21305-21383) with the Twiss functions at
+   the start as arguments (the reference takes them from Cal_Twiss).  npart
+   particles at nsigma sigma (emittance split by `phase`), phases from the
+   reference's LCG whose state *seed is updated; nstep steps of nturn turns,
+   each particle continuing from where the last step left it with its stable
+   flag reset; per step the row ((j+1)*nturn, ncount, mean and rms of 2Jx,
+   mean and rms of 2Jy) with the reference's statistics, including that the
+   mean runs over the first ncount entries of the per-particle array, whose
+   entries of lost particles keep earlier (here initially zero) values. */
+extern "C" int st_action_diff(st_table *t, double betx, double alfx, double bety, double alfy, double nsigma,
+                              double phase, double emitx0, double emity0, int nturn, int nstep, int npart,
+                              double *seed, double *rows)
+{
+    if (!t || !seed || !rows || nturn < 1 || nstep < 0 || npart < 1 || !(betx > 0.) || !(bety > 0.))
+        return fail(ST_ERR_ARG, "st_action_diff: bad argument");
+    const double PI = 3.14159265358979323;
+    double nsigmax = nsigma * st_cos(phase), nsigmay = nsigma * st_sin(phase);
+    double TwoJx0 = (nsigmax * nsigmax) * emitx0, TwoJy0 = (nsigmay * nsigmay) * emity0;
+    std::vector<double> x((size_t)npart * 6), jx(npart, 0.), jy(npart, 0.);
+    for (int i = 0; i < npart; i++) {
+        double phix0 = rnd_ref(*seed) * 360;
+        double phiy0 = rnd_ref(*seed) * 360;
+        double xn = std::sqrt(TwoJx0) * st_cos(phix0 * PI / 180), pxn = -std::sqrt(TwoJx0) * st_sin(phix0 * PI / 180);
+        double *p = &x[(size_t)i * 6];
+        p[0] = xn * std::sqrt(betx);
+        p[1] = -alfx * xn / std::sqrt(betx) + pxn / std::sqrt(betx);
+        double yn = std::sqrt(TwoJy0) * st_cos(phiy0 * PI / 180), pyn = -std::sqrt(TwoJy0) * st_sin(phiy0 * PI / 180);
+        p[2] = yn * std::sqrt(bety);
+        p[3] = -alfy * yn / std::sqrt(bety) + pyn / std::sqrt(bety);
+        p[4] = 0.;
+        p[5] = 0.;
+    }
+    std::vector<double> tbt((size_t)npart * nturn * 6);
+    std::vector<int> st(npart), lt(npart), lp(npart);
+    for (int j = 0; j < nstep; j++) {
+        for (int i = 0; i < npart; i++) {
+            st[i] = 1;
+            lt[i] = lp[i] = 0;
+            if (x[(size_t)i * 6] == 0.) x[(size_t)i * 6] = x[(size_t)i * 6] + 1.0e-09;
+            if (x[(size_t)i * 6 + 2] == 0.) x[(size_t)i * 6 + 2] = x[(size_t)i * 6 + 2] + 1.0e-09;
+        }
+        int rc = track_host(t, x.data(), npart, nturn, st.data(), lt.data(), lp.data(), false, false, tbt.data(), -1);
+        if (rc) return rc;
+        int ncount = 0;
+        for (int i = 0; i < npart; i++) {
+            if (st[i] != 1) continue;
+            ncount++;
+            const double *b = &tbt[(size_t)i * nturn * 6];
+            jx[i] = twiss_emit_ref(b + 0, b + 1, 6, nturn);
+            jy[i] = twiss_emit_ref(b + 2, b + 3, 6, nturn);
+        }
+        double *o = rows + (size_t)j * 6;
+        o[0] = (double)((j + 1) * nturn);
+        o[1] = ncount;
+        mean_rms_ref(jx.data(), ncount, o[2], o[3]);
+        mean_rms_ref(jy.data(), ncount, o[4], o[5]);
+    }
+    return ST_OK;
+}
+
 /* --------------------------------------------- many tables, one launch */
 
 struct st_batch {

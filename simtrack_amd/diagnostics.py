@@ -8,6 +8,10 @@ driver names and output files.
   (st_fma_scan), written to FMA-output.dat.
 - Track_tbt_Lyapunov (cpp:21243-21303): twin-particle divergence scan
   (st_lyapunov_scan), written to Lmap-output.dat.
+- Track_tbt_ActionDiff (cpp:21305-21383): action diffusion of particles at
+  fixed amplitude and random phases (st_action_diff), Diff-output.dat.
+- twiss_at_start: closed orbit, one-turn matrix and uncoupled Twiss from
+  GPU-tracked finite differences (what Cal_Twiss gives those drivers).
 
 The reference writes its files through one fstream that it reopens per
 line without resetting its format flags: the first line prints the grid
@@ -66,6 +70,79 @@ def lyapunov_scan(tab, deltap0, sigmax0, sigmay0, nturn=2048, sigma_step=0.05, s
     return rows[:n.value]
 
 
+def one_turn_matrix(ctx, gp, records, x_co, h=1e-6):
+    """The 6x6 one-turn matrix about x_co: symmetric finite differences of
+    bare GPU-tracked turns (Element::Pass, no loss test) with every RF, crab
+    and CCMULT voltage off, as Cal_OneTurnMap (simtrack.cpp:15122) turns them
+    off for its TPS map.  The step h = 1e-6 balances truncation (O(h^2)
+    from the sextupoles) against roundoff (O(eps/h), largest in the delta
+    column): on RHIC the symplectic defect |M^T S M - S| is 1e-7 at
+    h = 1e-6 and 2.7e-6 at h = 1e-8."""
+    import torch
+
+    from ._capi import TRACK_BARE, Table
+    rec = np.array(records, copy=True)
+    rf = np.isin(rec["type"], [20, 21, 22])          # RFCAV, CRABRF, CCMULT
+    rec["vrf"][rf] = 0.
+    tab = Table(ctx, gp, rec)
+    x = np.repeat(np.asarray(x_co, dtype=np.float64)[None, :6], 12, axis=0)
+    for k in range(6):
+        x[2 * k, k] += h
+        x[2 * k + 1, k] -= h
+    dev = torch.device("cuda:0")
+    d_x = torch.tensor(x, dtype=torch.float64, device=dev)
+    d_s = torch.ones(12, dtype=torch.int32, device=dev)
+    d_l = torch.zeros(24, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    tab.track_device(d_x.data_ptr(), 12, 0, 1, d_s.data_ptr(), d_l.data_ptr(), TRACK_BARE)
+    ctx.sync()
+    o = d_x.cpu().numpy()
+    return np.stack([(o[2 * k] - o[2 * k + 1]) / (2 * h) for k in range(6)], axis=1)
+
+
+def twiss_from_matrix(m):
+    """(beta_x, alpha_x, tune_x, beta_y, alpha_y, tune_y) of the uncoupled
+    2x2 blocks of a one-turn matrix (tunes fractional)."""
+    out = []
+    for b in (m[0:2, 0:2], m[2:4, 2:4]):
+        cosmu = (b[0, 0] + b[1, 1]) / 2
+        if abs(cosmu) >= 1:
+            raise STError("one-turn matrix: unstable plane")
+        sinmu = np.sign(b[0, 1]) * np.sqrt(1 - cosmu * cosmu)
+        mu = np.arctan2(sinmu, cosmu) % (2 * np.pi)
+        out += [b[0, 1] / sinmu, (b[0, 0] - b[1, 1]) / (2 * sinmu), mu / (2 * np.pi)]
+    return tuple(float(v) for v in out)
+
+
+def twiss_at_start(ctx, gp, records, deltap=0.):
+    """Closed orbit, one-turn matrix and uncoupled Twiss at the start of the
+    line (the quantities Cal_Twiss gives the reference's drivers)."""
+    x_co, _, _ = cal_orbit_num(ctx, gp, records, deltap)
+    m = one_turn_matrix(ctx, gp, records, x_co)
+    return x_co, m, twiss_from_matrix(m)
+
+
+def action_diff(tab, betx, alfx, bety, alfy, nsigma, phase, emitx0, emity0, nturn, nstep, npart, seed=1234.):
+    """Track_tbt_ActionDiff on the GPU: (rows, seed after).  Rows: turns,
+    surviving count, mean and rms of 2Jx, mean and rms of 2Jy per step."""
+    rows = np.zeros((nstep, 6))
+    sd = np.array([seed], dtype=np.float64)
+    check(tab.lib.st_action_diff(tab.h, betx, alfx, bety, alfy, nsigma, phase, emitx0, emity0, nturn, nstep, npart,
+                                 _ptr(sd), _ptr(rows)), "st_action_diff")
+    return rows, float(sd[0])
+
+
+def Track_tbt_ActionDiff(tab, twiss, nsigma, phase, emitx0, emity0, nturn, nstep, npart, seed=1234.,
+                         filename="./Diff-output.dat"):
+    """Track_tbt_ActionDiff with the reference's output lines (cpp:21377-21379);
+    twiss = (beta_x, alpha_x, beta_y, alpha_y) at the start.  Returns (rows, seed)."""
+    rows, sd = action_diff(tab, *twiss, nsigma, phase, emitx0, emity0, nturn, nstep, npart, seed)
+    with open(filename, "w") as f:
+        for r in rows:
+            f.write(f"{int(r[0])}   {int(r[1])}    {r[2]:g}  {r[3]:g}     {r[4]:g}   {r[5]:g}\n")
+    return rows, sd
+
+
 def _amp(v, first):
     return f"{v:g}" if first else f"{v:.18e}"
 
@@ -96,4 +173,5 @@ def Track_tbt_Lyapunov(tab, deltap0, sigmax0, sigmay0, filename="./Lmap-output.d
     return rows
 
 
-__all__ = ["cal_orbit_num", "fma_scan", "lyapunov_scan", "Track_tbt_FMA", "Track_tbt_Lyapunov", "STError"]
+__all__ = ["cal_orbit_num", "fma_scan", "lyapunov_scan", "Track_tbt_FMA", "Track_tbt_Lyapunov", "one_turn_matrix",
+           "twiss_from_matrix", "twiss_at_start", "action_diff", "Track_tbt_ActionDiff", "STError"]

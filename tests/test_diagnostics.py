@@ -116,3 +116,24 @@ def test_writers_reproduce_the_reference_lines(tmp_path, monkeypatch):
     lines = g.read_text().split("\n")
     assert lines[0] == "0.05  0" + f"{1.5e-3:.18e}".rjust(25)
     assert lines[1] == "1.000000000000000056e-01  5.000000000000000278e-02" + f"{-2e-4:.18e}".rjust(25)
+
+
+def test_action_diffusion_of_a_linear_ring_is_zero():
+    """Particles at a fixed action in a linear ring keep it: every step's mean
+    2Jx equals nsigma^2 cos^2(phase) emitx0 (the rms fit of a matched ellipse
+    sampled over many turns), and the LCG advances two draws per particle."""
+    gp, ln = _fodo()
+    opt = O.Optics(ln)
+    orc = pyoracle.Oracle(gp, ln.records(), nthreads=1)
+    nsig, ph, ex, ey = 3.0, 0.6, 1e-8, 1e-8
+    rows, sd = orc.action_diff(opt.beta_x, opt.alpha_x, opt.beta_y, opt.alpha_y, nsig, ph, ex, ey, 256, 2, 8)
+    assert rows[0, 0] == 256 and rows[1, 0] == 512 and (rows[:, 1] == 8).all()
+    # the rms fit of one particle's turn-by-turn ellipse is its Courant-Snyder action J (= 2J/2)
+    jx = (nsig * math.cos(ph)) ** 2 * ex / 2
+    jy = (nsig * math.sin(ph)) ** 2 * ey / 2
+    assert np.allclose(rows[:, 2], jx, rtol=2e-2) and np.allclose(rows[:, 4], jy, rtol=2e-2)
+    r = 1234.
+    for _ in range(16):
+        r = (2053.0 * (r - int(r / 65536.0) * 65536.0) + 13849.0)
+        r = r - int(r / 65536.0) * 65536.0
+    assert sd == r

@@ -62,3 +62,30 @@ def test_lyapunov_scan_bitwise(ctx):
     ref = pyoracle.Oracle(gp, rec).lyapunov_scan(*args)
     assert len(got) == len(ref) > 5
     assert_bitwise(got, ref, "Lyapunov rows")
+
+
+def test_twiss_at_start_rhic(ctx):
+    """One-turn matrix from GPU-tracked finite differences: RHIC's tunes equal
+    MAD-X's (header Q1, Q2), Twiss agrees with the thick-lens optics."""
+    gp, ln = W.rhic_example(RHIC)
+    x_co, m, (bx, ax, qx, by, ay, qy) = D.twiss_at_start(ctx, gp, ln.records())
+    opt = O.Optics(ln)
+    assert abs(qx - 0.23) < 2e-6 and abs(qy - 0.22) < 2e-6
+    assert abs(bx / opt.beta_x - 1) < 1e-3 and abs(by / opt.beta_y - 1) < 1e-3
+    s = np.zeros((6, 6))
+    for k in range(3):
+        s[2 * k, 2 * k + 1], s[2 * k + 1, 2 * k] = 1., -1.
+    assert np.abs(m.T @ s @ m - s).max() < 1e-6
+
+
+def test_action_diff_bitwise(ctx):
+    gp, ln = W.config0()[:2]
+    rec = ln.records()
+    opt = O.Optics(ln)
+    eps, _ = W.beam_sizes(gp)
+    args = (opt.beta_x, opt.alpha_x, opt.beta_y, opt.alpha_y, 4.0, 0.7, eps, eps, 128, 3, 40)
+    got, sd = D.action_diff(st.Table(ctx, gp, rec), *args)
+    ref, sdo = pyoracle.Oracle(gp, rec).action_diff(*args)
+    assert sd == sdo
+    assert_bitwise(got, ref, "action diffusion rows")
+    assert (got[:, 1] > 30).all()
