@@ -400,6 +400,16 @@ int st_closed_orbit(st_ctx *ctx, const st_globals *gp, const st_element *rec, in
                     double x_co[6], double x_end[6], int *iterations);
 
 /*
+ * The 6x6 one-turn matrix about x_co (row-major, m[6*i+k] = dx_i/dx_k):
+ * symmetric differences of step h of bare passes (no loss test) with the
+ * RFCAV voltages off (crab cavities and CCMULT on) -- the linear part of
+ * the map that Cal_OneTurnMap (simtrack.cpp:15122-15137) expands in TPS
+ * and Cal_Twiss (cpp:15540) analyses.  h = 1e-6 suits proton and ion rings.
+ */
+int st_one_turn_matrix(st_ctx *ctx, const st_globals *gp, const st_element *rec, int n, const double x_co[6],
+                       double h, double m[36]);
+
+/*
  * Track_tbt_FMA (simtrack.cpp:21179-21241) with its grid and turns as
  * arguments (the reference: nturn 2048, sigma_step 0.1, sigma_max 6):
  * rows of 8 doubles (i*step, j*step, tunex1, tunex2, tuney1, tuney2,

@@ -13,6 +13,7 @@
  *   Track_DA_Uniform, Track_DA_Radial_Uniform
  *   Read_MADX_Lattice, Read_BMAD_Lattice               simtrack.cpp:13591, 14054
  *   Cal_Orbit_Num, Track_tbt_FMA, Track_tbt_Lyapunov   simtrack.cpp:14947, 21179, 21243
+ *   Cal_OneTurnMap (matrix only), Track_tbt_ActionDiff simtrack.cpp:15122, 21309
  *
  * Element classes: DRIFT, MARKER, BPM, HBPM, VBPM, SBEND, GSBEND, QUAD,
  * SKEWQ, SEXT, OCT, MULT, GMULT, SBENDMULT, SMULT, GSBENDMULT, KICKER,
@@ -129,6 +130,8 @@ public:
     double DTHETA = 0., DPHI = 0., DPSI = 0.;
     double APx = 1., APy = 1.; /* Element::Element, simtrack.cpp:8394-8395 */
     double X[6] = {0., 0., 0., 0., 0., 0.}; /* closed orbit after the element (Cal_Orbit_Num) */
+    double M[36] = {};                       /* one-turn matrix, last element only (Cal_OneTurnMap) */
+    double Beta1 = 0., Alfa1 = 0., Beta2 = 0., Alfa2 = 0.; /* Twiss at the end (Cal_Twiss_End) */
 
 protected:
     struct Param {
@@ -1259,6 +1262,69 @@ inline void Track_tbt_Lyapunov(Line &l, double deltap0, double sigmax0, double s
         const double *r = &rows[(size_t)m * 3];
         std::fprintf(f, "%s  %s%s\n", detail::amp(r[0], m == 0).c_str(), detail::amp(r[1], m == 0).c_str(),
                      detail::sci25(r[2]).c_str());
+    }
+    std::fclose(f);
+}
+
+/* the reference's global random seed of rnd(), simtrack.cpp:51 */
+inline double seed = 1234.;
+
+/* Cal_OneTurnMap (simtrack.cpp:15122-15179), its linear part: the 6x6
+   one-turn matrix about the last element's X (set by Cal_Orbit_Num) with
+   z = 0 and delta = deltap, RFCAV voltages off, by symmetric differences of
+   1e-6 of GPU-tracked turns; stored row-major in the last element's M. */
+inline void Cal_OneTurnMap(Line &l, double deltap)
+{
+    if (l.Cell.empty()) throw Error("Cal_OneTurnMap: empty line");
+    std::vector<st_element> rec = records_from_line(l);
+    Element *last = l.Cell.back();
+    double x0[6];
+    for (int k = 0; k < 6; ++k) x0[k] = last->X[k];
+    x0[4] = 0.;
+    x0[5] = deltap;
+    check(st_one_turn_matrix(default_context(), &GP, rec.data(), (int)rec.size(), x0, 1e-6, last->M),
+          "Cal_OneTurnMap");
+}
+
+/* The uncoupled Twiss at the end of the line from the last element's M:
+   what Track_tbt_ActionDiff reads of Cal_Twiss (simtrack.cpp:21313-21317)
+   for a ring whose planes do not couple. */
+inline void Cal_Twiss_End(Line &l)
+{
+    Element *e = l.Cell.back();
+    const double *m = e->M;
+    for (int p = 0; p < 2; ++p) {
+        const double a = m[(2 * p) * 6 + 2 * p], b = m[(2 * p) * 6 + 2 * p + 1], d = m[(2 * p + 1) * 6 + 2 * p + 1];
+        const double cosmu = (a + d) / 2;
+        if (!(std::fabs(cosmu) < 1.)) throw Error("Cal_Twiss: one-turn map unstable");
+        const double sinmu = std::copysign(std::sqrt(1 - cosmu * cosmu), b);
+        (p == 0 ? e->Beta1 : e->Beta2) = b / sinmu;
+        (p == 0 ? e->Alfa1 : e->Alfa2) = (a - d) / (2 * sinmu);
+    }
+}
+
+/* Track_tbt_ActionDiff (simtrack.cpp:21309-21383): npart particles at
+   nsigma (split by phase) with rnd(seed) betatron phases, nstep blocks of
+   nturn turns; per block the surviving count and the mean and rms of the
+   2Jx, 2Jy fitted to each survivor's turns, appended to ./Diff-output.dat.
+   The Twiss come from Cal_Orbit_Num + Cal_OneTurnMap at deltap = 0. */
+inline void Track_tbt_ActionDiff(Line &l, double nsigma, double phase, double emitx0, double emity0, int nturn,
+                                 int nstep, int npart)
+{
+    Cal_Orbit_Num(l, 0.);
+    Cal_OneTurnMap(l, 0.);
+    Cal_Twiss_End(l);
+    const Element *e = l.Cell.back();
+    Tracker &tr = tracker_for(l);
+    std::vector<double> rows((size_t)(nstep > 0 ? nstep : 1) * 6);
+    check(st_action_diff(tr.table(), e->Beta1, e->Alfa1, e->Beta2, e->Alfa2, nsigma, phase, emitx0, emity0, nturn,
+                         nstep, npart, &seed, rows.data()),
+          "Track_tbt_ActionDiff");
+    FILE *f = std::fopen("./Diff-output.dat", "w");
+    if (!f) throw Error("cannot open ./Diff-output.dat");
+    for (int j = 0; j < nstep; ++j) {
+        const double *r = &rows[(size_t)j * 6];
+        std::fprintf(f, "%d   %d    %g  %g     %g   %g\n", (int)r[0], (int)r[1], r[2], r[3], r[4], r[5]);
     }
     std::fclose(f);
 }

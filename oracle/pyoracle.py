@@ -40,6 +40,7 @@ def load(glibc=False):
     lib.orc_bb_slice_centers.argtypes = [I, D, P]
     lib.orc_fine_tune.argtypes = [I, P, D, D, P]
     lib.orc_closed_orbit.argtypes = [P, I, D, P, P, ctypes.POINTER(I)]
+    lib.orc_one_turn_matrix.argtypes = [P, I, P, D, P]
     lib.orc_fma_scan.argtypes = [P, I, D, D, D, P, D, D, I, D, D, P, I]
     lib.orc_lyapunov_scan.argtypes = [P, I, D, D, D, I, D, D, P, I]
     lib.orc_action_diff.argtypes = [P, I, D, D, D, D, D, D, D, D, I, I, I, P, P]
@@ -113,6 +114,17 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"oracle closed orbit failed ({rc})")
         return xc, xe, it.value
+
+    def one_turn_matrix(self, x_co, h=1e-6):
+        """The 6x6 one-turn matrix of Cal_OneTurnMap (simtrack.cpp:15122-15137)
+        by symmetric differences of step h about x_co, RFCAV voltages off."""
+        self._set()
+        m = np.zeros((6, 6))
+        xc = np.ascontiguousarray(np.asarray(x_co, dtype=np.float64)[:6])
+        rc = self.lib.orc_one_turn_matrix(_p(self.records), len(self.records), _p(xc), float(h), _p(m))
+        if rc != 0:
+            raise RuntimeError(f"oracle one-turn matrix failed ({rc})")
+        return m
 
     def fma_scan(self, deltap0, sigmax0, sigmay0, x_end, tune_low, tune_high, nturn, sigma_step, sigma_max):
         """Track_tbt_FMA (simtrack.cpp:21179-21241) one particle at a time: rows of 8."""

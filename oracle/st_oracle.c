@@ -2327,6 +2327,31 @@ out:
     return rc;
 }
 
+/* The one-turn matrix of Cal_OneTurnMap (simtrack.cpp:15122-15137: RFCAV
+   voltages zeroed, crab cavities and CCMULT on) by symmetric differences of
+   step h about x_co, one displaced particle after another (particle q moves
+   coordinate q/2 by +h for even q, -h for odd q); m[6*i+k] = dx_i/dx_k. */
+EXPORT int orc_one_turn_matrix(const st_element *line0, int n, const double x_co[6], double h, double m[36])
+{
+    st_element *line = (st_element *)malloc(sizeof(st_element) * (size_t)(n > 0 ? n : 1));
+    if (!line) return -5;
+    memcpy(line, line0, sizeof(st_element) * (size_t)n);
+    for (int i = 0; i < n; i++)
+        if (line[i].type == ST_RFCAV) line[i].vrf = 0.;
+    double xp[6], xm[6];
+    int rc = 0;
+    for (int k = 0; k < 6 && !rc; k++) {
+        for (int j = 0; j < 6; j++) xp[j] = xm[j] = x_co[j];
+        xp[k] += h;
+        xm[k] -= h;
+        if ((rc = pass_turn(line, n, xp, (uint32_t)(2 * k), 0))) break;
+        if ((rc = pass_turn(line, n, xm, (uint32_t)(2 * k + 1), 0))) break;
+        for (int i = 0; i < 6; i++) m[6 * i + k] = (xp[i] - xm[i]) / (2 * h);
+    }
+    free(line);
+    return rc;
+}
+
 /* Track_tbt_FMA, simtrack.cpp:21179-21241, one particle after another;
    rows as st_fma_scan.  Returns the number of rows, or a negative code. */
 EXPORT int orc_fma_scan(const st_element *line, int n, double deltap0, double sigmax0, double sigmay0,

@@ -69,7 +69,7 @@ EXPORTS = [
     "st_batch_track", "st_batch_track_device", "st_batch_autotune", "st_mlp_create", "st_mlp_destroy",
     "st_mlp_num_params", "st_mlp_get_params", "st_mlp_set_params", "st_mlp_forward", "st_mlp_gradients",
     "st_mlp_train_step", "st_batch_track_tbt_device", "st_surrogate_features", "st_closed_orbit", "st_fma_scan",
-    "st_lyapunov_scan", "st_fine_tune_window_device", "st_action_diff",
+    "st_one_turn_matrix", "st_lyapunov_scan", "st_fine_tune_window_device", "st_action_diff",
 ]
 
 
@@ -148,6 +148,7 @@ def load(path=None):
     lib.st_batch_track_tbt_device.argtypes = [P, P, P, I, I, P, P, P, I, BC]
     lib.st_surrogate_features.argtypes = [P, P, I, P, I, D, D, D, D, P]
     lib.st_closed_orbit.argtypes = [P, ctypes.POINTER(Globals), P, I, D, P, P, ctypes.POINTER(I)]
+    lib.st_one_turn_matrix.argtypes = [P, ctypes.POINTER(Globals), P, I, P, D, P]
     lib.st_fma_scan.argtypes = [P, D, D, D, P, D, D, I, D, D, P, I, ctypes.POINTER(I)]
     lib.st_lyapunov_scan.argtypes = [P, D, D, D, I, D, D, P, I, ctypes.POINTER(I)]
     lib.st_action_diff.argtypes = [P, D, D, D, D, D, D, D, D, I, I, I, P, P]

@@ -1648,6 +1648,38 @@ extern "C" int st_closed_orbit(st_ctx *c, const st_globals *gp, const st_element
     return ST_OK;
 }
 
+/* The 6x6 one-turn matrix about x_co: symmetric differences of step h of
+   bare passes (no loss test) with the RFCAV voltages off (crab cavities and
+   CCMULT stay on), the map Cal_OneTurnMap (simtrack.cpp:15122-15137)
+   expands.  The twelve displaced particles are one batch; m is row-major,
+   m[6*i + k] = d x_i / d x_k. */
+extern "C" int st_one_turn_matrix(st_ctx *c, const st_globals *gp, const st_element *rec, int n,
+                                  const double x_co[6], double h, double m[36])
+{
+    if (!c || !gp || (n > 0 && !rec) || n < 0 || !x_co || !m || !(h > 0.))
+        return fail(ST_ERR_ARG, "st_one_turn_matrix: bad argument");
+    std::vector<st_element> r(rec, rec + n);
+    for (auto &e : r)
+        if (e.type == ST_RFCAV) e.vrf = 0.;
+    st_table *t = nullptr;
+    int rc = st_table_build(c, gp, r.data(), n, &t);
+    if (rc) return rc;
+    std::unique_ptr<st_table, int (*)(st_table *)> guard(t, st_table_destroy);
+    double xb[12 * 6];
+    int st[12], lt[12], lp[12];
+    for (int q = 0; q < 12; ++q) {
+        for (int j = 0; j < 6; j++) xb[q * 6 + j] = x_co[j];
+        xb[q * 6 + q / 2] += (q & 1) ? -h : h;
+        st[q] = 1;
+        lt[q] = lp[q] = 0;
+    }
+    rc = track_host(t, xb, 12, 1, st, lt, lp, false, false, nullptr, -1, true);
+    if (rc) return rc;
+    for (int k = 0; k < 6; k++)
+        for (int i = 0; i < 6; i++) m[6 * i + k] = (xb[2 * k * 6 + i] - xb[(2 * k + 1) * 6 + i]) / (2 * h);
+    return ST_OK;
+}
+
 /* TBT of np particles on the device; the buffer stays in the context */
 static int tbt_device(st_table *t, const std::vector<double> &x, int nturn, std::vector<int> &stable,
                       double **d_tbt_out)

@@ -71,33 +71,20 @@ def lyapunov_scan(tab, deltap0, sigmax0, sigmay0, nturn=2048, sigma_step=0.05, s
 
 
 def one_turn_matrix(ctx, gp, records, x_co, h=1e-6):
-    """The 6x6 one-turn matrix about x_co: symmetric finite differences of
-    bare GPU-tracked turns (Element::Pass, no loss test) with every RF, crab
-    and CCMULT voltage off, as Cal_OneTurnMap (simtrack.cpp:15122) turns them
-    off for its TPS map.  The step h = 1e-6 balances truncation (O(h^2)
-    from the sextupoles) against roundoff (O(eps/h), largest in the delta
-    column): on RHIC the symplectic defect |M^T S M - S| is 1e-7 at
-    h = 1e-6 and 2.7e-6 at h = 1e-8."""
-    import torch
-
-    from ._capi import TRACK_BARE, Table
-    rec = np.array(records, copy=True)
-    rf = np.isin(rec["type"], [20, 21, 22])          # RFCAV, CRABRF, CCMULT
-    rec["vrf"][rf] = 0.
-    tab = Table(ctx, gp, rec)
-    x = np.repeat(np.asarray(x_co, dtype=np.float64)[None, :6], 12, axis=0)
-    for k in range(6):
-        x[2 * k, k] += h
-        x[2 * k + 1, k] -= h
-    dev = torch.device("cuda:0")
-    d_x = torch.tensor(x, dtype=torch.float64, device=dev)
-    d_s = torch.ones(12, dtype=torch.int32, device=dev)
-    d_l = torch.zeros(24, dtype=torch.int32, device=dev)
-    torch.cuda.synchronize()
-    tab.track_device(d_x.data_ptr(), 12, 0, 1, d_s.data_ptr(), d_l.data_ptr(), TRACK_BARE)
-    ctx.sync()
-    o = d_x.cpu().numpy()
-    return np.stack([(o[2 * k] - o[2 * k + 1]) / (2 * h) for k in range(6)], axis=1)
+    """The 6x6 one-turn matrix about x_co (st_one_turn_matrix): symmetric
+    finite differences of bare GPU-tracked turns (Element::Pass, no loss
+    test) with the RFCAV voltages off, as Cal_OneTurnMap (simtrack.cpp:
+    15122-15137) turns them off for its TPS map.  The step h = 1e-6 balances
+    truncation (O(h^2) from the sextupoles) against roundoff (O(eps/h),
+    largest in the delta column): on RHIC the symplectic defect
+    |M^T S M - S| is 1e-7 at h = 1e-6 and 2.7e-6 at h = 1e-8."""
+    lib = load()
+    rec = np.ascontiguousarray(records)
+    xc = np.ascontiguousarray(np.asarray(x_co, dtype=np.float64)[:6])
+    m = np.zeros((6, 6))
+    check(lib.st_one_turn_matrix(ctx.h, ctypes.byref(gp), _ptr(rec), len(rec), _ptr(xc), float(h), _ptr(m)),
+          "st_one_turn_matrix")
+    return m
 
 
 def twiss_from_matrix(m):
@@ -116,9 +103,13 @@ def twiss_from_matrix(m):
 
 def twiss_at_start(ctx, gp, records, deltap=0.):
     """Closed orbit, one-turn matrix and uncoupled Twiss at the start of the
-    line (the quantities Cal_Twiss gives the reference's drivers)."""
-    x_co, _, _ = cal_orbit_num(ctx, gp, records, deltap)
-    m = one_turn_matrix(ctx, gp, records, x_co)
+    line, the quantities Cal_Twiss (simtrack.cpp:15540) gives the reference's
+    drivers: the matrix is expanded about the closed orbit at the end of the
+    line with z = 0 and delta = deltap (Cal_OneTurnMap, cpp:15140-15142)."""
+    x_co, x_end, _ = cal_orbit_num(ctx, gp, records, deltap)
+    x0 = x_end.copy()
+    x0[4], x0[5] = 0., deltap
+    m = one_turn_matrix(ctx, gp, records, x0)
     return x_co, m, twiss_from_matrix(m)
 
 

@@ -73,6 +73,31 @@ def test_closed_orbit_momentum_dependence_is_the_dispersion():
     assert np.abs(xc0[:4]).max() < 1e-12
 
 
+def test_one_turn_matrix_is_the_linear_map():
+    """On a ring without sextupoles the map is linear: the matrix maps any
+    small displacement to the tracked one (a transposed or mis-indexed
+    matrix fails), does not depend on the step, and its tunes are the
+    thick-lens optics' (the engine's quads are kick-drift slices, 2e-3)."""
+    gp, ln = _fodo(0.)
+    orc = pyoracle.Oracle(gp, ln.records(), nthreads=1)
+    xc = np.zeros(6)
+    m = orc.one_turn_matrix(xc, 1e-6)
+    # the exact drift is not quite linear: O(h^2) terms of ~3e-8 relative at h = 1e-4
+    assert np.abs(m - orc.one_turn_matrix(xc, 1e-4)).max() < 1e-6 * np.abs(m).max()
+    r = np.random.Generator(np.random.PCG64(11))
+    dx = r.normal(size=6) * [1e-6, 1e-7, 1e-6, 1e-7, 1e-5, 1e-6]
+    out, _, _, _ = orc.track(dx[None, :].copy(), 1)
+    # second-order terms leave ~5e-7 of |out|; the transposed matrix misses by ~6
+    assert np.abs(out[0] - m @ dx).max() < 1e-5 * np.abs(out[0]).max()
+    assert np.abs(out[0] - m.T @ dx).max() > 1.0 * np.abs(out[0]).max()
+    bx, ax, qx, by, ay, qy = D.twiss_from_matrix(m)
+    opt = O.Optics(ln)
+    assert abs(qx - opt.tune_x) < 2e-3 and abs(qy - opt.tune_y) < 2e-3
+    assert abs(bx / opt.beta_x - 1) < 2e-3 and abs(by / opt.beta_y - 1) < 2e-3
+    # a flat ring: no x-y coupling
+    assert np.abs(m[0:2, 2:4]).max() == 0. and np.abs(m[2:4, 0:2]).max() == 0.
+
+
 def test_fma_tunes_at_small_amplitude_are_the_linear_tunes():
     gp, ln = _fodo()
     opt = O.Optics(ln)

@@ -65,13 +65,20 @@ def test_lyapunov_scan_bitwise(ctx):
 
 
 def test_twiss_at_start_rhic(ctx):
-    """One-turn matrix from GPU-tracked finite differences: RHIC's tunes equal
-    MAD-X's (header Q1, Q2), Twiss agrees with the thick-lens optics."""
+    """One-turn matrix from GPU-tracked finite differences: GPU = oracle bit
+    for bit; RHIC's tunes equal MAD-X's (header Q1, Q2), Twiss agrees with
+    the thick-lens optics, and the matrix is symplectic to the accuracy of
+    the differences."""
     gp, ln = W.rhic_example(RHIC)
-    x_co, m, (bx, ax, qx, by, ay, qy) = D.twiss_at_start(ctx, gp, ln.records())
+    rec = ln.records()
+    x_co, m, (bx, ax, qx, by, ay, qy) = D.twiss_at_start(ctx, gp, rec)
+    _, x_end, _ = D.cal_orbit_num(ctx, gp, rec, 0.)
+    x0 = x_end.copy()
+    x0[4:6] = 0.
+    assert_bitwise(m, pyoracle.Oracle(gp, rec).one_turn_matrix(x0), "RHIC one-turn matrix")
     opt = O.Optics(ln)
     assert abs(qx - 0.23) < 2e-6 and abs(qy - 0.22) < 2e-6
-    assert abs(bx / opt.beta_x - 1) < 1e-3 and abs(by / opt.beta_y - 1) < 1e-3
+    assert abs(bx / opt.beta_x - 1) < 1e-5 and abs(by / opt.beta_y - 1) < 1e-5
     s = np.zeros((6, 6))
     for k in range(3):
         s[2 * k, 2 * k + 1], s[2 * k + 1, 2 * k] = 1., -1.
@@ -89,3 +96,36 @@ def test_action_diff_bitwise(ctx):
     assert sd == sdo
     assert_bitwise(got, ref, "action diffusion rows")
     assert (got[:, 1] > 30).all()
+
+
+def test_cpp_action_diff_matches_python(ctx, tmp_path):
+    """Cal_Orbit_Num, Cal_OneTurnMap and Track_tbt_ActionDiff of the C++ API
+    (tests/cpp/rhic_diagnostics.cpp) on RHIC: the one-turn matrix, the Twiss,
+    the seed and Diff-output.dat equal the Python path's."""
+    import gzip
+    import subprocess
+    exe = os.path.join(os.path.dirname(RHIC), "..", "..", "cpp", "build", "rhic_diagnostics")
+    if not os.path.exists(exe):
+        pytest.fail("tests/cpp/build/rhic_diagnostics is not built (run __graft_entry__.build())")
+    plain = tmp_path / "rhic.tfs"
+    plain.write_bytes(gzip.open(RHIC).read())
+    r = subprocess.run([exe, str(plain)], cwd=tmp_path, capture_output=True, timeout=300)
+    assert r.returncode == 0, r.stderr.decode()
+    v = np.frombuffer(r.stdout, dtype=np.float64)
+    assert v.size == 6 + 36 + 5
+    x_last, m_cpp, tw, seed_cpp = v[:6], v[6:42].reshape(6, 6), v[42:46], v[46]
+    gp, ln = W.rhic_example(RHIC)
+    rec = ln.records()
+    _, x_end, _ = D.cal_orbit_num(ctx, gp, rec, 0.)
+    assert np.abs(x_last - x_end).max() < 1e-15
+    x0 = x_last.copy()
+    x0[4:6] = 0.
+    assert_bitwise(m_cpp, D.one_turn_matrix(ctx, gp, rec, x0), "C++ Cal_OneTurnMap")
+    bx, ax, _, by, ay, _ = D.twiss_from_matrix(m_cpp)
+    assert np.allclose(tw, [bx, ax, by, ay], rtol=1e-14, atol=0)
+    out = tmp_path / "py-Diff-output.dat"
+    rows, sd = D.Track_tbt_ActionDiff(st.Table(ctx, gp, rec), tuple(tw), 4.0, 0.7, 1e-8, 1e-8, 64, 2, 16,
+                                      filename=str(out))
+    assert sd == seed_cpp
+    assert (tmp_path / "Diff-output.dat").read_text() == out.read_text()
+    assert rows[:, 1].min() == 16
