@@ -37,6 +37,10 @@ def load(glibc=False):
     lib.orc_draws_per_turn.restype = ctypes.c_uint64
     lib.orc_cerrf.argtypes = [D, D, P, P]
     lib.orc_bb4d.argtypes = [D, D, D, D, D, D, P, P]
+    lib.orc_bb_soft_gaussian_4d.argtypes = [D, I, P, D, I, P, D, D, D, D]
+    lib.orc_bb_soft_gaussian_6d.argtypes = [D, I, P, D, I, P, D, D, D, D, I, I, P, P]
+    lib.orc_bb_soft_gaussian_6d_zb.argtypes = [D, I, P, D, I, P, D, D, D, D, I, I, P, P]
+    lib.orc_zboundary_slices_gaussian.argtypes = [P, I, D]
     lib.orc_bb_slice_centers.argtypes = [I, D, P]
     lib.orc_fine_tune.argtypes = [I, P, D, D, P]
     lib.orc_closed_orbit.argtypes = [P, I, D, P, P, ctypes.POINTER(I)]
@@ -159,3 +163,53 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"oracle action diffusion failed ({rc})")
         return rows, float(sd[0])
+
+
+# ------------------------------------------- strong-strong beam-beam
+
+def _beam(x):
+    return np.ascontiguousarray(np.array(x, dtype=np.float64, copy=True).reshape(-1, 6))
+
+
+def bb_soft_gaussian_4d(Np1, x1, Np2, x2, gamma1, gamma2, bbscale1, bbscale2):
+    """BeamBeam4D_SoftGaussian (simtrack.cpp:22846): new (x1, x2)."""
+    lib = load()
+    a, b = _beam(x1), _beam(x2)
+    lib.orc_bb_soft_gaussian_4d(Np1, len(a), _p(a), Np2, len(b), _p(b), gamma1, gamma2, bbscale1, bbscale2)
+    return a, b
+
+
+def bb_soft_gaussian_6d(Np1, x1, Np2, x2, gamma1, gamma2, bbscale1, bbscale2, nslice1, nslice2):
+    """BeamBeam6D_SoftGaussian (simtrack.cpp:22874): new (x1, x2) sorted by z
+    descending, and the slice centres (Z1, Z2)."""
+    lib = load()
+    a, b = _beam(x1), _beam(x2)
+    z1, z2 = np.zeros(nslice1), np.zeros(nslice2)
+    lib.orc_bb_soft_gaussian_6d(Np1, len(a), _p(a), Np2, len(b), _p(b), gamma1, gamma2, bbscale1, bbscale2, nslice1,
+                                nslice2, _p(z1), _p(z2))
+    return a, b, z1, z2
+
+
+def bb_soft_gaussian_6d_zb(Np1, x1, Np2, x2, gamma1, gamma2, bbscale1, bbscale2, zb1, zb2):
+    """BeamBeam6D_SoftGaussian with slice boundaries (simtrack.cpp:23134): new (x1, x2)."""
+    lib = load()
+    a, b = _beam(x1), _beam(x2)
+    zb1 = np.ascontiguousarray(zb1, dtype=np.float64)
+    zb2 = np.ascontiguousarray(zb2, dtype=np.float64)
+    lib.orc_bb_soft_gaussian_6d_zb(Np1, len(a), _p(a), Np2, len(b), _p(b), gamma1, gamma2, bbscale1, bbscale2,
+                                   len(zb1) - 1, len(zb2) - 1, _p(zb1), _p(zb2))
+    return a, b
+
+
+def zboundary_slices_gaussian(nslice, sigmal):
+    """Zboundary_slices_Gaussian (simtrack.cpp:3157): nslice + 1 boundaries, descending."""
+    zb = np.zeros(nslice + 1)
+    load().orc_zboundary_slices_gaussian(_p(zb), nslice, sigmal)
+    return zb
+
+
+def bb4d(x, y, gamma, N, sigmax, sigmay):
+    """BB4D (simtrack.cpp:6325-6382): (Dpx, Dpy)."""
+    a, b = ctypes.c_double(), ctypes.c_double()
+    load().orc_bb4d(x, y, gamma, N, sigmax, sigmay, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value

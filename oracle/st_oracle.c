@@ -2590,3 +2590,251 @@ EXPORT int orc_action_diff(const st_element *line, int n, double betx, double al
     free(tp);
     return rc;
 }
+
+/* ------------------------------------------- strong-strong beam-beam */
+
+/* Cal_Mean_RMS(Npart, PartDist, mean[2], rms[2]), simtrack.cpp:724-742:
+   sequential sums of x, y, x^2, y^2 over rows of 6 */
+static void mean_rms_xy(int n, const double *p, double mean[2], double rms[2])
+{
+    double sumx = 0, sumy = 0, sumx2 = 0, sumy2 = 0;
+    for (int i = 0; i < n; i++) {
+        double tx = p[6 * i], ty = p[6 * i + 2];
+        sumx = sumx + tx;
+        sumy = sumy + ty;
+        sumx2 = sumx2 + tx * tx;
+        sumy2 = sumy2 + ty * ty;
+    }
+    mean[0] = sumx / n;
+    mean[1] = sumy / n;
+    rms[0] = sqrt(sumx2 / n - mean[0] * mean[0]);
+    rms[1] = sqrt(sumy2 / n - mean[1] * mean[1]);
+}
+
+/* BeamBeam4D_SoftGaussian, simtrack.cpp:22846-22872: both beams' moments
+   first, then every particle of each beam kicked by the other beam's
+   Gaussian (BB4D about its centroid) */
+EXPORT void orc_bb_soft_gaussian_4d(double Np1, int Mp1, double *x1, double Np2, int Mp2, double *x2, double gamma1,
+                                    double gamma2, double bbscale1, double bbscale2)
+{
+    double m1[2], r1[2], m2[2], r2[2], dpx, dpy;
+    mean_rms_xy(Mp1, x1, m1, r1);
+    mean_rms_xy(Mp2, x2, m2, r2);
+    for (int i = 0; i < Mp1; i++) {
+        orc_bb4d(x1[6 * i] - m2[0], x1[6 * i + 2] - m2[1], gamma1, Np2, r2[0], r2[1], &dpx, &dpy);
+        x1[6 * i + 1] += dpx * bbscale1;
+        x1[6 * i + 3] += dpy * bbscale1;
+    }
+    for (int i = 0; i < Mp2; i++) {
+        orc_bb4d(x2[6 * i] - m1[0], x2[6 * i + 2] - m1[1], gamma2, Np1, r1[0], r1[1], &dpx, &dpy);
+        x2[6 * i + 1] += dpx * bbscale2;
+        x2[6 * i + 3] += dpy * bbscale2;
+    }
+}
+
+/* the collision order of both 6-D variants (cpp:22992-23015, 23192-23213):
+   slice pairs (i, j) with S = (Z1[i] - Z2[j]) / 2, ordered by
+   Z10 = -(Z1[i] + Z2[j]) / 2 ascending with the reference's exchange sort */
+static void ss_schedule(int n1, const double *Z1, int n2, const double *Z2, int *I1, int *I2, double *S)
+{
+    int n = n1 * n2;
+    double *Z10 = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int i = 0; i < n1; i++)
+        for (int j = 0; j < n2; j++) {
+            I1[i * n2 + j] = i;
+            I2[i * n2 + j] = j;
+            S[i * n2 + j] = (Z1[i] - Z2[j]) / 2.;
+            Z10[i * n2 + j] = -(Z1[i] + Z2[j]) / 2.;
+        }
+    for (int i = 0; i < n; i++)
+        for (int j = i + 1; j < n; j++)
+            if (Z10[j] < Z10[i]) {
+                double t = Z10[j];
+                Z10[j] = Z10[i];
+                Z10[i] = t;
+                t = S[j];
+                S[j] = S[i];
+                S[i] = t;
+                int it = I1[j];
+                I1[j] = I1[i];
+                I1[i] = it;
+                it = I2[j];
+                I2[j] = I2[i];
+                I2[i] = it;
+            }
+    free(Z10);
+}
+
+/* the exchange sort of cpp:22895-22915: rows of 6 by z descending */
+static void sort_z_desc(int n, double *x)
+{
+    double t[6];
+    for (int i = 0; i < n; i++)
+        for (int j = i + 1; j < n; j++)
+            if (x[j * 6 + 4] > x[i * 6 + 4]) {
+                memcpy(t, x + j * 6, sizeof t);
+                memcpy(x + j * 6, x + i * 6, sizeof t);
+                memcpy(x + i * 6, t, sizeof t);
+            }
+}
+
+/* drift by s, the moments, the kicks, the drift back (cpp:23035-23110) on
+   slice copies c1 (m1 rows, drift +S) and c2 (m2 rows, drift -S) */
+static void ss_collide_pair(int m1, double *c1, int m2, double *c2, double S, double N1, double N2, double gamma1,
+                            double gamma2, double bbscale1, double bbscale2)
+{
+    double mean1[2], rms1[2], mean2[2], rms2[2], dpx, dpy;
+    for (int j = 0; j < m1; j++) {
+        c1[j * 6 + 0] = c1[j * 6 + 0] + c1[j * 6 + 1] * S;
+        c1[j * 6 + 2] = c1[j * 6 + 2] + c1[j * 6 + 3] * S;
+    }
+    for (int j = 0; j < m2; j++) {
+        c2[j * 6 + 0] = c2[j * 6 + 0] - c2[j * 6 + 1] * S;
+        c2[j * 6 + 2] = c2[j * 6 + 2] - c2[j * 6 + 3] * S;
+    }
+    mean_rms_xy(m1, c1, mean1, rms1);
+    mean_rms_xy(m2, c2, mean2, rms2);
+    for (int j = 0; j < m1; j++) {
+        orc_bb4d(c1[6 * j] - mean2[0], c1[6 * j + 2] - mean2[1], gamma1, N2, rms2[0], rms2[1], &dpx, &dpy);
+        c1[6 * j + 1] = c1[6 * j + 1] + dpx * bbscale1;
+        c1[6 * j + 3] = c1[6 * j + 3] + dpy * bbscale1;
+    }
+    for (int j = 0; j < m2; j++) {
+        orc_bb4d(c2[6 * j] - mean1[0], c2[6 * j + 2] - mean1[1], gamma2, N1, rms1[0], rms1[1], &dpx, &dpy);
+        c2[6 * j + 1] = c2[6 * j + 1] + dpx * bbscale2;
+        c2[6 * j + 3] = c2[6 * j + 3] + dpy * bbscale2;
+    }
+    for (int j = 0; j < m1; j++) {
+        c1[j * 6 + 0] = c1[j * 6 + 0] - c1[j * 6 + 1] * S;
+        c1[j * 6 + 2] = c1[j * 6 + 2] - c1[j * 6 + 3] * S;
+    }
+    for (int j = 0; j < m2; j++) {
+        c2[j * 6 + 0] = c2[j * 6 + 0] + c2[j * 6 + 1] * S;
+        c2[j * 6 + 2] = c2[j * 6 + 2] + c2[j * 6 + 3] * S;
+    }
+}
+
+/* BeamBeam6D_SoftGaussian, simtrack.cpp:22874-23132: both beams sorted by z
+   (descending, left sorted), Mp/Nslice particles per slice (the last
+   Mp % Nslice rows collide with nothing), slice pairs collided in order of
+   their collision point.  Z1/Z2 (may be null) receive the slice centres the
+   reference writes to bunch1_Z.dat / bunch2_Z.dat. */
+EXPORT void orc_bb_soft_gaussian_6d(double Np1, int Mp1, double *x1, double Np2, int Mp2, double *x2, double gamma1,
+                                    double gamma2, double bbscale1, double bbscale2, int Nslice1, int Nslice2,
+                                    double *Z1o, double *Z2o)
+{
+    int n = Nslice1 * Nslice2;
+    int mc1 = Mp1 / Nslice1, mc2 = Mp2 / Nslice2;
+    double *Z1 = (double *)malloc(sizeof(double) * (size_t)Nslice1);
+    double *Z2 = (double *)malloc(sizeof(double) * (size_t)Nslice2);
+    int *I1 = (int *)malloc(sizeof(int) * (size_t)n), *I2 = (int *)malloc(sizeof(int) * (size_t)n);
+    double *S = (double *)malloc(sizeof(double) * (size_t)n);
+    sort_z_desc(Mp1, x1);
+    sort_z_desc(Mp2, x2);
+    for (int i = 0; i < Nslice1; i++) {
+        double t = 0;
+        for (int j = i * mc1; j < (i + 1) * mc1; j++) t = t + x1[6 * j + 4];
+        Z1[i] = t / mc1;
+    }
+    for (int i = 0; i < Nslice2; i++) {
+        double t = 0;
+        for (int j = i * mc2; j < (i + 1) * mc2; j++) t = t + x2[6 * j + 4];
+        Z2[i] = t / mc2;
+    }
+    if (Z1o) memcpy(Z1o, Z1, sizeof(double) * (size_t)Nslice1);
+    if (Z2o) memcpy(Z2o, Z2, sizeof(double) * (size_t)Nslice2);
+    ss_schedule(Nslice1, Z1, Nslice2, Z2, I1, I2, S);
+    for (int i = 0; i < n; i++)
+        ss_collide_pair(mc1, x1 + (size_t)I1[i] * mc1 * 6, mc2, x2 + (size_t)I2[i] * mc2 * 6, S[i], Np1 / Nslice1,
+                        Np2 / Nslice2, gamma1, gamma2, bbscale1, bbscale2);
+    free(Z1);
+    free(Z2);
+    free(I1);
+    free(I2);
+    free(S);
+}
+
+/* BeamBeam6D_SoftGaussian with slice boundaries, simtrack.cpp:23134-23284:
+   slice i of a beam holds, in row order, the particles with
+   zb[i+1] <= z < zb[i]; rows stay in place.  Each pair collides through
+   BeamBeam4D_SoftGaussian with charges scaled by the slice populations.
+   The reference clears the slice counters of both beams over Nslice1
+   entries (cpp:23161-23165); here every counter starts at zero. */
+EXPORT void orc_bb_soft_gaussian_6d_zb(double Np1, int Mp1, double *x1, double Np2, int Mp2, double *x2,
+                                       double gamma1, double gamma2, double bbscale1, double bbscale2, int Nslice1,
+                                       int Nslice2, const double *zb1, const double *zb2)
+{
+    int n = Nslice1 * Nslice2;
+    double *Z1 = (double *)calloc((size_t)Nslice1, sizeof(double));
+    double *Z2 = (double *)calloc((size_t)Nslice2, sizeof(double));
+    int *I1 = (int *)malloc(sizeof(int) * (size_t)n), *I2 = (int *)malloc(sizeof(int) * (size_t)n);
+    double *S = (double *)malloc(sizeof(double) * (size_t)n);
+    double *c1 = (double *)malloc(sizeof(double) * 6 * (size_t)(Mp1 > 0 ? Mp1 : 1));
+    double *c2 = (double *)malloc(sizeof(double) * 6 * (size_t)(Mp2 > 0 ? Mp2 : 1));
+    for (int i = 0; i < Nslice1; i++) {
+        double t = 0;
+        int m = 0;
+        for (int j = 0; j < Mp1; j++)
+            if (x1[j * 6 + 4] < zb1[i] && x1[j * 6 + 4] >= zb1[i + 1]) {
+                t = t + x1[6 * j + 4];
+                m++;
+            }
+        Z1[i] = t / m;
+    }
+    for (int i = 0; i < Nslice2; i++) {
+        double t = 0;
+        int m = 0;
+        for (int j = 0; j < Mp2; j++)
+            if (x2[j * 6 + 4] < zb2[i] && x2[j * 6 + 4] >= zb2[i + 1]) {
+                t = t + x2[6 * j + 4];
+                m++;
+            }
+        Z2[i] = t / m;
+    }
+    ss_schedule(Nslice1, Z1, Nslice2, Z2, I1, I2, S);
+    for (int i = 0; i < n; i++) {
+        const double hi1 = zb1[I1[i]], lo1 = zb1[I1[i] + 1], hi2 = zb2[I2[i]], lo2 = zb2[I2[i] + 1];
+        int m1 = 0, m2 = 0;
+        for (int j = 0; j < Mp1; j++)
+            if (x1[j * 6 + 4] < hi1 && x1[j * 6 + 4] >= lo1) memcpy(c1 + 6 * m1++, x1 + 6 * j, 6 * sizeof(double));
+        for (int j = 0; j < Mp2; j++)
+            if (x2[j * 6 + 4] < hi2 && x2[j * 6 + 4] >= lo2) memcpy(c2 + 6 * m2++, x2 + 6 * j, 6 * sizeof(double));
+        for (int j = 0; j < m1; j++) {
+            c1[j * 6 + 0] = c1[j * 6 + 0] + c1[j * 6 + 1] * S[i];
+            c1[j * 6 + 2] = c1[j * 6 + 2] + c1[j * 6 + 3] * S[i];
+        }
+        for (int j = 0; j < m2; j++) {
+            c2[j * 6 + 0] = c2[j * 6 + 0] - c2[j * 6 + 1] * S[i];
+            c2[j * 6 + 2] = c2[j * 6 + 2] - c2[j * 6 + 3] * S[i];
+        }
+        orc_bb_soft_gaussian_4d(Np1 * m1 / Mp1, m1, c1, Np2 * m2 / Mp2, m2, c2, gamma1, gamma2, bbscale1, bbscale2);
+        for (int j = 0; j < m1; j++) {
+            c1[j * 6 + 0] = c1[j * 6 + 0] - c1[j * 6 + 1] * S[i];
+            c1[j * 6 + 2] = c1[j * 6 + 2] - c1[j * 6 + 3] * S[i];
+        }
+        for (int j = 0; j < m2; j++) {
+            c2[j * 6 + 0] = c2[j * 6 + 0] + c2[j * 6 + 1] * S[i];
+            c2[j * 6 + 2] = c2[j * 6 + 2] + c2[j * 6 + 3] * S[i];
+        }
+        m1 = m2 = 0;
+        for (int j = 0; j < Mp1; j++)
+            if (x1[j * 6 + 4] < hi1 && x1[j * 6 + 4] >= lo1) memcpy(x1 + 6 * j, c1 + 6 * m1++, 6 * sizeof(double));
+        for (int j = 0; j < Mp2; j++)
+            if (x2[j * 6 + 4] < hi2 && x2[j * 6 + 4] >= lo2) memcpy(x2 + 6 * j, c2 + 6 * m2++, 6 * sizeof(double));
+    }
+    free(Z1);
+    free(Z2);
+    free(I1);
+    free(I2);
+    free(S);
+    free(c1);
+    free(c2);
+}
+
+/* Zboundary_slices_Gaussian, simtrack.cpp:3157-3168 */
+EXPORT void orc_zboundary_slices_gaussian(double *zb, int Nslice, double sigmal)
+{
+    zb[0] = sigmal * 20;
+    zb[Nslice] = -sigmal * 20;
+    for (int i = 1; i <= Nslice - 1; i++) zb[i] = -st_ugaussian_pinv(i * 1.0 / Nslice) * sigmal;
+}

@@ -1,0 +1,193 @@
+"""Strong-strong soft-Gaussian beam-beam on the oracle (CPU): the
+restatement of BeamBeam4D_SoftGaussian (simtrack.cpp:22846),
+BeamBeam6D_SoftGaussian (cpp:22874, equal-population slices after a z
+sort; cpp:23134, fixed slice boundaries) and Zboundary_slices_Gaussian
+(cpp:3157), pinned by:
+- the closed form of a round Gaussian beam's kick (the moments, the
+  centroid offset and which beam's charge and energy enter);
+- mirror symmetry: a beam and its mirror image in (x, px, y, py) must come
+  out as mirror images, bit for bit (any swapped index, sign or drift
+  direction breaks it);
+- a head-on single slice (all z = 0) of either 6-D variant equals the 4-D
+  collision bit for bit;
+- the z sort (descending, rows left sorted) and the Mp % Nslice rows that
+  collide with nothing;
+- an independent pure-Python restatement on tiny beams (collision order,
+  drifts, per-pair charges);
+- the boundaries against scipy's Gaussian quantiles."""
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle
+
+RP = 1.53469826769e-18
+
+
+def _beam(n, seed, sx=1e-4, sy=6e-5, sz=0.05, off=(0., 0.)):
+    r = np.random.Generator(np.random.PCG64(seed))
+    x = np.zeros((n, 6))
+    x[:, 0] = r.normal(off[0], sx, n)
+    x[:, 1] = r.normal(0, sx / 0.5, n)
+    x[:, 2] = r.normal(off[1], sy, n)
+    x[:, 3] = r.normal(0, sy / 0.5, n)
+    x[:, 4] = r.normal(0, sz, n)
+    x[:, 5] = r.normal(0, 1e-4, n)
+    return x
+
+
+def _mirror(x):
+    m = x.copy()
+    m[:, 0:4] = -m[:, 0:4]
+    return m
+
+
+def test_zboundary_is_the_gaussian_quantiles():
+    from scipy.stats import norm
+    zb = pyoracle.zboundary_slices_gaussian(7, 0.08)
+    assert zb[0] == 0.08 * 20 and zb[-1] == -0.08 * 20
+    ref = -norm.ppf(np.arange(1, 7) / 7) * 0.08
+    assert np.allclose(zb[1:-1], ref, rtol=1e-13, atol=1e-17)
+    assert np.all(np.diff(zb) < 0)
+
+
+def test_4d_round_beam_kick_is_the_closed_form():
+    r = np.random.Generator(np.random.PCG64(4))
+    # a round beam 2: every (a, b) also appears as (b, a), so its x and y
+    # moments are equal and BB4D takes the round branch
+    h = r.normal(0, 1e-4, size=(500, 2))
+    b2 = np.zeros((1000, 6))
+    b2[:500, 0], b2[:500, 2] = h[:, 0] + 3e-5, h[:, 1] + 3e-5
+    b2[500:, 0], b2[500:, 2] = h[:, 1] + 3e-5, h[:, 0] + 3e-5
+    b1 = _beam(64, 5, sx=2e-4, sy=2e-4)
+    N1, N2, g1, g2, k1, k2 = 2.2e11, 1.4e11, 293.0, 105.0, 0.8, 1.3
+    o1, o2 = pyoracle.bb_soft_gaussian_4d(N1, b1, N2, b2, g1, g2, k1, k2)
+    mx, my = b2[:, 0].mean(), b2[:, 2].mean()
+    sig = math.sqrt(np.mean(b2[:, 0] ** 2) - mx * mx)
+    dx, dy = b1[:, 0] - mx, b1[:, 2] - my
+    r2 = dx * dx + dy * dy
+    f = 2 * N2 * RP / g1 / r2 * (1 - np.exp(-r2 / 2 / sig / sig)) * k1
+    assert np.allclose(o1[:, 1] - b1[:, 1], f * dx, rtol=1e-9, atol=0)
+    assert np.allclose(o1[:, 3] - b1[:, 3], f * dy, rtol=1e-9, atol=0)
+    assert np.array_equal(o1[:, [0, 2, 4, 5]], b1[:, [0, 2, 4, 5]])
+    # beam 2 is kicked by beam 1's charge at beam 2's energy
+    assert np.array_equal(o2[:, [0, 2, 4, 5]], b2[:, [0, 2, 4, 5]])
+    assert not np.array_equal(o2[:, 1], b2[:, 1])
+
+
+def test_4d_mirror_symmetry():
+    b1 = _beam(300, 6, off=(2e-5, -1e-5))
+    o1, o2 = pyoracle.bb_soft_gaussian_4d(1e11, b1, 1e11, _mirror(b1), 200., 200., 1., 1.)
+    assert np.array_equal(o2, _mirror(o1))
+
+
+@pytest.mark.parametrize("ns", [(1, 1), (4, 4), (5, 3)])
+def test_6d_mirror_symmetry(ns):
+    b1 = _beam(240, 7, off=(3e-5, 1e-5))
+    o1, o2, z1, z2 = pyoracle.bb_soft_gaussian_6d(1e11, b1, 1e11, _mirror(b1), 200., 200., 1., 1., ns[0], ns[1])
+    if ns[0] == ns[1]:
+        assert np.array_equal(o2, _mirror(o1))
+        assert np.array_equal(z1, z2)
+    zb = pyoracle.zboundary_slices_gaussian(ns[0], 0.05)
+    p1, p2 = pyoracle.bb_soft_gaussian_6d_zb(1e11, b1, 1e11, _mirror(b1), 200., 200., 1., 1., zb, zb)
+    assert np.array_equal(p2, _mirror(p1))
+
+
+def test_6d_head_on_single_slice_is_4d():
+    b1, b2 = _beam(200, 8), _beam(150, 9, off=(1e-5, 2e-5))
+    b1[:, 4] = 0.
+    b2[:, 4] = 0.
+    args = (1.1e11, 0.9e11, 250., 260., 0.7, 1.2)
+    f1, f2 = pyoracle.bb_soft_gaussian_4d(args[0], b1, args[1], b2, *args[2:])
+    s1, s2, _, _ = pyoracle.bb_soft_gaussian_6d(args[0], b1, args[1], b2, *args[2:], 1, 1)
+    assert np.array_equal(s1, f1) and np.array_equal(s2, f2)
+    z1, z2 = pyoracle.bb_soft_gaussian_6d_zb(args[0], b1, args[1], b2, *args[2:], np.array([1., -1.]),
+                                              np.array([1., -1.]))
+    assert np.array_equal(z1, f1) and np.array_equal(z2, f2)
+
+
+def test_6d_sorts_by_z_and_leaves_the_tail():
+    b1, b2 = _beam(103, 10), _beam(98, 11)
+    o1, o2, z1, z2 = pyoracle.bb_soft_gaussian_6d(1e11, b1, 1e11, b2, 200., 200., 1., 1., 5, 4)
+    for b, o, ns in ((b1, o1, 5), (b2, o2, 4)):
+        order = np.argsort(-b[:, 4], kind="stable")
+        assert np.array_equal(o[:, 4], b[order, 4])
+        assert np.array_equal(o[:, 5], b[order, 5])
+        tail = len(b) - (len(b) // ns) * ns
+        assert np.array_equal(o[len(b) - tail:], b[order][len(b) - tail:])
+    m = 103 // 5
+    assert np.allclose(z1, [b1[np.argsort(-b1[:, 4])][i * m:(i + 1) * m, 4].mean() for i in range(5)], rtol=1e-13)
+
+
+def _py_pair(c1, c2, S, N1, N2, g1, g2, k1, k2):
+    """one slice pair in plain Python: drift, moments, BB4D kicks, drift back"""
+    def moments(c):
+        n = len(c)
+        sx = sy = sx2 = sy2 = 0.
+        for row in c:
+            sx += row[0]
+            sy += row[2]
+            sx2 += row[0] * row[0]
+            sy2 += row[2] * row[2]
+        mx, my = sx / n, sy / n
+        return mx, my, math.sqrt(sx2 / n - mx * mx), math.sqrt(sy2 / n - my * my)
+    for row in c1:
+        row[0] += row[1] * S
+        row[2] += row[3] * S
+    for row in c2:
+        row[0] -= row[1] * S
+        row[2] -= row[3] * S
+    m1, m2 = moments(c1), moments(c2)
+    for c, (mx, my, sx, sy), N, g, k in ((c1, m2, N2, g1, k1), (c2, m1, N1, g2, k2)):
+        for row in c:
+            dpx, dpy = pyoracle.bb4d(row[0] - mx, row[2] - my, g, N, sx, sy)
+            row[1] += dpx * k
+            row[3] += dpy * k
+    for row in c1:
+        row[0] -= row[1] * S
+        row[2] -= row[3] * S
+    for row in c2:
+        row[0] += row[1] * S
+        row[2] += row[3] * S
+
+
+def _py_order(Z1, Z2):
+    pairs = [(-(a + b) / 2., i, j) for i, a in enumerate(Z1) for j, b in enumerate(Z2)]
+    pairs.sort(key=lambda t: t[0])
+    return [(i, j, (Z1[i] - Z2[j]) / 2.) for _, i, j in pairs]
+
+
+def test_6d_sorted_variant_against_python():
+    b1, b2 = _beam(23, 12), _beam(17, 13, off=(2e-5, -3e-5))
+    N1, N2, g1, g2, k1, k2 = 1.3e11, 0.8e11, 230., 240., 0.9, 1.1
+    o1, o2, Z1, Z2 = pyoracle.bb_soft_gaussian_6d(N1, b1, N2, b2, g1, g2, k1, k2, 3, 2)
+    p1 = [list(r) for r in b1[np.argsort(-b1[:, 4], kind="stable")]]
+    p2 = [list(r) for r in b2[np.argsort(-b2[:, 4], kind="stable")]]
+    m1, m2 = 23 // 3, 17 // 2
+    z1 = [sum(p1[k][4] for k in range(i * m1, (i + 1) * m1)) / m1 for i in range(3)]
+    z2 = [sum(p2[k][4] for k in range(i * m2, (i + 1) * m2)) / m2 for i in range(2)]
+    assert z1 == list(Z1) and z2 == list(Z2)
+    for i, j, S in _py_order(z1, z2):
+        _py_pair(p1[i * m1:(i + 1) * m1], p2[j * m2:(j + 1) * m2], S, N1 / 3, N2 / 2, g1, g2, k1, k2)
+    assert np.allclose(o1, np.array(p1), rtol=1e-12, atol=1e-18)
+    assert np.allclose(o2, np.array(p2), rtol=1e-12, atol=1e-18)
+
+
+def test_6d_boundary_variant_against_python():
+    b1, b2 = _beam(31, 14), _beam(27, 15, off=(-2e-5, 1e-5))
+    zb1, zb2 = pyoracle.zboundary_slices_gaussian(3, 0.05), pyoracle.zboundary_slices_gaussian(3, 0.05)
+    N1, N2, g1, g2, k1, k2 = 1.3e11, 0.8e11, 230., 240., 0.9, 1.1
+    o1, o2 = pyoracle.bb_soft_gaussian_6d_zb(N1, b1, N2, b2, g1, g2, k1, k2, zb1, zb2)
+    p1, p2 = [list(r) for r in b1], [list(r) for r in b2]
+
+    def members(p, zb, i):
+        return [k for k in range(len(p)) if zb[i + 1] <= p[k][4] < zb[i]]
+    z1 = [sum(p1[k][4] for k in members(p1, zb1, i)) / len(members(p1, zb1, i)) for i in range(3)]
+    z2 = [sum(p2[k][4] for k in members(p2, zb2, i)) / len(members(p2, zb2, i)) for i in range(3)]
+    for i, j, S in _py_order(z1, z2):
+        k1s, k2s = members(p1, zb1, i), members(p2, zb2, j)
+        c1, c2 = [p1[k] for k in k1s], [p2[k] for k in k2s]
+        _py_pair(c1, c2, S, N1 * len(c1) / len(p1), N2 * len(c2) / len(p2), g1, g2, k1, k2)
+    assert np.allclose(o1, np.array(p1), rtol=1e-12, atol=1e-18)
+    assert np.allclose(o2, np.array(p2), rtol=1e-12, atol=1e-18)
