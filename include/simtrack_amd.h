@@ -439,6 +439,48 @@ int st_lyapunov_scan(st_table *t, double deltap0, double sigmax0, double sigmay0
 int st_action_diff(st_table *t, double betx, double alfx, double bety, double alfy, double nsigma, double phase,
                    double emitx0, double emity0, int nturn, int nstep, int npart, double *seed, double *rows);
 
+/* ------------------------------------ strong-strong beam-beam */
+
+/*
+ * Soft-Gaussian strong-strong beam-beam between two macroparticle beams of
+ * Mp1 and Mp2 rows of 6 doubles (the reference's layout), charges Np1 and
+ * Np2, Lorentz factors gamma1 and gamma2.  Each beam is kicked by BB4D
+ * about the other beam's (or slice's) centroid with its rms sizes, times
+ * its bbscale.  Moments are summed in a fixed order (repeatable bit for
+ * bit), not the reference's sequential order, so they agree with it to
+ * rounding.  Blocking; d_x1/d_x2 are device arrays, the _host forms take
+ * host arrays and copy through the context.
+ *
+ * st_bb_soft_gaussian_4d: BeamBeam4D_SoftGaussian (simtrack.cpp:22846).
+ * st_bb_soft_gaussian_6d: BeamBeam6D_SoftGaussian (cpp:22874): both beams
+ *   sorted by z (descending; the rows are left sorted; equal z keep row
+ *   order), Mp/Nslice rows per slice (the last Mp % Nslice collide with
+ *   nothing), slice pairs collided in order of their collision point with
+ *   drifts of +-S.  Z1/Z2 (host, may be null) receive the slice centres the
+ *   reference writes to bunch1_Z.dat / bunch2_Z.dat.
+ * st_bb_soft_gaussian_6d_zb: the slice-boundary form (cpp:23134): slice i
+ *   holds the rows with zb[i+1] <= z < zb[i] (zb: Nslice + 1 host doubles,
+ *   strictly decreasing); rows stay in place; pair charges Np * m / Mp.
+ * st_zboundary_slices_gaussian: Zboundary_slices_Gaussian (cpp:3157).
+ */
+int st_bb_soft_gaussian_4d(st_ctx *ctx, double Np1, int Mp1, double *d_x1, double Np2, int Mp2, double *d_x2,
+                           double gamma1, double gamma2, double bbscale1, double bbscale2);
+int st_bb_soft_gaussian_6d(st_ctx *ctx, double Np1, int Mp1, double *d_x1, double Np2, int Mp2, double *d_x2,
+                           double gamma1, double gamma2, double bbscale1, double bbscale2, int Nslice1, int Nslice2,
+                           double *Z1, double *Z2);
+int st_bb_soft_gaussian_6d_zb(st_ctx *ctx, double Np1, int Mp1, double *d_x1, double Np2, int Mp2, double *d_x2,
+                              double gamma1, double gamma2, double bbscale1, double bbscale2, int Nslice1,
+                              int Nslice2, const double *zb1, const double *zb2);
+int st_bb_soft_gaussian_4d_host(st_ctx *ctx, double Np1, int Mp1, double *x1, double Np2, int Mp2, double *x2,
+                                double gamma1, double gamma2, double bbscale1, double bbscale2);
+int st_bb_soft_gaussian_6d_host(st_ctx *ctx, double Np1, int Mp1, double *x1, double Np2, int Mp2, double *x2,
+                                double gamma1, double gamma2, double bbscale1, double bbscale2, int Nslice1,
+                                int Nslice2, double *Z1, double *Z2);
+int st_bb_soft_gaussian_6d_zb_host(st_ctx *ctx, double Np1, int Mp1, double *x1, double Np2, int Mp2, double *x2,
+                                   double gamma1, double gamma2, double bbscale1, double bbscale2, int Nslice1,
+                                   int Nslice2, const double *zb1, const double *zb2);
+int st_zboundary_slices_gaussian(double *zb, int Nslice, double sigmal);
+
 #ifdef __cplusplus
 }
 #endif

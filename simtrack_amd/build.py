@@ -53,6 +53,7 @@ def build(verbose=False):
     for src, extra in (("st_kernels.hip", ["--offload-arch=" + ARCH]),
                        ("st_tune.hip", ["--offload-arch=" + ARCH]),
                        ("st_mlp.hip", ["--offload-arch=" + ARCH]),
+                       ("st_ssbb.hip", ["--offload-arch=" + ARCH]),
                        ("st_capi.cpp", ["--offload-arch=" + ARCH, "-x", "hip"]),
                        ("st_da.cpp", [])):
         obj = os.path.join(objdir, src.rsplit(".", 1)[0] + ".o")

@@ -1,0 +1,133 @@
+"""Strong-strong soft-Gaussian beam-beam on the GPU (csrc/st_ssbb.hip)
+against the oracle's restatement of simtrack.cpp:22846-23284.
+
+The GPU sums slice moments in a fixed blocked order, the reference
+sequentially, so kicks agree to rounding: the bar for one collision is
+1e-11 of each column's scale.  Over turns of tracking and collisions the
+bulk stays at rounding and single particles are bounded by the complex
+error function's own accuracy (see the multi-turn test).  Everything that
+involves no sum is exact: the z sort's permutation, the z and pt columns,
+and mirror symmetry (a beam against its mirror image in x, px, y, py must
+come out mirrored bit for bit)."""
+import numpy as np
+import pytest
+import torch
+
+import pyoracle
+import simtrack_amd as st
+from simtrack_amd import strong_strong as SS
+from simtrack_amd import workloads as W
+
+from test_diagnostics import _fodo
+from test_strong_strong import _beam, _mirror
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-11
+
+
+def _close(g, o, what):
+    for k in range(6):
+        scale = max(np.abs(o[:, k]).max(), 1e-300)
+        err = np.abs(g[:, k] - o[:, k]).max()
+        assert err <= TOL * scale, f"{what}: column {k} differs by {err:.3e} (scale {scale:.3e})"
+
+
+ARGS = (1.2e11, 0.9e11, 230., 240., 0.8, 1.1)   # Np1, Np2, gamma1, gamma2, bbscale1, bbscale2
+
+
+def test_4d_matches_oracle(ctx):
+    b1, b2 = _beam(20000, 1), _beam(15000, 2, sx=8e-5, sy=5e-5, off=(1e-5, -2e-5))
+    Np1, Np2, g1, g2, k1, k2 = ARGS
+    o1, o2 = pyoracle.bb_soft_gaussian_4d(Np1, b1, Np2, b2, g1, g2, k1, k2)
+    h1, h2 = b1.copy(), b2.copy()
+    SS.BeamBeam4D_SoftGaussian(ctx, Np1, h1, Np2, h2, g1, g2, k1, k2)
+    for g, o, b in ((h1, o1, b1), (h2, o2, b2)):
+        assert np.array_equal(g[:, [0, 2, 4, 5]], b[:, [0, 2, 4, 5]])
+        _close(g, o, "4-D")
+    # the device form on torch tensors gives the same bits as the host form
+    d1 = torch.tensor(b1, device="cuda:0")
+    d2 = torch.tensor(b2, device="cuda:0")
+    SS.BeamBeam4D_SoftGaussian(ctx, Np1, d1, Np2, d2, g1, g2, k1, k2)
+    assert np.array_equal(d1.cpu().numpy(), h1) and np.array_equal(d2.cpu().numpy(), h2)
+
+
+@pytest.mark.parametrize("ns", [(5, 4), (1, 1), (8, 8)])
+def test_6d_sorted_matches_oracle(ctx, ns):
+    b1, b2 = _beam(12003, 3), _beam(9001, 4, off=(2e-5, 1e-5))
+    Np1, Np2, g1, g2, k1, k2 = ARGS
+    o1, o2, oz1, oz2 = pyoracle.bb_soft_gaussian_6d(Np1, b1, Np2, b2, g1, g2, k1, k2, *ns)
+    h1, h2 = b1.copy(), b2.copy()
+    z1, z2 = SS.BeamBeam6D_SoftGaussian(ctx, Np1, h1, Np2, h2, g1, g2, k1, k2, *ns)
+    for g, o in ((h1, o1), (h2, o2)):
+        assert np.array_equal(g[:, 4], o[:, 4]) and np.array_equal(g[:, 5], o[:, 5]), "z sort permutation"
+        _close(g, o, f"6-D sorted {ns}")
+    # a slice centre is a mean of z values that largely cancel: its rounding
+    # is relative to the z scale, not to the centre itself
+    for z, oz, b in ((z1, oz1, b1), (z2, oz2, b2)):
+        assert np.abs(z - oz).max() <= 1e-13 * np.abs(b[:, 4]).max()
+
+
+def test_6d_boundaries_match_oracle(ctx):
+    b1, b2 = _beam(11000, 5), _beam(13000, 6, off=(-1e-5, 2e-5))
+    zb1, zb2 = SS.Zboundary_slices_Gaussian(6, 0.05), SS.Zboundary_slices_Gaussian(4, 0.05)
+    Np1, Np2, g1, g2, k1, k2 = ARGS
+    o1, o2 = pyoracle.bb_soft_gaussian_6d_zb(Np1, b1, Np2, b2, g1, g2, k1, k2, zb1, zb2)
+    h1, h2 = b1.copy(), b2.copy()
+    assert SS.BeamBeam6D_SoftGaussian(ctx, Np1, h1, Np2, h2, g1, g2, k1, k2, 6, 4, zb1, zb2) is None
+    for g, o, b in ((h1, o1, b1), (h2, o2, b2)):
+        assert np.array_equal(g[:, 4:6], b[:, 4:6]), "rows stay in place"
+        _close(g, o, "6-D boundaries")
+
+
+def test_mirror_symmetry_is_exact_on_the_gpu(ctx):
+    b1 = _beam(30000, 7, off=(3e-5, 1e-5))
+    for zb in (None, SS.Zboundary_slices_Gaussian(5, 0.05)):
+        h1, h2 = b1.copy(), _mirror(b1)
+        SS.BeamBeam6D_SoftGaussian(ctx, 1e11, h1, 1e11, h2, 200., 200., 1., 1., 5, 5, zb, zb)
+        assert np.array_equal(h2, _mirror(h1))
+        assert not np.array_equal(h1, b1)
+
+
+def test_ten_turns_of_tracking_and_collisions(ctx):
+    gp, ln = _fodo(0.)
+    rec = ln.records()
+    tab = st.Table(ctx, gp, rec)
+    orc = pyoracle.Oracle(gp, rec, nthreads=8)
+    b1, b2 = _beam(4000, 8, sx=2e-4, sy=2e-4), _beam(3000, 9, sx=2e-4, sy=2e-4, off=(1e-5, 0.))
+    for b in (b1, b2):
+        b[:, 1] *= 0.05
+        b[:, 3] *= 0.05
+    zb = SS.Zboundary_slices_Gaussian(4, 0.05)
+    # 2e10 per beam: a beam-beam parameter of ~0.01 per collision at these
+    # sizes.  The moments differ from the reference's sequential sums by
+    # rounding (~1e-16 of a kick); the bulk of the beams stays that close.
+    # A particle whose complex-error-function argument sits at one of the
+    # algorithm's region boundaries (xlim, ylim, simtrack.cpp:6258-6323) can
+    # be moved across it by such a difference; the two sides agree to the
+    # algorithm's own accuracy (2e-10 relative), which bounds the worst
+    # particle.  (Seen here: 1e-16 .. 2e-14 for six turns, then 1.6e-11.)
+    Np1, Np2, g1, g2, k1, k2 = 2e10, 2e10, gp.gamma, gp.gamma, 1., 1.
+    g = [b1.copy(), b2.copy()]
+    o = [b1.copy(), b2.copy()]
+    worst, bulk = [], []
+    for turn in range(10):
+        for k in range(2):
+            g[k], sg, _, _ = tab.track(g[k], 1)
+            o[k], so, _, _ = orc.track(o[k], 1)
+            assert sg.all() and so.all()
+        SS.BeamBeam6D_SoftGaussian(ctx, Np1, g[0], Np2, g[1], g1, g2, k1, k2, 4, 4, zb, zb)
+        o[0], o[1] = pyoracle.bb_soft_gaussian_6d_zb(Np1, o[0], Np2, o[1], g1, g2, k1, k2, zb, zb)
+        rel = [np.abs(g[k][:, c] - o[k][:, c]) / np.abs(o[k][:, c]).max() for k in range(2) for c in range(4)]
+        worst.append(max(r.max() for r in rel))
+        bulk.append(max(np.median(r) for r in rel))
+    print("worst / median relative difference per turn:",
+          " ".join(f"{w:.1e}/{m:.1e}" for w, m in zip(worst, bulk)))
+    assert max(bulk) <= 1e-13
+    assert max(worst) <= 2e-10
+
+
+def test_bad_boundaries_are_rejected(ctx):
+    b1, b2 = _beam(100, 10), _beam(100, 11)
+    with pytest.raises(st.STError, match="decrease"):
+        SS.BeamBeam6D_SoftGaussian(ctx, 1e11, b1, 1e11, b2, 200., 200., 1., 1., 2, 2, np.array([1., 2., -1.]),
+                                   np.array([1., 0., -1.]))

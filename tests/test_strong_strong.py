@@ -191,3 +191,9 @@ def test_6d_boundary_variant_against_python():
         _py_pair(c1, c2, S, N1 * len(c1) / len(p1), N2 * len(c2) / len(p2), g1, g2, k1, k2)
     assert np.allclose(o1, np.array(p1), rtol=1e-12, atol=1e-18)
     assert np.allclose(o2, np.array(p2), rtol=1e-12, atol=1e-18)
+
+
+def test_zboundary_c_abi_equals_the_oracle():
+    from simtrack_amd import strong_strong as SS
+    for n, sig in ((1, 0.3), (5, 0.05), (11, 0.08)):
+        assert np.array_equal(SS.Zboundary_slices_Gaussian(n, sig), pyoracle.zboundary_slices_gaussian(n, sig))
