@@ -60,10 +60,13 @@ def main():
                          "calibrating the HBM byte counters)")
     ap.add_argument("--cpu-sample", type=str, default=None,
                     help="particles x turns for the CPU leg (default 1024x64; config4: 2048x24)")
-    ap.add_argument("--workload", choices=("config1", "config2", "config3", "config4", "rhic"), default="config1",
+    ap.add_argument("--workload", choices=("config1", "config2", "config3", "config4", "rhic", "strong_strong"),
+                    default="config1",
                     help="config1: the headline workload; config2: 64 uneven lattices in one persistent launch; "
                          "config3: surrogate DA scan; config4: radiating electron ring, 1,048,576 particles; "
-                         "rhic: the reference's RHIC example lattice")
+                         "rhic: the reference's RHIC example lattice; strong_strong: soft-Gaussian 6-D "
+                         "collisions of two 1,048,576-macroparticle beams")
+    ap.add_argument("--slices", type=int, default=8, help="strong_strong: z slices per beam")
     ap.add_argument("--seeds", type=int, default=64, help="config2: lattices")
     ap.add_argument("--per-seed", type=int, default=4096, help="config2/3: particles per lattice")
     ap.add_argument("--train-seeds", type=int, default=16, help="config3: lattices for the training set")
@@ -88,6 +91,8 @@ def main():
         return bench_config4(args)
     if args.workload == "rhic":
         return bench_rhic(args)
+    if args.workload == "strong_strong":
+        return bench_strong_strong(args)
 
     from simtrack_amd import workloads as W
     gp, ln, x = W.config1(n_particles=args.particles)
@@ -196,6 +201,107 @@ def bench_tracking(args, gp, rec, x, spin, config, traffic, data):
                      "algorithmic_bytes": 2 * ncol * 8 * npart,
                      "f_turn_flops": f_turn, "kernel": kname,
                      "avg_launch_ms": 1e3 * k_sec / max(k_launches, 1), "launches": k_launches},
+        "cpu_baseline": baseline,
+    }
+    print(json.dumps(out))
+
+
+def _ss_beams(n, seed):
+    """Two 100 GeV proton beams at an IP with beta* = 0.5 m: sigma_x = 0.1 mm,
+    sigma_y = 0.06 mm (elliptic, so BB4D takes the complex-error-function
+    branch), sigma_z = 5 cm, sigma_delta = 1e-4, opposite 5 um offsets."""
+    import numpy as np
+    r = np.random.default_rng(seed)
+    beams = []
+    for k, off in enumerate((5e-6, -5e-6)):
+        x = np.empty((n, 6))
+        x[:, 0] = r.normal(off, 1e-4, n)
+        x[:, 1] = r.normal(0, 1e-4 / 0.5, n)
+        x[:, 2] = r.normal(0, 6e-5, n)
+        x[:, 3] = r.normal(0, 6e-5 / 0.5, n)
+        x[:, 4] = r.normal(0, 0.05, n)
+        x[:, 5] = r.normal(0, 1e-4, n)
+        beams.append(x)
+    return beams
+
+
+def bench_strong_strong(args):
+    """Strong-strong beam-beam (BeamBeam6D_SoftGaussian with Gaussian slice
+    boundaries, simtrack.cpp:23134): a step is one collision of the two
+    beams, every slice pair in order.  value = macroparticle kicks per
+    second (rows of both slices, summed over slice pairs)."""
+    import numpy as np
+    import torch
+
+    import simtrack_amd as st
+    from simtrack_amd import flops
+    from simtrack_amd import strong_strong as SS
+    from simtrack_amd import workloads as W
+
+    n = args.particles if args.particles != 65536 else 1048576
+    ns = args.slices
+    gp = W.proton_globals(100000.0)
+    b1, b2 = _ss_beams(n, 2031)
+    zb = SS.Zboundary_slices_Gaussian(ns, 0.05)
+    Np = 1.5e11
+    ctx = st.Context(0)
+    dev = torch.device("cuda:0")
+    d1 = torch.tensor(b1, device=dev)
+    d2 = torch.tensor(b2, device=dev)
+    torch.cuda.synchronize()
+
+    def collide():
+        SS.BeamBeam6D_SoftGaussian(ctx, Np, d1, Np, d2, gp.gamma, gp.gamma, 1., 1., ns, ns, zb, zb)
+    for _ in range(args.warmup):
+        collide()
+    ctx.set_timing(True)
+    ctx.kernel_stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        collide()
+    ctx.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    k_sec, k_pairs, k_rows = ctx.kernel_stats()
+    ctx.set_timing(False)
+
+    baseline = None
+    if not args.no_cpu_baseline:
+        import pyoracle
+        m = min(n, 131072)
+        c1, c2 = _ss_beams(m, 2031)
+        t = time.perf_counter()
+        pyoracle.bb_soft_gaussian_6d_zb(Np, c1, Np, c2, gp.gamma, gp.gamma, 1., 1., zb, zb)
+        dt = time.perf_counter() - t
+        inside = lambda b: int(((b[:, 4] < zb[0]) & (b[:, 4] >= zb[-1])).sum())
+        baseline = {"value": ns * (inside(c1) + inside(c2)) / dt, "unit": "macroparticle kicks/s", "cores": 1,
+                    "kind": "port", "sample": f"one collision of two {m}-macroparticle beams, {ns} x {ns} slices "
+                                              f"(oracle restatement, one thread), {dt:.1f} s"}
+
+    bytes_ = flops.SS_BYTES * k_rows
+    fl = (flops.SS_MOMENTS + flops.SS_KICK_ELLIPTIC) * k_rows
+    out = {
+        "metric": "strong-strong macroparticle kicks/s",
+        "value": k_rows / wall,
+        "unit": "macroparticle kicks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Gaussian beams)",
+        "config": {"workload": "strong_strong: two 100 GeV proton beams, soft-Gaussian 6-D collision with "
+                               f"{ns} x {ns} Gaussian z slices (BeamBeam6D_SoftGaussian, slice boundaries)",
+                   "particles_per_beam": n, "slices": ns, "charge_per_beam": Np},
+        "roofline": {"bound": "hbm", "achieved": bytes_ / k_sec / 1e9, "peak": 8000.0, "unit": "GB/s",
+                     "frac": bytes_ / k_sec / 8e12, "traffic": None, "algorithmic_bytes_per_row": flops.SS_BYTES,
+                     "fp64_frac": fl / k_sec / flops.PEAK_FP64_VECTOR, "kernel": "k_ss_partial + k_ss_kick",
+                     "avg_pair_ms": 1e3 * k_sec / max(k_pairs, 1), "pairs": k_pairs,
+                     "kernel_share_of_wall": k_sec / wall},
         "cpu_baseline": baseline,
     }
     print(json.dumps(out))

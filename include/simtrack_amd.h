@@ -234,10 +234,12 @@ int st_last_gpu_seconds(st_ctx *ctx, double *seconds);
 
 /*
  * Per-launch timing of the tracking kernel: when enabled, every tracking
- * launch on this context is bracketed by HIP events on the context stream.
- * st_ctx_kernel_stats returns (after st_ctx_sync) the summed kernel time,
- * the number of launches and the sum over launches of launched particles x
- * turns, then clears the counters.
+ * launch on this context is bracketed by HIP events on the context stream,
+ * and so is every slice pair of a strong-strong collision (its two
+ * launches).  st_ctx_kernel_stats returns (after st_ctx_sync) the summed
+ * kernel time, the number of launches and the sum over launches of the work
+ * (tracking: launched particles x turns; slice pairs: rows of both slices),
+ * then clears the counters.
  */
 int st_ctx_set_timing(st_ctx *ctx, int enable);
 int st_ctx_kernel_stats(st_ctx *ctx, double *kernel_seconds, int64_t *launches, int64_t *launched_particle_turns);

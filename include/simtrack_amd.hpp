@@ -13,6 +13,8 @@
  *   Track_DA_Uniform, Track_DA_Radial_Uniform
  *   Read_MADX_Lattice, Read_BMAD_Lattice               simtrack.cpp:13591, 14054
  *   Cal_Orbit_Num, Track_tbt_FMA, Track_tbt_Lyapunov   simtrack.cpp:14947, 21179, 21243
+ *   BeamBeam4D_SoftGaussian, BeamBeam6D_SoftGaussian   simtrack.cpp:22846, 22874, 23134
+ *   (both forms), Zboundary_slices_Gaussian             simtrack.cpp:3157
  *   Cal_OneTurnMap (matrix only), Track_tbt_ActionDiff simtrack.cpp:15122, 21309
  *
  * Element classes: DRIFT, MARKER, BPM, HBPM, VBPM, SBEND, GSBEND, QUAD,
@@ -1327,6 +1329,58 @@ inline void Track_tbt_ActionDiff(Line &l, double nsigma, double phase, double em
         std::fprintf(f, "%d   %d    %g  %g     %g   %g\n", (int)r[0], (int)r[1], r[2], r[3], r[4], r[5]);
     }
     std::fclose(f);
+}
+
+/* ------------------------------------------ strong-strong beam-beam */
+
+/* BeamBeam4D_SoftGaussian (simtrack.cpp:22846-22872): both beams (rows of
+   6) kicked in place on the GPU */
+inline void BeamBeam4D_SoftGaussian(double Np1, int Npart_1, double PartDist_1[], double Np2, int Npart_2,
+                                    double PartDist_2[], double gamma1, double gamma2, double bbscale1,
+                                    double bbscale2)
+{
+    check(st_bb_soft_gaussian_4d_host(default_context(), Np1, Npart_1, PartDist_1, Np2, Npart_2, PartDist_2, gamma1,
+                                      gamma2, bbscale1, bbscale2),
+          "BeamBeam4D_SoftGaussian");
+}
+
+/* BeamBeam6D_SoftGaussian (simtrack.cpp:22874-23132): beams sorted by z in
+   place and collided slice by slice; writes ./bunch1_Z.dat and
+   ./bunch2_Z.dat (slice number and centre) as the reference does */
+inline void BeamBeam6D_SoftGaussian(double Np1, int Mp1, double x1[], double Np2, int Mp2, double x2[],
+                                    double gamma1, double gamma2, double bbscale1, double bbscale2, int Nslice1,
+                                    int Nslice2)
+{
+    std::vector<double> Z1((size_t)(Nslice1 > 0 ? Nslice1 : 1)), Z2((size_t)(Nslice2 > 0 ? Nslice2 : 1));
+    check(st_bb_soft_gaussian_6d_host(default_context(), Np1, Mp1, x1, Np2, Mp2, x2, gamma1, gamma2, bbscale1,
+                                      bbscale2, Nslice1, Nslice2, Z1.data(), Z2.data()),
+          "BeamBeam6D_SoftGaussian");
+    const char *names[2] = {"./bunch1_Z.dat", "./bunch2_Z.dat"};
+    const std::vector<double> *Z[2] = {&Z1, &Z2};
+    const int ns[2] = {Nslice1, Nslice2};
+    for (int b = 0; b < 2; ++b) {
+        FILE *f = std::fopen(names[b], "w");
+        if (!f) throw Error(std::string("cannot open ") + names[b]);
+        for (int j = 0; j < ns[b]; ++j) std::fprintf(f, "%d  %g\n", j, (*Z[b])[(size_t)j]);
+        std::fclose(f);
+    }
+}
+
+/* BeamBeam6D_SoftGaussian with predefined slice boundaries
+   (simtrack.cpp:23134-23284): zb1/zb2 hold Nslice + 1 decreasing values */
+inline void BeamBeam6D_SoftGaussian(double Np1, int Mp1, double x1[], double Np2, int Mp2, double x2[],
+                                    double gamma1, double gamma2, double bbscale1, double bbscale2, int Nslice1,
+                                    int Nslice2, double zb1[], double zb2[])
+{
+    check(st_bb_soft_gaussian_6d_zb_host(default_context(), Np1, Mp1, x1, Np2, Mp2, x2, gamma1, gamma2, bbscale1,
+                                         bbscale2, Nslice1, Nslice2, zb1, zb2),
+          "BeamBeam6D_SoftGaussian");
+}
+
+/* Zboundary_slices_Gaussian (simtrack.cpp:3157-3168) */
+inline void Zboundary_slices_Gaussian(double zb[], int Nslice, double sigmal)
+{
+    check(st_zboundary_slices_gaussian(zb, Nslice, sigmal), "Zboundary_slices_Gaussian");
 }
 
 } // namespace simtrack_amd

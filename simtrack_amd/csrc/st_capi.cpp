@@ -2451,7 +2451,18 @@ static int ss_pair(st_ctx *c, ss_work &w, double *d_x1, const int *idx1, int sta
     a.bbscale[0] = bbscale1;
     a.bbscale[1] = bbscale2;
     a.part = w.part;
+    int ev = -1;
+    if (c->timing) {
+        int rc = timed_event(c, &ev);
+        if (rc) return rc;
+        HIPCHK(hipEventRecord(c->ev_pool[ev], c->stream));
+    }
     HIPCHK(st_ss_collide(a, c->stream));
+    if (c->timing) {
+        HIPCHK(hipEventRecord(c->ev_pool[ev + 1], c->stream));
+        c->pending.push_back({ev, ev + 1});
+        c->pending_work.push_back((int64_t)m1 + m2);
+    }
     return ST_OK;
 }
 
@@ -2514,7 +2525,6 @@ static int ss_6d_zb(st_ctx *c, ss_work &w, double Np1, int Mp1, double *d_x1, do
     std::vector<int> st[2], ct[2];
     for (int b = 0; b < 2; ++b) {
         HIPCHK(hipMemcpyAsync(w.zb[b], zb[b], (Ns[b] + 1) * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipMemsetAsync(w.s.counts, 0, (Ns[b] + 1) * sizeof(int), c->stream));
         HIPCHK(st_ss_group_by_slice(dx[b], Mp[b], w.zb[b], Ns[b], w.s, c->stream));
         if (Mp[b] > 0)
             HIPCHK(hipMemcpyAsync(w.rows[b], w.s.row_out, Mp[b] * sizeof(int), hipMemcpyDeviceToDevice, c->stream));

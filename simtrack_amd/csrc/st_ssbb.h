@@ -28,7 +28,7 @@ struct ss_sort_buffers {
     int *ikey_in, *ikey_out;
     int *row_in, *row_out;
     double *rows;        /* gathered rows (n x 6)                              */
-    int *counts;         /* rows per slice number (nslice + 1, zeroed by the caller) */
+    int *counts;         /* rows per slice number (nslice + 1)                  */
     void *temp;
     size_t temp_bytes;
 };

@@ -195,11 +195,25 @@ __global__ void k_ss_slice_id(const double *x, int n, const double *zb, int nsli
     row[i] = i;
 }
 
-/* rows per slice number (integer counts: the order of the additions does not matter) */
-__global__ void k_ss_count(const int *key, int n, int *count)
+/* rows per slice number from the sorted slice numbers: the first row of
+   slice s is the lower bound of s (binary search, one thread per slice) */
+__global__ void k_ss_count(const int *key, int n, int nslice, int *count)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(&count[key[i]], 1);
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > nslice) return;
+    int lo[2];
+    for (int q = 0; q < 2; q++) {
+        int a = 0, b = n, v = s + q;
+        while (a < b) {
+            const int m = a + (b - a) / 2;
+            if (key[m] < v)
+                a = m + 1;
+            else
+                b = m;
+        }
+        lo[q] = a;
+    }
+    count[s] = lo[1] - lo[0];
 }
 
 __global__ void k_ss_gather(const double *x, const int *row, int n, double *out)
@@ -270,7 +284,7 @@ hipError_t st_ss_group_by_slice(const double *x, int n, const double *d_zb, int 
     e = hipcub::DeviceRadixSort::SortPairs(s.temp, need, s.ikey_in, s.ikey_out, s.row_in, s.row_out, n, 0, 32,
                                            stream);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_ss_count, dim3(g), dim3(256), 0, stream, s.ikey_out, n, s.counts);
+    hipLaunchKernelGGL(k_ss_count, dim3((nslice + 64) / 64), dim3(64), 0, stream, s.ikey_out, n, nslice, s.counts);
     return hipGetLastError();
 }
 

@@ -30,6 +30,13 @@ RADIATE = 44 + PT_UPDATE
 RADIATE1 = 60 + PT_UPDATE
 BENDKICK = 7
 GBENDKICK = 9
+BB4D_ROUND = 40       # BB4D round branch, cpp:6336-6342
+BB4D_ELLIPTIC = 40 + 2 * 150   # two cerrf calls, cpp:6343-6380
+# strong-strong slice pair, per row (cpp:23035-23110): moments of the drifted
+# row (drift 4, sums 6), then drift 4, BB4D, kick 4, drift back 4
+SS_MOMENTS = 10
+SS_KICK_ELLIPTIC = 12 + BB4D_ELLIPTIC
+SS_BYTES = 96         # x, px, y, py read by both passes and written by the kick
 
 
 def mult_kick(norder, mask, bend=False):

@@ -131,3 +131,41 @@ def test_bad_boundaries_are_rejected(ctx):
     with pytest.raises(st.STError, match="decrease"):
         SS.BeamBeam6D_SoftGaussian(ctx, 1e11, b1, 1e11, b2, 200., 200., 1., 1., 2, 2, np.array([1., 2., -1.]),
                                    np.array([1., 0., -1.]))
+
+
+def test_cpp_api_equals_python(ctx, tmp_path):
+    """tests/cpp/strong_strong.cpp (BeamBeam4D_SoftGaussian and both forms of
+    BeamBeam6D_SoftGaussian through include/simtrack_amd.hpp) gives the
+    Python API's bits, and writes the reference's bunch1_Z.dat/bunch2_Z.dat."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "build", "strong_strong")
+    if not os.path.exists(exe):
+        pytest.fail("tests/cpp/build/strong_strong is not built (run __graft_entry__.build())")
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, timeout=300)
+    assert r.returncode == 0, r.stderr.decode()
+    n1, n2 = 3001, 2500
+    v = np.frombuffer(r.stdout, dtype=np.float64)
+    assert v.size == 3 * 2 * 6 * (n1 + n2)
+    at = 0
+    Np1, Np2, g1, g2, k1, k2 = 1.2e11, 0.9e11, 230., 240., 0.8, 1.1
+
+    def take(n):
+        nonlocal at
+        a = v[at:at + 6 * n].reshape(n, 6).copy()
+        at += 6 * n
+        return a
+    for form in range(3):
+        a, b = take(n1), take(n2)
+        ra, rb = take(n1), take(n2)
+        if form == 0:
+            SS.BeamBeam4D_SoftGaussian(ctx, Np1, a, Np2, b, g1, g2, k1, k2)
+        elif form == 1:
+            z1, z2 = SS.BeamBeam6D_SoftGaussian(ctx, Np1, a, Np2, b, g1, g2, k1, k2, 5, 4)
+            lines = (tmp_path / "bunch1_Z.dat").read_text().splitlines()
+            assert lines == [f"{j}  {z:g}" for j, z in enumerate(z1)]
+            assert len((tmp_path / "bunch2_Z.dat").read_text().splitlines()) == 4
+        else:
+            zb = SS.Zboundary_slices_Gaussian(4, 0.04)
+            SS.BeamBeam6D_SoftGaussian(ctx, Np1, a, Np2, b, g1, g2, k1, k2, 4, 4, zb, zb)
+        assert np.array_equal(a, ra) and np.array_equal(b, rb), f"form {form}"
