@@ -41,6 +41,9 @@ def load(glibc=False):
     lib.orc_bb_soft_gaussian_6d.argtypes = [D, I, P, D, I, P, D, D, D, D, I, I, P, P]
     lib.orc_bb_soft_gaussian_6d_zb.argtypes = [D, I, P, D, I, P, D, D, D, D, I, I, P, P]
     lib.orc_zboundary_slices_gaussian.argtypes = [P, I, D]
+    lib.orc_bb_pic_4d.argtypes = [D, I, P, D, I, P, D, D, D, D]
+    lib.orc_bb_pic_6d.argtypes = [D, I, P, D, I, P, D, D, D, D, I, I]
+    lib.orc_fft2d_128.argtypes = [P, P, I]
     lib.orc_bb_slice_centers.argtypes = [I, D, P]
     lib.orc_fine_tune.argtypes = [I, P, D, D, P]
     lib.orc_closed_orbit.argtypes = [P, I, D, P, P, ctypes.POINTER(I)]
@@ -49,7 +52,7 @@ def load(glibc=False):
     lib.orc_lyapunov_scan.argtypes = [P, I, D, D, D, I, D, D, P, I]
     lib.orc_action_diff.argtypes = [P, I, D, D, D, D, D, D, D, D, I, I, I, P, P]
     lib.orc_fine_tune_xpx.argtypes = [I, P, P, D, D, P]
-    for f in ("orc_sin", "orc_cos", "orc_tan", "orc_asin", "orc_exp", "orc_log", "orc_ugaussian_pinv"):
+    for f in ("orc_sin", "orc_cos", "orc_tan", "orc_asin", "orc_exp", "orc_log", "orc_atan", "orc_ugaussian_pinv"):
         getattr(lib, f).argtypes = [D]
         getattr(lib, f).restype = D
     lib.orc_powi.argtypes = [D, I]
@@ -213,3 +216,26 @@ def bb4d(x, y, gamma, N, sigmax, sigmay):
     a, b = ctypes.c_double(), ctypes.c_double()
     load().orc_bb4d(x, y, gamma, N, sigmax, sigmay, ctypes.byref(a), ctypes.byref(b))
     return a.value, b.value
+
+
+def bb_pic_4d(Np1, x1, Np2, x2, gamma1, gamma2, bbscale1, bbscale2):
+    """BeamBeam4D_PIC (simtrack.cpp:23286): new (x1, x2)."""
+    a, b = _beam(x1), _beam(x2)
+    load().orc_bb_pic_4d(Np1, len(a), _p(a), Np2, len(b), _p(b), gamma1, gamma2, bbscale1, bbscale2)
+    return a, b
+
+
+def bb_pic_6d(Np1, x1, Np2, x2, gamma1, gamma2, bbscale1, bbscale2, nslice1, nslice2):
+    """BeamBeam6D_PIC (simtrack.cpp:23380): new (x1, x2), sorted by z descending."""
+    a, b = _beam(x1), _beam(x2)
+    load().orc_bb_pic_6d(Np1, len(a), _p(a), Np2, len(b), _p(b), gamma1, gamma2, bbscale1, bbscale2, nslice1,
+                         nslice2)
+    return a, b
+
+
+def fft2d_128(re, im, direction):
+    """FFT2D (simtrack.cpp:2649) of a 128 x 128 complex grid: new (re, im)."""
+    x = np.ascontiguousarray(np.array(re, dtype=np.float64, copy=True).reshape(128, 128))
+    y = np.ascontiguousarray(np.array(im, dtype=np.float64, copy=True).reshape(128, 128))
+    load().orc_fft2d_128(_p(x), _p(y), int(direction))
+    return x, y

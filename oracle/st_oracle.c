@@ -2158,6 +2158,7 @@ EXPORT void orc_sincos(double x, double *s, double *c) { st_sincos(x, s, c); }
 #endif
 EXPORT double orc_powi(double x, int n) { return OPOW(x, n); }
 EXPORT double orc_log(double x) { return st_log(x); }
+EXPORT double orc_atan(double x) { return st_atan(x); }
 EXPORT double orc_rng_normal(uint64_t seed, uint32_t particle, uint64_t counter)
 {
     return st_rng_normal(seed, particle, counter);
@@ -2837,4 +2838,360 @@ EXPORT void orc_zboundary_slices_gaussian(double *zb, int Nslice, double sigmal)
     zb[0] = sigmal * 20;
     zb[Nslice] = -sigmal * 20;
     for (int i = 1; i <= Nslice - 1; i++) zb[i] = -st_ugaussian_pinv(i * 1.0 / Nslice) * sigmal;
+}
+
+/* ------------------------------------------- strong-strong PIC (4-D/6-D) */
+
+#define PIC_NX 64
+#define PIC_NY 64
+#define PIC_NSIGMA 15
+
+/* Cal_Grid_Weight, simtrack.cpp:2755-2791: the 3-point (TSC) weights */
+static void pic_grid_weight(double x1, double y1, double x0, double y0, double hx, double hy, int xm[3], int ym[3],
+                            double xw[3], double yw[3])
+{
+    double dx = x1 - x0, dy = y1 - y0, fx, fy, rx, rx2, ry, ry2;
+    int ifx, ify;
+    fx = dx / hx;
+    ifx = (int)fx;
+    fy = dy / hy;
+    ify = (int)fy;
+    if ((fx - ifx) <= 0.5) {
+        xm[0] = ifx;
+        xm[1] = ifx + 1;
+        xm[2] = ifx - 1;
+        rx = fx - ifx;
+        rx2 = rx * rx;
+    } else {
+        xm[0] = ifx + 1;
+        xm[1] = ifx;
+        xm[2] = ifx + 2;
+        rx = ifx + 1 - fx;
+        rx2 = rx * rx;
+    }
+    xw[0] = 0.75 - rx2;
+    xw[1] = 0.5 * (0.25 + rx + rx2);
+    xw[2] = 0.5 * (0.25 - rx + rx2);
+    if ((fy - ify) <= 0.5) {
+        ym[0] = ify;
+        ym[1] = ify + 1;
+        ym[2] = ify - 1;
+        ry = fy - ify;
+        ry2 = ry * ry;
+    } else {
+        ym[0] = ify + 1;
+        ym[1] = ify;
+        ym[2] = ify + 2;
+        ry = ify + 1 - fy;
+        ry2 = ry * ry;
+    }
+    yw[0] = 0.75 - ry2;
+    yw[1] = 0.5 * (0.25 + ry + ry2);
+    yw[2] = 0.5 * (0.25 - ry + ry2);
+}
+
+/* int_green and calc_egreen, simtrack.cpp:2808-2879 */
+static double pic_int_green(double x, double y)
+{
+    if (0 == x || 0 == y) return 0;
+    double x2 = x * x, y2 = y * y;
+    return -3 * x * y + x2 * st_atan(y / x) + y2 * st_atan(x / y) + x * y * st_log(x2 + y2);
+}
+
+static double pic_egreen(double x1, double x2, double y1, double y2, double hx, double hy)
+{
+    if (x1 * x2 * y1 * y2 != 0.) {
+        int s11 = 1, s12 = 1, s21 = 1, s22 = 1;
+        if (x1 > 0) {
+            s11 *= -1;
+            s12 *= -1;
+        } else if (x1 < 0 && x2 < 0) {
+            s21 *= -1;
+            s22 *= -1;
+        }
+        if (y1 > 0) {
+            s11 *= -1;
+            s21 *= -1;
+        } else if (y1 < 0 && y2 < 0) {
+            s12 *= -1;
+            s22 *= -1;
+        }
+        x1 = fabs(x1);
+        x2 = fabs(x2);
+        y1 = fabs(y1);
+        y2 = fabs(y2);
+        double v11 = pic_int_green(x1, y1), v12 = pic_int_green(x1, y2);
+        double v21 = pic_int_green(x2, y1), v22 = pic_int_green(x2, y2);
+        double r = s11 * v11 + s12 * v12 + s21 * v21 + s22 * v22;
+        return -1.0 * r / hx / hy / 2;
+    }
+    double r = 0.;
+    for (int i = 0; i < 20; i++)
+        for (int j = 0; j < 20; j++) {
+            double tx = x1 + (i - 1) * hx / 20 + hx / 40;
+            double ty = y1 + (j - 1) * hy / 20 + hy / 40;
+            if (tx != 0. && ty != 0.) r = r - 0.5 * st_log(tx * tx + ty * ty);
+        }
+    return r / 400.;
+}
+
+/* Cal_Effective_Green_Function, simtrack.cpp:2881-2921 (before its FFT) */
+static void pic_green(double xoff, double yoff, double hx, double hy, double *g)
+{
+    const int Nx = PIC_NX, Ny = PIC_NY;
+    for (int i = 0; i < 2 * Nx; i++)
+        for (int j = 0; j < 2 * Ny; j++) {
+            double x1 = (i <= Nx - 1) ? xoff + i * hx - 0.5 * hx : xoff - (2 * Nx - i) * hx - 0.5 * hx;
+            double y1 = (j <= Ny - 1) ? yoff + j * hy - 0.5 * hy : yoff - (2 * Ny - j) * hy - 0.5 * hy;
+            g[i * 2 * Nx + j] = pic_egreen(x1, x1 + hx, y1, y1 + hy, hx, hy);
+        }
+}
+
+/* FFT1D / FFT2D, simtrack.cpp:2583-2698: radix 2, twiddles by recurrence,
+   dir == 1 divides by the length */
+static void pic_fft1d(double *x, double *y, int m, int dir)
+{
+    long nn = 1, i, i1, j, k, i2, l, l1, l2;
+    double c1, c2, tx, ty, t1, t2, u1, u2, z;
+    for (i = 0; i < m; i++) nn *= 2;
+    i2 = nn >> 1;
+    j = 0;
+    for (i = 0; i < nn - 1; i++) {
+        if (i < j) {
+            tx = x[i];
+            ty = y[i];
+            x[i] = x[j];
+            y[i] = y[j];
+            x[j] = tx;
+            y[j] = ty;
+        }
+        k = i2;
+        while (k <= j) {
+            j -= k;
+            k >>= 1;
+        }
+        j += k;
+    }
+    c1 = -1.0;
+    c2 = 0.0;
+    l2 = 1;
+    for (l = 0; l < m; l++) {
+        l1 = l2;
+        l2 <<= 1;
+        u1 = 1.0;
+        u2 = 0.0;
+        for (j = 0; j < l1; j++) {
+            for (i = j; i < nn; i += l2) {
+                i1 = i + l1;
+                t1 = u1 * x[i1] - u2 * y[i1];
+                t2 = u1 * y[i1] + u2 * x[i1];
+                x[i1] = x[i] - t1;
+                y[i1] = y[i] - t2;
+                x[i] += t1;
+                y[i] += t2;
+            }
+            z = u1 * c1 - u2 * c2;
+            u2 = u1 * c2 + u2 * c1;
+            u1 = z;
+        }
+        c2 = sqrt((1.0 - c1) / 2.0);
+        if (dir == 1) c2 = -c2;
+        c1 = sqrt((1.0 + c1) / 2.0);
+    }
+    if (dir == 1)
+        for (i = 0; i < nn; i++) {
+            x[i] /= (double)nn;
+            y[i] /= (double)nn;
+        }
+}
+
+/* FFT2D for the square 2Nx x 2Ny grid (n = 128, m = 7) */
+static void pic_fft2d(double *x, double *y, int n, int m, int dir)
+{
+    double re[128], im[128];
+    for (int j = 0; j < n; j++) {
+        for (int i = 0; i < n; i++) {
+            re[i] = x[i * n + j];
+            im[i] = y[i * n + j];
+        }
+        pic_fft1d(re, im, m, dir);
+        for (int i = 0; i < n; i++) {
+            x[i * n + j] = re[i];
+            y[i * n + j] = im[i];
+        }
+    }
+    for (int i = 0; i < n; i++) {
+        for (int j = 0; j < n; j++) {
+            re[j] = x[i * n + j];
+            im[j] = y[i * n + j];
+        }
+        pic_fft1d(re, im, m, dir);
+        for (int j = 0; j < n; j++) {
+            x[i * n + j] = re[j];
+            y[i * n + j] = im[j];
+        }
+    }
+}
+
+EXPORT void orc_fft2d_128(double *x, double *y, int dir) { pic_fft2d(x, y, 128, 7, dir); }
+
+/* the field of one beam's charge (Charge2Grid, Cal_Electric_PIC with
+   Cal_Potential_Convolution and Cal_Electric_Field, cpp:2793-3045) on the
+   grid at (x0, y0), as seen at the other beam's offset */
+static void pic_field(int m, const double *xc, const double mean[2], double x0, double y0, double xoff, double yoff,
+                      double hx, double hy, double *Ex, double *Ey)
+{
+    const int Nx = PIC_NX, Ny = PIC_NY, N2 = 4 * PIC_NX * PIC_NY;
+    const double eps0 = 8.854187817e-12;
+    double *src = (double *)calloc((size_t)Nx * Ny, sizeof(double));
+    double *gr = (double *)calloc((size_t)N2, sizeof(double)), *gi = (double *)calloc((size_t)N2, sizeof(double));
+    double *sr = (double *)calloc((size_t)N2, sizeof(double)), *si = (double *)calloc((size_t)N2, sizeof(double));
+    double *pr = (double *)calloc((size_t)N2, sizeof(double)), *pi = (double *)calloc((size_t)N2, sizeof(double));
+    double *phi = (double *)calloc((size_t)Nx * Ny, sizeof(double));
+    int xm[3], ym[3];
+    double xw[3], yw[3];
+    for (int k = 0; k < m; k++) {
+        double xp = xc[6 * k], yp = xc[6 * k + 2];
+        if (fabs(xp - mean[0]) < (Nx / 2 - 3) * hx && fabs(yp - mean[1]) < (Ny / 2 - 3) * hy) {
+            pic_grid_weight(xp, yp, x0, y0, hx, hy, xm, ym, xw, yw);
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) src[xm[i] * Nx + ym[j]] = src[xm[i] * Nx + ym[j]] + xw[i] * yw[j];
+        }
+    }
+    pic_green(xoff, yoff, hx, hy, gr);
+    pic_fft2d(gr, gi, 2 * Nx, 7, 1);
+    for (int i = 0; i < Nx; i++)
+        for (int j = 0; j < Ny; j++) sr[i * 2 * Nx + j] = src[i * Nx + j];
+    pic_fft2d(sr, si, 2 * Nx, 7, 1);
+    for (int i = 0; i < N2; i++) {
+        pr[i] = sr[i] * gr[i] - si[i] * gi[i];
+        pi[i] = sr[i] * gi[i] + si[i] * gr[i];
+    }
+    pic_fft2d(pr, pi, 2 * Nx, 7, -1);
+    for (int i = 0; i < Nx; i++)
+        for (int j = 0; j < Ny; j++) phi[i * Nx + j] = pr[i * 2 * Nx + j] * 2 * Nx * 2 * Ny / 2 / 3.14159265 / eps0;
+    for (int i = 0; i < Nx * Ny; i++) Ex[i] = Ey[i] = 0.;
+    for (int i = 1; i < Nx - 1; i++)
+        for (int j = 1; j < Ny - 1; j++) {
+            Ex[i * Nx + j] = -((phi[(i + 1) * Nx + j + 1] - phi[(i - 1) * Nx + j + 1]) +
+                               4 * (phi[(i + 1) * Nx + j] - phi[(i - 1) * Nx + j]) +
+                               (phi[(i + 1) * Nx + j - 1] - phi[(i - 1) * Nx + j - 1])) /
+                             12. / hx;
+            Ey[i * Nx + j] = -((phi[(i + 1) * Nx + j + 1] - phi[(i + 1) * Nx + j - 1]) +
+                               4 * (phi[i * Nx + j + 1] - phi[i * Nx + j - 1]) +
+                               (phi[(i - 1) * Nx + j + 1] - phi[(i - 1) * Nx + j - 1])) /
+                             12. / hy;
+        }
+    free(src);
+    free(gr);
+    free(gi);
+    free(sr);
+    free(si);
+    free(pr);
+    free(pi);
+    free(phi);
+}
+
+/* Interpolate_Electric_Field (cpp:3021-3036) and the kick of cpp:23337-23340 */
+static void pic_kick(int m, double *xc, const double mean[2], double x0, double y0, double xoff, double yoff,
+                     double hx, double hy, const double *Ex, const double *Ey, double Nsrc, int Msrc, double gamma,
+                     double bbscale)
+{
+    const int Nx = PIC_NX, Ny = PIC_NY;
+    int xm[3], ym[3];
+    double xw[3], yw[3];
+    for (int k = 0; k < m; k++) {
+        double xp = xc[6 * k], yp = xc[6 * k + 2];
+        if (fabs(xp - mean[0]) < (Nx / 2 - 3) * hx && fabs(yp - mean[1]) < (Ny / 2 - 3) * hy) {
+            pic_grid_weight(xp - xoff, yp - yoff, x0, y0, hx, hy, xm, ym, xw, yw);
+            double ex = 0., ey = 0.;
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    ex = ex + Ex[xm[i] * Nx + ym[j]] * xw[i] * yw[j];
+                    ey = ey + Ey[xm[i] * Nx + ym[j]] * xw[i] * yw[j];
+                }
+            xc[6 * k + 1] = xc[6 * k + 1] + ex * 1.60217662e-19 * bbscale / (gamma * 938.27201e6) * (Nsrc / Msrc);
+            xc[6 * k + 3] = xc[6 * k + 3] + ey * 1.60217662e-19 * bbscale / (gamma * 938.27201e6) * (Nsrc / Msrc);
+        }
+    }
+}
+
+/* BeamBeam4D_PIC, simtrack.cpp:23286-23378: a 64 x 64 grid of nsigma = 15
+   times the smaller rms size, centred on the source beam; beam 2 is kicked
+   by beam 1's field, then beam 1 by beam 2's */
+EXPORT void orc_bb_pic_4d(double Np1, int Mp1, double *x1, double Np2, int Mp2, double *x2, double gamma1,
+                          double gamma2, double bbscale1, double bbscale2)
+{
+    const int Nx = PIC_NX, Ny = PIC_NY;
+    double m1[2], r1[2], m2[2], r2[2];
+    double *Ex = (double *)malloc(sizeof(double) * Nx * Ny), *Ey = (double *)malloc(sizeof(double) * Nx * Ny);
+    mean_rms_xy(Mp1, x1, m1, r1);
+    mean_rms_xy(Mp2, x2, m2, r2);
+    double rmsx = r1[0] > r2[0] ? r2[0] : r1[0];
+    double rmsy = r1[1] > r2[1] ? r2[1] : r1[1];
+    double hx = PIC_NSIGMA * rmsx / Nx, hy = PIC_NSIGMA * rmsy / Ny;
+    double x0 = m1[0] - Nx * hx / 2, y0 = m1[1] - Ny * hy / 2;
+    double xoff = m2[0] - m1[0], yoff = m2[1] - m1[1];
+    pic_field(Mp1, x1, m1, x0, y0, xoff, yoff, hx, hy, Ex, Ey);
+    pic_kick(Mp2, x2, m2, x0, y0, xoff, yoff, hx, hy, Ex, Ey, Np1, Mp1, gamma2, bbscale2);
+    x0 = m2[0] - Nx * hx / 2;
+    y0 = m2[1] - Ny * hy / 2;
+    xoff = m1[0] - m2[0];
+    yoff = m1[1] - m2[1];
+    pic_field(Mp2, x2, m2, x0, y0, xoff, yoff, hx, hy, Ex, Ey);
+    pic_kick(Mp1, x1, m1, x0, y0, xoff, yoff, hx, hy, Ex, Ey, Np2, Mp2, gamma1, bbscale1);
+    free(Ex);
+    free(Ey);
+}
+
+/* BeamBeam6D_PIC, simtrack.cpp:23380-23615: the z-sorted slicing and the
+   collision order of BeamBeam6D_SoftGaussian, BeamBeam4D_PIC per pair on
+   the drifted slices with charges Np/Nslice */
+EXPORT void orc_bb_pic_6d(double Np1, int Mp1, double *x1, double Np2, int Mp2, double *x2, double gamma1,
+                          double gamma2, double bbscale1, double bbscale2, int Nslice1, int Nslice2)
+{
+    int n = Nslice1 * Nslice2;
+    int mc1 = Mp1 / Nslice1, mc2 = Mp2 / Nslice2;
+    double *Z1 = (double *)malloc(sizeof(double) * (size_t)Nslice1);
+    double *Z2 = (double *)malloc(sizeof(double) * (size_t)Nslice2);
+    int *I1 = (int *)malloc(sizeof(int) * (size_t)n), *I2 = (int *)malloc(sizeof(int) * (size_t)n);
+    double *S = (double *)malloc(sizeof(double) * (size_t)n);
+    sort_z_desc(Mp1, x1);
+    sort_z_desc(Mp2, x2);
+    for (int i = 0; i < Nslice1; i++) {
+        double t = 0;
+        for (int j = i * mc1; j < (i + 1) * mc1; j++) t = t + x1[6 * j + 4];
+        Z1[i] = t / mc1;
+    }
+    for (int i = 0; i < Nslice2; i++) {
+        double t = 0;
+        for (int j = i * mc2; j < (i + 1) * mc2; j++) t = t + x2[6 * j + 4];
+        Z2[i] = t / mc2;
+    }
+    ss_schedule(Nslice1, Z1, Nslice2, Z2, I1, I2, S);
+    for (int p = 0; p < n; p++) {
+        double *c1 = x1 + (size_t)I1[p] * mc1 * 6, *c2 = x2 + (size_t)I2[p] * mc2 * 6;
+        for (int j = 0; j < mc1; j++) {
+            c1[j * 6 + 0] = c1[j * 6 + 0] + c1[j * 6 + 1] * S[p];
+            c1[j * 6 + 2] = c1[j * 6 + 2] + c1[j * 6 + 3] * S[p];
+        }
+        for (int j = 0; j < mc2; j++) {
+            c2[j * 6 + 0] = c2[j * 6 + 0] - c2[j * 6 + 1] * S[p];
+            c2[j * 6 + 2] = c2[j * 6 + 2] - c2[j * 6 + 3] * S[p];
+        }
+        orc_bb_pic_4d(Np1 / Nslice1, mc1, c1, Np2 / Nslice2, mc2, c2, gamma1, gamma2, bbscale1, bbscale2);
+        for (int j = 0; j < mc1; j++) {
+            c1[j * 6 + 0] = c1[j * 6 + 0] - c1[j * 6 + 1] * S[p];
+            c1[j * 6 + 2] = c1[j * 6 + 2] - c1[j * 6 + 3] * S[p];
+        }
+        for (int j = 0; j < mc2; j++) {
+            c2[j * 6 + 0] = c2[j * 6 + 0] + c2[j * 6 + 1] * S[p];
+            c2[j * 6 + 2] = c2[j * 6 + 2] + c2[j * 6 + 3] * S[p];
+        }
+    }
+    free(Z1);
+    free(Z2);
+    free(I1);
+    free(I2);
+    free(S);
 }

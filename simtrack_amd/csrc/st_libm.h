@@ -484,6 +484,76 @@ ST_FN double st_log(double x)
     return a.hi + ((a.lo + p.lo) - t);
 }
 
+/* --------------------------------------------------------------- atan */
+
+/*
+ * atan by the argument reduction of fdlibm's s_atan.c (a published
+ * method): |x| in [0, 7/16) directly; [7/16, 11/16) about atan(1/2) with
+ * t = (2x - 1)/(2 + x); [11/16, 19/16) about pi/4 with t = (x - 1)/(x + 1);
+ * [19/16, 39/16) about atan(3/2) with t = (x - 1.5)/(1 + 1.5x); beyond, about
+ * pi/2 with t = -1/x.  atan(t) for |t| < 7/16 is this project's odd Taylor
+ * series t + t z P(z), z = t^2, P = sum_{k=1..22} (-1)^k z^(k-1) / (2k+1)
+ * (truncation below 4e-18 relative), the anchors as hi + lo pairs.
+ */
+ST_FN double st_atan(double x)
+{
+    const double a = st_fabs(x);
+    if (!(a <= 0x1.fffffffffffffp+1023)) {
+        if (a != a) return x + x;                         /* nan */
+        return x > 0 ? 0x1.921fb54442d18p+0 : -0x1.921fb54442d18p+0;
+    }
+    if (a < 0x1p-27) return x;
+    if (a >= 0x1p66) return x > 0 ? 0x1.921fb54442d18p+0 : -0x1.921fb54442d18p+0;
+    double hi = 0., lo = 0., t;
+    int anchored = 1;
+    if (a < 0.4375) {
+        t = a;
+        anchored = 0;
+    } else if (a < 0.6875) {
+        t = (2.0 * a - 1.0) / (2.0 + a);
+        hi = 0x1.dac670561bb4fp-2;
+        lo = 0x1.a2b7f222f65e2p-56;
+    } else if (a < 1.1875) {
+        t = (a - 1.0) / (a + 1.0);
+        hi = 0x1.921fb54442d18p-1;
+        lo = 0x1.1a62633145c07p-55;
+    } else if (a < 2.4375) {
+        t = (a - 1.5) / (1.0 + 1.5 * a);
+        hi = 0x1.f730bd281f69bp-1;
+        lo = 0x1.007887af0cbbdp-56;
+    } else {
+        t = -1.0 / a;
+        hi = 0x1.921fb54442d18p+0;
+        lo = 0x1.1a62633145c07p-54;
+    }
+    const double z = t * t;
+    double P = 0x1.6c16c16c16c17p-6;
+    P = ST_FMA(z, P, -0x1.7d05f417d05f4p-6);
+    P = ST_FMA(z, P, 0x1.8f9c18f9c18fap-6);
+    P = ST_FMA(z, P, -0x1.a41a41a41a41ap-6);
+    P = ST_FMA(z, P, 0x1.bacf914c1bad0p-6);
+    P = ST_FMA(z, P, -0x1.d41d41d41d41dp-6);
+    P = ST_FMA(z, P, 0x1.f07c1f07c1f08p-6);
+    P = ST_FMA(z, P, -0x1.0842108421084p-5);
+    P = ST_FMA(z, P, 0x1.1a7b9611a7b96p-5);
+    P = ST_FMA(z, P, -0x1.2f684bda12f68p-5);
+    P = ST_FMA(z, P, 0x1.47ae147ae147bp-5);
+    P = ST_FMA(z, P, -0x1.642c8590b2164p-5);
+    P = ST_FMA(z, P, 0x1.8618618618618p-5);
+    P = ST_FMA(z, P, -0x1.af286bca1af28p-5);
+    P = ST_FMA(z, P, 0x1.e1e1e1e1e1e1ep-5);
+    P = ST_FMA(z, P, -0x1.1111111111111p-4);
+    P = ST_FMA(z, P, 0x1.3b13b13b13b14p-4);
+    P = ST_FMA(z, P, -0x1.745d1745d1746p-4);
+    P = ST_FMA(z, P, 0x1.c71c71c71c71cp-4);
+    P = ST_FMA(z, P, -0x1.2492492492492p-3);
+    P = ST_FMA(z, P, 0x1.999999999999ap-3);
+    P = ST_FMA(z, P, -0x1.5555555555555p-2);
+    const double r = t * z * P;
+    const double res = anchored ? hi + (t + (r + lo)) : t + r;
+    return x < 0 ? -res : res;
+}
+
 /* ---------------------------------------------------------------- tan */
 
 /* tan = sin/cos of one reduction (element edge angles, host-side constants) */

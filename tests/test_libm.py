@@ -45,6 +45,11 @@ CASES = {
     "orc_log": (mpmath.log, np.concatenate([10 ** RNG.uniform(-300, 300, 600), RNG.uniform(0.5, 2, 400),
                                             [5e-324, 1e-310, 1.0, 2.0]])),
     "orc_tan": (mpmath.tan, np.concatenate([RNG.uniform(-1.5, 1.5, 500), [1e-9, 0.0]])),
+    "orc_atan": (mpmath.atan, np.concatenate([RNG.uniform(-3, 3, 800), 10 ** RNG.uniform(-30, 30, 300) *
+                                              RNG.choice([-1, 1], 300),
+                                              [0.4375, np.nextafter(0.4375, 0), 0.6875, np.nextafter(0.6875, 0),
+                                               1.1875, np.nextafter(1.1875, 0), 2.4375, np.nextafter(2.4375, 0),
+                                               1.0, 0.5, 1.5, 2.0 ** 66, 2.0 ** -27, 1e-300]])),
 }
 BOUND = {"orc_tan": 2.0}   # tan = sin/cos: two roundings plus the quotient
 

@@ -197,3 +197,64 @@ def test_zboundary_c_abi_equals_the_oracle():
     from simtrack_amd import strong_strong as SS
     for n, sig in ((1, 0.3), (5, 0.05), (11, 0.08)):
         assert np.array_equal(SS.Zboundary_slices_Gaussian(n, sig), pyoracle.zboundary_slices_gaussian(n, sig))
+
+
+# ------------------------------------------------------------------ PIC
+
+def test_pic_fft_is_the_dft():
+    """FFT2D (simtrack.cpp:2649) on 128 x 128: dir = 1 is the DFT divided by
+    the point count, dir = -1 the unnormalised inverse."""
+    r = np.random.Generator(np.random.PCG64(1))
+    re, im = r.normal(size=(128, 128)), r.normal(size=(128, 128))
+    a, b = pyoracle.fft2d_128(re, im, 1)
+    f = np.fft.fft2(re + 1j * im) / 128 / 128
+    assert np.abs(a + 1j * b - f).max() < 1e-13 * np.abs(f).max()
+    a, b = pyoracle.fft2d_128(re, im, -1)
+    f = np.fft.ifft2(re + 1j * im) * 128 * 128
+    assert np.abs(a + 1j * b - f).max() < 1e-13 * np.abs(f).max()
+
+
+def test_pic_kick_is_the_gaussian_field():
+    """BeamBeam4D_PIC's kick on a probe beam from a Gaussian source beam is
+    the Gaussian field (BB4D) of the source smoothed by the grid (h = 15 rms
+    / 64): the 3-point weights that deposit and interpolate add h^2/4 of
+    variance each, the field stencil's central difference along and 1-4-1
+    average across add h^2/3, 5 h^2 / 6 in all.  This pins the Green
+    function, the convolution, the field stencil and the kick's constants
+    (without the smoothing the kicks are 3 % weaker than the Gaussian's)."""
+    src = _beam(200000, 3, sx=1e-4, sy=6e-5)
+    probe = _beam(2000, 4, sx=1e-4, sy=6e-5)
+    N, g = 1e11, 200.
+    _, p2 = pyoracle.bb_pic_4d(N, src, N, probe, g, g, 1., 1.)
+    mx, my = src[:, 0].mean(), src[:, 2].mean()
+    sx, sy = src[:, 0].std(), src[:, 2].std()
+    rx, ry = min(sx, probe[:, 0].std()), min(sy, probe[:, 2].std())
+    hx, hy = 15 * rx / 64, 15 * ry / 64
+    sm = 5. / 6.
+    ex = np.array([pyoracle.bb4d(x - mx, y - my, g, N, np.sqrt(sx ** 2 + sm * hx ** 2), np.sqrt(sy ** 2 + sm * hy ** 2))
+                   for x, y in probe[:, [0, 2]]])
+    for col, k in ((1, 0), (3, 1)):
+        kick = p2[:, col] - probe[:, col]
+        err = np.sqrt(np.mean((kick - ex[:, k]) ** 2)) / np.sqrt(np.mean(ex[:, k] ** 2))
+        assert err < 0.01, (col, err)
+
+
+def test_pic_6d_head_on_single_slice_is_4d():
+    b1, b2 = _beam(3000, 8), _beam(2500, 9, off=(1e-5, 2e-5))
+    b1[:, 4] = 0.
+    b2[:, 4] = 0.
+    args = (1.1e11, 0.9e11, 250., 260., 0.7, 1.2)
+    f1, f2 = pyoracle.bb_pic_4d(args[0], b1, args[1], b2, *args[2:])
+    g1, g2 = pyoracle.bb_pic_6d(args[0], b1, args[1], b2, *args[2:], 1, 1)
+    assert np.array_equal(g1, f1) and np.array_equal(g2, f2)
+    assert not np.array_equal(f1, b1)
+
+
+def test_pic_mirror_symmetry():
+    """a beam and its mirror image come out as mirror images (to rounding:
+    the grid positions of mirrored particles are computed from the other
+    edge)"""
+    b1 = _beam(20000, 6, off=(2e-5, -1e-5))
+    o1, o2 = pyoracle.bb_pic_4d(1e11, b1, 1e11, _mirror(b1), 200., 200., 1., 1.)
+    k1, k2 = o1 - b1, _mirror(o2) - b1
+    assert np.abs(k1 - k2).max() < 1e-9 * np.abs(k1).max()
