@@ -483,6 +483,31 @@ int st_bb_soft_gaussian_6d_zb_host(st_ctx *ctx, double Np1, int Mp1, double *x1,
                                    int Nslice2, const double *zb1, const double *zb2);
 int st_zboundary_slices_gaussian(double *zb, int Nslice, double sigmal);
 
+/*
+ * Particle-in-cell strong-strong beam-beam (the same beams and arguments):
+ * st_bb_pic_4d: BeamBeam4D_PIC (simtrack.cpp:23286): a 64 x 64 grid of
+ *   15 times the smaller rms size centred on the source beam, 3 x 3 charge
+ *   weights, the integrated Green function convolved by the reference's
+ *   radix-2 FFT on the doubled grid, the field by its 4th-order stencil,
+ *   interpolated back; beam 2 is kicked by beam 1's field, then beam 1 by
+ *   beam 2's.  Only particles within 29 cells of their beam's centre are
+ *   deposited or kicked.  Every step after the moments follows the
+ *   reference's operations in its order (deposition sums each cell in row
+ *   order); the moments are fixed-order sums as above.
+ * st_bb_pic_6d: BeamBeam6D_PIC (cpp:23380): the z-sorted slicing and the
+ *   collision order of st_bb_soft_gaussian_6d, st_bb_pic_4d per slice pair
+ *   with charges Np/Nslice.
+ */
+int st_bb_pic_4d(st_ctx *ctx, double Np1, int Mp1, double *d_x1, double Np2, int Mp2, double *d_x2, double gamma1,
+                 double gamma2, double bbscale1, double bbscale2);
+int st_bb_pic_6d(st_ctx *ctx, double Np1, int Mp1, double *d_x1, double Np2, int Mp2, double *d_x2, double gamma1,
+                 double gamma2, double bbscale1, double bbscale2, int Nslice1, int Nslice2, double *Z1, double *Z2);
+int st_bb_pic_4d_host(st_ctx *ctx, double Np1, int Mp1, double *x1, double Np2, int Mp2, double *x2, double gamma1,
+                      double gamma2, double bbscale1, double bbscale2);
+int st_bb_pic_6d_host(st_ctx *ctx, double Np1, int Mp1, double *x1, double Np2, int Mp2, double *x2, double gamma1,
+                      double gamma2, double bbscale1, double bbscale2, int Nslice1, int Nslice2, double *Z1,
+                      double *Z2);
+
 #ifdef __cplusplus
 }
 #endif

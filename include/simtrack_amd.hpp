@@ -15,6 +15,7 @@
  *   Cal_Orbit_Num, Track_tbt_FMA, Track_tbt_Lyapunov   simtrack.cpp:14947, 21179, 21243
  *   BeamBeam4D_SoftGaussian, BeamBeam6D_SoftGaussian   simtrack.cpp:22846, 22874, 23134
  *   (both forms), Zboundary_slices_Gaussian             simtrack.cpp:3157
+ *   BeamBeam4D_PIC, BeamBeam6D_PIC                     simtrack.cpp:23286, 23380
  *   Cal_OneTurnMap (matrix only), Track_tbt_ActionDiff simtrack.cpp:15122, 21309
  *
  * Element classes: DRIFT, MARKER, BPM, HBPM, VBPM, SBEND, GSBEND, QUAD,
@@ -1375,6 +1376,34 @@ inline void BeamBeam6D_SoftGaussian(double Np1, int Mp1, double x1[], double Np2
     check(st_bb_soft_gaussian_6d_zb_host(default_context(), Np1, Mp1, x1, Np2, Mp2, x2, gamma1, gamma2, bbscale1,
                                          bbscale2, Nslice1, Nslice2, zb1, zb2),
           "BeamBeam6D_SoftGaussian");
+}
+
+/* BeamBeam4D_PIC (simtrack.cpp:23286-23378): particle-in-cell kicks in place */
+inline void BeamBeam4D_PIC(double Np1, int Mp1, double xc1[], double Np2, int Mp2, double xc2[], double gamma1,
+                           double gamma2, double bbscale1, double bbscale2)
+{
+    check(st_bb_pic_4d_host(default_context(), Np1, Mp1, xc1, Np2, Mp2, xc2, gamma1, gamma2, bbscale1, bbscale2),
+          "BeamBeam4D_PIC");
+}
+
+/* BeamBeam6D_PIC (simtrack.cpp:23380-23615): beams sorted by z in place,
+   slice pairs by BeamBeam4D_PIC; writes ./bunch1_Z.dat and ./bunch2_Z.dat */
+inline void BeamBeam6D_PIC(double Np1, int Mp1, double x1[], double Np2, int Mp2, double x2[], double gamma1,
+                           double gamma2, double bbscale1, double bbscale2, int Nslice1, int Nslice2)
+{
+    std::vector<double> Z1((size_t)(Nslice1 > 0 ? Nslice1 : 1)), Z2((size_t)(Nslice2 > 0 ? Nslice2 : 1));
+    check(st_bb_pic_6d_host(default_context(), Np1, Mp1, x1, Np2, Mp2, x2, gamma1, gamma2, bbscale1, bbscale2,
+                            Nslice1, Nslice2, Z1.data(), Z2.data()),
+          "BeamBeam6D_PIC");
+    const char *names[2] = {"./bunch1_Z.dat", "./bunch2_Z.dat"};
+    const std::vector<double> *Z[2] = {&Z1, &Z2};
+    const int ns[2] = {Nslice1, Nslice2};
+    for (int b = 0; b < 2; ++b) {
+        FILE *f = std::fopen(names[b], "w");
+        if (!f) throw Error(std::string("cannot open ") + names[b]);
+        for (int j = 0; j < ns[b]; ++j) std::fprintf(f, "%d  %g\n", j, (*Z[b])[(size_t)j]);
+        std::fclose(f);
+    }
 }
 
 /* Zboundary_slices_Gaussian (simtrack.cpp:3157-3168) */

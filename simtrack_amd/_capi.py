@@ -72,6 +72,7 @@ EXPORTS = [
     "st_one_turn_matrix", "st_lyapunov_scan", "st_fine_tune_window_device", "st_action_diff",
     "st_bb_soft_gaussian_4d", "st_bb_soft_gaussian_6d", "st_bb_soft_gaussian_6d_zb", "st_bb_soft_gaussian_4d_host",
     "st_bb_soft_gaussian_6d_host", "st_bb_soft_gaussian_6d_zb_host", "st_zboundary_slices_gaussian",
+    "st_bb_pic_4d", "st_bb_pic_6d", "st_bb_pic_4d_host", "st_bb_pic_6d_host",
 ]
 
 
@@ -151,10 +152,10 @@ def load(path=None):
     lib.st_surrogate_features.argtypes = [P, P, I, P, I, D, D, D, D, P]
     lib.st_closed_orbit.argtypes = [P, ctypes.POINTER(Globals), P, I, D, P, P, ctypes.POINTER(I)]
     lib.st_one_turn_matrix.argtypes = [P, ctypes.POINTER(Globals), P, I, P, D, P]
-    for f in ("st_bb_soft_gaussian_4d", "st_bb_soft_gaussian_4d_host"):
+    for f in ("st_bb_soft_gaussian_4d", "st_bb_soft_gaussian_4d_host", "st_bb_pic_4d", "st_bb_pic_4d_host"):
         getattr(lib, f).argtypes = [P, D, I, P, D, I, P, D, D, D, D]
     for f in ("st_bb_soft_gaussian_6d", "st_bb_soft_gaussian_6d_host", "st_bb_soft_gaussian_6d_zb",
-              "st_bb_soft_gaussian_6d_zb_host"):
+              "st_bb_soft_gaussian_6d_zb_host", "st_bb_pic_6d", "st_bb_pic_6d_host"):
         getattr(lib, f).argtypes = [P, D, I, P, D, I, P, D, D, D, D, I, I, P, P]
     lib.st_zboundary_slices_gaussian.argtypes = [P, I, D]
     lib.st_fma_scan.argtypes = [P, D, D, D, P, D, D, I, D, D, P, I, ctypes.POINTER(I)]

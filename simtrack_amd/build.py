@@ -54,6 +54,7 @@ def build(verbose=False):
                        ("st_tune.hip", ["--offload-arch=" + ARCH]),
                        ("st_mlp.hip", ["--offload-arch=" + ARCH]),
                        ("st_ssbb.hip", ["--offload-arch=" + ARCH]),
+                       ("st_pic.hip", ["--offload-arch=" + ARCH]),
                        ("st_capi.cpp", ["--offload-arch=" + ARCH, "-x", "hip"]),
                        ("st_da.cpp", [])):
         obj = os.path.join(objdir, src.rsplit(".", 1)[0] + ".o")

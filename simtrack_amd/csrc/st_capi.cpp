@@ -2319,10 +2319,17 @@ struct ss_work {
     int *seg_i, *cnt;
     double *hx[2];
     size_t cap_part, cap_seg;
+    /* particle-in-cell */
+    int *pkey_in, *pkey_out, *pval_in, *pval_out;
+    double *pw, *grid, *params, *tw;
+    void *ptemp;
+    size_t ptemp_bytes;
 };
 
-/* lay out (base == null: measure) the work area for beams of n1, n2 rows */
-size_t ss_layout(char *base, int n1, int n2, int ns1, int ns2, bool host_rows, size_t temp_bytes, ss_work &w)
+/* lay out (base == null: measure) the work area for beams of n1, n2 rows;
+   pic_rows > 0 adds the particle-in-cell buffers for slices of that size */
+size_t ss_layout(char *base, int n1, int n2, int ns1, int ns2, bool host_rows, size_t temp_bytes, ss_work &w,
+                 int pic_rows = 0, size_t pic_temp = 0)
 {
     ss_arena a;
     a.base = base;
@@ -2349,16 +2356,33 @@ size_t ss_layout(char *base, int n1, int n2, int ns1, int ns2, bool host_rows, s
     w.cnt = a.take<int>((size_t)std::max(ns1, ns2) + 1);
     w.hx[0] = host_rows ? a.take<double>((size_t)(n1 > 0 ? n1 : 1) * 6) : nullptr;
     w.hx[1] = host_rows ? a.take<double>((size_t)(n2 > 0 ? n2 : 1) * 6) : nullptr;
+    w.pkey_in = w.pkey_out = w.pval_in = w.pval_out = nullptr;
+    w.pw = w.grid = w.params = w.tw = nullptr;
+    w.ptemp = nullptr;
+    w.ptemp_bytes = pic_temp;
+    if (pic_rows > 0) {
+        const size_t nc = (size_t)pic_rows * 9;
+        w.pkey_in = a.take<int>(nc);
+        w.pkey_out = a.take<int>(nc);
+        w.pval_in = a.take<int>(nc);
+        w.pval_out = a.take<int>(nc);
+        w.pw = a.take<double>(nc);
+        w.ptemp = a.take<char>(pic_temp);
+        w.grid = a.take<double>((size_t)4096 * 4 + (size_t)16384 * 6);
+        w.params = a.take<double>(16);
+        w.tw = a.take<double>(2 * 254);
+    }
     return a.off;
 }
 
 } // namespace
 
-static int ss_prepare(st_ctx *c, int n1, int n2, int ns1, int ns2, bool host_rows, ss_work &w)
+static int ss_prepare(st_ctx *c, int n1, int n2, int ns1, int ns2, bool host_rows, ss_work &w, int pic_rows = 0)
 {
-    size_t temp = 0;
+    size_t temp = 0, ptemp = 0;
     HIPCHK(st_ss_sort_temp_bytes(std::max(std::max(n1, n2), 1), &temp));
-    size_t need = ss_layout(nullptr, n1, n2, ns1, ns2, host_rows, temp, w);
+    if (pic_rows > 0) HIPCHK(st_sort_cells_bytes(pic_rows * 9, &ptemp));
+    size_t need = ss_layout(nullptr, n1, n2, ns1, ns2, host_rows, temp, w, pic_rows, ptemp);
     if (need > c->cap_ss) {
         hipFree(c->d_ss);
         c->d_ss = nullptr;
@@ -2366,7 +2390,34 @@ static int ss_prepare(st_ctx *c, int n1, int n2, int ns1, int ns2, bool host_row
         HIPCHK(hipMalloc((void **)&c->d_ss, need));
         c->cap_ss = need;
     }
-    ss_layout(c->d_ss, n1, n2, ns1, ns2, host_rows, temp, w);
+    ss_layout(c->d_ss, n1, n2, ns1, ns2, host_rows, temp, w, pic_rows, ptemp);
+    if (pic_rows > 0) {
+        /* FFT1D's twiddles (simtrack.cpp:2614-2637), both directions, by its recurrence */
+        static std::vector<double> tw;
+        if (tw.empty()) {
+            tw.assign(2 * 254, 0.);
+            for (int d = 0; d < 2; ++d) {
+                const int dir = d == 0 ? 1 : -1;
+                double *t = tw.data() + 254 * d;
+                double c1 = -1.0, c2 = 0.0;
+                for (int l = 0; l < 7; l++) {
+                    const int l1 = 1 << l;
+                    double u1 = 1.0, u2 = 0.0;
+                    for (int j = 0; j < l1; j++) {
+                        t[2 * (l1 - 1 + j)] = u1;
+                        t[2 * (l1 - 1 + j) + 1] = u2;
+                        const double z = u1 * c1 - u2 * c2;
+                        u2 = u1 * c2 + u2 * c1;
+                        u1 = z;
+                    }
+                    c2 = std::sqrt((1.0 - c1) / 2.0);
+                    if (dir == 1) c2 = -c2;
+                    c1 = std::sqrt((1.0 + c1) / 2.0);
+                }
+            }
+        }
+        HIPCHK(hipMemcpyAsync(w.tw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
     return ST_OK;
 }
 
@@ -2682,6 +2733,169 @@ extern "C" int st_bb_soft_gaussian_6d_zb_host(st_ctx *c, double Np1, int Mp1, do
     if ((rc = ss_prepare(c, Mp1, Mp2, Nslice1, Nslice2, true, w)) || (rc = ss_host_in(c, w, Mp1, x1, Mp2, x2)) ||
         (rc = ss_6d_zb(c, w, Np1, Mp1, w.hx[0], Np2, Mp2, w.hx[1], gamma1, gamma2, bbscale1, bbscale2, Nslice1,
                        Nslice2, zb1, zb2)))
+        return rc;
+    return ss_host_out(c, w, Mp1, x1, Mp2, x2);
+}
+
+/* BeamBeam4D_PIC on rows [s1, s1 + m1) of beam 1 and [s2, s2 + m2) of beam 2
+   (simtrack.cpp:23286-23378): beam 1's charge kicks beam 2, then beam 2's
+   charge kicks beam 1 */
+static int pic_pair(st_ctx *c, ss_work &w, double *d_x1, int s1, int m1, double *d_x2, int s2, int m2, double Np1,
+                    double Np2, double gamma1, double gamma2, double bbscale1, double bbscale2)
+{
+    int ev = -1;
+    if (c->timing) {
+        int rc = timed_event(c, &ev);
+        if (rc) return rc;
+        HIPCHK(hipEventRecord(c->ev_pool[ev], c->stream));
+    }
+    ss_pair_args a{};
+    a.x[0] = d_x1;
+    a.x[1] = d_x2;
+    a.start[0] = s1;
+    a.start[1] = s2;
+    a.count[0] = m1;
+    a.count[1] = m2;
+    a.part = w.part;
+    HIPCHK(st_ss_partials(a, c->stream));
+    HIPCHK(st_pic_params(w.part, a.nb[0], a.nb[1], m1, m2, w.params, c->stream));
+    pic_args p{};
+    p.x[0] = d_x1;
+    p.x[1] = d_x2;
+    p.start[0] = s1;
+    p.start[1] = s2;
+    p.count[0] = m1;
+    p.count[1] = m2;
+    p.params = w.params;
+    p.key_in = w.pkey_in;
+    p.key_out = w.pkey_out;
+    p.val_in = w.pval_in;
+    p.val_out = w.pval_out;
+    p.w = w.pw;
+    p.temp = w.ptemp;
+    p.temp_bytes = w.ptemp_bytes;
+    double *g = w.grid;
+    p.src = g;
+    p.phi = g + 4096;
+    p.Ex = g + 2 * 4096;
+    p.Ey = g + 3 * 4096;
+    p.gr = g + 4 * 4096;
+    p.gi = p.gr + 16384;
+    p.sr = p.gi + 16384;
+    p.si = p.sr + 16384;
+    p.pr = p.si + 16384;
+    p.pi = p.pr + 16384;
+    p.tw_fwd = w.tw;
+    p.tw_inv = w.tw + 254;
+    p.nsm[0] = Np1 / m1; /* (Np1/Mp1), cpp:23338 */
+    p.nsm[1] = Np2 / m2;
+    p.gamma[0] = gamma1;
+    p.gamma[1] = gamma2;
+    p.bbscale[0] = bbscale1;
+    p.bbscale[1] = bbscale2;
+    HIPCHK(st_pic_solve_and_kick(p, 0, c->stream));
+    HIPCHK(st_pic_solve_and_kick(p, 1, c->stream));
+    if (c->timing) {
+        HIPCHK(hipEventRecord(c->ev_pool[ev + 1], c->stream));
+        c->pending.push_back({ev, ev + 1});
+        c->pending_work.push_back((int64_t)m1 + m2);
+    }
+    return ST_OK;
+}
+
+/* BeamBeam6D_PIC, simtrack.cpp:23380-23615 */
+static int pic_6d(st_ctx *c, ss_work &w, double Np1, int Mp1, double *d_x1, double Np2, int Mp2, double *d_x2,
+                  double gamma1, double gamma2, double bbscale1, double bbscale2, int Ns1, int Ns2, double *Z1o,
+                  double *Z2o)
+{
+    HIPCHK(st_ss_sort_by_z(d_x1, Mp1, w.s, c->stream));
+    HIPCHK(st_ss_sort_by_z(d_x2, Mp2, w.s, c->stream));
+    const int mc1 = Mp1 / Ns1, mc2 = Mp2 / Ns2;
+    std::vector<int> st1(Ns1), ct1(Ns1, mc1), st2(Ns2), ct2(Ns2, mc2);
+    for (int i = 0; i < Ns1; i++) st1[i] = i * mc1;
+    for (int i = 0; i < Ns2; i++) st2[i] = i * mc2;
+    int rc = ss_slice_centres(c, w, d_x1, nullptr, st1, ct1, w.Z);
+    if (!rc) rc = ss_slice_centres(c, w, d_x2, nullptr, st2, ct2, w.Z + Ns1);
+    if (rc) return rc;
+    std::vector<double> Z1(Ns1), Z2(Ns2);
+    HIPCHK(hipMemcpyAsync(Z1.data(), w.Z, Ns1 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(Z2.data(), w.Z + Ns1, Ns2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (Z1o) std::copy(Z1.begin(), Z1.end(), Z1o);
+    if (Z2o) std::copy(Z2.begin(), Z2.end(), Z2o);
+    std::vector<int> I1, I2;
+    std::vector<double> S;
+    ss_schedule(Z1, Z2, I1, I2, S);
+    for (size_t i = 0; i < S.size(); ++i) {
+        const int a = I1[i] * mc1, b = I2[i] * mc2;
+        HIPCHK(st_pic_drift(d_x1, a, mc1, S[i], 1, c->stream));
+        HIPCHK(st_pic_drift(d_x2, b, mc2, S[i], -1, c->stream));
+        rc = pic_pair(c, w, d_x1, a, mc1, d_x2, b, mc2, Np1 / Ns1, Np2 / Ns2, gamma1, gamma2, bbscale1, bbscale2);
+        if (rc) return rc;
+        HIPCHK(st_pic_drift(d_x1, a, mc1, S[i], -1, c->stream));
+        HIPCHK(st_pic_drift(d_x2, b, mc2, S[i], 1, c->stream));
+    }
+    return ST_OK;
+}
+
+extern "C" int st_bb_pic_4d(st_ctx *c, double Np1, int Mp1, double *d_x1, double Np2, int Mp2, double *d_x2,
+                            double gamma1, double gamma2, double bbscale1, double bbscale2)
+{
+    int rc = ss_check(c, Mp1, d_x1, Mp2, d_x2, "st_bb_pic_4d");
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(c->device));
+    ss_work w;
+    if ((rc = ss_prepare(c, Mp1, Mp2, 1, 1, false, w, std::max(std::max(Mp1, Mp2), 1)))) return rc;
+    if ((rc = pic_pair(c, w, d_x1, 0, Mp1, d_x2, 0, Mp2, Np1, Np2, gamma1, gamma2, bbscale1, bbscale2))) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return ST_OK;
+}
+
+extern "C" int st_bb_pic_6d(st_ctx *c, double Np1, int Mp1, double *d_x1, double Np2, int Mp2, double *d_x2,
+                            double gamma1, double gamma2, double bbscale1, double bbscale2, int Nslice1, int Nslice2,
+                            double *Z1, double *Z2)
+{
+    int rc = ss_check(c, Mp1, d_x1, Mp2, d_x2, "st_bb_pic_6d");
+    if (rc) return rc;
+    if (Nslice1 < 1 || Nslice2 < 1) return fail(ST_ERR_PARAM, "st_bb_pic_6d: Nslice must be >= 1");
+    HIPCHK(hipSetDevice(c->device));
+    ss_work w;
+    const int pr = std::max(std::max(Mp1 / Nslice1, Mp2 / Nslice2), 1);
+    if ((rc = ss_prepare(c, Mp1, Mp2, Nslice1, Nslice2, false, w, pr))) return rc;
+    if ((rc = pic_6d(c, w, Np1, Mp1, d_x1, Np2, Mp2, d_x2, gamma1, gamma2, bbscale1, bbscale2, Nslice1, Nslice2, Z1,
+                     Z2)))
+        return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return ST_OK;
+}
+
+extern "C" int st_bb_pic_4d_host(st_ctx *c, double Np1, int Mp1, double *x1, double Np2, int Mp2, double *x2,
+                                 double gamma1, double gamma2, double bbscale1, double bbscale2)
+{
+    int rc = ss_check(c, Mp1, x1, Mp2, x2, "st_bb_pic_4d_host");
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(c->device));
+    ss_work w;
+    if ((rc = ss_prepare(c, Mp1, Mp2, 1, 1, true, w, std::max(std::max(Mp1, Mp2), 1))) ||
+        (rc = ss_host_in(c, w, Mp1, x1, Mp2, x2)) ||
+        (rc = pic_pair(c, w, w.hx[0], 0, Mp1, w.hx[1], 0, Mp2, Np1, Np2, gamma1, gamma2, bbscale1, bbscale2)))
+        return rc;
+    return ss_host_out(c, w, Mp1, x1, Mp2, x2);
+}
+
+extern "C" int st_bb_pic_6d_host(st_ctx *c, double Np1, int Mp1, double *x1, double Np2, int Mp2, double *x2,
+                                 double gamma1, double gamma2, double bbscale1, double bbscale2, int Nslice1,
+                                 int Nslice2, double *Z1, double *Z2)
+{
+    int rc = ss_check(c, Mp1, x1, Mp2, x2, "st_bb_pic_6d_host");
+    if (rc) return rc;
+    if (Nslice1 < 1 || Nslice2 < 1) return fail(ST_ERR_PARAM, "st_bb_pic_6d_host: Nslice must be >= 1");
+    HIPCHK(hipSetDevice(c->device));
+    ss_work w;
+    const int pr = std::max(std::max(Mp1 / Nslice1, Mp2 / Nslice2), 1);
+    if ((rc = ss_prepare(c, Mp1, Mp2, Nslice1, Nslice2, true, w, pr)) || (rc = ss_host_in(c, w, Mp1, x1, Mp2, x2)) ||
+        (rc = pic_6d(c, w, Np1, Mp1, w.hx[0], Np2, Mp2, w.hx[1], gamma1, gamma2, bbscale1, bbscale2, Nslice1,
+                     Nslice2, Z1, Z2)))
         return rc;
     return ss_host_out(c, w, Mp1, x1, Mp2, x2);
 }

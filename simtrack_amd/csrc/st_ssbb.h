@@ -33,7 +33,31 @@ struct ss_sort_buffers {
     size_t temp_bytes;
 };
 
+/* one particle-in-cell pair (st_pic.hip): beam s's charge on the grid,
+   felt by beam 1 - s */
+struct pic_args {
+    double *x[2];         /* beam rows; the slice is rows [start, start + count) */
+    int start[2], count[2];
+    const double *params; /* mean, rms of both beams, hx, hy (k_pic_params)      */
+    int *key_in, *key_out, *val_in, *val_out; /* 9 x count contributions          */
+    double *w;
+    void *temp;
+    size_t temp_bytes;
+    double *src, *gr, *gi, *sr, *si, *pr, *pi, *phi, *Ex, *Ey;
+    const double *tw_fwd, *tw_inv; /* FFT1D twiddles for dir = 1 and dir = -1   */
+    double nsm[2];        /* charge / rows of beam s as the source               */
+    double gamma[2], bbscale[2];
+};
+
 hipError_t st_ss_collide(const ss_pair_args &a, hipStream_t stream);
+/* only the partial sums of st_ss_collide (no kick); sets a.nb */
+hipError_t st_ss_partials(ss_pair_args &a, hipStream_t stream);
+hipError_t st_sort_cells_bytes(int n, size_t *bytes);
+hipError_t st_sort_cells(void *temp, size_t bytes, const int *kin, int *kout, const int *vin, int *vout, int n,
+                         hipStream_t stream);
+hipError_t st_pic_params(const double *part, int nb0, int nb1, int m0, int m1, double *params, hipStream_t stream);
+hipError_t st_pic_solve_and_kick(const pic_args &a, int s, hipStream_t stream);
+hipError_t st_pic_drift(double *x, int start, int m, double S, int sgn, hipStream_t stream);
 int st_ss_partial_blocks(int count);
 hipError_t st_ss_zsums(const double *x, const int *idx, const int *seg_start, const int *seg_count,
                        const int *blk_seg, const int *blk_chunk, int nblk, const int *seg_first_blk,

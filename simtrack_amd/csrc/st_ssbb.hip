@@ -238,6 +238,32 @@ hipError_t st_ss_collide(const ss_pair_args &a0, hipStream_t stream)
     return hipGetLastError();
 }
 
+hipError_t st_ss_partials(ss_pair_args &a, hipStream_t stream)
+{
+    a.nb[0] = (a.count[0] + SS_CHUNK - 1) / SS_CHUNK;
+    a.nb[1] = (a.count[1] + SS_CHUNK - 1) / SS_CHUNK;
+    if (a.nb[0] + a.nb[1] == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ss_partial, dim3(a.nb[0] + a.nb[1]), dim3(SS_BLK), 0, stream, a);
+    return hipGetLastError();
+}
+
+/* stable sort of (cell, contribution) pairs, cells in [0, 4096] */
+hipError_t st_sort_cells_bytes(int n, size_t *bytes)
+{
+    return hipcub::DeviceRadixSort::SortPairs(nullptr, *bytes, (const int *)nullptr, (int *)nullptr,
+                                              (const int *)nullptr, (int *)nullptr, n, 0, 13);
+}
+
+hipError_t st_sort_cells(void *temp, size_t bytes, const int *kin, int *kout, const int *vin, int *vout, int n,
+                         hipStream_t stream)
+{
+    size_t need = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, need, kin, kout, vin, vout, n, 0, 13, stream);
+    if (e != hipSuccess) return e;
+    if (need > bytes) return hipErrorInvalidValue;
+    return hipcub::DeviceRadixSort::SortPairs(temp, need, kin, kout, vin, vout, n, 0, 13, stream);
+}
+
 int st_ss_partial_blocks(int count) { return (count + SS_CHUNK - 1) / SS_CHUNK; }
 
 hipError_t st_ss_zsums(const double *x, const int *idx, const int *seg_start, const int *seg_count,

@@ -1,5 +1,6 @@
-"""Strong-strong soft-Gaussian beam-beam between two macroparticle beams on
-the GPU, with the reference's function names (simtrack.cpp:22846-23284).
+"""Strong-strong beam-beam between two macroparticle beams on the GPU, with
+the reference's function names: soft-Gaussian (simtrack.cpp:22846-23284)
+and particle-in-cell (cpp:23286-23615).
 
 Beams are (Mp, 6) arrays of rows (x, px, y, py, z, pt).  numpy arrays are
 copied to the device and back (the reference's in-place update is kept:
@@ -75,6 +76,26 @@ def BeamBeam6D_SoftGaussian(ctx, Np1, x1, Np2, x2, gamma1, gamma2, bbscale1, bbs
     return None
 
 
+def BeamBeam4D_PIC(ctx, Np1, x1, Np2, x2, gamma1, gamma2, bbscale1, bbscale2):
+    """BeamBeam4D_PIC (simtrack.cpp:23286): particle-in-cell kicks, in place."""
+    lib = load()
+    pa, n1, pb, n2, dev = _pair(x1, x2, "BeamBeam4D_PIC")
+    f = lib.st_bb_pic_4d if dev else lib.st_bb_pic_4d_host
+    check(f(ctx.h, Np1, n1, pa, Np2, n2, pb, gamma1, gamma2, bbscale1, bbscale2), "BeamBeam4D_PIC")
+
+
+def BeamBeam6D_PIC(ctx, Np1, x1, Np2, x2, gamma1, gamma2, bbscale1, bbscale2, Nslice1, Nslice2):
+    """BeamBeam6D_PIC (simtrack.cpp:23380): beams sorted by z in place, slice
+    pairs collided by BeamBeam4D_PIC; returns the slice centres (Z1, Z2)."""
+    lib = load()
+    pa, n1, pb, n2, dev = _pair(x1, x2, "BeamBeam6D_PIC")
+    z1, z2 = np.zeros(int(Nslice1)), np.zeros(int(Nslice2))
+    f = lib.st_bb_pic_6d if dev else lib.st_bb_pic_6d_host
+    check(f(ctx.h, Np1, n1, pa, Np2, n2, pb, gamma1, gamma2, bbscale1, bbscale2, int(Nslice1), int(Nslice2),
+            _ptr(z1), _ptr(z2)), "BeamBeam6D_PIC")
+    return z1, z2
+
+
 def Zboundary_slices_Gaussian(Nslice, sigmal):
     """Zboundary_slices_Gaussian (simtrack.cpp:3157): Nslice + 1 boundaries, decreasing."""
     zb = np.zeros(int(Nslice) + 1)
@@ -82,4 +103,5 @@ def Zboundary_slices_Gaussian(Nslice, sigmal):
     return zb
 
 
-__all__ = ["BeamBeam4D_SoftGaussian", "BeamBeam6D_SoftGaussian", "Zboundary_slices_Gaussian"]
+__all__ = ["BeamBeam4D_SoftGaussian", "BeamBeam6D_SoftGaussian", "BeamBeam4D_PIC", "BeamBeam6D_PIC",
+           "Zboundary_slices_Gaussian"]
