@@ -67,6 +67,8 @@ def main():
                          "rhic: the reference's RHIC example lattice; strong_strong: soft-Gaussian 6-D "
                          "collisions of two 1,048,576-macroparticle beams")
     ap.add_argument("--slices", type=int, default=8, help="strong_strong: z slices per beam")
+    ap.add_argument("--pic", action="store_true", help="strong_strong: the particle-in-cell solver "
+                    "(BeamBeam6D_PIC, z-sorted slices) instead of the soft-Gaussian one")
     ap.add_argument("--seeds", type=int, default=64, help="config2: lattices")
     ap.add_argument("--per-seed", type=int, default=4096, help="config2/3: particles per lattice")
     ap.add_argument("--train-seeds", type=int, default=16, help="config3: lattices for the training set")
@@ -251,7 +253,10 @@ def bench_strong_strong(args):
     torch.cuda.synchronize()
 
     def collide():
-        SS.BeamBeam6D_SoftGaussian(ctx, Np, d1, Np, d2, gp.gamma, gp.gamma, 1., 1., ns, ns, zb, zb)
+        if args.pic:
+            SS.BeamBeam6D_PIC(ctx, Np, d1, Np, d2, gp.gamma, gp.gamma, 1., 1., ns, ns)
+        else:
+            SS.BeamBeam6D_SoftGaussian(ctx, Np, d1, Np, d2, gp.gamma, gp.gamma, 1., 1., ns, ns, zb, zb)
     for _ in range(args.warmup):
         collide()
     ctx.set_timing(True)
@@ -269,17 +274,26 @@ def bench_strong_strong(args):
     baseline = None
     if not args.no_cpu_baseline:
         import pyoracle
-        m = min(n, 131072)
+        m = min(n, 131072 if not args.pic else 16384)
         c1, c2 = _ss_beams(m, 2031)
         t = time.perf_counter()
-        pyoracle.bb_soft_gaussian_6d_zb(Np, c1, Np, c2, gp.gamma, gp.gamma, 1., 1., zb, zb)
+        if args.pic:
+            # the reference's z sort is an exchange sort (O(m^2)); sort first, untimed
+            c1, c2 = c1[np.argsort(-c1[:, 4], kind="stable")], c2[np.argsort(-c2[:, 4], kind="stable")]
+            t = time.perf_counter()
+            pyoracle.bb_pic_6d(Np, c1, Np, c2, gp.gamma, gp.gamma, 1., 1., ns, ns)
+            rows = ns * 2 * (m // ns) * ns
+        else:
+            pyoracle.bb_soft_gaussian_6d_zb(Np, c1, Np, c2, gp.gamma, gp.gamma, 1., 1., zb, zb)
+            inside = lambda b: int(((b[:, 4] < zb[0]) & (b[:, 4] >= zb[-1])).sum())
+            rows = ns * (inside(c1) + inside(c2))
         dt = time.perf_counter() - t
-        inside = lambda b: int(((b[:, 4] < zb[0]) & (b[:, 4] >= zb[-1])).sum())
-        baseline = {"value": ns * (inside(c1) + inside(c2)) / dt, "unit": "macroparticle kicks/s", "cores": 1,
+        baseline = {"value": rows / dt, "unit": "macroparticle kicks/s", "cores": 1,
                     "kind": "port", "sample": f"one collision of two {m}-macroparticle beams, {ns} x {ns} slices "
-                                              f"(oracle restatement, one thread), {dt:.1f} s"}
+                                              f"(oracle restatement, one thread{', beams pre-sorted' if args.pic else ''}), "
+                                              f"{dt:.1f} s"}
 
-    bytes_ = flops.SS_BYTES * k_rows
+    bytes_ = (flops.PIC_BYTES if args.pic else flops.SS_BYTES) * k_rows
     fl = (flops.SS_MOMENTS + flops.SS_KICK_ELLIPTIC) * k_rows
     out = {
         "metric": "strong-strong macroparticle kicks/s",
@@ -294,12 +308,16 @@ def bench_strong_strong(args):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded Gaussian beams)",
-        "config": {"workload": "strong_strong: two 100 GeV proton beams, soft-Gaussian 6-D collision with "
-                               f"{ns} x {ns} Gaussian z slices (BeamBeam6D_SoftGaussian, slice boundaries)",
+        "config": {"workload": ("strong_strong: two 100 GeV proton beams, particle-in-cell 6-D collision with "
+                                f"{ns} x {ns} z-sorted slices (BeamBeam6D_PIC, 64 x 64 grid)") if args.pic else
+                               ("strong_strong: two 100 GeV proton beams, soft-Gaussian 6-D collision with "
+                                f"{ns} x {ns} Gaussian z slices (BeamBeam6D_SoftGaussian, slice boundaries)"),
                    "particles_per_beam": n, "slices": ns, "charge_per_beam": Np},
         "roofline": {"bound": "hbm", "achieved": bytes_ / k_sec / 1e9, "peak": 8000.0, "unit": "GB/s",
-                     "frac": bytes_ / k_sec / 8e12, "traffic": None, "algorithmic_bytes_per_row": flops.SS_BYTES,
-                     "fp64_frac": fl / k_sec / flops.PEAK_FP64_VECTOR, "kernel": "k_ss_partial + k_ss_kick",
+                     "frac": bytes_ / k_sec / 8e12, "traffic": None,
+                     "algorithmic_bytes_per_row": flops.PIC_BYTES if args.pic else flops.SS_BYTES,
+                     "fp64_frac": None if args.pic else fl / k_sec / flops.PEAK_FP64_VECTOR,
+                     "kernel": "st_pic.hip pair (two field solves)" if args.pic else "k_ss_partial + k_ss_kick",
                      "avg_pair_ms": 1e3 * k_sec / max(k_pairs, 1), "pairs": k_pairs,
                      "kernel_share_of_wall": k_sec / wall},
         "cpu_baseline": baseline,

@@ -2321,7 +2321,7 @@ struct ss_work {
     size_t cap_part, cap_seg;
     /* particle-in-cell */
     int *pkey_in, *pkey_out, *pval_in, *pval_out;
-    double *pw, *grid, *params, *tw;
+    double *pw, *pws, *grid, *params, *tw;
     void *ptemp;
     size_t ptemp_bytes;
 };
@@ -2357,7 +2357,7 @@ size_t ss_layout(char *base, int n1, int n2, int ns1, int ns2, bool host_rows, s
     w.hx[0] = host_rows ? a.take<double>((size_t)(n1 > 0 ? n1 : 1) * 6) : nullptr;
     w.hx[1] = host_rows ? a.take<double>((size_t)(n2 > 0 ? n2 : 1) * 6) : nullptr;
     w.pkey_in = w.pkey_out = w.pval_in = w.pval_out = nullptr;
-    w.pw = w.grid = w.params = w.tw = nullptr;
+    w.pw = w.pws = w.grid = w.params = w.tw = nullptr;
     w.ptemp = nullptr;
     w.ptemp_bytes = pic_temp;
     if (pic_rows > 0) {
@@ -2367,6 +2367,7 @@ size_t ss_layout(char *base, int n1, int n2, int ns1, int ns2, bool host_rows, s
         w.pval_in = a.take<int>(nc);
         w.pval_out = a.take<int>(nc);
         w.pw = a.take<double>(nc);
+        w.pws = a.take<double>(nc);
         w.ptemp = a.take<char>(pic_temp);
         w.grid = a.take<double>((size_t)4096 * 4 + (size_t)16384 * 6);
         w.params = a.take<double>(16);
@@ -2772,6 +2773,7 @@ static int pic_pair(st_ctx *c, ss_work &w, double *d_x1, int s1, int m1, double 
     p.val_in = w.pval_in;
     p.val_out = w.pval_out;
     p.w = w.pw;
+    p.ws = w.pws;
     p.temp = w.ptemp;
     p.temp_bytes = w.ptemp_bytes;
     double *g = w.grid;

@@ -6,7 +6,8 @@
  *   k_pic_contrib    each source row's 3 x 3 weights (Cal_Grid_Weight) as
  *                    (cell, weight) contributions, row-major;
  *   (radix sort of the contributions by cell, stable)
- *   k_pic_cell_sum   each cell's charge, its contributions added in row order
+ *   k_pic_gather_w, k_pic_cell_sum
+ *                    each cell's charge, its contributions added in row order
  *                    from zero: the reference's Charge2Grid sums, bit for bit;
  *   k_pic_green      the integrated Green function on the doubled grid
  *                    (Cal_Effective_Green_Function, before its FFT);
@@ -177,13 +178,30 @@ ST_DEV int pic_lower_bound(const int *key, int n, int v)
     return a;
 }
 
-__global__ void k_pic_cell_sum(const int *key, const int *val, int n, const double *w, double *src)
+/* the weights in cell order (ws[q] = w[val[q]]), so that each cell's sum
+   below reads consecutive memory */
+__global__ void k_pic_gather_w(const int *val, int n, const double *w, double *ws)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) ws[q] = w[val[q]];
+}
+
+__global__ void k_pic_cell_sum(const int *key, int n, const double *ws, double *src)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= PIC_NCELL) return;
     const int lo = pic_lower_bound(key, n, c), hi = pic_lower_bound(key, n, c + 1);
     double t = 0;
-    for (int q = lo; q < hi; ++q) t = t + w[val[q]];
+    int q = lo;
+    /* sixteen loads in flight, then the additions in row order */
+    for (; q + 16 <= hi; q += 16) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = ws[q + u];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) t = t + v[u];
+    }
+    for (; q < hi; ++q) t = t + ws[q];
     src[c] = t;
 }
 
@@ -396,9 +414,9 @@ hipError_t st_pic_solve_and_kick(const pic_args &a, int s, hipStream_t stream)
                            a.params, s, a.key_in, a.val_in, a.w);
         hipError_t e = st_sort_cells(a.temp, a.temp_bytes, a.key_in, a.key_out, a.val_in, a.val_out, n, stream);
         if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_pic_gather_w, dim3((n + 255) / 256), dim3(256), 0, stream, a.val_out, n, a.w, a.ws);
     }
-    hipLaunchKernelGGL(k_pic_cell_sum, dim3(PIC_NCELL / 256), dim3(256), 0, stream, a.key_out, a.val_out, n, a.w,
-                       a.src);
+    hipLaunchKernelGGL(k_pic_cell_sum, dim3(PIC_NCELL / 64), dim3(64), 0, stream, a.key_out, n, a.ws, a.src);
     const int g2 = PIC_N2 * PIC_N2 / 256;
     hipLaunchKernelGGL(k_pic_green, dim3(g2), dim3(256), 0, stream, a.params, s, a.gr, a.gi);
     hipError_t e = pic_fft2d(a.gr, a.gi, a.tw_fwd, 1, stream);

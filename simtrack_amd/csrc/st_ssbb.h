@@ -40,7 +40,7 @@ struct pic_args {
     int start[2], count[2];
     const double *params; /* mean, rms of both beams, hx, hy (k_pic_params)      */
     int *key_in, *key_out, *val_in, *val_out; /* 9 x count contributions          */
-    double *w;
+    double *w, *ws;       /* weights in contribution order and in cell order      */
     void *temp;
     size_t temp_bytes;
     double *src, *gr, *gi, *sr, *si, *pr, *pi, *phi, *Ex, *Ey;

@@ -37,6 +37,10 @@ BB4D_ELLIPTIC = 40 + 2 * 150   # two cerrf calls, cpp:6343-6380
 SS_MOMENTS = 10
 SS_KICK_ELLIPTIC = 12 + BB4D_ELLIPTIC
 SS_BYTES = 96         # x, px, y, py read by both passes and written by the kick
+# particle-in-cell pair, per row: x, y read for the moments (16) and by the
+# deposition (16), x, y, px, py read and px, py written by the kick (48),
+# 9 contributions of (cell, row, weight) written, sorted and read (~9 x 48)
+PIC_BYTES = 16 + 16 + 48 + 9 * 48
 
 
 def mult_kick(norder, mask, bend=False):

@@ -1,9 +1,10 @@
 // Strong-strong beam-beam through the C++ API: two beams of rows built from
 // a closed formula (no random numbers), collided by BeamBeam4D_SoftGaussian,
 // by BeamBeam6D_SoftGaussian with 5 x 4 equal-population slices (writing
-// ./bunch1_Z.dat, ./bunch2_Z.dat in the working directory) and with 4 x 4
-// Gaussian slice boundaries.  Writes to stdout, as raw doubles, for each of
-// the three collisions the two beams before and after it.
+// ./bunch1_Z.dat, ./bunch2_Z.dat in the working directory), with 4 x 4
+// Gaussian slice boundaries, by BeamBeam4D_PIC and by BeamBeam6D_PIC with
+// 3 x 2 slices.  Writes to stdout, as raw doubles, for each of the five
+// collisions the two beams before and after it.
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -31,7 +32,7 @@ int main()
 {
     try {
         const int n1 = 3001, n2 = 2500;
-        for (int form = 0; form < 3; ++form) {
+        for (int form = 0; form < 5; ++form) {
             std::vector<double> a = beam(n1, 0.1, 1e-5), b = beam(n2, 0.7, -2e-5);
             std::fwrite(a.data(), sizeof(double), a.size(), stdout);
             std::fwrite(b.data(), sizeof(double), b.size(), stdout);
@@ -39,11 +40,15 @@ int main()
                 BeamBeam4D_SoftGaussian(1.2e11, n1, a.data(), 0.9e11, n2, b.data(), 230., 240., 0.8, 1.1);
             } else if (form == 1) {
                 BeamBeam6D_SoftGaussian(1.2e11, n1, a.data(), 0.9e11, n2, b.data(), 230., 240., 0.8, 1.1, 5, 4);
-            } else {
+            } else if (form == 2) {
                 double zb[5];
                 Zboundary_slices_Gaussian(zb, 4, 0.04);
                 BeamBeam6D_SoftGaussian(1.2e11, n1, a.data(), 0.9e11, n2, b.data(), 230., 240., 0.8, 1.1, 4, 4, zb,
                                         zb);
+            } else if (form == 3) {
+                BeamBeam4D_PIC(1.2e11, n1, a.data(), 0.9e11, n2, b.data(), 230., 240., 0.8, 1.1);
+            } else {
+                BeamBeam6D_PIC(1.2e11, n1, a.data(), 0.9e11, n2, b.data(), 230., 240., 0.8, 1.1, 3, 2);
             }
             std::fwrite(a.data(), sizeof(double), a.size(), stdout);
             std::fwrite(b.data(), sizeof(double), b.size(), stdout);

@@ -135,9 +135,10 @@ def test_bad_boundaries_are_rejected(ctx):
 
 
 def test_cpp_api_equals_python(ctx, tmp_path):
-    """tests/cpp/strong_strong.cpp (BeamBeam4D_SoftGaussian and both forms of
-    BeamBeam6D_SoftGaussian through include/simtrack_amd.hpp) gives the
-    Python API's bits, and writes the reference's bunch1_Z.dat/bunch2_Z.dat."""
+    """tests/cpp/strong_strong.cpp (BeamBeam4D_SoftGaussian, both forms of
+    BeamBeam6D_SoftGaussian, BeamBeam4D_PIC and BeamBeam6D_PIC through
+    include/simtrack_amd.hpp) gives the Python API's bits, and writes the
+    reference's bunch1_Z.dat/bunch2_Z.dat."""
     import os
     import subprocess
     exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "build", "strong_strong")
@@ -147,7 +148,7 @@ def test_cpp_api_equals_python(ctx, tmp_path):
     assert r.returncode == 0, r.stderr.decode()
     n1, n2 = 3001, 2500
     v = np.frombuffer(r.stdout, dtype=np.float64)
-    assert v.size == 3 * 2 * 6 * (n1 + n2)
+    assert v.size == 5 * 2 * 6 * (n1 + n2)
     at = 0
     Np1, Np2, g1, g2, k1, k2 = 1.2e11, 0.9e11, 230., 240., 0.8, 1.1
 
@@ -156,19 +157,24 @@ def test_cpp_api_equals_python(ctx, tmp_path):
         a = v[at:at + 6 * n].reshape(n, 6).copy()
         at += 6 * n
         return a
-    for form in range(3):
+    for form in range(5):
         a, b = take(n1), take(n2)
         ra, rb = take(n1), take(n2)
         if form == 0:
             SS.BeamBeam4D_SoftGaussian(ctx, Np1, a, Np2, b, g1, g2, k1, k2)
         elif form == 1:
-            z1, z2 = SS.BeamBeam6D_SoftGaussian(ctx, Np1, a, Np2, b, g1, g2, k1, k2, 5, 4)
-            lines = (tmp_path / "bunch1_Z.dat").read_text().splitlines()
-            assert lines == [f"{j}  {z:g}" for j, z in enumerate(z1)]
-            assert len((tmp_path / "bunch2_Z.dat").read_text().splitlines()) == 4
-        else:
+            SS.BeamBeam6D_SoftGaussian(ctx, Np1, a, Np2, b, g1, g2, k1, k2, 5, 4)
+        elif form == 2:
             zb = SS.Zboundary_slices_Gaussian(4, 0.04)
             SS.BeamBeam6D_SoftGaussian(ctx, Np1, a, Np2, b, g1, g2, k1, k2, 4, 4, zb, zb)
+        elif form == 3:
+            SS.BeamBeam4D_PIC(ctx, Np1, a, Np2, b, g1, g2, k1, k2)
+        else:
+            # the last call that writes bunch1_Z.dat / bunch2_Z.dat
+            z1, z2 = SS.BeamBeam6D_PIC(ctx, Np1, a, Np2, b, g1, g2, k1, k2, 3, 2)
+            lines = (tmp_path / "bunch1_Z.dat").read_text().splitlines()
+            assert lines == [f"{j}  {z:g}" for j, z in enumerate(z1)]
+            assert len((tmp_path / "bunch2_Z.dat").read_text().splitlines()) == 2
         assert np.array_equal(a, ra) and np.array_equal(b, rb), f"form {form}"
 
 
