@@ -78,8 +78,8 @@ def main():
                     "instead of spin tracking")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the tracking "
-              f"path shards by particles or seeds with no collective, see the sharding notes", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the tracking path shards by "
+              f"particles or lattice seeds with no collective", file=sys.stderr)
         sys.exit(2)
 
     if args.cpu_sample is None:

@@ -35,7 +35,7 @@
  *
  * records_from_line() lowers any Line-like object whose Cell[] elements have
  * TYPE, L, DX..DPSI, APx, APy and GetP() -- this header's Line or the
- * reference's own (see INTEGRATION.md).
+ * reference's own.
  */
 #ifndef SIMTRACK_AMD_HPP
 #define SIMTRACK_AMD_HPP
