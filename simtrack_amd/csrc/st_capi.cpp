@@ -2393,10 +2393,10 @@ static int ss_prepare(st_ctx *c, int n1, int n2, int ns1, int ns2, bool host_row
     }
     ss_layout(c->d_ss, n1, n2, ns1, ns2, host_rows, temp, w, pic_rows, ptemp);
     if (pic_rows > 0) {
-        /* FFT1D's twiddles (simtrack.cpp:2614-2637), both directions, by its recurrence */
-        static std::vector<double> tw;
-        if (tw.empty()) {
-            tw.assign(2 * 254, 0.);
+        /* FFT1D's twiddles (simtrack.cpp:2614-2637), both directions, by its
+           recurrence; built once (thread-safe static initialisation) */
+        static const std::vector<double> tw = [] {
+            std::vector<double> tw(2 * 254, 0.);
             for (int d = 0; d < 2; ++d) {
                 const int dir = d == 0 ? 1 : -1;
                 double *t = tw.data() + 254 * d;
@@ -2416,7 +2416,8 @@ static int ss_prepare(st_ctx *c, int n1, int n2, int ns1, int ns2, bool host_row
                     c1 = std::sqrt((1.0 + c1) / 2.0);
                 }
             }
-        }
+            return tw;
+        }();
         HIPCHK(hipMemcpyAsync(w.tw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     }
     return ST_OK;
