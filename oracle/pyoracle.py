@@ -63,6 +63,9 @@ def load(glibc=False):
     lib.orc_rng_normal.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]
     lib.orc_rng_normal.restype = D
     lib.orc_philox.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, P]
+    lib.orc_rad2_sample.argtypes = [ctypes.c_uint64, I, ctypes.c_uint64, D, P, P]
+    lib.orc_rad2_energy.argtypes = [D]
+    lib.orc_rad2_energy.restype = D
     _libs[key] = lib
     return lib
 
@@ -239,3 +242,17 @@ def fft2d_128(re, im, direction):
     y = np.ascontiguousarray(np.array(im, dtype=np.float64, copy=True).reshape(128, 128))
     load().orc_fft2d_128(_p(x), _p(y), int(direction))
     return x, y
+
+
+def rad2_sample(seed, n, draw, nr):
+    """radiate2's photon sampler for particles 0..n-1 at one draw number:
+    (photon counts, summed energies in units of the critical energy)."""
+    cnt = np.zeros(n, dtype=np.int32)
+    et = np.zeros(n)
+    load().orc_rad2_sample(int(seed), int(n), int(draw), float(nr), _p(cnt), _p(et))
+    return cnt, et
+
+
+def rad2_energy(r):
+    """Photon energy / critical energy from a uniform r (radiate2's inverse CDF)."""
+    return load().orc_rad2_energy(float(r))

@@ -627,6 +627,47 @@ static void radiate1(double x[], double L, double href, const double BLbrho[], o
     x[py_] = x0[py_] * ratio;
 }
 
+/*
+ * radiate2, simtrack.cpp:4290-4366.  gsl_ran_poisson(gsl_r1, Nr) and the
+ * gsl_rng_uniform(gsl_r2) photon draws are replaced by st_rad2_photons: a
+ * Poisson count by Knuth's product of uniforms (exact, in chunks of mean
+ * <= 30) and one uniform per photon, all from the sub-stream of one draw
+ * number, so that every particle's photons are independent of scheduling.
+ * The photon-energy table and its linear interpolation (splint, cpp:333)
+ * are the reference's.  Kept as the reference has them: yp from x[1]
+ * (cpp:4331) and rho1 from 1 + x[5] (cpp:4339).
+ */
+static void radiate2(double x[], double L, double href, const double BLbrho[], orc_rng *rng)
+{
+    double alpha = 1.0 / 137.0359997;
+    double x0[6], delta1, gamma1, beta1, delta2, pz, xp, yp, temp1, ratio;
+    double u[3], BLper[3], BLper2, rho1, Nr, Etot, Ec;
+    for (int i = 0; i < 6; i++) x0[i] = x[i];
+    delta1 = sqrt(1.0 + 2 * x[5] / GP.beta + x[5] * x[5]) - 1.0;
+    gamma1 = GP.gamma + sqrt(GP.gamma * GP.gamma - 1) * x[5];
+    beta1 = sqrt(1.0 - 1.0 / gamma1 / gamma1);
+    pz = sqrt((1 + delta1) * (1 + delta1) - x[1] * x[1] - x[3] * x[3]);
+    xp = (1 + x[0] * href) * x[1] / pz;
+    yp = (1 + x[0] * href) * x[1] / pz;
+    temp1 = sqrt((1 + x[0] * href) * (1 + x[0] * href) + xp * xp + yp * yp);
+    u[0] = xp / temp1;
+    u[1] = yp / temp1;
+    u[2] = (1 + x[0] * href) / temp1;
+    vector_cross_product(BLbrho, u, BLper);
+    BLper2 = BLper[0] * BLper[0] + BLper[1] * BLper[1] + BLper[2] * BLper[2];
+    rho1 = (1 + x[5]) / sqrt(BLper2) * L;
+    Ec = 1.5 * 6.582119e-16 * 2.99792458e8 * gamma1 * gamma1 * gamma1 / rho1 / 1.0e6;
+    Nr = 5. * alpha * gamma1 / 2. / sqrt(3.) / rho1 * L * temp1 / beta1;
+    Etot = st_rad2_photons(rng->seed, rng->particle, rng->counter, Nr);
+    rng->counter++;
+    Etot = Etot * Ec;
+    x[5] = x[5] - Etot / GP.energy;
+    delta2 = sqrt(1.0 + 2 * x[5] / GP.beta + x[5] * x[5]) - 1.0;
+    ratio = (1 + delta2) / (1 + delta1);
+    x[1] = x0[1] * ratio;
+    x[3] = x0[3] * ratio;
+}
+
 /* ------------------------------------------------------------- spin */
 
 /* spin(), simtrack.cpp:4574-4630 */
@@ -733,17 +774,19 @@ static void SBEND_Pass(double x[], double L, int Nint, double Angle, double E1, 
     edge(x, E2, href);
 }
 
-/* GSBEND_Pass, simtrack.cpp:4818-4880 (radiation path radiate2 not restated) */
-static int GSBEND_Pass(double x[], double L, int Nint, double Angle, double K0L, double E1, double E2)
+/* GSBEND_Pass, simtrack.cpp:4818-4880 */
+static void GSBEND_Pass(double x[], double L, int Nint, double Angle, double K0L, double E1, double E2, orc_rng *rng)
 {
-    double href = Angle / L, hreal = K0L / L, Lint = L / Nint;
+    double href = Angle / L, hreal = K0L / L, Lint = L / Nint, BLbrho[3];
     if (Angle == 0.) {
         DRIFT_Pass(x, L);
-        return 0;
+        return;
     }
-    if (GP.radiate) return -1;
     edge(x, E1, hreal);
     if (GP.H_expand) {
+        BLbrho[0] = 0.;
+        BLbrho[1] = K0L / Nint;
+        BLbrho[2] = 0.;
         for (int i = 0; i < Nint; i++) {
             DRIFT_Pass(x, Fdrift1 * Lint);
             general_bend_kick_pass(x, Fkick1 * Lint, href, hreal);
@@ -752,12 +795,21 @@ static int GSBEND_Pass(double x[], double L, int Nint, double Angle, double K0L,
             DRIFT_Pass(x, Fdrift2 * Lint);
             general_bend_kick_pass(x, Fkick1 * Lint, href, hreal);
             DRIFT_Pass(x, Fdrift1 * Lint);
+            if (GP.radiate) radiate2(x, Lint, href, BLbrho, rng);
         }
-    } else {
+    } else if (!GP.radiate) {
         general_sbend_exact_pass(x, L, Angle, K0L, OCOS(Angle), OSIN(Angle));
+    } else {
+        BLbrho[0] = 0.;
+        BLbrho[1] = K0L / Nint;
+        BLbrho[2] = 0.;
+        double cosAngle = OCOS(Angle / Nint), sinAngle = OSIN(Angle / Nint);
+        for (int i = 0; i < Nint; i++) {
+            general_sbend_exact_pass(x, Lint, Angle / Nint, K0L / Nint, cosAngle, sinAngle);
+            radiate2(x, Lint, href, BLbrho, rng);
+        }
     }
     edge(x, E2, hreal);
-    return 0;
 }
 
 /* QUAD_Pass, simtrack.cpp:4882-4959 */
@@ -1793,12 +1845,11 @@ static int elem_pass(const st_element *e, double x[], orc_rng *rng)
         SBEND_Pass(x, e->l, e->nint, e->angle, e->e1, e->e2, rng);
         LtoG(x, e->l, e->angle, MIS(e));
         return 0;
-    case ST_GSBEND: {
+    case ST_GSBEND:
         GtoL(x, e->l, e->angle, MIS(e));
-        int rc = GSBEND_Pass(x, e->l, e->nint, e->angle, e->k0l, e->e1, e->e2);
+        GSBEND_Pass(x, e->l, e->nint, e->angle, e->k0l, e->e1, e->e2, rng);
         LtoG(x, e->l, e->angle, MIS(e));
-        return rc;
-    }
+        return 0;
     case ST_QUAD:
         GtoL(x, e->l, 0., MIS(e));
         QUAD_Pass(x, e->l, e->nint, e->knl[1], e->knsl[1]);
@@ -2027,16 +2078,18 @@ static int track_one(const st_element *line, int n, double *x, int nturn, int *s
 }
 
 /*
- * Quantum-excitation draws per turn: radiate1 runs once per slice of every
- * bending SBEND when GP.radiate (SBEND_Pass, cpp:4781-4813); spin passes
- * radiate classically only.  Draw k of turn t has number t*D + k.
+ * Random draws per turn: radiate1 runs once per slice of every bending SBEND
+ * (SBEND_Pass, cpp:4781-4813) and radiate2 once per slice of every bending
+ * GSBEND (GSBEND_Pass, cpp:4848-4871) when GP.radiate; spin passes radiate
+ * classically only.  Draw k of turn t has number t*D + k.
  */
 EXPORT uint64_t orc_draws_per_turn(const st_element *line, int n, int spin_on)
 {
     uint64_t d = 0;
     if (!GP.radiate || spin_on) return 0;
     for (int j = 0; j < n; j++)
-        if (line[j].type == ST_SBEND && line[j].angle != 0.) d += (uint64_t)(line[j].nint > 0 ? line[j].nint : 0);
+        if ((line[j].type == ST_SBEND || line[j].type == ST_GSBEND) && line[j].angle != 0.)
+            d += (uint64_t)(line[j].nint > 0 ? line[j].nint : 0);
     return d;
 }
 
@@ -2163,6 +2216,17 @@ EXPORT double orc_rng_normal(uint64_t seed, uint32_t particle, uint64_t counter)
 {
     return st_rng_normal(seed, particle, counter);
 }
+/* the photon sampler of radiate2 for particles 0..n-1 at one draw number:
+   photon count and summed energy (units of the critical energy) */
+EXPORT void orc_rad2_sample(uint64_t seed, int n, uint64_t draw, double nr, int *count, double *etot)
+{
+    for (int i = 0; i < n; i++) {
+        uint32_t j;
+        count[i] = st_rad2_count(seed, (uint32_t)i, draw, nr, &j);
+        etot[i] = st_rad2_photons(seed, (uint32_t)i, draw, nr);
+    }
+}
+EXPORT double orc_rad2_energy(double r) { return st_rad2_energy(r); }
 EXPORT int orc_libm_kind(void)
 {
 #ifdef ORACLE_GLIBC

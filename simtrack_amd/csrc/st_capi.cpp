@@ -552,15 +552,16 @@ struct Builder {
         return 0;
     }
 
-    /* GSBEND_Pass (cpp:4818), without the radiate2 branch */
+    /* GSBEND_Pass (cpp:4818); with GP.radiate, radiate2 after every slice */
     int gsbend_op(const st_element &e)
     {
         double L = e.l, Angle = e.angle, K0L = e.k0l;
         int Nint = e.nint;
         if (Angle == 0.) return drift_op(L, &e, L, Angle);
         double href = Angle / L, hreal = K0L / L;
+        bool rad = gp.radiate != 0;
+        if ((gp.H_expand || rad) && Nint < 1) return -1;
         if (gp.H_expand) {
-            if (Nint < 1) return -1;
             st_op op = new_op(K_GSBEND_YOSHIDA);
             op.nint = Nint;
             double Lint = L / Nint;
@@ -574,6 +575,11 @@ struct Builder {
             put(Fkick2 * Lint);
             put(href);
             put(hreal);
+            if (rad) {
+                put(K0L / Nint);
+                put(Lint);
+                op.flags |= ST_F_RAD;
+            }
             misalign(op, e, L, Angle);
             t.ops.push_back(op);
             return 0;
@@ -581,7 +587,17 @@ struct Builder {
         st_op op = new_op(K_GSBEND_EXACT);
         put(st_tan(e.e1) * hreal);
         put(st_tan(e.e2) * hreal);
-        general_piece(L, Angle, K0L, st_cos(Angle), st_sin(Angle));
+        if (rad) {
+            double Lint = L / Nint, An = Angle / Nint;
+            op.nint = Nint;
+            general_piece(Lint, An, K0L / Nint, st_cos(An), st_sin(An));
+            put(href);
+            put(K0L / Nint);
+            put(Lint);
+            op.flags |= ST_F_RAD;
+        } else {
+            general_piece(L, Angle, K0L, st_cos(Angle), st_sin(Angle));
+        }
         misalign(op, e, L, Angle);
         t.ops.push_back(op);
         return 0;
@@ -721,10 +737,6 @@ struct Builder {
             return sbend_op(e);
         case ST_GSBEND:
             if (spin) return sbend_op(e); /* GSBEND::sPass calls SBEND_sPass, cpp:8619-8623 */
-            if (gp.radiate) {
-                err = "GSBEND with GP.radiate (radiate2 photon sampling) is not supported";
-                return -2;
-            }
             return gsbend_op(e);
         case ST_QUAD:
             return multipole_op(e, K_QUAD, K_QUAD_THIN, e.knl[1], e.knsl[1]);
@@ -1166,7 +1178,7 @@ extern "C" int st_table_build(st_ctx *ctx, const st_globals *gp, const st_elemen
     for (int j = 0; j < n; ++j) {
         ap[2 * j] = elems[j].apx;
         ap[2 * j + 1] = elems[j].apy;
-        if (gp->radiate && elems[j].type == ST_SBEND && elems[j].angle != 0.)
+        if (gp->radiate && (elems[j].type == ST_SBEND || elems[j].type == ST_GSBEND) && elems[j].angle != 0.)
             t->draws_per_turn += elems[j].nint > 0 ? elems[j].nint : 0;
     }
     int rc = upload(t->orbit);

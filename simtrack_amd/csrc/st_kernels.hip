@@ -380,13 +380,22 @@ ST_DEV void sbend_sliced(St &s, Ptc &c, int nint, st_pp P, st_gref g,
     edge(s, P[1]);
 }
 
-/* GSBEND_Pass without radiation, cpp:4818-4880 */
-template <bool HEXP>
-ST_DEV void gsbend(St &s, Ptc &c, int kind, int nint, st_pp P)
+/* GSBEND_Pass, cpp:4818-4880.  With RAD the exact branch runs nint slices
+   of general_sbend_exact_pass, and both branches call radiate2 after each
+   slice with BLbrho = (0, K0L/Nint, 0) */
+template <bool HEXP, bool RAD>
+ST_DEV void gsbend(St &s, Ptc &c, int kind, int nint, st_pp P, st_gref g, uint32_t pid, uint64_t &rctr)
 {
     edge(s, P[0]);
     if (kind == K_GSBEND_EXACT) {
-        general_sbend_exact(s, P + 2, c);
+        if (RAD) {
+            for (int i = 0; i < nint; i++) {
+                general_sbend_exact(s, P + 2, c);
+                radiate2(s, P[11], P[9], P[10], c, g, pid, rctr);
+            }
+        } else {
+            general_sbend_exact(s, P + 2, c);
+        }
     } else {
         double d1 = P[2], d1b = P[3], d2 = P[4], d2b = P[5], kl1 = P[6], kl2 = P[7], href = P[8], hreal = P[9];
         for (int i = 0; i < nint; i++) {
@@ -397,6 +406,7 @@ ST_DEV void gsbend(St &s, Ptc &c, int kind, int nint, st_pp P)
             drift<HEXP>(s, d2, d2b, c);
             general_bend_kick(s, kl1, href, hreal);
             drift<HEXP>(s, d1, d1b, c);
+            if (RAD) radiate2(s, P[11], href, P[10], c, g, pid, rctr);
         }
     }
     edge(s, P[1]);
@@ -486,7 +496,7 @@ ST_DEV void apply_op(St &s, Ptc &c, const st_op &op, st_pp par, st_gref g,
         break;
     case K_GSBEND_EXACT:
     case K_GSBEND_YOSHIDA:
-        gsbend<HEXP>(s, c, op.kind, op.nint, P);
+        gsbend<HEXP, RAD && !SPIN>(s, c, op.kind, op.nint, P, g, pid, rctr);
         break;
     case K_BENDMULT:
         bendmult<SPIN, HEXP, RAD>(s, c, op, P, g);

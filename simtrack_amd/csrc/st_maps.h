@@ -608,6 +608,41 @@ ST_DEV void radiate1(St &s, double L, double href, double B0, double B1, double 
     ptc_update(c, x[5], g);
 }
 
+/*
+ * radiate2, simtrack.cpp:4290-4366 (photon emission by Monte Carlo), for a
+ * field BLbrho = (0, B1, 0).  As in the reference, yp is formed from x[1]
+ * (cpp:4331) and the bending radius from 1 + pt (cpp:4339).  One draw
+ * number per call; the photons come from its sub-stream (st_rad2_photons).
+ */
+ST_DEV void radiate2(St &s, double L, double href, double B1, Ptc &c, st_gref g, uint32_t pid, uint64_t &ctr)
+{
+    double *x = s.x;
+    double px0 = x[1], py0 = x[3];
+    const double B0 = 0., B2 = 0.;
+    double pz = st_fsqrt(c.opd2 - x[1] * x[1] - x[3] * x[3]);
+    double xp = (1 + x[0] * href) * x[1] / pz;
+    double yp = (1 + x[0] * href) * x[1] / pz;
+    double temp1 = st_fsqrt((1 + x[0] * href) * (1 + x[0] * href) + xp * xp + yp * yp);
+    double u0 = xp / temp1, u1 = yp / temp1, u2 = (1 + x[0] * href) / temp1;
+    double c0 = B1 * u2 - B2 * u1;
+    double c1 = -1.0 * B0 * u2 + B2 * u0;
+    double c2 = B0 * u1 - B1 * u0;
+    double BLper2 = c0 * c0 + c1 * c1 + c2 * c2;
+    double gm = c.g1;
+    double rho1 = (1 + x[5]) / st_fsqrt(BLper2) * L;
+    double Ec = 1.5 * 6.582119e-16 * 2.99792458e8 * gm * gm * gm / rho1 / 1.0e6;
+    double Nr = 5. * (1.0 / 137.0359997) * gm / 2. / 0x1.bb67ae8584caap+0 / rho1 * L * temp1 / c.b1;
+    double Etot = st_rad2_photons(g.rng_seed, pid, ctr, Nr);
+    ctr++;
+    Etot = Etot * Ec;
+    x[5] = x[5] - Etot / g.energy;
+    double delta2 = st_fsqrt(1.0 + 2 * x[5] / g.beta + x[5] * x[5]) - 1.0;
+    double ratio = (1 + delta2) / c.opd;
+    x[1] = px0 * ratio;
+    x[3] = py0 * ratio;
+    ptc_update(c, x[5], g);
+}
+
 /* ------------------------------------------------------- beam-beam */
 
 /*

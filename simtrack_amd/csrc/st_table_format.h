@@ -35,8 +35,10 @@ enum st_kind {
     K_SBEND_SLICED,   /* 0 ef1,1 ef2,2 href,3 Lint,4 Angle/Nint,5 Nint,
                          exact: 6 hs,7 1/hs,8 cos,9 sin,10 Lint/beta
                          H_expand: 6 d1,7 d1b,8 d2,9 d2b,10 kl1,11 kl2           */
-    K_GSBEND_EXACT,   /* 0 ef1,1 ef2, 2.. general piece (7)                       */
-    K_GSBEND_YOSHIDA, /* 0 ef1,1 ef2, 2 d1,3 d1b,4 d2,5 d2b,6 kl1,7 kl2,8 href,9 hreal */
+    K_GSBEND_EXACT,   /* 0 ef1,1 ef2, 2.. general piece (7); radiating: the piece
+                         of one slice, 9 href,10 K0L/Nint,11 Lint                 */
+    K_GSBEND_YOSHIDA, /* 0 ef1,1 ef2, 2 d1,3 d1b,4 d2,5 d2b,6 kl1,7 kl2,8 href,9 hreal;
+                         radiating: 10 K0L/Nint,11 Lint                           */
     K_BENDMULT,       /* 0 ef1,1 ef2,2 href,3 Lint,4 Nint,5 Angle,6 Angle/Nint,7 mask0,
                          exact: 8..13 piece1 (L,h,1/h,cos,sin,L/beta), 14..19 piece2
                          H_expand: 8 d1,9 d1b,10 d2,11 d2b,12 kl1,13 kl2

@@ -13,6 +13,8 @@
    RF sits in the dispersion-free straights: RF at dispersive points couples
    synchrotron and betatron motion, which the uncoupled integrals omit (it
    raises the tracked emittance by ~20 % when every arc cell has a cavity).
+3. Photon emission: the same ring with radiating GSBENDs (radiate2, Poisson
+   photons from the counter-based stream) equals the oracle bit for bit.
 """
 import math
 
@@ -78,6 +80,46 @@ def test_turn_ranges_continue_the_stream(ctx, ring4):
         tab.track_device(d_x.data_ptr(), len(x), t0, nt, d_s.data_ptr(), d_l.data_ptr(), 0)
     ctx.sync()
     assert_bitwise(d_x.cpu().numpy(), one, "turn ranges 0-1, 1-3 vs one run of 3 turns")
+
+
+def _gsbend_line(ln):
+    """The ring with every SBEND a GSBEND whose field is 0.1 % above the
+    frame's bend (K0L != ANGLE), so it radiates through radiate2."""
+    cells = []
+    for e in ln.Cell:
+        if e.TYPE == "SBEND":
+            g = st.GSBEND(e.NAME, e.L, e.ANGLE, e.ANGLE * 1.001, e.E1, e.E2)
+            g.Nint = e.Nint
+            e = g
+        cells.append(e)
+    return st.Line(cells)
+
+
+@pytest.mark.parametrize("hexp", [1, 0], ids=["yoshida", "exact"])
+def test_gsbend_photon_emission_bitwise(ctx, ring4, hexp):
+    """radiate2 on the ring (400 radiating GSBENDs, Poisson photons):
+    GPU = oracle bit for bit over 2 turns, and a run split into turn ranges
+    draws the same photons."""
+    gp0, ln, x9 = ring4
+    gp = W.electron_globals_ring()
+    gp.rng_seed = 4242
+    gp.H_expand = hexp
+    x = np.ascontiguousarray(x9[:, :6])
+    rec = _gsbend_line(ln).records()
+    tab = st.Table(ctx, gp, rec)
+    xg, sg, _, _ = tab.track(x.copy(), 2)
+    xo, so, _, _ = pyoracle.Oracle(gp, rec).track(x.copy(), 2)
+    assert sg.all() and np.array_equal(sg, so)
+    assert_bitwise(xg, xo, "GSBEND ring + radiate2")
+    dev = torch.device("cuda:0")
+    d_x = torch.tensor(x, dtype=torch.float64, device=dev)
+    d_s = torch.ones(len(x), dtype=torch.int32, device=dev)
+    d_l = torch.zeros(2 * len(x), dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    for t0, nt in ((0, 1), (1, 1)):
+        tab.track_device(d_x.data_ptr(), len(x), t0, nt, d_s.data_ptr(), d_l.data_ptr(), 0)
+    ctx.sync()
+    assert_bitwise(d_x.cpu().numpy(), xg, "turn ranges 0-1, 1-2 vs one run of 2 turns")
 
 
 def _damped_ring():

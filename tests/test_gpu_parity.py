@@ -51,8 +51,7 @@ FLAG_SETS = [
 
 def _line_and_globals(flags, electron=False):
     g = W.electron_globals(18000.0, **flags) if electron else W.proton_globals(100000.0, **flags)
-    # GSBEND's radiating branch (radiate2, Poisson photon sampling) is not built
-    ln = W.all_types_line(seed=11, with_gsbend=not flags.get("radiate"))
+    ln = W.all_types_line(seed=11)
     return g, ln
 
 
