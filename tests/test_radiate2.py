@@ -40,6 +40,7 @@ def _cdf(y):
     return (head + y * _tail(y)) / (5 * math.pi / 3)
 
 
+@pytest.mark.filterwarnings("ignore::scipy.integrate.IntegrationWarning")
 def test_photon_energy_table_is_the_synchrotron_spectrum():
     """The inverse CDF that radiate2 samples: at its nodes the table (used
     for r >= 0.26) agrees with the exact spectrum to 1e-3 in probability,
