@@ -412,6 +412,18 @@ int st_one_turn_matrix(st_ctx *ctx, const st_globals *gp, const st_element *rec,
                        double h, double m[36]);
 
 /*
+ * The closed orbit after every element and every element's linear map, the
+ * inputs of Trace_A (simtrack.cpp:15480) that Cal_Orbit_Num (cpp:14947,
+ * its last pass) and Cal_ElementMap (cpp:15205-15253) compute: X[6*k..] is
+ * x_co passed through elements 0..k with every RF, crab and CCMULT voltage
+ * off; T[36*k..] (row-major) is element k's map about X of element k-1 (of
+ * the last element for k = 0) with z = 0 and x[5] = x5, RFCAV voltages off,
+ * as symmetric differences of step h.  T: n*36 doubles, X: n*6.
+ */
+int st_element_maps(st_ctx *ctx, const st_globals *gp, const st_element *rec, int n, const double x_co[6],
+                    double x5, double h, double *T, double *X);
+
+/*
  * Track_tbt_FMA (simtrack.cpp:21179-21241) with its grid and turns as
  * arguments (the reference: nturn 2048, sigma_step 0.1, sigma_max 6):
  * rows of 8 doubles (i*step, j*step, tunex1, tunex2, tuney1, tuney2,

@@ -48,6 +48,7 @@ def load(glibc=False):
     lib.orc_fine_tune.argtypes = [I, P, D, D, P]
     lib.orc_closed_orbit.argtypes = [P, I, D, P, P, ctypes.POINTER(I)]
     lib.orc_one_turn_matrix.argtypes = [P, I, P, D, P]
+    lib.orc_element_maps.argtypes = [P, I, P, D, D, P, P]
     lib.orc_fma_scan.argtypes = [P, I, D, D, D, P, D, D, I, D, D, P, I]
     lib.orc_lyapunov_scan.argtypes = [P, I, D, D, D, I, D, D, P, I]
     lib.orc_action_diff.argtypes = [P, I, D, D, D, D, D, D, D, D, I, I, I, P, P]
@@ -135,6 +136,17 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"oracle one-turn matrix failed ({rc})")
         return m
+
+    def element_maps(self, x_co, x5, h=1e-6):
+        """(T, X): every element's 6x6 map and the orbit after it (st_element_maps)."""
+        self._set()
+        n = len(self.records)
+        T, X = np.zeros((n, 6, 6)), np.zeros((n, 6))
+        xc = np.ascontiguousarray(np.asarray(x_co, dtype=np.float64)[:6])
+        rc = self.lib.orc_element_maps(_p(self.records), n, _p(xc), float(x5), float(h), _p(T), _p(X))
+        if rc:
+            raise RuntimeError(f"orc_element_maps: {rc}")
+        return T, X
 
     def fma_scan(self, deltap0, sigmax0, sigmay0, x_end, tune_low, tune_high, nturn, sigma_step, sigma_max):
         """Track_tbt_FMA (simtrack.cpp:21179-21241) one particle at a time: rows of 8."""

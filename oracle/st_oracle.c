@@ -2417,6 +2417,54 @@ EXPORT int orc_one_turn_matrix(const st_element *line0, int n, const double x_co
     return rc;
 }
 
+/* Cal_Orbit_Num's last pass (simtrack.cpp:15020-15029) and Cal_ElementMap
+   (cpp:15205-15253) as st_element_maps: orbit after every element, and every
+   element's map by symmetric differences about the previous element's orbit
+   with z = 0, x[5] = x5 */
+EXPORT int orc_element_maps(const st_element *line0, int n, const double x_co[6], double x5, double h, double *T,
+                            double *X)
+{
+    st_element *lo = (st_element *)malloc(sizeof(st_element) * (size_t)(n > 0 ? n : 1));
+    st_element *lm = (st_element *)malloc(sizeof(st_element) * (size_t)(n > 0 ? n : 1));
+    int rc = 0;
+    if (!lo || !lm) {
+        free(lo);
+        free(lm);
+        return -5;
+    }
+    memcpy(lo, line0, sizeof(st_element) * (size_t)n);
+    memcpy(lm, line0, sizeof(st_element) * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        if (lo[i].type == ST_RFCAV || lo[i].type == ST_CRABRF || lo[i].type == ST_CCMULT) lo[i].vrf = 0.;
+        if (lm[i].type == ST_RFCAV) lm[i].vrf = 0.;
+    }
+    double xo[6], xp[6], xm[6];
+    for (int j = 0; j < 6; j++) xo[j] = x_co[j];
+    for (int k = 0; k < n && !rc; k++) {
+        orc_rng rng = {GP.rng_seed, 0u, 0u};
+        rc = elem_pass(&lo[k], xo, &rng);
+        for (int j = 0; j < 6; j++) X[6 * k + j] = xo[j];
+    }
+    for (int k = 0; k < n && !rc; k++) {
+        const double *x0 = X + 6 * (k > 0 ? k - 1 : n - 1);
+        double *m = T + 36 * (size_t)k;
+        for (int a = 0; a < 6 && !rc; a++) {
+            for (int j = 0; j < 6; j++) xp[j] = xm[j] = x0[j];
+            xp[4] = xm[4] = 0.;
+            xp[5] = xm[5] = x5;
+            xp[a] += h;
+            xm[a] -= h;
+            orc_rng rp = {GP.rng_seed, (uint32_t)(2 * a), 0u}, rmi = {GP.rng_seed, (uint32_t)(2 * a + 1), 0u};
+            if ((rc = elem_pass(&lm[k], xp, &rp))) break;
+            if ((rc = elem_pass(&lm[k], xm, &rmi))) break;
+            for (int i = 0; i < 6; i++) m[6 * i + a] = (xp[i] - xm[i]) / (2 * h);
+        }
+    }
+    free(lo);
+    free(lm);
+    return rc;
+}
+
 /* Track_tbt_FMA, simtrack.cpp:21179-21241, one particle after another;
    rows as st_fma_scan.  Returns the number of rows, or a negative code. */
 EXPORT int orc_fma_scan(const st_element *line, int n, double deltap0, double sigmax0, double sigmay0,

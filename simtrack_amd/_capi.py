@@ -69,7 +69,7 @@ EXPORTS = [
     "st_batch_track", "st_batch_track_device", "st_batch_autotune", "st_mlp_create", "st_mlp_destroy",
     "st_mlp_num_params", "st_mlp_get_params", "st_mlp_set_params", "st_mlp_forward", "st_mlp_gradients",
     "st_mlp_train_step", "st_batch_track_tbt_device", "st_surrogate_features", "st_closed_orbit", "st_fma_scan",
-    "st_one_turn_matrix", "st_lyapunov_scan", "st_fine_tune_window_device", "st_action_diff",
+    "st_one_turn_matrix", "st_element_maps", "st_lyapunov_scan", "st_fine_tune_window_device", "st_action_diff",
     "st_bb_soft_gaussian_4d", "st_bb_soft_gaussian_6d", "st_bb_soft_gaussian_6d_zb", "st_bb_soft_gaussian_4d_host",
     "st_bb_soft_gaussian_6d_host", "st_bb_soft_gaussian_6d_zb_host", "st_zboundary_slices_gaussian",
     "st_bb_pic_4d", "st_bb_pic_6d", "st_bb_pic_4d_host", "st_bb_pic_6d_host",
@@ -152,6 +152,7 @@ def load(path=None):
     lib.st_surrogate_features.argtypes = [P, P, I, P, I, D, D, D, D, P]
     lib.st_closed_orbit.argtypes = [P, ctypes.POINTER(Globals), P, I, D, P, P, ctypes.POINTER(I)]
     lib.st_one_turn_matrix.argtypes = [P, ctypes.POINTER(Globals), P, I, P, D, P]
+    lib.st_element_maps.argtypes = [P, ctypes.POINTER(Globals), P, I, P, D, D, P, P]
     for f in ("st_bb_soft_gaussian_4d", "st_bb_soft_gaussian_4d_host", "st_bb_pic_4d", "st_bb_pic_4d_host"):
         getattr(lib, f).argtypes = [P, D, I, P, D, I, P, D, D, D, D]
     for f in ("st_bb_soft_gaussian_6d", "st_bb_soft_gaussian_6d_host", "st_bb_soft_gaussian_6d_zb",

@@ -1699,6 +1699,57 @@ extern "C" int st_one_turn_matrix(st_ctx *c, const st_globals *gp, const st_elem
     return ST_OK;
 }
 
+/*
+ * The closed orbit after every element and every element's linear map, as
+ * Cal_Orbit_Num's last pass (simtrack.cpp:15020-15029) and Cal_ElementMap
+ * (cpp:15205-15253) leave them for Trace_A: X[k] is x_co passed through
+ * elements 0..k with every RF, crab and CCMULT voltage off; T[k] is element
+ * k's 6x6 map about X[k-1] (X[n-1] for k = 0) with z = 0 and x[5] = x5,
+ * through the line with the RFCAV voltages off, as symmetric differences of
+ * step h of GPU element passes (the reference differentiates with TPS).
+ */
+extern "C" int st_element_maps(st_ctx *c, const st_globals *gp, const st_element *rec, int n, const double x_co[6],
+                               double x5, double h, double *T, double *X)
+{
+    if (!c || !gp || n <= 0 || !rec || !x_co || !T || !X || !(h > 0.))
+        return fail(ST_ERR_ARG, "st_element_maps: bad argument");
+    std::vector<st_element> ro(rec, rec + n), rm(rec, rec + n);
+    for (auto &e : ro)
+        if (e.type == ST_RFCAV || e.type == ST_CRABRF || e.type == ST_CCMULT) e.vrf = 0.;
+    for (auto &e : rm)
+        if (e.type == ST_RFCAV) e.vrf = 0.;
+    st_table *to = nullptr, *tm = nullptr;
+    int rc = st_table_build(c, gp, ro.data(), n, &to);
+    if (rc) return rc;
+    std::unique_ptr<st_table, int (*)(st_table *)> go(to, st_table_destroy);
+    rc = st_table_build(c, gp, rm.data(), n, &tm);
+    if (rc) return rc;
+    std::unique_ptr<st_table, int (*)(st_table *)> gm(tm, st_table_destroy);
+    double xo[6];
+    for (int j = 0; j < 6; j++) xo[j] = x_co[j];
+    for (int k = 0; k < n; ++k) {
+        rc = st_element_pass(to, k, xo, 1, 0);
+        if (rc) return rc;
+        for (int j = 0; j < 6; j++) X[6 * k + j] = xo[j];
+    }
+    double xb[12 * 6];
+    for (int k = 0; k < n; ++k) {
+        const double *x0 = X + 6 * (k > 0 ? k - 1 : n - 1);
+        for (int q = 0; q < 12; ++q) {
+            for (int j = 0; j < 6; j++) xb[q * 6 + j] = x0[j];
+            xb[q * 6 + 4] = 0.;
+            xb[q * 6 + 5] = x5;
+            xb[q * 6 + q / 2] += (q & 1) ? -h : h;
+        }
+        rc = st_element_pass(tm, k, xb, 12, 0);
+        if (rc) return rc;
+        double *m = T + 36 * (size_t)k;
+        for (int a = 0; a < 6; a++)
+            for (int i = 0; i < 6; i++) m[6 * i + a] = (xb[2 * a * 6 + i] - xb[(2 * a + 1) * 6 + i]) / (2 * h);
+    }
+    return ST_OK;
+}
+
 /* TBT of np particles on the device; the buffer stays in the context */
 static int tbt_device(st_table *t, const std::vector<double> &x, int nturn, std::vector<int> &stable,
                       double **d_tbt_out)

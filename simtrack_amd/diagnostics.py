@@ -12,6 +12,11 @@ driver names and output files.
   fixed amplitude and random phases (st_action_diff), Diff-output.dat.
 - twiss_at_start: closed orbit, one-turn matrix and uncoupled Twiss from
   GPU-tracked finite differences (what Cal_Twiss gives those drivers).
+- cal_twiss: Cal_Twiss (cpp:15540-15603) along the whole line, with the
+  reference's coupled normal-mode convention: Cal_A's normalising matrix
+  of the one-turn map (cpp:15299-15477), carried through every element's
+  GPU-differentiated map by Trace_A (cpp:15480-15538), and the mode
+  betas, alphas, phases and the coupling r, c11..c22 read off it.
 
 The reference writes its files through one fstream that it reopens per
 line without resetting its format flags: the first line prints the grid
@@ -19,6 +24,8 @@ amplitudes in the default format, every later line in scientific notation
 with 18 digits, which the writers below reproduce.
 """
 import ctypes
+import math
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -113,6 +120,132 @@ def twiss_at_start(ctx, gp, records, deltap=0.):
     return x_co, m, twiss_from_matrix(m)
 
 
+def delta_to_pt(gp, delta):
+    """DeltaToPt, simtrack.cpp:125-129."""
+    return (-2.0 / gp.beta + math.sqrt(4.0 / gp.beta / gp.beta - 4.0 * (1.0 - (1 + delta) * (1 + delta)))) / 2.
+
+
+def element_maps(ctx, gp, records, x_co, x5, h=1e-6):
+    """(T, X): every element's 6x6 map about the closed orbit entering it
+    (z = 0, x[5] = x5) and the closed orbit after it (st_element_maps)."""
+    lib = load()
+    rec = np.ascontiguousarray(records)
+    n = len(rec)
+    T, X = np.zeros((n, 6, 6)), np.zeros((n, 6))
+    xc = np.ascontiguousarray(np.asarray(x_co, dtype=np.float64)[:6])
+    check(lib.st_element_maps(ctx.h, ctypes.byref(gp), _ptr(rec), n, _ptr(xc), float(x5), float(h), _ptr(T),
+                              _ptr(X)), "st_element_maps")
+    return T, X
+
+
+def cal_a(m):
+    """Cal_A (simtrack.cpp:15299-15477): the normalising matrix A (4x4) of
+    the transverse block of a one-turn matrix, columns a1..a4 built from
+    the eigenvectors.  Mode 1 is the eigenvector pair with the larger x, px
+    content (cpp:15318-15345); each eigenvector is turned so that A[0][1] =
+    A[2][3] = 0 (cpp:15347-15381); a2 and a4 are flipped to make A[1][1],
+    A[3][3] positive (cpp:15400-15410); each pair is scaled to unit
+    symplectic product (cpp:15440-15452).  The eigenvectors come from numpy
+    instead of GSL; the conventions fix A, so the result agrees to rounding."""
+    w, v = np.linalg.eig(np.asarray(m, dtype=np.float64)[:4, :4])
+    up = [i for i in range(4) if w[i].imag > 0]
+    if len(up) != 2:
+        raise STError("one-turn map: unstable (eigenvalues off the unit circle)")
+    content = [abs(v[0, i].real) + abs(v[0, i].imag) + abs(v[1, i].real) + abs(v[1, i].imag) for i in up]
+    m1, m2 = (up[0], up[1]) if content[0] >= content[1] else (up[1], up[0])
+    a = np.zeros((4, 4))
+    for row, i, comp in ((0, m1, 0), (2, m2, 2)):
+        vec = v[:, i]
+        th = math.atan2(vec[comp].imag, vec[comp].real)
+        vec = vec * complex(math.cos(-math.pi / 2 - th), math.sin(-math.pi / 2 - th))
+        a[row] = -2 * vec.imag
+        a[row + 1] = 2 * vec.real
+    if a[1, 1] < 0:
+        a[1] = -a[1]
+    if a[3, 3] < 0:
+        a[3] = -a[3]
+    t1 = -a[0, 1] * a[1, 0] + a[0, 0] * a[1, 1] - a[0, 3] * a[1, 2] + a[0, 2] * a[1, 3]
+    t2 = -a[2, 1] * a[3, 0] + a[2, 0] * a[3, 1] - a[2, 3] * a[3, 2] + a[2, 2] * a[3, 3]
+    if not (t1 > 0 and t2 > 0):
+        raise STError("one-turn map: normal modes not normalisable")
+    a[0:2] /= math.sqrt(t1)
+    a[2:4] /= math.sqrt(t2)
+    return a.T.copy()
+
+
+def trace_a(T, a_end):
+    """Trace_A (simtrack.cpp:15480-15538): A after every element (each
+    element's map applied, then each mode turned back to A[0][1] = A[2][3] =
+    0) and the accumulated phase advances in turns.  Returns (A, mu1, mu2)."""
+    n = len(T)
+    A = np.array(a_end, dtype=np.float64)
+    out = np.zeros((n, 4, 4))
+    mu = np.zeros((n, 2))
+    m1 = m2 = 0.
+    for k in range(n):
+        G = T[k][:4, :4] @ A
+        d1 = math.atan2(G[0, 1], G[0, 0])
+        d2 = math.atan2(G[2, 3], G[2, 2])
+        s1 = math.sqrt(G[0, 0] * G[0, 0] + G[0, 1] * G[0, 1])
+        s2 = math.sqrt(G[2, 2] * G[2, 2] + G[2, 3] * G[2, 3])
+        A = np.empty((4, 4))
+        A[:, 0] = (G[:, 0] * G[0, 0] + G[:, 1] * G[0, 1]) / s1
+        A[:, 1] = (-G[:, 0] * G[0, 1] + G[:, 1] * G[0, 0]) / s1
+        A[:, 2] = (G[:, 2] * G[2, 2] + G[:, 3] * G[2, 3]) / s2
+        A[:, 3] = (-G[:, 2] * G[2, 3] + G[:, 3] * G[2, 2]) / s2
+        out[k] = A
+        m1 += d1 / 2. / math.pi
+        m2 += d2 / 2. / math.pi
+        mu[k] = (m1, m2)
+    return out, mu[:, 0], mu[:, 1]
+
+
+@dataclass
+class LineTwiss:
+    """Per-element optics after each element, the Element fields Cal_Twiss
+    fills (Beta1, Alfa1, Beta2, Alfa2, Mu1, Mu2, r, c11, c12, c21, c22, X),
+    plus A and the tunes (Line.Tune1, Line.Tune2)."""
+    beta1: np.ndarray
+    alfa1: np.ndarray
+    beta2: np.ndarray
+    alfa2: np.ndarray
+    mu1: np.ndarray
+    mu2: np.ndarray
+    r: np.ndarray
+    c: np.ndarray
+    X: np.ndarray
+    A: np.ndarray
+    tune1: float
+    tune2: float
+    M: np.ndarray
+
+
+def cal_twiss(ctx, gp, records, deltap=0.):
+    """Cal_Twiss (simtrack.cpp:15540-15603): deltap is dp/p; without
+    H_expand it enters x[5] as pt (DeltaToPt, cpp:15556).  The closed orbit,
+    one-turn matrix and element maps come from the GPU (st_closed_orbit,
+    st_one_turn_matrix, st_element_maps); the 4x4 analysis runs on the host,
+    as in the reference."""
+    x5 = deltap if gp.H_expand else delta_to_pt(gp, deltap)
+    x_co, x_end, _ = cal_orbit_num(ctx, gp, records, x5)
+    x0 = x_end.copy()
+    x0[4], x0[5] = 0., x5
+    m = one_turn_matrix(ctx, gp, records, x0)
+    T, X = element_maps(ctx, gp, records, x_co, x5)
+    A, mu1, mu2 = trace_a(T, cal_a(m))
+    beta1 = A[:, 0, 0] / A[:, 1, 1]
+    alfa1 = -A[:, 1, 0] / A[:, 1, 1]
+    beta2 = A[:, 2, 2] / A[:, 3, 3]
+    alfa2 = -A[:, 3, 2] / A[:, 3, 3]
+    a22 = A[:, 2:4, 2:4]
+    r = np.sqrt(a22[:, 0, 0] * a22[:, 1, 1] - a22[:, 0, 1] * a22[:, 1, 0])
+    p = a22[:, 0, 0] * a22[:, 1, 1]
+    inv = np.stack([np.stack([a22[:, 1, 1] / p, -a22[:, 0, 1] / p], -1),
+                    np.stack([-a22[:, 1, 0] / p, a22[:, 0, 0] / p], -1)], -2)
+    c = (A[:, 0:2, 2:4] @ inv) * r[:, None, None]
+    return LineTwiss(beta1, alfa1, beta2, alfa2, mu1, mu2, r, c, X, A, float(mu1[-1]), float(mu2[-1]), m)
+
+
 def action_diff(tab, betx, alfx, bety, alfy, nsigma, phase, emitx0, emity0, nturn, nstep, npart, seed=1234.):
     """Track_tbt_ActionDiff on the GPU: (rows, seed after).  Rows: turns,
     surviving count, mean and rms of 2Jx, mean and rms of 2Jy per step."""
@@ -165,4 +298,5 @@ def Track_tbt_Lyapunov(tab, deltap0, sigmax0, sigmay0, filename="./Lmap-output.d
 
 
 __all__ = ["cal_orbit_num", "fma_scan", "lyapunov_scan", "Track_tbt_FMA", "Track_tbt_Lyapunov", "one_turn_matrix",
-           "twiss_from_matrix", "twiss_at_start", "action_diff", "Track_tbt_ActionDiff", "STError"]
+           "twiss_from_matrix", "twiss_at_start", "action_diff", "Track_tbt_ActionDiff", "STError", "delta_to_pt",
+           "element_maps", "cal_a", "trace_a", "cal_twiss", "LineTwiss"]

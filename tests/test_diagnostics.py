@@ -162,3 +162,141 @@ def test_action_diffusion_of_a_linear_ring_is_zero():
         r = (2053.0 * (r - int(r / 65536.0) * 65536.0) + 13849.0)
         r = r - int(r / 65536.0) * 65536.0
     assert sd == r
+
+
+# ------------------------------------------------- coupled Twiss (Cal_A, Trace_A)
+
+def _m2(beta, alfa, mu):
+    c, s = math.cos(mu), math.sin(mu)
+    gam = (1 + alfa * alfa) / beta
+    return np.array([[c + alfa * s, beta * s], [-gam * s, c - alfa * s]])
+
+
+def _a2(beta, alfa):
+    return np.array([[math.sqrt(beta), 0.], [-alfa / math.sqrt(beta), 1 / math.sqrt(beta)]])
+
+
+def _rot(th):
+    """x-y rotation of (x, px, y, py): symplectic, couples the planes."""
+    c, s = math.cos(th), math.sin(th)
+    return np.array([[c, 0, s, 0], [0, c, 0, s], [-s, 0, c, 0], [0, -s, 0, c]])
+
+
+def _blk(a, b):
+    m = np.zeros((4, 4))
+    m[:2, :2], m[2:, 2:] = a, b
+    return m
+
+
+def _six(m4):
+    m = np.eye(6)
+    m[:4, :4] = m4
+    return m
+
+
+S4 = np.array([[0, 1, 0, 0], [-1, 0, 0, 0], [0, 0, 0, 1], [0, 0, -1, 0]], float)
+
+
+@pytest.mark.parametrize("th", [0., 0.3, -0.5])
+def test_cal_a_of_a_rotated_uncoupled_map(th):
+    """M = R(th) diag(M1, M2) R(th)^-1: the normal modes are the rotated
+    planes, so A = R(th) diag(A1, A2) exactly in the reference's conventions
+    (A[0][1] = A[2][3] = 0, A[1][1], A[3][3] > 0, unit symplectic pairs):
+    Beta1, Alfa1, Beta2, Alfa2 are the plane values, r = cos th and the
+    coupling matrix C = sin th I."""
+    b1, a1, q1, b2, a2, q2 = 31.7, -1.3, 0.2312, 7.9, 0.45, 0.3276
+    R = _rot(th)
+    M = R @ _blk(_m2(b1, a1, 2 * math.pi * q1), _m2(b2, a2, 2 * math.pi * q2)) @ np.linalg.inv(R)
+    A = D.cal_a(_six(M))
+    assert np.allclose(A, R @ _blk(_a2(b1, a1), _a2(b2, a2)), rtol=0, atol=1e-12)
+    assert np.allclose(A.T @ S4 @ A, S4, atol=1e-12)
+    # A diagonalises M into two rotations by the tunes
+    Rm = np.linalg.inv(A) @ M @ A
+    assert np.allclose(Rm, _blk(_m2(1., 0., 2 * math.pi * q1), _m2(1., 0., 2 * math.pi * q2)), atol=1e-12)
+
+
+def test_trace_a_through_a_coupled_period():
+    """Elements R^-1, diag(M1, M2), R: Trace_A decouples after the first,
+    turns each mode by its tune in the second and returns to the periodic A
+    after the third; the accumulated phases are the tunes, and the Cal_Twiss
+    read-off gives r = cos th, C = sin th I at the ends and r = 1, C = 0
+    inside."""
+    b1, a1, q1, b2, a2, q2, th = 12.0, 0.7, 0.181, 20.5, -2.1, 0.407, 0.35
+    R = _rot(th)
+    T = np.array([_six(np.linalg.inv(R)), _six(_blk(_m2(b1, a1, 2 * math.pi * q1), _m2(b2, a2, 2 * math.pi * q2))),
+                  _six(R)])
+    M = T[2] @ T[1] @ T[0]
+    A0 = D.cal_a(M)
+    A, mu1, mu2 = D.trace_a(T, A0)
+    assert np.allclose(A[0], _blk(_a2(b1, a1), _a2(b2, a2)), atol=1e-12)
+    assert np.allclose(A[2], A0, atol=1e-12)
+    assert abs(mu1[-1] - q1) < 1e-12 and abs(mu2[-1] - q2) < 1e-12
+    assert mu1[0] == pytest.approx(0., abs=1e-12) and mu2[0] == pytest.approx(0., abs=1e-12)
+
+
+def test_trace_a_betas_follow_courant_snyder_in_a_fodo():
+    """A thin-lens FODO period of drifts and quads: after every element
+    beta, alpha are those of beta' = (m11 beta - m12 alpha)^2 + m12^2) / beta
+    propagation, and the phase sums to the tune of cos mu = tr/2."""
+    f, L = 6.0, 4.0
+
+    def drift(l):
+        d = np.eye(2)
+        d[0, 1] = l
+        return d
+
+    def lens(k):
+        q = np.eye(2)
+        q[1, 0] = -k
+        return q
+
+    seq = [lens(1 / (2 * f)), drift(L), lens(-1 / f), drift(L), lens(1 / (2 * f))]
+    T = np.array([_six(_blk(e, e if i % 2 else np.array([[1, 0], [e[1, 0] * -1, 1]]))) for i, e in enumerate(seq)])
+    Mx, My = np.eye(2), np.eye(2)
+    for t in T:
+        Mx, My = t[:2, :2] @ Mx, t[2:4, 2:4] @ My
+    M = np.eye(6)
+    M[:2, :2], M[2:4, 2:4] = Mx, My
+    A, mu1, mu2 = D.trace_a(T, D.cal_a(M))
+    bx, ax, _, by, ay, _ = D.twiss_from_matrix(M)
+    for k, t in enumerate(T):
+        for (b, a), blk, (col, bet, alf) in (((bx, ax), t[:2, :2], (0, A[k, 0, 0] / A[k, 1, 1], -A[k, 1, 0] / A[k, 1, 1])),
+                                             ((by, ay), t[2:4, 2:4], (2, A[k, 2, 2] / A[k, 3, 3],
+                                                                      -A[k, 3, 2] / A[k, 3, 3]))):
+            g = (1 + a * a) / b
+            m11, m12, m21, m22 = blk[0, 0], blk[0, 1], blk[1, 0], blk[1, 1]
+            bn = m11 * m11 * b - 2 * m11 * m12 * a + m12 * m12 * g
+            an = -m11 * m21 * b + (m11 * m22 + m12 * m21) * a - m12 * m22 * g
+            assert bet == pytest.approx(bn, rel=1e-12) and alf == pytest.approx(an, rel=1e-9, abs=1e-12)
+        if k < len(T) - 1:
+            bx, ax = A[k, 0, 0] / A[k, 1, 1], -A[k, 1, 0] / A[k, 1, 1]
+            by, ay = A[k, 2, 2] / A[k, 3, 3], -A[k, 3, 2] / A[k, 3, 3]
+    assert mu1[-1] == pytest.approx(math.acos(np.trace(Mx) / 2) / (2 * math.pi), abs=1e-12)
+    assert mu2[-1] == pytest.approx(math.acos(np.trace(My) / 2) / (2 * math.pi), abs=1e-12)
+
+
+def test_cal_a_rejects_an_unstable_map():
+    M = np.eye(6)
+    M[0, 0], M[1, 1] = 2.0, 0.5
+    M[2:4, 2:4] = _m2(5., 0., 1.0)
+    with pytest.raises(st.STError, match="unstable"):
+        D.cal_a(M)
+
+
+def test_element_maps_compose_to_the_one_turn_map():
+    """The oracle's element maps (Cal_ElementMap) multiply to its one-turn
+    matrix, and the orbit after the last element is Cal_Orbit_Num's x_end."""
+    gp, ln = _fodo(kick=2e-5)
+    orc = pyoracle.Oracle(gp, ln.records())
+    x_co, x_end, _ = orc.closed_orbit(0.)
+    T, X = orc.element_maps(x_co, 0.)
+    assert np.array_equal(X[-1], x_end)
+    x0 = x_end.copy()
+    x0[4] = 0.
+    M = orc.one_turn_matrix(x0)
+    P = np.eye(6)
+    for t in T:
+        P = t @ P
+    # each element is expanded about its own orbit point with z = 0, the
+    # one-turn map about one orbit: equal to ~1e-8 of the matrix scale
+    assert np.abs(P[:4, :4] - M[:4, :4]).max() < 1e-7 * np.abs(M[:4, :4]).max()

@@ -85,6 +85,36 @@ def test_twiss_at_start_rhic(ctx):
     assert np.abs(m.T @ s @ m - s).max() < 1e-6
 
 
+def test_cal_twiss_rhic(ctx):
+    """Cal_Twiss along RHIC's 6,858 elements: the element maps and orbits
+    (st_element_maps) equal the oracle's bit for bit and the last orbit is
+    Cal_Orbit_Num's; the accumulated phases are MAD-X's tunes Q1 = 28.23,
+    Q2 = 29.22 with their integer parts; the mode betas after every element
+    equal the thick-lens uncoupled optics propagated element by element, and
+    r = 1, C = 0 (RHIC has no coupling)."""
+    gp, ln = W.rhic_example(RHIC)
+    rec = ln.records()
+    tw = D.cal_twiss(ctx, gp, rec)
+    x_co, x_end, _ = D.cal_orbit_num(ctx, gp, rec, 0.)
+    T, X = D.element_maps(ctx, gp, rec, x_co, 0.)
+    To, Xo = pyoracle.Oracle(gp, rec).element_maps(x_co, 0.)
+    assert_bitwise(T, To, "RHIC element maps")
+    assert_bitwise(X, Xo, "RHIC element orbits")
+    assert np.array_equal(X[-1], x_end)
+    assert abs(tw.tune1 - 28.23000005) < 1e-6 and abs(tw.tune2 - 29.22000006) < 1e-6
+    opt = O.Optics(ln)
+    tx = np.array([[opt.beta_x, -opt.alpha_x], [-opt.alpha_x, (1 + opt.alpha_x ** 2) / opt.beta_x]])
+    ty = np.array([[opt.beta_y, -opt.alpha_y], [-opt.alpha_y, (1 + opt.alpha_y ** 2) / opt.beta_y]])
+    worst = 0.
+    for k, e in enumerate(ln.Cell):
+        a, b = O.element_matrices(e)
+        tx = a[:2, :2] @ tx @ a[:2, :2].T
+        ty = b @ ty @ b.T
+        worst = max(worst, abs(tw.beta1[k] / tx[0, 0] - 1), abs(tw.beta2[k] / ty[0, 0] - 1))
+    assert worst < 1e-5, worst
+    assert np.abs(tw.r - 1).max() < 1e-9 and np.abs(tw.c).max() < 1e-6
+
+
 def test_action_diff_bitwise(ctx):
     gp, ln = W.config0()[:2]
     rec = ln.records()
