@@ -16,7 +16,9 @@ driver names and output files.
   reference's coupled normal-mode convention: Cal_A's normalising matrix
   of the one-turn map (cpp:15299-15477), carried through every element's
   GPU-differentiated map by Trace_A (cpp:15480-15538), and the mode
-  betas, alphas, phases and the coupling r, c11..c22 read off it.
+  betas, alphas, phases and the coupling r, c11..c22 read off it;
+  cal_dispersion and cal_chrom (Cal_Dispersion cpp:17049, Cal_Chrom
+  cpp:16964) on its maps and tunes.
 
 The reference writes its files through one fstream that it reopens per
 line without resetting its format flags: the first line prints the grid
@@ -204,7 +206,8 @@ def trace_a(T, a_end):
 class LineTwiss:
     """Per-element optics after each element, the Element fields Cal_Twiss
     fills (Beta1, Alfa1, Beta2, Alfa2, Mu1, Mu2, r, c11, c12, c21, c22, X),
-    plus A and the tunes (Line.Tune1, Line.Tune2)."""
+    plus A, the tunes (Line.Tune1, Line.Tune2), the one-turn matrix M (the
+    last element's M) and the element maps T (each element's T)."""
     beta1: np.ndarray
     alfa1: np.ndarray
     beta2: np.ndarray
@@ -218,6 +221,7 @@ class LineTwiss:
     tune1: float
     tune2: float
     M: np.ndarray
+    T: np.ndarray
 
 
 def cal_twiss(ctx, gp, records, deltap=0.):
@@ -243,7 +247,40 @@ def cal_twiss(ctx, gp, records, deltap=0.):
     inv = np.stack([np.stack([a22[:, 1, 1] / p, -a22[:, 0, 1] / p], -1),
                     np.stack([-a22[:, 1, 0] / p, a22[:, 0, 0] / p], -1)], -2)
     c = (A[:, 0:2, 2:4] @ inv) * r[:, None, None]
-    return LineTwiss(beta1, alfa1, beta2, alfa2, mu1, mu2, r, c, X, A, float(mu1[-1]), float(mu2[-1]), m)
+    return LineTwiss(beta1, alfa1, beta2, alfa2, mu1, mu2, r, c, X, A, float(mu1[-1]), float(mu2[-1]), m, T)
+
+
+def cal_dispersion(M, T, gp):
+    """Cal_Dispersion (simtrack.cpp:17049-17105): the periodic dispersion
+    -(M44 - I)^-1 M[0:4, 5] at the end of the line, carried through every
+    element's map (x[5] = 1), times beta without H_expand.  M, T: the
+    one-turn matrix and element maps of cal_twiss.  Returns an (n, 4) array
+    of (Etax, Etaxp, Etay, Etayp) after every element."""
+    r44 = -np.linalg.inv(np.asarray(M)[:4, :4] - np.eye(4))
+    x = r44 @ np.asarray(M)[:4, 5]
+    out = np.zeros((len(T), 4))
+    for k, t in enumerate(T):
+        x = t[:4, :4] @ x + t[:4, 5]
+        out[k] = x
+    if not gp.H_expand:
+        out *= gp.beta
+    return out
+
+
+def cal_chrom(ctx, gp, records, step_deltap=0.0001):
+    """Cal_Chrom (simtrack.cpp:16964-16992): first and second order
+    chromaticities from the tunes of cal_twiss at +-step and 0.  As in the
+    reference, without H_expand the step is converted to pt before
+    cal_twiss converts it again (cpp:16973).  Returns (Chromx1, Chromy1,
+    Chromx2, Chromy2); step_deltap is GP.step_deltap (default 1e-4,
+    simtrack.h:48)."""
+    d = step_deltap if gp.H_expand else delta_to_pt(gp, step_deltap)
+    p = cal_twiss(ctx, gp, records, d)
+    m = cal_twiss(ctx, gp, records, -d)
+    z = cal_twiss(ctx, gp, records, 0.)
+    return ((p.tune1 - m.tune1) / 2. / step_deltap, (p.tune2 - m.tune2) / 2. / step_deltap,
+            (p.tune1 + m.tune1 - 2 * z.tune1) / 2. / step_deltap / step_deltap,
+            (p.tune2 + m.tune2 - 2 * z.tune2) / 2. / step_deltap / step_deltap)
 
 
 def action_diff(tab, betx, alfx, bety, alfy, nsigma, phase, emitx0, emity0, nturn, nstep, npart, seed=1234.):
@@ -299,4 +336,4 @@ def Track_tbt_Lyapunov(tab, deltap0, sigmax0, sigmay0, filename="./Lmap-output.d
 
 __all__ = ["cal_orbit_num", "fma_scan", "lyapunov_scan", "Track_tbt_FMA", "Track_tbt_Lyapunov", "one_turn_matrix",
            "twiss_from_matrix", "twiss_at_start", "action_diff", "Track_tbt_ActionDiff", "STError", "delta_to_pt",
-           "element_maps", "cal_a", "trace_a", "cal_twiss", "LineTwiss"]
+           "element_maps", "cal_a", "trace_a", "cal_twiss", "LineTwiss", "cal_dispersion", "cal_chrom"]

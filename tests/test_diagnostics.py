@@ -300,3 +300,25 @@ def test_element_maps_compose_to_the_one_turn_map():
     # each element is expanded about its own orbit point with z = 0, the
     # one-turn map about one orbit: equal to ~1e-8 of the matrix scale
     assert np.abs(P[:4, :4] - M[:4, :4]).max() < 1e-7 * np.abs(M[:4, :4]).max()
+
+
+def test_dispersion_is_the_momentum_derivative_of_the_orbit():
+    """Cal_Dispersion on the oracle's maps equals the central difference of
+    the closed orbit after every element at dp/p = +-1e-6 (x[5] = pt, so
+    the derivative is taken in delta as the reference's beta factor does)."""
+    gp, ln = _fodo(kick=2e-5)
+    orc = pyoracle.Oracle(gp, ln.records())
+    x_co, x_end, _ = orc.closed_orbit(0.)
+    T, X = orc.element_maps(x_co, 0.)
+    x0 = x_end.copy()
+    x0[4] = 0.
+    eta = D.cal_dispersion(orc.one_turn_matrix(x0), T, gp)
+    d = 1e-6
+    orb = []
+    for s in (d, -d):
+        pt = D.delta_to_pt(gp, s)
+        xc, _, _ = orc.closed_orbit(pt)
+        orb.append(orc.element_maps(xc, pt)[1])
+    fd = (orb[0][:, :4] - orb[1][:, :4]) / (2 * d)
+    assert np.abs(eta[:, 0]).max() > 0.5
+    assert np.abs(eta - fd).max() < 1e-5 * np.abs(eta).max()

@@ -115,6 +115,22 @@ def test_cal_twiss_rhic(ctx):
     assert np.abs(tw.r - 1).max() < 1e-9 and np.abs(tw.c).max() < 1e-6
 
 
+def test_dispersion_and_chromaticity_rhic(ctx):
+    """Cal_Dispersion and Cal_Chrom on RHIC against the MAD-X table's header:
+    DXMAX = 1.831440463 and DXRMS = 0.9981399112 to 2e-5, no vertical
+    dispersion, and the linear chromaticities DQ1 = DQ2 = 1 to 1 %."""
+    gp, ln = W.rhic_example(RHIC)
+    rec = ln.records()
+    tw = D.cal_twiss(ctx, gp, rec)
+    eta = D.cal_dispersion(tw.M, tw.T, gp)
+    assert abs(eta[:, 0].max() / 1.831440463 - 1) < 2e-5
+    assert abs(np.sqrt((eta[:, 0] ** 2).mean()) / 0.9981399112 - 1) < 2e-5
+    assert np.abs(eta[:, 2]).max() < 1e-10
+    cx1, cy1, cx2, cy2 = D.cal_chrom(ctx, gp, rec)
+    assert abs(cx1 - 1) < 0.01 and abs(cy1 - 1) < 0.01
+    assert math.isfinite(cx2) and math.isfinite(cy2)
+
+
 def test_action_diff_bitwise(ctx):
     gp, ln = W.config0()[:2]
     rec = ln.records()
