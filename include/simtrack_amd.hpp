@@ -17,6 +17,7 @@
  *   (both forms), Zboundary_slices_Gaussian             simtrack.cpp:3157
  *   BeamBeam4D_PIC, BeamBeam6D_PIC                     simtrack.cpp:23286, 23380
  *   Cal_OneTurnMap (matrix only), Track_tbt_ActionDiff simtrack.cpp:15122, 21309
+ *   Cal_Twiss, Cal_Dispersion, Cal_Chrom, Print_Twiss   simtrack.cpp:15540, 17049, 16964, 19846
  *
  * Element classes: DRIFT, MARKER, BPM, HBPM, VBPM, SBEND, GSBEND, QUAD,
  * SKEWQ, SEXT, OCT, MULT, GMULT, SBENDMULT, SMULT, GSBENDMULT, KICKER,
@@ -41,10 +42,12 @@
 #define SIMTRACK_AMD_HPP
 
 #include <cmath>
+#include <complex>
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iomanip>
 #include <memory>
 #include <sstream>
 #include <stdexcept>
@@ -134,7 +137,12 @@ public:
     double APx = 1., APy = 1.; /* Element::Element, simtrack.cpp:8394-8395 */
     double X[6] = {0., 0., 0., 0., 0., 0.}; /* closed orbit after the element (Cal_Orbit_Num) */
     double M[36] = {};                       /* one-turn matrix, last element only (Cal_OneTurnMap) */
-    double Beta1 = 0., Alfa1 = 0., Beta2 = 0., Alfa2 = 0.; /* Twiss at the end (Cal_Twiss_End) */
+    double Beta1 = 0., Alfa1 = 0., Beta2 = 0., Alfa2 = 0.; /* Twiss after the element (Cal_Twiss) */
+    double Mu1 = 0., Mu2 = 0.;                               /* phase advance in turns (Trace_A)  */
+    double r = 0., c11 = 0., c12 = 0., c21 = 0., c22 = 0.;   /* coupling (Cal_Twiss)              */
+    double Etax = 0., Etaxp = 0., Etay = 0., Etayp = 0.;     /* dispersion (Cal_Dispersion)       */
+    double T[36] = {};                                       /* element map (Cal_ElementMap)      */
+    double A[36] = {};                                       /* normalising matrix (Trace_A)      */
 
 protected:
     struct Param {
@@ -601,6 +609,8 @@ public:
    (Delete/Replace/Empty delete them, simtrack.cpp:12381-12431) */
 class Line {
 public:
+    double Tune1 = 0., Tune2 = 0.;                                 /* Trace_A                  */
+    double Chromx1 = 0., Chromy1 = 0., Chromx2 = 0., Chromy2 = 0.; /* Cal_Chrom                */
     Line() = default;
     Line(const Line &) = delete;
     Line &operator=(const Line &) = delete;
@@ -1304,6 +1314,333 @@ inline void Cal_Twiss_End(Line &l)
         (p == 0 ? e->Beta1 : e->Beta2) = b / sinmu;
         (p == 0 ? e->Alfa1 : e->Alfa2) = (a - d) / (2 * sinmu);
     }
+}
+
+/* GP.step_deltap (simtrack.h:48): the momentum step of Cal_Chrom */
+inline double step_deltap = 0.0001;
+
+/* DeltaToPt, simtrack.cpp:125-129 */
+inline double DeltaToPt(double delta)
+{
+    return (-2.0 / GP.beta + std::sqrt(4.0 / GP.beta / GP.beta - 4.0 * (1.0 - (1 + delta) * (1 + delta)))) / 2.;
+}
+
+namespace detail {
+/* in-place inverse of a row-major 4x4 by Gauss-Jordan elimination with
+   partial pivoting; 0 if singular */
+inline int mat_inv4(double *a)
+{
+    double b[4][8];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 8; ++j) b[i][j] = j < 4 ? a[4 * i + j] : (j - 4 == i ? 1. : 0.);
+    for (int k = 0; k < 4; ++k) {
+        int p = k;
+        for (int i = k + 1; i < 4; ++i)
+            if (std::fabs(b[i][k]) > std::fabs(b[p][k])) p = i;
+        if (b[p][k] == 0.) return 0;
+        for (int j = 0; j < 8; ++j) std::swap(b[k][j], b[p][j]);
+        const double d = b[k][k];
+        for (int j = 0; j < 8; ++j) b[k][j] /= d;
+        for (int i = 0; i < 4; ++i)
+            if (i != k) {
+                const double f = b[i][k];
+                for (int j = 0; j < 8; ++j) b[i][j] -= f * b[k][j];
+            }
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) a[4 * i + j] = b[i][j + 4];
+    return 1;
+}
+
+/* determinant of a row-major 4x4 (overwritten) by elimination with
+   partial pivoting */
+inline double det4(double *a)
+{
+    double d = 1.;
+    for (int k = 0; k < 4; ++k) {
+        int p = k;
+        for (int i = k + 1; i < 4; ++i)
+            if (std::fabs(a[4 * i + k]) > std::fabs(a[4 * p + k])) p = i;
+        if (a[4 * p + k] == 0.) return 0.;
+        if (p != k) {
+            for (int j = 0; j < 4; ++j) std::swap(a[4 * k + j], a[4 * p + j]);
+            d = -d;
+        }
+        d *= a[4 * k + k];
+        for (int i = k + 1; i < 4; ++i) {
+            const double f = a[4 * i + k] / a[4 * k + k];
+            for (int j = k; j < 4; ++j) a[4 * i + j] -= f * a[4 * k + j];
+        }
+    }
+    return d;
+}
+
+/* the null vector of the complex 4x4 matrix b (rank 3) by Gaussian
+   elimination with full pivoting, scaled to unit Euclidean norm */
+inline void null_vector4(std::complex<double> b[4][4], std::complex<double> v[4])
+{
+    int col[4] = {0, 1, 2, 3};
+    for (int k = 0; k < 3; ++k) {
+        int pi = k, pj = k;
+        for (int i = k; i < 4; ++i)
+            for (int j = k; j < 4; ++j)
+                if (std::abs(b[i][j]) > std::abs(b[pi][pj])) {
+                    pi = i;
+                    pj = j;
+                }
+        for (int j = 0; j < 4; ++j) std::swap(b[k][j], b[pi][j]);
+        for (int i = 0; i < 4; ++i) std::swap(b[i][k], b[i][pj]);
+        std::swap(col[k], col[pj]);
+        if (std::abs(b[k][k]) == 0.) throw Error("Cal_Twiss: degenerate one-turn map");
+        for (int i = k + 1; i < 4; ++i) {
+            std::complex<double> f = b[i][k] / b[k][k];
+            for (int j = k; j < 4; ++j) b[i][j] -= f * b[k][j];
+        }
+    }
+    std::complex<double> y[4];
+    y[3] = 1.;
+    for (int k = 2; k >= 0; --k) {
+        std::complex<double> acc = 0.;
+        for (int j = k + 1; j < 4; ++j) acc += b[k][j] * y[j];
+        y[k] = -acc / b[k][k];
+    }
+    double nrm = 0.;
+    for (int k = 0; k < 4; ++k) nrm += std::norm(y[k]);
+    nrm = std::sqrt(nrm);
+    for (int k = 0; k < 4; ++k) v[col[k]] = y[k] / nrm;
+}
+
+/* Cal_A (simtrack.cpp:15299-15477) on the transverse block of a 6x6
+   one-turn matrix (row-major): A[i][j].  The eigenvalues of a symplectic
+   4x4 come as lambda, 1/lambda: u = lambda + 1/lambda solves
+   u^2 - tr(M) u + (c2 - 2) = 0, c2 = (tr(M)^2 - tr(M^2)) / 2, and a stable
+   mode has |u| < 2, lambda ~ u/2 + i sqrt(1 - u^2/4).  The eigenvectors are
+   null vectors of M - lambda I.  Then the reference's conventions: mode 1
+   is the pair with the larger x, px content, each vector is turned so that
+   A[0][1] = A[2][3] = 0, the second vector of a mode is flipped to make
+   A[1][1], A[3][3] positive, and each pair is scaled to unit symplectic
+   product. */
+inline void cal_a_4d(const double *m66, double A[4][4])
+{
+    typedef std::complex<double> cd;
+    double m[4][4];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) m[i][j] = m66[6 * i + j];
+    double tr = m[0][0] + m[1][1] + m[2][2] + m[3][3], tr2 = 0.;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) tr2 += m[i][j] * m[j][i];
+    const double c2 = (tr * tr - tr2) / 2, disc = tr * tr - 4 * (c2 - 2);
+    if (!(disc >= 0.)) throw Error("Cal_Twiss: one-turn map unstable");
+    const double us[2] = {(tr + std::sqrt(disc)) / 2, (tr - std::sqrt(disc)) / 2};
+    /* a one-turn matrix from finite differences is symplectic only to its
+       accuracy, so the eigenvalues sit slightly off the unit circle: the
+       estimates are refined by Newton's method on the characteristic
+       polynomial of M itself (Faddeev-LeVerrier coefficients), as an
+       eigen-solver of the given matrix would find them */
+    double m2[4][4], m3[4][4], t3 = 0.;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            m2[i][j] = 0.;
+            for (int q = 0; q < 4; ++q) m2[i][j] += m[i][q] * m[q][j];
+        }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            m3[i][j] = 0.;
+            for (int q = 0; q < 4; ++q) m3[i][j] += m2[i][q] * m[q][j];
+        }
+    for (int i = 0; i < 4; ++i) t3 += m3[i][i];
+    double lu[16];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) lu[4 * i + j] = m[i][j];
+    const double k1 = tr, k2 = c2, k3 = (tr * tr * tr - 3 * tr * tr2 + 2 * t3) / 6, k4 = det4(lu);
+    cd vec[2][4];
+    double content[2];
+    for (int q = 0; q < 2; ++q) {
+        if (!(std::fabs(us[q]) < 2.)) throw Error("Cal_Twiss: one-turn map unstable");
+        cd lam(us[q] / 2, std::sqrt(1 - us[q] * us[q] / 4));
+        for (int it = 0; it < 8; ++it) {
+            const cd pl = (((lam - k1) * lam + k2) * lam - k3) * lam + k4;
+            const cd dp = ((4. * lam - 3. * k1) * lam + 2. * k2) * lam - k3;
+            if (std::abs(dp) == 0.) break;
+            const cd step = pl / dp;
+            lam -= step;
+            if (std::abs(step) < 1e-16) break;
+        }
+        cd b[4][4];
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) b[i][j] = cd(m[i][j], 0.) - (i == j ? lam : cd(0., 0.));
+        null_vector4(b, vec[q]);
+        content[q] = std::fabs(vec[q][0].real()) + std::fabs(vec[q][0].imag()) + std::fabs(vec[q][1].real()) +
+                     std::fabs(vec[q][1].imag());
+    }
+    const int mode[2] = {content[0] >= content[1] ? 0 : 1, content[0] >= content[1] ? 1 : 0};
+    double a[4][4];
+    for (int p = 0; p < 2; ++p) {
+        cd *v = vec[mode[p]];
+        const int comp = 2 * p;
+        const double th = std::atan2(v[comp].imag(), v[comp].real());
+        const cd rot(std::cos(-M_PI / 2 - th), std::sin(-M_PI / 2 - th));
+        for (int i = 0; i < 4; ++i) {
+            cd w = v[i] * rot;
+            a[2 * p][i] = -2 * w.imag();
+            a[2 * p + 1][i] = 2 * w.real();
+        }
+    }
+    if (a[1][1] < 0.)
+        for (int i = 0; i < 4; ++i) a[1][i] = -a[1][i];
+    if (a[3][3] < 0.)
+        for (int i = 0; i < 4; ++i) a[3][i] = -a[3][i];
+    const double t1 = -a[0][1] * a[1][0] + a[0][0] * a[1][1] - a[0][3] * a[1][2] + a[0][2] * a[1][3];
+    const double t2 = -a[2][1] * a[3][0] + a[2][0] * a[3][1] - a[2][3] * a[3][2] + a[2][2] * a[3][3];
+    if (!(t1 > 0. && t2 > 0.)) throw Error("Cal_Twiss: normal modes not normalisable");
+    for (int i = 0; i < 4; ++i) {
+        a[0][i] /= std::sqrt(t1);
+        a[1][i] /= std::sqrt(t1);
+        a[2][i] /= std::sqrt(t2);
+        a[3][i] /= std::sqrt(t2);
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) A[i][j] = a[j][i];
+}
+} // namespace detail
+
+/* Cal_Twiss (simtrack.cpp:15540-15603): the closed orbit and element maps
+   (Cal_Orbit_Num, Cal_ElementMap: every element's X and T from
+   st_element_maps), the one-turn matrix (Cal_OneTurnMap, the last
+   element's M), Cal_A and Trace_A (cpp:15480-15538), then every element's
+   Beta1, Alfa1, Beta2, Alfa2, Mu1, Mu2, r, c11..c22 and the Line's Tune1,
+   Tune2.  deltap is dp/p; without H_expand it enters x[5] as pt. */
+inline void Cal_Twiss(Line &l, double deltap)
+{
+    if (l.Cell.empty()) throw Error("Cal_Twiss: empty line");
+    const double x5 = GP.H_expand ? deltap : DeltaToPt(deltap);
+    std::vector<st_element> rec = records_from_line(l);
+    const int n = (int)rec.size();
+    st_ctx *c = default_context();
+    double xc[6], xe[6];
+    int it = 0;
+    check(st_closed_orbit(c, &GP, rec.data(), n, x5, xc, xe, &it), "Cal_Twiss");
+    std::vector<double> T((size_t)n * 36), X((size_t)n * 6);
+    check(st_element_maps(c, &GP, rec.data(), n, xc, x5, 1e-6, T.data(), X.data()), "Cal_Twiss");
+    for (int k = 0; k < n; ++k) {
+        for (int j = 0; j < 6; ++j) l.Cell[k]->X[j] = X[6 * (size_t)k + j];
+        for (int j = 0; j < 36; ++j) l.Cell[k]->T[j] = T[36 * (size_t)k + j];
+    }
+    Element *last = l.Cell.back();
+    double x0[6];
+    for (int k = 0; k < 6; ++k) x0[k] = last->X[k];
+    x0[4] = 0.;
+    x0[5] = x5;
+    check(st_one_turn_matrix(c, &GP, rec.data(), n, x0, 1e-6, last->M), "Cal_Twiss");
+    double A[4][4], G[4][4];
+    detail::cal_a_4d(last->M, A);
+    double mu1 = 0., mu2 = 0.;
+    for (int k = 0; k < n; ++k) {
+        Element *e = l.Cell[k];
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) {
+                double acc = 0.;
+                for (int q = 0; q < 4; ++q) acc += e->T[6 * i + q] * A[q][j];
+                G[i][j] = acc;
+            }
+        const double d1 = std::atan2(G[0][1], G[0][0]), d2 = std::atan2(G[2][3], G[2][2]);
+        const double s1 = std::sqrt(G[0][0] * G[0][0] + G[0][1] * G[0][1]);
+        const double s2 = std::sqrt(G[2][2] * G[2][2] + G[2][3] * G[2][3]);
+        for (int i = 0; i < 4; ++i) {
+            A[i][0] = (G[i][0] * G[0][0] + G[i][1] * G[0][1]) / s1;
+            A[i][1] = (-G[i][0] * G[0][1] + G[i][1] * G[0][0]) / s1;
+            A[i][2] = (G[i][2] * G[2][2] + G[i][3] * G[2][3]) / s2;
+            A[i][3] = (-G[i][2] * G[2][3] + G[i][3] * G[2][2]) / s2;
+        }
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) e->A[6 * i + j] = (i < 4 && j < 4) ? A[i][j] : 0.;
+        mu1 = mu1 + d1 / 2. / M_PI;
+        mu2 = mu2 + d2 / 2. / M_PI;
+        e->Mu1 = mu1;
+        e->Mu2 = mu2;
+        e->Beta1 = A[0][0] / A[1][1];
+        e->Alfa1 = -A[1][0] / A[1][1];
+        e->Beta2 = A[2][2] / A[3][3];
+        e->Alfa2 = -A[3][2] / A[3][3];
+        const double p = A[2][2] * A[3][3];
+        e->r = std::sqrt(A[2][2] * A[3][3] - A[2][3] * A[3][2]);
+        const double inv[2][2] = {{A[3][3] / p, -A[2][3] / p}, {-A[3][2] / p, A[2][2] / p}};
+        double cm[2][2];
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) cm[i][j] = (A[i][2] * inv[0][j] + A[i][3] * inv[1][j]) * e->r;
+        e->c11 = cm[0][0];
+        e->c12 = cm[0][1];
+        e->c21 = cm[1][0];
+        e->c22 = cm[1][1];
+    }
+    l.Tune1 = mu1;
+    l.Tune2 = mu2;
+}
+
+/* Cal_Dispersion (simtrack.cpp:17049-17105) on the maps of the last
+   Cal_Twiss: the periodic dispersion -(M44 - I)^-1 M[0:4][5] carried
+   through every element's T, times beta without H_expand. */
+inline void Cal_Dispersion(Line &l)
+{
+    if (l.Cell.empty()) throw Error("Cal_Dispersion: empty line");
+    const double *m = l.Cell.back()->M;
+    double r44[16];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) r44[4 * i + j] = m[6 * i + j] - (i == j ? 1.0 : 0.);
+    if (detail::mat_inv4(r44) == 0) throw Error("Cal_Dispersion: singular matrix");
+    double x[4], t[4];
+    for (int i = 0; i < 4; ++i) {
+        x[i] = 0.;
+        for (int j = 0; j < 4; ++j) x[i] += -r44[4 * i + j] * m[6 * j + 5];
+    }
+    const double scale = GP.H_expand ? 1.0 : GP.beta;
+    for (Element *e : l.Cell) {
+        for (int i = 0; i < 4; ++i) {
+            t[i] = e->T[6 * i + 5];
+            for (int j = 0; j < 4; ++j) t[i] += e->T[6 * i + j] * x[j];
+        }
+        for (int i = 0; i < 4; ++i) x[i] = t[i];
+        e->Etax = x[0] * scale;
+        e->Etaxp = x[1] * scale;
+        e->Etay = x[2] * scale;
+        e->Etayp = x[3] * scale;
+    }
+}
+
+/* Cal_Chrom (simtrack.cpp:16964-16992): chromaticities from the tunes of
+   Cal_Twiss at +-step_deltap and 0 (converted to pt first without H_expand,
+   as the reference does before Cal_Twiss converts again); the optics of
+   deltap = 0 stay in the line. */
+inline void Cal_Chrom(Line &l)
+{
+    const double d = GP.H_expand ? step_deltap : DeltaToPt(step_deltap);
+    Cal_Twiss(l, d);
+    const double qxp = l.Tune1, qyp = l.Tune2;
+    Cal_Twiss(l, -d);
+    const double qxm = l.Tune1, qym = l.Tune2;
+    Cal_Twiss(l, 0.);
+    const double qx0 = l.Tune1, qy0 = l.Tune2;
+    l.Chromx1 = (qxp - qxm) / 2. / step_deltap;
+    l.Chromy1 = (qyp - qym) / 2. / step_deltap;
+    l.Chromx2 = (qxp + qxm - 2 * qx0) / 2. / step_deltap / step_deltap;
+    l.Chromy2 = (qyp + qym - 2 * qy0) / 2. / step_deltap / step_deltap;
+}
+
+/* Print_Twiss (simtrack.cpp:19846-19862): one line per element, fields of
+   width 15 in the stream's default format */
+inline void Print_Twiss(Line &l, const char *filename)
+{
+    std::ofstream f(filename);
+    if (!f) throw Error(std::string("cannot open ") + filename);
+    const char *h[] = {"NAME", "TYPE", "S", "BETA1", "ALFA1", "MU1", "BETA2", "ALFA2", "MU2",
+                       "ETAX", "ETAXP", "ETAY", "ETAYP"};
+    for (const char *c : h) f << std::setw(15) << c;
+    f << "\n";
+    for (const Element *e : l.Cell)
+        f << std::setw(15) << e->NAME << std::setw(15) << e->TYPE << std::setw(15) << e->S << std::setw(15) << e->Beta1
+          << std::setw(15) << e->Alfa1 << std::setw(15) << e->Mu1 << std::setw(15) << e->Beta2 << std::setw(15)
+          << e->Alfa2 << std::setw(15) << e->Mu2 << std::setw(15) << e->Etax << std::setw(15) << e->Etaxp
+          << std::setw(15) << e->Etay << std::setw(15) << e->Etayp << "\n";
 }
 
 /* Track_tbt_ActionDiff (simtrack.cpp:21309-21383): npart particles at

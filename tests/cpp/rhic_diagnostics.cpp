@@ -5,7 +5,10 @@
 // (4 sigma, phase 0.7, emittances 1e-8 m, 64 turns x 2 steps, 16
 // particles; ./Diff-output.dat in the working directory) and writes to
 // stdout, as raw doubles: the last element's X[6], M[36], Beta1, Alfa1,
-// Beta2, Alfa2, then the seed after the scan.
+// Beta2, Alfa2, then the seed after the scan.  Then Cal_Twiss,
+// Cal_Dispersion and Cal_Chrom, ./twiss0.dat by Print_Twiss, and to stdout
+// Tune1, Tune2, Chromx1, Chromy1, Chromx2, Chromy2 and per element (n of
+// each) Beta1, Alfa1, Beta2, Alfa2, Mu1, Mu2, r, Etax.
 #include <cstdio>
 
 #include "simtrack_amd.hpp"
@@ -34,6 +37,16 @@ int main(int argc, char **argv)
         std::fwrite(last->M, sizeof(double), 36, stdout);
         const double tw[5] = {last->Beta1, last->Alfa1, last->Beta2, last->Alfa2, seed};
         std::fwrite(tw, sizeof(double), 5, stdout);
+        Cal_Chrom(l);
+        Cal_Dispersion(l);
+        Print_Twiss(l, "./twiss0.dat");
+        const double q[6] = {l.Tune1, l.Tune2, l.Chromx1, l.Chromy1, l.Chromx2, l.Chromy2};
+        std::fwrite(q, sizeof(double), 6, stdout);
+        for (int f = 0; f < 8; ++f)
+            for (const Element *e : l.Cell) {
+                const double v[8] = {e->Beta1, e->Alfa1, e->Beta2, e->Alfa2, e->Mu1, e->Mu2, e->r, e->Etax};
+                std::fwrite(&v[f], sizeof(double), 1, stdout);
+            }
     } catch (const Error &e) {
         std::fprintf(stderr, "%s\n", e.what());
         return 1;

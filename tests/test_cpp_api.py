@@ -96,3 +96,33 @@ def test_cpp_lattice_readers_match_python(tmp_path, fmt, fixture):
     for i in range(len(ref)):
         assert got[i].tobytes() == ref[i].tobytes(), (i, ln.Cell[i].NAME, ln.Cell[i].TYPE)
     assert circ == ln.circumference
+
+
+def test_cpp_normal_modes_match_python(tmp_path):
+    """The C++ API's Cal_A (its own 4x4 eigenvectors) against the Python
+    path's (numpy's) on coupled one-turn matrices R(th) diag(M1, M2) R^-1
+    and products of them: the same A to rounding, in particular the same
+    mode order, rotation and signs."""
+    from test_diagnostics import _blk, _m2, _rot, _six
+
+    from simtrack_amd import diagnostics as D
+    lib = st.build.library_path()
+    if not os.path.exists(lib):
+        pytest.skip("library not built")
+    exe = tmp_path / "normal_modes"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "normal_modes.cpp"), "-o", str(exe),
+                    "-L", os.path.dirname(lib), "-lsimtrack_amd", "-Wl,-rpath," + os.path.dirname(lib)], check=True)
+    rng = np.random.default_rng(8)
+    mats = []
+    for _ in range(20):
+        th = rng.uniform(-0.7, 0.7)
+        R = _rot(th)
+        m = R @ _blk(_m2(rng.uniform(1, 50), rng.uniform(-2, 2), 2 * math.pi * rng.uniform(0.05, 0.45)),
+                     _m2(rng.uniform(1, 50), rng.uniform(-2, 2), 2 * math.pi * rng.uniform(0.55, 0.95))) @ R.T
+        mats.append(_six(m))
+    r = subprocess.run([str(exe)], input=np.ascontiguousarray(mats).tobytes(), capture_output=True, check=True)
+    got = np.frombuffer(r.stdout, dtype=np.float64).reshape(-1, 4, 4)
+    assert len(got) == len(mats)
+    for g, m in zip(got, mats):
+        assert np.abs(g - D.cal_a(m)).max() < 1e-11

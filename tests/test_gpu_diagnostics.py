@@ -145,9 +145,11 @@ def test_action_diff_bitwise(ctx):
 
 
 def test_cpp_action_diff_matches_python(ctx, tmp_path):
-    """Cal_Orbit_Num, Cal_OneTurnMap and Track_tbt_ActionDiff of the C++ API
+    """Cal_Orbit_Num, Cal_OneTurnMap, Track_tbt_ActionDiff, Cal_Twiss,
+    Cal_Dispersion, Cal_Chrom and Print_Twiss of the C++ API
     (tests/cpp/rhic_diagnostics.cpp) on RHIC: the one-turn matrix, the Twiss,
-    the seed and Diff-output.dat equal the Python path's."""
+    the seed, Diff-output.dat and the optics along the line equal the Python
+    path's."""
     import gzip
     import subprocess
     exe = os.path.join(os.path.dirname(RHIC), "..", "..", "cpp", "build", "rhic_diagnostics")
@@ -158,10 +160,12 @@ def test_cpp_action_diff_matches_python(ctx, tmp_path):
     r = subprocess.run([exe, str(plain)], cwd=tmp_path, capture_output=True, timeout=300)
     assert r.returncode == 0, r.stderr.decode()
     v = np.frombuffer(r.stdout, dtype=np.float64)
-    assert v.size == 6 + 36 + 5
-    x_last, m_cpp, tw, seed_cpp = v[:6], v[6:42].reshape(6, 6), v[42:46], v[46]
     gp, ln = W.rhic_example(RHIC)
     rec = ln.records()
+    n = len(rec)
+    assert v.size == 6 + 36 + 5 + 6 + 8 * n
+    x_last, m_cpp, tw, seed_cpp = v[:6], v[6:42].reshape(6, 6), v[42:46], v[46]
+    q, per = v[47:53], v[53:].reshape(8, n)
     _, x_end, _ = D.cal_orbit_num(ctx, gp, rec, 0.)
     assert np.abs(x_last - x_end).max() < 1e-15
     x0 = x_last.copy()
@@ -175,3 +179,16 @@ def test_cpp_action_diff_matches_python(ctx, tmp_path):
     assert sd == seed_cpp
     assert (tmp_path / "Diff-output.dat").read_text() == out.read_text()
     assert rows[:, 1].min() == 16
+    # Cal_Twiss / Cal_Dispersion / Cal_Chrom of the C++ API (its own 4x4
+    # eigenvectors) against the Python path (numpy's): equal to rounding
+    pt = D.cal_twiss(ctx, gp, rec)
+    eta = D.cal_dispersion(pt.M, pt.T, gp)
+    ref = np.array([pt.beta1, pt.alfa1, pt.beta2, pt.alfa2, pt.mu1, pt.mu2, pt.r, eta[:, 0]])
+    for f in range(8):
+        assert np.abs(per[f] - ref[f]).max() < 1e-8 * max(1., np.abs(ref[f]).max()), f
+    assert abs(q[0] - pt.tune1) < 1e-10 and abs(q[1] - pt.tune2) < 1e-10
+    chrom = D.cal_chrom(ctx, gp, rec)
+    assert np.allclose(q[2:], chrom, rtol=1e-6, atol=1e-6)
+    lines = (tmp_path / "twiss0.dat").read_text().splitlines()
+    assert len(lines) == n + 1 and lines[0].split()[:4] == ["NAME", "TYPE", "S", "BETA1"]
+    assert float(lines[-1].split()[3]) == pytest.approx(pt.beta1[-1], rel=1e-5)
