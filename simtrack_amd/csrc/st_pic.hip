@@ -186,23 +186,55 @@ __global__ void k_pic_gather_w(const int *val, int n, const double *w, double *w
     if (q < n) ws[q] = w[val[q]];
 }
 
-__global__ void k_pic_cell_sum(const int *key, int n, const double *ws, double *src)
+/* lane j's double, read by every lane (two v_readlane) */
+ST_DEV double pic_readlane(double v, int j)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, j);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+#define PIC_CS_AHEAD 8 /* blocks of 64 weights in flight per wavefront */
+
+/*
+ * One wavefront per cell.  The lanes load the cell's weights 64 at a time,
+ * coalesced and PIC_CS_AHEAD blocks ahead of the additions; each block is
+ * then added to one running sum in row order, every weight broadcast from
+ * its lane, so the sum is the sequential one of the deposition loop.
+ */
+__global__ __launch_bounds__(256) void k_pic_cell_sum(const int *key, int n, const double *ws, double *src)
+{
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (c >= PIC_NCELL) return;
     const int lo = pic_lower_bound(key, n, c), hi = pic_lower_bound(key, n, c + 1);
-    double t = 0;
-    int q = lo;
-    /* sixteen loads in flight, then the additions in row order */
-    for (; q + 16 <= hi; q += 16) {
-        double v[16];
+    const int nb = (hi - lo + 63) >> 6;
+    double buf[PIC_CS_AHEAD];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = ws[q + u];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) t = t + v[u];
+    for (int d = 0; d < PIC_CS_AHEAD; ++d) {
+        const int q = lo + d * 64 + lane;
+        buf[d] = q < hi ? ws[q] : 0.;
     }
-    for (; q < hi; ++q) t = t + ws[q];
-    src[c] = t;
+    double t = 0;
+    for (int b = 0; b < nb; b += PIC_CS_AHEAD) {
+#pragma unroll
+        for (int d = 0; d < PIC_CS_AHEAD; ++d) {
+            if (b + d < nb) {
+                const double v = buf[d];
+                const int q = lo + (b + d + PIC_CS_AHEAD) * 64 + lane;
+                buf[d] = q < hi ? ws[q] : 0.;
+                const int m = min(64, hi - lo - (b + d) * 64);
+                if (m == 64) {
+#pragma unroll
+                    for (int u = 0; u < 64; ++u) t = t + pic_readlane(v, u);
+                } else {
+                    for (int u = 0; u < m; ++u) t = t + pic_readlane(v, u);
+                }
+            }
+        }
+    }
+    if (lane == 0) src[c] = t;
 }
 
 /* int_green and calc_egreen, simtrack.cpp:2808-2879 */
@@ -416,7 +448,7 @@ hipError_t st_pic_solve_and_kick(const pic_args &a, int s, hipStream_t stream)
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_pic_gather_w, dim3((n + 255) / 256), dim3(256), 0, stream, a.val_out, n, a.w, a.ws);
     }
-    hipLaunchKernelGGL(k_pic_cell_sum, dim3(PIC_NCELL / 64), dim3(64), 0, stream, a.key_out, n, a.ws, a.src);
+    hipLaunchKernelGGL(k_pic_cell_sum, dim3(PIC_NCELL / 4), dim3(256), 0, stream, a.key_out, n, a.ws, a.src);
     const int g2 = PIC_N2 * PIC_N2 / 256;
     hipLaunchKernelGGL(k_pic_green, dim3(g2), dim3(256), 0, stream, a.params, s, a.gr, a.gi);
     hipError_t e = pic_fft2d(a.gr, a.gi, a.tw_fwd, 1, stream);
