@@ -128,6 +128,8 @@ extern "C" void st_globals_set_energy(st_globals *gp, double gamma, double E0, d
 
 /* ------------------------------------------------------- structures */
 
+#define SS_LANES 8 /* slice pairs in flight at once */
+
 struct st_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -153,6 +155,11 @@ struct st_ctx {
     /* strong-strong beam-beam work area (grow-only) */
     char *d_ss = nullptr;
     size_t cap_ss = 0;
+    /* streams for slice pairs that touch disjoint slices (created on first
+       use, default priority) and the events that fork and join them */
+    hipStream_t lane[SS_LANES] = {};
+    hipEvent_t lane_ev[SS_LANES] = {};
+    hipEvent_t fork_ev = nullptr;
 };
 
 struct dev_table {
@@ -1040,6 +1047,14 @@ extern "C" int st_ctx_destroy(st_ctx *c)
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
+    for (int l = 0; l < SS_LANES; ++l) {
+        if (c->lane[l]) {
+            hipStreamSynchronize(c->lane[l]);
+            hipStreamDestroy(c->lane[l]);
+        }
+        if (c->lane_ev[l]) hipEventDestroy(c->lane_ev[l]);
+    }
+    if (c->fork_ev) hipEventDestroy(c->fork_ev);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return ST_OK;
@@ -2382,17 +2397,21 @@ struct ss_work {
     int *seg_i, *cnt;
     double *hx[2];
     size_t cap_part, cap_seg;
-    /* particle-in-cell */
-    int *pkey_in, *pkey_out, *pval_in, *pval_out;
-    double *pw, *pws, *grid, *params, *tw;
-    void *ptemp;
+    /* particle-in-cell: one set of buffers per lane */
+    struct pic_lane {
+        int *pkey_in, *pkey_out, *pval_in, *pval_out;
+        double *pw, *pws, *grid, *params, *part;
+        void *ptemp;
+    } pk[SS_LANES];
+    int lanes;
+    double *tw;
     size_t ptemp_bytes;
 };
 
 /* lay out (base == null: measure) the work area for beams of n1, n2 rows;
    pic_rows > 0 adds the particle-in-cell buffers for slices of that size */
 size_t ss_layout(char *base, int n1, int n2, int ns1, int ns2, bool host_rows, size_t temp_bytes, ss_work &w,
-                 int pic_rows = 0, size_t pic_temp = 0)
+                 int pic_rows = 0, size_t pic_temp = 0, int lanes = 1)
 {
     ss_arena a;
     a.base = base;
@@ -2419,21 +2438,25 @@ size_t ss_layout(char *base, int n1, int n2, int ns1, int ns2, bool host_rows, s
     w.cnt = a.take<int>((size_t)std::max(ns1, ns2) + 1);
     w.hx[0] = host_rows ? a.take<double>((size_t)(n1 > 0 ? n1 : 1) * 6) : nullptr;
     w.hx[1] = host_rows ? a.take<double>((size_t)(n2 > 0 ? n2 : 1) * 6) : nullptr;
-    w.pkey_in = w.pkey_out = w.pval_in = w.pval_out = nullptr;
-    w.pw = w.pws = w.grid = w.params = w.tw = nullptr;
-    w.ptemp = nullptr;
+    std::memset(w.pk, 0, sizeof w.pk);
+    w.tw = nullptr;
     w.ptemp_bytes = pic_temp;
+    w.lanes = pic_rows > 0 ? lanes : 0;
     if (pic_rows > 0) {
         const size_t nc = (size_t)pic_rows * 9;
-        w.pkey_in = a.take<int>(nc);
-        w.pkey_out = a.take<int>(nc);
-        w.pval_in = a.take<int>(nc);
-        w.pval_out = a.take<int>(nc);
-        w.pw = a.take<double>(nc);
-        w.pws = a.take<double>(nc);
-        w.ptemp = a.take<char>(pic_temp);
-        w.grid = a.take<double>((size_t)4096 * 4 + (size_t)16384 * 6);
-        w.params = a.take<double>(16);
+        for (int l = 0; l < lanes; ++l) {
+            ss_work::pic_lane &k = w.pk[l];
+            k.pkey_in = a.take<int>(nc);
+            k.pkey_out = a.take<int>(nc);
+            k.pval_in = a.take<int>(nc);
+            k.pval_out = a.take<int>(nc);
+            k.pw = a.take<double>(nc);
+            k.pws = a.take<double>(nc);
+            k.ptemp = a.take<char>(pic_temp);
+            k.grid = a.take<double>((size_t)4096 * 4 + (size_t)16384 * 6);
+            k.params = a.take<double>(16);
+            k.part = l == 0 ? w.part : a.take<double>(w.cap_part);
+        }
         w.tw = a.take<double>(2 * 254);
     }
     return a.off;
@@ -2446,7 +2469,9 @@ static int ss_prepare(st_ctx *c, int n1, int n2, int ns1, int ns2, bool host_row
     size_t temp = 0, ptemp = 0;
     HIPCHK(st_ss_sort_temp_bytes(std::max(std::max(n1, n2), 1), &temp));
     if (pic_rows > 0) HIPCHK(st_sort_cells_bytes(pic_rows * 9, &ptemp));
-    size_t need = ss_layout(nullptr, n1, n2, ns1, ns2, host_rows, temp, w, pic_rows, ptemp);
+    /* particle-in-cell: as many lanes as the smaller beam has slices */
+    const int lanes = pic_rows > 0 ? std::max(1, std::min(SS_LANES, std::min(ns1, ns2))) : 1;
+    size_t need = ss_layout(nullptr, n1, n2, ns1, ns2, host_rows, temp, w, pic_rows, ptemp, lanes);
     if (need > c->cap_ss) {
         hipFree(c->d_ss);
         c->d_ss = nullptr;
@@ -2454,7 +2479,12 @@ static int ss_prepare(st_ctx *c, int n1, int n2, int ns1, int ns2, bool host_row
         HIPCHK(hipMalloc((void **)&c->d_ss, need));
         c->cap_ss = need;
     }
-    ss_layout(c->d_ss, n1, n2, ns1, ns2, host_rows, temp, w, pic_rows, ptemp);
+    ss_layout(c->d_ss, n1, n2, ns1, ns2, host_rows, temp, w, pic_rows, ptemp, lanes);
+    for (int l = 1; l < lanes; ++l) {
+        if (!c->lane[l]) HIPCHK(hipStreamCreateWithFlags(&c->lane[l], hipStreamNonBlocking));
+        if (!c->lane_ev[l]) HIPCHK(hipEventCreateWithFlags(&c->lane_ev[l], hipEventDisableTiming));
+    }
+    if (lanes > 1 && !c->fork_ev) HIPCHK(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
     if (pic_rows > 0) {
         /* FFT1D's twiddles (simtrack.cpp:2614-2637), both directions, by its
            recurrence; built once (thread-safe static initialisation) */
@@ -2806,13 +2836,15 @@ extern "C" int st_bb_soft_gaussian_6d_zb_host(st_ctx *c, double Np1, int Mp1, do
    (simtrack.cpp:23286-23378): beam 1's charge kicks beam 2, then beam 2's
    charge kicks beam 1 */
 static int pic_pair(st_ctx *c, ss_work &w, double *d_x1, int s1, int m1, double *d_x2, int s2, int m2, double Np1,
-                    double Np2, double gamma1, double gamma2, double bbscale1, double bbscale2)
+                    double Np2, double gamma1, double gamma2, double bbscale1, double bbscale2, int lane = 0)
 {
+    const ss_work::pic_lane &k = w.pk[lane];
+    hipStream_t st = lane == 0 ? c->stream : c->lane[lane];
     int ev = -1;
     if (c->timing) {
         int rc = timed_event(c, &ev);
         if (rc) return rc;
-        HIPCHK(hipEventRecord(c->ev_pool[ev], c->stream));
+        HIPCHK(hipEventRecord(c->ev_pool[ev], st));
     }
     ss_pair_args a{};
     a.x[0] = d_x1;
@@ -2821,9 +2853,9 @@ static int pic_pair(st_ctx *c, ss_work &w, double *d_x1, int s1, int m1, double 
     a.start[1] = s2;
     a.count[0] = m1;
     a.count[1] = m2;
-    a.part = w.part;
-    HIPCHK(st_ss_partials(a, c->stream));
-    HIPCHK(st_pic_params(w.part, a.nb[0], a.nb[1], m1, m2, w.params, c->stream));
+    a.part = k.part;
+    HIPCHK(st_ss_partials(a, st));
+    HIPCHK(st_pic_params(k.part, a.nb[0], a.nb[1], m1, m2, k.params, st));
     pic_args p{};
     p.x[0] = d_x1;
     p.x[1] = d_x2;
@@ -2831,16 +2863,16 @@ static int pic_pair(st_ctx *c, ss_work &w, double *d_x1, int s1, int m1, double 
     p.start[1] = s2;
     p.count[0] = m1;
     p.count[1] = m2;
-    p.params = w.params;
-    p.key_in = w.pkey_in;
-    p.key_out = w.pkey_out;
-    p.val_in = w.pval_in;
-    p.val_out = w.pval_out;
-    p.w = w.pw;
-    p.ws = w.pws;
-    p.temp = w.ptemp;
+    p.params = k.params;
+    p.key_in = k.pkey_in;
+    p.key_out = k.pkey_out;
+    p.val_in = k.pval_in;
+    p.val_out = k.pval_out;
+    p.w = k.pw;
+    p.ws = k.pws;
+    p.temp = k.ptemp;
     p.temp_bytes = w.ptemp_bytes;
-    double *g = w.grid;
+    double *g = k.grid;
     p.src = g;
     p.phi = g + 4096;
     p.Ex = g + 2 * 4096;
@@ -2859,10 +2891,10 @@ static int pic_pair(st_ctx *c, ss_work &w, double *d_x1, int s1, int m1, double 
     p.gamma[1] = gamma2;
     p.bbscale[0] = bbscale1;
     p.bbscale[1] = bbscale2;
-    HIPCHK(st_pic_solve_and_kick(p, 0, c->stream));
-    HIPCHK(st_pic_solve_and_kick(p, 1, c->stream));
+    HIPCHK(st_pic_solve_and_kick(p, 0, st));
+    HIPCHK(st_pic_solve_and_kick(p, 1, st));
     if (c->timing) {
-        HIPCHK(hipEventRecord(c->ev_pool[ev + 1], c->stream));
+        HIPCHK(hipEventRecord(c->ev_pool[ev + 1], st));
         c->pending.push_back({ev, ev + 1});
         c->pending_work.push_back((int64_t)m1 + m2);
     }
@@ -2892,14 +2924,35 @@ static int pic_6d(st_ctx *c, ss_work &w, double Np1, int Mp1, double *d_x1, doub
     std::vector<int> I1, I2;
     std::vector<double> S;
     ss_schedule(Z1, Z2, I1, I2, S);
-    for (size_t i = 0; i < S.size(); ++i) {
-        const int a = I1[i] * mc1, b = I2[i] * mc2;
-        HIPCHK(st_pic_drift(d_x1, a, mc1, S[i], 1, c->stream));
-        HIPCHK(st_pic_drift(d_x2, b, mc2, S[i], -1, c->stream));
-        rc = pic_pair(c, w, d_x1, a, mc1, d_x2, b, mc2, Np1 / Ns1, Np2 / Ns2, gamma1, gamma2, bbscale1, bbscale2);
-        if (rc) return rc;
-        HIPCHK(st_pic_drift(d_x1, a, mc1, S[i], -1, c->stream));
-        HIPCHK(st_pic_drift(d_x2, b, mc2, S[i], 1, c->stream));
+    /* runs of consecutive pairs that touch disjoint slices of both beams
+       commute, so each run goes out at once, one pair per lane; pairs that
+       share a slice stay in schedule order (they fall in different runs) */
+    const size_t np = S.size();
+    size_t i0 = 0;
+    while (i0 < np) {
+        size_t i1 = i0;
+        std::vector<char> u1(Ns1, 0), u2(Ns2, 0);
+        while (i1 < np && (int)(i1 - i0) < w.lanes && !u1[I1[i1]] && !u2[I2[i1]]) {
+            u1[I1[i1]] = u2[I2[i1]] = 1;
+            ++i1;
+        }
+        if (i1 - i0 > 1) HIPCHK(hipEventRecord(c->fork_ev, c->stream));
+        for (size_t i = i0; i < i1; ++i) {
+            const int l = (int)(i - i0);
+            hipStream_t st = l == 0 ? c->stream : c->lane[l];
+            if (l > 0) HIPCHK(hipStreamWaitEvent(st, c->fork_ev, 0));
+            const int a = I1[i] * mc1, b = I2[i] * mc2;
+            HIPCHK(st_pic_drift(d_x1, a, mc1, S[i], 1, st));
+            HIPCHK(st_pic_drift(d_x2, b, mc2, S[i], -1, st));
+            rc = pic_pair(c, w, d_x1, a, mc1, d_x2, b, mc2, Np1 / Ns1, Np2 / Ns2, gamma1, gamma2, bbscale1, bbscale2,
+                          l);
+            if (rc) return rc;
+            HIPCHK(st_pic_drift(d_x1, a, mc1, S[i], -1, st));
+            HIPCHK(st_pic_drift(d_x2, b, mc2, S[i], 1, st));
+            if (l > 0) HIPCHK(hipEventRecord(c->lane_ev[l], st));
+        }
+        for (size_t i = i0 + 1; i < i1; ++i) HIPCHK(hipStreamWaitEvent(c->stream, c->lane_ev[i - i0], 0));
+        i0 = i1;
     }
     return ST_OK;
 }
