@@ -2441,7 +2441,8 @@ size_t ss_layout(char *base, int n1, int n2, int ns1, int ns2, bool host_rows, s
     std::memset(w.pk, 0, sizeof w.pk);
     w.tw = nullptr;
     w.ptemp_bytes = pic_temp;
-    w.lanes = pic_rows > 0 ? lanes : 0;
+    w.lanes = lanes;
+    for (int l = 0; l < lanes; ++l) w.pk[l].part = l == 0 ? w.part : a.take<double>(w.cap_part);
     if (pic_rows > 0) {
         const size_t nc = (size_t)pic_rows * 9;
         for (int l = 0; l < lanes; ++l) {
@@ -2455,7 +2456,6 @@ size_t ss_layout(char *base, int n1, int n2, int ns1, int ns2, bool host_rows, s
             k.ptemp = a.take<char>(pic_temp);
             k.grid = a.take<double>((size_t)4096 * 4 + (size_t)16384 * 6);
             k.params = a.take<double>(16);
-            k.part = l == 0 ? w.part : a.take<double>(w.cap_part);
         }
         w.tw = a.take<double>(2 * 254);
     }
@@ -2469,8 +2469,8 @@ static int ss_prepare(st_ctx *c, int n1, int n2, int ns1, int ns2, bool host_row
     size_t temp = 0, ptemp = 0;
     HIPCHK(st_ss_sort_temp_bytes(std::max(std::max(n1, n2), 1), &temp));
     if (pic_rows > 0) HIPCHK(st_sort_cells_bytes(pic_rows * 9, &ptemp));
-    /* particle-in-cell: as many lanes as the smaller beam has slices */
-    const int lanes = pic_rows > 0 ? std::max(1, std::min(SS_LANES, std::min(ns1, ns2))) : 1;
+    /* as many lanes as the smaller beam has slices */
+    const int lanes = std::max(1, std::min(SS_LANES, std::min(ns1, ns2)));
     size_t need = ss_layout(nullptr, n1, n2, ns1, ns2, host_rows, temp, w, pic_rows, ptemp, lanes);
     if (need > c->cap_ss) {
         hipFree(c->d_ss);
@@ -2576,10 +2576,47 @@ static void ss_schedule(const std::vector<double> &Z1, const std::vector<double>
             }
 }
 
+/*
+ * The slice pairs of a schedule in runs: consecutive pairs that touch
+ * disjoint slices of both beams commute, so each run goes out at once, pair
+ * k of the run on lane k (lane 0 is the context stream); pairs that share a
+ * slice fall in different runs and keep their order, so the result is the
+ * sequential one bit for bit.  pair(i, lane, stream) enqueues pair i.
+ */
+template <class F>
+static int ss_runs(st_ctx *c, const ss_work &w, const std::vector<int> &I1, const std::vector<int> &I2, int Ns1,
+                   int Ns2, F &&pair)
+{
+    const size_t np = I1.size();
+    const int lanes = std::max(1, w.lanes);
+    size_t i0 = 0;
+    while (i0 < np) {
+        size_t i1 = i0;
+        std::vector<char> u1(Ns1, 0), u2(Ns2, 0);
+        while (i1 < np && (int)(i1 - i0) < lanes && !u1[I1[i1]] && !u2[I2[i1]]) {
+            u1[I1[i1]] = u2[I2[i1]] = 1;
+            ++i1;
+        }
+        if (i1 - i0 > 1) HIPCHK(hipEventRecord(c->fork_ev, c->stream));
+        for (size_t i = i0; i < i1; ++i) {
+            const int l = (int)(i - i0);
+            hipStream_t st = l == 0 ? c->stream : c->lane[l];
+            if (l > 0) HIPCHK(hipStreamWaitEvent(st, c->fork_ev, 0));
+            int rc = pair(i, l, st);
+            if (rc) return rc;
+            if (l > 0) HIPCHK(hipEventRecord(c->lane_ev[l], st));
+        }
+        for (size_t i = i0 + 1; i < i1; ++i) HIPCHK(hipStreamWaitEvent(c->stream, c->lane_ev[i - i0], 0));
+        i0 = i1;
+    }
+    return ST_OK;
+}
+
 static int ss_pair(st_ctx *c, ss_work &w, double *d_x1, const int *idx1, int start1, int m1, double *d_x2,
                    const int *idx2, int start2, int m2, double S, int drift, double N1, double N2, double gamma1,
-                   double gamma2, double bbscale1, double bbscale2)
+                   double gamma2, double bbscale1, double bbscale2, int lane = 0)
 {
+    hipStream_t st = lane == 0 ? c->stream : c->lane[lane];
     ss_pair_args a{};
     a.x[0] = d_x1;
     a.x[1] = d_x2;
@@ -2596,16 +2633,16 @@ static int ss_pair(st_ctx *c, ss_work &w, double *d_x1, const int *idx1, int sta
     a.c2Nrg[1] = 2 * N1 * SS_RP / gamma2;
     a.bbscale[0] = bbscale1;
     a.bbscale[1] = bbscale2;
-    a.part = w.part;
+    a.part = w.pk[lane].part;
     int ev = -1;
     if (c->timing) {
         int rc = timed_event(c, &ev);
         if (rc) return rc;
-        HIPCHK(hipEventRecord(c->ev_pool[ev], c->stream));
+        HIPCHK(hipEventRecord(c->ev_pool[ev], st));
     }
-    HIPCHK(st_ss_collide(a, c->stream));
+    HIPCHK(st_ss_collide(a, st));
     if (c->timing) {
-        HIPCHK(hipEventRecord(c->ev_pool[ev + 1], c->stream));
+        HIPCHK(hipEventRecord(c->ev_pool[ev + 1], st));
         c->pending.push_back({ev, ev + 1});
         c->pending_work.push_back((int64_t)m1 + m2);
     }
@@ -2652,12 +2689,10 @@ static int ss_6d(st_ctx *c, ss_work &w, double Np1, int Mp1, double *d_x1, doubl
     std::vector<int> I1, I2;
     std::vector<double> S;
     ss_schedule(Z1, Z2, I1, I2, S);
-    for (size_t i = 0; i < S.size(); ++i) {
-        rc = ss_pair(c, w, d_x1, nullptr, I1[i] * mc1, mc1, d_x2, nullptr, I2[i] * mc2, mc2, S[i], 1, Np1 / Ns1,
-                     Np2 / Ns2, gamma1, gamma2, bbscale1, bbscale2);
-        if (rc) return rc;
-    }
-    return ST_OK;
+    return ss_runs(c, w, I1, I2, Ns1, Ns2, [&](size_t i, int l, hipStream_t) {
+        return ss_pair(c, w, d_x1, nullptr, I1[i] * mc1, mc1, d_x2, nullptr, I2[i] * mc2, mc2, S[i], 1, Np1 / Ns1,
+                       Np2 / Ns2, gamma1, gamma2, bbscale1, bbscale2, l);
+    });
 }
 
 /* BeamBeam6D_SoftGaussian with slice boundaries, simtrack.cpp:23134-23284 */
@@ -2696,15 +2731,13 @@ static int ss_6d_zb(st_ctx *c, ss_work &w, double Np1, int Mp1, double *d_x1, do
     std::vector<int> I1, I2;
     std::vector<double> S;
     ss_schedule(Z1, Z2, I1, I2, S);
-    for (size_t i = 0; i < S.size(); ++i) {
+    return ss_runs(c, w, I1, I2, Ns1, Ns2, [&](size_t i, int l, hipStream_t) {
         const int a = I1[i], b = I2[i];
         const int m1 = ct[0][a], m2 = ct[1][b];
         /* BeamBeam4D_SoftGaussian(Np1*Mp1_coll/Mp1, ...) on the drifted slices (cpp:23252) */
-        int rc = ss_pair(c, w, d_x1, w.rows[0], st[0][a], m1, d_x2, w.rows[1], st[1][b], m2, S[i], 1,
-                         Np1 * m1 / Mp1, Np2 * m2 / Mp2, gamma1, gamma2, bbscale1, bbscale2);
-        if (rc) return rc;
-    }
-    return ST_OK;
+        return ss_pair(c, w, d_x1, w.rows[0], st[0][a], m1, d_x2, w.rows[1], st[1][b], m2, S[i], 1,
+                       Np1 * m1 / Mp1, Np2 * m2 / Mp2, gamma1, gamma2, bbscale1, bbscale2, l);
+    });
 }
 
 static int ss_check_zb(int Ns, const double *zb, const char *what)
@@ -2924,37 +2957,17 @@ static int pic_6d(st_ctx *c, ss_work &w, double Np1, int Mp1, double *d_x1, doub
     std::vector<int> I1, I2;
     std::vector<double> S;
     ss_schedule(Z1, Z2, I1, I2, S);
-    /* runs of consecutive pairs that touch disjoint slices of both beams
-       commute, so each run goes out at once, one pair per lane; pairs that
-       share a slice stay in schedule order (they fall in different runs) */
-    const size_t np = S.size();
-    size_t i0 = 0;
-    while (i0 < np) {
-        size_t i1 = i0;
-        std::vector<char> u1(Ns1, 0), u2(Ns2, 0);
-        while (i1 < np && (int)(i1 - i0) < w.lanes && !u1[I1[i1]] && !u2[I2[i1]]) {
-            u1[I1[i1]] = u2[I2[i1]] = 1;
-            ++i1;
-        }
-        if (i1 - i0 > 1) HIPCHK(hipEventRecord(c->fork_ev, c->stream));
-        for (size_t i = i0; i < i1; ++i) {
-            const int l = (int)(i - i0);
-            hipStream_t st = l == 0 ? c->stream : c->lane[l];
-            if (l > 0) HIPCHK(hipStreamWaitEvent(st, c->fork_ev, 0));
-            const int a = I1[i] * mc1, b = I2[i] * mc2;
-            HIPCHK(st_pic_drift(d_x1, a, mc1, S[i], 1, st));
-            HIPCHK(st_pic_drift(d_x2, b, mc2, S[i], -1, st));
-            rc = pic_pair(c, w, d_x1, a, mc1, d_x2, b, mc2, Np1 / Ns1, Np2 / Ns2, gamma1, gamma2, bbscale1, bbscale2,
-                          l);
-            if (rc) return rc;
-            HIPCHK(st_pic_drift(d_x1, a, mc1, S[i], -1, st));
-            HIPCHK(st_pic_drift(d_x2, b, mc2, S[i], 1, st));
-            if (l > 0) HIPCHK(hipEventRecord(c->lane_ev[l], st));
-        }
-        for (size_t i = i0 + 1; i < i1; ++i) HIPCHK(hipStreamWaitEvent(c->stream, c->lane_ev[i - i0], 0));
-        i0 = i1;
-    }
-    return ST_OK;
+    return ss_runs(c, w, I1, I2, Ns1, Ns2, [&](size_t i, int l, hipStream_t st) {
+        const int a = I1[i] * mc1, b = I2[i] * mc2;
+        HIPCHK(st_pic_drift(d_x1, a, mc1, S[i], 1, st));
+        HIPCHK(st_pic_drift(d_x2, b, mc2, S[i], -1, st));
+        int rc2 = pic_pair(c, w, d_x1, a, mc1, d_x2, b, mc2, Np1 / Ns1, Np2 / Ns2, gamma1, gamma2, bbscale1,
+                           bbscale2, l);
+        if (rc2) return rc2;
+        HIPCHK(st_pic_drift(d_x1, a, mc1, S[i], -1, st));
+        HIPCHK(st_pic_drift(d_x2, b, mc2, S[i], 1, st));
+        return (int)ST_OK;
+    });
 }
 
 extern "C" int st_bb_pic_4d(st_ctx *c, double Np1, int Mp1, double *d_x1, double Np2, int Mp2, double *d_x2,
