@@ -283,6 +283,37 @@ def cal_chrom(ctx, gp, records, step_deltap=0.0001):
             (p.tune2 + m.tune2 - 2 * z.tune2) / 2. / step_deltap / step_deltap)
 
 
+def set_line_optics(line, tw, eta=None):
+    """Write a cal_twiss result into the Line's elements as Cal_Twiss does
+    (Beta1, Alfa1, Beta2, Alfa2, Mu1, Mu2, r, c11..c22, X) and the tunes into
+    Line.Tune1, Line.Tune2; eta (cal_dispersion) fills Etax..Etayp."""
+    for k, e in enumerate(line.Cell):
+        e.Beta1, e.Alfa1, e.Beta2, e.Alfa2 = (float(tw.beta1[k]), float(tw.alfa1[k]), float(tw.beta2[k]),
+                                              float(tw.alfa2[k]))
+        e.Mu1, e.Mu2, e.r = float(tw.mu1[k]), float(tw.mu2[k]), float(tw.r[k])
+        e.c11, e.c12, e.c21, e.c22 = (float(v) for v in tw.c[k].ravel())
+        e.X = [float(v) for v in tw.X[k]]
+        if eta is not None:
+            e.Etax, e.Etaxp, e.Etay, e.Etayp = (float(v) for v in eta[k])
+    line.Tune1, line.Tune2 = tw.tune1, tw.tune2
+
+
+def Print_Twiss(line, filename):
+    """Print_Twiss (simtrack.cpp:19846-19862): a header and one line per
+    element, each field right-aligned in 15 columns in C++'s default stream
+    format (%g).  Reads the fields set_line_optics wrote."""
+    def f(v):
+        return (f"{v:g}" if isinstance(v, float) else str(v)).rjust(15)
+    line.Update()
+    with open(filename, "w") as out:
+        out.write("".join(h.rjust(15) for h in ("NAME", "TYPE", "S", "BETA1", "ALFA1", "MU1", "BETA2", "ALFA2", "MU2",
+                                                 "ETAX", "ETAXP", "ETAY", "ETAYP")) + "\n")
+        for e in line.Cell:
+            out.write("".join(f(v) for v in (e.NAME, e.TYPE, float(e.S), e.Beta1, e.Alfa1, e.Mu1, e.Beta2, e.Alfa2,
+                                             e.Mu2, getattr(e, "Etax", 0.), getattr(e, "Etaxp", 0.),
+                                             getattr(e, "Etay", 0.), getattr(e, "Etayp", 0.))) + "\n")
+
+
 def action_diff(tab, betx, alfx, bety, alfy, nsigma, phase, emitx0, emity0, nturn, nstep, npart, seed=1234.):
     """Track_tbt_ActionDiff on the GPU: (rows, seed after).  Rows: turns,
     surviving count, mean and rms of 2Jx, mean and rms of 2Jy per step."""
@@ -336,4 +367,5 @@ def Track_tbt_Lyapunov(tab, deltap0, sigmax0, sigmay0, filename="./Lmap-output.d
 
 __all__ = ["cal_orbit_num", "fma_scan", "lyapunov_scan", "Track_tbt_FMA", "Track_tbt_Lyapunov", "one_turn_matrix",
            "twiss_from_matrix", "twiss_at_start", "action_diff", "Track_tbt_ActionDiff", "STError", "delta_to_pt",
-           "element_maps", "cal_a", "trace_a", "cal_twiss", "LineTwiss", "cal_dispersion", "cal_chrom"]
+           "element_maps", "cal_a", "trace_a", "cal_twiss", "LineTwiss", "cal_dispersion", "cal_chrom",
+           "set_line_optics", "Print_Twiss"]

@@ -192,3 +192,12 @@ def test_cpp_action_diff_matches_python(ctx, tmp_path):
     lines = (tmp_path / "twiss0.dat").read_text().splitlines()
     assert len(lines) == n + 1 and lines[0].split()[:4] == ["NAME", "TYPE", "S", "BETA1"]
     assert float(lines[-1].split()[3]) == pytest.approx(pt.beta1[-1], rel=1e-5)
+    # the Python writer gives the same table from the Python optics
+    D.set_line_optics(ln, pt, eta)
+    D.Print_Twiss(ln, str(tmp_path / "py-twiss0.dat"))
+    py = (tmp_path / "py-twiss0.dat").read_text().splitlines()
+    assert len(py) == len(lines) and py[0] == lines[0]
+    # numbers compared as printed (6 digits): a value within rounding of a
+    # printed boundary may differ in its last digit between the two paths
+    same = sum(a.split()[-11:] == b.split()[-11:] for a, b in zip(py[1:], lines[1:]))
+    assert same >= 0.99 * n, same
