@@ -2,6 +2,8 @@
 # Build the CPU oracle (test infrastructure; see st_oracle.c).
 #   liboracle.so        project libm (st_libm.h): bit-comparable with the GPU
 #   liboracle_glibc.so  C library libm: what the reference itself calls
+#   liboracle_glibc_{trig,asin,exp,pow}.so  the C library for one function
+#                       family only (sin/cos/tan, asin, exp, pow)
 set -e
 here=$(cd "$(dirname "$0")" && pwd)
 root=$(dirname "$here")
@@ -12,3 +14,8 @@ gcc $CFLAGS $INC "$here/st_oracle.c" -o "$here/build/liboracle.so.tmp" -lm
 mv "$here/build/liboracle.so.tmp" "$here/build/liboracle.so"
 gcc $CFLAGS -DORACLE_GLIBC $INC "$here/st_oracle.c" -o "$here/build/liboracle_glibc.so.tmp" -lm
 mv "$here/build/liboracle_glibc.so.tmp" "$here/build/liboracle_glibc.so"
+for v in trig:1 asin:2 exp:4 pow:8; do
+  name=${v%%:*}; mask=${v##*:}
+  gcc $CFLAGS -DORACLE_GLIBC_MASK=$mask $INC "$here/st_oracle.c" -o "$here/build/liboracle_glibc_$name.so.tmp" -lm
+  mv "$here/build/liboracle_glibc_$name.so.tmp" "$here/build/liboracle_glibc_$name.so"
+done

@@ -18,10 +18,17 @@ def build():
 
 
 def load(glibc=False):
-    key = bool(glibc)
+    """The oracle library: glibc=False the project libm, True the C library
+    for every function, or one of "trig", "asin", "exp", "pow" for the C
+    library in that function family only."""
+    key = glibc if isinstance(glibc, str) else bool(glibc)
     if key in _libs:
         return _libs[key]
-    path = os.path.join(HERE, "build", "liboracle_glibc.so" if glibc else "liboracle.so")
+    if isinstance(glibc, str):
+        name = f"liboracle_glibc_{glibc}.so"
+    else:
+        name = "liboracle_glibc.so" if glibc else "liboracle.so"
+    path = os.path.join(HERE, "build", name)
     if not os.path.exists(path):
         build()
     lib = ctypes.CDLL(path)
@@ -58,9 +65,11 @@ def load(glibc=False):
         getattr(lib, f).restype = D
     lib.orc_powi.argtypes = [D, I]
     lib.orc_powi.restype = D
-    if not glibc:
+    if glibc is False:
         lib.orc_asin_small.restype = D
         lib.orc_asin_small.argtypes = [D]
+        lib.orc_exp_phase.restype = D
+        lib.orc_exp_phase.argtypes = [D, I]
     lib.orc_rng_normal.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]
     lib.orc_rng_normal.restype = D
     lib.orc_philox.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, P]

@@ -49,7 +49,7 @@ def build(verbose=False):
     objdir = os.path.join(LIBDIR, "obj")
     os.makedirs(objdir, exist_ok=True)
     hc = hipcc()
-    objs = []
+    jobs = []
     for src, extra in (("st_kernels.hip", ["--offload-arch=" + ARCH]),
                        ("st_tune.hip", ["--offload-arch=" + ARCH]),
                        ("st_mlp.hip", ["--offload-arch=" + ARCH]),
@@ -58,8 +58,14 @@ def build(verbose=False):
                        ("st_capi.cpp", ["--offload-arch=" + ARCH, "-x", "hip"]),
                        ("st_da.cpp", [])):
         obj = os.path.join(objdir, src.rsplit(".", 1)[0] + ".o")
-        _run([hc, *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
-        objs.append(obj)
+        jobs.append(([hc, *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj], obj))
+    # independent translation units: compile them side by side
+    from concurrent.futures import ThreadPoolExecutor
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", "0")) or (os.cpu_count() or 1)))
+    with ThreadPoolExecutor(workers) as ex:
+        for f in [ex.submit(_run, cmd, verbose) for cmd, _ in jobs]:
+            f.result()
+    objs = [obj for _, obj in jobs]
     out = os.path.join(LIBDIR, LIBNAME)
     tmp = out + ".tmp"
     _run([hc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs], verbose)

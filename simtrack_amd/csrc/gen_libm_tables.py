@@ -77,6 +77,33 @@ def main(out):
     for n in range(2, 15):
         w(f"#define ST_EXP_{n} {hexd(rn(Fraction(1, factorial(n))))} /* 1/{n}! */")
     w("")
+    # exp, table-driven (Tang): x = k ln2/128 + r, 2^(j/128) as a triple of doubles
+    ln2_128 = mpmath.log(2) / 128
+    l1 = rn(mpmath.nint(ln2_128 * mpmath.mpf(2) ** 42) / mpmath.mpf(2) ** 42)   # 35 significant bits
+    l2 = rn(ln2_128 - l1)
+    l3 = rn(ln2_128 - l1 - l2)
+    w("/* exp: k = rint(x * 128/ln2); ln2/128 = L1 + L2 + L3 + O(2^-150), L1 with 35")
+    w("   significant bits so that k*L1 is exact for |k| < 2^18 */")
+    w(f"#define ST_EXP_INVL {hexd(rn(128 / mpmath.log(2)))}")
+    w(f"#define ST_EXP_L1 {hexd(l1)}")
+    w(f"#define ST_EXP_L2 {hexd(l2)}")
+    w(f"#define ST_EXP_L3 {hexd(l3)}")
+    # 1/n! for n = 2..6 as double-double (hi, lo)
+    for n in range(2, 7):
+        hi = rn(Fraction(1, factorial(n)))
+        lo = rn(Fraction(1, factorial(n)) - Fraction(hi))
+        w(f"#define ST_EXPC{n}_HI {hexd(hi)}")
+        w(f"#define ST_EXPC{n}_LO {hexd(lo)}")
+    w("/* 2^(j/128) = T[3j] + T[3j+1] + T[3j+2] + O(2^-160), j = 0..127 */")
+    w("#define ST_EXP_TAB \\")
+    trip = []
+    for j in range(128):
+        trip += split(mpmath.mpf(2) ** (mpmath.mpf(j) / 128), 3)
+    rows = []
+    for j in range(0, len(trip), 3):
+        rows.append(", ".join(hexd(v) for v in trip[j:j + 3]))
+    w("  { " + ", \\\n    ".join(rows) + " }")
+    w("")
     # log: log(1+f) = f - s*(f - R), s = f/(2+f), R = sum_{n>=1} 2/(2n+1) s^(2n)
     for n in range(1, 15):
         w(f"#define ST_LOG_{n} {hexd(rn(Fraction(2, 2 * n + 1)))} /* 2/{2*n+1} */")

@@ -22,11 +22,12 @@
  *   asin     Maclaurin series of asin for |x| <= 1/2 and the half-angle
  *            identity asin(x) = pi/2 - 2 asin(sqrt((1-x)/2)) above, with a
  *            fma-exact square-root residual.
- *   exp      k = rint(x/ln2), r = x - k ln2 in double-double, Taylor to
- *            t^14/14!, scaled by 2^k through the exponent field.
+ *   exp      correctly rounded: Tang's table-driven reduction with Ziv's
+ *            two-phase rounding test (details at st_exp).
  *   powi     h^n for integer n by binary powering in double-double
  *            arithmetic (exact products via fma) and one final rounding.
- * Accuracy (tested against mpmath at 50 digits): < 1 ulp for all four.
+ * Accuracy (tested against mpmath at 50 digits): < 1 ulp; exp is
+ * correctly rounded.
  *
  * The includer defines, before including:
  *   ST_FN            function qualifiers ("static inline" for C,
@@ -67,6 +68,14 @@ ST_FN double st_pow2i(int k) { return st_u2d((uint64_t)(k + 1023) << 52); }
 /* ------------------------------------------------------ double-double */
 
 typedef struct { double hi, lo; } st_dd;
+
+ST_FN st_dd st_dd_make(double hi, double lo)
+{
+    st_dd r;
+    r.hi = hi;
+    r.lo = lo;
+    return r;
+}
 
 ST_FN st_dd st_two_sum(double a, double b)
 {
@@ -352,42 +361,166 @@ ST_FN double st_asin(double x)
 
 /* ---------------------------------------------------------------- exp */
 
-ST_FN double st_exp(double x)
+/*
+ * exp, correctly rounded to nearest.  Method: Tang's table-driven reduction
+ * (ACM TOMS 15 (1989) 144-157) x = (128 m + j) ln2/128 + r, |r| <= ln2/256,
+ * exp(x) = 2^m 2^(j/128) exp(r), with Ziv's two-phase rounding test
+ * (ACM TOMS 17 (1991) 410-423):
+ *   fast phase      r as a double-double, exp(r)-1 to degree 6, 2^(j/128)
+ *                   as a double-double; the result yh + yl carries a
+ *                   relative error below ST_EXP_EPS1 = 2^-64 (the terms are
+ *                   bounded in the comments below).  If the exact value
+ *                   cannot lie on the other side of a rounding boundary of
+ *                   yh, yh is the correctly rounded result.
+ *   accurate phase  (one call in ~2^10) exp(r)-1 to degree 11 by Horner in
+ *                   double-double, 2^(j/128) as a triple; relative error
+ *                   below ST_EXP_EPS2 = 2^-100, the same test.  An argument
+ *                   whose exp lies within 2^-100 of a midpoint between two
+ *                   doubles (a measure-2^-47 set) keeps the double-double's
+ *                   rounding.
+ * Subnormal results are rounded once, on the 2^-1074 grid.
+ */
+#define ST_EXP_EPS1 0x1p-64
+#define ST_EXP_EPS2 0x1p-100
+ST_CONST_TABLE double st_exp_tab[384] = ST_EXP_TAB;
+
+/* Round 2^m (yh + yl) to the nearest double, given that the exact value is
+   within eps*|yh| of yh + yl (|yl| <= ulp(yh)/2, yh in [0.99, 2)).  Returns 1
+   and sets *res when the rounding is decided, 0 when the exact value may lie
+   on either side of a rounding boundary. */
+ST_FN int st_exp_round(double yh, double yl, int m, double eps, double *res)
+{
+    double err = eps * yh;
+    if (m > -1022 || (m == -1022 && yh >= 1.0)) {
+        uint64_t b = st_d2u(yh);
+        double u = st_u2d(b & 0x7ff0000000000000ull) * 0x1p-52;       /* ulp(yh) */
+        /* half the gap to the neighbour on yl's side; below a power of two
+           the gap is half as wide */
+        double half = ((b & 0x000fffffffffffffull) == 0 && yl < 0.0) ? 0.25 * u : 0.5 * u;
+        if (st_fabs(yl) + err >= half)
+            return 0;
+        if (m > 1023)
+            *res = (yh * 2.0) * st_pow2i(m - 1);                      /* may overflow to inf */
+        else
+            *res = yh * st_pow2i(m);
+        return 1;
+    }
+    /* subnormal (or zero) result: n = rint(2^(m+1074) (yh + yl)), value n 2^-1074 */
+    int s = m + 1074;
+    if (s < -60) {                                   /* far below 2^-1075 */
+        *res = 0.0;
+        return 1;
+    }
+    double sc = s >= 0 ? st_pow2i(s) : 0x1p-60 * st_pow2i(s + 60);
+    double V = yh * sc, Vl = yl * sc;                /* exact scalings */
+    double n = ST_RINT(V);
+    double d = (V - n) + Vl;
+    if (d > 0.5) {
+        n += 1.0;
+        d -= 1.0;
+    } else if (d < -0.5) {
+        n -= 1.0;
+        d += 1.0;
+    }
+    if (0.5 - st_fabs(d) <= err * sc)
+        return 0;
+    *res = n * st_u2d(1);                            /* n * 2^-1074, exact */
+    return 1;
+}
+
+/* The accurate phase: exp(rh + rl) 2^(j/128) as a double-double with
+   relative error below ST_EXP_EPS2, rounded by st_exp_round. */
+ST_FN_COLD double st_exp_accurate(double rh, double rl, int j, int m)
+{
+    /* degrees 7..11 in double: |r^7/7!| < 2^-71.9, so their rounding
+       errors are below 2^-124 */
+    double hp = ST_FMA(rh, ST_EXP_11, ST_EXP_10);
+    hp = ST_FMA(rh, hp, ST_EXP_9);
+    hp = ST_FMA(rh, hp, ST_EXP_8);
+    hp = ST_FMA(rh, hp, ST_EXP_7);
+    st_dd r;
+    r.hi = rh;
+    r.lo = rl;
+    /* A = 1/6! + r*(1/7! + ...), then A = 1/n! + r*A for n = 5..2, 1 + r*A,
+       e = r*A: Horner in double-double */
+    st_dd A = st_dd_add(st_dd_make(ST_EXPC6_HI, ST_EXPC6_LO), st_dd_mul(r, st_dd_make(hp, 0.0)));
+    A = st_dd_add(st_dd_make(ST_EXPC5_HI, ST_EXPC5_LO), st_dd_mul(r, A));
+    A = st_dd_add(st_dd_make(ST_EXPC4_HI, ST_EXPC4_LO), st_dd_mul(r, A));
+    A = st_dd_add(st_dd_make(ST_EXPC3_HI, ST_EXPC3_LO), st_dd_mul(r, A));
+    A = st_dd_add(st_dd_make(ST_EXPC2_HI, ST_EXPC2_LO), st_dd_mul(r, A));
+    A = st_dd_add(st_dd_make(1.0, 0.0), st_dd_mul(r, A));
+    st_dd e = st_dd_mul(r, A);                       /* exp(r) - 1 */
+    st_dd T;
+    T.hi = st_exp_tab[3 * j];
+    T.lo = st_exp_tab[3 * j + 1];
+    double tl = st_exp_tab[3 * j + 2];
+    /* T (1 + e) + tl (1 + e) = T + (T e + tl) */
+    st_dd p = st_dd_mul(T, e);
+    p = st_dd_add(p, st_dd_make(tl, 0.0));
+    st_dd y = st_dd_add(T, p);
+    double res;
+    if (st_exp_round(y.hi, y.lo, m, ST_EXP_EPS2, &res))
+        return res;
+    /* within 2^-100 of a midpoint: the double-double's own rounding */
+    if (m > 1023)
+        return ((y.hi + y.lo) * 2.0) * st_pow2i(m - 1);
+    if (m >= -1021)
+        return (y.hi + y.lo) * st_pow2i(m);
+    return ((y.hi + y.lo) * st_pow2i(m + 60)) * 0x1p-60;
+}
+
+/* phase = 0: both phases (st_exp); 1: the accurate phase only; 2: the fast
+   phase only, nan where it leaves the rounding undecided (phases 1 and 2
+   let tests exercise each phase on its own) */
+ST_FN double st_exp_phase(double x, int phase)
 {
     if (st_isnan(x))
         return x + x;
-    if (x > 709.782712893384)
-        return 0x1p1023 * 2.0;                        /* +inf */
-    if (x < -745.1332191019412)
-        return 0.0;
+    if (x > 709.79)
+        return 0x1p1023 * 2.0;                       /* +inf: exp(709.79) > 2^1024 */
+    if (x < -745.2)
+        return 0.0;                                  /* exp(-745.2) < 2^-1075 */
     if (st_fabs(x) < 0x1p-54)
-        return 1.0 + x;
-    double k = ST_RINT(x * ST_INV_LN2);
-    double rhi = ST_FMA(-k, ST_LN2_HI, x);           /* exact */
-    double rlo = -k * ST_LN2_LO;
-    double t = rhi + rlo;
-    double tlo = (rhi - t) + rlo;
-    double p = ST_FMA(t, ST_EXP_14, ST_EXP_13);
-    p = ST_FMA(t, p, ST_EXP_12);
-    p = ST_FMA(t, p, ST_EXP_11);
-    p = ST_FMA(t, p, ST_EXP_10);
-    p = ST_FMA(t, p, ST_EXP_9);
-    p = ST_FMA(t, p, ST_EXP_8);
-    p = ST_FMA(t, p, ST_EXP_7);
-    p = ST_FMA(t, p, ST_EXP_6);
-    p = ST_FMA(t, p, ST_EXP_5);
-    p = ST_FMA(t, p, ST_EXP_4);
-    p = ST_FMA(t, p, ST_EXP_3);
-    p = ST_FMA(t, p, ST_EXP_2);
-    double e = t + (tlo + (t * t) * p);
-    double y = 1.0 + e;
-    int ki = (int)k;
-    if (ki > 1023)
-        return y * 0x1p1023 * st_pow2i(ki - 1023);
-    if (ki < -1021)
-        return (y * st_pow2i(ki + 1000)) * 0x1p-1000;
-    return y * st_pow2i(ki);
+        return 1.0 + x;                              /* exp(x) within a quarter ulp of 1 */
+    double kd = ST_RINT(x * ST_EXP_INVL);            /* |kd| <= 137612 < 2^18 */
+    int k = (int)kd;
+    int j = k & 127;
+    int m = (k - j) / 128;
+    /* r = x - kd ln2/128 as rh + rl.  x - kd*L1 is exact: kd*L1 has at most
+       53 significant bits, and both terms are multiples of 2^-61 (or kd = 0)
+       with a difference below 2^-8. */
+    double r1 = x - kd * ST_EXP_L1;
+    st_dd p2 = st_two_prod(kd, ST_EXP_L2);          /* exact */
+    st_dd s = st_two_sum(r1, -p2.hi);
+    double lo = (s.lo - p2.lo) - kd * ST_EXP_L3;     /* error < 2^-112 */
+    st_dd r = st_fast_two_sum(s.hi, lo);
+    double rh = r.hi, rl = r.lo;                     /* |rh| < 2^-8.52 */
+    if (phase == 1)
+        return st_exp_accurate(rh, rl, j, m);
+    /* q = rh^2/2 + ... + rh^6/720: |q| < 2^-18, error < 2^-69.4; the
+       omitted r^7/7! is below 2^-71.9 */
+    double q = ST_FMA(rh, ST_EXP_6, ST_EXP_5);
+    q = ST_FMA(rh, q, ST_EXP_4);
+    q = ST_FMA(rh, q, ST_EXP_3);
+    q = ST_FMA(rh, q, ST_EXP_2);
+    q = (rh * rh) * q;
+    /* exp(r) - 1 = rh + sl, sl = rl + rh*rl + q (rl^2/2 < 2^-124 omitted) */
+    double sl = rl + (rh * rl + q);
+    double th = st_exp_tab[3 * j], tm = st_exp_tab[3 * j + 1];
+    /* (th + tm)(1 + rh + sl) = th + th*rh + [th*sl + tm + tm*(rh + sl)] */
+    st_dd ph = st_two_prod(th, rh);                  /* exact */
+    st_dd hs = st_fast_two_sum(th, ph.hi);           /* exact, th >= 1 > |ph| */
+    double small = hs.lo + (ph.lo + (th * sl + (tm + tm * rh)));   /* |small| < 2^-16.9, error < 2^-66.3 */
+    st_dd y = st_fast_two_sum(hs.hi, small);
+    double res;
+    if (st_exp_round(y.hi, y.lo, m, ST_EXP_EPS1, &res))
+        return res;
+    if (phase == 2)
+        return x - x + (x - x) / (x - x);            /* nan: undecided */
+    return st_exp_accurate(rh, rl, j, m);
 }
+
+ST_FN double st_exp(double x) { return st_exp_phase(x, 0); }
 
 /* --------------------------------------------------------------- powi */
 

@@ -3,6 +3,7 @@ the oracle and the kernels use, against mpmath, scipy, glibc and published
 known-answer vectors."""
 import ctypes
 import math
+import zlib
 
 import mpmath
 import numpy as np
@@ -89,18 +90,90 @@ def test_special_values(lib):
     assert lib.orc_log(0.0) == -math.inf and lib.orc_asin(-0.0) == 0.0
 
 
-def test_agrees_with_glibc(lib, glibc_lib):
-    """The project libm and the C library agree to within one ulp (both < 1 ulp
-    from the exact value); identical results in the large majority of cases."""
-    xs = RNG.uniform(-3, 3, 5000)
-    same = 0
-    for name in ("orc_sin", "orc_cos", "orc_exp"):
-        a, b = getattr(lib, name), getattr(glibc_lib, name)
-        for x in xs:
-            u, v = a(float(x)), b(float(x))
-            assert abs(np.float64(u).view(np.int64) - np.float64(v).view(np.int64)) <= 1
-            same += u == v
-    assert same > 0.9 * 3 * len(xs)
+def correctly_rounded(fn, x):
+    """fn(x) rounded once to the nearest double (ties to even), subnormals
+    on the 2^-1074 grid."""
+    with mpmath.workprec(300):
+        v = fn(mpmath.mpf(x))
+        if abs(v) < mpmath.mpf(2) ** -1022:
+            n = v * mpmath.mpf(2) ** 1074
+            fl = mpmath.floor(n)
+            fr = n - fl
+            if fr > 0.5 or (fr == 0.5 and int(fl) % 2 == 1):
+                fl += 1
+            return float(fl * mpmath.mpf(2) ** -1074)
+        return float(v)
+
+
+EXP_ARGS = {
+    "beam-beam [-18, 0]": RNG.uniform(-18, 0, 6000),
+    "full range": RNG.uniform(-745.2, 709.79, 2000),
+    "near 0": RNG.uniform(-1e-3, 1e-3, 1000) * 10 ** RNG.uniform(-12, 0, 1000),
+    "subnormal results": RNG.uniform(-745.2, -708.3, 1000),
+    "near overflow": RNG.uniform(709.7, 709.79, 300),
+}
+
+
+@pytest.mark.parametrize("rng_name", list(EXP_ARGS))
+def test_exp_correctly_rounded(lib, rng_name):
+    """st_exp (Tang's table reduction + Ziv's rounding test) returns the
+    correctly rounded exp; so does its accurate phase on its own, and the
+    fast phase either agrees or leaves the rounding undecided (nan)."""
+    undecided = 0
+    for x in EXP_ARGS[rng_name]:
+        x = float(x)
+        want = correctly_rounded(mpmath.exp, x)
+        assert lib.orc_exp(x) == want, f"exp({x.hex()}) = {lib.orc_exp(x)!r}, want {want!r}"
+        assert lib.orc_exp_phase(x, 1) == want, f"accurate phase at {x.hex()}"
+        f = lib.orc_exp_phase(x, 2)
+        if f != f:
+            undecided += 1
+        else:
+            assert f == want, f"fast phase at {x.hex()}"
+    # the fast phase's error bound 2^-64 leaves about one call in 2^10 undecided
+    assert undecided <= 0.01 * len(EXP_ARGS[rng_name])
+
+
+def test_exp_special_values(lib):
+    assert lib.orc_exp(0.0) == 1.0 and lib.orc_exp(-0.0) == 1.0
+    assert lib.orc_exp(709.79) == math.inf and lib.orc_exp(math.inf) == math.inf
+    assert lib.orc_exp(-745.2) == 0.0 and lib.orc_exp(-math.inf) == 0.0
+    assert math.isnan(lib.orc_exp(math.nan))
+    assert lib.orc_exp(709.782712893383973096) == 1.7976931348622732e308   # largest finite result
+    assert lib.orc_exp(-745.1332191019411) == 5e-324                      # smallest subnormal
+    for x in (1.0, -1.0, 0.5, 2.0, 10.0, -10.0):
+        assert lib.orc_exp(x) == correctly_rounded(mpmath.exp, x)
+
+
+# Ceilings on how often the project libm returns another double than the C
+# library (glibc is not correctly rounded everywhere: on these ranges it
+# misrounds exp 0.08 %, asin 0.07-0.2 %, sin/cos 0.1-0.3 % of arguments).
+GLIBC_CEILING = [
+    # (function, argument range, largest share of arguments that may differ)
+    ("orc_exp", (-18.0, 0.0), 0.002),          # beam-beam kicks (BB4D/BB6D)
+    ("orc_exp", (-700.0, 700.0), 0.002),
+    ("orc_asin", (-1e-2, 1e-2), 0.002),        # exact bends: px/pz-sized arguments
+    ("orc_asin", (-1.0, 1.0), 0.03),
+    ("orc_sin", (-1e-2, 1e-2), 0.001),         # spin rotation angles per slice
+    ("orc_cos", (-1e-2, 1e-2), 0.001),
+    ("orc_sin", (-10.0, 10.0), 0.035),         # RF and crab phases
+    ("orc_cos", (-10.0, 10.0), 0.035),
+]
+
+
+@pytest.mark.parametrize("name,rng,ceiling", GLIBC_CEILING, ids=lambda v: str(v))
+def test_agrees_with_glibc(lib, glibc_lib, name, rng, ceiling):
+    """Per function and argument range: the project libm and the C library
+    agree to within one ulp, and return different doubles for at most the
+    stated share of arguments."""
+    xs = np.random.default_rng(zlib.crc32(f"{name}{rng}".encode())).uniform(rng[0], rng[1], 5000)
+    a, b = getattr(lib, name), getattr(glibc_lib, name)
+    differ = 0
+    for x in xs:
+        u, v = a(float(x)), b(float(x))
+        assert abs(int(np.float64(u).view(np.int64)) - int(np.float64(v).view(np.int64))) <= 1
+        differ += u != v
+    assert differ <= ceiling * len(xs), f"{name} on {rng}: {differ / len(xs):.3%} differ"
 
 
 def test_yoshida_constants():
