@@ -56,6 +56,11 @@ def build(verbose=False):
                        ("st_ssbb.hip", ["--offload-arch=" + ARCH]),
                        ("st_pic.hip", ["--offload-arch=" + ARCH]),
                        ("st_capi.cpp", ["--offload-arch=" + ARCH, "-x", "hip"]),
+                       ("st_lower.cpp", ["--offload-arch=" + ARCH, "-x", "hip"]),
+                       ("st_track.cpp", ["--offload-arch=" + ARCH, "-x", "hip"]),
+                       ("st_scan.cpp", ["--offload-arch=" + ARCH, "-x", "hip"]),
+                       ("st_batch.cpp", ["--offload-arch=" + ARCH, "-x", "hip"]),
+                       ("st_ss_host.cpp", ["--offload-arch=" + ARCH, "-x", "hip"]),
                        ("st_da.cpp", [])):
         obj = os.path.join(objdir, src.rsplit(".", 1)[0] + ".o")
         jobs.append(([hc, *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj], obj))
