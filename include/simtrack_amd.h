@@ -227,6 +227,23 @@ int st_element_pass(st_table *t, int elem_index, double *x, int np, int spin);
 #define ST_TRACK_BARE 4
 int st_track_device(st_table *t, double *d_x, int np, int turn0, int nturn,
                     int *d_state, int *d_lost, int flags);
+
+/*
+ * st_track_device without any allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as st_track_device, except that the survivor lists
+ * and compaction counts live in a fixed per-GPU scratch array held until the
+ * process ends (about 8.5 MB), and per-launch timing is not recorded.
+ * np <= ST_NOALLOC_MAX_PARTICLES, and with ST_TRACK_RADIAL the table has at
+ * most ST_NOALLOC_MAX_ELEMENTS elements; larger inputs return ST_ERR_ARG.
+ * Every call on one GPU shares that GPU's scratch: calls that overlap on one
+ * GPU (from several contexts or threads) race.
+ */
+#define ST_NOALLOC_MAX_PARTICLES (1 << 20)
+#define ST_NOALLOC_MAX_ELEMENTS 16384
+int st_track_device_noalloc(st_table *t, double *d_x, int np, int turn0, int nturn,
+                            int *d_state, int *d_lost, int flags);
 int st_ctx_sync(st_ctx *ctx);
 
 /* Seconds spent in the last blocking call's GPU work (HIP events). */

@@ -57,13 +57,16 @@ TYPES = {
 TRACK_SPIN = 1
 TRACK_RADIAL = 2
 TRACK_BARE = 4
+# limits of st_track_device_noalloc's fixed scratch (include/simtrack_amd.h)
+NOALLOC_MAX_PARTICLES = 1 << 20
+NOALLOC_MAX_ELEMENTS = 16384
 
 # every symbol declared in include/simtrack_amd.h
 EXPORTS = [
     "st_globals_default", "st_globals_set_energy", "st_abi_version", "st_struct_sizes",
     "st_last_error", "st_ctx_create", "st_ctx_destroy", "st_table_build", "st_table_destroy",
     "st_table_size", "st_track", "st_track_spin", "st_track_tbt", "st_track_radial",
-    "st_element_pass", "st_track_device", "st_ctx_sync", "st_last_gpu_seconds",
+    "st_element_pass", "st_track_device", "st_track_device_noalloc", "st_ctx_sync", "st_last_gpu_seconds",
     "st_ctx_set_timing", "st_ctx_kernel_stats", "st_da_angles", "st_da_bisect", "st_da_uniform",
     "st_fine_tune", "st_fine_tune_device", "st_batch_create", "st_batch_destroy", "st_batch_default_config",
     "st_batch_track", "st_batch_track_device", "st_batch_autotune", "st_mlp_create", "st_mlp_destroy",
@@ -131,6 +134,7 @@ def load(path=None):
     lib.st_track_tbt.argtypes = [P, P, I, I, P, I, I, P, P, P]
     lib.st_element_pass.argtypes = [P, I, P, I, I]
     lib.st_track_device.argtypes = [P, P, I, I, I, P, P, I]
+    lib.st_track_device_noalloc.argtypes = [P, P, I, I, I, P, P, I]
     lib.st_last_gpu_seconds.argtypes = [P, ctypes.POINTER(D)]
     lib.st_ctx_set_timing.argtypes = [P, I]
     lib.st_ctx_kernel_stats.argtypes = [P, ctypes.POINTER(D), ctypes.POINTER(ctypes.c_int64),
@@ -317,6 +321,13 @@ class Table:
         check(self.lib.st_track_device(self.h, ctypes.c_void_p(d_x), int(npart), int(turn0), int(nturn),
                                        ctypes.c_void_p(d_state), ctypes.c_void_p(d_lost), int(flags)),
               "st_track_device")
+
+    def track_device_noalloc(self, d_x, npart, turn0, nturn, d_state, d_lost, flags=0):
+        """As track_device, with the work buffers in the library's fixed
+        per-GPU scratch (st_track_device_noalloc): no device allocation."""
+        check(self.lib.st_track_device_noalloc(self.h, ctypes.c_void_p(d_x), int(npart), int(turn0), int(nturn),
+                                               ctypes.c_void_p(d_state), ctypes.c_void_p(d_lost), int(flags)),
+              "st_track_device_noalloc")
 
 
 class Batch:

@@ -120,6 +120,11 @@ extern "C" int st_ctx_create(int device, st_ctx **out)
         return fail(ST_ERR_DEVICE, "st_ctx_create: stream/event/pinned allocation failed");
     }
     init_yoshida();
+    int rc = noalloc_scratch_init(device);
+    if (rc) {
+        st_ctx_destroy(c);
+        return rc;
+    }
     *out = c;
     return ST_OK;
 }

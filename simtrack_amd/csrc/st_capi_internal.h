@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <memory>
+#include <mutex>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -112,6 +113,7 @@ ST_HIDDEN int timed_event(st_ctx *c, int *idx);
 ST_HIDDEN int turn_block(int n_elems, int np);
 ST_HIDDEN int track_device_core(st_table *t, double *d_x, int np, int turn0, int nturn, int *d_state, int *d_lost,
                                 bool spin, bool radial, double *d_tbt, int tbt_bpm, int tbt_nturn, bool bare = false);
+ST_HIDDEN int noalloc_scratch_init(int device);
 ST_HIDDEN int track_host(st_table *t, double *x, int np, int nturn, int *stable, int *lost_turn, int *lost_post,
                          bool spin, bool radial, double *x_tbt, int tbt_bpm, bool bare = false);
 

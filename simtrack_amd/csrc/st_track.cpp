@@ -36,11 +36,35 @@ int turn_block(int n_elems, int np)
 }
 
 /* device-side core: d_x/d_state/d_lost already on the device */
+/* The work buffers of one tracking call: the survivor index lists, the
+   per-block counts and total of their compaction and Track_Radial's x04.
+   Given (fixed != nullptr), nothing is allocated and no timing event is
+   recorded; otherwise the context's grow-only buffers are used. */
+struct track_scratch {
+    int *idx[2];
+    int *block;
+    int *total;
+    double *x04;
+};
+
+static int track_device_run(st_table *t, double *d_x, int np, int turn0, int nturn, int *d_state, int *d_lost,
+                            bool spin, bool radial, double *d_tbt, int tbt_bpm, int tbt_nturn, bool bare,
+                            const track_scratch *fixed);
+
 int track_device_core(st_table *t, double *d_x, int np, int turn0, int nturn, int *d_state, int *d_lost,
                              bool spin, bool radial, double *d_tbt, int tbt_bpm, int tbt_nturn, bool bare)
 {
+    return track_device_run(t, d_x, np, turn0, nturn, d_state, d_lost, spin, radial, d_tbt, tbt_bpm, tbt_nturn, bare,
+                            nullptr);
+}
+
+static int track_device_run(st_table *t, double *d_x, int np, int turn0, int nturn, int *d_state, int *d_lost,
+                            bool spin, bool radial, double *d_tbt, int tbt_bpm, int tbt_nturn, bool bare,
+                            const track_scratch *fixed)
+{
     st_ctx *c = t->ctx;
-    if (ensure_particles(c, (size_t)np, 1)) return ST_ERR_NOMEM;
+    if (!fixed && ensure_particles(c, (size_t)np, 1)) return ST_ERR_NOMEM;
+    const bool timing = c->timing && !fixed;
     st_track_args a = base_args(t, spin);
     if (bare) a.ap = nullptr; /* Element::Pass turn after turn: no loss test */
     a.x = d_x;
@@ -50,7 +74,11 @@ int track_device_core(st_table *t, double *d_x, int np, int turn0, int nturn, in
     a.tbt_bpm = tbt_bpm;
     a.tbt_nturn = tbt_nturn;
     a.tbt_turn0 = 0;
-    if (radial) {
+    if (radial && fixed) {
+        st_track_args ar = base_args(t, false);
+        HIPCHK(st_launch_radial_x04(ar, fixed->x04, t->gp.H_expand != 0, c->stream));
+        a.x04 = fixed->x04;
+    } else if (radial) {
         if ((size_t)t->n > c->cap_x04) {
             hipFree(c->d_x04);
             c->d_x04 = nullptr;
@@ -66,7 +94,9 @@ int track_device_core(st_table *t, double *d_x, int np, int turn0, int nturn, in
     int tb = turn_block(t->n, np);
     int nact = np;
     const int *idx = nullptr;
-    int *bufs[2] = {c->d_idx0, c->d_idx1};
+    int *bufs[2] = {fixed ? fixed->idx[0] : c->d_idx0, fixed ? fixed->idx[1] : c->d_idx1};
+    int *d_block = fixed ? fixed->block : c->d_block;
+    int *d_total = fixed ? fixed->total : c->d_total;
     int which = 0;
     bool first = true;
     for (int t0 = turn0; t0 < turn0 + nturn && nact > 0;) {
@@ -77,13 +107,13 @@ int track_device_core(st_table *t, double *d_x, int np, int turn0, int nturn, in
         a.idx = idx;
         a.nact = nact;
         int ev = -1;
-        if (c->timing) {
+        if (timing) {
             int rc2 = timed_event(c, &ev);
             if (rc2) return rc2;
             HIPCHK(hipEventRecord(c->ev_pool[ev], c->stream));
         }
         HIPCHK(st_launch_track(a, spin, hexp, rad, c->stream));
-        if (c->timing) {
+        if (timing) {
             HIPCHK(hipEventRecord(c->ev_pool[ev + 1], c->stream));
             c->pending.push_back({ev, ev + 1});
             c->pending_work.push_back((int64_t)nact * nt);
@@ -92,10 +122,10 @@ int track_device_core(st_table *t, double *d_x, int np, int turn0, int nturn, in
         t0 += nt;
         if (t0 < turn0 + nturn) {
             int *out = bufs[which];
-            HIPCHK(st_launch_compact(idx, nact, d_state, c->d_block, out, c->d_total, c->stream));
-            HIPCHK(hipMemcpyAsync(c->h_total, c->d_total, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(st_launch_compact(idx, nact, d_state, d_block, out, d_total, c->stream));
+            HIPCHK(hipMemcpyAsync(c->h_total, d_total, sizeof(int), hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
-            if (c->timing) {
+            if (timing) {
                 int rc2 = drain_timing(c);
                 if (rc2) return rc2;
             }
@@ -234,4 +264,64 @@ extern "C" int st_element_pass(st_table *t, int elem_index, double *x, int np, i
     HIPCHK(hipMemcpyAsync(x, c->d_x, (size_t)np * ncol * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return ST_OK;
+}
+
+/* ------------------------------------------- allocation-free tracking */
+
+/* Scratch of st_track_device_noalloc, one copy per GPU, held until the
+   process ends: Track_Radial's x04 (ST_NOALLOC_MAX_ELEMENTS doubles), then
+   two survivor index lists of ST_NOALLOC_MAX_PARTICLES ints, the per-block
+   counts of their compaction (one int per 1,024 particles, plus one) and the
+   survivor total.  Each launch writes what the next one reads, so nothing in
+   it has to start at zero. */
+#define NA_BLOCKS (ST_NOALLOC_MAX_PARTICLES / 1024 + 1)
+#define NA_INTS (2 * (size_t)ST_NOALLOC_MAX_PARTICLES + NA_BLOCKS + 1)
+#define NA_DOUBLES ((size_t)ST_NOALLOC_MAX_ELEMENTS + (NA_INTS + 1) / 2)
+static __device__ __align__(256) double scratch[NA_DOUBLES];
+
+/* device address of each GPU's scratch (the array's name is only a host-side
+   stand-in), looked up once per device by st_ctx_create */
+#define NA_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[NA_MAX_DEVICES];
+
+int noalloc_scratch_init(int device)
+{
+    if (device < 0 || device >= NA_MAX_DEVICES)
+        return fail(ST_ERR_ARG, "device %d outside the %d the allocation-free path supports", device, NA_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return ST_OK;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)); /* the current device's copy */
+    if (e != hipSuccess || !ptr)
+        return fail(ST_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_scratch_ptr[device] = (double *)ptr;
+    return ST_OK;
+}
+
+extern "C" int st_track_device_noalloc(st_table *t, double *d_x, int np, int turn0, int nturn, int *d_state,
+                                       int *d_lost, int flags)
+{
+    if (!t || np < 0 || nturn < 0 || turn0 < 0 || (np > 0 && (!d_x || !d_state || !d_lost)))
+        return fail(ST_ERR_ARG, "st_track_device_noalloc: bad argument");
+    if (np > ST_NOALLOC_MAX_PARTICLES)
+        return fail(ST_ERR_ARG, "st_track_device_noalloc: %d particles exceed the scratch maximum of %d", np,
+                    ST_NOALLOC_MAX_PARTICLES);
+    if ((flags & ST_TRACK_RADIAL) && t->n > ST_NOALLOC_MAX_ELEMENTS)
+        return fail(ST_ERR_ARG, "st_track_device_noalloc: radial tracking of %d elements exceeds the scratch "
+                    "maximum of %d", t->n, ST_NOALLOC_MAX_ELEMENTS);
+    if (np == 0) return ST_OK;
+    const int dev = t->ctx->device;
+    HIPCHK(hipSetDevice(dev));
+    double *base = (dev >= 0 && dev < NA_MAX_DEVICES) ? g_scratch_ptr[dev] : nullptr;
+    if (!base) return fail(ST_ERR_DEVICE, "st_track_device_noalloc: no scratch address for device %d", dev);
+    track_scratch s;
+    s.x04 = base;
+    int *ints = (int *)(base + ST_NOALLOC_MAX_ELEMENTS);
+    s.idx[0] = ints;
+    s.idx[1] = ints + ST_NOALLOC_MAX_PARTICLES;
+    s.block = ints + 2 * (size_t)ST_NOALLOC_MAX_PARTICLES;
+    s.total = s.block + NA_BLOCKS;
+    return track_device_run(t, d_x, np, turn0, nturn, d_state, d_lost, (flags & ST_TRACK_SPIN) != 0,
+                            (flags & ST_TRACK_RADIAL) != 0, nullptr, -1, 0, (flags & ST_TRACK_BARE) != 0, &s);
 }
