@@ -33,7 +33,36 @@ def cpu_baseline(gp, records, x, spin, sample_particles, sample_turns, threads):
     pturns = _tracked_turns(stable, lt, 0, sample_turns, prelost=None)
     return {"value": pturns / dt, "unit": "particle-turns/s", "cores": threads, "kind": "port",
             "sample": f"{sample_particles} particles x {sample_turns} turns of the same workload "
-                      f"(oracle restatement, OpenMP over particles), {dt:.1f} s"}
+                      f"(oracle restatement, OpenMP over particles), {dt:.1f} s",
+            "host": host_info()}
+
+
+def host_threads():
+    """Threads for the CPU baseline: the cores this process may run on, at
+    most 16 (a one-GPU box's CPU share, whatever nproc shows there)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def host_info():
+    """nproc, the affinity mask's size and the CPU model of this host."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = None
+    return {"nproc": os.cpu_count(), "affinity": aff, "model": model}
 
 
 def _tracked_turns(stable, lost_turn, t_begin, t_end, prelost):
@@ -55,6 +84,9 @@ def main():
     ap.add_argument("--turns-per-step", type=int, default=20)
     ap.add_argument("--particles", type=int, default=65536)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--perf-mode", action="store_true",
+                    help="config1/config4: the perf-mode kernels (FMA contraction, division by reciprocal "
+                         "multiplication; results differ from the reference's operations by rounding)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (a kernel of known traffic for "
                          "calibrating the HBM byte counters)")
@@ -83,7 +115,7 @@ def main():
         sys.exit(2)
 
     if args.cpu_sample is None:
-        args.cpu_sample = "2048x24" if args.workload == "config4" else "1024x64"
+        args.cpu_sample = {"config4": "2048x24", "config2": "2048x32"}.get(args.workload, "1024x64")
     if args.workload == "config2":
         return bench_config2(args)
     if args.workload == "config3":
@@ -119,6 +151,11 @@ def bench_tracking(args, gp, rec, x, spin, config, traffic, data):
 
     npart = len(x)
     flags = st.TRACK_SPIN if spin else 0
+    if getattr(args, "perf_mode", False):
+        flags |= st.TRACK_PERF
+        config = dict(config, arithmetic="perf mode: FMA contraction and reciprocal division (not bit-equal)")
+    else:
+        config = dict(config, arithmetic="bit-exact: the reference's operations, no contraction")
     f_turn = flops.flops_per_turn(rec, gp, spin=spin)
 
     ctx = st.Context(0)
@@ -177,11 +214,11 @@ def bench_tracking(args, gp, rec, x, spin, config, traffic, data):
     baseline = None
     if not args.no_cpu_baseline:
         sp, stn = (int(v) for v in args.cpu_sample.split("x"))
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         baseline = cpu_baseline(gp, rec, x, spin, sp, stn, threads)
 
     ncol = 9 if spin else 6
-    kname = "k_track<spin>" if spin else "k_track"
+    kname = ("k_track<spin>" if spin else "k_track") + (" (perf mode)" if flags & st.TRACK_PERF else "")
     out = {
         "metric": "FP64 particle-turns/s",
         "value": value,
@@ -196,9 +233,8 @@ def bench_tracking(args, gp, rec, x, spin, config, traffic, data):
         "dtype": "f64",
         "data": data,
         "config": config,
-        # compute-bound: the roof is MI355X's FP64 peak (78.6 TFLOP/s, the same figure for
-        # the vector and matrix pipes); the kernel runs on the FP64 vector pipe
-        "roofline": {"bound": "mfma", "pipe": "fp64 VALU", "achieved": achieved / 1e12, "peak": peak / 1e12,
+        # compute-bound on the FP64 vector ALU: the roof is MI355X's FP64 vector peak, 78.6 TFLOP/s
+        "roofline": {"bound": "fp64_valu", "pipe": "fp64 VALU", "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
                      "algorithmic_bytes": 2 * ncol * 8 * npart,
                      "f_turn_flops": f_turn, "kernel": kname,
@@ -388,6 +424,7 @@ def bench_config2(args):
     f_seed = np.array([flops.flops_per_turn(ln.records(), gp) for ln in lines], dtype=float)
     batch = st.Batch(ctx, tabs)
     best, table = batch.autotune(x, offs, 32)
+    bflags = st.TRACK_PERF if args.perf_mode else 0
     dev = torch.device("cuda:0")
     T = args.turns_per_step
 
@@ -402,7 +439,7 @@ def bench_config2(args):
         t = turn0
         for _ in range(nsteps):
             if batched:
-                batch.track_device(d_x.data_ptr(), offs, t, T, d_s.data_ptr(), d_l.data_ptr(), 0, best)
+                batch.track_device(d_x.data_ptr(), offs, t, T, d_s.data_ptr(), d_l.data_ptr(), bflags, best)
             else:
                 for k, tb in enumerate(tabs):
                     o = int(offs[k])
@@ -431,6 +468,12 @@ def bench_config2(args):
         per = np.where(s0 == 0, 0, per)
         flops_done = sum(float(per[offs[k]:offs[k + 1]].sum()) * f_seed[k] for k in range(len(tabs)))
         results[mode] = dict(pturns=int(per.sum()), wall=wall, k_sec=k_sec, launches=k_launch, flops=flops_done)
+    baseline = None
+    if not args.no_cpu_baseline:
+        sp, stn = (int(v) for v in args.cpu_sample.split("x"))
+        o0, o1 = int(offs[0]), int(offs[1])
+        baseline = cpu_baseline(gp, lines[0].records(), x[o0:o1], False, min(sp, o1 - o0), stn, host_threads())
+        baseline["sample"] = "lattice 0: " + baseline["sample"]
     r = results["batched"]
     value = r["pturns"] / r["wall"]
     achieved = r["flops"] / r["k_sec"] if r["k_sec"] > 0 else float("nan")
@@ -445,15 +488,16 @@ def bench_config2(args):
                                f"{args.per_seed} DA-grid particles each", "turns_per_step": T,
                    "autotuned": {"wg_waves": best.wg_waves, "waves_per_simd": best.waves_per_simd,
                                  "turn_block": best.turn_block}},
-        "roofline": {"bound": "mfma", "pipe": "fp64 VALU", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None, "kernel": "k_track_batch",
+        "roofline": {"bound": "fp64_valu", "pipe": "fp64 VALU", "achieved": achieved / 1e12, "peak": peak / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                     "kernel": "k_track_batch" + (" (perf mode)" if args.perf_mode else ""),
                      "launches": r["launches"]},
         "per_lattice_launches": {"value": results["per_lattice"]["pturns"] / results["per_lattice"]["wall"],
                                  "launches": results["per_lattice"]["launches"],
                                  "speedup_of_batched": (value / (results["per_lattice"]["pturns"] /
                                                                  results["per_lattice"]["wall"]))},
         "autotune_seconds": {f"{c[0]}x{c[1]}x{c[2]}": round(t_, 5) for c, t_ in table},
-        "cpu_baseline": None,
+        "cpu_baseline": baseline,
     }
     print(json.dumps(out))
 

@@ -219,12 +219,16 @@ int st_element_pass(st_table *t, int elem_index, double *x, int np, int spin);
  * tracked; the 0.1 m pre-check is applied only when turn0 == 0, so a run
  * split into ranges equals one call.  flags: ST_TRACK_SPIN, ST_TRACK_RADIAL,
  * ST_TRACK_BARE (no pre-check and no loss test: Element::Pass turn after
- * turn, as Cal_Orbit_Num, simtrack.cpp:14970-14982, passes particles).
+ * turn, as Cal_Orbit_Num, simtrack.cpp:14970-14982, passes particles),
+ * ST_TRACK_PERF (the perf-mode kernels: the same maps with FMA contraction
+ * and division by a reciprocal multiplication, so results differ from the
+ * reference's operations by rounding; never the default).
  * Returns after launching on the context stream; st_ctx_sync waits.
  */
 #define ST_TRACK_SPIN 1
 #define ST_TRACK_RADIAL 2
 #define ST_TRACK_BARE 4
+#define ST_TRACK_PERF 8
 int st_track_device(st_table *t, double *d_x, int np, int turn0, int nturn,
                     int *d_state, int *d_lost, int flags);
 

@@ -4,6 +4,7 @@
 #   liboracle_glibc.so  C library libm: what the reference itself calls
 #   liboracle_glibc_{trig,asin,exp,pow}.so  the C library for one function
 #                       family only (sin/cos/tan, asin, exp, pow)
+#   liboracle_opcount.so  the maps' algorithmic operation counts (opcount.cpp)
 set -e
 here=$(cd "$(dirname "$0")" && pwd)
 root=$(dirname "$here")
@@ -19,3 +20,7 @@ for v in trig:1 asin:2 exp:4 pow:8; do
   gcc $CFLAGS -DORACLE_GLIBC_MASK=$mask $INC "$here/st_oracle.c" -o "$here/build/liboracle_glibc_$name.so.tmp" -lm
   mv "$here/build/liboracle_glibc_$name.so.tmp" "$here/build/liboracle_glibc_$name.so"
 done
+# op-counting build (C++: st_oracle.c with a tracked scalar type), for the F_turn test
+g++ -x c++ -std=c++17 -O1 -ffp-contract=off -mfma -fPIC -shared -fvisibility=hidden -w $INC -I"$here" \
+    "$here/opcount.cpp" -o "$here/build/liboracle_opcount.so.tmp"
+mv "$here/build/liboracle_opcount.so.tmp" "$here/build/liboracle_opcount.so"

@@ -277,3 +277,29 @@ def rad2_sample(seed, n, draw, nr):
 def rad2_energy(r):
     """Photon energy / critical energy from a uniform r (radiate2's inverse CDF)."""
     return load().orc_rad2_energy(float(r))
+
+
+# --------------------------------------------------- operation counts
+
+_opc = None
+
+
+def opcount(records, index, gp, x, spin=False):
+    """Algorithmic operations of element `index`'s map on particle state x
+    (oracle/opcount.cpp): (operations, pt-derived operations, pt-cache
+    updates)."""
+    global _opc
+    if _opc is None:
+        path = os.path.join(HERE, "build", "liboracle_opcount.so")
+        if not os.path.exists(path):
+            build()
+        _opc = ctypes.CDLL(path)
+        _opc.opc_element_ops.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_void_p]
+        _opc.opc_element_ops.restype = ctypes.c_int
+    rec = np.ascontiguousarray(records[index:index + 1])
+    xx = np.ascontiguousarray(x, dtype=np.float64)
+    c = np.zeros(3, np.int64)
+    rc = _opc.opc_element_ops(_p(rec), ctypes.byref(gp), _p(xx), 1 if spin else 0, _p(c))
+    if rc != 0:
+        raise RuntimeError(f"opcount failed ({rc})")
+    return int(c[0]), int(c[1]), int(c[2])

@@ -72,6 +72,14 @@
 #endif
 
 #define EXPORT __attribute__((visibility("default")))
+/* hooks of the op-counting build (oracle/opcount.cpp); no-ops here */
+#ifndef ORC_UNCOUNTED_IF
+#define ORC_UNCOUNTED_IF(cond)
+#endif
+#ifndef ORC_PT_CACHED
+#define ORC_PT_CACHED(pt, v) (v)
+#define ORC_PT_DERIVED(v) (v)
+#endif
 
 enum { x_ = 0, px_ = 1, y_ = 2, py_ = 3, z_ = 4, pt_ = 5 };
 
@@ -341,6 +349,7 @@ static void LtoG(double x[], double length, double angle, double dx, double dy, 
 /* DRIFT_Pass, simtrack.cpp:3743-3766 */
 static void DRIFT_Pass(double x[], double L)
 {
+    ORC_UNCOUNTED_IF(L == 0.);      /* op-counting build only: a zero-length drift counts nothing */
     if (GP.H_expand) {
         double u = L / (1.0 + x[5]);
         x[x_] = x[x_] + x[px_] * u;
@@ -348,9 +357,9 @@ static void DRIFT_Pass(double x[], double L)
         x[z_] = x[z_] - (x[px_] * x[px_] + x[py_] * x[py_]) * u / 2.0 / (1 + x[5]);
     } else {
         double pz, betaz, delta1, gamma1, beta1;
-        delta1 = sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0;
-        gamma1 = GP.gamma + sqrt(GP.gamma * GP.gamma - 1.) * x[pt_];
-        beta1 = sqrt(1.0 - 1.0 / gamma1 / gamma1);
+        delta1 = ORC_PT_CACHED(x[pt_], sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0);
+        gamma1 = ORC_PT_CACHED(x[pt_], GP.gamma + sqrt(GP.gamma * GP.gamma - 1.) * x[pt_]);
+        beta1 = ORC_PT_DERIVED(sqrt(1.0 - 1.0 / gamma1 / gamma1));
         pz = sqrt((1.0 + delta1) * (1.0 + delta1) - x[px_] * x[px_] - x[py_] * x[py_]);
         betaz = pz * beta1 / (1.0 + delta1);
         x[x_] = x[x_] + x[px_] * L / pz;
@@ -368,9 +377,9 @@ static void PATCH_Pass(double x[], double DX, double DY, double DS, double THETA
     x[4] = x[4] - DS;
     if (fabs(THETA) < 1.0e-9) return;
     double xtemp[6], temp, pz;
-    double delta1 = sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0;
-    double gamma1 = GP.gamma + sqrt(GP.gamma * GP.gamma - 1.) * x[pt_];
-    double beta1 = sqrt(1.0 - 1.0 / gamma1 / gamma1);
+    double delta1 = ORC_PT_CACHED(x[pt_], sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0);
+    double gamma1 = ORC_PT_CACHED(x[pt_], GP.gamma + sqrt(GP.gamma * GP.gamma - 1.) * x[pt_]);
+    double beta1 = ORC_PT_DERIVED(sqrt(1.0 - 1.0 / gamma1 / gamma1));
     double cosT = OCOS(THETA), sinT = OSIN(THETA), tanT = OTAN(THETA);
     for (int i = 0; i < 6; i++) xtemp[i] = x[i];
     pz = sqrt((1 + delta1) * (1 + delta1) - xtemp[1] * xtemp[1] - xtemp[3] * xtemp[3]);
@@ -403,7 +412,7 @@ static void sbend_exact_pass(double x[], double L, double Angle, double cosAngle
     double href = Angle / L;
     double x0[6], pz0, pz1, delta, gamma, beta, derivative, temp, arcsin1, arcsin2;
     for (int i = 0; i < 6; i++) x0[i] = x[i];
-    delta = sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0;
+    delta = ORC_PT_CACHED(x[pt_], sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0);
     gamma = GP.gamma + sqrt(GP.gamma * GP.gamma - 1) * x[pt_];
     beta = sqrt(1.0 - 1.0 / gamma / gamma);
     pz0 = sqrt((1. + delta) * (1. + delta) - x0[1] * x0[1] - x0[3] * x0[3]);
@@ -427,7 +436,7 @@ static void general_sbend_exact_pass(double x[], double L, double Angle, double 
     double b0 = K0L / L;
     double x0[6], pz0, pz1, delta, gamma, beta, derivative, temp, arcsin1, arcsin2;
     for (int i = 0; i < 6; i++) x0[i] = x[i];
-    delta = sqrt(1.0 + 2 * x[5] / GP.beta + x[5] * x[5]) - 1.0;
+    delta = ORC_PT_CACHED(x[5], sqrt(1.0 + 2 * x[5] / GP.beta + x[5] * x[5]) - 1.0);
     gamma = GP.gamma + sqrt(GP.gamma * GP.gamma - 1) * x[5];
     beta = sqrt(1.0 - 1.0 / gamma / gamma);
     pz0 = sqrt((1. + delta) * (1. + delta) - x0[1] * x0[1] - x0[3] * x0[3]);
@@ -593,9 +602,9 @@ static void radiate(double x[], double L, double href, const double BLbrho[])
     double x0[6], delta1, gamma1, beta1, delta2, pz, xp, yp, temp1, temp2, ratio;
     double u[3], BLper[3], BLper2;
     for (int i = 0; i < 6; i++) x0[i] = x[i];
-    delta1 = sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0;
-    gamma1 = GP.gamma + sqrt(GP.gamma * GP.gamma - 1) * x[pt_];
-    beta1 = sqrt(1.0 - 1.0 / gamma1 / gamma1);
+    delta1 = ORC_PT_CACHED(x[pt_], sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0);
+    gamma1 = ORC_PT_CACHED(x[pt_], GP.gamma + sqrt(GP.gamma * GP.gamma - 1) * x[pt_]);
+    beta1 = ORC_PT_DERIVED(sqrt(1.0 - 1.0 / gamma1 / gamma1));
     (void)beta1;
     pz = sqrt((1 + delta1) * (1 + delta1) - x[px_] * x[px_] - x[py_] * x[py_]);
     xp = (1 + x[x_] * href) * x[px_] / pz;
@@ -608,7 +617,7 @@ static void radiate(double x[], double L, double href, const double BLbrho[])
     BLper2 = BLper[0] * BLper[0] + BLper[1] * BLper[1] + BLper[2] * BLper[2];
     temp2 = Cr * (1 + delta1) * (1 + delta1) * OPOW(GP.energy / 1000., 3) / 2 / PI;
     x[pt_] = x[pt_] - temp2 * BLper2 * temp1 / L;
-    delta2 = sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0;
+    delta2 = ORC_PT_CACHED(x[pt_], sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0);
     ratio = (1 + delta2) / (1 + delta1);
     x[px_] = x0[px_] * ratio;
     x[py_] = x0[py_] * ratio;
@@ -620,9 +629,9 @@ static void radiate1(double x[], double L, double href, const double BLbrho[], o
     double x0[6], delta1, gamma1, beta1, delta2, pz, xp, yp, temp1, temp2, ratio;
     double u[3], BLper[3], BLper2, g1, dtdl, ptloss, ptfluct, randgaus;
     for (int i = 0; i < 6; i++) x0[i] = x[i];
-    delta1 = sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0;
-    gamma1 = GP.gamma + sqrt(GP.gamma * GP.gamma - 1) * x[pt_];
-    beta1 = sqrt(1.0 - 1.0 / gamma1 / gamma1);
+    delta1 = ORC_PT_CACHED(x[pt_], sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0);
+    gamma1 = ORC_PT_CACHED(x[pt_], GP.gamma + sqrt(GP.gamma * GP.gamma - 1) * x[pt_]);
+    beta1 = ORC_PT_DERIVED(sqrt(1.0 - 1.0 / gamma1 / gamma1));
     pz = sqrt((1 + delta1) * (1 + delta1) - x[px_] * x[px_] - x[py_] * x[py_]);
     xp = (1 + x[x_] * href) * x[px_] / pz;
     yp = (1 + x[x_] * href) * x[py_] / pz;
@@ -640,7 +649,7 @@ static void radiate1(double x[], double L, double href, const double BLbrho[], o
     rng->counter++;
     ptfluct = sqrt(Cf * gamma1 * gamma1 * gamma1 * gamma1 * gamma1 * g1 * g1 * g1 * dtdl * L) * randgaus;
     x[pt_] = x[pt_] - ptloss + ptfluct;
-    delta2 = sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0;
+    delta2 = ORC_PT_CACHED(x[pt_], sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0);
     ratio = (1.0 + delta2) / (1 + delta1);
     x[px_] = x0[px_] * ratio;
     x[py_] = x0[py_] * ratio;
@@ -662,9 +671,9 @@ static void radiate2(double x[], double L, double href, const double BLbrho[], o
     double x0[6], delta1, gamma1, beta1, delta2, pz, xp, yp, temp1, ratio;
     double u[3], BLper[3], BLper2, rho1, Nr, Etot, Ec;
     for (int i = 0; i < 6; i++) x0[i] = x[i];
-    delta1 = sqrt(1.0 + 2 * x[5] / GP.beta + x[5] * x[5]) - 1.0;
-    gamma1 = GP.gamma + sqrt(GP.gamma * GP.gamma - 1) * x[5];
-    beta1 = sqrt(1.0 - 1.0 / gamma1 / gamma1);
+    delta1 = ORC_PT_CACHED(x[5], sqrt(1.0 + 2 * x[5] / GP.beta + x[5] * x[5]) - 1.0);
+    gamma1 = ORC_PT_CACHED(x[5], GP.gamma + sqrt(GP.gamma * GP.gamma - 1) * x[5]);
+    beta1 = ORC_PT_DERIVED(sqrt(1.0 - 1.0 / gamma1 / gamma1));
     pz = sqrt((1 + delta1) * (1 + delta1) - x[1] * x[1] - x[3] * x[3]);
     xp = (1 + x[0] * href) * x[1] / pz;
     yp = (1 + x[0] * href) * x[1] / pz;
@@ -681,7 +690,7 @@ static void radiate2(double x[], double L, double href, const double BLbrho[], o
     rng->counter++;
     Etot = Etot * Ec;
     x[5] = x[5] - Etot / GP.energy;
-    delta2 = sqrt(1.0 + 2 * x[5] / GP.beta + x[5] * x[5]) - 1.0;
+    delta2 = ORC_PT_CACHED(x[5], sqrt(1.0 + 2 * x[5] / GP.beta + x[5] * x[5]) - 1.0);
     ratio = (1 + delta2) / (1 + delta1);
     x[1] = x0[1] * ratio;
     x[3] = x0[3] * ratio;
@@ -693,8 +702,8 @@ static void radiate2(double x[], double L, double href, const double BLbrho[], o
 static void spin(double x[], double angle, double href, const double BLbrho[])
 {
     double u[3], Bpar[3], udotB, temp;
-    double delta1 = sqrt(1.0 + 2. * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0;
-    double gamma1 = GP.gamma + sqrt(GP.gamma * GP.gamma - 1.) * x[pt_];
+    double delta1 = ORC_PT_CACHED(x[pt_], sqrt(1.0 + 2. * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0);
+    double gamma1 = ORC_PT_CACHED(x[pt_], GP.gamma + sqrt(GP.gamma * GP.gamma - 1.) * x[pt_]);
     double Gr = GP.G * gamma1;
     double F[3], Fabs, uF[3], theta, costheta, sintheta, rot[9], spin0[3], sp[3];
     if (BLbrho[0] * BLbrho[0] + BLbrho[1] * BLbrho[1] + BLbrho[2] * BLbrho[2] != 0.) {
@@ -1905,8 +1914,9 @@ static int elem_pass(const st_element *e, double x[], orc_rng *rng)
         LtoG(x, e->l, e->angle, MIS(e));
         return 0;
     case ST_KICKER: case ST_HKICKER: case ST_VKICKER: {
-        double hk = e->type == ST_VKICKER ? 0. : e->hkick;
-        double vk = e->type == ST_HKICKER ? 0. : e->vkick;
+        double hk = 0., vk = 0.;
+        if (e->type != ST_VKICKER) hk = e->hkick;
+        if (e->type != ST_HKICKER) vk = e->vkick;
         GtoL(x, e->l, 0., MIS(e));
         KICK_Pass(x, e->l, hk, vk);
         LtoG(x, e->l, 0., MIS(e));
@@ -2014,8 +2024,9 @@ static int elem_spass(const st_element *e, double x[])
         LtoG(x, e->l, e->angle, MIS(e));
         return 0;
     case ST_KICKER: case ST_HKICKER: case ST_VKICKER: {
-        double hk = e->type == ST_VKICKER ? 0. : e->hkick;
-        double vk = e->type == ST_HKICKER ? 0. : e->vkick;
+        double hk = 0., vk = 0.;
+        if (e->type != ST_VKICKER) hk = e->hkick;
+        if (e->type != ST_HKICKER) vk = e->vkick;
         GtoL(x, e->l, 0., MIS(e));
         KICK_sPass(x, e->l, hk, vk);
         LtoG(x, e->l, 0., MIS(e));
@@ -2222,7 +2233,7 @@ EXPORT double orc_cos(double x) { return OCOS(x); }
 EXPORT double orc_tan(double x) { return OTAN(x); }
 EXPORT double orc_asin(double x) { return OASIN(x); }
 EXPORT double orc_exp(double x) { return OEXP(x); }
-#ifndef ORACLE_GLIBC
+#if !defined(ORACLE_GLIBC) && !defined(ORC_OPCOUNT)
 /* the branch-free common paths the GPU kernels evaluate first */
 EXPORT double orc_exp_phase(double x, int phase) { return st_exp_phase(x, phase); }
 EXPORT double orc_asin_small(double x) { return st_asin_small(x); }

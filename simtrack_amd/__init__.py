@@ -8,6 +8,6 @@ workloads.
 """
 from . import build  # noqa: F401
 from ._capi import (ELEMENT_DTYPE, ST_DA_TRACK_DA, ST_DA_TRACK_DA1, ST_DA_TRACK_DA_RADIAL,  # noqa: F401
-                    TRACK_RADIAL, TRACK_SPIN, NOALLOC_MAX_PARTICLES, NOALLOC_MAX_ELEMENTS, TYPES, Batch, BatchConfig, Context, Globals, Mlp,
+                    TRACK_RADIAL, TRACK_SPIN, TRACK_PERF, NOALLOC_MAX_PARTICLES, NOALLOC_MAX_ELEMENTS, TYPES, Batch, BatchConfig, Context, Globals, Mlp,
                     STError, Table, load)
 from .line import *  # noqa: F401,F403

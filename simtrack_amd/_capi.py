@@ -57,6 +57,7 @@ TYPES = {
 TRACK_SPIN = 1
 TRACK_RADIAL = 2
 TRACK_BARE = 4
+TRACK_PERF = 8    # perf-mode kernels: contraction and reciprocal division, not bit-equal
 # limits of st_track_device_noalloc's fixed scratch (include/simtrack_amd.h)
 NOALLOC_MAX_PARTICLES = 1 << 20
 NOALLOC_MAX_ELEMENTS = 16384

@@ -51,6 +51,7 @@ def build(verbose=False):
     hc = hipcc()
     jobs = []
     for src, extra in (("st_kernels.hip", ["--offload-arch=" + ARCH]),
+                       ("st_kernels_perf.hip", ["--offload-arch=" + ARCH, "-ffp-contract=fast"]),
                        ("st_tune.hip", ["--offload-arch=" + ARCH]),
                        ("st_mlp.hip", ["--offload-arch=" + ARCH]),
                        ("st_ssbb.hip", ["--offload-arch=" + ARCH]),

@@ -98,7 +98,7 @@ static void build_items(const st_batch *b, const std::vector<int> &state, const 
 }
 
 static int batch_core(st_batch *b, double *d_x, const int *offsets, int turn0, int nturn, int *d_state, int *d_lost,
-                      bool spin, const st_batch_config &cfg, double *d_tbt = nullptr)
+                      bool spin, const st_batch_config &cfg, double *d_tbt = nullptr, bool perf = false)
 {
     st_ctx *c = b->c;
     const int ntab = (int)b->tabs.size();
@@ -160,7 +160,10 @@ static int batch_core(st_batch *b, double *d_x, const int *offsets, int turn0, i
             if (rc) return rc;
             HIPCHK(hipEventRecord(c->ev_pool[ev], c->stream));
         }
-        HIPCHK(st_launch_track_batch(a, grid, block, spin, b->hexp, b->rad, c->stream));
+        if (perf)
+            HIPCHK(st_perf::st_launch_track_batch(a, grid, block, spin, b->hexp, b->rad, c->stream));
+        else
+            HIPCHK(st_launch_track_batch(a, grid, block, spin, b->hexp, b->rad, c->stream));
         if (c->timing) {
             HIPCHK(hipEventRecord(c->ev_pool[ev + 1], c->stream));
             c->pending.push_back({ev, ev + 1});
@@ -206,7 +209,8 @@ extern "C" int st_batch_track_device(st_batch *b, double *d_x, const int *offset
     st_batch_config dc;
     st_batch_default_config(&dc);
     HIPCHK(hipSetDevice(b->c->device));
-    return batch_core(b, d_x, offsets, turn0, nturn, d_state, d_lost, (flags & ST_TRACK_SPIN) != 0, cfg ? *cfg : dc);
+    return batch_core(b, d_x, offsets, turn0, nturn, d_state, d_lost, (flags & ST_TRACK_SPIN) != 0, cfg ? *cfg : dc,
+                      nullptr, (flags & ST_TRACK_PERF) != 0);
 }
 
 extern "C" int st_batch_track_tbt_device(st_batch *b, double *d_x, const int *offsets, int turn0, int nturn,

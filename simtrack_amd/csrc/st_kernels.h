@@ -64,6 +64,13 @@ hipError_t st_launch_track(const st_track_args &a, bool spin, bool hexp, bool ra
 hipError_t st_launch_track_batch(const st_batch_args &b, int grid, int block, bool spin, bool hexp, bool rad,
                                  hipStream_t stream);
 hipError_t st_launch_radial_x04(const st_track_args &a, double *x04, bool hexp, hipStream_t stream);
+/* the perf-mode kernels (st_kernels_perf.hip): the same maps with FMA
+   contraction and division by reciprocal multiplication */
+namespace st_perf {
+hipError_t st_launch_track(const st_track_args &a, bool spin, bool hexp, bool rad, hipStream_t stream);
+hipError_t st_launch_track_batch(const st_batch_args &b, int grid, int block, bool spin, bool hexp, bool rad,
+                                 hipStream_t stream);
+}
 hipError_t st_launch_compact(const int *idx, int n, const int *state, int *block_buf, int *idx_out, int *d_total,
                              hipStream_t stream);
 

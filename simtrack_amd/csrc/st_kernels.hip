@@ -27,6 +27,12 @@
 #include "st_maps.h"
 #include "st_kernels.h"
 
+/* st_kernels_perf.hip compiles this file a second time as the perf-mode
+   kernels, in their own namespace so their symbols stay apart */
+#ifdef ST_PERF_MODE
+namespace st_perf {
+#endif
+
 /* ----------------------------------------------------- element dispatch */
 
 /* quadrupole hard-edge fringe of QUAD_Pass, simtrack.cpp:4895-4911 (entry, sgn=+1)
@@ -700,8 +706,14 @@ ST_DEV void track_turns(const st_track_args &a, const st_opp ops, const st_pp pa
     }
 }
 
+/* minimum resident wavefronts per SIMD the register allocation must allow
+   (1: the compiler's choice; 2: at most 256 VGPRs, 4: at most 128) */
+#ifndef ST_TRACK_WAVES_PER_EU
+#define ST_TRACK_WAVES_PER_EU 1
+#endif
+
 template <bool SPIN, bool HEXP, bool RAD>
-__global__ __launch_bounds__(256) void k_track(st_track_args a)
+__global__ __launch_bounds__(256, ST_TRACK_WAVES_PER_EU) void k_track(st_track_args a)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = t < a.nact;
@@ -717,7 +729,7 @@ __global__ __launch_bounds__(256) void k_track(st_track_args a)
  * after at most nitems claims.  The host orders items longest table first.
  */
 template <bool SPIN, bool HEXP, bool RAD>
-__global__ __launch_bounds__(256) void k_track_batch(st_batch_args b)
+__global__ __launch_bounds__(256, ST_TRACK_WAVES_PER_EU) void k_track_batch(st_batch_args b)
 {
     const int lane = threadIdx.x & 63;
     st_track_args a;
@@ -900,3 +912,7 @@ hipError_t st_launch_compact(const int *idx, int n, const int *state, int *block
     hipLaunchKernelGGL(k_compact, dim3(nb), dim3(1024), 0, stream, idx, n, state, block_buf, idx_out);
     return hipGetLastError();
 }
+
+#ifdef ST_PERF_MODE
+} /* namespace st_perf */
+#endif

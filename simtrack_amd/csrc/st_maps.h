@@ -113,6 +113,40 @@ ST_DEV void ptc_update(Ptc &c, double pt, st_gref g)
  * A NaN never reaches lo or hi (min/max return the other operand); NaN in
  * gives NaN out on both paths, and NaN sign and payload carry no meaning.
  */
+#ifdef ST_PERF_MODE
+/* Perf mode: a division by a prepared divisor is a * (refined 1/b), within an
+   ulp or so of a / b; the square root skips the range bookkeeping.  Only the
+   asin/sincos argument ranges are still checked (their small-argument forms
+   are inaccurate beyond), and a lane outside them replays the step with the
+   full functions. */
+struct ArFast {
+    bool ok = true;
+    ST_DEV bool bad() const { return !ok; }
+    ST_DEV double div(double a, const StRcp &r) { return a * r.y; }
+    ST_DEV double sqrt(double x)
+    {
+        double y = __builtin_amdgcn_rsq(x);
+        double g = x * y;
+        double h = y * 0.5;
+        double r = __builtin_fma(-h, g, 0.5);
+        g = __builtin_fma(g, r, g);
+        h = __builtin_fma(h, r, h);
+        double d = __builtin_fma(-g, g, x);
+        g = __builtin_fma(d, h, g);
+        return x > 0.0 && x < __builtin_inf() ? g : __builtin_sqrt(x);
+    }
+    ST_DEV double asin(double x)
+    {
+        ok = ok & (__builtin_fabs(x) < 0.125);
+        return st_asin_small(x);
+    }
+    ST_DEV void sincos(double x, double *sn, double *cs)
+    {
+        ok = ok & (__builtin_fabs(x) <= ST_PIO4);
+        st_sincos_small(x, sn, cs);
+    }
+};
+#else
 struct ArFast {
     double lo = __builtin_inf(), hi = 0.;
     bool ok = true;     /* every divisor in range, every asin/sincos argument small */
@@ -155,6 +189,8 @@ struct ArFast {
         st_sincos_small(x, sn, cs);
     }
 };
+
+#endif
 
 struct ArExact {
     ST_DEV double div(double a, const StRcp &r) { return a / r.b; }

@@ -49,7 +49,7 @@ struct track_scratch {
 
 static int track_device_run(st_table *t, double *d_x, int np, int turn0, int nturn, int *d_state, int *d_lost,
                             bool spin, bool radial, double *d_tbt, int tbt_bpm, int tbt_nturn, bool bare,
-                            const track_scratch *fixed);
+                            const track_scratch *fixed, bool perf = false);
 
 int track_device_core(st_table *t, double *d_x, int np, int turn0, int nturn, int *d_state, int *d_lost,
                              bool spin, bool radial, double *d_tbt, int tbt_bpm, int tbt_nturn, bool bare)
@@ -60,8 +60,9 @@ int track_device_core(st_table *t, double *d_x, int np, int turn0, int nturn, in
 
 static int track_device_run(st_table *t, double *d_x, int np, int turn0, int nturn, int *d_state, int *d_lost,
                             bool spin, bool radial, double *d_tbt, int tbt_bpm, int tbt_nturn, bool bare,
-                            const track_scratch *fixed)
+                            const track_scratch *fixed, bool perf)
 {
+    auto launch = perf ? st_perf::st_launch_track : st_launch_track;
     st_ctx *c = t->ctx;
     if (!fixed && ensure_particles(c, (size_t)np, 1)) return ST_ERR_NOMEM;
     const bool timing = c->timing && !fixed;
@@ -112,7 +113,7 @@ static int track_device_run(st_table *t, double *d_x, int np, int turn0, int ntu
             if (rc2) return rc2;
             HIPCHK(hipEventRecord(c->ev_pool[ev], c->stream));
         }
-        HIPCHK(st_launch_track(a, spin, hexp, rad, c->stream));
+        HIPCHK(launch(a, spin, hexp, rad, c->stream));
         if (timing) {
             HIPCHK(hipEventRecord(c->ev_pool[ev + 1], c->stream));
             c->pending.push_back({ev, ev + 1});
@@ -141,7 +142,7 @@ static int track_device_run(st_table *t, double *d_x, int np, int turn0, int ntu
         a.precheck = 1;
         a.idx = nullptr;
         a.nact = np;
-        HIPCHK(st_launch_track(a, spin, hexp, rad, c->stream));
+        HIPCHK(launch(a, spin, hexp, rad, c->stream));
     }
     return ST_OK;
 }
@@ -232,8 +233,9 @@ extern "C" int st_track_device(st_table *t, double *d_x, int np, int turn0, int 
         return fail(ST_ERR_ARG, "st_track_device: bad argument");
     if (np == 0) return ST_OK;
     HIPCHK(hipSetDevice(t->ctx->device));
-    return track_device_core(t, d_x, np, turn0, nturn, d_state, d_lost, (flags & ST_TRACK_SPIN) != 0,
-                             (flags & ST_TRACK_RADIAL) != 0, nullptr, -1, 0, (flags & ST_TRACK_BARE) != 0);
+    return track_device_run(t, d_x, np, turn0, nturn, d_state, d_lost, (flags & ST_TRACK_SPIN) != 0,
+                            (flags & ST_TRACK_RADIAL) != 0, nullptr, -1, 0, (flags & ST_TRACK_BARE) != 0, nullptr,
+                            (flags & ST_TRACK_PERF) != 0);
 }
 
 extern "C" int st_element_pass(st_table *t, int elem_index, double *x, int np, int spin)
@@ -323,5 +325,6 @@ extern "C" int st_track_device_noalloc(st_table *t, double *d_x, int np, int tur
     s.block = ints + 2 * (size_t)ST_NOALLOC_MAX_PARTICLES;
     s.total = s.block + NA_BLOCKS;
     return track_device_run(t, d_x, np, turn0, nturn, d_state, d_lost, (flags & ST_TRACK_SPIN) != 0,
-                            (flags & ST_TRACK_RADIAL) != 0, nullptr, -1, 0, (flags & ST_TRACK_BARE) != 0, &s);
+                            (flags & ST_TRACK_RADIAL) != 0, nullptr, -1, 0, (flags & ST_TRACK_BARE) != 0, &s,
+                            (flags & ST_TRACK_PERF) != 0);
 }

@@ -20,18 +20,31 @@ PEAK_FP64_VECTOR = 78.6e12
 
 PT_UPDATE = 15        # delta1 (7), gamma1 (2), beta1 (4), 1+delta1, (1+delta1)^2
 DRIFT = 16            # DRIFT_Pass exact, cpp:3753-3765 with delta1/gamma1/beta1 cached
-DRIFT_HEXP = 11       # DRIFT_Pass H_expand branch, cpp:3745-3750
+DRIFT_HEXP = 14       # DRIFT_Pass H_expand branch, cpp:3745-3750 (1 + pt formed twice)
 QUAD_KICK, SEXT_KICK, OCT_KICK = 8, 15, 22
-SBEND_EXACT = 51      # sbend_exact_pass, cpp:3833-3858 (2 asin counted as 1 each)
-GSBEND_EXACT = 56
+SBEND_EXACT = 56      # sbend_exact_pass, cpp:3833-3858 (2 asin counted as 1 each)
+GSBEND_EXACT = 57
 EDGE = 4
-SPIN = 114            # spin(), cpp:4574-4630 (sin and cos counted)
-RADIATE = 44 + PT_UPDATE
-RADIATE1 = 60 + PT_UPDATE
-BENDKICK = 7
+SPIN = 112            # spin(), cpp:4574-4630 (sin and cos counted)
+SPIN_UNIFORM = 110    # spin() with a field given by the element (bends, kickers)
+# spin field of QUAD/SEXT/OCT_sPass at the slice midpoint, cpp:7538-7660:
+# zero-length elements, and thick slices (which also divide by Nint)
+SPIN_FIELD = {1: 6, 2: 13, 3: 22}
+SPIN_FIELD_SLICE = {1: 8, 2: 15, 3: 22}
+RADIATE = 48 + PT_UPDATE
+RADIATE1 = 63 + PT_UPDATE
+BENDKICK = 8
 GBENDKICK = 9
-BB4D_ROUND = 40       # BB4D round branch, cpp:6336-6342
-BB4D_ELLIPTIC = 40 + 2 * 150   # two cerrf calls, cpp:6343-6380
+QUAD_FRINGE = 72      # one end of QUAD_Pass's fringe map, cpp:4895-4911
+BB4D_ROUND = 17       # BB4D round branch, cpp:6336-6342
+# cerrf (cpp:6258-6323) is data-dependent: ~100 operations far from the core,
+# up to ~600 inside it (Taylor/continued-fraction lengths 7 + 23 q, 10 + 21 q);
+# 550 is its mean over Gaussian beam-core arguments (instrumented)
+CERRF = 550
+BB4D_ELLIPTIC = 24 + 2 * CERRF   # two cerrf calls, cpp:6343-6380
+BB6D_SLICE_ROUND = 78            # one slice of BB6D (cpp:6622-6676) with the round BB4D
+BB6D_SLICE_ELLIPTIC = 113 + 2 * CERRF
+CRBB_BOOSTS = 60                 # BB6D_Angle's boost and inverse boost, cpp:6958-7078
 # strong-strong slice pair, per row (cpp:23035-23110): moments of the drifted
 # row (drift 4, sums 6), then drift 4, BB4D, kick 4, drift back 4
 SS_MOMENTS = 10
@@ -60,6 +73,19 @@ def _mask(kn, ks, norder):
     return m
 
 
+def _misalignment(r):
+    """GtoL + LtoG (simtrack.cpp:3701-3741): shifts (2 each way) and the
+    tilt rotation (12 each way) of GtoL1/LtoG1, or the pitched frame change
+    of GtoL2/LtoG2 (W-matrix product and path-length terms); none of them
+    changes pt."""
+    a = [abs(float(r[k])) for k in ("dx", "dy", "ds", "dtheta", "dphi", "dpsi")]
+    if sum(a) < 1e-9:
+        return 0
+    if a[3] + a[4] < 1e-9:
+        return 2 * ((2 if a[0] + a[1] > 1e-10 else 0) + (12 if a[5] > 1e-10 else 0))
+    return 2 * (72 + 5)
+
+
 def element_ops(r, gp, spin=False):
     t = int(r["type"])
     L = float(r["l"])
@@ -69,16 +95,12 @@ def element_ops(r, gp, spin=False):
     d = DRIFT_HEXP if hexp else DRIFT
     drift_l = (lambda length: 0 if length == 0. else d)
     name = {v: k for k, v in TYPES.items()}[t]
-    mis = 0
-    if abs(r["dx"]) + abs(r["dy"]) + abs(r["ds"]) + abs(r["dtheta"]) + abs(r["dphi"]) + abs(r["dpsi"]) >= 1e-9:
-        if abs(r["dtheta"]) + abs(r["dphi"]) < 1e-9:
-            mis = 2 * (4 + 12)
-        else:
-            mis = 2 * (72 + 5)
+    mis = _misalignment(r)
     if name in ("DRIFT", "MARKER", "BPM", "HBPM", "VBPM"):
         return drift_l(L)
     if name in ("ROTAT", "SNAKE"):
-        return 2 * drift_l(L / 2) + (18 if spin else 0)
+        # Pass is one drift; sPass two half drifts around the rotation
+        return 2 * drift_l(L / 2) + 18 if spin else drift_l(L)
     if name in ("QUAD", "SKEWQ", "SEXT", "OCT"):
         order = {"QUAD": 1, "SKEWQ": 1, "SEXT": 2, "OCT": 3}[name]
         kick = {1: QUAD_KICK, 2: SEXT_KICK, 3: OCT_KICK}[order]
@@ -86,16 +108,20 @@ def element_ops(r, gp, spin=False):
         if kn == 0. and r["knsl"][order] == 0.:
             return drift_l(L) + mis
         if L == 0.:
-            return kick + (SPIN + 4 + kick // 2 + 2 if spin else 0) + mis
-        per = 4 * d + 3 * kick + (SPIN + 4 + kick + 2 if spin else 0)
-        return nint * per + mis
+            return kick + (SPIN + 4 + SPIN_FIELD[order] if spin else 0) + mis
+        per = 4 * d + 3 * kick + (SPIN + 4 + SPIN_FIELD_SLICE[order] if spin else 0)
+        # QUAD_Pass's hard-edge fringe maps at both ends (cpp:4895-4957)
+        fringe = 2 * QUAD_FRINGE if gp.quad_fringe and order == 1 and not spin else 0
+        return nint * per + fringe + mis
     if name == "MULT" or (name in ("GMULT", "SBENDMULT", "SMULT", "GSBENDMULT") and L == 0.):
         n = int(r["norder"])
         m0 = _mask(r["knl"], r["knsl"], n)
         if L == 0.:
-            return mult_kick(n, m0) + (SPIN + mult_kick(n, m0) if spin else 0) + mis
+            # the bend families' zero-length sPass forms the midpoint of x and y too
+            extra = 1 if name == "MULT" else 5
+            return mult_kick(n, m0) + (SPIN + mult_kick(n, m0) + extra if spin else 0) + mis
         k1 = _mask(r["knl"] * 1.0, r["knsl"] * 1.0, n)
-        per = 4 * d + 3 * mult_kick(n, k1) + (SPIN + 4 + mult_kick(n, m0) + 3 if spin else 0)
+        per = 4 * d + 3 * mult_kick(n, k1) + (SPIN + 4 + mult_kick(n, m0) + 1 if spin else 0)
         return nint * per + mis
     if name in ("SBEND", "GSBEND"):
         if r["angle"] == 0.:
@@ -106,26 +132,27 @@ def element_ops(r, gp, spin=False):
         if not spin and not hexp and not rad:
             return 2 * EDGE + SBEND_EXACT + mis
         step = SBEND_EXACT if not hexp else 4 * d + 3 * BENDKICK
-        extra = (SPIN if spin else 0) + ((RADIATE if spin else RADIATE1) if rad else 0)
+        extra = (SPIN_UNIFORM if spin else 0) + ((RADIATE if spin else RADIATE1) if rad else 0)
         return 2 * EDGE + nint * (step + extra) + mis
     if name in ("GMULT", "SBENDMULT", "SMULT", "GSBENDMULT"):
         n = int(r["norder"])
         m = _mask(r["knl"], r["knsl"], n)
         gen = name == "GSBENDMULT" and not spin
         if hexp:
-            step = 4 * d + 3 * mult_kick(n, m, bend=True)
+            # general_bend_mult_kick_pass (GSBENDMULT) has the K0L term besides the geometric one
+            step = 4 * d + 3 * (mult_kick(n, m, bend=True) + (1 if gen else 0))
         else:
             step = 4 * (GSBEND_EXACT if gen else SBEND_EXACT) + 3 * mult_kick(n, m)
         extra = 0
         if spin or rad:
-            extra += 4 + mult_kick(n, m) + 3
+            extra += 4 + mult_kick(n, m) + (2 if rad else 1)
         if spin:
             extra += SPIN
         if rad:
             extra += RADIATE
         return 2 * EDGE + nint * (step + extra) + mis
     if name in ("KICKER", "HKICKER", "VKICKER"):
-        sp = SPIN if spin and (r["hkick"] != 0. or r["vkick"] != 0.) else 0
+        sp = SPIN_UNIFORM if spin and (r["hkick"] != 0. or r["vkick"] != 0.) else 0
         return 2 * drift_l(L / 2) + 2 + sp + mis
     if name == "RFCAV":
         if r["vrf"] == 0.:
@@ -138,24 +165,29 @@ def element_ops(r, gp, spin=False):
     if name == "CCMULT":
         if r["vrf"] == 0.:
             return drift_l(L) + mis
-        return 2 * drift_l(L / 2) + 140 + PT_UPDATE + mis
+        return 2 * drift_l(L / 2) + 147 + PT_UPDATE + mis
     if name in ("LBT", "ILBT"):
-        return (30 if name == "LBT" else 28) + PT_UPDATE + (0 if spin else mis)
+        return (41 if name == "LBT" else 34) + PT_UPDATE + (0 if spin else mis)
     if name == "BEAMBEAM":
         if r["np"] == 0.:
             return mis
         if int(r["treatment"]) == 6:
             sx, sy = r["emitx"] * r["betax"], r["emity"] * r["betay"]
-            per = 97 if abs(np.sqrt(sx) - np.sqrt(sy)) / np.sqrt(sx) < 1e-6 else 97 + 2 * 150 + 20
+            round_ = abs(np.sqrt(sx) - np.sqrt(sy)) / np.sqrt(sx) < 1e-6
+            per = BB6D_SLICE_ROUND if round_ else BB6D_SLICE_ELLIPTIC
             return int(r["nslice"]) * per + PT_UPDATE + mis
-        return 40 + 2 * 150 + mis
+        sx, sy = r["emitx"] * r["betax"], r["emity"] * r["betay"]
+        return (BB4D_ROUND if abs(np.sqrt(sx) - np.sqrt(sy)) / np.sqrt(sx) < 1e-6 else BB4D_ELLIPTIC) + mis
     if name == "CRBB":
-        return 60 + int(r["nslice"]) * (97 + 2 * 150 + 20) + PT_UPDATE + mis
+        return CRBB_BOOSTS + int(r["nslice"]) * BB6D_SLICE_ELLIPTIC + PT_UPDATE + mis
     if name == "MATRIX":
-        return 6 + 72 + 6 + PT_UPDATE
+        m = np.asarray(r["m66"], dtype=float).reshape(6, 6)
+        keeps_pt = (np.array_equal(m[5], [0., 0., 0., 0., 0., 1.]) and r["xco_in"][5] == 0. and
+                    r["xco_out"][5] == 0.)
+        return 6 + 72 + 6 + (0 if keeps_pt else PT_UPDATE)
     if name == "PATCH":
         # TRANS_Pass 3; YROTAT_Pass: pz 5 (sqrt), temp 3, x 2, px 3, y 5, z 7
-        return 3 + (0 if abs(r["theta"]) < 1e-9 else 25)
+        return 3 + (0 if abs(r["theta"]) < 1e-9 else 24)
     if name == "SOLEN":
         return drift_l(L) + mis
     return 0
