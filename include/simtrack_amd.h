@@ -484,7 +484,15 @@ int st_action_diff(st_table *t, double betx, double alfx, double bety, double al
  * its bbscale.  Moments are summed in a fixed order (repeatable bit for
  * bit), not the reference's sequential order, so they agree with it to
  * rounding.  Blocking; d_x1/d_x2 are device arrays, the _host forms take
- * host arrays and copy through the context.
+ * host arrays and copy through the context.  The work runs on the context's
+ * own stream, which is not ordered after other streams: a caller that
+ * writes d_x1/d_x2 on another stream (a framework's current stream) waits
+ * for that write before the call (simtrack_amd.strong_strong does).  The
+ * same holds for every *_device entry point.
+ *
+ * Slice pairs that touch disjoint slices go out together on up to 8 streams
+ * (the result is the sequential one bit for bit); st_ctx_set_ss_lanes
+ * bounds their number (1: one pair at a time).
  *
  * st_bb_soft_gaussian_4d: BeamBeam4D_SoftGaussian (simtrack.cpp:22846).
  * st_bb_soft_gaussian_6d: BeamBeam6D_SoftGaussian (cpp:22874): both beams
@@ -498,6 +506,7 @@ int st_action_diff(st_table *t, double betx, double alfx, double bety, double al
  *   strictly decreasing); rows stay in place; pair charges Np * m / Mp.
  * st_zboundary_slices_gaussian: Zboundary_slices_Gaussian (cpp:3157).
  */
+int st_ctx_set_ss_lanes(st_ctx *ctx, int lanes);
 int st_bb_soft_gaussian_4d(st_ctx *ctx, double Np1, int Mp1, double *d_x1, double Np2, int Mp2, double *d_x2,
                            double gamma1, double gamma2, double bbscale1, double bbscale2);
 int st_bb_soft_gaussian_6d(st_ctx *ctx, double Np1, int Mp1, double *d_x1, double Np2, int Mp2, double *d_x2,

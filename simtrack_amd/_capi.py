@@ -67,7 +67,7 @@ EXPORTS = [
     "st_globals_default", "st_globals_set_energy", "st_abi_version", "st_struct_sizes",
     "st_last_error", "st_ctx_create", "st_ctx_destroy", "st_table_build", "st_table_destroy",
     "st_table_size", "st_track", "st_track_spin", "st_track_tbt", "st_track_radial",
-    "st_element_pass", "st_track_device", "st_track_device_noalloc", "st_ctx_sync", "st_last_gpu_seconds",
+    "st_element_pass", "st_track_device", "st_track_device_noalloc", "st_ctx_sync", "st_ctx_set_ss_lanes", "st_last_gpu_seconds",
     "st_ctx_set_timing", "st_ctx_kernel_stats", "st_da_angles", "st_da_bisect", "st_da_uniform",
     "st_fine_tune", "st_fine_tune_device", "st_batch_create", "st_batch_destroy", "st_batch_default_config",
     "st_batch_track", "st_batch_track_device", "st_batch_autotune", "st_mlp_create", "st_mlp_destroy",
@@ -138,6 +138,7 @@ def load(path=None):
     lib.st_track_device_noalloc.argtypes = [P, P, I, I, I, P, P, I]
     lib.st_last_gpu_seconds.argtypes = [P, ctypes.POINTER(D)]
     lib.st_ctx_set_timing.argtypes = [P, I]
+    lib.st_ctx_set_ss_lanes.argtypes = [P, I]
     lib.st_ctx_kernel_stats.argtypes = [P, ctypes.POINTER(D), ctypes.POINTER(ctypes.c_int64),
                                         ctypes.POINTER(ctypes.c_int64)]
     lib.st_da_angles.argtypes = [I, P, ctypes.POINTER(I), ctypes.POINTER(D)]
@@ -221,6 +222,10 @@ class Context:
 
     def sync(self):
         check(self.lib.st_ctx_sync(self.h), "st_ctx_sync")
+
+    def set_ss_lanes(self, lanes):
+        """At most `lanes` strong-strong slice pairs in flight (1..8)."""
+        check(self.lib.st_ctx_set_ss_lanes(self.h, int(lanes)), "st_ctx_set_ss_lanes")
 
     def set_timing(self, on=True):
         check(self.lib.st_ctx_set_timing(self.h, 1 if on else 0), "st_ctx_set_timing")

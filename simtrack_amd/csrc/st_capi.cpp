@@ -159,6 +159,14 @@ int timed_event(st_ctx *c, int *idx)
     return ST_OK;
 }
 
+extern "C" int st_ctx_set_ss_lanes(st_ctx *c, int lanes)
+{
+    if (!c || lanes < 1 || lanes > SS_LANES)
+        return fail(ST_ERR_ARG, "st_ctx_set_ss_lanes: lanes %d not in [1, %d]", lanes, SS_LANES);
+    c->ss_max_lanes = lanes;
+    return ST_OK;
+}
+
 extern "C" int st_ctx_set_timing(st_ctx *c, int enable)
 {
     if (!c) return fail(ST_ERR_ARG, "st_ctx_set_timing: null context");

@@ -82,6 +82,7 @@ struct st_ctx {
     hipStream_t lane[SS_LANES] = {};
     hipEvent_t lane_ev[SS_LANES] = {};
     hipEvent_t fork_ev = nullptr;
+    int ss_max_lanes = SS_LANES;   /* st_ctx_set_ss_lanes */
 };
 
 struct dev_table {

@@ -102,8 +102,8 @@ static int ss_prepare(st_ctx *c, int n1, int n2, int ns1, int ns2, bool host_row
     size_t temp = 0, ptemp = 0;
     HIPCHK(st_ss_sort_temp_bytes(std::max(std::max(n1, n2), 1), &temp));
     if (pic_rows > 0) HIPCHK(st_sort_cells_bytes(pic_rows * 9, &ptemp));
-    /* as many lanes as the smaller beam has slices */
-    const int lanes = std::max(1, std::min(SS_LANES, std::min(ns1, ns2)));
+    /* as many lanes as the smaller beam has slices, at most the context's limit */
+    const int lanes = std::max(1, std::min(std::min(SS_LANES, c->ss_max_lanes), std::min(ns1, ns2)));
     size_t need = ss_layout(nullptr, n1, n2, ns1, ns2, host_rows, temp, w, pic_rows, ptemp, lanes);
     if (need > c->cap_ss) {
         hipFree(c->d_ss);
@@ -236,7 +236,17 @@ static int ss_runs(st_ctx *c, const ss_work &w, const std::vector<int> &I1, cons
             hipStream_t st = l == 0 ? c->stream : c->lane[l];
             if (l > 0) HIPCHK(hipStreamWaitEvent(st, c->fork_ev, 0));
             int rc = pair(i, l, st);
-            if (rc) return rc;
+            if (rc) {
+                /* join the lanes already launched before the context stream
+                   goes on, so a later call cannot reuse their buffers while
+                   their kernels run */
+                for (size_t k = i0 + 1; k <= i; ++k) {
+                    const int lk = (int)(k - i0);
+                    if (hipEventRecord(c->lane_ev[lk], c->lane[lk]) == hipSuccess)
+                        (void)hipStreamWaitEvent(c->stream, c->lane_ev[lk], 0);
+                }
+                return rc;
+            }
             if (l > 0) HIPCHK(hipEventRecord(c->lane_ev[l], st));
         }
         for (size_t i = i0 + 1; i < i1; ++i) HIPCHK(hipStreamWaitEvent(c->stream, c->lane_ev[i - i0], 0));

@@ -41,6 +41,11 @@ def _pair(x1, x2, what):
     b, pb, n2 = _rows(x2, what)
     if _is_device(a) != _is_device(b):
         raise STError(f"{what}: both beams must be on the host or both on the device")
+    if _is_device(a):
+        # the library works on its own stream: let torch's pending writes of
+        # the beams finish first
+        import torch
+        torch.cuda.current_stream(a.device).synchronize()
     return pa, n1, pb, n2, _is_device(a)
 
 

@@ -299,3 +299,28 @@ def test_coherent_beam_beam_modes(ctx):
     assert d_inc < -0.009                       # the defocusing beam-beam of like charges lowers the tune
     assert 0.95 < d_sg / d_inc < 1.08
     assert 1.02 < d_pic / d_sg < 1.3
+
+
+@pytest.mark.parametrize("pic", [False, True], ids=["soft_gaussian", "pic"])
+def test_lanes_do_not_change_the_result(pic):
+    """8 x 8 slices: pairs that touch disjoint slices run concurrently on up
+    to 8 streams; with one lane they run one after another.  Both give the
+    same bits (pairs that share a slice keep their order either way)."""
+    b1, b2 = _beam(20000, 5), _beam(20000, 6, sx=8e-5, sy=5e-5, off=(1e-5, -2e-5))
+    zb = SS.Zboundary_slices_Gaussian(8, 0.05)
+    out = []
+    for lanes in (8, 1):
+        c = st.Context(0)
+        c.set_ss_lanes(lanes)
+        d1 = torch.tensor(b1, device="cuda:0")
+        d2 = torch.tensor(b2, device="cuda:0")
+        if pic:
+            SS.BeamBeam6D_PIC(c, ARGS[0], d1, ARGS[1], d2, ARGS[2], ARGS[3], ARGS[4], ARGS[5], 8, 8)
+        else:
+            SS.BeamBeam6D_SoftGaussian(c, ARGS[0], d1, ARGS[1], d2, ARGS[2], ARGS[3], ARGS[4], ARGS[5], 8, 8,
+                                       zb, zb)
+        torch.cuda.synchronize()
+        out.append((d1.cpu().numpy(), d2.cpu().numpy()))
+        c.close()
+    assert_bitwise(out[0][0], out[1][0], "beam 1, 8 lanes vs 1")
+    assert_bitwise(out[0][1], out[1][1], "beam 2, 8 lanes vs 1")
