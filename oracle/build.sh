@@ -5,6 +5,7 @@
 #   liboracle_glibc_{trig,asin,exp,pow}.so  the C library for one function
 #                       family only (sin/cos/tan, asin, exp, pow)
 #   liboracle_opcount.so  the maps' algorithmic operation counts (opcount.cpp)
+#   liboracle_dual.so   the maps' exact Jacobians (dualmap.cpp)
 set -e
 here=$(cd "$(dirname "$0")" && pwd)
 root=$(dirname "$here")
@@ -24,3 +25,7 @@ done
 g++ -x c++ -std=c++17 -O1 -ffp-contract=off -mfma -fPIC -shared -fvisibility=hidden -w $INC -I"$here" \
     "$here/opcount.cpp" -o "$here/build/liboracle_opcount.so.tmp"
 mv "$here/build/liboracle_opcount.so.tmp" "$here/build/liboracle_opcount.so"
+# first-order (dual-number) build, the exact Jacobians of the maps (dualmap.cpp)
+g++ -x c++ -std=c++17 -O2 -ffp-contract=off -mfma -fPIC -shared -fvisibility=hidden -w $INC -I"$here" \
+    "$here/dualmap.cpp" -o "$here/build/liboracle_dual.so.tmp"
+mv "$here/build/liboracle_dual.so.tmp" "$here/build/liboracle_dual.so"

@@ -180,7 +180,7 @@ static inline Tracked opc_cached(const Tracked &pt, F f)
 #define ORC_PT_CACHED(pt, v) opc_cached((pt), [&]() -> Tracked { return (v); })
 #define ORC_PT_DERIVED(v) opc_derived([&]() -> Tracked { return (v); })
 
-#define ORC_OPCOUNT 1
+#define ORC_SCALAR_BUILD 1   /* st_oracle.c: leave out the exports that take plain double pointers */
 #define st_element opc_element
 #define double Tracked
 #define st_rng_normal(s, p, c) opc_draw(st_rng_normal((s), (p), (c)))

@@ -303,3 +303,45 @@ def opcount(records, index, gp, x, spin=False):
     if rc != 0:
         raise RuntimeError(f"opcount failed ({rc})")
     return int(c[0]), int(c[1]), int(c[2])
+
+
+# ------------------------------------------- first-order (dual) maps
+
+_dual = None
+
+
+def _dual_lib():
+    global _dual
+    if _dual is None:
+        path = os.path.join(HERE, "build", "liboracle_dual.so")
+        if not os.path.exists(path):
+            build()
+        _dual = ctypes.CDLL(path)
+        P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        _dual.dual_one_turn_matrix.argtypes = [P, I, P, P, P]
+        _dual.dual_element_maps.argtypes = [P, I, P, P, D, P]
+    return _dual
+
+
+def dual_one_turn_matrix(records, gp, x_co):
+    """The exact first-order one-turn matrix about x_co, RF voltages off
+    (oracle/dualmap.cpp)."""
+    rec = np.ascontiguousarray(records)
+    m = np.zeros((6, 6))
+    xc = np.ascontiguousarray(np.asarray(x_co, dtype=np.float64)[:6])
+    rc = _dual_lib().dual_one_turn_matrix(_p(rec), len(rec), ctypes.byref(gp), _p(xc), _p(m))
+    if rc:
+        raise RuntimeError(f"dual_one_turn_matrix: {rc}")
+    return m
+
+
+def dual_element_maps(records, gp, X, x5):
+    """Every element's exact first-order matrix about the orbit X (n x 6,
+    after each element), z = 0 and x[5] = x5, RF voltages off."""
+    rec = np.ascontiguousarray(records)
+    Xc = np.ascontiguousarray(X, dtype=np.float64)
+    T = np.zeros((len(rec), 6, 6))
+    rc = _dual_lib().dual_element_maps(_p(rec), len(rec), ctypes.byref(gp), _p(Xc), float(x5), _p(T))
+    if rc:
+        raise RuntimeError(f"dual_element_maps: {rc}")
+    return T

@@ -2233,7 +2233,7 @@ EXPORT double orc_cos(double x) { return OCOS(x); }
 EXPORT double orc_tan(double x) { return OTAN(x); }
 EXPORT double orc_asin(double x) { return OASIN(x); }
 EXPORT double orc_exp(double x) { return OEXP(x); }
-#if !defined(ORACLE_GLIBC) && !defined(ORC_OPCOUNT)
+#if !defined(ORACLE_GLIBC) && !defined(ORC_SCALAR_BUILD)
 /* the branch-free common paths the GPU kernels evaluate first */
 EXPORT double orc_exp_phase(double x, int phase) { return st_exp_phase(x, phase); }
 EXPORT double orc_asin_small(double x) { return st_asin_small(x); }

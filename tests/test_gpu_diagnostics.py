@@ -201,3 +201,21 @@ def test_cpp_action_diff_matches_python(ctx, tmp_path):
     # printed boundary may differ in its last digit between the two paths
     same = sum(a.split()[-11:] == b.split()[-11:] for a, b in zip(py[1:], lines[1:]))
     assert same >= 0.99 * n, same
+
+
+def test_gpu_maps_against_exact_first_order_maps(ctx):
+    """st_one_turn_matrix and st_element_maps (symmetric differences of step
+    1e-6 on the GPU) against the exact Jacobians of the same maps (the dual
+    build of the oracle, a first-order tps as Cal_OneTurnMap/Cal_ElementMap
+    take, simtrack.cpp:15122, 15205): within the finite-difference error."""
+    gp, ln = W.rhic_example(RHIC)
+    rec = ln.records()
+    x_co, x_end, _ = D.cal_orbit_num(ctx, gp, rec, 0.)
+    x0 = x_end.copy()
+    x0[4], x0[5] = 0., 0.
+    m = D.one_turn_matrix(ctx, gp, rec, x0)
+    md = pyoracle.dual_one_turn_matrix(rec, gp, x0)
+    assert np.abs(m - md).max() < 5e-7
+    T, X = D.element_maps(ctx, gp, rec, x_co, 0.)
+    Td = pyoracle.dual_element_maps(rec, gp, X, 0.)
+    assert np.abs(T - Td).max() < 2e-7
