@@ -106,6 +106,8 @@ def main():
     ap.add_argument("--train-seeds", type=int, default=16, help="config3: lattices for the training set")
     ap.add_argument("--n-long", type=int, default=30000, help="config3: survival horizon in turns")
     ap.add_argument("--epochs", type=int, default=60, help="config3: training epochs")
+    ap.add_argument("--fallback", choices=("boundary", "band"), default="boundary",
+                    help="config3: which particles fall back to full tracking")
     ap.add_argument("--orbit", action="store_true", help="config4: orbit tracking with quantum excitation "
                     "instead of spin tracking")
     args = ap.parse_args()
@@ -545,7 +547,8 @@ def bench_config3(args):
 
     etabs, ex, eoffs = lattices(range(2026, 2026 + args.seeds))
     ref, t_full, _, _ = S.full(ctx, etabs, ex, eoffs, args.n_long, progress=progress)
-    surv, info = S.scan(ctx, etabs, ex, eoffs, args.n_long, mlp, std, twiss, progress=progress)
+    surv, info = S.scan(ctx, etabs, ex, eoffs, args.n_long, mlp, std, twiss, progress=progress, mode=args.fallback,
+                        n_angles=n_ang)
     amps = (np.arange(len(x1) // n_ang) + 1) * 20. / (len(x1) // n_ang)
     err_net, err_plain = [], []
     for k in range(len(etabs)):
@@ -563,6 +566,7 @@ def bench_config3(args):
                                f"lattices from their first 1,024 turns",
                    "mlp": "12-256-256-256-1 BF16 MFMA"},
         "full_track_s": t_full, "surrogate_scan_s": info["wall"], "features_s": info["t_features"],
+        "fallback": args.fallback, "fallback_particles": info["n_fallback"],
         "fallback_fraction": info["n_fallback"] / max(info["n_alive_1k"], 1),
         "da_error_sigma": {"max": float(err_net.max()), "mean": float(err_net.mean()),
                            "lattices_within_0.25": int((err_net.max(axis=1) <= 0.25).sum())},

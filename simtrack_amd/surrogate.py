@@ -8,14 +8,21 @@ The tracker itself produces the training data and the features:
      the initial actions and angles they form 12 FP32 features
      (st_surrogate_features);
   3. the MLP (st_mlp_*, BF16 MFMA) predicts survival to the long horizon;
-  4. particles with predicted probability between 0.1 and 0.9 fall back to
-     full FP64 tracking from where step 1 left them, to the long horizon.
-A particle lost within the first 1,024 turns is lost; above 0.9 it is taken
-as surviving, below 0.1 as lost.  Full tracking of every particle to the
-long horizon is the reference, and judges the surrogate's errors.
-
-The DA boundary of one angle of one lattice is the amplitude of the first
-particle (by increasing amplitude) that does not survive.
+  4. the particles that can decide the DA boundary fall back to full FP64
+     tracking from where step 1 left them, to the long horizon.
+A particle lost within the first 1,024 turns is lost.  The DA boundary of
+one angle of one lattice is the amplitude of the first particle (by
+increasing amplitude) that does not survive, so only the particles below
+the first early loss can move it, and of those only the ones the network
+does not call safe (probability above 0.9).  The boundary-aware fallback
+(the default) tracks, on each line of the grid, those particles in order of
+amplitude up to and including the third one the network predicts lost
+(probability below 0.5), like Track_DA's search for the first loss
+(simtrack.cpp:21403-21435), here resolved in one batched pass; untracked
+particles take the network's verdict.  The band fallback tracks every
+particle with probability between 0.1 and 0.9.  Full tracking of every
+particle to the long horizon is the reference, and judges the surrogate's
+errors.
 """
 import time
 
@@ -166,8 +173,37 @@ def da_boundary(survive, amps, n_angles):
     return out
 
 
-def scan(ctx, tabs, x, offsets, n_long, mlp, std, twiss, cfg=None, progress=None):
-    """The surrogate-plus-fallback scan.  Returns (survive, info)."""
+def boundary_fallback(alive, prob, offsets, n_angles, n_lost=3):
+    """The particles that can decide the DA boundary: on each line (lattice,
+    angle; rows angle-major, amplitude increasing) those below the first
+    particle lost within the feature turns whose probability is not above
+    P_HIGH, in order of amplitude, up to and including the n_lost-th with
+    probability below 0.5.  alive, prob: per particle (host arrays).
+    Returns a boolean mask."""
+    alive = np.asarray(alive).astype(bool)
+    prob = np.asarray(prob)
+    sel = np.zeros(len(alive), dtype=bool)
+    for k in range(len(offsets) - 1):
+        m = (offsets[k + 1] - offsets[k]) // n_angles
+        for a in range(n_angles):
+            base = offsets[k] + a * m
+            seen = 0
+            for i in range(base, base + m):
+                if not alive[i]:
+                    break
+                if prob[i] > P_HIGH:
+                    continue
+                sel[i] = True
+                if prob[i] < 0.5:
+                    seen += 1
+                    if seen == n_lost:
+                        break
+    return sel
+
+
+def scan(ctx, tabs, x, offsets, n_long, mlp, std, twiss, cfg=None, progress=None, mode="boundary", n_angles=8):
+    """The surrogate-plus-fallback scan (mode "boundary" or "band", see the
+    module's description).  Returns (survive, info)."""
     torch = _torch()
     t0 = time.perf_counter()
     batch = Batch(ctx, tabs)
@@ -175,8 +211,15 @@ def scan(ctx, tabs, x, offsets, n_long, mlp, std, twiss, cfg=None, progress=None
     t_feat = time.perf_counter() - t0
     p = predict(ctx, mlp, std, d_f)
     alive = d_s != 0
-    fallback = alive & (p >= P_LOW) & (p <= P_HIGH)
-    decided_ok = alive & (p > P_HIGH)
+    if mode == "boundary":
+        sel = boundary_fallback(alive.cpu().numpy(), p.cpu().numpy(), offsets, n_angles)
+        fallback = torch.as_tensor(sel, device=alive.device)
+        decided_ok = alive & ~fallback & (p >= 0.5)
+    elif mode == "band":
+        fallback = alive & (p >= P_LOW) & (p <= P_HIGH)
+        decided_ok = alive & (p > P_HIGH)
+    else:
+        raise ValueError(f"unknown fallback mode {mode!r}")
     if progress:
         progress(f"surrogate: features and predictions in {time.perf_counter() - t0:.1f} s")
     n_fb = track_to(ctx, tabs, d_x, d_s, d_l, offsets, fallback, 2 * WINDOW, n_long, cfg, progress)
@@ -185,7 +228,8 @@ def scan(ctx, tabs, x, offsets, n_long, mlp, std, twiss, cfg=None, progress=None
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return surv, dict(wall=wall, t_features=t_feat, n_fallback=int(n_fb), n_alive_1k=int(alive.sum().item()),
-                      plain=alive.cpu().numpy().astype(np.int32), prob=p.cpu().numpy())
+                      plain=alive.cpu().numpy().astype(np.int32), prob=p.cpu().numpy(),
+                      fallback=fallback.cpu().numpy().astype(bool))
 
 
 def full(ctx, tabs, x, offsets, n_long, cfg=None, progress=None):

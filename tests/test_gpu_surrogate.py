@@ -1,8 +1,9 @@
 """Surrogate-assisted DA scan on a small set of lattices: the pipeline
 (turn-by-turn capture, GPU tune finder, features, MLP training and
-inference, fallback tracking) runs end to end; fallback particles get
-exactly the full-tracking outcome; the network-assisted scan is at least as
-accurate as the plain method (survived the first 1,024 turns = survives)."""
+inference, boundary-aware fallback tracking) runs end to end; fallback
+particles get exactly the full-tracking outcome; the network-assisted DA
+boundaries are closer to full tracking's than the plain method's (survived
+the first 1,024 turns = survives)."""
 import numpy as np
 import pytest
 import torch
@@ -38,7 +39,7 @@ def test_surrogate_pipeline(ctx):
     mlp, std = S.train(ctx, f_tr, y_tr, epochs=20, batch=256, log=log)
     assert log[-1] < log[0]
 
-    gp, etabs, ex, eoffs = lattices(ctx, [2026, 2027], 512)
+    gp, etabs, ex, eoffs = lattices(ctx, [2026, 2027, 2028], 512)
     ref, _, rx, rl = S.full(ctx, etabs, ex, eoffs, N_LONG)
     surv, info = S.scan(ctx, etabs, ex, eoffs, N_LONG, mlp, std, twiss)
     p = info["prob"]
@@ -47,8 +48,20 @@ def test_surrogate_pipeline(ctx):
     # decided-lost particles (lost within 1,024 turns) agree with the reference
     assert np.all(ref[plain == 0] == 0)
     # fallback particles carry exactly the full-tracking outcome
-    fb = (plain == 1) & (p >= S.P_LOW) & (p <= S.P_HIGH)
+    fb = info["fallback"]
+    assert 0 < fb.sum() < (plain == 1).sum()
     assert np.array_equal(surv[fb], ref[fb])
-    acc_net = (surv == ref).mean()
-    acc_plain = (plain == ref).mean()
-    assert acc_net >= acc_plain - 0.002, (acc_net, acc_plain)
+    # DA boundary per lattice and angle: the network's is strictly closer to
+    # full tracking's than the plain method's
+    amps = (np.arange(512 // 8) + 1) * 20. / (512 // 8)
+    err_net, err_plain = [], []
+    for k in range(len(etabs)):
+        sl = slice(eoffs[k], eoffs[k + 1])
+        d_ref = S.da_boundary(ref[sl], amps, 8)
+        err_net.append(np.abs(S.da_boundary(surv[sl], amps, 8) - d_ref))
+        err_plain.append(np.abs(S.da_boundary(plain[sl], amps, 8) - d_ref))
+    err_net, err_plain = np.concatenate(err_net), np.concatenate(err_plain)
+    print(f"\nDA boundary error (sigma): network mean {err_net.mean():.3f} max {err_net.max():.3f}; "
+          f"plain mean {err_plain.mean():.3f} max {err_plain.max():.3f}; fallback {fb.sum()} of {(plain == 1).sum()}")
+    assert err_plain.mean() > 0, "the horizon must see losses after the feature turns"
+    assert err_net.mean() < err_plain.mean()
