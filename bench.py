@@ -108,6 +108,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=60, help="config3: training epochs")
     ap.add_argument("--fallback", choices=("boundary", "band"), default="boundary",
                     help="config3: which particles fall back to full tracking")
+    ap.add_argument("--dump", default=None,
+                    help="config3: write per-particle outcomes (full tracking, network, plain method) to this .npz")
     ap.add_argument("--orbit", action="store_true", help="config4: orbit tracking with quantum excitation "
                     "instead of spin tracking")
     args = ap.parse_args()
@@ -557,6 +559,11 @@ def bench_config3(args):
         err_net.append(np.abs(S.da_boundary(surv[sl], amps, n_ang) - d_ref))
         err_plain.append(np.abs(S.da_boundary(info["plain"][sl], amps, n_ang) - d_ref))
     err_net, err_plain = np.array(err_net), np.array(err_plain)
+    if args.dump:
+        np.savez_compressed(args.dump, ref=ref.astype(np.int8), surv=surv.astype(np.int8),
+                            plain=info["plain"].astype(np.int8), prob=info["prob"].astype(np.float32),
+                            fallback=info["fallback"], offsets=eoffs, amps=amps, n_angles=n_ang,
+                            n_long=args.n_long, t_full=t_full, t_scan=info["wall"], t_features=info["t_features"])
     speedup = t_full / info["wall"]
     out = {
         "metric": "DA-scan wall time, surrogate+fallback vs full track", "value": speedup, "unit": "x (speedup)",
