@@ -108,6 +108,19 @@ def main():
     ap.add_argument("--epochs", type=int, default=60, help="config3: training epochs")
     ap.add_argument("--fallback", choices=("boundary", "band"), default="boundary",
                     help="config3: which particles fall back to full tracking")
+    ap.add_argument("--p-safe", default="0.999",
+                    help="config3, boundary fallback: trust threshold(s) of the network's 'safe' verdict; a comma "
+                         "list runs one scan per value (the first is the reported one)")
+    ap.add_argument("--stage", choices=("all", "train", "full", "scan"), default="all",
+                    help="config3: run one part of the comparison and save its state, so that a long horizon "
+                         "can be split over several processes (train -> --state-out; full and scan: from "
+                         "--state-in, if given, to --turns, then --state-out; scan needs --mlp, and --ref "
+                         "once it reaches --n-long)")
+    ap.add_argument("--turns", type=int, default=None, help="config3 --stage full/scan: stop at this turn")
+    ap.add_argument("--state-in", default=None, help="config3 --stage full/scan: state to continue")
+    ap.add_argument("--state-out", default=None, help="config3 --stage: where to write the state")
+    ap.add_argument("--mlp", default=None, help="config3 --stage scan: the trained network (--stage train's output)")
+    ap.add_argument("--ref", default=None, help="config3 --stage scan: full tracking's final state")
     ap.add_argument("--dump", default=None,
                     help="config3: write per-particle outcomes (full tracking, network, plain method) to this .npz")
     ap.add_argument("--orbit", action="store_true", help="config4: orbit tracking with quantum excitation "
@@ -527,13 +540,19 @@ def bench_config3(args):
     _, _, x1, _ = W.config2(n_seeds=1, n_per_seed=args.per_seed, n_angles=n_ang)
     rng = np.random.default_rng(7)
 
-    def lattices(seeds):
-        lines = [W.crab_ring(s, int(rng.integers(1000, 1801)), gp) for s in seeds]
+    def lattices(seeds, build=True):
+        sizes = [int(rng.integers(1000, 1801)) for _ in seeds]     # drawn in the same order in every stage
+        if not build:
+            return None
+        lines = [W.crab_ring(s, n, gp) for s, n in zip(seeds, sizes)]
         tabs = [st.Table(ctx, gp, ln.records()) for ln in lines]
         x = np.concatenate([x1] * len(seeds))
         offs = np.arange(len(seeds) + 1, dtype=np.int32) * len(x1)
         return tabs, x, offs
 
+    if args.stage in ("full", "scan"):
+        lattices(range(3000, 3000 + args.train_seeds), build=False)
+        return bench_config3_stage(args, ctx, lattices, x1, n_ang, twiss, progress)
     t0 = time.perf_counter()
     tabs, x, offs = lattices(range(3000, 3000 + args.train_seeds))
     batch = st.Batch(ctx, tabs)
@@ -546,19 +565,34 @@ def bench_config3(args):
     progress(f"trained: loss {log[0]:.4f} -> {log[-1]:.4f}")
     t_train = time.perf_counter() - t0
     del d_x, d_s, d_l, d_f, batch
+    if args.stage == "train":
+        np.savez(args.state_out, params=mlp.get_params(), mean=std.mean.cpu().numpy(), std=std.std.cpu().numpy(),
+                 log=np.array(log), t_train=t_train, n_long=args.n_long, train_seeds=args.train_seeds)
+        print(json.dumps({"stage": "train", "n_long": args.n_long, "train_s": t_train,
+                          "train_loss_first_last": [log[0], log[-1]], "state": args.state_out}))
+        return
 
     etabs, ex, eoffs = lattices(range(2026, 2026 + args.seeds))
     ref, t_full, _, _ = S.full(ctx, etabs, ex, eoffs, args.n_long, progress=progress)
-    surv, info = S.scan(ctx, etabs, ex, eoffs, args.n_long, mlp, std, twiss, progress=progress, mode=args.fallback,
-                        n_angles=n_ang)
     amps = (np.arange(len(x1) // n_ang) + 1) * 20. / (len(x1) // n_ang)
-    err_net, err_plain = [], []
-    for k in range(len(etabs)):
-        sl = slice(eoffs[k], eoffs[k + 1])
-        d_ref = S.da_boundary(ref[sl], amps, n_ang)
-        err_net.append(np.abs(S.da_boundary(surv[sl], amps, n_ang) - d_ref))
-        err_plain.append(np.abs(S.da_boundary(info["plain"][sl], amps, n_ang) - d_ref))
-    err_net, err_plain = np.array(err_net), np.array(err_plain)
+    d_ref = [S.da_boundary(ref[eoffs[k]:eoffs[k + 1]], amps, n_ang) for k in range(len(etabs))]
+
+    def da_errors(surv):
+        return np.array([np.abs(S.da_boundary(surv[eoffs[k]:eoffs[k + 1]], amps, n_ang) - d_ref[k])
+                         for k in range(len(etabs))])
+
+    def summary(err):
+        return {"max": float(err.max()), "mean": float(err.mean()),
+                "lattices_within_0.25": int((err.max(axis=1) <= 0.25).sum())}
+
+    runs = []
+    for p_safe in [float(v) for v in args.p_safe.split(",")]:
+        surv, info = S.scan(ctx, etabs, ex, eoffs, args.n_long, mlp, std, twiss, progress=progress,
+                            mode=args.fallback, n_angles=n_ang, p_safe=p_safe)
+        runs.append((p_safe, surv, info, da_errors(surv)))
+        progress(f"p_safe {p_safe}: {t_full / info['wall']:.2f}x, DA error {summary(runs[-1][3])}")
+    p_safe, surv, info, err_net = runs[0]
+    err_plain = da_errors(info["plain"])
     if args.dump:
         np.savez_compressed(args.dump, ref=ref.astype(np.int8), surv=surv.astype(np.int8),
                             plain=info["plain"].astype(np.int8), prob=info["prob"].astype(np.float32),
@@ -573,17 +607,92 @@ def bench_config3(args):
                                f"lattices from their first 1,024 turns",
                    "mlp": "12-256-256-256-1 BF16 MFMA"},
         "full_track_s": t_full, "surrogate_scan_s": info["wall"], "features_s": info["t_features"],
-        "fallback": args.fallback, "fallback_particles": info["n_fallback"],
+        "fallback": args.fallback, "p_safe": p_safe if args.fallback == "boundary" else None,
+        "fallback_particles": info["n_fallback"],
         "fallback_fraction": info["n_fallback"] / max(info["n_alive_1k"], 1),
-        "da_error_sigma": {"max": float(err_net.max()), "mean": float(err_net.mean()),
-                           "lattices_within_0.25": int((err_net.max(axis=1) <= 0.25).sum())},
-        "plain_method_da_error_sigma": {"max": float(err_plain.max()), "mean": float(err_plain.mean()),
-                                        "lattices_within_0.25": int((err_plain.max(axis=1) <= 0.25).sum())},
+        "da_error_sigma": summary(err_net),
+        "plain_method_da_error_sigma": summary(err_plain),
         "survival_accuracy": {"surrogate": float((surv == ref).mean()), "plain": float((info["plain"] == ref).mean())},
+        "tradeoff": [{"p_safe": ps, "speedup": t_full / inf["wall"], "surrogate_scan_s": inf["wall"],
+                      "fallback_particles": inf["n_fallback"], "da_error_sigma": summary(er)}
+                     for ps, _, inf, er in runs[1:]],
         "train_s": t_train, "train_loss_first_last": [log[0], log[-1]], "train_lattices": args.train_seeds,
         "features": "tunes of Hann-windowed 512-turn windows (FineTuneFinder on w*x)",
     }
     print(json.dumps(out))
+
+
+def bench_config3_stage(args, ctx, lattices, x1, n_ang, twiss, progress):
+    """One part of the config-3 comparison on the evaluation lattices, from a
+    saved state to args.turns: full tracking, or the surrogate scan.  Each
+    part's wall time accumulates in its state; the scan's last part prints
+    the same line as the one-process run."""
+    import numpy as np
+    import torch
+
+    import simtrack_amd as st
+    from simtrack_amd import surrogate as S
+
+    etabs, ex, eoffs = lattices(range(2026, 2026 + args.seeds))
+    turn1 = min(args.turns or args.n_long, args.n_long)
+    if args.stage == "full":
+        state = S.load_state(args.state_in) if args.state_in else S.full_start(ex)
+        S.full_track(ctx, etabs, state, eoffs, turn1, progress=progress)
+        S.save_state(state, args.state_out)
+        print(json.dumps({"stage": "full", "turn": state["turn"], "n_long": args.n_long, "elapsed_s": state["elapsed"],
+                          "alive": int((state["d_s"] != 0).sum().item()), "state": args.state_out}))
+        return
+    z = np.load(args.mlp)
+    if int(z["n_long"]) != args.n_long:
+        raise SystemExit(f"--mlp was trained for survival to {int(z['n_long'])} turns, not {args.n_long}")
+    mlp = st.Mlp(ctx, 12, 256)
+    mlp.set_params(z["params"])
+    dev = torch.device("cuda:0")
+    std = S.Standardizer(mean=torch.as_tensor(z["mean"], device=dev), std=torch.as_tensor(z["std"], device=dev))
+    p_safe = float(args.p_safe.split(",")[0])
+    if args.state_in:
+        state = S.load_state(args.state_in)
+    else:
+        state = S.scan_start(ctx, etabs, ex, eoffs, mlp, std, twiss, progress=progress, mode=args.fallback,
+                             n_angles=n_ang, p_safe=p_safe)
+    S.scan_track(ctx, etabs, state, eoffs, turn1, progress=progress)
+    if state["turn"] < args.n_long:
+        S.save_state(state, args.state_out)
+        print(json.dumps({"stage": "scan", "turn": state["turn"], "n_long": args.n_long,
+                          "elapsed_s": state["elapsed"], "state": args.state_out}))
+        return
+    surv, info = S.scan_finish(state)
+    r = np.load(args.ref)
+    if int(r["turn"]) != args.n_long:
+        raise SystemExit(f"--ref reached turn {int(r['turn'])}, not {args.n_long}")
+    ref, t_full = (r["d_s"] != 0).astype(np.int32), float(r["elapsed"])
+    amps = (np.arange(len(x1) // n_ang) + 1) * 20. / (len(x1) // n_ang)
+    err_net, err_plain = [], []
+    for k in range(len(etabs)):
+        sl = slice(eoffs[k], eoffs[k + 1])
+        d_ref = S.da_boundary(ref[sl], amps, n_ang)
+        err_net.append(np.abs(S.da_boundary(surv[sl], amps, n_ang) - d_ref))
+        err_plain.append(np.abs(S.da_boundary(info["plain"][sl], amps, n_ang) - d_ref))
+    err_net, err_plain = np.array(err_net), np.array(err_plain)
+
+    def summary(err):
+        return {"max": float(err.max()), "mean": float(err.mean()),
+                "lattices_within_0.25": int((err.max(axis=1) <= 0.25).sum())}
+    print(json.dumps({
+        "metric": "DA-scan wall time, surrogate+fallback vs full track", "value": t_full / info["wall"],
+        "unit": "x (speedup)", "n_gpus": 1, "higher_is_better": True, "dtype": "f64 tracking, bf16/fp32 network",
+        "data": "synthetic",
+        "config": {"workload": f"config3: {args.seeds} evaluation lattices x {len(x1)} DA-grid particles, "
+                               f"survival to {args.n_long} turns, run in parts (times summed)",
+                   "mlp": "12-256-256-256-1 BF16 MFMA"},
+        "full_track_s": t_full, "surrogate_scan_s": info["wall"], "features_s": info["t_features"],
+        "fallback": args.fallback, "p_safe": p_safe if args.fallback == "boundary" else None,
+        "fallback_particles": info["n_fallback"],
+        "fallback_fraction": info["n_fallback"] / max(info["n_alive_1k"], 1),
+        "da_error_sigma": summary(err_net), "plain_method_da_error_sigma": summary(err_plain),
+        "survival_accuracy": {"surrogate": float((surv == ref).mean()), "plain": float((info["plain"] == ref).mean())},
+        "train_s": float(z["t_train"]), "train_n_long": int(z["n_long"]), "train_lattices": int(z["train_seeds"]),
+    }))
 
 
 if __name__ == "__main__":

@@ -27,6 +27,13 @@
  * context are serialised by the caller.  All calls block until the GPU work
  * they issue has finished, except the *_device entry points, which are
  * ordered on the context's stream and return after launching.
+ *
+ * Stream ordering: the context's stream is a non-blocking stream, not
+ * ordered after any other.  A caller whose device inputs were written on
+ * another stream (a framework's current stream, a copy stream) makes that
+ * work finish first (synchronise the stream, or make it wait on an event),
+ * and reads device outputs only after st_ctx_sync or a blocking call on the
+ * same context.
  */
 #ifndef SIMTRACK_AMD_H
 #define SIMTRACK_AMD_H

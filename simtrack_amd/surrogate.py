@@ -12,17 +12,18 @@ The tracker itself produces the training data and the features:
      tracking from where step 1 left them, to the long horizon.
 A particle lost within the first 1,024 turns is lost.  The DA boundary of
 one angle of one lattice is the amplitude of the first particle (by
-increasing amplitude) that does not survive, so only the particles below
-the first early loss can move it, and of those only the ones the network
-does not call safe (probability above 0.9).  The boundary-aware fallback
-(the default) tracks, on each line of the grid, those particles in order of
-amplitude up to and including the third one the network predicts lost
-(probability below 0.5), like Track_DA's search for the first loss
-(simtrack.cpp:21403-21435), here resolved in one batched pass; untracked
-particles take the network's verdict.  The band fallback tracks every
-particle with probability between 0.1 and 0.9.  Full tracking of every
-particle to the long horizon is the reference, and judges the surrogate's
-errors.
+increasing amplitude) that does not survive, as Track_DA_Uniform's first
+loss (simtrack.cpp:21574), so only the particles below the first early loss
+can move it, and of those only the ones the network does not call safe.
+The boundary-aware fallback (the default) tracks, on each line of the grid,
+every particle below the first early loss whose predicted survival
+probability is at most p_safe (0.999 by default), in one batched pass;
+untracked particles take the network's verdict.  A lost particle that the
+network calls safe is the only way the boundary can come out wrong, so
+p_safe trades fallback particles for boundary errors.  The band fallback
+tracks every particle with probability between 0.1 and 0.9.  Full tracking
+of every particle to the long horizon is the reference, and judges the
+surrogate's errors.
 """
 import time
 
@@ -39,6 +40,7 @@ def hann(n):
     (NAFF's choice), whose leakage error falls as 1/n^3 instead of 1/n."""
     return 1.0 - np.cos(2 * np.pi * np.arange(n) / n)
 P_LOW, P_HIGH = 0.1, 0.9
+P_SAFE = 0.999
 
 
 def _torch():
@@ -118,9 +120,12 @@ def track_to(ctx, tabs, d_x, d_s, d_l, offsets, sel, turn0, turn1, cfg=None, pro
 
 
 class Standardizer:
-    def __init__(self, f):
-        self.mean = f.mean(dim=0)
-        self.std = f.std(dim=0).clamp_min(1e-6)
+    def __init__(self, f=None, mean=None, std=None):
+        """From features f (per-column mean and standard deviation), or from
+        a saved mean and std."""
+        if f is not None:
+            mean, std = f.mean(dim=0), f.std(dim=0).clamp_min(1e-6)
+        self.mean, self.std = mean, std
 
     def __call__(self, f):
         return ((f - self.mean) / self.std).contiguous()
@@ -173,37 +178,32 @@ def da_boundary(survive, amps, n_angles):
     return out
 
 
-def boundary_fallback(alive, prob, offsets, n_angles, n_lost=3):
+def boundary_fallback(alive, prob, offsets, n_angles, p_safe=P_SAFE):
     """The particles that can decide the DA boundary: on each line (lattice,
     angle; rows angle-major, amplitude increasing) those below the first
-    particle lost within the feature turns whose probability is not above
-    P_HIGH, in order of amplitude, up to and including the n_lost-th with
-    probability below 0.5.  alive, prob: per particle (host arrays).
-    Returns a boolean mask."""
+    particle lost within the feature turns, except the ones the network
+    calls safe (probability above p_safe).  alive, prob: per particle (host
+    arrays).  Returns a boolean mask."""
     alive = np.asarray(alive).astype(bool)
     prob = np.asarray(prob)
     sel = np.zeros(len(alive), dtype=bool)
     for k in range(len(offsets) - 1):
-        m = (offsets[k + 1] - offsets[k]) // n_angles
-        for a in range(n_angles):
-            base = offsets[k] + a * m
-            seen = 0
-            for i in range(base, base + m):
-                if not alive[i]:
-                    break
-                if prob[i] > P_HIGH:
-                    continue
-                sel[i] = True
-                if prob[i] < 0.5:
-                    seen += 1
-                    if seen == n_lost:
-                        break
+        a = alive[offsets[k]:offsets[k + 1]].reshape(n_angles, -1)
+        p = prob[offsets[k]:offsets[k + 1]].reshape(n_angles, -1)
+        m = a.shape[1]
+        first_lost = np.where((~a).any(axis=1), (~a).argmax(axis=1), m)
+        below = np.arange(m)[None, :] < first_lost[:, None]
+        sel[offsets[k]:offsets[k + 1]] = (below & (p <= p_safe)).reshape(-1)
     return sel
 
 
-def scan(ctx, tabs, x, offsets, n_long, mlp, std, twiss, cfg=None, progress=None, mode="boundary", n_angles=8):
-    """The surrogate-plus-fallback scan (mode "boundary" or "band", see the
-    module's description).  Returns (survive, info)."""
+def scan_start(ctx, tabs, x, offsets, mlp, std, twiss, cfg=None, progress=None, mode="boundary", n_angles=8,
+               p_safe=P_SAFE):
+    """First part of the scan: the feature turns of every particle, the
+    network's predictions and the choice of fallback particles (mode
+    "boundary" or "band", see the module's description; p_safe: the
+    boundary mode's trust threshold).  Returns the scan state, which
+    scan_track continues and scan_finish reads."""
     torch = _torch()
     t0 = time.perf_counter()
     batch = Batch(ctx, tabs)
@@ -212,7 +212,7 @@ def scan(ctx, tabs, x, offsets, n_long, mlp, std, twiss, cfg=None, progress=None
     p = predict(ctx, mlp, std, d_f)
     alive = d_s != 0
     if mode == "boundary":
-        sel = boundary_fallback(alive.cpu().numpy(), p.cpu().numpy(), offsets, n_angles)
+        sel = boundary_fallback(alive.cpu().numpy(), p.cpu().numpy(), offsets, n_angles, p_safe)
         fallback = torch.as_tensor(sel, device=alive.device)
         decided_ok = alive & ~fallback & (p >= 0.5)
     elif mode == "band":
@@ -220,29 +220,104 @@ def scan(ctx, tabs, x, offsets, n_long, mlp, std, twiss, cfg=None, progress=None
         decided_ok = alive & (p > P_HIGH)
     else:
         raise ValueError(f"unknown fallback mode {mode!r}")
+    torch.cuda.synchronize()
     if progress:
         progress(f"surrogate: features and predictions in {time.perf_counter() - t0:.1f} s")
-    n_fb = track_to(ctx, tabs, d_x, d_s, d_l, offsets, fallback, 2 * WINDOW, n_long, cfg, progress)
-    surv = torch.where(fallback, d_s != 0, decided_ok)
+    return dict(d_x=d_x, d_s=d_s, d_l=d_l, p=p, alive=alive, fallback=fallback, decided_ok=decided_ok,
+                turn=2 * WINDOW, elapsed=time.perf_counter() - t0, t_features=t_feat,
+                n_fallback=int(fallback.sum().item()))
+
+
+def scan_track(ctx, tabs, st, offsets, turn1, cfg=None, progress=None):
+    """Continue the scan's fallback particles from st["turn"] to turn1; the
+    time is added to st["elapsed"]."""
+    t0 = time.perf_counter()
+    track_to(ctx, tabs, st["d_x"], st["d_s"], st["d_l"], offsets, st["fallback"], st["turn"], turn1, cfg, progress)
+    st["turn"] = max(st["turn"], turn1)
+    st["elapsed"] += time.perf_counter() - t0
+
+
+def scan_finish(st):
+    """(survive, info) of a scan whose fallback particles reached the horizon."""
+    torch = _torch()
+    surv = torch.where(st["fallback"], st["d_s"] != 0, st["decided_ok"])
     surv = surv.cpu().numpy().astype(np.int32)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    return surv, dict(wall=wall, t_features=t_feat, n_fallback=int(n_fb), n_alive_1k=int(alive.sum().item()),
-                      plain=alive.cpu().numpy().astype(np.int32), prob=p.cpu().numpy(),
-                      fallback=fallback.cpu().numpy().astype(bool))
+    return surv, dict(wall=st["elapsed"], t_features=st["t_features"], n_fallback=st["n_fallback"],
+                      n_alive_1k=int(st["alive"].sum().item()),
+                      plain=st["alive"].cpu().numpy().astype(np.int32), prob=st["p"].cpu().numpy(),
+                      fallback=st["fallback"].cpu().numpy().astype(bool))
 
 
-def full(ctx, tabs, x, offsets, n_long, cfg=None, progress=None):
-    """Reference: every particle tracked to n_long.  Returns (survive, wall)."""
+def scan(ctx, tabs, x, offsets, n_long, mlp, std, twiss, cfg=None, progress=None, mode="boundary", n_angles=8,
+         p_safe=P_SAFE):
+    """The surrogate-plus-fallback scan to n_long turns in one go (scan_start,
+    scan_track, scan_finish).  Returns (survive, info); info["wall"] is the
+    scan's wall time."""
+    st = scan_start(ctx, tabs, x, offsets, mlp, std, twiss, cfg, progress, mode, n_angles, p_safe)
+    scan_track(ctx, tabs, st, offsets, n_long, cfg, progress)
+    return scan_finish(st)
+
+
+def save_state(st, path):
+    """Write a scan or full-tracking state (a dict of device tensors and
+    numbers) to an .npz, keeping x only for particles still being tracked,
+    so that a long horizon can be split over several processes."""
+    torch = _torch()
+    live = st["d_s"] != 0
+    if "fallback" in st:
+        live = live & st["fallback"]
+    idx = torch.nonzero(live).flatten()
+    out = {"_idx": idx.cpu().numpy(), "_x": st["d_x"][idx].cpu().numpy(), "_n": len(st["d_s"])}
+    for k, v in st.items():
+        if k == "d_x":
+            continue
+        out[k] = v.cpu().numpy() if torch.is_tensor(v) else np.asarray(v)
+    np.savez_compressed(path, **out)
+
+
+def load_state(path, device="cuda:0"):
+    """The state save_state wrote, back on the device."""
+    torch = _torch()
+    z = np.load(path)
+    st = {}
+    for k in z.files:
+        if k.startswith("_"):
+            continue
+        v = z[k]
+        st[k] = torch.as_tensor(v, device=device) if v.ndim else v.item()
+    d_x = torch.zeros((int(z["_n"]), 6), dtype=torch.float64, device=device)
+    d_x[torch.as_tensor(z["_idx"], device=device)] = torch.as_tensor(z["_x"], device=device)
+    st["d_x"] = d_x
+    return st
+
+
+def full_start(x, device="cuda:0"):
+    """Full-tracking state of the particles x at turn 0."""
+    torch = _torch()
+    d_x = torch.tensor(x, dtype=torch.float64, device=device).contiguous()
+    return dict(d_x=d_x, d_s=torch.ones(len(x), dtype=torch.int32, device=device),
+                d_l=torch.zeros(2 * len(x), dtype=torch.int32, device=device), turn=0, elapsed=0.)
+
+
+def full_track(ctx, tabs, st, offsets, turn1, cfg=None, progress=None):
+    """Continue full tracking of every particle from st["turn"] to turn1; the
+    time is added to st["elapsed"]."""
     torch = _torch()
     t0 = time.perf_counter()
     batch = Batch(ctx, tabs)
-    dev = torch.device("cuda:0")
-    d_x = torch.tensor(x, dtype=torch.float64, device=dev).contiguous()
-    d_s = torch.ones(len(x), dtype=torch.int32, device=dev)
-    d_l = torch.zeros(2 * len(x), dtype=torch.int32, device=dev)
+    d_x, d_s, d_l = st["d_x"], st["d_s"], st["d_l"]
     torch.cuda.synchronize()
     _chunked(lambda t, nt: (batch.track_device(d_x.data_ptr(), offsets, t, nt, d_s.data_ptr(), d_l.data_ptr(), 0, cfg),
-                            ctx.sync()), 0, n_long, progress, "full tracking")
-    s = d_s.cpu().numpy().astype(np.int32)
-    return s, time.perf_counter() - t0, d_x, d_l
+                            ctx.sync()), st["turn"], turn1, progress, "full tracking")
+    st["turn"] = max(st["turn"], turn1)
+    st["elapsed"] += time.perf_counter() - t0
+
+
+def full(ctx, tabs, x, offsets, n_long, cfg=None, progress=None):
+    """Reference: every particle tracked to n_long.  Returns (survive, wall,
+    d_x, d_lost)."""
+    t0 = time.perf_counter()
+    st = full_start(x)
+    full_track(ctx, tabs, st, offsets, n_long, cfg, progress)
+    s = st["d_s"].cpu().numpy().astype(np.int32)
+    return s, time.perf_counter() - t0, st["d_x"], st["d_l"]
