@@ -369,14 +369,22 @@ ST_DEV void oct_kick(St &s, double k3l, double k3sl)
 /* n! as double for n = 0..10 (the reference's int fac, converted at the divide) */
 __device__ __constant__ const double st_fact[11] = {1., 1., 2., 6., 24., 120., 720., 5040., 40320., 362880.,
                                                     3628800.};
+/* 1/n! rounded to nearest.  With the correctly rounded reciprocal y of b,
+   q0 = a*y, r = fma(-b, q0, a) (exact) and q = fma(r, y, q0) is the
+   correctly rounded a/b whenever nothing underflows or overflows
+   (Markstein's residual correction), so ArFast::div with {n!, 1/n!} gives
+   the bits of a / n! inside its [2^-767, 2^700) window on |q0|; outside it
+   the step is replayed with the division itself. */
+__device__ __constant__ const double st_rfact[11] = {1., 1., 1. / 2., 1. / 6., 1. / 24., 1. / 120., 1. / 720.,
+                                                     1. / 5040., 1. / 40320., 1. / 362880., 1. / 3628800.};
 
 /*
  * Field sum of mult_kick_pass / bend_mult_kick_pass / cal_Bfield
  * (simtrack.cpp:4086-4111, 4113-4137, 4169-4196): By + i Bx accumulated over
  * orders 1..Norder whose coefficients are not both zero (mask bit i).
  */
-ST_DEV void mult_field(const St &s, int norder, int mask, st_pp kn,
-                       st_pp ks, double &By, double &Bx)
+template <class A>
+ST_DEV void mult_field_t(A &ar, const St &s, int norder, int mask, st_pp kn, st_pp ks, double &By, double &Bx)
 {
     const double *x = s.x;
     double Xn = 1., Yn = 0.;
@@ -387,10 +395,26 @@ ST_DEV void mult_field(const St &s, int norder, int mask, st_pp kn,
         Xn = Xn0 * x[0] - Yn0 * x[2];
         Yn = Xn0 * x[2] + Yn0 * x[0];
         if ((mask >> i) & 1) {
-            double fac = st_fact[i];
-            By = By + (kn[i] * Xn - ks[i] * Yn) / fac;
-            Bx = Bx + (kn[i] * Yn + ks[i] * Xn) / fac;
+            StRcp rf;
+            rf.b = st_fact[i];
+            rf.y = st_rfact[i];
+            rf.ok = true;
+            By = By + ar.div(kn[i] * Xn - ks[i] * Yn, rf);
+            Bx = Bx + ar.div(kn[i] * Yn + ks[i] * Xn, rf);
         }
+    }
+}
+
+/* the field sum with the factorial divisions as reciprocal steps, replayed
+   with the divisions in the lanes outside their window */
+ST_DEV void mult_field(const St &s, int norder, int mask, st_pp kn, st_pp ks, double &By, double &Bx)
+{
+    ArFast af;
+    mult_field_t(af, s, norder, mask, kn, ks, By, Bx);
+    if (__builtin_expect(af.bad(), 0)) {
+        ST_FD_COLD_PATH();
+        ArExact ae;
+        mult_field_t(ae, s, norder, mask, kn, ks, By, Bx);
     }
 }
 
