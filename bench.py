@@ -132,7 +132,7 @@ def main():
         sys.exit(2)
 
     if args.cpu_sample is None:
-        args.cpu_sample = {"config4": "2048x24", "config2": "2048x32"}.get(args.workload, "1024x64")
+        args.cpu_sample = {"config4": "2048x24", "config2": "2048x512"}.get(args.workload, "1024x64")
     if args.workload == "config2":
         return bench_config2(args)
     if args.workload == "config3":
