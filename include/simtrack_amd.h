@@ -219,6 +219,15 @@ int st_track_radial(st_table *t, double *x, int np, int nturn,
 int st_element_pass(st_table *t, int elem_index, double *x, int np, int spin);
 
 /*
+ * Track_wrt_OnMomentumPart (simtrack.cpp:20942-20966): Track with dz0[j]
+ * (one value per element, the on-momentum particle's z after element j)
+ * subtracted from z after element j, the caller's version of
+ * st_track_radial's own x04.
+ */
+int st_track_dz(st_table *t, const double *dz0, double *x, int np, int nturn, int *stable, int *lost_turn,
+                int *lost_post);
+
+/*
  * Device-resident tracking of a turn range, for callers that keep particle
  * state in HBM (device pointers from hipMalloc or a framework allocator):
  * d_x is np x 6 (x 9 with spin), d_state is np ints holding stable, d_lost
@@ -279,24 +288,33 @@ int st_ctx_kernel_stats(st_ctx *ctx, double *kernel_seconds, int64_t *launches, 
  * ST_DA_TRACK_DA     Track_DA        (simtrack.cpp:21389): 10 angles
  *                    (90/11)(i+1) deg, sigma in [0, 50];
  * ST_DA_TRACK_DA1    Track_DA1       (cpp:21443): 5 angles 15(i+1) deg, [0, 50];
- * ST_DA_TRACK_DA_RADIAL Track_DA_Radial (cpp:21682): 10 angles, [0, 15].
+ * ST_DA_TRACK_DA_RADIAL Track_DA_Radial (cpp:21682): 10 angles, [0, 15];
+ * ST_DA_TRACK_DA2    Track_DA2       (cpp:21497): 10 angles, [0, 50];
+ * ST_DA_TRACK_DA3    Track_DA3       (cpp:21535): 5 angles 15(i+1) deg, [0, 30].
  * angles (radians) needs room for 10.
  */
 #define ST_DA_TRACK_DA 0
 #define ST_DA_TRACK_DA1 1
 #define ST_DA_TRACK_DA_RADIAL 2
+#define ST_DA_TRACK_DA2 3
+#define ST_DA_TRACK_DA3 4
 int st_da_angles(int variant, double *angles, int *nangle, double *sigma_max);
 
 /*
- * The twin-particle bisection of Track_DA / Track_DA1 / Track_DA_Radial
- * (cpp:21389-21441, 21443-21495, 21682-21734) for nangle angles at once:
- * particles x = (s cos(a) sigmax0, 0, s sin(a) sigmay0, 0, 0, deltap0),
- * twins at s and s + 0.05 must both survive nturn turns (Track, or
- * Track_Radial if radial != 0); the interval [0, sigma_max] is halved until
- * it is <= 0.1.  sigma_out[i] is the last midpoint tested -- the number the
- * reference writes as "angle sigma".  Replaces the per-particle Track calls
- * (cpp:21417, 21426) with one batched launch over the whole bisection tree.
+ * The bisection of Track_DA / Track_DA1 / Track_DA_Radial (cpp:21389-21441,
+ * 21443-21495, 21682-21734) and Track_DA2 / Track_DA3 (cpp:21497-21572)
+ * for nangle angles at once: particles x = (s cos(a) sigmax0, 0,
+ * s sin(a) sigmay0, 0, 0, deltap0); a midpoint s is stable when its twins
+ * at s and s + 0.05 both survive nturn turns, or with ST_DA_SINGLE in
+ * `radial` when the one particle at s does (Track_DA2/DA3); Track, or
+ * Track_Radial when `radial` has ST_DA_RADIAL (any odd value).  The
+ * interval [0, sigma_max] is halved until it is <= 0.1.  sigma_out[i] is
+ * the last midpoint tested -- the number the reference writes as "angle
+ * sigma".  Replaces the per-particle Track calls (cpp:21417, 21426, 21522)
+ * with one batched launch over the whole bisection tree.
  */
+#define ST_DA_RADIAL 1
+#define ST_DA_SINGLE 2
 int st_da_bisect(st_table *t, int nturn, double deltap0, double sigmax0, double sigmay0,
                  const double *angles, int nangle, double sigma_max, int radial, double *sigma_out);
 
@@ -428,6 +446,38 @@ int st_fine_tune_window_device(st_ctx *ctx, const double *d_x, const double *d_p
  */
 int st_closed_orbit(st_ctx *ctx, const st_globals *gp, const st_element *rec, int n, double deltap,
                     double x_co[6], double x_end[6], int *iterations);
+
+/*
+ * Cal_Spin_Orbit (simtrack.cpp:20510-20572): Cal_Orbit_Num at deltap, then
+ * the periodic spin n0 by the reference's Newton iteration on (Sx, Ss) with
+ * one-turn sPass maps of the line as given (RF on, no loss test; spin
+ * difference 1e-6, tolerance 1e-12, at most 20 iterations, Sy = sqrt(1 -
+ * Sx^2 - Ss^2) after each step).  x0: the orbit Cell[Ncell-1]->X and the
+ * converged spin at the start; n0 (3*n doubles, may be null): the
+ * normalised spin after every element of one more pass, the reference's
+ * Cell[j]->n0.  ST_ERR_PARAM when the reference would report failure.
+ */
+int st_spin_orbit(st_ctx *ctx, const st_globals *gp, const st_element *rec, int n, double deltap, double x0[9],
+                  double *n0, int *iterations);
+
+/*
+ * Cal_Spin_Tune (simtrack.cpp:20574-20601): st_spin_orbit, then 500 turns
+ * of sPass of the closed orbit with a spin perpendicular to the last
+ * element's n0, and FineTuneFinder (on [0.001, 0.499]) of its Sx after
+ * each turn: the reference's Line::SpinTune.  n0_end (may be null): the
+ * last element's n0.
+ */
+int st_spin_tune(st_ctx *ctx, const st_globals *gp, const st_element *rec, int n, double deltap, double *tune,
+                 double n0_end[3]);
+
+/*
+ * Set_Initial_Spin (simtrack.cpp:20603-20635): x's orbit with unit spins
+ * along x, y and s, each tracked nturn turns of sPass (no loss test); the
+ * projections on n0 after every turn are summed, and x[6..8] becomes the
+ * normalised sums.
+ */
+int st_set_initial_spin(st_ctx *ctx, const st_globals *gp, const st_element *rec, int n, int nturn,
+                        const double n0[3], double x[9]);
 
 /*
  * The 6x6 one-turn matrix about x_co (row-major, m[6*i+k] = dx_i/dx_k):

@@ -9,10 +9,13 @@
  *   Element and the element classes below              simtrack.h:373-959
  *   Line                                               simtrack.h:1067-1100, simtrack.cpp:12341-12431
  *   Track, Track_spin, Track_Spin, Track_tbt(_spin),   simtrack.h:1360-1390
- *   Track_Radial, Track_DA, Track_DA1, Track_DA_Radial,
- *   Track_DA_Uniform, Track_DA_Radial_Uniform
+ *   Track_wrt_OnMomentumPart                           simtrack.cpp:20942
+ *   Track_Radial, Track_DA, Track_DA1, Track_DA2, Track_DA3, simtrack.cpp:21389-21790
+ *   Track_DA_Radial, Track_DA_Uniform, Track_DA_Radial_Uniform
  *   Read_MADX_Lattice, Read_BMAD_Lattice               simtrack.cpp:13591, 14054
  *   Cal_Orbit_Num, Track_tbt_FMA, Track_tbt_Lyapunov   simtrack.cpp:14947, 21179, 21243
+ *   Cal_Spin_Orbit, Cal_Spin_Tune, Set_Initial_Spin    simtrack.cpp:20510, 20574, 20603
+ *   Cal_Tunes_Track (both forms), Cal_Chrom_Track      simtrack.cpp:20971, 20992, 21028
  *   BeamBeam4D_SoftGaussian, BeamBeam6D_SoftGaussian   simtrack.cpp:22846, 22874, 23134
  *   (both forms), Zboundary_slices_Gaussian             simtrack.cpp:3157
  *   BeamBeam4D_PIC, BeamBeam6D_PIC                     simtrack.cpp:23286, 23380
@@ -142,6 +145,7 @@ public:
     double r = 0., c11 = 0., c12 = 0., c21 = 0., c22 = 0.;   /* coupling (Cal_Twiss)              */
     double Etax = 0., Etaxp = 0., Etay = 0., Etayp = 0.;     /* dispersion (Cal_Dispersion)       */
     double T[36] = {};                                       /* element map (Cal_ElementMap)      */
+    double n0[3] = {0., 0., 0.};                             /* periodic spin (Cal_Spin_Orbit)    */
     double A[36] = {};                                       /* normalising matrix (Trace_A)      */
 
 protected:
@@ -611,6 +615,7 @@ class Line {
 public:
     double Tune1 = 0., Tune2 = 0.;                                 /* Trace_A                  */
     double Chromx1 = 0., Chromy1 = 0., Chromx2 = 0., Chromy2 = 0.; /* Cal_Chrom                */
+    double SpinTune = 0.;                                          /* Cal_Spin_Tune            */
     Line() = default;
     Line(const Line &) = delete;
     Line &operator=(const Line &) = delete;
@@ -1040,6 +1045,11 @@ public:
     {
         check(st_track_radial(t_, x, np, nturn, stable, lost_turn, lost_post), "Track_Radial");
     }
+    void Track_wrt_OnMomentumPart(double *x, const double *dz0, int np, int nturn, int *stable, int *lost_turn,
+                                  int *lost_post)
+    {
+        check(st_track_dz(t_, dz0, x, np, nturn, stable, lost_turn, lost_post), "Track_wrt_OnMomentumPart");
+    }
     void Track_tbt(double *x, int np, int nturn, double *x_tbt, int bpm_index, bool spin, int *stable,
                    int *lost_turn, int *lost_post)
     {
@@ -1088,6 +1098,12 @@ inline void Track_Radial(Line &l, double x[], int nturn, int &stable, int &lost_
 {
     tracker_for(l).Track_Radial(x, 1, nturn, &stable, &lost_turn, &lost_post);
 }
+/* Track_wrt_OnMomentumPart (simtrack.cpp:20942-20966) */
+inline void Track_wrt_OnMomentumPart(Line &l, double x[], double dz0[], int nturn, int &stable, int &lost_turn,
+                                     int &lost_post)
+{
+    tracker_for(l).Track_wrt_OnMomentumPart(x, dz0, 1, nturn, &stable, &lost_turn, &lost_post);
+}
 inline void Track_tbt(Line &l, double x[], int nturn, double x_tbt[], int &stable, int &lost_turn, int &lost_post)
 {
     tracker_for(l).Track_tbt(x, 1, nturn, x_tbt, -1, false, &stable, &lost_turn, &lost_post);
@@ -1132,13 +1148,14 @@ inline void da_bisect_file(Line &l, int variant, int nturn, double deltap0, doub
     int n;
     check(st_da_angles(variant, angles, &n, &smax), "st_da_angles");
     std::vector<double> sig((size_t)n);
-    check(st_da_bisect(tracker_for(l).table(), nturn, deltap0, sigmax0, sigmay0, angles, n, smax,
-                       variant == ST_DA_TRACK_DA_RADIAL ? 1 : 0, sig.data()),
+    const int flags = (variant == ST_DA_TRACK_DA_RADIAL ? ST_DA_RADIAL : 0) |
+                      (variant == ST_DA_TRACK_DA2 || variant == ST_DA_TRACK_DA3 ? ST_DA_SINGLE : 0);
+    check(st_da_bisect(tracker_for(l).table(), nturn, deltap0, sigmax0, sigmay0, angles, n, smax, flags, sig.data()),
           "st_da_bisect");
     FILE *f = std::fopen(filename, "w");
     if (!f) throw Error(std::string("cannot open ") + filename);
     for (int i = 0; i < n; ++i) {
-        double deg = variant == ST_DA_TRACK_DA1 ? 15. * (i + 1) : (90 / 11.) * (i + 1);
+        double deg = (variant == ST_DA_TRACK_DA1 || variant == ST_DA_TRACK_DA3) ? 15. * (i + 1) : (90 / 11.) * (i + 1);
         std::fprintf(f, "%f %f \n", deg, sig[i]);
     }
     std::fclose(f);
@@ -1171,6 +1188,19 @@ inline void Track_DA1(Line &l, int nturn, double deltap0, double sigmax0, double
 inline void Track_DA_Radial(Line &l, int nturn, double deltap0, double sigmax0, double sigmay0, const char *filename)
 {
     detail::da_bisect_file(l, ST_DA_TRACK_DA_RADIAL, nturn, deltap0, sigmax0, sigmay0, filename);
+}
+/* Track_DA2 (simtrack.cpp:21497-21533): one particle per bisection midpoint */
+inline void Track_DA2(Line &l, int nturn, double deltap0, double sigmax0, double sigmay0, const char *filename)
+{
+    detail::da_bisect_file(l, ST_DA_TRACK_DA2, nturn, deltap0, sigmax0, sigmay0, filename);
+}
+inline void Cal_Orbit_Num(Line &l, double deltap);
+/* Track_DA3 (simtrack.cpp:21535-21572): Cal_Orbit_Num(l, 0) (filling every
+   element's X, as the reference does), then one particle per midpoint */
+inline void Track_DA3(Line &l, int nturn, double deltap0, double sigmax0, double sigmay0, const char *filename)
+{
+    Cal_Orbit_Num(l, 0);
+    detail::da_bisect_file(l, ST_DA_TRACK_DA3, nturn, deltap0, sigmax0, sigmay0, filename);
 }
 inline void Track_DA_Uniform(Line &l, int nturn, double deltap0, double angle, double sigmax0, double sigmay0,
                              const char *filename)
@@ -1213,6 +1243,103 @@ inline void Cal_Orbit_Num(Line &l, double deltap)
         for (int k = 0; k < 6; ++k) l.Cell[j]->X[k] = x[k];
     }
     st_table_destroy(t);
+}
+
+/* Cal_Spin_Orbit (simtrack.cpp:20510-20572): Cal_Orbit_Num (every element's
+   X), then the periodic spin by Newton iteration on one-turn sPass maps;
+   fills every element's n0 with the normalised spin after it. */
+inline void Cal_Spin_Orbit(Line &l, double deltap)
+{
+    Cal_Orbit_Num(l, deltap);
+    std::vector<st_element> rec = records_from_line(l);
+    std::vector<double> n0(3 * rec.size());
+    double x0[9];
+    int it = 0;
+    check(st_spin_orbit(default_context(), &GP, rec.data(), (int)rec.size(), deltap, x0, n0.data(), &it),
+          "Cal_Spin_Orbit");
+    for (size_t j = 0; j < rec.size(); ++j)
+        for (int i = 0; i < 3; ++i) l.Cell[j]->n0[i] = n0[3 * j + i];
+}
+
+/* Cal_Spin_Tune (simtrack.cpp:20574-20601): Cal_Spin_Orbit, then the spin
+   tune of 500 tracked turns (FineTuneFinder on [0.001, 0.499]) into
+   l.SpinTune. */
+inline void Cal_Spin_Tune(Line &l, double deltap)
+{
+    Cal_Spin_Orbit(l, deltap);
+    std::vector<st_element> rec = records_from_line(l);
+    double tune = 0., n0[3];
+    check(st_spin_tune(default_context(), &GP, rec.data(), (int)rec.size(), deltap, &tune, n0), "Cal_Spin_Tune");
+    l.SpinTune = tune;
+}
+
+/* Set_Initial_Spin (simtrack.cpp:20603-20635): x[6..8] becomes the
+   normalised projection on n0 of the three unit spins, summed over Nturn
+   tracked turns from x's orbit. */
+inline void Set_Initial_Spin(Line &l, int Nturn, double n0[3], double x[9])
+{
+    std::vector<st_element> rec = records_from_line(l);
+    check(st_set_initial_spin(default_context(), &GP, rec.data(), (int)rec.size(), Nturn, n0, x),
+          "Set_Initial_Spin");
+}
+
+/* Cal_Tunes_Track (simtrack.cpp:20971-20990): x[0] and x[2] below 1e-9 are
+   raised by 1e-9 (in the caller's array, as the reference does), Track_tbt
+   for Nturn turns, and FineTuneFinder of the x and y samples in the global
+   window [tune_low, tune_high]; the tunes are left untouched when the
+   particle is lost (the reference leaves them unset). */
+inline void Cal_Tunes_Track(Line &l, double x[], int Nturn, double &tune1, double &tune2)
+{
+    int stable = 1, lost_turn = 0, lost_post = 0;
+    std::vector<double> x_tbt((size_t)Nturn * 6), xtemp((size_t)Nturn);
+    if (x[0] < 1.0e-9) x[0] = x[0] + 1.0e-9;
+    if (x[2] < 1.0e-9) x[2] = x[2] + 1.0e-9;
+    Track_tbt(l, x, Nturn, x_tbt.data(), stable, lost_turn, lost_post);
+    if (stable == 1) {
+        st_ctx *c = default_context();
+        for (int k = 0; k < Nturn; k++) xtemp[k] = x_tbt[(size_t)k * 6 + 0];
+        check(st_fine_tune(c, xtemp.data(), nullptr, Nturn, 1, tune_low, tune_high, &tune1), "FineTuneFinder");
+        for (int k = 0; k < Nturn; k++) xtemp[k] = x_tbt[(size_t)k * 6 + 2];
+        check(st_fine_tune(c, xtemp.data(), nullptr, Nturn, 1, tune_low, tune_high, &tune2), "FineTuneFinder");
+    }
+}
+
+/* Cal_Tunes_Track (simtrack.cpp:20992-21026): Cal_Orbit_Num at deltap0,
+   then the tunes of 1,024 tracked turns from the closed orbit (raised by
+   1e-9 where below it) into l.Tune1, l.Tune2 */
+inline void Cal_Tunes_Track(Line &l, double deltap0)
+{
+    Cal_Orbit_Num(l, deltap0);
+    const Element *last = l.Cell[l.Ncell - 1];
+    double x[6] = {last->X[0], last->X[1], last->X[2], last->X[3], 0., deltap0};
+    double tunex1 = l.Tune1, tuney1 = l.Tune2;
+    Cal_Tunes_Track(l, x, 1024, tunex1, tuney1);
+    l.Tune1 = tunex1;
+    l.Tune2 = tuney1;
+}
+
+/* Cal_Chrom_Track (simtrack.cpp:21028-21051): tracked tunes at deltap
+   +-0.0003 and 0; first and second order chromaticities by differences */
+inline void Cal_Chrom_Track(Line &l)
+{
+    double deltap;
+    double qx0, qy0, qxp, qyp, qxm, qym;
+    deltap = 0.0003;
+    Cal_Tunes_Track(l, deltap);
+    qxp = l.Tune1;
+    qyp = l.Tune2;
+    deltap = -0.0003;
+    Cal_Tunes_Track(l, deltap);
+    qxm = l.Tune1;
+    qym = l.Tune2;
+    deltap = 0.00;
+    Cal_Tunes_Track(l, deltap);
+    qx0 = l.Tune1;
+    qy0 = l.Tune2;
+    l.Chromx1 = (qxp - qxm) / 2. / 0.0003;
+    l.Chromy1 = (qyp - qym) / 2. / 0.0003;
+    l.Chromx2 = (qxp + qxm - 2 * qx0) / 2. / 0.0003 / 0.0003;
+    l.Chromy2 = (qyp + qym - 2 * qy0) / 2. / 0.0003 / 0.0003;
 }
 
 namespace detail {

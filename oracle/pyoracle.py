@@ -54,6 +54,10 @@ def load(glibc=False):
     lib.orc_bb_slice_centers.argtypes = [I, D, P]
     lib.orc_fine_tune.argtypes = [I, P, D, D, P]
     lib.orc_closed_orbit.argtypes = [P, I, D, P, P, ctypes.POINTER(I)]
+    lib.orc_spin_orbit.argtypes = [P, I, D, P, P, ctypes.POINTER(I)]
+    lib.orc_track_dz.argtypes = [P, I, P, P, I, I, P, P, P]
+    lib.orc_spin_tune.argtypes = [P, I, D, P, P]
+    lib.orc_set_initial_spin.argtypes = [P, I, I, P, P]
     lib.orc_one_turn_matrix.argtypes = [P, I, P, D, P]
     lib.orc_element_maps.argtypes = [P, I, P, D, D, P, P]
     lib.orc_fma_scan.argtypes = [P, I, D, D, D, P, D, D, I, D, D, P, I]
@@ -135,6 +139,47 @@ class Oracle:
             raise RuntimeError(f"oracle closed orbit failed ({rc})")
         return xc, xe, it.value
 
+    def track_dz(self, x, nturn, dz0):
+        """Track_wrt_OnMomentumPart (simtrack.cpp:20942-20966)."""
+        self._set()
+        x = np.ascontiguousarray(x, dtype=np.float64).copy()
+        np_ = x.shape[0]
+        s, lt, lp = np.ones(np_, np.int32), np.zeros(np_, np.int32), np.zeros(np_, np.int32)
+        dz0 = np.ascontiguousarray(dz0, dtype=np.float64)
+        rc = self.lib.orc_track_dz(_p(self.records), len(self.records), _p(dz0), _p(x), np_, int(nturn), _p(s),
+                                   _p(lt), _p(lp))
+        if rc != 0:
+            raise RuntimeError(f"oracle track failed ({rc})")
+        return x, s, lt, lp
+
+    def spin_orbit(self, deltap):
+        """Cal_Spin_Orbit (simtrack.cpp:20510-20572): (x0[9], n0[n, 3], iterations)."""
+        self._set()
+        x0, n0, it = np.zeros(9), np.zeros((len(self.records), 3)), ctypes.c_int()
+        rc = self.lib.orc_spin_orbit(_p(self.records), len(self.records), deltap, _p(x0), _p(n0), ctypes.byref(it))
+        if rc != 0:
+            raise RuntimeError(f"oracle spin closed orbit failed ({rc})")
+        return x0, n0, it.value
+
+    def spin_tune(self, deltap):
+        """Cal_Spin_Tune (simtrack.cpp:20574-20601): (tune, n0 of the last element)."""
+        self._set()
+        tune, n0 = np.zeros(1), np.zeros(3)
+        rc = self.lib.orc_spin_tune(_p(self.records), len(self.records), deltap, _p(tune), _p(n0))
+        if rc != 0:
+            raise RuntimeError(f"oracle spin tune failed ({rc})")
+        return float(tune[0]), n0
+
+    def set_initial_spin(self, nturn, n0, x):
+        """Set_Initial_Spin (simtrack.cpp:20603-20635): x (9) with its spin replaced."""
+        self._set()
+        x = np.array(x, dtype=np.float64).reshape(9).copy()
+        n0 = np.ascontiguousarray(n0, dtype=np.float64).reshape(3)
+        rc = self.lib.orc_set_initial_spin(_p(self.records), len(self.records), int(nturn), _p(n0), _p(x))
+        if rc != 0:
+            raise RuntimeError(f"oracle Set_Initial_Spin failed ({rc})")
+        return x
+
     def one_turn_matrix(self, x_co, h=1e-6):
         """The 6x6 one-turn matrix of Cal_OneTurnMap (simtrack.cpp:15122-15137)
         by symmetric differences of step h about x_co, RFCAV voltages off."""
@@ -193,6 +238,15 @@ class Oracle:
 
 
 # ------------------------------------------- strong-strong beam-beam
+
+def fine_tune(x, tune_lower, tune_upper):
+    """FineTuneFinder (simtrack.cpp:2288-2319) of the samples x."""
+    lib = load()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = ctypes.c_double()
+    lib.orc_fine_tune(len(x), _p(x), float(tune_lower), float(tune_upper), ctypes.byref(out))
+    return out.value
+
 
 def _beam(x):
     return np.ascontiguousarray(np.array(x, dtype=np.float64, copy=True).reshape(-1, 6))

@@ -2170,6 +2170,20 @@ EXPORT int orc_track(const st_element *line, int n, double *x, int np, int nturn
     return err ? -2 : 0;
 }
 
+/* Track_wrt_OnMomentumPart, simtrack.cpp:20942-20966: Track with dz0[j]
+   subtracted from z after element j (the radial path with the caller's
+   offsets) */
+EXPORT int orc_track_dz(const st_element *line, int n, const double *dz0, double *x, int np, int nturn, int *stable,
+                        int *lost_turn, int *lost_post)
+{
+    int err = 0;
+    uint64_t draws = orc_draws_per_turn(line, n, 0);
+    for (int p = 0; p < np; p++)
+        err |= track_one(line, n, x + (size_t)p * 6, nturn, stable + p, lost_turn + p, lost_post + p, 2, dz0, NULL,
+                         -1, (uint32_t)p, draws);
+    return err ? -2 : 0;
+}
+
 /* FineTuneFinder, simtrack.cpp:2288-2319 */
 EXPORT void orc_fine_tune(int n, const double *x, double tune_lower, double tune_upper, double *peaktune)
 {
@@ -2421,6 +2435,160 @@ EXPORT int orc_closed_orbit(const st_element *line0, int n, double deltap, doubl
 out:
     free(line);
     return rc;
+}
+
+/* sPass of every element with no loss test: the spin drivers walk
+   linename.Cell[j]->sPass(x) directly (simtrack.cpp:20527, 20618, 20666) */
+static int spass_turn(const st_element *line, int n, double x[])
+{
+    for (int j = 0; j < n; j++) {
+        int rc = elem_spass(&line[j], x);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+/*
+ * Cal_Spin_Orbit, simtrack.cpp:20510-20572: Cal_Orbit_Num, then Newton on
+ * the spin's (Sx, Ss) at the start with one-turn sPass maps (RF on),
+ * difference 1e-6, tolerance 1e-12, at most 20 iterations, Sy re-normalised
+ * after each step; mat_inv's result is not checked (cpp:20546).  x0: the
+ * orbit Cell[Ncell-1]->X and the converged spin at the start; n0 (3 per
+ * element, may be null): the normalised spin after every element of one
+ * more pass.  Returns -1 where the reference reports failure.
+ */
+EXPORT int orc_spin_orbit(const st_element *line, int n, double deltap, double x0[9], double *n0, int *iterations)
+{
+    double x_co[6], x_end[6];
+    int rc = orc_closed_orbit(line, n, deltap, x_co, x_end, NULL);
+    if (rc) return rc;
+    int i, j, iter = 0;
+    double x01[9], x1[9], dx[3], d = 1.0e-06, mat[2][2], chi, codeps = 1e-12;
+    int Max_iter = 20;
+    for (i = 0; i < 6; i++) x0[i] = x_end[i];
+    x0[6] = 0.;
+    x0[7] = 1.0;
+    x0[8] = 0.;
+    do {
+        iter++;
+        for (j = 0; j < 9; j++) x01[j] = x0[j];
+        if ((rc = spass_turn(line, n, x01))) return rc;
+        for (j = 0; j < 9; j++) x1[j] = x0[j];
+        x1[6] = x0[6] + d;
+        if ((rc = spass_turn(line, n, x1))) return rc;
+        mat[0][0] = (x1[6] - x01[6]) / d;
+        mat[1][0] = (x1[8] - x01[8]) / d;
+        for (j = 0; j < 9; j++) x1[j] = x0[j];
+        x1[8] = x0[8] + d;
+        if ((rc = spass_turn(line, n, x1))) return rc;
+        mat[0][1] = (x1[6] - x01[6]) / d;
+        mat[1][1] = (x1[8] - x01[8]) / d;
+        dx[0] = x0[6] - x01[6];
+        dx[1] = x0[8] - x01[8];
+        chi = dx[0] * dx[0] + dx[1] * dx[1];
+        chi = sqrt(chi / 2.);
+        if (chi > codeps) {
+            for (i = 0; i < 2; i++) mat[i][i] = mat[i][i] - 1.0000001;
+            mat_inv(&mat[0][0], 2);
+            x0[6] = x0[6] + mat[0][0] * dx[0] + mat[0][1] * dx[1];
+            x0[8] = x0[8] + mat[1][0] * dx[0] + mat[1][1] * dx[1];
+            x0[7] = sqrt(1. - x0[6] * x0[6] - x0[8] * x0[8]);
+        }
+    } while (chi > codeps && iter < Max_iter);
+    if (iterations) *iterations = iter;
+    if (iter == Max_iter - 1) return -1;
+    if (n0) {
+        double xs[9];
+        for (j = 0; j < 9; j++) xs[j] = x0[j];
+        for (j = 0; j < n; j++) {
+            if ((rc = elem_spass(&line[j], xs))) return rc;
+            chi = sqrt(xs[6] * xs[6] + xs[7] * xs[7] + xs[8] * xs[8]);
+            for (i = 0; i < 3; i++) n0[3 * j + i] = xs[6 + i] / chi;
+        }
+    }
+    return 0;
+}
+
+/*
+ * Cal_Spin_Tune, simtrack.cpp:20574-20601: Cal_Spin_Orbit, a spin m
+ * perpendicular to the last element's n0 (with the reference's grouping
+ * n0^2 / n1 * n1, cpp:20587), 500 turns of sPass from the orbit
+ * Cell[Ncell-1]->X recording Sx after each turn, and FineTuneFinder on
+ * [0.001, 0.499].  n0_end (may be null): the last element's n0.
+ */
+EXPORT int orc_spin_tune(const st_element *line, int n, double deltap, double *tune, double n0_end[3])
+{
+    enum { NTURN = 500 };
+    double x0[9], x[9], nv[3], m[3], sample[NTURN];
+    double *n0 = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+    if (!n0) return -5;
+    int rc = orc_spin_orbit(line, n, deltap, x0, n0, NULL);
+    if (rc) {
+        free(n0);
+        return rc;
+    }
+    for (int i = 0; i < 3; i++) nv[i] = n0[3 * (n - 1) + i];
+    free(n0);
+    if (n0_end)
+        for (int i = 0; i < 3; i++) n0_end[i] = nv[i];
+    m[2] = 0.;
+    m[0] = 1.0 / sqrt(1. + nv[0] * nv[0] / nv[1] * nv[1]);
+    m[1] = -nv[0] * m[0] / nv[1];
+    for (int i = 0; i < 6; i++) x[i] = x0[i];
+    x[6] = m[0];
+    x[7] = m[1];
+    x[8] = m[2];
+    for (int j = 0; j < NTURN; j++) {
+        if ((rc = spass_turn(line, n, x))) return rc;
+        sample[j] = x[6];
+    }
+    orc_fine_tune(NTURN, sample, 0.001, 0.499, tune);
+    return 0;
+}
+
+/*
+ * Set_Initial_Spin, simtrack.cpp:20603-20635: the orbit of x with unit
+ * spins along x, y and s, each tracked nturn turns of sPass; after every
+ * turn its projection on n0 is summed; x's spin becomes the normalised
+ * sums.
+ */
+EXPORT int orc_set_initial_spin(const st_element *line, int n, int nturn, const double n0[3], double x[9])
+{
+    int j, rc;
+    double b = 0, b1 = 0., b2 = 0., b3 = 0.;
+    double x1[9], x2[9], x3[9];
+    for (j = 0; j < 6; j++) x1[j] = x[j];
+    for (j = 0; j < 6; j++) x2[j] = x[j];
+    for (j = 0; j < 6; j++) x3[j] = x[j];
+    x1[6] = 1.;
+    x1[7] = 0.;
+    x1[8] = 0.;
+    x2[6] = 0.;
+    x2[7] = 1.;
+    x2[8] = 0.;
+    x3[6] = 0.;
+    x3[7] = 0.;
+    x3[8] = 1.;
+    for (j = 0; j < nturn; j++) {
+        if ((rc = spass_turn(line, n, x1))) return rc;
+        b1 = b1 + x1[6] * n0[0] + x1[7] * n0[1] + x1[8] * n0[2];
+    }
+    for (j = 0; j < nturn; j++) {
+        if ((rc = spass_turn(line, n, x2))) return rc;
+        b2 = b2 + x2[6] * n0[0] + x2[7] * n0[1] + x2[8] * n0[2];
+    }
+    for (j = 0; j < nturn; j++) {
+        if ((rc = spass_turn(line, n, x3))) return rc;
+        b3 = b3 + x3[6] * n0[0] + x3[7] * n0[1] + x3[8] * n0[2];
+    }
+    b = sqrt(b1 * b1 + b2 * b2 + b3 * b3);
+    b1 = b1 / b;
+    b2 = b2 / b;
+    b3 = b3 / b;
+    x[6] = b1;
+    x[7] = b2;
+    x[8] = b3;
+    return 0;
 }
 
 /* The one-turn matrix of Cal_OneTurnMap (simtrack.cpp:15122-15137: RFCAV

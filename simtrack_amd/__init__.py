@@ -7,7 +7,8 @@ construction API mirroring the reference's C++ classes, and synthetic
 workloads.
 """
 from . import build  # noqa: F401
-from ._capi import (ELEMENT_DTYPE, ST_DA_TRACK_DA, ST_DA_TRACK_DA1, ST_DA_TRACK_DA_RADIAL,  # noqa: F401
-                    TRACK_RADIAL, TRACK_SPIN, TRACK_PERF, NOALLOC_MAX_PARTICLES, NOALLOC_MAX_ELEMENTS, TYPES, Batch, BatchConfig, Context, Globals, Mlp,
+from ._capi import (ELEMENT_DTYPE, ST_DA_RADIAL, ST_DA_SINGLE, ST_DA_TRACK_DA, ST_DA_TRACK_DA1,  # noqa: F401
+                    ST_DA_TRACK_DA2, ST_DA_TRACK_DA3, ST_DA_TRACK_DA_RADIAL, TRACK_RADIAL, TRACK_SPIN, TRACK_PERF,
+                    NOALLOC_MAX_PARTICLES, NOALLOC_MAX_ELEMENTS, TYPES, Batch, BatchConfig, Context, Globals, Mlp,
                     STError, Table, load)
 from .line import *  # noqa: F401,F403

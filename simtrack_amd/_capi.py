@@ -76,7 +76,8 @@ EXPORTS = [
     "st_one_turn_matrix", "st_element_maps", "st_lyapunov_scan", "st_fine_tune_window_device", "st_action_diff",
     "st_bb_soft_gaussian_4d", "st_bb_soft_gaussian_6d", "st_bb_soft_gaussian_6d_zb", "st_bb_soft_gaussian_4d_host",
     "st_bb_soft_gaussian_6d_host", "st_bb_soft_gaussian_6d_zb_host", "st_zboundary_slices_gaussian",
-    "st_bb_pic_4d", "st_bb_pic_6d", "st_bb_pic_4d_host", "st_bb_pic_6d_host",
+    "st_bb_pic_4d", "st_bb_pic_6d", "st_bb_pic_4d_host", "st_bb_pic_6d_host", "st_spin_orbit", "st_spin_tune",
+    "st_set_initial_spin", "st_track_dz",
 ]
 
 
@@ -87,7 +88,8 @@ class BatchConfig(ctypes.Structure):
     def __repr__(self):
         return f"BatchConfig(wg_waves={self.wg_waves}, waves_per_simd={self.waves_per_simd}, " \
                f"turn_block={self.turn_block})"
-ST_DA_TRACK_DA, ST_DA_TRACK_DA1, ST_DA_TRACK_DA_RADIAL = 0, 1, 2
+ST_DA_TRACK_DA, ST_DA_TRACK_DA1, ST_DA_TRACK_DA_RADIAL, ST_DA_TRACK_DA2, ST_DA_TRACK_DA3 = 0, 1, 2, 3, 4
+ST_DA_RADIAL, ST_DA_SINGLE = 1, 2
 
 _lib = None
 
@@ -133,6 +135,7 @@ def load(path=None):
     for f in ("st_track", "st_track_spin", "st_track_radial"):
         getattr(lib, f).argtypes = [P, P, I, I, P, P, P]
     lib.st_track_tbt.argtypes = [P, P, I, I, P, I, I, P, P, P]
+    lib.st_track_dz.argtypes = [P, P, P, I, I, P, P, P]
     lib.st_element_pass.argtypes = [P, I, P, I, I]
     lib.st_track_device.argtypes = [P, P, I, I, I, P, P, I]
     lib.st_track_device_noalloc.argtypes = [P, P, I, I, I, P, P, I]
@@ -159,6 +162,9 @@ def load(path=None):
     lib.st_closed_orbit.argtypes = [P, ctypes.POINTER(Globals), P, I, D, P, P, ctypes.POINTER(I)]
     lib.st_one_turn_matrix.argtypes = [P, ctypes.POINTER(Globals), P, I, P, D, P]
     lib.st_element_maps.argtypes = [P, ctypes.POINTER(Globals), P, I, P, D, D, P, P]
+    lib.st_spin_orbit.argtypes = [P, ctypes.POINTER(Globals), P, I, D, P, P, ctypes.POINTER(I)]
+    lib.st_spin_tune.argtypes = [P, ctypes.POINTER(Globals), P, I, D, P, P]
+    lib.st_set_initial_spin.argtypes = [P, ctypes.POINTER(Globals), P, I, I, P, P]
     for f in ("st_bb_soft_gaussian_4d", "st_bb_soft_gaussian_4d_host", "st_bb_pic_4d", "st_bb_pic_4d_host"):
         getattr(lib, f).argtypes = [P, D, I, P, D, I, P, D, D, D, D]
     for f in ("st_bb_soft_gaussian_6d", "st_bb_soft_gaussian_6d_host", "st_bb_soft_gaussian_6d_zb",
@@ -300,6 +306,20 @@ class Table:
         f = self.lib.st_track_spin if spin else (self.lib.st_track_radial if radial else self.lib.st_track)
         assert not (spin and radial)
         check(f(self.h, _ptr(x), npart, int(nturn), _ptr(stable), _ptr(lost_turn), _ptr(lost_post)), "st_track")
+        return x, stable, lost_turn, lost_post
+
+    def track_dz(self, x, nturn, dz0, stable=None, lost_turn=None, lost_post=None):
+        """Track_wrt_OnMomentumPart: Track with dz0[j] subtracted from z after
+        element j.  Arrays are updated in place."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert x.ndim == 2 and x.shape[1] == 6
+        dz0 = np.ascontiguousarray(dz0, dtype=np.float64)
+        npart = x.shape[0]
+        stable = np.ones(npart, np.int32) if stable is None else np.ascontiguousarray(stable, np.int32)
+        lost_turn = np.zeros(npart, np.int32) if lost_turn is None else np.ascontiguousarray(lost_turn, np.int32)
+        lost_post = np.zeros(npart, np.int32) if lost_post is None else np.ascontiguousarray(lost_post, np.int32)
+        check(self.lib.st_track_dz(self.h, _ptr(dz0), _ptr(x), npart, int(nturn), _ptr(stable), _ptr(lost_turn),
+                                   _ptr(lost_post)), "st_track_dz")
         return x, stable, lost_turn, lost_post
 
     def track_tbt(self, x, nturn, bpm_index=-1, spin=False, x_tbt=None):

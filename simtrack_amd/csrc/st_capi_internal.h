@@ -110,12 +110,15 @@ ST_HIDDEN int ensure_particles(st_ctx *c, size_t np, int ncol);
 ST_HIDDEN int drain_timing(st_ctx *c);
 ST_HIDDEN int timed_event(st_ctx *c, int *idx);
 
-/* tracking helpers (st_track.cpp) */
+/* tracking helpers (st_track.cpp); radial: 0 none, 1 Track_Radial's own
+   x04, RADIAL_GIVEN the per-element z offsets already in the context's x04
+   buffer */
+#define RADIAL_GIVEN 2
 ST_HIDDEN int turn_block(int n_elems, int np);
 ST_HIDDEN int track_device_core(st_table *t, double *d_x, int np, int turn0, int nturn, int *d_state, int *d_lost,
-                                bool spin, bool radial, double *d_tbt, int tbt_bpm, int tbt_nturn, bool bare = false);
+                                bool spin, int radial, double *d_tbt, int tbt_bpm, int tbt_nturn, bool bare = false);
 ST_HIDDEN int noalloc_scratch_init(int device);
 ST_HIDDEN int track_host(st_table *t, double *x, int np, int nturn, int *stable, int *lost_turn, int *lost_post,
-                         bool spin, bool radial, double *x_tbt, int tbt_bpm, bool bare = false);
+                         bool spin, int radial, double *x_tbt, int tbt_bpm, bool bare = false);
 
 #endif

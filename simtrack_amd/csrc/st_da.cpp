@@ -3,7 +3,8 @@
  *
  * The reference's drivers call Track() once per particle and amplitude:
  *   Track_DA (simtrack.cpp:21389), Track_DA1 (cpp:21443) and
- *   Track_DA_Radial (cpp:21682) bisect the amplitude with twin particles;
+ *   Track_DA_Radial (cpp:21682) bisect the amplitude with twin particles,
+ *   Track_DA2 (cpp:21497) and Track_DA3 (cpp:21535) with one particle;
  *   Track_DA_Uniform (cpp:21574), Track_DA_Radial_Uniform (cpp:21736) and
  *   Track_DA_Radial_Uniform_Test (Example2/DA_cal.cpp:61) step it uniformly
  *   and stop at the first loss.
@@ -16,7 +17,8 @@
  * [0, sigma_max] (all exactly representable), at most `depth` of them per
  * angle, where depth is the number of halvings until the interval is
  * <= 0.1.  The whole tree (2^depth - 1 midpoints, each with its twin at
- * +0.05) is tracked speculatively -- enough parallel work to fill the GPU,
+ * +0.05 for the twin drivers) is tracked speculatively -- enough parallel
+ * work to fill the GPU,
  * in one launch, instead of depth dependent launches of two particles.
  *
  * Initial conditions use the C library's cos/sin, as the reference drivers
@@ -92,18 +94,22 @@ extern "C" int st_da_bisect(st_table *t, int nturn, double deltap0, double sigma
             }
         }
     }
-    const size_t np = (size_t)nangle * (size_t)nodes * 2;
+    /* twins at sigma and sigma + 0.05 (Track_DA, cpp:21410-21426), or one
+       particle at sigma (Track_DA2, cpp:21513-21522) */
+    const int per = (radial & ST_DA_SINGLE) ? 1 : 2;
+    const int rad = radial & ST_DA_RADIAL;
+    const size_t np = (size_t)nangle * (size_t)nodes * per;
     if (np > (size_t)1 << 30) return st_internal_fail(ST_ERR_ARG, "st_da_bisect: too many particles");
     std::vector<double> x(np * 6);
     for (int a = 0; a < nangle; ++a)
         for (int k = 0; k < nodes; ++k) {
-            size_t p = ((size_t)a * nodes + k) * 2;
+            size_t p = ((size_t)a * nodes + k) * per;
             double sigma1 = mid[k], sigma2 = mid[k] + 0.05;
             set_particle(&x[p * 6], sigma1, angles[a], sigmax0, sigmay0, deltap0);
-            set_particle(&x[(p + 1) * 6], sigma2, angles[a], sigmax0, sigmay0, deltap0);
+            if (per == 2) set_particle(&x[(p + 1) * 6], sigma2, angles[a], sigmax0, sigmay0, deltap0);
         }
     std::vector<int> stable;
-    int rc = track_batch(t, x, (int)np, nturn, radial, stable);
+    int rc = track_batch(t, x, (int)np, nturn, rad, stable);
     if (rc) return rc;
     for (int a = 0; a < nangle; ++a) {
         /* replay of the do-while loop of Track_DA (cpp:21403-21435) */
@@ -111,8 +117,8 @@ extern "C" int st_da_bisect(st_table *t, int nturn, double deltap0, double sigma
         int k = 0;
         do {
             sigma = (sigma_s + sigma_e) / 2.0;
-            size_t p = ((size_t)a * nodes + k) * 2;
-            int st = (stable[p] * stable[p + 1] == 1) ? 1 : 0;
+            size_t p = ((size_t)a * nodes + k) * per;
+            int st = per == 1 ? stable[p] : ((stable[p] * stable[p + 1] == 1) ? 1 : 0);
             if (st == 0) sigma_e = sigma;
             if (st == 1) sigma_s = sigma;
             k = 2 * k + (st ? 2 : 1);
@@ -163,6 +169,16 @@ extern "C" int st_da_angles(int variant, double *angles, int *nangle, double *si
         *nangle = 5;
         for (int i = 0; i < 5; ++i) angles[i] = 15 * (i + 1) * REF_PI / 180;
         *sigma_max = 50.0;
+        return ST_OK;
+    case ST_DA_TRACK_DA2: /* cpp:21507-21510 */
+        *nangle = 10;
+        for (int i = 0; i < 10; ++i) angles[i] = (90 / 11.) * (i + 1) * REF_PI / 180;
+        *sigma_max = 50.0;
+        return ST_OK;
+    case ST_DA_TRACK_DA3: /* cpp:21545-21550 */
+        *nangle = 5;
+        for (int i = 0; i < 5; ++i) angles[i] = 15 * (i + 1) * REF_PI / 180;
+        *sigma_max = 30.0;
         return ST_OK;
     default:
         return st_internal_fail(ST_ERR_ARG, "st_da_angles: unknown variant");

@@ -233,6 +233,170 @@ extern "C" int st_closed_orbit(st_ctx *c, const st_globals *gp, const st_element
     return ST_OK;
 }
 
+/* ----------------------------------------------------- spin drivers */
+
+/* Cal_Spin_Orbit (simtrack.cpp:20510-20572): Cal_Orbit_Num, then Newton on
+   the spin's (Sx, Ss) at the start with one-turn sPass maps of the line as
+   given (RF on), no loss test; the three particles of an iteration (x0 and
+   the two displaced spins) are one batch.  n0: one more pass, the
+   normalised spin after every element. */
+extern "C" int st_spin_orbit(st_ctx *c, const st_globals *gp, const st_element *rec, int n, double deltap,
+                             double x0[9], double *n0, int *iterations)
+{
+    if (!c || !gp || n <= 0 || !rec || !x0) return fail(ST_ERR_ARG, "st_spin_orbit: bad argument");
+    double x_co[6], x_end[6];
+    int rc = st_closed_orbit(c, gp, rec, n, deltap, x_co, x_end, nullptr);
+    if (rc) return rc;
+    st_table *t = nullptr;
+    rc = st_table_build(c, gp, rec, n, &t);
+    if (rc) return rc;
+    std::unique_ptr<st_table, int (*)(st_table *)> guard(t, st_table_destroy);
+    const double d = 1.0e-06, codeps = 1e-12;
+    const int Max_iter = 20;
+    double mat[2][2], dx[2], chi, xb[3 * 9];
+    int st[3], lt[3], lp[3];
+    for (int i = 0; i < 6; i++) x0[i] = x_end[i];
+    x0[6] = 0.;
+    x0[7] = 1.0;
+    x0[8] = 0.;
+    int iter = 0;
+    do {
+        iter++;
+        for (int q = 0; q < 3; ++q) {
+            for (int j = 0; j < 9; j++) xb[q * 9 + j] = x0[j];
+            st[q] = 1;
+            lt[q] = lp[q] = 0;
+        }
+        xb[9 + 6] = x0[6] + d;
+        xb[18 + 8] = x0[8] + d;
+        rc = track_host(t, xb, 3, 1, st, lt, lp, true, false, nullptr, -1, true);
+        if (rc) return rc;
+        const double *x01 = xb, *x1 = xb + 9, *x2 = xb + 18;
+        mat[0][0] = (x1[6] - x01[6]) / d;
+        mat[1][0] = (x1[8] - x01[8]) / d;
+        mat[0][1] = (x2[6] - x01[6]) / d;
+        mat[1][1] = (x2[8] - x01[8]) / d;
+        dx[0] = x0[6] - x01[6];
+        dx[1] = x0[8] - x01[8];
+        chi = dx[0] * dx[0] + dx[1] * dx[1];
+        chi = std::sqrt(chi / 2.);
+        if (chi > codeps) {
+            for (int i = 0; i < 2; i++) mat[i][i] = mat[i][i] - 1.0000001;
+            mat_inv_ref(&mat[0][0], 2); /* unchecked, as the reference's (cpp:20546) */
+            x0[6] = x0[6] + mat[0][0] * dx[0] + mat[0][1] * dx[1];
+            x0[8] = x0[8] + mat[1][0] * dx[0] + mat[1][1] * dx[1];
+            x0[7] = std::sqrt(1. - x0[6] * x0[6] - x0[8] * x0[8]);
+        }
+    } while (chi > codeps && iter < Max_iter);
+    if (iterations) *iterations = iter;
+    /* the reference's test, cpp:20554 */
+    if (iter == Max_iter - 1) return fail(ST_ERR_PARAM, "spin closed orbit: failed to find spin COD");
+    if (n0) {
+        double xs[9];
+        for (int j = 0; j < 9; j++) xs[j] = x0[j];
+        for (int k = 0; k < n; ++k) {
+            rc = st_element_pass(t, k, xs, 1, 1);
+            if (rc) return rc;
+            const double a = std::sqrt(xs[6] * xs[6] + xs[7] * xs[7] + xs[8] * xs[8]);
+            for (int i = 0; i < 3; i++) n0[3 * k + i] = xs[6 + i] / a;
+        }
+    }
+    return ST_OK;
+}
+
+/* `nturn` turns of bare spin tracking (no loss test) of np particles,
+   out[(p*nturn + j)*9 + k] = coordinate k of particle p after turn j + 1 */
+static int spin_turns(st_table *t, double *x, int np, int nturn, std::vector<double> &out)
+{
+    std::vector<int> st(np, 1), lt(np, 0), lp(np, 0);
+    std::vector<double> tbt((size_t)np * (size_t)nturn * 9, 0.);
+    int rc = track_host(t, x, np, nturn, st.data(), lt.data(), lp.data(), true, false, tbt.data(), -1, true);
+    if (rc) return rc;
+    /* the capture holds the state at the start of each turn: shift by one */
+    out.assign(tbt.size(), 0.);
+    for (int p = 0; p < np; ++p) {
+        const size_t b = (size_t)p * (size_t)nturn * 9;
+        for (int j = 0; j + 1 < nturn; ++j)
+            for (int k = 0; k < 9; ++k) out[b + (size_t)j * 9 + k] = tbt[b + (size_t)(j + 1) * 9 + k];
+        for (int k = 0; k < 9; ++k) out[b + (size_t)(nturn - 1) * 9 + k] = x[(size_t)p * 9 + k];
+    }
+    return ST_OK;
+}
+
+/* Cal_Spin_Tune (simtrack.cpp:20574-20601): Cal_Spin_Orbit; a spin m
+   perpendicular to the last element's n0, with the reference's grouping
+   n0^2 / n1 * n1 (cpp:20587); 500 turns of sPass from Cell[Ncell-1]->X;
+   FineTuneFinder of Sx after each turn on [0.001, 0.499]. */
+extern "C" int st_spin_tune(st_ctx *c, const st_globals *gp, const st_element *rec, int n, double deltap,
+                            double *tune, double n0_end[3])
+{
+    if (!c || !gp || n <= 0 || !rec || !tune) return fail(ST_ERR_ARG, "st_spin_tune: bad argument");
+    const int Nturn = 500;
+    double x0[9];
+    std::vector<double> n0((size_t)n * 3);
+    int rc = st_spin_orbit(c, gp, rec, n, deltap, x0, n0.data(), nullptr);
+    if (rc) return rc;
+    double nv[3], m[3];
+    for (int i = 0; i < 3; i++) nv[i] = n0[3 * (size_t)(n - 1) + i];
+    if (n0_end)
+        for (int i = 0; i < 3; i++) n0_end[i] = nv[i];
+    m[2] = 0.;
+    m[0] = 1.0 / std::sqrt(1. + nv[0] * nv[0] / nv[1] * nv[1]);
+    m[1] = -nv[0] * m[0] / nv[1];
+    double x[9];
+    for (int i = 0; i < 6; i++) x[i] = x0[i];
+    x[6] = m[0];
+    x[7] = m[1];
+    x[8] = m[2];
+    st_table *t = nullptr;
+    rc = st_table_build(c, gp, rec, n, &t);
+    if (rc) return rc;
+    std::unique_ptr<st_table, int (*)(st_table *)> guard(t, st_table_destroy);
+    std::vector<double> turns;
+    rc = spin_turns(t, x, 1, Nturn, turns);
+    if (rc) return rc;
+    std::vector<double> sample(Nturn);
+    for (int j = 0; j < Nturn; j++) sample[j] = turns[(size_t)j * 9 + 6];
+    return st_fine_tune(c, sample.data(), nullptr, Nturn, 1, 0.001, 0.499, tune);
+}
+
+/* Set_Initial_Spin (simtrack.cpp:20603-20635): the orbit of x with unit
+   spins along x, y and s (one batch), nturn turns of sPass each, their
+   projections on n0 summed after every turn in the reference's order; x's
+   spin becomes the normalised sums. */
+extern "C" int st_set_initial_spin(st_ctx *c, const st_globals *gp, const st_element *rec, int n, int nturn,
+                                   const double n0[3], double x[9])
+{
+    if (!c || !gp || n <= 0 || !rec || nturn < 0 || !n0 || !x)
+        return fail(ST_ERR_ARG, "st_set_initial_spin: bad argument");
+    st_table *t = nullptr;
+    int rc = st_table_build(c, gp, rec, n, &t);
+    if (rc) return rc;
+    std::unique_ptr<st_table, int (*)(st_table *)> guard(t, st_table_destroy);
+    double xb[3 * 9];
+    for (int q = 0; q < 3; ++q) {
+        for (int j = 0; j < 6; j++) xb[q * 9 + j] = x[j];
+        for (int k = 0; k < 3; k++) xb[q * 9 + 6 + k] = (k == q) ? 1. : 0.;
+    }
+    double bq[3] = {0., 0., 0.};
+    if (nturn > 0) {
+        std::vector<double> turns;
+        rc = spin_turns(t, xb, 3, nturn, turns);
+        if (rc) return rc;
+        for (int q = 0; q < 3; ++q)
+            for (int j = 0; j < nturn; j++) {
+                const double *s = turns.data() + ((size_t)q * (size_t)nturn + (size_t)j) * 9 + 6;
+                bq[q] = bq[q] + s[0] * n0[0] + s[1] * n0[1] + s[2] * n0[2];
+            }
+    }
+    double b = std::sqrt(bq[0] * bq[0] + bq[1] * bq[1] + bq[2] * bq[2]);
+    double b1 = bq[0] / b, b2 = bq[1] / b, b3 = bq[2] / b;
+    x[6] = b1;
+    x[7] = b2;
+    x[8] = b3;
+    return ST_OK;
+}
+
 /* The 6x6 one-turn matrix about x_co: symmetric differences of step h of
    bare passes (no loss test) with the RFCAV voltages off (crab cavities and
    CCMULT stay on), the map Cal_OneTurnMap (simtrack.cpp:15122-15137)

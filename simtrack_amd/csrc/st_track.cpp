@@ -48,18 +48,18 @@ struct track_scratch {
 };
 
 static int track_device_run(st_table *t, double *d_x, int np, int turn0, int nturn, int *d_state, int *d_lost,
-                            bool spin, bool radial, double *d_tbt, int tbt_bpm, int tbt_nturn, bool bare,
+                            bool spin, int radial, double *d_tbt, int tbt_bpm, int tbt_nturn, bool bare,
                             const track_scratch *fixed, bool perf = false);
 
 int track_device_core(st_table *t, double *d_x, int np, int turn0, int nturn, int *d_state, int *d_lost,
-                             bool spin, bool radial, double *d_tbt, int tbt_bpm, int tbt_nturn, bool bare)
+                             bool spin, int radial, double *d_tbt, int tbt_bpm, int tbt_nturn, bool bare)
 {
     return track_device_run(t, d_x, np, turn0, nturn, d_state, d_lost, spin, radial, d_tbt, tbt_bpm, tbt_nturn, bare,
                             nullptr);
 }
 
 static int track_device_run(st_table *t, double *d_x, int np, int turn0, int nturn, int *d_state, int *d_lost,
-                            bool spin, bool radial, double *d_tbt, int tbt_bpm, int tbt_nturn, bool bare,
+                            bool spin, int radial, double *d_tbt, int tbt_bpm, int tbt_nturn, bool bare,
                             const track_scratch *fixed, bool perf)
 {
     auto launch = perf ? st_perf::st_launch_track : st_launch_track;
@@ -75,7 +75,9 @@ static int track_device_run(st_table *t, double *d_x, int np, int turn0, int ntu
     a.tbt_bpm = tbt_bpm;
     a.tbt_nturn = tbt_nturn;
     a.tbt_turn0 = 0;
-    if (radial && fixed) {
+    if (radial == RADIAL_GIVEN && !fixed) {
+        a.x04 = c->d_x04; /* filled by the caller (st_track_dz) */
+    } else if (radial && fixed) {
         st_track_args ar = base_args(t, false);
         HIPCHK(st_launch_radial_x04(ar, fixed->x04, t->gp.H_expand != 0, c->stream));
         a.x04 = fixed->x04;
@@ -148,7 +150,7 @@ static int track_device_run(st_table *t, double *d_x, int np, int turn0, int ntu
 }
 
 int track_host(st_table *t, double *x, int np, int nturn, int *stable, int *lost_turn, int *lost_post,
-                      bool spin, bool radial, double *x_tbt, int tbt_bpm, bool bare)
+                      bool spin, int radial, double *x_tbt, int tbt_bpm, bool bare)
 {
     if (!t || (np > 0 && (!x || !stable || !lost_turn || !lost_post)) || np < 0 || nturn < 0)
         return fail(ST_ERR_ARG, "track: bad argument");
@@ -224,6 +226,25 @@ extern "C" int st_track_radial(st_table *t, double *x, int np, int nturn, int *s
                                int *lost_post)
 {
     return track_host(t, x, np, nturn, stable, lost_turn, lost_post, false, true, nullptr, -1);
+}
+
+/* Track_wrt_OnMomentumPart (simtrack.cpp:20942-20966): Track with dz0[j]
+   subtracted from z after element j, as Track_Radial does with its own x04 */
+extern "C" int st_track_dz(st_table *t, const double *dz0, double *x, int np, int nturn, int *stable, int *lost_turn,
+                           int *lost_post)
+{
+    if (!t || !dz0) return fail(ST_ERR_ARG, "st_track_dz: bad argument");
+    st_ctx *c = t->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    if ((size_t)t->n > c->cap_x04) {
+        hipFree(c->d_x04);
+        c->d_x04 = nullptr;
+        c->cap_x04 = 0;
+        HIPCHK(hipMalloc(&c->d_x04, (size_t)(t->n > 0 ? t->n : 1) * sizeof(double)));
+        c->cap_x04 = (size_t)t->n;
+    }
+    HIPCHK(hipMemcpyAsync(c->d_x04, dz0, (size_t)t->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return track_host(t, x, np, nturn, stable, lost_turn, lost_post, false, RADIAL_GIVEN, nullptr, -1);
 }
 
 extern "C" int st_track_device(st_table *t, double *d_x, int np, int turn0, int nturn, int *d_state, int *d_lost,

@@ -3,6 +3,12 @@ driver names and output files.
 
 - cal_orbit_num: Cal_Orbit_Num (simtrack.cpp:14947-15030), the closed orbit
   by Newton iteration on the GPU-tracked one-turn map (st_closed_orbit).
+- cal_tunes_track(_x), cal_chrom_track: Cal_Tunes_Track (cpp:20971,
+  20992) and Cal_Chrom_Track (cpp:21028), tunes of tracked turns.
+- cal_spin_orbit, cal_spin_tune, set_initial_spin: Cal_Spin_Orbit
+  (cpp:20510-20572), Cal_Spin_Tune (cpp:20574-20601) and Set_Initial_Spin
+  (cpp:20603-20635) on GPU-tracked sPass turns (st_spin_orbit,
+  st_spin_tune, st_set_initial_spin).
 - Track_tbt_FMA (cpp:21179-21241): frequency-map scan, tunes of two
   windows of turn-by-turn data for every stable grid particle
   (st_fma_scan), written to FMA-output.dat.
@@ -49,6 +55,92 @@ def cal_orbit_num(ctx, gp, records, deltap):
     check(lib.st_closed_orbit(ctx.h, ctypes.byref(gp), _ptr(rec), len(rec), float(deltap), _ptr(xc), _ptr(xe),
                               ctypes.byref(it)), "st_closed_orbit")
     return xc, xe, it.value
+
+
+def cal_spin_orbit(ctx, gp, records, deltap):
+    """Cal_Spin_Orbit: returns (x0, n0, iterations).  x0 is the closed orbit
+    at the start (Cell[Ncell-1]->X) with the periodic spin; n0[j] is the
+    normalised spin after element j (the reference's Cell[j]->n0)."""
+    lib = load()
+    rec = np.ascontiguousarray(records)
+    x0, n0, it = np.zeros(9), np.zeros((len(rec), 3)), ctypes.c_int()
+    check(lib.st_spin_orbit(ctx.h, ctypes.byref(gp), _ptr(rec), len(rec), float(deltap), _ptr(x0), _ptr(n0),
+                            ctypes.byref(it)), "st_spin_orbit")
+    return x0, n0, it.value
+
+
+def cal_spin_tune(ctx, gp, records, deltap):
+    """Cal_Spin_Tune: (spin tune on [0.001, 0.499], n0 of the last element)."""
+    lib = load()
+    rec = np.ascontiguousarray(records)
+    tune, n0 = np.zeros(1), np.zeros(3)
+    check(lib.st_spin_tune(ctx.h, ctypes.byref(gp), _ptr(rec), len(rec), float(deltap), _ptr(tune), _ptr(n0)),
+          "st_spin_tune")
+    return float(tune[0]), n0
+
+
+def set_initial_spin(ctx, gp, records, nturn, n0, x):
+    """Set_Initial_Spin: x (9 coordinates) with its spin set to the
+    normalised turn-averaged projection of the three unit spins on n0."""
+    lib = load()
+    rec = np.ascontiguousarray(records)
+    x = np.array(x, dtype=np.float64).reshape(9).copy()
+    n0 = np.ascontiguousarray(n0, dtype=np.float64).reshape(3)
+    check(lib.st_set_initial_spin(ctx.h, ctypes.byref(gp), _ptr(rec), len(rec), int(nturn), _ptr(n0), _ptr(x)),
+          "st_set_initial_spin")
+    return x
+
+
+def cal_tunes_track_x(tab, x, nturn, tune_low=0.660, tune_high=0.70):
+    """Cal_Tunes_Track(Line, x, Nturn, tune1, tune2), simtrack.cpp:20971-20990:
+    x[0] and x[2] below 1e-9 are raised by 1e-9, Track_tbt for nturn turns,
+    FineTuneFinder of the x and y samples in [tune_low, tune_high] (one window
+    for both, the reference's globals).  Returns (tune1, tune2, x as
+    started); the tunes are NaN for a lost particle (the reference leaves
+    them unset)."""
+    x = np.array(x, dtype=np.float64).reshape(1, 6).copy()
+    if x[0, 0] < 1.0e-9:
+        x[0, 0] = x[0, 0] + 1.0e-9
+    if x[0, 2] < 1.0e-9:
+        x[0, 2] = x[0, 2] + 1.0e-9
+    x0 = x[0].copy()
+    _, tbt, stable, _, _ = tab.track_tbt(x, int(nturn))
+    if stable[0] != 1:
+        return math.nan, math.nan, x0
+    q = tab.ctx.fine_tune(np.stack([tbt[0, :, 0], tbt[0, :, 2]]), tune_low, tune_high)
+    return float(q[0]), float(q[1]), x0
+
+
+def cal_tunes_track(ctx, gp, records, deltap0, tune_low=0.660, tune_high=0.70, tab=None):
+    """Cal_Tunes_Track(Line, deltap0), simtrack.cpp:20992-21026: Cal_Orbit_Num,
+    then the tunes of 1,024 tracked turns from the closed orbit (the line as
+    given, RF on).  Returns (Tune1, Tune2)."""
+    from ._capi import Table
+    _, xe, _ = cal_orbit_num(ctx, gp, records, deltap0)
+    x = np.array([xe[0], xe[1], xe[2], xe[3], 0., deltap0])
+    own = tab is None
+    tab = Table(ctx, gp, records) if own else tab
+    try:
+        q1, q2, _ = cal_tunes_track_x(tab, x, 1024, tune_low, tune_high)
+    finally:
+        if own:
+            tab.close()
+    return q1, q2
+
+
+def cal_chrom_track(ctx, gp, records, tune_low=0.660, tune_high=0.70):
+    """Cal_Chrom_Track, simtrack.cpp:21028-21051: tracked tunes at deltap
+    +0.0003, -0.0003 and 0.  Returns (Chromx1, Chromy1, Chromx2, Chromy2)."""
+    from ._capi import Table
+    tab = Table(ctx, gp, records)
+    try:
+        qxp, qyp = cal_tunes_track(ctx, gp, records, 0.0003, tune_low, tune_high, tab)
+        qxm, qym = cal_tunes_track(ctx, gp, records, -0.0003, tune_low, tune_high, tab)
+        qx0, qy0 = cal_tunes_track(ctx, gp, records, 0.00, tune_low, tune_high, tab)
+    finally:
+        tab.close()
+    return ((qxp - qxm) / 2. / 0.0003, (qyp - qym) / 2. / 0.0003,
+            (qxp + qxm - 2 * qx0) / 2. / 0.0003 / 0.0003, (qyp + qym - 2 * qy0) / 2. / 0.0003 / 0.0003)
 
 
 def fma_scan(tab, deltap0, sigmax0, sigmay0, x_end, tune_low=0.660, tune_high=0.70, nturn=2048, sigma_step=0.1,

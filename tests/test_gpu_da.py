@@ -28,13 +28,14 @@ def oracle_stable(orc, x, radial):
     return int(s[0])
 
 
-def oracle_bisect(orc, angle, sigma_max, sx, sy, dp, radial):
-    """do-while loop of Track_DA, simtrack.cpp:21403-21435"""
+def oracle_bisect(orc, angle, sigma_max, sx, sy, dp, radial, single=False):
+    """do-while loop of Track_DA, simtrack.cpp:21403-21435 (twins), or of
+    Track_DA2, cpp:21511-21526 (one particle)"""
     s, e = 0., sigma_max
     while True:
         sigma = (s + e) / 2.0
         st1 = oracle_stable(orc, particle(sigma, angle, sx, sy, dp), radial)
-        st2 = oracle_stable(orc, particle(sigma + 0.05, angle, sx, sy, dp), radial)
+        st2 = 1 if single else oracle_stable(orc, particle(sigma + 0.05, angle, sx, sy, dp), radial)
         stable = 1 if st1 * st2 == 1 else 0
         if stable == 0:
             e = sigma
@@ -67,6 +68,25 @@ def test_bisection_matches_sequential_loop(ring, variant):
     ref = [oracle_bisect(orc, a, smax, sx, sy, dp, radial) for a in angles]
     assert list(got) == ref
     assert 0 < min(got) and max(got) < smax, "the scan should find a finite aperture"
+
+
+@pytest.mark.parametrize("variant", [st.ST_DA_TRACK_DA2, st.ST_DA_TRACK_DA3])
+def test_single_particle_bisection_matches_sequential_loop(ring, variant):
+    """Track_DA2 (cpp:21497) and Track_DA3 (cpp:21535): one particle per
+    midpoint, angles (90/11)(i+1) deg on [0, 50] and 15(i+1) deg on [0, 30]"""
+    tab, orc = ring
+    sx, sy, dp = 6e-4, 4e-4, 2e-4
+    angles, smax = da.da_angles(tab.lib, variant)
+    if variant == st.ST_DA_TRACK_DA3:
+        assert np.array_equal(angles, [15 * (i + 1) * PI / 180 for i in range(5)]) and smax == 30.0
+    else:
+        assert np.array_equal(angles, [(90 / 11.) * (i + 1) * PI / 180 for i in range(10)]) and smax == 50.0
+    got = da.da_bisect(tab, NTURN, dp, sx, sy, angles, smax, single=True)
+    ref = [oracle_bisect(orc, a, smax, sx, sy, dp, False, single=True) for a in angles]
+    assert list(got) == ref
+    assert 0 < min(got) and max(got) < smax, "the scan should find a finite aperture"
+    twins = da.da_bisect(tab, NTURN, dp, sx, sy, angles, smax)
+    assert np.all(twins <= got + 0.1), "the twin criterion is the stricter one"
 
 
 @pytest.mark.parametrize("radial,sigma0,n", [(False, 1.0, 150), (True, 3.0, 100)])
