@@ -14,6 +14,7 @@
  *   Track_DA_Radial, Track_DA_Uniform, Track_DA_Radial_Uniform
  *   Read_MADX_Lattice, Read_BMAD_Lattice               simtrack.cpp:13591, 14054
  *   Cal_Orbit_Num, Track_tbt_FMA, Track_tbt_Lyapunov   simtrack.cpp:14947, 21179, 21243
+ *   Track_tbt_tune_footprint                           simtrack.cpp:21100
  *   Cal_Spin_Orbit, Cal_Spin_Tune, Set_Initial_Spin    simtrack.cpp:20510, 20574, 20603
  *   Cal_Tunes_Track (both forms), Cal_Chrom_Track      simtrack.cpp:20971, 20992, 21028
  *   BeamBeam4D_SoftGaussian, BeamBeam6D_SoftGaussian   simtrack.cpp:22846, 22874, 23134
@@ -1402,6 +1403,72 @@ inline void Track_tbt_Lyapunov(Line &l, double deltap0, double sigmax0, double s
         const double *r = &rows[(size_t)m * 3];
         std::fprintf(f, "%s  %s%s\n", detail::amp(r[0], m == 0).c_str(), detail::amp(r[1], m == 0).c_str(),
                      detail::sci25(r[2]).c_str());
+    }
+    std::fclose(f);
+}
+
+inline void Cal_Twiss(Line &l, double deltap);
+
+/* Track_tbt_tune_footprint (simtrack.cpp:21100-21176): Cal_Twiss and
+   Cal_Orbit_Num at deltap0; on 5 angles x 6 amplitudes (0.1, 1..5 sigma of
+   the unnormalised emittances emitx, emity, with the Twiss at the start)
+   about the closed orbit, 2,048 turns of Track_tbt (one batch) and
+   FineTuneFinder of turns [0, 1024) and [1023, 2047) in [tune_low,
+   tune_high]: ./footprint-output.dat */
+inline void Track_tbt_tune_footprint(Line &l, double deltap0, double emitx, double emity)
+{
+    const double PI_ = 3.14159265358979323; /* PI, simtrack.cpp:23 */
+    const int Nturn = 2048;
+    Cal_Twiss(l, deltap0);
+    const Element *last = l.Cell[l.Ncell - 1];
+    const double betax0 = last->Beta1, betay0 = last->Beta2, alfax0 = last->Alfa1, alfay0 = last->Alfa2;
+    Cal_Orbit_Num(l, deltap0);
+    std::vector<double> x(30 * 6);
+    for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 6; j++) {
+            double *p = &x[(size_t)(i * 6 + j) * 6];
+            double nsigma = j;
+            if (nsigma == 0) nsigma = 0.1;
+            double nsigmax = nsigma * std::cos((i + 1) * 15 * PI_ / 180);
+            double nsigmay = nsigma * std::sin((i + 1) * 15 * PI_ / 180);
+            double xn = std::sqrt(emitx) * nsigmax, pxn = 0.;
+            double yn = std::sqrt(emity) * nsigmay, pyn = 0.;
+            p[0] = std::sqrt(betax0) * xn;
+            p[1] = -alfax0 * xn / std::sqrt(betax0) + 1.0 * pxn / std::sqrt(betax0);
+            p[2] = std::sqrt(betay0) * yn;
+            p[3] = -alfay0 * yn / std::sqrt(betay0) + 1.0 * pyn / std::sqrt(betay0);
+            p[4] = 0.;
+            p[5] = deltap0;
+            for (int k = 0; k < 4; k++) p[k] = p[k] + last->X[k];
+            if (p[0] == 0.) p[0] = p[0] + 1.0e-09;
+            if (p[2] == 0.) p[2] = p[2] + 1.0e-09;
+        }
+    std::vector<double> tbt((size_t)30 * Nturn * 6);
+    std::vector<int> st(30, 1), lt(30, 0), lp(30, 0);
+    tracker_for(l).Track_tbt(x.data(), 30, Nturn, tbt.data(), -1, false, st.data(), lt.data(), lp.data());
+    std::vector<double> sig;
+    std::vector<int> good;
+    for (int q = 0; q < 30; ++q) {
+        if (st[q] != 1) continue;
+        good.push_back(q);
+        const double *b = &tbt[(size_t)q * Nturn * 6];
+        for (int w = 0; w < 4; ++w) {
+            const int first = (w & 1) ? 1023 : 0, col = (w & 2) ? 2 : 0;
+            for (int k = 0; k < 1024; k++) sig.push_back(b[(size_t)(k + first) * 6 + col]);
+        }
+    }
+    std::vector<double> tunes(good.size() * 4);
+    if (!good.empty())
+        check(st_fine_tune(default_context(), sig.data(), nullptr, 1024, (int)tunes.size(), tune_low, tune_high,
+                           tunes.data()),
+              "FineTuneFinder");
+    FILE *f = std::fopen("./footprint-output.dat", "w");
+    if (!f) throw Error("cannot open ./footprint-output.dat");
+    for (size_t g = 0; g < good.size(); ++g) {
+        const double *t = &tunes[g * 4]; /* x1, x2, y1, y2 */
+        std::fprintf(f, "%d %d %s %s %s %s  %s %s\n", (good[g] / 6 + 1) * 15, good[g] % 6, detail::sci25(t[0]).c_str(),
+                     detail::sci25(t[1]).c_str(), detail::sci25(t[2]).c_str(), detail::sci25(t[3]).c_str(),
+                     detail::sci25(t[0] - t[1]).c_str(), detail::sci25(t[2] - t[3]).c_str());
     }
     std::fclose(f);
 }

@@ -3,6 +3,9 @@ driver names and output files.
 
 - cal_orbit_num: Cal_Orbit_Num (simtrack.cpp:14947-15030), the closed orbit
   by Newton iteration on the GPU-tracked one-turn map (st_closed_orbit).
+- Track_tbt_tune_footprint (cpp:21100-21176): tunes of two windows of
+  turn-by-turn data on a 5-angle x 6-amplitude grid normalised by the
+  Twiss at the start (footprint_scan), footprint-output.dat.
 - cal_tunes_track(_x), cal_chrom_track: Cal_Tunes_Track (cpp:20971,
   20992) and Cal_Chrom_Track (cpp:21028), tunes of tracked turns.
 - cal_spin_orbit, cal_spin_tune, set_initial_spin: Cal_Spin_Orbit
@@ -38,6 +41,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from ._capi import STError, check, load
+
+PI = 3.14159265358979323    # the reference's PI, simtrack.cpp:23
 
 
 def _ptr(a):
@@ -448,6 +453,76 @@ def Track_tbt_FMA(tab, deltap0, sigmax0, sigmay0, x_end, tune_low=0.660, tune_hi
     return rows
 
 
+def footprint_scan(tab, deltap0, emitx, emity, twiss, x_end, tune_low=0.660, tune_high=0.70, nturn=2048):
+    """The grid of Track_tbt_tune_footprint (simtrack.cpp:21100-21176): at
+    angles 15(i+1) deg and nsigma = j (0.1 for j = 0), i < 5, j < 6, the
+    particle (sqrt(beta) xn, -alpha xn / sqrt(beta), ...) with
+    xn = sqrt(emit) nsigma cos or sin, plus the closed orbit x_end, x or y
+    exactly 0 raised by 1e-9; Track_tbt for nturn turns, FineTuneFinder of
+    turns [0, 1024) and [1023, 2047).  twiss: (betx, alfx, bety, alfy) at the
+    start.  Rows (angle deg, j, tunex1, tunex2, tuney1, tuney2, dqx, dqy) of
+    the stable particles, in the reference's order."""
+    betax0, alfax0, betay0, alfay0 = twiss
+    xs, keys = [], []
+    for i in range(5):
+        for j in range(6):
+            nsigma = float(j)
+            if nsigma == 0:
+                nsigma = 0.1
+            nsigmax = nsigma * math.cos((i + 1) * 15 * PI / 180)
+            nsigmay = nsigma * math.sin((i + 1) * 15 * PI / 180)
+            xn = math.sqrt(emitx) * nsigmax
+            pxn = 0.
+            yn = math.sqrt(emity) * nsigmay
+            pyn = 0.
+            x = [math.sqrt(betax0) * xn, -alfax0 * xn / math.sqrt(betax0) + 1.0 * pxn / math.sqrt(betax0),
+                 math.sqrt(betay0) * yn, -alfay0 * yn / math.sqrt(betay0) + 1.0 * pyn / math.sqrt(betay0), 0.,
+                 deltap0]
+            for k in range(4):
+                x[k] = x[k] + x_end[k]
+            if x[0] == 0.:
+                x[0] = x[0] + 1.0e-09
+            if x[2] == 0.:
+                x[2] = x[2] + 1.0e-09
+            xs.append(x)
+            keys.append(((i + 1) * 15, j))
+    _, tbt, stable, _, _ = tab.track_tbt(np.array(xs), int(nturn))
+    rows = []
+    good = [p for p in range(len(xs)) if stable[p] == 1]
+    if good:
+        sig = []
+        for p in good:
+            sig += [tbt[p, 0:1024, 0], tbt[p, 1023:2047, 0], tbt[p, 0:1024, 2], tbt[p, 1023:2047, 2]]
+        q = tab.ctx.fine_tune(np.stack(sig), tune_low, tune_high).reshape(len(good), 4)
+        for p, (qx1, qx2, qy1, qy2) in zip(good, q):
+            a, j = keys[p]
+            rows.append((a, j, qx1, qx2, qy1, qy2, qx1 - qx2, qy1 - qy2))
+    return rows
+
+
+def Track_tbt_tune_footprint(ctx, gp, records, deltap0, emitx, emity, tune_low=0.660, tune_high=0.70,
+                             filename="./footprint-output.dat", tab=None):
+    """Track_tbt_tune_footprint (simtrack.cpp:21100-21176): Cal_Twiss at
+    deltap0 (the last element's Beta/Alfa), Cal_Orbit_Num at deltap0, the
+    grid of footprint_scan, and the reference's footprint-output.dat lines.
+    emitx, emity: unnormalised emittances.  Returns the rows."""
+    from ._capi import Table
+    tw = cal_twiss(ctx, gp, records, deltap0)
+    twiss = (float(tw.beta1[-1]), float(tw.alfa1[-1]), float(tw.beta2[-1]), float(tw.alfa2[-1]))
+    _, x_end, _ = cal_orbit_num(ctx, gp, records, deltap0)
+    own = tab is None
+    tab = Table(ctx, gp, records) if own else tab
+    try:
+        rows = footprint_scan(tab, deltap0, emitx, emity, twiss, x_end, tune_low, tune_high)
+    finally:
+        if own:
+            tab.close()
+    with open(filename, "w") as f:
+        for r in rows:
+            f.write(f"{r[0]} {r[1]} {_sci(r[2])} {_sci(r[3])} {_sci(r[4])} {_sci(r[5])}  {_sci(r[6])} {_sci(r[7])}\n")
+    return rows
+
+
 def Track_tbt_Lyapunov(tab, deltap0, sigmax0, sigmay0, filename="./Lmap-output.dat", **kw):
     """Track_tbt_Lyapunov with the reference's output lines (cpp:21295-21297)."""
     rows = lyapunov_scan(tab, deltap0, sigmax0, sigmay0, **kw)
@@ -458,6 +533,8 @@ def Track_tbt_Lyapunov(tab, deltap0, sigmax0, sigmay0, filename="./Lmap-output.d
 
 
 __all__ = ["cal_orbit_num", "fma_scan", "lyapunov_scan", "Track_tbt_FMA", "Track_tbt_Lyapunov", "one_turn_matrix",
+           "footprint_scan", "Track_tbt_tune_footprint", "cal_tunes_track", "cal_tunes_track_x", "cal_chrom_track",
+           "cal_spin_orbit", "cal_spin_tune", "set_initial_spin",
            "twiss_from_matrix", "twiss_at_start", "action_diff", "Track_tbt_ActionDiff", "STError", "delta_to_pt",
            "element_maps", "cal_a", "trace_a", "cal_twiss", "LineTwiss", "cal_dispersion", "cal_chrom",
            "set_line_optics", "Print_Twiss"]

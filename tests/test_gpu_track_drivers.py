@@ -42,3 +42,25 @@ def test_tunes_and_chromaticity_equal_oracle(ctx):
     co = ((qp[0] - qm[0]) / 2. / 0.0003, (qp[1] - qm[1]) / 2. / 0.0003,
           (qp[0] + qm[0] - 2 * q0[0]) / 2. / 0.0003 / 0.0003, (qp[1] + qm[1] - 2 * q0[1]) / 2. / 0.0003 / 0.0003)
     assert cg == co
+
+
+def test_footprint_equals_oracle(ctx, tmp_path):
+    """Track_tbt_tune_footprint (cpp:21100-21176): the GPU's turn-by-turn
+    tracking and FineTuneFinder against the oracle's on the same grid, bit
+    for bit; and the full driver (Cal_Twiss and Cal_Orbit_Num on the GPU)
+    writes one line per stable grid particle"""
+    from test_track_drivers import OracleTable
+    gp, ln = tune_ring()
+    rec = ln.records()
+    tab = st.Table(ctx, gp, rec)
+    twiss = (33.0, 0.4, 28.0, -0.3)
+    x_end = np.array([1e-4, -2e-6, 5e-5, 1e-6, 0., 0.])
+    for emit in (5e-9, 2e-6):
+        rg = D.footprint_scan(tab, 0., emit, emit, twiss, x_end, *WINDOW)
+        ro = D.footprint_scan(OracleTable(gp, rec), 0., emit, emit, twiss, x_end, *WINDOW)
+        assert rg == ro
+    f = tmp_path / "footprint.dat"
+    rows = D.Track_tbt_tune_footprint(ctx, gp, rec, 0., 5e-9, 5e-9, *WINDOW, filename=str(f), tab=tab)
+    lines = f.read_text().splitlines()
+    assert len(lines) == len(rows) == 30 and lines[0].startswith("15 0 ")
+    tab.close()

@@ -141,3 +141,36 @@ def test_tracked_chromaticity_is_the_matrix_chromaticity():
     mx, my = (mp[0] - mm[0]) / 2. / 0.0003, (mp[1] - mm[1]) / 2. / 0.0003
     assert mx < -0.3 and my < -0.3, "a FODO ring without sextupoles has a negative natural chromaticity"
     assert abs(cx / mx - 1) < 0.02 and abs(cy / my - 1) < 0.02
+
+
+class OracleTable:
+    """The oracle behind the Table interface footprint_scan uses
+    (track_tbt, ctx.fine_tune), so the same host code runs on both."""
+
+    def __init__(self, gp, rec):
+        self.orc = pyoracle.Oracle(gp, rec, nthreads=1)
+        self.ctx = self
+
+    def track_tbt(self, x, nturn):
+        xo, tbt, s, lt, lp = self.orc.track(np.asarray(x, dtype=np.float64), nturn, tbt=True)
+        return xo, tbt, s, lt, lp
+
+    def fine_tune(self, sig, lo, hi):
+        return np.array([pyoracle.fine_tune(r, lo, hi) for r in np.atleast_2d(sig)])
+
+
+def test_footprint_of_a_linear_ring_is_the_linear_tunes():
+    """Track_tbt_tune_footprint's grid (cpp:21100-21176) on a ring without
+    sextupoles: every amplitude has the linear tunes, in both windows"""
+    gp, ln = tune_ring()
+    rec = ln.records()
+    orc = pyoracle.Oracle(gp, rec, nthreads=1)
+    opt = O.Optics(ln)
+    m = orc.one_turn_matrix(np.zeros(6), 1e-6)
+    bx, ax, _, by, ay, _ = D.twiss_from_matrix(m)
+    emit = 5e-9
+    rows = D.footprint_scan(OracleTable(gp, rec), 0., emit, emit, (bx, ax, by, ay), np.zeros(6), *WINDOW)
+    assert [(r[0], r[1]) for r in rows] == [(15 * (i + 1), j) for i in range(5) for j in range(6)]
+    for r in rows:
+        assert abs(r[2] - opt.tune_x) < 2e-3 and abs(r[4] - opt.tune_y) < 2e-3
+        assert abs(r[6]) < 1e-5 and abs(r[7]) < 1e-5, "no tune drift between the windows of a linear ring"
