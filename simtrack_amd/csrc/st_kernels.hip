@@ -310,6 +310,29 @@ ST_DEV void sbend_slice_t(A &ar, St &s, double &pz0, double &as1, double x3, con
     as1 = as2;
 }
 
+/* nint slices with or without the spin rotation: the loop is written once
+   per case, so each slice body (orbit and spin) is one scheduling region */
+template <bool ROT>
+ST_DEV void sbend_slice_loop(St &s, const Ptc &c, int nint, double &pz0, double &as1, double x3, const StRcp &rt,
+                             const StRcp &rh, double L, double hs, double ih, double cosA, double sinA, double Lb,
+                             double An, double href, st_gref g)
+{
+    for (int i = 0; i < nint; i++) {
+        St s0 = s;
+        double pz00 = pz0, as10 = as1;
+        ArFast af;
+        sbend_slice_t<ROT>(af, s, pz0, as1, x3, rt, rh, L, hs, ih, cosA, sinA, Lb, An, href, c, g);
+        if (__builtin_expect(af.bad(), 0)) {
+            ST_FD_COLD_PATH();
+            s = s0;
+            pz0 = pz00;
+            as1 = as10;
+            ArExact ae;
+            sbend_slice_t<ROT>(ae, s, pz0, as1, x3, rt, rh, L, hs, ih, cosA, sinA, Lb, An, href, c, g);
+        }
+    }
+}
+
 template <bool SPIN>
 ST_DEV void sbend_slices_exact(St &s, Ptc &c, int nint, double L, double hs, double ih, double cosA, double sinA,
                                double Lb, double An, double href, st_gref g)
@@ -323,26 +346,10 @@ ST_DEV void sbend_slices_exact(St &s, Ptc &c, int nint, double L, double hs, dou
     double as1 = st_asin(x[1] / temp);
     /* spin() runs only for a non-zero field; B = (0, An, 0) here */
     const bool rot = SPIN && (0. * 0. + An * An + 0. * 0. != 0.);
-    for (int i = 0; i < nint; i++) {
-        St s0 = s;
-        double pz00 = pz0, as10 = as1;
-        ArFast af;
-        if (rot)
-            sbend_slice_t<true>(af, s, pz0, as1, x3, rt, rh, L, hs, ih, cosA, sinA, Lb, An, href, c, g);
-        else
-            sbend_slice_t<false>(af, s, pz0, as1, x3, rt, rh, L, hs, ih, cosA, sinA, Lb, An, href, c, g);
-        if (__builtin_expect(af.bad(), 0)) {
-            ST_FD_COLD_PATH();
-            s = s0;
-            pz0 = pz00;
-            as1 = as10;
-            ArExact ae;
-            if (rot)
-                sbend_slice_t<true>(ae, s, pz0, as1, x3, rt, rh, L, hs, ih, cosA, sinA, Lb, An, href, c, g);
-            else
-                sbend_slice_t<false>(ae, s, pz0, as1, x3, rt, rh, L, hs, ih, cosA, sinA, Lb, An, href, c, g);
-        }
-    }
+    if (rot)
+        sbend_slice_loop<true>(s, c, nint, pz0, as1, x3, rt, rh, L, hs, ih, cosA, sinA, Lb, An, href, g);
+    else
+        sbend_slice_loop<false>(s, c, nint, pz0, as1, x3, rt, rh, L, hs, ih, cosA, sinA, Lb, An, href, g);
 }
 
 /*
