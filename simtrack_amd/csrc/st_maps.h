@@ -390,17 +390,19 @@ ST_DEV void mult_field_t(A &ar, const St &s, int norder, int mask, st_pp kn, st_
     double Xn = 1., Yn = 0.;
     By = kn[0];
     Bx = ks[0];
+#pragma unroll 2
     for (int i = 1; i < norder + 1; i++) {
+        const double kni = kn[i], ksi = ks[i], fi = st_fact[i], yi = st_rfact[i];
         double Xn0 = Xn, Yn0 = Yn;
         Xn = Xn0 * x[0] - Yn0 * x[2];
         Yn = Xn0 * x[2] + Yn0 * x[0];
         if ((mask >> i) & 1) {
             StRcp rf;
-            rf.b = st_fact[i];
-            rf.y = st_rfact[i];
+            rf.b = fi;
+            rf.y = yi;
             rf.ok = true;
-            By = By + ar.div(kn[i] * Xn - ks[i] * Yn, rf);
-            Bx = Bx + ar.div(kn[i] * Yn + ks[i] * Xn, rf);
+            By = By + ar.div(kni * Xn - ksi * Yn, rf);
+            Bx = Bx + ar.div(kni * Yn + ksi * Xn, rf);
         }
     }
 }
