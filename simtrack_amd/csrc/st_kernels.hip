@@ -19,6 +19,16 @@
 #define ST_FN_COLD __device__ __noinline__
 #define ST_CONST_TABLE static __device__ __constant__ const
 #define ST_FMA(a, b, c) __builtin_fma((a), (b), (c))
+/* a fused multiply-add with a constant addend, written as the three-address
+   v_fma_f64: the compiler's two-address v_fmac_f64 would first copy the
+   constant (kept in a VGPR across the loop) into the destination */
+__device__ __forceinline__ double st_fma_k(double a, double b, double c)
+{
+    double r;
+    __asm__("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+#define ST_FMA_K(a, b, c) st_fma_k((a), (b), (c))
 #define ST_RINT(x) __builtin_rint(x)
 #include "st_fastdiv.h"
 #define ST_SQRT(x) st_fsqrt(x)

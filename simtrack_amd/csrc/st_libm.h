@@ -43,6 +43,9 @@
 #include <stdint.h>
 #include "st_libm_tables.h"
 
+#ifndef ST_FMA_K
+#define ST_FMA_K(a, b, c) ST_FMA(a, b, c)
+#endif
 #if !defined(ST_FN) || !defined(ST_CONST_TABLE) || !defined(ST_FMA) || \
     !defined(ST_RINT) || !defined(ST_SQRT)
 #error "define ST_FN, ST_CONST_TABLE, ST_FMA, ST_RINT, ST_SQRT before including st_libm.h"
@@ -126,14 +129,14 @@ ST_FN double st_ksin(double x, double y, int has_y)
 {
     double z = x * x;
     double v = z * x;
-    double r = ST_FMA(z, ST_SIN_8, ST_SIN_7);
-    r = ST_FMA(z, r, ST_SIN_6);
-    r = ST_FMA(z, r, ST_SIN_5);
-    r = ST_FMA(z, r, ST_SIN_4);
-    r = ST_FMA(z, r, ST_SIN_3);
-    r = ST_FMA(z, r, ST_SIN_2);
+    double r = ST_FMA_K(z, ST_SIN_8, ST_SIN_7);
+    r = ST_FMA_K(z, r, ST_SIN_6);
+    r = ST_FMA_K(z, r, ST_SIN_5);
+    r = ST_FMA_K(z, r, ST_SIN_4);
+    r = ST_FMA_K(z, r, ST_SIN_3);
+    r = ST_FMA_K(z, r, ST_SIN_2);
     if (!has_y)
-        return x + v * ST_FMA(z, r, ST_SIN_1);
+        return x + v * ST_FMA_K(z, r, ST_SIN_1);
     return x - ((z * (0.5 * y - v * r) - y) - v * ST_SIN_1);
 }
 
@@ -141,13 +144,13 @@ ST_FN double st_ksin(double x, double y, int has_y)
 ST_FN double st_kcos(double x, double y)
 {
     double z = x * x;
-    double r = ST_FMA(z, ST_COS_8, ST_COS_7);
-    r = ST_FMA(z, r, ST_COS_6);
-    r = ST_FMA(z, r, ST_COS_5);
-    r = ST_FMA(z, r, ST_COS_4);
-    r = ST_FMA(z, r, ST_COS_3);
-    r = ST_FMA(z, r, ST_COS_2);
-    r = ST_FMA(z, r, ST_COS_1);
+    double r = ST_FMA_K(z, ST_COS_8, ST_COS_7);
+    r = ST_FMA_K(z, r, ST_COS_6);
+    r = ST_FMA_K(z, r, ST_COS_5);
+    r = ST_FMA_K(z, r, ST_COS_4);
+    r = ST_FMA_K(z, r, ST_COS_3);
+    r = ST_FMA_K(z, r, ST_COS_2);
+    r = ST_FMA_K(z, r, ST_COS_1);
     r = z * r;
     double hz = 0.5 * z;
     double w = 1.0 - hz;
@@ -300,7 +303,7 @@ ST_FN double st_cos(double x)
 /* Horner evaluation c0 + t*(c1 + t*(...)) of the first 10 and all 27
    series coefficients, written out so the coefficients are immediate
    operands rather than table loads. */
-#define ST_ASIN_STEP(p, t, i) p = ST_FMA(p, t, ST_ASIN_C##i)
+#define ST_ASIN_STEP(p, t, i) p = ST_FMA_K(p, t, ST_ASIN_C##i)
 ST_FN double st_asin_poly10(double t)
 {
     double p = ST_ASIN_C9;
