@@ -150,7 +150,7 @@ def main():
     # HBM bytes per launch of this shape, measured with rocprofv3 FETCH_SIZE and
     # WRITE_SIZE against a copy of known size (profiles/k_track_spin_config1_traffic.txt);
     # the state is read and written once per launch, independent of the turn count
-    traffic = 13.75e6 if args.particles == 65536 else None
+    traffic = 13.87e6 if args.particles == 65536 else None
     config = {"workload": "config1: 998-element DA ring + 6-D beam-beam at 2 IPs + spin, 100 GeV p",
               "particles": args.particles, "turns_per_step": args.turns_per_step, "elements": int(len(ln.Cell)),
               "spin": True, "beam_beam_slices": 5}
