@@ -37,7 +37,8 @@ struct StRcp {
     bool ok;    /* 2^-60 < |b| < 2^60                                       */
 };
 
-ST_FD StRcp st_rcp_prep(double b)
+/* the refined reciprocal alone, for a caller that checks the range of b itself */
+ST_FD StRcp st_rcp_y(double b)
 {
     StRcp r;
     r.b = b;
@@ -47,6 +48,13 @@ ST_FD StRcp st_rcp_prep(double b)
     e = __builtin_fma(-b, y, 1.0);
     y = __builtin_fma(y, e, y);
     r.y = y;
+    r.ok = true;
+    return r;
+}
+
+ST_FD StRcp st_rcp_prep(double b)
+{
+    StRcp r = st_rcp_y(b);
     double ab = __builtin_fabs(b);
     r.ok = ab > 0x1p-60 && ab < 0x1p60;
     return r;

@@ -103,15 +103,27 @@ ST_DEV void ptc_update(Ptc &c, double pt, st_gref g)
  * the scheduler can interleave freely -- the point on a GPU running one
  * wavefront per SIMD, whose speed is set by instruction-level parallelism.
  *
- * The preconditions are tracked cheaply: the smallest and the largest of
- * every |a * y| (the first product of a division) and every sqrt argument,
- * two FP64 min/max per operation, tested once per step.  The division needs
- * 2^-60 < |b| < 2^60 (StRcp::ok) and 2^-900 < |a/b| < 2^700; sqrt needs
- * 2^-767 <= x < inf.  One window [2^-767, 2^700) for both is stricter than
- * either (|a * y| is within 2 ulp of a/b), so it only sends a few more lanes
- * to the exact replay.  Zero, infinite and negative values fall outside it.
- * A NaN never reaches lo or hi (min/max return the other operand); NaN in
- * gives NaN out on both paths, and NaN sign and payload carry no meaning.
+ * The preconditions are tracked cheaply, tested once per step.  The
+ * compiler's division sequence (st_fastdiv.h) reduces to y = refined 1/b,
+ * q0 = a*y, r = fma(-b, q0, a), q = fma(r, y, q0) unless an exponent is
+ * extreme: b, 1/b or a/b subnormal, exp(a) - exp(b) >= 768, |a| < 2^-969,
+ * or a zero, infinite or NaN operand or result; sqrt reduces to its
+ * refinement for 2^-767 <= x < inf.  One window W = [2^-262, 2^263) serves
+ * every check: with |q0| (within 2 ulp of |a/b|) and |b| in W, |a| lies in
+ * [2^-525, 2^526), so none of those conditions can hold; a divisor prepared
+ * before the step (StRcp::ok, 2^-60 < |b| < 2^60) is inside W as well.  The
+ * window is far wider than any coordinate, quotient or sqrt argument of a
+ * particle that is still in the machine, so it only sends zeros and
+ * diverging lanes to the exact replay.
+ * Membership is a test on the biased exponent E, 761 <= E < 1286: the high
+ * word shifted left by one (sign dropped) minus 761 << 21 is below
+ * (1286 - 761) << 21 as an unsigned number exactly then, and smaller
+ * exponents wrap around to large values.  So one 32-bit running maximum
+ * covers both ends and every kind of value (one shift-add per value, one
+ * three-way max per two values), where FP64 min/max took two instructions
+ * per value and a divisor's range two compares.  Zero, subnormal, infinite
+ * and NaN values fall outside the window; the sign of a general sqrt
+ * argument is kept apart, as the OR of its high words.
  */
 #ifdef ST_PERF_MODE
 /* Perf mode: a division by a prepared divisor is a * (refined 1/b), within an
@@ -122,6 +134,7 @@ ST_DEV void ptc_update(Ptc &c, double pt, st_gref g)
 struct ArFast {
     bool ok = true;
     ST_DEV bool bad() const { return !ok; }
+    ST_DEV StRcp rcp(double b) { return st_rcp_y(b); }
     ST_DEV double div(double a, const StRcp &r) { return a * r.y; }
     ST_DEV double sqrt(double x)
     {
@@ -135,6 +148,7 @@ struct ArFast {
         g = __builtin_fma(d, h, g);
         return x > 0.0 && x < __builtin_inf() ? g : __builtin_sqrt(x);
     }
+    ST_DEV double sqrt_sq(double x) { return sqrt(x); }
     ST_DEV double asin(double x)
     {
         ok = ok & (__builtin_fabs(x) < 0.125);
@@ -147,22 +161,38 @@ struct ArFast {
     }
 };
 #else
+#define ST_WIN_BASE 0x5F200000u  /* 761 << 21: 2^-262 */
+#define ST_WIN_SPAN 0x41A00000u  /* (1286 - 761) << 21: 2^263 excluded */
 struct ArFast {
-    double lo = __builtin_inf(), hi = 0.;
-    bool ok = true;     /* every divisor in range, every asin/sincos argument small */
-    ST_DEV bool bad() const { return !(ok && lo >= 0x1p-767 && hi < 0x1p700); }
+    unsigned int w = 0;     /* max of (high word << 1) - ST_WIN_BASE over the checked magnitudes */
+    unsigned int sg = 0;    /* OR of the high words of general sqrt arguments: bit 31 = a negative one */
+    bool ok = true;         /* every divisor in range, every asin/sincos argument small */
+    ST_DEV bool bad() const { return !(ok && w < ST_WIN_SPAN && (int)sg >= 0); }
+    ST_DEV void mag(double v)
+    {
+        /* the high word taken as a vector element: one v_lshl_add_u32 (from
+           a 64-bit shift the compiler would shift both words and mask) */
+        typedef unsigned int st_u2 __attribute__((ext_vector_type(2)));
+        unsigned int t = (__builtin_bit_cast(st_u2, v).y << 1) - ST_WIN_BASE;
+        w = t > w ? t : w;
+    }
+    /* a divisor prepared inside the step: its range joins the window */
+    ST_DEV StRcp rcp(double b)
+    {
+        mag(b);
+        return st_rcp_y(b);
+    }
     ST_DEV double div(double a, const StRcp &r)
     {
         double q0 = a * r.y;
         double rr = __builtin_fma(-r.b, q0, a);
         double q = __builtin_fma(rr, r.y, q0);
-        double aq = __builtin_fabs(q0);
-        lo = __builtin_fmin(lo, aq);
-        hi = __builtin_fmax(hi, aq);
+        mag(q0);
         ok = ok & r.ok;
         return q;
     }
-    ST_DEV double sqrt(double x)
+    /* sqrt of a sum of squares (never negative or -0; NaN falls outside the window) */
+    ST_DEV double sqrt_sq(double x)
     {
         double y = __builtin_amdgcn_rsq(x);
         double g = x * y;
@@ -174,9 +204,14 @@ struct ArFast {
         g = __builtin_fma(d, h, g);
         d = __builtin_fma(-g, g, x);
         g = __builtin_fma(d, h, g);
-        lo = __builtin_fmin(lo, x);
-        hi = __builtin_fmax(hi, x);
+        mag(x);
         return g;
+    }
+    ST_DEV double sqrt(double x)
+    {
+        typedef unsigned int st_u2 __attribute__((ext_vector_type(2)));
+        sg |= __builtin_bit_cast(st_u2, x).y;
+        return sqrt_sq(x);
     }
     ST_DEV double asin(double x)
     {
@@ -193,8 +228,10 @@ struct ArFast {
 #endif
 
 struct ArExact {
+    ST_DEV StRcp rcp(double b) { return st_rcp_y(b); }
     ST_DEV double div(double a, const StRcp &r) { return a / r.b; }
     ST_DEV double sqrt(double x) { return __builtin_sqrt(x); }
+    ST_DEV double sqrt_sq(double x) { return __builtin_sqrt(x); }
     ST_DEV double asin(double x) { return st_asin(x); }
     ST_DEV void sincos(double x, double *sn, double *cs) { st_sincos(x, sn, cs); }
 };
@@ -207,14 +244,14 @@ ST_DEV void drift_t(A &ar, St &s, double L, double Lb, const Ptc &c)
 {
     double *x = s.x;
     if (HEXP) {
-        StRcp r5 = st_rcp_prep(1.0 + x[5]);
+        StRcp r5 = ar.rcp(1.0 + x[5]);
         double u = ar.div(L, r5);
         x[0] = x[0] + x[1] * u;
         x[2] = x[2] + x[3] * u;
         x[4] = x[4] - ar.div((x[1] * x[1] + x[3] * x[3]) * u / 2.0, r5);
     } else {
         double pz = ar.sqrt(c.opd2 - x[1] * x[1] - x[3] * x[3]);
-        StRcp rz = st_rcp_prep(pz);
+        StRcp rz = ar.rcp(pz);
         double betaz = ar.div(pz * c.b1, c.ropd);
         x[0] = x[0] + ar.div(x[1] * L, rz);
         x[2] = x[2] + ar.div(x[3] * L, rz);
@@ -373,15 +410,37 @@ __device__ __constant__ const double st_fact[11] = {1., 1., 2., 6., 24., 120., 7
    q0 = a*y, r = fma(-b, q0, a) (exact) and q = fma(r, y, q0) is the
    correctly rounded a/b whenever nothing underflows or overflows
    (Markstein's residual correction), so ArFast::div with {n!, 1/n!} gives
-   the bits of a / n! inside its [2^-767, 2^700) window on |q0|; outside it
-   the step is replayed with the division itself. */
+   the bits of a / n! inside its window on |q0|; outside it the step is
+   replayed with the division itself. */
 __device__ __constant__ const double st_rfact[11] = {1., 1., 1. / 2., 1. / 6., 1. / 24., 1. / 120., 1. / 720.,
                                                      1. / 5040., 1. / 40320., 1. / 362880., 1. / 3628800.};
+
+/* order I of the field sum: Xn + i Yn = (x + i y)^I, and the order's term
+   when its coefficients kni, ksi are not both zero (mask bit I) */
+template <int I, class A>
+ST_DEV void mult_order(A &ar, const double *x, int mask, double kni, double ksi, double &Xn, double &Yn, double &By,
+                       double &Bx)
+{
+    double Xn0 = Xn, Yn0 = Yn;
+    Xn = Xn0 * x[0] - Yn0 * x[2];
+    Yn = Xn0 * x[2] + Yn0 * x[0];
+    if ((mask >> I) & 1) {
+        StRcp rf;
+        rf.b = st_fact[I];
+        rf.y = st_rfact[I];
+        rf.ok = true;
+        By = By + ar.div(kni * Xn - ksi * Yn, rf);
+        Bx = Bx + ar.div(kni * Yn + ksi * Xn, rf);
+    }
+}
 
 /*
  * Field sum of mult_kick_pass / bend_mult_kick_pass / cal_Bfield
  * (simtrack.cpp:4086-4111, 4113-4137, 4169-4196): By + i Bx accumulated over
- * orders 1..Norder whose coefficients are not both zero (mask bit i).
+ * orders 1..Norder whose coefficients are not both zero (mask bit i).  The
+ * orders (norder <= 10, checked when the table is built) are written out one
+ * after the other: n! and 1/n! are constants and the coefficients load at
+ * fixed offsets, with no loop counter or pointer arithmetic.
  */
 template <class A>
 ST_DEV void mult_field_t(A &ar, const St &s, int norder, int mask, st_pp kn, st_pp ks, double &By, double &Bx)
@@ -390,21 +449,12 @@ ST_DEV void mult_field_t(A &ar, const St &s, int norder, int mask, st_pp kn, st_
     double Xn = 1., Yn = 0.;
     By = kn[0];
     Bx = ks[0];
-#pragma unroll 2
-    for (int i = 1; i < norder + 1; i++) {
-        const double kni = kn[i], ksi = ks[i], fi = st_fact[i], yi = st_rfact[i];
-        double Xn0 = Xn, Yn0 = Yn;
-        Xn = Xn0 * x[0] - Yn0 * x[2];
-        Yn = Xn0 * x[2] + Yn0 * x[0];
-        if ((mask >> i) & 1) {
-            StRcp rf;
-            rf.b = fi;
-            rf.y = yi;
-            rf.ok = true;
-            By = By + ar.div(kni * Xn - ksi * Yn, rf);
-            Bx = Bx + ar.div(kni * Yn + ksi * Xn, rf);
-        }
-    }
+#define ST_ORDER(I)                                                                                                    \
+    if (norder < I) return;                                                                                            \
+    mult_order<I>(ar, x, mask, kn[I], ks[I], Xn, Yn, By, Bx);
+    ST_ORDER(1) ST_ORDER(2) ST_ORDER(3) ST_ORDER(4) ST_ORDER(5)
+    ST_ORDER(6) ST_ORDER(7) ST_ORDER(8) ST_ORDER(9) ST_ORDER(10)
+#undef ST_ORDER
 }
 
 /* the field sum with the factorial divisions as reciprocal steps, replayed
@@ -485,7 +535,7 @@ ST_DEV void sbend_exact_t(A &ar, St &s, double L, double href, double ih, double
     x[1] = nx1;
     x[0] = ar.div(pz1, rh) - ar.div(ar.div(derivative, rh), rh) - ih;
     double temp = ar.sqrt(c.opd2 - x3 * x3);
-    StRcp rt = st_rcp_prep(temp);
+    StRcp rt = ar.rcp(temp);
     double arcsin1 = ar.asin(ar.div(x1, rt));
     double arcsin2 = ar.asin(ar.div(nx1, rt));
     x[2] = x[2] + x3 * L + ar.div(x3 * (arcsin1 - arcsin2), rh);
@@ -555,10 +605,10 @@ ST_DEV void spin_rotate_t(A &ar, St &s, double xe, double pxe, double pye, doubl
     double *x = s.x;
     {
         double Gr = g.G * c.g1;
-        StRcp rt = st_rcp_prep(pze);
+        StRcp rt = ar.rcp(pze);
         double u0 = ar.div(pxe, rt), u1 = ar.div(pye, rt), u2 = 1.0;
-        double temp = ar.sqrt(u0 * u0 + u1 * u1 + u2 * u2);
-        StRcp rn = st_rcp_prep(temp);
+        double temp = ar.sqrt_sq(u0 * u0 + u1 * u1 + u2 * u2);
+        StRcp rn = ar.rcp(temp);
         u0 = ar.div(u0, rn);
         u1 = ar.div(u1, rn);
         u2 = ar.div(u2, rn);
@@ -573,8 +623,8 @@ ST_DEV void spin_rotate_t(A &ar, St &s, double xe, double pxe, double pye, doubl
         F1 = ar.div(F1 * temp, c.ropd);
         F2 = ar.div(F2 * temp, c.ropd);
         F1 = F1 - angle;
-        double Fabs = ar.sqrt(F0 * F0 + F1 * F1 + F2 * F2);
-        StRcp rf = st_rcp_prep(Fabs);
+        double Fabs = ar.sqrt_sq(F0 * F0 + F1 * F1 + F2 * F2);
+        StRcp rf = ar.rcp(Fabs);
         double a = ar.div(F0, rf), b = ar.div(F1, rf), d = ar.div(F2, rf);
         double sn, cs;
         ar.sincos(-Fabs, &sn, &cs);
