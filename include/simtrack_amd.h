@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define ST_ABI_VERSION 2
+#define ST_ABI_VERSION 3
 
 /* ------------------------------------------------------------ status */
 enum st_status {
@@ -90,7 +90,19 @@ enum st_element_type {
     ST_ROTAT = 28,     /* ROTAT      cpp:11784 (spin rotator; orbit: drift)   */
     ST_SNAKE = 29,     /* SNAKE      cpp:11864 (spin rotator; orbit: drift)   */
     ST_PATCH = 30,     /* PATCH      cpp:12262 dx,dy,ds = DX,DY,DS, theta = THETA, L = DS */
-    ST_SOLEN = 31      /* SOLEN      cpp:9645  ks = KS (KS = 0 only: a misaligned drift) */
+    ST_SOLEN = 31,     /* SOLEN      cpp:9645  ks = KS (KS = 0 only: a misaligned drift) */
+    ST_TRANS = 32,     /* TRANS      cpp:12056 dx,dy,ds = DX,DY,DS (not misalignments), L = DS */
+    ST_SROTAT = 33,    /* SROTAT     cpp:12121 psi = PSI   */
+    ST_YROTAT = 34,    /* YROTAT     cpp:12168 theta = THETA */
+    ST_XROTAT = 35,    /* XROTAT     cpp:12215 phi = PHI   */
+    ST_COOLING = 36,   /* COOLING    cpp:12009 alpha = ALPHA */
+    ST_LRBB = 37,      /* LRBB       cpp:11220 np, bbscale, sepx, sepy, sigmax, sigmay */
+    ST_ELENS = 38,     /* ELENS      cpp:11562 l, ne, bbscale, nslice, betae, sigmax, sigmay */
+    ST_HELENS = 39,    /* HELENS     cpp:11678 l, ne, bbscale, nslice, betae, rout, rin */
+    ST_HACDIP = 40,    /* HACDIP     cpp:10278 l, hkick = HKICKMAX, nud, phid, turns, turne */
+    ST_VACDIP = 41,    /* VACDIP     cpp:10374 l, vkick = VKICKMAX, nud, phid, turns, turne */
+    ST_HACMULT = 42,   /* HACMULT    cpp:10104 l, norder, klmax = KLMAX, tturns, phi0 */
+    ST_VACMULT = 43    /* VACMULT    cpp:10191 l, norder, klmax = KSLMAX, tturns, phi0 */
 };
 
 /*
@@ -128,6 +140,16 @@ typedef struct st_element {
     double m66[36];         /* MATRIX M66 (row major)                        */
     double xco_in[6], xco_out[6]; /* MATRIX XCO_IN, XCO_OUT                  */
     double ks;              /* SOLEN KS                                      */
+    /* ABI 3: the frame, cooling, e-lens, long-range beam-beam and AC elements */
+    double psi, phi, alpha; /* SROTAT PSI, XROTAT PHI, COOLING ALPHA         */
+    double sepx, sepy;      /* LRBB SEPX, SEPY                               */
+    double sigmax, sigmay;  /* LRBB, ELENS SIGMAX, SIGMAY                    */
+    double ne, betae;       /* ELENS, HELENS NE, BETAE                       */
+    double rout, rin;       /* HELENS ROUT, RIN                              */
+    double nud, phid;       /* HACDIP/VACDIP NUD, PHID                       */
+    double turns, turne;    /* HACDIP/VACDIP TURNS, TURNE (integers)         */
+    double klmax;           /* HACMULT KLMAX, VACMULT KSLMAX                 */
+    double tturns, phi0;    /* HACMULT/VACMULT TTURNS (integer), PHI0        */
 } st_element;
 
 /*

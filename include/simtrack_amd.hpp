@@ -26,8 +26,10 @@
  * Element classes: DRIFT, MARKER, BPM, HBPM, VBPM, SBEND, GSBEND, QUAD,
  * SKEWQ, SEXT, OCT, MULT, GMULT, SBENDMULT, SMULT, GSBENDMULT, KICKER,
  * HKICKER, VKICKER, RFCAV, CRABRF, CCMULT, LBT, ILBT, BEAMBEAM, CRBB,
- * MATRIX, ROTAT, SNAKE, PATCH, SOLEN, with the reference's constructor arguments, Nint and
- * Norder defaults (simtrack.cpp:8451, 8635, 8943-8950) and SetP/GetP names.
+ * MATRIX, ROTAT, SNAKE, PATCH, SOLEN, TRANS, SROTAT, YROTAT, XROTAT, COOLING,
+ * LRBB, ELENS, HELENS, HACDIP, VACDIP, HACMULT, VACMULT, with the reference's
+ * constructor arguments, Nint and Norder defaults (simtrack.cpp:8451, 8635,
+ * 8943-8950) and SetP/GetP names.
  *
  * Differences, all deliberate:
  *  - errors throw simtrack_amd::Error instead of printing and calling exit();
@@ -608,6 +610,213 @@ public:
     }
 };
 
+/* TRANS (simtrack.cpp:12056-12118): frame translation by the element's own
+   DX, DY, DS (not misalignments), L = DS */
+class TRANS : public Element {
+public:
+    TRANS(std::string name, double dx, double dy, double ds) : Element(std::move(name))
+    {
+        TYPE = "TRANS";
+        rec.type = ST_TRANS;
+        L = ds;
+        DX = dx;
+        DY = dy;
+        DS = ds;
+    }
+    void SetP(const char *name, double value) override
+    {
+        if (!std::strcmp(name, "DX")) DX = value;
+        else if (!std::strcmp(name, "DY")) DY = value;
+        else if (!std::strcmp(name, "DS")) DS = value;
+        else Element::SetP(name, value);
+    }
+    double GetP(const char *name) override
+    {
+        if (!std::strcmp(name, "DX")) return DX;
+        if (!std::strcmp(name, "DY")) return DY;
+        if (!std::strcmp(name, "DS")) return DS;
+        return Element::GetP(name);
+    }
+};
+
+/* SROTAT (cpp:12121), YROTAT (cpp:12168), XROTAT (cpp:12215): frame
+   rotations about s, y and x; COOLING (cpp:12009): 1 - ALPHA scaling */
+class SROTAT : public Element {
+public:
+    SROTAT(std::string name, double psi) : Element(std::move(name))
+    {
+        init("SROTAT", ST_SROTAT, 0.);
+        rec.psi = psi;
+        bind("PSI", rec.psi);
+    }
+};
+class YROTAT : public Element {
+public:
+    YROTAT(std::string name, double theta) : Element(std::move(name))
+    {
+        init("YROTAT", ST_YROTAT, 0.);
+        rec.theta = theta;
+        bind("THETA", rec.theta);
+    }
+};
+class XROTAT : public Element {
+public:
+    XROTAT(std::string name, double phi) : Element(std::move(name))
+    {
+        init("XROTAT", ST_XROTAT, 0.);
+        rec.phi = phi;
+        bind("PHI", rec.phi);
+    }
+};
+class COOLING : public Element {
+public:
+    COOLING(std::string name, double alpha) : Element(std::move(name))
+    {
+        init("COOLING", ST_COOLING, 0.);
+        rec.alpha = alpha;
+        bind("ALPHA", rec.alpha);
+    }
+};
+
+/* LRBB (simtrack.cpp:11220-11313): long-range 4-D beam-beam at a separation */
+class LRBB : public Element {
+public:
+    LRBB(std::string name, double np, double bbscale, double sepx, double sepy, double sigmax, double sigmay)
+        : Element(std::move(name))
+    {
+        init("LRBB", ST_LRBB, 0.);
+        rec.np = np;
+        rec.bbscale = bbscale;
+        rec.sepx = sepx;
+        rec.sepy = sepy;
+        rec.sigmax = sigmax;
+        rec.sigmay = sigmay;
+        bind("NP", rec.np);
+        bind("BBSCALE", rec.bbscale);
+        bind("SEPX", rec.sepx);
+        bind("SEPY", rec.sepy);
+        bind("SIGMAX", rec.sigmax);
+        bind("SIGMAY", rec.sigmay);
+    }
+};
+
+/* ELENS (simtrack.cpp:11562-11675) and HELENS (cpp:11678-11781): Gaussian
+   and hollow electron lenses, NSLICE kicks between half drifts */
+class ELENS : public Element {
+public:
+    ELENS(std::string name, double l, double ne, double bbscale, int nslice, double betae, double sigmax,
+          double sigmay)
+        : Element(std::move(name))
+    {
+        if (l < 0.) throw Error("Error: check signs of L ELENS ");
+        init("ELENS", ST_ELENS, l);
+        rec.ne = ne;
+        rec.bbscale = bbscale;
+        rec.nslice = nslice;
+        rec.betae = betae;
+        rec.sigmax = sigmax;
+        rec.sigmay = sigmay;
+        bind("NE", rec.ne);
+        bind("BBSCALE", rec.bbscale);
+        bind_int("NSLICE", rec.nslice);
+        bind("BETAE", rec.betae);
+        bind("SIGMAX", rec.sigmax);
+        bind("SIGMAY", rec.sigmay);
+    }
+};
+class HELENS : public Element {
+public:
+    HELENS(std::string name, double l, double ne, double bbscale, int nslice, double betae, double rout, double rin)
+        : Element(std::move(name))
+    {
+        init("HELENS", ST_HELENS, l);
+        rec.ne = ne;
+        rec.bbscale = bbscale;
+        rec.nslice = nslice;
+        rec.betae = betae;
+        rec.rout = rout;
+        rec.rin = rin;
+        bind("NE", rec.ne);
+        bind("BBSCALE", rec.bbscale);
+        bind_int("NSLICE", rec.nslice);
+        bind("BETAE", rec.betae);
+        bind("ROUT", rec.rout);
+        bind("RIN", rec.rin);
+    }
+};
+
+/* HACDIP (cpp:10278-10371) / VACDIP (cpp:10374-10467): AC dipoles, kick
+   KICKMAX sin(2 pi NUD (turn - TURNS) + PHID), ramped from TURNS to TURNE */
+class HACDIP : public Element {
+public:
+    HACDIP(std::string name, double l, double hkickmax, double nud, double phid, int turns, int turne)
+        : Element(std::move(name))
+    {
+        init("HACDIP", ST_HACDIP, l);
+        rec.hkick = hkickmax;
+        bind("HKICKMAX", rec.hkick);
+        set(nud, phid, turns, turne);
+    }
+
+protected:
+    HACDIP(std::string name, const char *type, int code, double l) : Element(std::move(name)) { init(type, code, l); }
+    void set(double nud, double phid, int turns, int turne)
+    {
+        rec.nud = nud;
+        rec.phid = phid;
+        rec.turns = turns;
+        rec.turne = turne;
+        bind("NUD", rec.nud);
+        bind("PHID", rec.phid);
+        bind("TURNS", rec.turns);
+        bind("TURNE", rec.turne);
+    }
+};
+class VACDIP : public HACDIP {
+public:
+    VACDIP(std::string name, double l, double vkickmax, double nud, double phid, int turns, int turne)
+        : HACDIP(std::move(name), "VACDIP", ST_VACDIP, l)
+    {
+        rec.vkick = vkickmax;
+        bind("VKICKMAX", rec.vkick);
+        set(nud, phid, turns, turne);
+    }
+};
+
+/* HACMULT (cpp:10104-10187) / VACMULT (cpp:10191-10274): one multipole
+   order modulated by sin(2 pi turn / TTURNS + PHI0) */
+class HACMULT : public Element {
+public:
+    HACMULT(std::string name, double l, int norder, double klmax, int tturns, double phi0) : Element(std::move(name))
+    {
+        init("HACMULT", ST_HACMULT, l);
+        bind("KLMAX", rec.klmax);
+        set(norder, klmax, tturns, phi0);
+    }
+
+protected:
+    HACMULT(std::string name, const char *type, int code, double l) : Element(std::move(name)) { init(type, code, l); }
+    void set(int norder, double k, int tturns, double phi0)
+    {
+        rec.norder = norder;
+        rec.klmax = k;
+        rec.tturns = tturns;
+        rec.phi0 = phi0;
+        bind_int("Norder", rec.norder);
+        bind("TTURNS", rec.tturns);
+        bind("PHI0", rec.phi0);
+    }
+};
+class VACMULT : public HACMULT {
+public:
+    VACMULT(std::string name, double l, int norder, double kslmax, int tturns, double phi0)
+        : HACMULT(std::move(name), "VACMULT", ST_VACMULT, l)
+    {
+        bind("KSLMAX", rec.klmax);
+        set(norder, kslmax, tturns, phi0);
+    }
+};
+
 /* ----------------------------------------------------------------- line */
 
 /* Line (simtrack.h:1067-1100): owns its elements, as the reference's does
@@ -749,7 +958,7 @@ inline void Read_MADX_Lattice(Line &line, const char *filename)
         else if (type == "BEAMBEAM") {
             if (l != 0.) throw Error("Beam-Beam element should have zero length!");
             e = new BEAMBEAM(name, 6, 0., 1, 0.4545, 11, 2.5e-06, 2.5e-06, 0.53, 0.0, 0.53, 0.0);
-        } else if (type == "ELENS") throw Error(name + ": ELENS is not an element of this engine");
+        } else if (type == "ELENS") e = new ELENS(name, l, 0.0, 1, 0.14, 4, .31e-03, .31e-03);
         else e = new DRIFT(name, l);
         line.Cell.push_back(e);
     }
@@ -818,7 +1027,7 @@ inline void Read_BMAD_Lattice(Line &line, const char *filename)
         else if (type == "BEAMBEAM") {
             if (l != 0.) throw Error("Beam-Beam element should have zero length!");
             e = new BEAMBEAM(name, 6, 0., 1, 0.4545, 11, 2.5e-06, 2.5e-06, 0.53, 0.0, 0.53, 0.0);
-        } else if (type == "ELENS") throw Error(name + ": ELENS is not an element of this engine");
+        } else if (type == "ELENS") e = new ELENS(name, l, 0.0, 1, 0.14, 4, .31e-03, .31e-03);
         else if (type == "Patch") e = new PATCH(name, xoff, 0.0, zoff, -xpitch);
         else e = new DRIFT(name, l);
         line.Cell.push_back(e);
@@ -835,7 +1044,9 @@ inline int type_code(const std::string &t)
     static const char *names[] = {"", "DRIFT", "SBEND", "GSBEND", "QUAD", "SKEWQ", "SEXT", "OCT", "MULT", "GMULT",
                                   "SBENDMULT", "SMULT", "GSBENDMULT", "KICKER", "HKICKER", "VKICKER", "MARKER",
                                   "BPM", "HBPM", "VBPM", "RFCAV", "CRABRF", "CCMULT", "LBT", "ILBT", "BEAMBEAM",
-                                  "CRBB", "MATRIX", "ROTAT", "SNAKE", "PATCH", "SOLEN"};
+                                  "CRBB", "MATRIX", "ROTAT", "SNAKE", "PATCH", "SOLEN", "TRANS", "SROTAT",
+                                  "YROTAT", "XROTAT", "COOLING", "LRBB", "ELENS", "HELENS", "HACDIP", "VACDIP",
+                                  "HACMULT", "VACMULT"};
     for (int i = 1; i < (int)(sizeof(names) / sizeof(names[0])); ++i)
         if (t == names[i]) return i;
     return -1;
@@ -975,6 +1186,58 @@ st_element record_of(E &e)
         case ST_SOLEN:
             r.ks = P("KS");
             r.nint = 1;
+            break;
+        case ST_TRANS:
+            break;                  /* DX, DY, DS are the public fields read above */
+        case ST_SROTAT:
+            r.psi = P("PSI");
+            break;
+        case ST_YROTAT:
+            r.theta = P("THETA");
+            break;
+        case ST_XROTAT:
+            r.phi = P("PHI");
+            break;
+        case ST_COOLING:
+            r.alpha = P("ALPHA");
+            break;
+        case ST_LRBB:
+            r.np = P("NP");
+            r.bbscale = P("BBSCALE");
+            r.sepx = P("SEPX");
+            r.sepy = P("SEPY");
+            r.sigmax = P("SIGMAX");
+            r.sigmay = P("SIGMAY");
+            break;
+        case ST_ELENS:
+        case ST_HELENS:
+            r.ne = P("NE");
+            r.bbscale = P("BBSCALE");
+            r.nslice = (int)P("NSLICE");
+            r.betae = P("BETAE");
+            if (code == ST_ELENS) {
+                r.sigmax = P("SIGMAX");
+                r.sigmay = P("SIGMAY");
+            } else {
+                r.rout = P("ROUT");
+                r.rin = P("RIN");
+            }
+            break;
+        case ST_HACDIP:
+        case ST_VACDIP:
+            if (code == ST_HACDIP) r.hkick = P("HKICKMAX");
+            else r.vkick = P("VKICKMAX");
+            r.nud = P("NUD");
+            r.phid = P("PHID");
+            r.turns = P("TURNS");
+            r.turne = P("TURNE");
+            break;
+        case ST_HACMULT:
+        case ST_VACMULT:
+            r.norder = (int)P("Norder");
+            r.klmax = P(code == ST_HACMULT ? "KLMAX" : "KSLMAX");
+            r.tturns = P("TTURNS");
+            r.phi0 = P("PHI0");
             break;
         case ST_MATRIX:
             throw Error("element " + e.NAME + ": the reference's MATRIX exposes no parameters to read");

@@ -133,11 +133,12 @@ struct dual_element {
     int32_t type, nint, norder, nslice, treatment, reserved[3];
     Dual l, angle, e1, e2, k0l, knl[11], knsl[11], dx, dy, ds, dtheta, dphi, dpsi, apx, apy, vrf, frf, phase0,
         hkick, vkick, theta, np, bbscale, sigmal, emitx, betax, alfax, emity, betay, alfay, cc[8], rot_n[3],
-        rot_angle, m66[36], xco_in[6], xco_out[6], ks;
+        rot_angle, m66[36], xco_in[6], xco_out[6], ks, psi, phi, alpha, sepx, sepy, sigmax, sigmay, ne, betae,
+        rout, rin, nud, phid, turns, turne, klmax, tturns, phi0;
 };
-static const int DUAL_NDOUBLE = (int)((offsetof(st_element, ks) - offsetof(st_element, l)) / sizeof(double)) + 1;
-static_assert((offsetof(dual_element, ks) - offsetof(dual_element, l)) / sizeof(Dual) + 1 ==
-                  (offsetof(st_element, ks) - offsetof(st_element, l)) / sizeof(double) + 1,
+static const int DUAL_NDOUBLE = (int)((offsetof(st_element, phi0) - offsetof(st_element, l)) / sizeof(double)) + 1;
+static_assert((offsetof(dual_element, phi0) - offsetof(dual_element, l)) / sizeof(Dual) + 1 ==
+                  (offsetof(st_element, phi0) - offsetof(st_element, l)) / sizeof(double) + 1,
               "dual_element must mirror st_element");
 
 static void to_dual(const st_element *e, dual_element *o)
@@ -148,6 +149,7 @@ static void to_dual(const st_element *e, dual_element *o)
     for (int i = 0; i < DUAL_NDOUBLE; i++) dst[i] = Dual(src[i]);
 }
 
+#define ORC_VALUE(f) ((f).v)
 #define ORC_SCALAR_BUILD 1   /* st_oracle.c: leave out the exports that take plain double pointers */
 #define st_element dual_element
 #define double Dual

@@ -80,6 +80,10 @@
 #define ORC_PT_CACHED(pt, v) (v)
 #define ORC_PT_DERIVED(v) (v)
 #endif
+/* the plain value of an element field (the counting builds carry more) */
+#ifndef ORC_VALUE
+#define ORC_VALUE(v) (v)
+#endif
 
 enum { x_ = 0, px_ = 1, y_ = 2, py_ = 3, z_ = 4, pt_ = 5 };
 
@@ -368,13 +372,9 @@ static void DRIFT_Pass(double x[], double L)
     }
 }
 
-/* PATCH_Pass, simtrack.cpp:6248-6256: TRANS_Pass (cpp:6140-6151), then
-   YROTAT_Pass (cpp:6192-6219, the frame y rotation of Forest's book) */
-static void PATCH_Pass(double x[], double DX, double DY, double DS, double THETA)
+/* YROTAT_Pass, simtrack.cpp:6192-6218: the frame y rotation of Forest's book */
+static void YROTAT_Pass(double x[], double THETA)
 {
-    x[0] = x[0] - DX;
-    x[2] = x[2] - DY;
-    x[4] = x[4] - DS;
     if (fabs(THETA) < 1.0e-9) return;
     double xtemp[6], temp, pz;
     double delta1 = ORC_PT_CACHED(x[pt_], sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0);
@@ -390,6 +390,144 @@ static void PATCH_Pass(double x[], double DX, double DY, double DS, double THETA
     x[3] = xtemp[3];
     x[4] = xtemp[4] - (1 + delta1) * xtemp[0] * tanT / pz / temp / beta1;
     x[5] = xtemp[5];
+}
+
+/* TRANS_Pass, simtrack.cpp:6140-6151 (its DRIFT_Pass by DS is commented out) */
+static void TRANS_Pass(double x[], double DX, double DY, double DS)
+{
+    x[0] = x[0] - DX;
+    x[2] = x[2] - DY;
+    x[4] = x[4] - DS;
+}
+
+/* PATCH_Pass, simtrack.cpp:6248-6256: TRANS_Pass, then YROTAT_Pass */
+static void PATCH_Pass(double x[], double DX, double DY, double DS, double THETA)
+{
+    TRANS_Pass(x, DX, DY, DS);
+    YROTAT_Pass(x, THETA);
+}
+
+/* SROTAT_Pass, simtrack.cpp:6153-6170: frame rotation about s */
+static void SROTAT_Pass(double x[], double PSI)
+{
+    if (PSI == 0.) return;
+    double xtemp[6], cosT = OCOS(PSI), sinT = OSIN(PSI);
+    for (int i = 0; i < 6; i++) xtemp[i] = x[i];
+    x[0] = cosT * xtemp[0] + sinT * xtemp[2];
+    x[1] = cosT * xtemp[1] + sinT * xtemp[3];
+    x[2] = cosT * xtemp[2] - sinT * xtemp[0];
+    x[3] = cosT * xtemp[3] - sinT * xtemp[1];
+}
+
+/* XROTAT_Pass, simtrack.cpp:6220-6245: frame rotation about x (Forest) */
+static void XROTAT_Pass(double x[], double PHI)
+{
+    if (fabs(PHI) < 1.0e-9) return;
+    double xtemp[6], temp, pz;
+    double delta1 = ORC_PT_CACHED(x[pt_], sqrt(1.0 + 2 * x[pt_] / GP.beta + x[pt_] * x[pt_]) - 1.0);
+    double gamma1 = ORC_PT_CACHED(x[pt_], GP.gamma + sqrt(GP.gamma * GP.gamma - 1.) * x[pt_]);
+    double beta1 = ORC_PT_DERIVED(sqrt(1.0 - 1.0 / gamma1 / gamma1));
+    double cosT = OCOS(PHI), sinT = OSIN(PHI), tanT = OTAN(PHI);
+    for (int i = 0; i < 6; i++) xtemp[i] = x[i];
+    pz = sqrt((1 + delta1) * (1 + delta1) - xtemp[1] * xtemp[1] - xtemp[3] * xtemp[3]);
+    temp = 1 - xtemp[3] * tanT / pz;
+    x[2] = xtemp[2] / cosT / temp;
+    x[3] = xtemp[3] * cosT + pz * sinT;
+    x[0] = xtemp[0] + xtemp[1] * xtemp[2] * tanT / pz / temp;
+    x[1] = xtemp[1];
+    x[4] = xtemp[4] - (1 + delta1) * xtemp[2] * tanT / pz / temp / beta1;
+    x[5] = xtemp[5];
+}
+
+/* COOLING_Pass, simtrack.cpp:6132-6136 */
+static void COOLING_Pass(double x[], double ALPHA)
+{
+    for (int i = 0; i < 4; i++) x[i] = (1. - ALPHA) * x[i];
+}
+
+/* The turn being tracked: GP.turn of the reference, which Track runs from 0
+   (simtrack.cpp:20799) and the AC elements read. */
+static __thread int orc_turn = 0;   /* per thread: particles are tracked in parallel */
+
+/* ACDIP_Pass, simtrack.cpp:5930-5950 */
+static void ACDIP_Pass(double x[], double L, double HKICKMAX, double VKICKMAX, double NUD, double TURNS,
+                       double TURNE, double PHID)
+{
+    if (orc_turn < TURNS) {
+        DRIFT_Pass(x, L);
+    } else if (orc_turn >= TURNS && orc_turn <= TURNE) {
+        DRIFT_Pass(x, L / 2.);
+        x[1] += ((orc_turn - TURNS) * 1.0 * HKICKMAX / (TURNE - TURNS)) *
+                OSIN(2.0 * 3.14159265 * NUD * (orc_turn - TURNS) + PHID);
+        x[3] += ((orc_turn - TURNS) * 1.0 * VKICKMAX / (TURNE - TURNS)) *
+                OSIN(2.0 * 3.14159265 * NUD * (orc_turn - TURNS) + PHID);
+        DRIFT_Pass(x, L / 2.);
+    } else {
+        DRIFT_Pass(x, L / 2.);
+        x[1] += HKICKMAX * OSIN(2.0 * 3.14159265 * NUD * (orc_turn - TURNS) + PHID);
+        x[3] += VKICKMAX * OSIN(2.0 * 3.14159265 * NUD * (orc_turn - TURNS) + PHID);
+        DRIFT_Pass(x, L / 2.);
+    }
+}
+
+/* ACMULT_Pass, simtrack.cpp:5895-5928 (a single order, int factorial) */
+static void ACMULT_Pass(double x[], double L, int Norder, double KL, double KSL, int TTURNS, double PHI0)
+{
+    int fac = 1;
+    double Xn, Yn, Xn0, Yn0, KNL, KNSL, By, Bx;
+    DRIFT_Pass(x, L / 2.0);
+    if (Norder == 0) {
+        By = KL;
+        Bx = KSL;
+    } else {
+        KNL = KL;
+        KNSL = KSL;
+        By = 0.;
+        Bx = 0.;
+        Xn = 1.;
+        Yn = 0.;
+        for (int i = 1; i < Norder + 1; i++) {
+            Xn0 = Xn;
+            Yn0 = Yn;
+            Xn = Xn0 * x[x_] - Yn0 * x[y_];
+            Yn = Xn0 * x[y_] + Yn0 * x[x_];
+            fac = fac * i;
+        }
+        By = By + (KNL * Xn - KNSL * Yn) / fac;
+        Bx = Bx + (KNL * Yn + KNSL * Xn) / fac;
+    }
+    x[px_] = x[px_] - By * OSIN(2.0 * PI * orc_turn / TTURNS + PHI0);
+    x[py_] = x[py_] + Bx * OSIN(2.0 * PI * orc_turn / TTURNS + PHI0);
+    DRIFT_Pass(x, L / 2.0);
+}
+
+/* HELENS_Pass, simtrack.cpp:7392-7431 (speed_light = 299792458., cpp:24);
+   for a NaN radius the reference leaves `scale` unset, here NaN */
+static void HELENS_Pass(double x[], double Le, double Ne, double bbscale, double beta_e, int N_slice, double rout,
+                        double rin)
+{
+    double Ne_slice = Ne * 1.0 / N_slice;
+    double Le_slice = Le * 1.0 / N_slice;
+    double RINNER = rin, ROUTER = rout, scale, I_e, Theta_max, r;
+    (void)Ne_slice;
+    (void)bbscale;
+    I_e = 1.6021765e-19 * Ne * beta_e * 299792458. / Le;
+    Theta_max = -2e-7 * Le_slice * I_e * (1 + beta_e * GP.beta) / beta_e / GP.beta / ROUTER / GP.brho;
+    for (int i = 0; i < N_slice; i++) {
+        DRIFT_Pass(x, Le_slice / 2.);
+        r = sqrt(x[0] * x[0] + x[2] * x[2]);
+        if (r <= RINNER)
+            scale = 0;
+        else if (r > RINNER && r <= ROUTER)
+            scale = (r * r - RINNER * RINNER) * ROUTER / r / (ROUTER * ROUTER - RINNER * RINNER);
+        else if (r > ROUTER)
+            scale = ROUTER / r;
+        else
+            scale = r;
+        x[1] = x[1] + scale * Theta_max * x[0] / r;
+        x[3] = x[3] + scale * Theta_max * x[2] / r;
+        DRIFT_Pass(x, Le_slice / 2.);
+    }
 }
 
 /* bend_kick_pass, simtrack.cpp:3790-3794 */
@@ -1536,6 +1674,39 @@ static void BEAMBEAM_Pass(double x[], int TREATMENT, double NP, double BBSCALE, 
     }
 }
 
+/* LRBB_Pass, simtrack.cpp:7358-7363 (the 6-argument BB4D, cpp:6384-6447) */
+static void LRBB_Pass(double x[], double NP, double BBSCALE, double SEPX, double SEPY, double SIGMAX, double SIGMAY)
+{
+    double Dpx, Dpy;
+    x[0] = x[0] + SEPX;
+    x[2] = x[2] + SEPY;
+    orc_bb4d(x[0], x[2], GP.gamma, NP, SIGMAX, SIGMAY, &Dpx, &Dpy);
+    x[1] = x[1] + Dpx * BBSCALE;
+    x[3] = x[3] + Dpy * BBSCALE;
+    x[0] = x[0] - SEPX;
+    x[2] = x[2] - SEPY;
+}
+
+/* ELENS_Pass, simtrack.cpp:7370-7390 */
+static void ELENS_Pass(double x[], double Le, double Ne, double bbscale, double beta_e, int N_slice, double sigmax,
+                       double sigmay)
+{
+    double Le_slice = Le * 1.0 / N_slice;
+    double Ne_slice = Ne * 1.0 / N_slice;
+    double Dpx, Dpy;
+    if (Ne != 0.) {
+        for (int i = 0; i < N_slice; i++) {
+            DRIFT_Pass(x, Le_slice / 2.);
+            orc_bb4d(x[0], x[2], GP.gamma, Ne_slice, sigmax, sigmay, &Dpx, &Dpy);
+            x[1] = x[1] - Dpx * (1. + beta_e) * bbscale;
+            x[3] = x[3] - Dpy * (1. + beta_e) * bbscale;
+            DRIFT_Pass(x, Le_slice / 2.);
+        }
+    } else {
+        DRIFT_Pass(x, Le);
+    }
+}
+
 /* ------------------------------------------------------ spin passes */
 
 /* SBEND_sPass, simtrack.cpp:7486-7536 */
@@ -1831,6 +2002,73 @@ static void KICK_sPass(double x[], double L, double HKICK, double VKICK)
     DRIFT_Pass(x, L / 2.0);
 }
 
+/* ACMULT_sPass, simtrack.cpp:8210-8249 */
+static void ACMULT_sPass(double x[], double L, int Norder, double KL, double KSL, int TTURNS, double PHI0)
+{
+    int fac = 1;
+    double Xn, Yn, Xn0, Yn0, KNL, KNSL, By, Bx, BLbrho[3];
+    DRIFT_Pass(x, L / 2.0);
+    if (Norder == 0) {
+        By = KL;
+        Bx = KSL;
+    } else {
+        KNL = KL;
+        KNSL = KSL;
+        By = 0.;
+        Bx = 0.;
+        Xn = 1.;
+        Yn = 0.;
+        for (int i = 1; i < Norder + 1; i++) {
+            Xn0 = Xn;
+            Yn0 = Yn;
+            Xn = Xn0 * x[x_] - Yn0 * x[y_];
+            Yn = Xn0 * x[y_] + Yn0 * x[x_];
+            fac = fac * i;
+        }
+        By = By + (KNL * Xn - KNSL * Yn) / fac;
+        Bx = Bx + (KNL * Yn + KNSL * Xn) / fac;
+    }
+    BLbrho[0] = Bx * OSIN(2.0 * PI * orc_turn / TTURNS + PHI0);
+    BLbrho[1] = By * OSIN(2.0 * PI * orc_turn / TTURNS + PHI0);
+    BLbrho[2] = 0.;
+    spin(x, 0, 0, BLbrho);
+    x[px_] = x[px_] - By * OSIN(2.0 * PI * orc_turn / TTURNS + PHI0);
+    x[py_] = x[py_] + Bx * OSIN(2.0 * PI * orc_turn / TTURNS + PHI0);
+    DRIFT_Pass(x, L / 2.0);
+}
+
+/* ACDIP_sPass, simtrack.cpp:8251-8284 */
+static void ACDIP_sPass(double x[], double L, double HKICKMAX, double VKICKMAX, double NUD, double TURNS,
+                        double TURNE, double PHID)
+{
+    double BLbrho[3];
+    if (orc_turn < TURNS) {
+        DRIFT_Pass(x, L);
+    } else if (orc_turn >= TURNS && orc_turn <= TURNE) {
+        DRIFT_Pass(x, L / 2.);
+        BLbrho[0] = ((orc_turn - TURNS) * 1.0 * VKICKMAX / (TURNE - TURNS)) *
+                    OSIN(2.0 * 3.14159265 * NUD * (orc_turn - TURNS) + PHID);
+        BLbrho[1] = -((orc_turn - TURNS) * 1.0 * HKICKMAX / (TURNE - TURNS)) *
+                    OSIN(2.0 * 3.14159265 * NUD * (orc_turn - TURNS) + PHID);
+        BLbrho[2] = 0.;
+        spin(x, 0, 0, BLbrho);
+        x[1] += ((orc_turn - TURNS) * 1.0 * HKICKMAX / (TURNE - TURNS)) *
+                OSIN(2.0 * 3.14159265 * NUD * (orc_turn - TURNS) + PHID);
+        x[3] += ((orc_turn - TURNS) * 1.0 * VKICKMAX / (TURNE - TURNS)) *
+                OSIN(2.0 * 3.14159265 * NUD * (orc_turn - TURNS) + PHID);
+        DRIFT_Pass(x, L / 2.);
+    } else {
+        DRIFT_Pass(x, L / 2.);
+        BLbrho[0] = VKICKMAX * OSIN(2.0 * 3.14159265 * NUD * (orc_turn - TURNS) + PHID);
+        BLbrho[1] = -HKICKMAX * OSIN(2.0 * 3.14159265 * NUD * (orc_turn - TURNS) + PHID);
+        BLbrho[2] = 0;
+        spin(x, 0, 0, BLbrho);
+        x[1] += HKICKMAX * OSIN(2.0 * 3.14159265 * NUD * (orc_turn - TURNS) + PHID);
+        x[3] += VKICKMAX * OSIN(2.0 * 3.14159265 * NUD * (orc_turn - TURNS) + PHID);
+        DRIFT_Pass(x, L / 2.);
+    }
+}
+
 /* ROTAT_sPass / SNAKE_sPass, simtrack.cpp:8286-8348 (identical bodies) */
 static void ROTAT_sPass(double x[], double L, const double n[3], double angle)
 {
@@ -1973,6 +2211,56 @@ static int elem_pass(const st_element *e, double x[], orc_rng *rng)
         DRIFT_Pass(x, e->l);
         LtoG(x, e->l, 0., MIS(e));
         return 0;
+    case ST_TRANS:                          /* TRANS::Pass, cpp:12108-12110 */
+        TRANS_Pass(x, e->dx, e->dy, e->ds);
+        return 0;
+    case ST_YROTAT:                         /* YROTAT::Pass, cpp:12202-12204 */
+        YROTAT_Pass(x, e->theta);
+        return 0;
+    case ST_SROTAT:                         /* SROTAT::Pass, cpp:12155-12157 */
+        SROTAT_Pass(x, e->psi);
+        return 0;
+    case ST_XROTAT:                         /* XROTAT::Pass, cpp:12249-12251 */
+        XROTAT_Pass(x, e->phi);
+        return 0;
+    case ST_COOLING:                        /* COOLING::Pass, cpp:12043-12045 */
+        COOLING_Pass(x, e->alpha);
+        return 0;
+    case ST_LRBB:                           /* LRBB::Pass, cpp:11299-11303 */
+        GtoL(x, e->l, 0., MIS(e));
+        LRBB_Pass(x, e->np, e->bbscale, e->sepx, e->sepy, e->sigmax, e->sigmay);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_ELENS:                          /* ELENS::Pass, cpp:11649-11653 */
+        if (e->nslice < 1) return -1;
+        GtoL(x, e->l, 0., MIS(e));
+        ELENS_Pass(x, e->l, e->ne, e->bbscale, e->betae, e->nslice, e->sigmax, e->sigmay);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_HELENS:                         /* HELENS::Pass, cpp:11765-11769 */
+        if (e->nslice < 1) return -1;
+        GtoL(x, e->l, 0., MIS(e));
+        HELENS_Pass(x, e->l, e->ne, e->bbscale, e->betae, e->nslice, e->rout, e->rin);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    case ST_HACDIP: case ST_VACDIP: {       /* HACDIP::Pass cpp:10357-10361, VACDIP::Pass cpp:10453-10457 */
+        double hk = 0., vk = 0.;
+        if (e->type == ST_HACDIP) hk = e->hkick;
+        if (e->type == ST_VACDIP) vk = e->vkick;
+        GtoL(x, e->l, 0., MIS(e));
+        ACDIP_Pass(x, e->l, hk, vk, e->nud, e->turns, e->turne, e->phid);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    }
+    case ST_HACMULT: case ST_VACMULT: {     /* HACMULT::Pass cpp:10173-10177, VACMULT::Pass cpp:10260-10264 */
+        double kl = 0., ksl = 0.;
+        if (e->type == ST_HACMULT) kl = e->klmax;
+        else ksl = e->klmax;
+        GtoL(x, e->l, 0., MIS(e));
+        ACMULT_Pass(x, e->l, e->norder, kl, ksl, (int)ORC_VALUE(e->tturns), e->phi0);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    }
     default:
         return -1;
     }
@@ -2048,6 +2336,27 @@ static int elem_spass(const st_element *e, double x[])
     case ST_PATCH:                          /* PATCH::sPass, cpp:12331-12333: the orbit map */
     case ST_SOLEN:                          /* SOLEN::sPass, cpp:9700-9704; KS = 0: DRIFT_Pass, cpp:8125-8128 */
         return elem_pass(e, x, NULL);
+    case ST_TRANS: case ST_YROTAT: case ST_SROTAT: case ST_XROTAT: case ST_COOLING:
+    case ST_LRBB: case ST_ELENS: case ST_HELENS:
+        return elem_pass(e, x, NULL);       /* sPass == Pass (cpp:12116, 12210, 12163, 12257, 12051, 11309, 11671, 11777) */
+    case ST_HACDIP: case ST_VACDIP: {       /* HACDIP::sPass cpp:10367-10371, VACDIP::sPass cpp:10463-10467 */
+        double hk = 0., vk = 0.;
+        if (e->type == ST_HACDIP) hk = e->hkick;
+        if (e->type == ST_VACDIP) vk = e->vkick;
+        GtoL(x, e->l, 0., MIS(e));
+        ACDIP_sPass(x, e->l, hk, vk, e->nud, e->turns, e->turne, e->phid);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    }
+    case ST_HACMULT: case ST_VACMULT: {     /* HACMULT::sPass cpp:10183-10187, VACMULT::sPass cpp:10270-10274 */
+        double kl = 0., ksl = 0.;
+        if (e->type == ST_HACMULT) kl = e->klmax;
+        else ksl = e->klmax;
+        GtoL(x, e->l, 0., MIS(e));
+        ACMULT_sPass(x, e->l, e->norder, kl, ksl, (int)ORC_VALUE(e->tturns), e->phi0);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
+    }
     default:
         return -1;
     }
@@ -2058,6 +2367,7 @@ static int elem_spass(const st_element *e, double x[])
 EXPORT int orc_element_pass(const st_element *e, double *x, int spin_on, uint32_t pid)
 {
     orc_rng rng = {GP.rng_seed, pid, 0u};
+    orc_turn = 0;
     return spin_on ? elem_spass(e, x) : elem_pass(e, x, &rng);
 }
 
@@ -2087,6 +2397,7 @@ static int track_one(const st_element *line, int n, double *x, int nturn, int *s
         return 0;
     }
     for (int turn = 0; turn < nturn; turn++) {
+        orc_turn = turn;
         rng.counter = (uint64_t)turn * draws_per_turn;
         if (x_tbt && bpm_index < 0)
             for (int k = 0; k < ncol; k++) x_tbt[(size_t)turn * ncol + k] = x[k];

@@ -29,6 +29,11 @@ ELEMENT_DTYPE = np.dtype([
     ("m66", "<f8", (36,)),
     ("xco_in", "<f8", (6,)), ("xco_out", "<f8", (6,)),
     ("ks", "<f8"),
+    ("psi", "<f8"), ("phi", "<f8"), ("alpha", "<f8"),
+    ("sepx", "<f8"), ("sepy", "<f8"), ("sigmax", "<f8"), ("sigmay", "<f8"),
+    ("ne", "<f8"), ("betae", "<f8"), ("rout", "<f8"), ("rin", "<f8"),
+    ("nud", "<f8"), ("phid", "<f8"), ("turns", "<f8"), ("turne", "<f8"),
+    ("klmax", "<f8"), ("tturns", "<f8"), ("phi0", "<f8"),
 ])
 
 
@@ -53,6 +58,8 @@ TYPES = {
     "KICKER": 13, "HKICKER": 14, "VKICKER": 15, "MARKER": 16, "BPM": 17, "HBPM": 18,
     "VBPM": 19, "RFCAV": 20, "CRABRF": 21, "CCMULT": 22, "LBT": 23, "ILBT": 24,
     "BEAMBEAM": 25, "CRBB": 26, "MATRIX": 27, "ROTAT": 28, "SNAKE": 29, "PATCH": 30, "SOLEN": 31,
+    "TRANS": 32, "SROTAT": 33, "YROTAT": 34, "XROTAT": 35, "COOLING": 36, "LRBB": 37, "ELENS": 38,
+    "HELENS": 39, "HACDIP": 40, "VACDIP": 41, "HACMULT": 42, "VACMULT": 43,
 }
 TRACK_SPIN = 1
 TRACK_RADIAL = 2

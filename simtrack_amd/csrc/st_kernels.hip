@@ -470,7 +470,7 @@ ST_DEV void ccmult_kick(St &s, st_pp P, st_gref g)
  */
 template <bool SPIN, bool HEXP, bool RAD>
 ST_DEV void apply_op(St &s, Ptc &c, const st_op &op, st_pp par, st_gref g,
-                     uint32_t pid, uint64_t &rctr)
+                     uint32_t pid, uint64_t &rctr, int turn)
 {
     st_pp P = st_launder(par + op.po);
     if (op.pm >= 0) {
@@ -567,11 +567,35 @@ ST_DEV void apply_op(St &s, Ptc &c, const st_op &op, st_pp par, st_gref g,
         ilbt(s, P);
         ptc_update(c, x[5], g);
         break;
-    case K_BB4D: { /* BEAMBEAM_Pass TREATMENT != 6, cpp:7350-7353 */
-        double Dpx, Dpy;
-        bb4d(x[0], x[2], P[0], P[1], P[2], Dpx, Dpy);
-        x[1] = x[1] + Dpx * P[3];
-        x[3] = x[3] + Dpy * P[3];
+    case K_BB4D:  /* BEAMBEAM_Pass TREATMENT != 6, cpp:7350-7353 */
+    case K_LRBB:  /* LRBB_Pass, cpp:7358-7363, with the 6-argument BB4D (cpp:6384-6447) */
+    case K_ELENS: /* ELENS_Pass, cpp:7370-7390 (Ne != 0) */
+    {
+        /* one inlined BB4D (with its complex error function) for the three */
+        const bool lr = op.kind == K_LRBB, el = op.kind == K_ELENS;
+        const int o = op.kind == K_BB4D ? 0 : 2;
+        const int n = el ? op.nint : 1;
+        for (int i = 0; i < n; i++) {
+            if (el) drift<HEXP>(s, P[0], P[1], c);
+            if (lr) {
+                x[0] = x[0] + P[0];
+                x[2] = x[2] + P[1];
+            }
+            double Dpx, Dpy;
+            bb4d(x[0], x[2], P[o], P[o + 1], P[o + 2], Dpx, Dpy);
+            if (el) {
+                x[1] = x[1] - Dpx * P[5] * P[6];
+                x[3] = x[3] - Dpy * P[5] * P[6];
+                drift<HEXP>(s, P[0], P[1], c);
+            } else {
+                x[1] = x[1] + Dpx * P[o + 3];
+                x[3] = x[3] + Dpy * P[o + 3];
+            }
+            if (lr) {
+                x[0] = x[0] - P[0];
+                x[2] = x[2] - P[1];
+            }
+        }
         break;
     }
     case K_BB6D:
@@ -622,6 +646,102 @@ ST_DEV void apply_op(St &s, Ptc &c, const st_op &op, st_pp par, st_gref g,
             x[4] = x4 - c.opd * x0 * tanT / pz / temp / c.b1;
         }
         break;
+    case K_SROTAT: { /* SROTAT_Pass, cpp:6153-6170 */
+        const double cosT = P[0], sinT = P[1];
+        const double x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3];
+        x[0] = cosT * x0 + sinT * x2;
+        x[1] = cosT * x1 + sinT * x3;
+        x[2] = cosT * x2 - sinT * x0;
+        x[3] = cosT * x3 - sinT * x1;
+        break;
+    }
+    case K_XROTAT: { /* XROTAT_Pass, cpp:6220-6245 (delta1, beta1 from the cache) */
+        const double cosT = P[0], sinT = P[1], tanT = P[2];
+        const double x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3], x4 = x[4];
+        double pz = st_fsqrt(c.opd2 - x1 * x1 - x3 * x3);
+        double temp = 1 - x3 * tanT / pz;
+        x[2] = x2 / cosT / temp;
+        x[3] = x3 * cosT + pz * sinT;
+        x[0] = x0 + x1 * x2 * tanT / pz / temp;
+        x[4] = x4 - c.opd * x2 * tanT / pz / temp / c.b1;
+        break;
+    }
+    case K_COOLING: /* COOLING_Pass, cpp:6132-6136 */
+#pragma unroll
+        for (int i = 0; i < 4; i++) x[i] = P[0] * x[i];
+        break;
+    case K_HELENS: /* HELENS_Pass, cpp:7392-7431 */
+        for (int i = 0; i < op.nint; i++) {
+            drift<HEXP>(s, P[0], P[1], c);
+            double r = st_fsqrt(x[0] * x[0] + x[2] * x[2]);
+            double scale;
+            if (r <= P[3])
+                scale = 0;
+            else if (r > P[3] && r <= P[4])
+                scale = (r * r - P[5]) * P[4] / r / P[6];
+            else if (r > P[4])
+                scale = P[4] / r;
+            else
+                scale = r; /* NaN radius: the reference leaves scale unset */
+            x[1] = x[1] + scale * P[2] * x[0] / r;
+            x[3] = x[3] + scale * P[2] * x[2] / r;
+            drift<HEXP>(s, P[0], P[1], c);
+        }
+        break;
+    case K_ACDIP:  /* ACDIP_Pass cpp:5930-5950, ACDIP_sPass cpp:8251-8284; GP.turn = turn */
+    case K_ACMULT: /* ACMULT_Pass cpp:5895-5928, ACMULT_sPass cpp:8210-8249 */
+    {
+        /* the kicks dpx, dpy (x - a is x + (-a) bit for bit) and the spin
+           field (B0, B1) of either element; one inlined spin rotation */
+        const bool dip = op.kind == K_ACDIP;
+        if (dip && turn < P[8]) {
+            drift<HEXP>(s, P[0], P[1], c);
+            break;
+        }
+        const double hL = dip ? P[2] : P[0], hLb = dip ? P[3] : P[1];
+        drift<HEXP>(s, hL, hLb, c);
+        double dpx, dpy, B0, B1;
+        if (dip) {
+            const double TS = P[8], TE = P[9], HK = P[4], VK = P[5];
+            const double sn = st_sin(P[6] * (turn - TS) + P[7]);
+            if (turn >= TS && turn <= TE) {
+                dpx = ((turn - TS) * 1.0 * HK / (TE - TS)) * sn;
+                dpy = ((turn - TS) * 1.0 * VK / (TE - TS)) * sn;
+            } else {
+                dpx = HK * sn;
+                dpy = VK * sn;
+            }
+            B0 = dpy;
+            B1 = -dpx;
+        } else {
+            double By, Bx;
+            if (op.norder == 0) {
+                By = P[2];
+                Bx = P[3];
+            } else {
+                double Xn = 1., Yn = 0.;
+                int fac = 1;
+                for (int i = 1; i < op.norder + 1; i++) {
+                    double Xn0 = Xn, Yn0 = Yn;
+                    Xn = Xn0 * x[0] - Yn0 * x[2];
+                    Yn = Xn0 * x[2] + Yn0 * x[0];
+                    fac = fac * i;
+                }
+                By = 0. + (P[2] * Xn - P[3] * Yn) / fac;
+                Bx = 0. + (P[2] * Yn + P[3] * Xn) / fac;
+            }
+            const double sn = st_sin(P[4] * turn / P[5] + P[6]);
+            B0 = Bx * sn;
+            B1 = By * sn;
+            dpx = -(By * sn);
+            dpy = Bx * sn;
+        }
+        if (SPIN) spin_rotate(s, x[0], x[1], x[3], 0, 0, B0, B1, 0., c, g);
+        x[1] = x[1] + dpx;
+        x[3] = x[3] + dpy;
+        drift<HEXP>(s, hL, hLb, c);
+        break;
+    }
     default:
         break;
     }
@@ -692,7 +812,7 @@ ST_DEV void track_turns(const st_track_args &a, const st_opp ops, const st_pp pa
                 } else if (!SPIN && op.kind == K_DRIFT0 && op.pm < 0) {
                     drift0<HEXP>(s, c);
                 } else {
-                    apply_op<SPIN, HEXP, RAD>(s, c, op, par, *st_launder_g((st_gp)a.gdev), (uint32_t)p, rctr);
+                    apply_op<SPIN, HEXP, RAD>(s, c, op, par, *st_launder_g((st_gp)a.gdev), (uint32_t)p, rctr, turn);
                 }
                 if (x04) s.x[4] = s.x[4] - x04[e];
                 if (a.xtbt && e == a.tbt_bpm) {
@@ -792,7 +912,7 @@ __global__ void k_radial_x04(st_track_args a, double *x04)
         s.x[5] = 0.;
         ptc_update(c, s.x[5], *st_launder_g((st_gp)a.gdev));
         const st_op op = ((st_opp)a.ops)[e];
-        apply_op<false, HEXP, false>(s, c, op, (st_pp)a.par, *st_launder_g((st_gp)a.gdev), 0u, rctr);
+        apply_op<false, HEXP, false>(s, c, op, (st_pp)a.par, *st_launder_g((st_gp)a.gdev), 0u, rctr, 0);
         x04[e] = s.x[4];
     }
 }

@@ -750,6 +750,159 @@ struct Builder {
                 return -2;
             }
             return drift_op(e.l, &e, e.l, 0.);
+        case ST_TRANS: {
+            /* TRANS::Pass / sPass = TRANS_Pass (cpp:12108-12118, 6140-6151):
+               PATCH_Pass without its rotation; no GtoL/LtoG */
+            st_op op = new_op(K_PATCH);
+            put(e.dx);
+            put(e.dy);
+            put(e.ds);
+            put(0.);
+            put(1.);
+            put(0.);
+            put(0.);
+            t.ops.push_back(op);
+            return 0;
+        }
+        case ST_YROTAT: {
+            /* YROTAT::Pass / sPass = YROTAT_Pass (cpp:12202-12212, 6192-6218):
+               PATCH_Pass's rotation; the zero translation is exact (x - 0 = x,
+               also for -0 and NaN) */
+            st_op op = new_op(K_PATCH);
+            put(0.);
+            put(0.);
+            put(0.);
+            put(std::fabs(e.theta) < 1.0e-9 ? 0. : 1.);
+            put(st_cos(e.theta));
+            put(st_sin(e.theta));
+            put(st_tan(e.theta));
+            t.ops.push_back(op);
+            return 0;
+        }
+        case ST_SROTAT: {
+            /* SROTAT::Pass / sPass = SROTAT_Pass (cpp:12155-12165, 6153-6170),
+               a no-op for PSI == 0 */
+            st_op op = new_op(e.psi == 0. ? K_NOP : K_SROTAT);
+            if (e.psi != 0.) {
+                put(st_cos(e.psi));
+                put(st_sin(e.psi));
+            }
+            t.ops.push_back(op);
+            return 0;
+        }
+        case ST_XROTAT: {
+            /* XROTAT::Pass / sPass = XROTAT_Pass (cpp:12249-12259, 6220-6245),
+               a no-op for |PHI| < 1e-9 */
+            st_op op = new_op(std::fabs(e.phi) < 1.0e-9 ? K_NOP : K_XROTAT);
+            if (op.kind == K_XROTAT) {
+                put(st_cos(e.phi));
+                put(st_sin(e.phi));
+                put(st_tan(e.phi));
+            }
+            t.ops.push_back(op);
+            return 0;
+        }
+        case ST_COOLING: {
+            /* COOLING::Pass / sPass = COOLING_Pass (cpp:12043-12053, 6132-6136) */
+            st_op op = new_op(K_COOLING);
+            put(1. - e.alpha);
+            t.ops.push_back(op);
+            return 0;
+        }
+        case ST_LRBB: {
+            /* LRBB::Pass / sPass (cpp:11299-11313): GtoL, LRBB_Pass (cpp:7358-7363:
+               the 6-argument BB4D, cpp:6384-6447, at the separated position), LtoG */
+            st_op op = new_op(K_LRBB);
+            put(e.sepx);
+            put(e.sepy);
+            put(2 * e.np * 1.53469826769e-18 / gp.gamma);
+            put(e.sigmax);
+            put(e.sigmay);
+            put(e.bbscale);
+            misalign(op, e, e.l, 0.);
+            t.ops.push_back(op);
+            return 0;
+        }
+        case ST_ELENS: {
+            /* ELENS::Pass / sPass (cpp:11649-11675): GtoL, ELENS_Pass (cpp:7370-7390),
+               LtoG; NE == 0 is DRIFT_Pass(x, Le) */
+            if (e.nslice < 1) {
+                err = "ELENS NSLICE must be at least 1";
+                return -3;
+            }
+            if (e.ne == 0.) return drift_op(e.l, &e, e.l, 0.);
+            st_op op = new_op(K_ELENS);
+            op.nint = e.nslice;
+            double Le_slice = e.l * 1.0 / e.nslice, Ne_slice = e.ne * 1.0 / e.nslice;
+            put(Le_slice / 2.);
+            put((Le_slice / 2.) / gp.beta);
+            put(2 * Ne_slice * 1.53469826769e-18 / gp.gamma);
+            put(e.sigmax);
+            put(e.sigmay);
+            put(1. + e.betae);
+            put(e.bbscale);
+            misalign(op, e, e.l, 0.);
+            t.ops.push_back(op);
+            return 0;
+        }
+        case ST_HELENS: {
+            /* HELENS::Pass / sPass (cpp:11765-11781): GtoL, HELENS_Pass
+               (cpp:7392-7431), LtoG; speed_light = 299792458. (cpp:24) */
+            if (e.nslice < 1) {
+                err = "HELENS NSLICE must be at least 1";
+                return -3;
+            }
+            st_op op = new_op(K_HELENS);
+            op.nint = e.nslice;
+            double Le_slice = e.l * 1.0 / e.nslice;
+            double I_e = 1.6021765e-19 * e.ne * e.betae * 299792458. / e.l;
+            double Theta_max = -2e-7 * Le_slice * I_e * (1 + e.betae * gp.beta) / e.betae / gp.beta / e.rout / gp.brho;
+            put(Le_slice / 2.);
+            put((Le_slice / 2.) / gp.beta);
+            put(Theta_max);
+            put(e.rin);
+            put(e.rout);
+            put(e.rin * e.rin);
+            put(e.rout * e.rout - e.rin * e.rin);
+            misalign(op, e, e.l, 0.);
+            t.ops.push_back(op);
+            return 0;
+        }
+        case ST_HACDIP: case ST_VACDIP: {
+            /* HACDIP/VACDIP::Pass (cpp:10357, 10453): GtoL, ACDIP_Pass (cpp:5930-5950)
+               with the other plane's maximum 0, LtoG; sPass: ACDIP_sPass (cpp:8251-8284) */
+            st_op op = new_op(K_ACDIP);
+            put(e.l);
+            put(e.l / gp.beta);
+            put(e.l / 2.);
+            put((e.l / 2.) / gp.beta);
+            put(e.type == ST_HACDIP ? e.hkick : 0.);
+            put(e.type == ST_VACDIP ? e.vkick : 0.);
+            put(2.0 * 3.14159265 * e.nud);
+            put(e.phid);
+            put(e.turns);
+            put(e.turne);
+            misalign(op, e, e.l, 0.);
+            t.ops.push_back(op);
+            return 0;
+        }
+        case ST_HACMULT: case ST_VACMULT: {
+            /* HACMULT/VACMULT::Pass (cpp:10173, 10260): GtoL, ACMULT_Pass
+               (cpp:5895-5928) with KL = KLMAX, KSL = 0 or KL = 0, KSL = KSLMAX, LtoG;
+               sPass: ACMULT_sPass (cpp:8210-8249) */
+            st_op op = new_op(K_ACMULT);
+            op.norder = e.norder;
+            put(e.l / 2.0);
+            put((e.l / 2.0) / gp.beta);
+            put(e.type == ST_HACMULT ? e.klmax : 0.);
+            put(e.type == ST_VACMULT ? e.klmax : 0.);
+            put(2.0 * PI);
+            put(e.tturns);
+            put(e.phi0);
+            misalign(op, e, e.l, 0.);
+            t.ops.push_back(op);
+            return 0;
+        }
         case ST_MATRIX: {
             st_op op = new_op(K_MATRIX);
             for (int i = 0; i < 6; i++) put(e.xco_in[i]);
@@ -796,14 +949,17 @@ static int validate(const st_element &e, int j)
 {
     switch (e.type) {
     case ST_MULT: case ST_GMULT: case ST_SBENDMULT: case ST_SMULT: case ST_GSBENDMULT:
+    case ST_HACMULT: case ST_VACMULT:
         if (e.norder < 0 || e.norder > 10)
             return fail(ST_ERR_PARAM, "element %d: Norder %d outside [0, 10] (KNL has 11 entries)", j, e.norder);
         break;
     default:
         break;
     }
-    /* a PATCH's L is its DS, which may be negative (cpp:12267); no map uses it */
-    if (e.l < 0. && e.type != ST_PATCH) return fail(ST_ERR_PARAM, "element %d: negative length", j);
+    /* a PATCH's or TRANS's L is its DS, which may be negative (cpp:12267,
+       12062); no map uses it */
+    if (e.l < 0. && e.type != ST_PATCH && e.type != ST_TRANS)
+        return fail(ST_ERR_PARAM, "element %d: negative length", j);
     return ST_OK;
 }
 

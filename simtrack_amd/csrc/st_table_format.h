@@ -60,6 +60,18 @@ enum st_kind {
     K_MATRIX,         /* 0..5 XCO_IN, 6..41 M66, 42..47 XCO_OUT                   */
     K_ROTAT,          /* 0 L/2, 1 Lb, 2..10 rotation matrix                       */
     K_PATCH,          /* 0 DX,1 DY,2 DS, 3 rotate (1/0), 4 cos,5 sin,6 tan of THETA */
+    K_SROTAT,         /* 0 cos, 1 sin of PSI                                      */
+    K_XROTAT,         /* 0 cos, 1 sin, 2 tan of PHI                               */
+    K_COOLING,        /* 0 1 - ALPHA                                              */
+    K_LRBB,           /* 0 SEPX, 1 SEPY, 2 2N rp/gamma, 3 sigmax, 4 sigmay, 5 BBSCALE */
+    K_ELENS,          /* 0 Le/N/2, 1 (Le/N/2)/beta, 2 2(Ne/N) rp/gamma, 3 sigmax,
+                         4 sigmay, 5 1+beta_e, 6 BBSCALE (nint = NSLICE)          */
+    K_HELENS,         /* 0 Le/N/2, 1 (Le/N/2)/beta, 2 Theta_max, 3 RIN, 4 ROUT,
+                         5 RIN*RIN, 6 ROUT*ROUT-RIN*RIN (nint = NSLICE)           */
+    K_ACDIP,          /* 0 L, 1 L/beta, 2 L/2, 3 (L/2)/beta, 4 HKICKMAX, 5 VKICKMAX,
+                         6 2*3.14159265*NUD, 7 PHID, 8 TURNS, 9 TURNE             */
+    K_ACMULT,         /* 0 L/2, 1 (L/2)/beta, 2 KL, 3 KSL, 4 2*PI, 5 TTURNS, 6 PHI0
+                         (norder = Norder)                                        */
     K_COUNT
 };
 

@@ -12,6 +12,8 @@ maps (simtrack.cpp:3743-8348) as the kernels execute them.
 roofline.achieved = F_turn * particle-turns/s; the bound is the FP64 vector
 peak of MI355X (78.6 TFLOP/s, FMA = 2).
 """
+import math
+
 import numpy as np
 
 from ._capi import TYPES
@@ -86,7 +88,22 @@ def _misalignment(r):
     return 2 * (72 + 5)
 
 
-def element_ops(r, gp, spin=False):
+def _acdip_kick(r, turn):
+    """ACDIP_Pass's kick amplitudes at a turn (simtrack.cpp:5930-5950):
+    None before TURNS (a plain drift), else whether any kick is non-zero."""
+    ts, te = float(r["turns"]), float(r["turne"])
+    if turn < ts:
+        return None
+    amp = (turn - ts) / (te - ts) if ts <= turn <= te and te != ts else 1.
+    hk = float(r["hkick"]) if int(r["type"]) == TYPES["HACDIP"] else 0.
+    vk = float(r["vkick"]) if int(r["type"]) == TYPES["VACDIP"] else 0.
+    s = math.sin(2.0 * 3.14159265 * float(r["nud"]) * (turn - ts) + float(r["phid"]))
+    return (hk * amp * s != 0.) or (vk * amp * s != 0.)
+
+
+def element_ops(r, gp, spin=False, turn=0):
+    """Counted operations of one element map; turn is the reference's
+    GP.turn, which only the AC elements read."""
     t = int(r["type"])
     L = float(r["l"])
     nint = max(int(r["nint"]), 1)
@@ -190,6 +207,45 @@ def element_ops(r, gp, spin=False):
         return 3 + (0 if abs(r["theta"]) < 1e-9 else 24)
     if name == "SOLEN":
         return drift_l(L) + mis
+    # frame changes and cooling: no GtoL/LtoG
+    if name == "TRANS":
+        return 3
+    if name == "SROTAT":
+        return 0 if r["psi"] == 0. else 12
+    if name in ("YROTAT", "XROTAT"):
+        # pz 5 (sqrt), temp 3, the rotated pair 5, the other coordinate 5, z 7
+        a = r["theta"] if name == "YROTAT" else r["phi"]
+        return 0 if abs(a) < 1e-9 else 24
+    if name == "COOLING":
+        return 4
+    if name == "LRBB":
+        # the separation added and removed around the 4-D kick
+        sx, sy = float(r["sigmax"]), float(r["sigmay"])
+        return (BB4D_ROUND if abs(sx - sy) / sx < 1e-6 else BB4D_ELLIPTIC) + 4 + mis
+    if name == "ELENS":
+        if r["ne"] == 0.:
+            return drift_l(L) + mis
+        sx, sy = float(r["sigmax"]), float(r["sigmay"])
+        bb = BB4D_ROUND if abs(sx - sy) / sx < 1e-6 else BB4D_ELLIPTIC
+        # each kick scaled twice, (1 + beta_e) then BBSCALE
+        return int(r["nslice"]) * (2 * drift_l(L / int(r["nslice"]) / 2) + bb + 2) + mis
+    if name == "HELENS":
+        # r 4, scale inside the hollow band 5, two kicks of 4 each
+        return int(r["nslice"]) * (2 * drift_l(L / int(r["nslice"]) / 2) + 17) + mis
+    if name in ("HACDIP", "VACDIP"):
+        k = _acdip_kick(r, turn)
+        if k is None:
+            return drift_l(L) + mis
+        return 2 * drift_l(L / 2) + 2 + (SPIN_UNIFORM if spin and k else 0) + mis
+    if name in ("HACMULT", "VACMULT"):
+        n = int(r["norder"])
+        if n == 0:
+            # a constant field: two kicks, the spin rotation with constant B
+            sp = SPIN_UNIFORM if spin and r["klmax"] != 0. else 0
+            return 2 * drift_l(L / 2) + 2 + sp + mis
+        # (x + iy)^n 6 per order, By and Bx 5 each, two kicks 2 each; the
+        # spin field's two products and spin()
+        return 2 * drift_l(L / 2) + 6 * n + 10 + 4 + (2 + SPIN if spin else 0) + mis
     return 0
 
 

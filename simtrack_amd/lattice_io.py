@@ -76,8 +76,8 @@ def _madx_element(name, kind, l, angle, e1, e2, tilt, knl, knsl, ks):
         if l != 0.:
             raise LatticeError("Beam-Beam element should have zero length!")
         return L.BEAMBEAM(name, 6, 0., 1, 0.4545, 11, 2.5e-06, 2.5e-06, 0.53, 0.0, 0.53, 0.0)
-    if kind == "ELENS":
-        raise LatticeError(f"{name}: ELENS is not an element of this engine")
+    if kind == "ELENS":                                       # cpp:13784-13787 (no current)
+        return L.ELENS(name, l, 0.0, 1, 0.14, 4, .31e-03, .31e-03)
     return L.DRIFT(name, l)                                   # every other keyword, cpp:13788
 
 
@@ -171,8 +171,8 @@ def _bmad_element(name, kind, l, angle, e1, e2, xoff, zoff, xpitch, knl, knsl, k
         if l != 0.:
             raise LatticeError("Beam-Beam element should have zero length!")
         return L.BEAMBEAM(name, 6, 0., 1, 0.4545, 11, 2.5e-06, 2.5e-06, 0.53, 0.0, 0.53, 0.0)
-    if kind == "ELENS":
-        raise LatticeError(f"{name}: ELENS is not an element of this engine")
+    if kind == "ELENS":                                       # cpp:14218-14221 (no current)
+        return L.ELENS(name, l, 0.0, 1, 0.14, 4, .31e-03, .31e-03)
     if kind == "Patch":                                       # cpp:14222-14225
         return L.PATCH(name, xoff, 0.0, zoff, -xpitch)
     return L.DRIFT(name, l)                                   # cpp:14226-14230

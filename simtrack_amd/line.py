@@ -474,6 +474,191 @@ class SOLEN(Element):
         r["nint"] = self.Nint
 
 
+class TRANS(Element):
+    """TRANS (simtrack.cpp:12056-12118): a frame translation by DX, DY, DS
+    (the element's own DX, DY, DS, not misalignments; L = DS)."""
+    TYPE = "TRANS"
+    _params = {"DX": "DX", "DY": "DY", "DS": "DS"}
+
+    def __init__(self, name, dx, dy, ds):
+        super().__init__(name)
+        self.L = float(ds)
+        self.DX, self.DY, self.DS = float(dx), float(dy), float(ds)
+
+
+class SROTAT(Element):
+    """SROTAT (simtrack.cpp:12121-12165): frame rotation about s by PSI."""
+    TYPE = "SROTAT"
+    _params = {"PSI": "PSI"}
+
+    def __init__(self, name, psi):
+        super().__init__(name)
+        self.PSI = float(psi)
+
+    def _fill(self, r):
+        r["psi"] = self.PSI
+
+
+class YROTAT(Element):
+    """YROTAT (simtrack.cpp:12168-12212): frame rotation about y by THETA."""
+    TYPE = "YROTAT"
+    _params = {"THETA": "THETA"}
+
+    def __init__(self, name, theta):
+        super().__init__(name)
+        self.THETA = float(theta)
+
+    def _fill(self, r):
+        r["theta"] = self.THETA
+
+
+class XROTAT(Element):
+    """XROTAT (simtrack.cpp:12215-12259): frame rotation about x by PHI."""
+    TYPE = "XROTAT"
+    _params = {"PHI": "PHI"}
+
+    def __init__(self, name, phi):
+        super().__init__(name)
+        self.PHI = float(phi)
+
+    def _fill(self, r):
+        r["phi"] = self.PHI
+
+
+class COOLING(Element):
+    """COOLING (simtrack.cpp:12009-12053): x, px, y, py scaled by 1 - ALPHA."""
+    TYPE = "COOLING"
+    _params = {"ALPHA": "ALPHA"}
+
+    def __init__(self, name, alpha):
+        super().__init__(name)
+        self.ALPHA = float(alpha)
+
+    def _fill(self, r):
+        r["alpha"] = self.ALPHA
+
+
+class LRBB(Element):
+    """LRBB (simtrack.cpp:11220-11313): a long-range 4-D beam-beam kick of a
+    beam separated by (SEPX, SEPY)."""
+    TYPE = "LRBB"
+    _params = {"NP": "NP", "BBSCALE": "BBSCALE", "SEPX": "SEPX", "SEPY": "SEPY", "SIGMAX": "SIGMAX",
+               "SIGMAY": "SIGMAY"}
+
+    def __init__(self, name, np_, bbscale, sepx, sepy, sigmax, sigmay):
+        super().__init__(name)
+        self.NP, self.BBSCALE = float(np_), float(bbscale)
+        self.SEPX, self.SEPY, self.SIGMAX, self.SIGMAY = float(sepx), float(sepy), float(sigmax), float(sigmay)
+
+    def _fill(self, r):
+        r["np"], r["bbscale"] = self.NP, self.BBSCALE
+        r["sepx"], r["sepy"], r["sigmax"], r["sigmay"] = self.SEPX, self.SEPY, self.SIGMAX, self.SIGMAY
+
+
+class ELENS(Element):
+    """ELENS (simtrack.cpp:11562-11675): NSLICE Gaussian electron-lens kicks
+    between half drifts."""
+    TYPE = "ELENS"
+    _params = {"NE": "NE", "BBSCALE": "BBSCALE", "NSLICE": "NSLICE", "BETAE": "BETAE", "SIGMAX": "SIGMAX",
+               "SIGMAY": "SIGMAY"}
+
+    def __init__(self, name, l, ne, bbscale, nslice, betae, sigmax, sigmay):
+        super().__init__(name)
+        _check_len(l)
+        self.L, self.NE, self.BBSCALE, self.NSLICE = float(l), float(ne), float(bbscale), int(nslice)
+        self.BETAE, self.SIGMAX, self.SIGMAY = float(betae), float(sigmax), float(sigmay)
+
+    def _fill(self, r):
+        r["ne"], r["bbscale"], r["nslice"], r["betae"] = self.NE, self.BBSCALE, self.NSLICE, self.BETAE
+        r["sigmax"], r["sigmay"] = self.SIGMAX, self.SIGMAY
+
+
+class HELENS(Element):
+    """HELENS (simtrack.cpp:11678-11781): a hollow electron lens, NSLICE
+    radial kicks between half drifts."""
+    TYPE = "HELENS"
+    _params = {"NE": "NE", "BBSCALE": "BBSCALE", "NSLICE": "NSLICE", "BETAE": "BETAE", "ROUT": "ROUT",
+               "RIN": "RIN"}
+
+    def __init__(self, name, l, ne, bbscale, nslice, betae, rout, rin):
+        super().__init__(name)
+        _check_len(l)
+        self.L, self.NE, self.BBSCALE, self.NSLICE = float(l), float(ne), float(bbscale), int(nslice)
+        self.BETAE, self.ROUT, self.RIN = float(betae), float(rout), float(rin)
+
+    def _fill(self, r):
+        r["ne"], r["bbscale"], r["nslice"], r["betae"] = self.NE, self.BBSCALE, self.NSLICE, self.BETAE
+        r["rout"], r["rin"] = self.ROUT, self.RIN
+
+
+class HACDIP(Element):
+    """HACDIP (simtrack.cpp:10278-10371): horizontal AC dipole, kick
+    HKICKMAX sin(2 pi NUD (turn - TURNS) + PHID), ramped up between turns
+    TURNS and TURNE."""
+    TYPE = "HACDIP"
+    _params = {"HKICKMAX": "HKICKMAX", "NUD": "NUD", "PHID": "PHID", "TURNS": "TURNS", "TURNE": "TURNE"}
+    _kick = "HKICKMAX"
+
+    def __init__(self, name, l, kickmax, nud, phid, turns, turne):
+        super().__init__(name)
+        _check_len(l)
+        self.L = float(l)
+        setattr(self, self._kick, float(kickmax))
+        self.NUD, self.PHID, self.TURNS, self.TURNE = float(nud), float(phid), int(turns), int(turne)
+
+    def SetP(self, name, value):
+        if name in ("TURNS", "TURNE"):
+            if name not in self._params:
+                raise KeyError(f"{self.TYPE} does not have a parameter of {name}")
+            setattr(self, name, int(value))
+        else:
+            super().SetP(name, value)
+
+    def _fill(self, r):
+        r["hkick" if self._kick == "HKICKMAX" else "vkick"] = getattr(self, self._kick)
+        r["nud"], r["phid"], r["turns"], r["turne"] = self.NUD, self.PHID, self.TURNS, self.TURNE
+
+
+class VACDIP(HACDIP):
+    """VACDIP (simtrack.cpp:10374-10467): the vertical AC dipole."""
+    TYPE = "VACDIP"
+    _params = {"VKICKMAX": "VKICKMAX", "NUD": "NUD", "PHID": "PHID", "TURNS": "TURNS", "TURNE": "TURNE"}
+    _kick = "VKICKMAX"
+
+
+class HACMULT(Element):
+    """HACMULT (simtrack.cpp:10104-10187): one normal multipole order Norder
+    (0 = dipole) of integrated strength KLMAX modulated by sin(2 pi turn /
+    TTURNS + PHI0), between half drifts."""
+    TYPE = "HACMULT"
+    _params = {"Norder": "Norder", "KLMAX": "KLMAX", "TTURNS": "TTURNS", "PHI0": "PHI0"}
+    _k = "KLMAX"
+
+    def __init__(self, name, l, norder, klmax, tturns, phi0):
+        super().__init__(name)
+        _check_len(l)
+        self.L, self.Norder = float(l), int(norder)
+        setattr(self, self._k, float(klmax))
+        self.TTURNS, self.PHI0 = int(tturns), float(phi0)
+
+    def SetP(self, name, value):
+        if name == "TTURNS":
+            self.TTURNS = int(value)
+        else:
+            super().SetP(name, value)
+
+    def _fill(self, r):
+        r["norder"], r["klmax"] = self.Norder, getattr(self, self._k)
+        r["tturns"], r["phi0"] = self.TTURNS, self.PHI0
+
+
+class VACMULT(HACMULT):
+    """VACMULT (simtrack.cpp:10191-10274): the skew counterpart, KSLMAX."""
+    TYPE = "VACMULT"
+    _params = {"Norder": "Norder", "KSLMAX": "KSLMAX", "TTURNS": "TTURNS", "PHI0": "PHI0"}
+    _k = "KSLMAX"
+
+
 # -------------------------------------------------------------------- line
 
 class Line:

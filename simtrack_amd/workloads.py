@@ -71,12 +71,22 @@ def all_types_line(seed=7, misalign=True, with_bb=True, with_rf=True, with_gsben
     els += [L.HBPM("HB", 0.2), L.VBPM("VB", 0.)]
     els += [L.PATCH("PT", -0.012, 0.001, 3e-4, 0.021), L.PATCH("PT0", 0.003, 0., -1e-4, 0.),
             L.SOLEN("SOL", 1.1, 0.)]
+    els += [L.TRANS("TR", 2e-4, -1e-4, 0.05), L.SROTAT("SR", 0.013), L.YROTAT("YR", -0.007),
+            L.XROTAT("XR", 0.009), L.COOLING("CO", 1e-4), L.SROTAT("SR0", 0.), L.XROTAT("XR0", 1e-10)]
+    if with_bb:
+        els += [L.LRBB("LR", 1.5e11, 1.0, 8e-4, -3e-4, 2e-4, 2e-4),
+                L.ELENS("EL", 2.0, 1e11, 1.0, 3, 0.5, 3e-4, 3e-4), L.ELENS("EL0", 1.0, 0., 1.0, 2, 0.5, 3e-4, 3e-4),
+                L.HELENS("HE", 2.0, 5e11, 1.0, 2, 0.5, 1e-2, 1e-6)]
+    # AC elements: at turn 0 HAC is inside its ramp and VAC past it
+    els += [L.HACDIP("HAC", 0.3, 2e-5, 0.31, 0.2, -3, 5), L.VACDIP("VAC", 0.0, 1e-5, 0.27, 0.0, -10, -2),
+            L.HACMULT("HAM", 0.2, 2, 0.05, 100, 0.3), L.VACMULT("VAM", 0.0, 0, 1e-5, 50, 0.1)]
     for e in els:
         if hasattr(e, "Nint") and e.L > 0:
             e.Nint = max(1, min(e.Nint, 5))
     if misalign:
         for i, e in enumerate(els):
-            if e.TYPE in ("DRIFT", "MARKER", "BPM", "HBPM", "VBPM", "MATRIX", "ROTAT", "SNAKE", "PATCH"):
+            if e.TYPE in ("DRIFT", "MARKER", "BPM", "HBPM", "VBPM", "MATRIX", "ROTAT", "SNAKE", "PATCH", "TRANS",
+                          "SROTAT", "YROTAT", "XROTAT", "COOLING"):
                 continue
             k = i % 4
             if k == 1:

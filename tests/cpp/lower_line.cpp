@@ -58,6 +58,18 @@ int main()
     l.Append(new MATRIX("MAT", 0., z, xo, m66));
     l.Append(new ROTAT("ROT", 0.5, n1, 0.2));
     l.Append(new SNAKE("SNK", 0.1, n2, 1.5707963267948966));
+    l.Append(new TRANS("TR", 2e-4, -1e-4, 0.05));
+    l.Append(new SROTAT("SR", 0.013));
+    l.Append(new YROTAT("YR", -0.007));
+    l.Append(new XROTAT("XR", 0.009));
+    l.Append(new COOLING("CO", 1e-4));
+    l.Append(new LRBB("LR", 1.5e11, 1.0, 8e-4, -3e-4, 2e-4, 1e-4));
+    l.Append(new ELENS("EL", 2.0, 1e11, 1.0, 3, 0.5, 3e-4, 2e-4));
+    l.Append(new HELENS("HE", 2.0, 5e11, 1.0, 2, 0.5, 1e-2, 1e-6));
+    l.Append(new HACDIP("HAC", 0.3, 2e-5, 0.31, 0.2, -3, 5));
+    l.Append(new VACDIP("VAC", 0.0, 1e-5, 0.27, 0.0, -10, -2));
+    l.Append(new HACMULT("HAM", 0.2, 2, 0.05, 100, 0.3));
+    l.Append(new VACMULT("VAM", 0.0, 0, 1e-5, 50, 0.1));
     /* edits through the reference's parameter names */
     l.Cell[7]->SetP("K1L", 0.06);
     l.Cell[7]->SetP("Nint", 7.9);
@@ -66,6 +78,10 @@ int main()
     l.Cell[5]->DX = 1e-4;
     l.Cell[5]->DPSI = 2e-3;
     l.Cell[9]->APx = 0.03;
+    l.Cell[34]->SetP("SEPX", 9e-4);
+    l.Cell[37]->SetP("TURNE", 6);
+    l.Cell[39]->SetP("PHI0", 0.25);
+    l.Cell[36]->DX = 3e-5;
     std::vector<st_element> r = records_from_line(l);
     if (l.Cell[11]->GetP("Norder") != 7 || l.Cell[7]->GetP("Nint") != 7) return 3;
     bool threw = false;

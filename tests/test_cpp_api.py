@@ -34,6 +34,11 @@ def python_line():
         st.CRBB("CRB", 1.5e11, 0.01, 1.0, 0.05, 5, 5e-9, 0.4, 0.02, 2.5e-9, 0.2, 0.),
         st.MATRIX("MAT", 0., np.zeros(6), [1e-4, 0., -2e-4, 0., 0., 0.], m66),
         st.ROTAT("ROT", 0.5, [0.3, 0.9, 0.1], 0.2), st.SNAKE("SNK", 0.1, [1.0, 0.0, 0.2], 1.5707963267948966),
+        st.TRANS("TR", 2e-4, -1e-4, 0.05), st.SROTAT("SR", 0.013), st.YROTAT("YR", -0.007), st.XROTAT("XR", 0.009),
+        st.COOLING("CO", 1e-4), st.LRBB("LR", 1.5e11, 1.0, 8e-4, -3e-4, 2e-4, 1e-4),
+        st.ELENS("EL", 2.0, 1e11, 1.0, 3, 0.5, 3e-4, 2e-4), st.HELENS("HE", 2.0, 5e11, 1.0, 2, 0.5, 1e-2, 1e-6),
+        st.HACDIP("HAC", 0.3, 2e-5, 0.31, 0.2, -3, 5), st.VACDIP("VAC", 0.0, 1e-5, 0.27, 0.0, -10, -2),
+        st.HACMULT("HAM", 0.2, 2, 0.05, 100, 0.3), st.VACMULT("VAM", 0.0, 0, 1e-5, 50, 0.1),
     ]
     ln = st.Line(els)
     ln.Cell[7].SetP("K1L", 0.06)
@@ -43,6 +48,10 @@ def python_line():
     ln.Cell[5].DX = 1e-4
     ln.Cell[5].DPSI = 2e-3
     ln.Cell[9].APx = 0.03
+    ln.Cell[34].SetP("SEPX", 9e-4)
+    ln.Cell[37].SetP("TURNE", 6)
+    ln.Cell[39].SetP("PHI0", 0.25)
+    ln.Cell[36].DX = 3e-5
     return ln
 
 
@@ -59,7 +68,7 @@ def test_cpp_lowering_matches_python(tmp_path):
     got = np.frombuffer(r.stdout, dtype=st.ELEMENT_DTYPE)
     ln = python_line()
     ref = ln.records()
-    assert len(got) == len(ref) == 29
+    assert len(got) == len(ref) == 41
     for i in range(len(ref)):
         assert got[i].tobytes() == ref[i].tobytes(), ln.Cell[i].TYPE
     length = float(r.stderr.split()[1])

@@ -1,0 +1,75 @@
+"""GPU parity of the frame-change, cooling, long-range beam-beam, e-lens and
+AC elements inside a ring: multi-turn Track and Track_spin bit for bit
+against the oracle (the AC elements read the turn number, so the
+comparison runs over the ramp and past it), and the same result when the
+turns are split over two device calls (the kernels take the absolute turn)."""
+import numpy as np
+import pytest
+import torch
+
+import pyoracle
+import simtrack_amd as st
+from simtrack_amd import workloads as W
+
+from test_gpu_parity import _fodo_ring, _grid, assert_bitwise
+
+pytestmark = pytest.mark.gpu
+
+
+def _ring():
+    ln = _fodo_ring(ncell=8)
+    extra = [st.TRANS("TR", 1e-5, -2e-5, 0.01), st.TRANS("TRI", -1e-5, 2e-5, -0.01),
+             st.SROTAT("SR", 2e-3), st.SROTAT("SRI", -2e-3), st.YROTAT("YR", 2e-5), st.YROTAT("YRI", -2e-5),
+             st.XROTAT("XR", -1.5e-5), st.XROTAT("XRI", 1.5e-5), st.COOLING("CO", 2e-5),
+             st.LRBB("LR", 1e11, 1.0, 1.2e-3, -4e-4, 3e-4, 3e-4), st.LRBB("LRE", 1e11, 1.0, -9e-4, 6e-4, 4e-4, 2e-4),
+             st.ELENS("EL", 0.5, 2e11, 1.0, 3, 0.3, 5e-4, 5e-4), st.ELENS("ELE", 0.3, 1e11, 0.8, 2, 0.3, 6e-4, 3e-4),
+             st.HELENS("HE", 0.5, 3e11, 1.0, 2, 0.4, 4e-3, 1e-3),
+             st.HACDIP("HAC", 0.3, 2e-6, 0.27, 0.1, 3, 12), st.VACDIP("VAC", 0.2, 1.5e-6, 0.31, 0.0, 5, 9),
+             st.HACMULT("HAM", 0.2, 2, 0.02, 7, 0.2), st.VACMULT("VAM", 0.1, 0, 1e-6, 5, 0.3)]
+    for i, e in enumerate(extra):
+        e.APx = e.APy = 0.05
+        if e.TYPE in ("LRBB", "ELENS", "HACDIP", "HACMULT") and i % 2:
+            e.DX, e.DPSI = 2e-5, 1e-4
+    cells = list(ln.Cell)
+    for k, e in enumerate(extra):
+        cells.insert(3 + 7 * k, e)
+    ring = st.Line(cells)
+    # the lenses and AC elements lengthen the ring: RF back on harmonic 360
+    for e in ring.Cell:
+        if e.TYPE == "RFCAV":
+            e.FRF = 360 * 299792458. * 0.99995 / ring.Length
+    return ring
+
+
+@pytest.mark.parametrize("spin", [False, True], ids=["track", "track_spin"])
+def test_ring_with_new_elements_bitwise(ctx, spin):
+    ln = _ring()
+    g = W.proton_globals(100000.0)
+    rec = ln.records()
+    tab = st.Table(ctx, g, rec)
+    orc = pyoracle.Oracle(g, rec)
+    x = _grid(12, amax=2.4e-2, spin=spin)
+    nturn = 40
+    xg, sg, ltg, lpg = tab.track(x.copy(), nturn, spin=spin)
+    xo, so, lto, lpo = orc.track(x.copy(), nturn, spin=spin)
+    assert np.array_equal(sg, so) and np.array_equal(ltg, lto) and np.array_equal(lpg, lpo)
+    assert sg.sum() > len(sg) // 3, "most of the grid should survive"
+    assert_bitwise(xg, xo, f"ring with frame/lens/AC elements, spin={spin}")
+
+
+def test_turns_split_over_device_calls(ctx):
+    ln = _ring()
+    g = W.proton_globals(100000.0)
+    rec = ln.records()
+    tab = st.Table(ctx, g, rec)
+    x = _grid(8, amax=1.6e-2)
+    d_x = torch.tensor(x, dtype=torch.float64, device="cuda:0").contiguous()
+    d_s = torch.ones(len(x), dtype=torch.int32, device="cuda:0")
+    d_l = torch.zeros(2 * len(x), dtype=torch.int32, device="cuda:0")
+    torch.cuda.synchronize()
+    tab.track_device(d_x.data_ptr(), len(x), 0, 11, d_s.data_ptr(), d_l.data_ptr())
+    tab.track_device(d_x.data_ptr(), len(x), 11, 19, d_s.data_ptr(), d_l.data_ptr())
+    ctx.sync()
+    xo, so, lto, lpo = pyoracle.Oracle(g, rec).track(x.copy(), 30)
+    assert np.array_equal(d_s.cpu().numpy() != 0, so != 0)
+    assert_bitwise(d_x.cpu().numpy(), xo, "30 turns as 11 + 19")
