@@ -90,7 +90,7 @@ enum st_element_type {
     ST_ROTAT = 28,     /* ROTAT      cpp:11784 (spin rotator; orbit: drift)   */
     ST_SNAKE = 29,     /* SNAKE      cpp:11864 (spin rotator; orbit: drift)   */
     ST_PATCH = 30,     /* PATCH      cpp:12262 dx,dy,ds = DX,DY,DS, theta = THETA, L = DS */
-    ST_SOLEN = 31,     /* SOLEN      cpp:9645  ks = KS (KS = 0 only: a misaligned drift) */
+    ST_SOLEN = 31,     /* SOLEN      cpp:9645  ks = KS, nint = Nint (1)                    */
     ST_TRANS = 32,     /* TRANS      cpp:12056 dx,dy,ds = DX,DY,DS (not misalignments), L = DS */
     ST_SROTAT = 33,    /* SROTAT     cpp:12121 psi = PSI   */
     ST_YROTAT = 34,    /* YROTAT     cpp:12168 theta = THETA */

@@ -595,9 +595,8 @@ public:
     }
 };
 
-/* SOLEN (simtrack.cpp:9645-9704), KS = Bz / brho, Nint = 1.  The engine
-   tracks KS = 0 (then SOLEN_Pass is DRIFT_Pass inside GtoL/LtoG); building a
-   table with a non-zero KS fails. */
+/* SOLEN (simtrack.cpp:9645-9704), KS = Bz / brho, Nint = 1: a hard-edge
+   solenoid (SOLEN_Pass cpp:5804, SOLEN_sPass cpp:8118); KS = 0 is a drift */
 class SOLEN : public Element {
 public:
     SOLEN(std::string name, double l, double ks) : Element(std::move(name))

@@ -530,6 +530,52 @@ static void HELENS_Pass(double x[], double Le, double Ne, double bbscale, double
     }
 }
 
+/* SOLEN_Pass, simtrack.cpp:5804-5857: hard-edge solenoid in (x, x', y, y'):
+   entrance kick, Nint times the body matrix, exit kick */
+static void SOLEN_Pass(double x[], double L, int Nint, double KS)
+{
+    int i, j, k;
+    double xtemp[6], ps, gamma1, delta1, beta1, vz;
+    double Bz = KS * GP.brho, K, Lint = L / Nint, theta, sintheta, costheta, m44[16];
+    if (KS == 0) {
+        DRIFT_Pass(x, L);
+        return;
+    }
+    delta1 = ORC_PT_CACHED(x[5], sqrt(1.0 + 2 * x[5] / GP.beta + x[5] * x[5]) - 1.0);
+    gamma1 = ORC_PT_CACHED(x[5], GP.gamma + sqrt(GP.gamma * GP.gamma - 1.) * x[5]);
+    beta1 = ORC_PT_DERIVED(sqrt(1.0 - 1.0 / gamma1 / gamma1));
+    ps = sqrt((1 + delta1) * (1 + delta1) - x[1] * x[1] - x[3] * x[3]);
+    x[1] = x[1] / ps;
+    x[3] = x[3] / ps;
+    vz = 2.99792458e8 * beta1 / sqrt(1.0 + x[1] * x[1] + x[3] * x[3]);
+    K = GP.q * Bz / 2.0 / gamma1 / GP.m0 / vz;
+    x[1] = x[1] + x[2] * K;
+    x[3] = x[3] - x[0] * K;
+    theta = -2.0 * K * L / Nint;
+    sintheta = OSIN(theta);
+    costheta = OCOS(theta);
+    m44[0 * 4 + 0] = 1.0;  m44[0 * 4 + 1] = Lint * sintheta / theta;
+    m44[0 * 4 + 2] = 0.;   m44[0 * 4 + 3] = -(1 - costheta) * Lint / theta;
+    m44[1 * 4 + 0] = 0.;   m44[1 * 4 + 1] = costheta;
+    m44[1 * 4 + 2] = 0.;   m44[1 * 4 + 3] = -sintheta;
+    m44[2 * 4 + 0] = 0.;   m44[2 * 4 + 1] = (1 - costheta) * Lint / theta;
+    m44[2 * 4 + 2] = 1.;   m44[2 * 4 + 3] = Lint * sintheta / theta;
+    m44[3 * 4 + 0] = 0.;   m44[3 * 4 + 1] = sintheta;
+    m44[3 * 4 + 2] = 0.;   m44[3 * 4 + 3] = costheta;
+    for (i = 0; i < Nint; i++) {
+        for (k = 0; k < 4; k++) xtemp[k] = x[k];
+        for (k = 0; k < 4; k++) {
+            x[k] = 0.;
+            for (j = 0; j < 4; j++) x[k] = x[k] + m44[k * 4 + j] * xtemp[j];
+        }
+    }
+    x[1] = x[1] - x[2] * K;
+    x[3] = x[3] + x[0] * K;
+    x[1] = gamma1 * GP.m0 * x[1] * vz / GP.p;
+    x[3] = gamma1 * GP.m0 * x[3] * vz / GP.p;
+    x[4] = x[4] - 2.99792458e8 * (L / vz - L / GP.beta / 2.99792458e8);
+}
+
 /* bend_kick_pass, simtrack.cpp:3790-3794 */
 static void bend_kick_pass(double x[], double L, double href)
 {
@@ -2002,6 +2048,63 @@ static void KICK_sPass(double x[], double L, double HKICK, double VKICK)
     DRIFT_Pass(x, L / 2.0);
 }
 
+/* SOLEN_sPass, simtrack.cpp:8118-8177: as SOLEN_Pass with Bz = KS Brho / L,
+   and after each body step spin() of the step's entry state, its x', y'
+   converted back to px, py, in the longitudinal field Bz Lint / Brho */
+static void SOLEN_sPass(double x[], double L, int Nint, double KS)
+{
+    int i, j, k;
+    double ps, gamma1, delta1, beta1, vz;
+    double Bz = KS * GP.brho / L, K, Lint = L / Nint, theta, sintheta, costheta, m44[16];
+    double BLbrho[3], xtemp[9];
+    if (KS == 0) {
+        DRIFT_Pass(x, L);
+        return;
+    }
+    delta1 = ORC_PT_CACHED(x[5], sqrt(1.0 + 2 * x[5] / GP.beta + x[5] * x[5]) - 1.0);
+    gamma1 = ORC_PT_CACHED(x[5], GP.gamma + sqrt(GP.gamma * GP.gamma - 1.) * x[5]);
+    beta1 = ORC_PT_DERIVED(sqrt(1.0 - 1.0 / gamma1 / gamma1));
+    ps = sqrt((1 + delta1) * (1 + delta1) - x[1] * x[1] - x[3] * x[3]);
+    x[1] = x[1] / ps;
+    x[3] = x[3] / ps;
+    vz = 2.99792458e8 * beta1 / sqrt(1.0 + x[1] * x[1] + x[3] * x[3]);
+    K = GP.q * Bz / 2.0 / gamma1 / GP.m0 / vz;
+    x[1] = x[1] + x[2] * K;
+    x[3] = x[3] - x[0] * K;
+    theta = -2.0 * K * L / Nint;
+    sintheta = OSIN(theta);
+    costheta = OCOS(theta);
+    m44[0 * 4 + 0] = 1.0;  m44[0 * 4 + 1] = Lint * sintheta / theta;
+    m44[0 * 4 + 2] = 0.;   m44[0 * 4 + 3] = -(1 - costheta) * Lint / theta;
+    m44[1 * 4 + 0] = 0.;   m44[1 * 4 + 1] = costheta;
+    m44[1 * 4 + 2] = 0.;   m44[1 * 4 + 3] = -sintheta;
+    m44[2 * 4 + 0] = 0.;   m44[2 * 4 + 1] = (1 - costheta) * Lint / theta;
+    m44[2 * 4 + 2] = 1.;   m44[2 * 4 + 3] = Lint * sintheta / theta;
+    m44[3 * 4 + 0] = 0.;   m44[3 * 4 + 1] = sintheta;
+    m44[3 * 4 + 2] = 0.;   m44[3 * 4 + 3] = costheta;
+    for (i = 0; i < Nint; i++) {
+        for (k = 0; k < 9; k++) xtemp[k] = x[k];
+        for (k = 0; k < 4; k++) {
+            x[k] = 0.;
+            for (j = 0; j < 4; j++) x[k] = x[k] + m44[k * 4 + j] * xtemp[j];
+        }
+        xtemp[1] = gamma1 * GP.m0 * xtemp[1] * vz / GP.p;
+        xtemp[3] = gamma1 * GP.m0 * xtemp[3] * vz / GP.p;
+        BLbrho[0] = 0.;
+        BLbrho[1] = 0.;
+        BLbrho[2] = Bz * Lint / GP.brho;
+        spin(xtemp, 0, 0, BLbrho);
+        x[6] = xtemp[6];
+        x[7] = xtemp[7];
+        x[8] = xtemp[8];
+    }
+    x[1] = x[1] - x[2] * K;
+    x[3] = x[3] + x[0] * K;
+    x[1] = gamma1 * GP.m0 * x[1] * vz / GP.p;
+    x[3] = gamma1 * GP.m0 * x[3] * vz / GP.p;
+    x[4] = x[4] - 2.99792458e8 * (L / vz - L / GP.beta / 2.99792458e8);
+}
+
 /* ACMULT_sPass, simtrack.cpp:8210-8249 */
 static void ACMULT_sPass(double x[], double L, int Norder, double KL, double KSL, int TTURNS, double PHI0)
 {
@@ -2205,10 +2308,10 @@ static int elem_pass(const st_element *e, double x[], orc_rng *rng)
     case ST_PATCH:                          /* PATCH::Pass, cpp:12323-12325 */
         PATCH_Pass(x, e->dx, e->dy, e->ds, e->theta);
         return 0;
-    case ST_SOLEN:                          /* SOLEN::Pass, cpp:9688-9692; KS = 0: DRIFT_Pass, cpp:5811-5814 */
-        if (e->ks != 0.) return -1;
+    case ST_SOLEN:                          /* SOLEN::Pass, cpp:9688-9692 */
+        if (e->nint < 1) return -1;
         GtoL(x, e->l, 0., MIS(e));
-        DRIFT_Pass(x, e->l);
+        SOLEN_Pass(x, e->l, e->nint, e->ks);
         LtoG(x, e->l, 0., MIS(e));
         return 0;
     case ST_TRANS:                          /* TRANS::Pass, cpp:12108-12110 */
@@ -2334,8 +2437,13 @@ static int elem_spass(const st_element *e, double x[])
     case ST_RFCAV: case ST_CCMULT: case ST_BEAMBEAM: case ST_CRBB: case ST_MATRIX:
         return elem_pass(e, x, NULL);       /* sPass == Pass (orbit only) */
     case ST_PATCH:                          /* PATCH::sPass, cpp:12331-12333: the orbit map */
-    case ST_SOLEN:                          /* SOLEN::sPass, cpp:9700-9704; KS = 0: DRIFT_Pass, cpp:8125-8128 */
         return elem_pass(e, x, NULL);
+    case ST_SOLEN:                          /* SOLEN::sPass, cpp:9700-9704 */
+        if (e->nint < 1) return -1;
+        GtoL(x, e->l, 0., MIS(e));
+        SOLEN_sPass(x, e->l, e->nint, e->ks);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
     case ST_TRANS: case ST_YROTAT: case ST_SROTAT: case ST_XROTAT: case ST_COOLING:
     case ST_LRBB: case ST_ELENS: case ST_HELENS:
         return elem_pass(e, x, NULL);       /* sPass == Pass (cpp:12116, 12210, 12163, 12257, 12051, 11309, 11671, 11777) */

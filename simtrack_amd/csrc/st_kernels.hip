@@ -690,56 +690,102 @@ ST_DEV void apply_op(St &s, Ptc &c, const st_op &op, st_pp par, st_gref g,
         break;
     case K_ACDIP:  /* ACDIP_Pass cpp:5930-5950, ACDIP_sPass cpp:8251-8284; GP.turn = turn */
     case K_ACMULT: /* ACMULT_Pass cpp:5895-5928, ACMULT_sPass cpp:8210-8249 */
+    case K_SOLEN:  /* SOLEN_Pass cpp:5804-5857, SOLEN_sPass cpp:8118-8177 */
     {
-        /* the kicks dpx, dpy (x - a is x + (-a) bit for bit) and the spin
-           field (B0, B1) of either element; one inlined spin rotation */
-        const bool dip = op.kind == K_ACDIP;
+        /* one inlined spin rotation for the three: after the AC kick's first
+           half drift, or after each solenoid body step (delta1, gamma1,
+           beta1 from the cache) */
+        const bool dip = op.kind == K_ACDIP, sol = op.kind == K_SOLEN;
         if (dip && turn < P[8]) {
             drift<HEXP>(s, P[0], P[1], c);
             break;
         }
         const double hL = dip ? P[2] : P[0], hLb = dip ? P[3] : P[1];
-        drift<HEXP>(s, hL, hLb, c);
-        double dpx, dpy, B0, B1;
-        if (dip) {
-            const double TS = P[8], TE = P[9], HK = P[4], VK = P[5];
-            const double sn = st_sin(P[6] * (turn - TS) + P[7]);
-            if (turn >= TS && turn <= TE) {
-                dpx = ((turn - TS) * 1.0 * HK / (TE - TS)) * sn;
-                dpy = ((turn - TS) * 1.0 * VK / (TE - TS)) * sn;
-            } else {
-                dpx = HK * sn;
-                dpy = VK * sn;
-            }
-            B0 = dpy;
-            B1 = -dpx;
+        double dpx = 0., dpy = 0., B0 = 0., B1 = 0., B2 = 0.;
+        double K = 0., vz = 0., sn = 0., cs = 0., m01 = 0., m03 = 0., m21 = 0.;
+        int n = 1;
+        if (sol) {
+            const double L = P[0], Lint = P[1];
+            double ps = st_fsqrt(c.opd2 - x[1] * x[1] - x[3] * x[3]);
+            x[1] = x[1] / ps;
+            x[3] = x[3] / ps;
+            vz = 2.99792458e8 * c.b1 / st_fsqrt(1.0 + x[1] * x[1] + x[3] * x[3]);
+            K = P[2] / c.g1 / P[3] / vz;
+            x[1] = x[1] + x[2] * K;
+            x[3] = x[3] - x[0] * K;
+            const double theta = -2.0 * K * L / op.nint;
+            sn = st_sin(theta);
+            cs = st_cos(theta);
+            m01 = Lint * sn / theta;
+            m03 = -(1 - cs) * Lint / theta;
+            m21 = (1 - cs) * Lint / theta;
+            n = op.nint;
+            B2 = P[6];
         } else {
-            double By, Bx;
-            if (op.norder == 0) {
-                By = P[2];
-                Bx = P[3];
-            } else {
-                double Xn = 1., Yn = 0.;
-                int fac = 1;
-                for (int i = 1; i < op.norder + 1; i++) {
-                    double Xn0 = Xn, Yn0 = Yn;
-                    Xn = Xn0 * x[0] - Yn0 * x[2];
-                    Yn = Xn0 * x[2] + Yn0 * x[0];
-                    fac = fac * i;
+            /* the kicks dpx, dpy (x - a is x + (-a) bit for bit) and the field */
+            drift<HEXP>(s, hL, hLb, c);
+            if (dip) {
+                const double TS = P[8], TE = P[9], HK = P[4], VK = P[5];
+                const double s1 = st_sin(P[6] * (turn - TS) + P[7]);
+                if (turn >= TS && turn <= TE) {
+                    dpx = ((turn - TS) * 1.0 * HK / (TE - TS)) * s1;
+                    dpy = ((turn - TS) * 1.0 * VK / (TE - TS)) * s1;
+                } else {
+                    dpx = HK * s1;
+                    dpy = VK * s1;
                 }
-                By = 0. + (P[2] * Xn - P[3] * Yn) / fac;
-                Bx = 0. + (P[2] * Yn + P[3] * Xn) / fac;
+                B0 = dpy;
+                B1 = -dpx;
+            } else {
+                double By, Bx;
+                if (op.norder == 0) {
+                    By = P[2];
+                    Bx = P[3];
+                } else {
+                    double Xn = 1., Yn = 0.;
+                    int fac = 1;
+                    for (int i = 1; i < op.norder + 1; i++) {
+                        double Xn0 = Xn, Yn0 = Yn;
+                        Xn = Xn0 * x[0] - Yn0 * x[2];
+                        Yn = Xn0 * x[2] + Yn0 * x[0];
+                        fac = fac * i;
+                    }
+                    By = 0. + (P[2] * Xn - P[3] * Yn) / fac;
+                    Bx = 0. + (P[2] * Yn + P[3] * Xn) / fac;
+                }
+                const double s1 = st_sin(P[4] * turn / P[5] + P[6]);
+                B0 = Bx * s1;
+                B1 = By * s1;
+                dpx = -(By * s1);
+                dpy = Bx * s1;
             }
-            const double sn = st_sin(P[4] * turn / P[5] + P[6]);
-            B0 = Bx * sn;
-            B1 = By * sn;
-            dpx = -(By * sn);
-            dpy = Bx * sn;
         }
-        if (SPIN) spin_rotate(s, x[0], x[1], x[3], 0, 0, B0, B1, 0., c, g);
-        x[1] = x[1] + dpx;
-        x[3] = x[3] + dpy;
-        drift<HEXP>(s, hL, hLb, c);
+        for (int i = 0; i < n; i++) {
+            double xe = x[0], pxe = x[1], pye = x[3];
+            if (sol) {
+                const double t0 = x[0], t1 = x[1], t2 = x[2], t3 = x[3];
+                /* the reference's 4x4 product term by term, its zero entries included */
+                x[0] = (((0. + 1.0 * t0) + m01 * t1) + 0. * t2) + m03 * t3;
+                x[1] = (((0. + 0. * t0) + cs * t1) + 0. * t2) + -sn * t3;
+                x[2] = (((0. + 0. * t0) + m21 * t1) + 1. * t2) + m01 * t3;
+                x[3] = (((0. + 0. * t0) + sn * t1) + 0. * t2) + cs * t3;
+                xe = t0;
+                pxe = c.g1 * P[3] * t1 * vz / P[4];
+                pye = c.g1 * P[3] * t3 * vz / P[4];
+            }
+            if (SPIN) spin_rotate(s, xe, pxe, pye, 0, 0, B0, B1, B2, c, g);
+        }
+        if (sol) {
+            x[1] = x[1] - x[2] * K;
+            x[3] = x[3] + x[0] * K;
+            x[1] = c.g1 * P[3] * x[1] * vz / P[4];
+            x[3] = c.g1 * P[3] * x[3] * vz / P[4];
+            x[4] = x[4] - 2.99792458e8 * (P[0] / vz - P[5]);
+        } else {
+            x[1] = x[1] + dpx;
+            x[3] = x[3] + dpy;
+            drift<HEXP>(s, hL, hLb, c);
+        }
         break;
     }
     default:

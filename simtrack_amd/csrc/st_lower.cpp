@@ -742,14 +742,31 @@ struct Builder {
             t.ops.push_back(op);
             return 0;
         }
-        case ST_SOLEN:
-            /* SOLEN::Pass / sPass (cpp:9688-9704): GtoL, SOLEN_Pass, LtoG; with
-               KS = 0 SOLEN_Pass and SOLEN_sPass are DRIFT_Pass (cpp:5811, 8125) */
-            if (e.ks != 0.) {
-                err = "SOLEN with KS != 0 is not supported (only KS = 0, a drift)";
-                return -2;
+        case ST_SOLEN: {
+            /* SOLEN::Pass / sPass (cpp:9688-9704): GtoL, SOLEN_Pass (cpp:5804) or
+               SOLEN_sPass (cpp:8118), LtoG; with KS = 0 both are DRIFT_Pass.
+               The spin pass takes Bz = KS*brho/L where the orbit pass takes
+               KS*brho (the reference's two bodies differ so) */
+            if (e.ks == 0.) return drift_op(e.l, &e, e.l, 0.);
+            if (e.nint < 1) {
+                err = "SOLEN Nint must be at least 1";
+                return -3;
             }
-            return drift_op(e.l, &e, e.l, 0.);
+            st_op op = new_op(K_SOLEN);
+            op.nint = e.nint;
+            double Bz = spin ? e.ks * gp.brho / e.l : e.ks * gp.brho;
+            double Lint = e.l / e.nint;
+            put(e.l);
+            put(Lint);
+            put(gp.q * Bz / 2.0);
+            put(gp.m0);
+            put(gp.p);
+            put(e.l / gp.beta / 2.99792458e8);
+            put(Bz * Lint / gp.brho);
+            misalign(op, e, e.l, 0.);
+            t.ops.push_back(op);
+            return 0;
+        }
         case ST_TRANS: {
             /* TRANS::Pass / sPass = TRANS_Pass (cpp:12108-12118, 6140-6151):
                PATCH_Pass without its rotation; no GtoL/LtoG */

@@ -72,6 +72,8 @@ enum st_kind {
                          6 2*3.14159265*NUD, 7 PHID, 8 TURNS, 9 TURNE             */
     K_ACMULT,         /* 0 L/2, 1 (L/2)/beta, 2 KL, 3 KSL, 4 2*PI, 5 TTURNS, 6 PHI0
                          (norder = Norder)                                        */
+    K_SOLEN,          /* 0 L, 1 L/Nint, 2 q*Bz/2, 3 m0, 4 p, 5 L/beta/c,
+                         6 Bz*Lint/brho (nint = Nint)                             */
     K_COUNT
 };
 

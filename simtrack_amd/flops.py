@@ -206,7 +206,11 @@ def element_ops(r, gp, spin=False, turn=0):
         # TRANS_Pass 3; YROTAT_Pass: pz 5 (sqrt), temp 3, x 2, px 3, y 5, z 7
         return 3 + (0 if abs(r["theta"]) < 1e-9 else 24)
     if name == "SOLEN":
-        return drift_l(L) + mis
+        if r["ks"] == 0.:
+            return drift_l(L) + mis
+        # x', y' and vz 46; per body step the 4x4 product 32, and in the spin
+        # pass the step's px, py 8 and spin() in a constant longitudinal field 108
+        return 46 + nint * (32 + (116 if spin else 0)) + mis
     # frame changes and cooling: no GtoL/LtoG
     if name == "TRANS":
         return 3

@@ -457,9 +457,8 @@ class PATCH(Element):
 
 
 class SOLEN(Element):
-    """SOLEN (simtrack.cpp:9645-9704), KS = Bz / brho.  The engine tracks
-    KS = 0 (SOLEN_Pass is then DRIFT_Pass inside GtoL/LtoG); a table with a
-    non-zero KS is rejected when it is built."""
+    """SOLEN (simtrack.cpp:9645-9704), KS = Bz / brho, Nint = 1: a hard-edge
+    solenoid (SOLEN_Pass cpp:5804, SOLEN_sPass cpp:8118); KS = 0 is a drift."""
     TYPE = "SOLEN"
     _params = {"KS": "KS"}
 

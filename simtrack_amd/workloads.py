@@ -70,7 +70,7 @@ def all_types_line(seed=7, misalign=True, with_bb=True, with_rf=True, with_gsben
                 L.CRBB("CRB", 1.5e11, 0.01, 1.0, 0.05, 5, 2 * eps, 0.4, 0.02, eps, 0.2, 0.)]
     els += [L.HBPM("HB", 0.2), L.VBPM("VB", 0.)]
     els += [L.PATCH("PT", -0.012, 0.001, 3e-4, 0.021), L.PATCH("PT0", 0.003, 0., -1e-4, 0.),
-            L.SOLEN("SOL", 1.1, 0.)]
+            L.SOLEN("SOL", 1.1, 0.), L.SOLEN("SOLK", 1.2, 0.05)]
     els += [L.TRANS("TR", 2e-4, -1e-4, 0.05), L.SROTAT("SR", 0.013), L.YROTAT("YR", -0.007),
             L.XROTAT("XR", 0.009), L.COOLING("CO", 1e-4), L.SROTAT("SR0", 0.), L.XROTAT("XR0", 1e-10)]
     if with_bb:

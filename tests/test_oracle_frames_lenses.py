@@ -1,13 +1,14 @@
-"""The oracle's frame changes, cooling, long-range beam-beam, electron lenses
-and AC elements (TRANS, SROTAT, YROTAT, XROTAT, COOLING, LRBB, ELENS,
-HELENS, HACDIP/VACDIP, HACMULT/VACMULT) pinned by what the mathematics
-fixes, not by retyping their expressions: exact translations and scalings,
+"""The oracle's frame changes, cooling, long-range beam-beam, electron lenses,
+AC elements and solenoid (TRANS, SROTAT, YROTAT, XROTAT, COOLING, LRBB,
+ELENS, HELENS, HACDIP/VACDIP, HACMULT/VACMULT, SOLEN) pinned by what the
+mathematics fixes, not by retyping their expressions: exact translations and scalings,
 rotation matrices, the x<->y mirror between the two Forest rotations and
 their inverses, symplecticity, the round-beam and Bassetti-Erskine kicks at
 the separated position, the thin-lens limits of the lenses, and the AC
 kicks turn by turn (the reference's GP.turn counts from 0 in Track,
 simtrack.cpp:20799) against closed forms and against KICKER's own spin
-pass."""
+pass, and the solenoid's linear map against the textbook hard-edge matrix,
+its rotational symmetry about s and the (1 + G) KS spin precession."""
 import math
 
 import numpy as np
@@ -233,3 +234,64 @@ def test_spin_stays_a_unit_vector():
         o = pyoracle.Oracle(GP, st.Line([el]).records())
         y, *_ = o.track(x, 5, spin=True)
         np.testing.assert_allclose(np.linalg.norm(y[:, 6:9], axis=1), 1., atol=1e-14)
+
+
+def _transverse_jacobian(o, spin=False, h=1e-7):
+    J = np.zeros((4, 4))
+    for k in range(4):
+        d = np.zeros((2, 9 if spin else 6))
+        d[0, k], d[1, k] = h, -h
+        if spin:
+            d[:, 7] = 1.
+        y = o.element_pass(0, d, spin=spin)
+        J[:, k] = (y[0, :4] - y[1, :4]) / (2 * h)
+    return J
+
+
+def _solenoid_matrix(K, L):
+    """Hard-edge solenoid (fringes included), Larmor wave number K, in
+    (x, px, y, py) for the reference particle: the textbook matrix."""
+    C, S = math.cos(K * L), math.sin(K * L)
+    return np.array([[C * C, S * C / K, S * C, S * S / K], [-K * S * C, C * C, -K * S * S, S * C],
+                     [-S * C, -S * S / K, C * C, S * C / K], [K * S * S, -S * C, -K * S * C, C * C]])
+
+
+@pytest.mark.parametrize("nint", [1, 4])
+def test_solenoid_linear_map_is_the_textbook_matrix(nint):
+    L, KS = 1.2, 0.8
+    e = st.SOLEN("S", L, KS)
+    e.Nint = nint
+    o = orc(e)
+    # SOLEN_Pass: Bz = KS Brho, so K = KS / 2
+    np.testing.assert_allclose(_transverse_jacobian(o), _solenoid_matrix(KS / 2, L), rtol=0, atol=1e-11)
+    # SOLEN_sPass takes Bz = KS Brho / L (simtrack.cpp:8122): K = KS / (2 L)
+    np.testing.assert_allclose(_transverse_jacobian(o, spin=True), _solenoid_matrix(KS / 2 / L, L), rtol=0,
+                               atol=1e-11)
+
+
+def test_solenoid_commutes_with_rotations_about_s():
+    x = particles(8, scale=1e-3)
+    sol, rot = st.SOLEN("S", 0.9, 0.3), st.SROTAT("R", 0.7)
+    a = one_pass(sol, one_pass(rot, x))
+    b = one_pass(rot, one_pass(sol, x))
+    np.testing.assert_allclose(a, b, rtol=0, atol=1e-15)
+    x0 = np.zeros((1, 6))
+    y = one_pass(sol, x0)
+    assert np.all(y[0, :4] == 0.) and abs(y[0, 4]) < 1e-14 and y[0, 5] == 0.
+
+
+@pytest.mark.parametrize("nint", [1, 3])
+def test_solenoid_spin_precesses_by_one_plus_g_times_ks(nint):
+    """Thomas-BMT in a longitudinal field: the spin of the on-axis reference
+    particle turns about s by (1 + G) B L / Brho = (1 + G) KS."""
+    KS = 0.05
+    e = st.SOLEN("S", 1.3, KS)
+    e.Nint = nint
+    x = np.zeros((1, 9))
+    x[0, 6] = 1.
+    y = orc(e).element_pass(0, x, spin=True)
+    ang = (1 + GP.G) * KS
+    assert y[0, 6] == pytest.approx(math.cos(ang), abs=1e-13)
+    assert abs(y[0, 7]) == pytest.approx(math.sin(ang), abs=1e-13)
+    assert y[0, 8] == 0.
+    np.testing.assert_allclose(np.linalg.norm(y[0, 6:9]), 1., atol=1e-15)
