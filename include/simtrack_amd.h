@@ -102,7 +102,8 @@ enum st_element_type {
     ST_HACDIP = 40,    /* HACDIP     cpp:10278 l, hkick = HKICKMAX, nud, phid, turns, turne */
     ST_VACDIP = 41,    /* VACDIP     cpp:10374 l, vkick = VKICKMAX, nud, phid, turns, turne */
     ST_HACMULT = 42,   /* HACMULT    cpp:10104 l, norder, klmax = KLMAX, tturns, phi0 */
-    ST_VACMULT = 43    /* VACMULT    cpp:10191 l, norder, klmax = KSLMAX, tturns, phi0 */
+    ST_VACMULT = 43,   /* VACMULT    cpp:10191 l, norder, klmax = KSLMAX, tturns, phi0 */
+    ST_WIGGLER = 44    /* WIGGLER    cpp:9707  l, nint, b0, kx, kz, phiz0 */
 };
 
 /*
@@ -150,6 +151,7 @@ typedef struct st_element {
     double turns, turne;    /* HACDIP/VACDIP TURNS, TURNE (integers)         */
     double klmax;           /* HACMULT KLMAX, VACMULT KSLMAX                 */
     double tturns, phi0;    /* HACMULT/VACMULT TTURNS (integer), PHI0        */
+    double b0, kx, kz, phiz0; /* WIGGLER B0, KX, KZ, PHIZ0                   */
 } st_element;
 
 /*

@@ -27,7 +27,7 @@
  * SKEWQ, SEXT, OCT, MULT, GMULT, SBENDMULT, SMULT, GSBENDMULT, KICKER,
  * HKICKER, VKICKER, RFCAV, CRABRF, CCMULT, LBT, ILBT, BEAMBEAM, CRBB,
  * MATRIX, ROTAT, SNAKE, PATCH, SOLEN, TRANS, SROTAT, YROTAT, XROTAT, COOLING,
- * LRBB, ELENS, HELENS, HACDIP, VACDIP, HACMULT, VACMULT, with the reference's
+ * LRBB, ELENS, HELENS, HACDIP, VACDIP, HACMULT, VACMULT, WIGGLER, with the reference's
  * constructor arguments, Nint and Norder defaults (simtrack.cpp:8451, 8635,
  * 8943-8950) and SetP/GetP names.
  *
@@ -816,6 +816,28 @@ public:
     }
 };
 
+/* WIGGLER (simtrack.cpp:9707-9771): B0 [T], KX, KZ, PHIZ0; Pass by Forest's
+   implicit step (cpp:4400), sPass by Wu's splitting (cpp:4516), no spin */
+class WIGGLER : public Element {
+public:
+    WIGGLER(std::string name, double l, int nint, double b0, double kx, double kz, double phiz0)
+        : Element(std::move(name))
+    {
+        if (!(l > 0.)) throw Error("Wiggler: L must be non-zero.");
+        init("WIGGLER", ST_WIGGLER, l);
+        rec.nint = nint;
+        rec.b0 = b0;
+        rec.kx = kx;
+        rec.kz = kz;
+        rec.phiz0 = phiz0;
+        bind_int("Nint", rec.nint);
+        bind("B0", rec.b0);
+        bind("KX", rec.kx);
+        bind("KZ", rec.kz);
+        bind("PHIZ0", rec.phiz0);
+    }
+};
+
 /* ----------------------------------------------------------------- line */
 
 /* Line (simtrack.h:1067-1100): owns its elements, as the reference's does
@@ -1045,7 +1067,7 @@ inline int type_code(const std::string &t)
                                   "BPM", "HBPM", "VBPM", "RFCAV", "CRABRF", "CCMULT", "LBT", "ILBT", "BEAMBEAM",
                                   "CRBB", "MATRIX", "ROTAT", "SNAKE", "PATCH", "SOLEN", "TRANS", "SROTAT",
                                   "YROTAT", "XROTAT", "COOLING", "LRBB", "ELENS", "HELENS", "HACDIP", "VACDIP",
-                                  "HACMULT", "VACMULT"};
+                                  "HACMULT", "VACMULT", "WIGGLER"};
     for (int i = 1; i < (int)(sizeof(names) / sizeof(names[0])); ++i)
         if (t == names[i]) return i;
     return -1;
@@ -1230,6 +1252,13 @@ st_element record_of(E &e)
             r.phid = P("PHID");
             r.turns = P("TURNS");
             r.turne = P("TURNE");
+            break;
+        case ST_WIGGLER:
+            r.nint = (int)P("Nint");
+            r.b0 = P("B0");
+            r.kx = P("KX");
+            r.kz = P("KZ");
+            r.phiz0 = P("PHIZ0");
             break;
         case ST_HACMULT:
         case ST_VACMULT:

@@ -134,11 +134,11 @@ struct dual_element {
     Dual l, angle, e1, e2, k0l, knl[11], knsl[11], dx, dy, ds, dtheta, dphi, dpsi, apx, apy, vrf, frf, phase0,
         hkick, vkick, theta, np, bbscale, sigmal, emitx, betax, alfax, emity, betay, alfay, cc[8], rot_n[3],
         rot_angle, m66[36], xco_in[6], xco_out[6], ks, psi, phi, alpha, sepx, sepy, sigmax, sigmay, ne, betae,
-        rout, rin, nud, phid, turns, turne, klmax, tturns, phi0;
+        rout, rin, nud, phid, turns, turne, klmax, tturns, phi0, b0, kx, kz, phiz0;
 };
-static const int DUAL_NDOUBLE = (int)((offsetof(st_element, phi0) - offsetof(st_element, l)) / sizeof(double)) + 1;
-static_assert((offsetof(dual_element, phi0) - offsetof(dual_element, l)) / sizeof(Dual) + 1 ==
-                  (offsetof(st_element, phi0) - offsetof(st_element, l)) / sizeof(double) + 1,
+static const int DUAL_NDOUBLE = (int)((offsetof(st_element, phiz0) - offsetof(st_element, l)) / sizeof(double)) + 1;
+static_assert((offsetof(dual_element, phiz0) - offsetof(dual_element, l)) / sizeof(Dual) + 1 ==
+                  (offsetof(st_element, phiz0) - offsetof(st_element, l)) / sizeof(double) + 1,
               "dual_element must mirror st_element");
 
 static void to_dual(const st_element *e, dual_element *o)

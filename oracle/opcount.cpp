@@ -127,11 +127,11 @@ struct opc_element {
     Tracked l, angle, e1, e2, k0l, knl[11], knsl[11], dx, dy, ds, dtheta, dphi, dpsi, apx, apy, vrf, frf, phase0,
         hkick, vkick, theta, np, bbscale, sigmal, emitx, betax, alfax, emity, betay, alfay, cc[8], rot_n[3],
         rot_angle, m66[36], xco_in[6], xco_out[6], ks, psi, phi, alpha, sepx, sepy, sigmax, sigmay, ne, betae,
-        rout, rin, nud, phid, turns, turne, klmax, tturns, phi0;
+        rout, rin, nud, phid, turns, turne, klmax, tturns, phi0, b0, kx, kz, phiz0;
 };
-static const int OPC_NDOUBLE = (int)((offsetof(st_element, phi0) - offsetof(st_element, l)) / sizeof(double)) + 1;
-static_assert((offsetof(opc_element, phi0) - offsetof(opc_element, l)) / sizeof(Tracked) + 1 ==
-                  (offsetof(st_element, phi0) - offsetof(st_element, l)) / sizeof(double) + 1,
+static const int OPC_NDOUBLE = (int)((offsetof(st_element, phiz0) - offsetof(st_element, l)) / sizeof(double)) + 1;
+static_assert((offsetof(opc_element, phiz0) - offsetof(opc_element, l)) / sizeof(Tracked) + 1 ==
+                  (offsetof(st_element, phiz0) - offsetof(st_element, l)) / sizeof(double) + 1,
               "opc_element must mirror st_element");
 
 /* operations in the scope of a zero-length drift are not counted (the F_turn

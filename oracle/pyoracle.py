@@ -64,7 +64,8 @@ def load(glibc=False):
     lib.orc_lyapunov_scan.argtypes = [P, I, D, D, D, I, D, D, P, I]
     lib.orc_action_diff.argtypes = [P, I, D, D, D, D, D, D, D, D, I, I, I, P, P]
     lib.orc_fine_tune_xpx.argtypes = [I, P, P, D, D, P]
-    for f in ("orc_sin", "orc_cos", "orc_tan", "orc_asin", "orc_exp", "orc_log", "orc_atan", "orc_ugaussian_pinv"):
+    for f in ("orc_sin", "orc_cos", "orc_tan", "orc_asin", "orc_exp", "orc_cosh", "orc_sinh", "orc_log", "orc_atan",
+              "orc_ugaussian_pinv"):
         getattr(lib, f).argtypes = [D]
         getattr(lib, f).restype = D
     lib.orc_powi.argtypes = [D, I]

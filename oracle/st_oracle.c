@@ -62,8 +62,12 @@
 #endif
 #if ORACLE_GLIBC_MASK & 4
 #define OEXP(x) exp(x)
+#define OCOSH(x) cosh(x)
+#define OSINH(x) sinh(x)
 #else
 #define OEXP(x) st_exp(x)
+#define OCOSH(x) st_cosh(x)
+#define OSINH(x) st_sinh(x)
 #endif
 #if ORACLE_GLIBC_MASK & 8
 #define OPOW(x, n) pow((x), (double)(n))
@@ -574,6 +578,146 @@ static void SOLEN_Pass(double x[], double L, int Nint, double KS)
     x[1] = gamma1 * GP.m0 * x[1] * vz / GP.p;
     x[3] = gamma1 * GP.m0 * x[3] * vz / GP.p;
     x[4] = x[4] - 2.99792458e8 * (L / vz - L / GP.beta / 2.99792458e8);
+}
+
+/* get_Axy_wiggler, simtrack.cpp:4371-4397: vector potential of the
+   wiggler field and its derivatives (per unit B0 scaling as the reference) */
+static void get_Axy_wiggler(double AxoBrho[], double AyoBrho[], double z, const double x[], double b0, double kx,
+                            double kz, double phiz0)
+{
+    double ky = sqrt(kx * kx + kz * kz);
+    double cx = OCOS(kx * x[0]), sx = OSIN(kx * x[0]);
+    double chy = OCOSH(ky * x[2]), shy = OSINH(ky * x[2]);
+    double cz = OCOS(kz * z + phiz0), sz = OSIN(kz * z + phiz0);
+    AxoBrho[0] = (b0 / kz) * cx * chy * sz;
+    AyoBrho[0] = (b0 * kx / ky / kz) * sx * shy * sz;
+    AxoBrho[1] = -(b0 * kx / kz) * sx * chy * sz;
+    AyoBrho[1] = (b0 * kx * kx / ky / kz) * cx * shy * sz;
+    AxoBrho[2] = (b0 * ky / kz) * cx * shy * sz;
+    AyoBrho[2] = (b0 * kx / kz) * sx * chy * sz;
+    AxoBrho[3] = b0 * cx * chy * cz;
+    AyoBrho[3] = (b0 * kx / ky) * sx * shy * cz;
+}
+
+static void radiate(double x[], double L, double href, const double BLbrho[]);
+
+/* Wiggler_Pass_Forest, simtrack.cpp:4400-4452: implicit symplectic step per
+   slice (WIGGLER::Pass) */
+static void Wiggler_Pass_Forest(double x[], double l, int nint, double b0, double kx, double kz, double phiz0)
+{
+    double h = l / nint, z;
+    double AxoBrho[4], AyoBrho[4], psi, hodp, det, t1, t2, x01, x02, y01, y02, x0[6], BL[3];
+    for (int i = 0; i < nint; i++) {
+        z = h * i;
+        get_Axy_wiggler(AxoBrho, AyoBrho, z, x, b0, kx, kz, phiz0);
+        for (int j = 0; j < 4; j++) {
+            AxoBrho[j] = AxoBrho[j] / GP.brho;
+            AyoBrho[j] = AyoBrho[j] / GP.brho;
+        }
+        for (int j = 0; j < 6; j++) x0[j] = x[j];
+        psi = 1.0 + x0[5];
+        hodp = h / psi;
+        det = ((1 - AxoBrho[1] * hodp) * (1 - AyoBrho[2] * hodp) - AxoBrho[2] * AyoBrho[1] * hodp * hodp);
+        x01 = AxoBrho[0] * AxoBrho[1];
+        x02 = AxoBrho[0] * AxoBrho[2];
+        y01 = AyoBrho[0] * AyoBrho[1];
+        y02 = AyoBrho[0] * AyoBrho[2];
+        x[1] = (x0[1] - (x01 + y01) * hodp) * (1 - AyoBrho[2] * hodp) +
+               (x0[3] - (x02 + y02) * hodp) * AyoBrho[1] * hodp;
+        x[1] = x[1] / det;
+        x[3] = (x0[3] - (x02 + y02) * hodp) * (1 - AxoBrho[1] * hodp) +
+               (x0[1] - (x01 + y01) * hodp) * AxoBrho[2] * hodp;
+        x[3] = x[3] / det;
+        x[0] = x0[0] + hodp * (x[1] - AxoBrho[0]);
+        x[2] = x0[2] + hodp * (x[3] - AyoBrho[0]);
+        t1 = (x[1] - AxoBrho[0]) / psi;
+        t2 = (x[3] - AyoBrho[0]) / psi;
+        x[4] = x[4] - h * (t1 * t1 + t2 * t2) / 2.0;
+        if (GP.radiate) {
+            BL[0] = -AyoBrho[3] * GP.brho * h;
+            BL[1] = AxoBrho[3] * GP.brho * h;
+            BL[2] = (AyoBrho[1] - AxoBrho[2]) * GP.brho * h;
+            radiate(x, h, 0, BL);
+        }
+    }
+}
+
+/* get_Ax_Wu / get_Ay_Wu, simtrack.cpp:4456-4494 */
+static void get_Ax_Wu(double *AxoBrho, double *IntpAxpydx, double z, const double x[], double b0, double kx,
+                      double kz, double phiz0)
+{
+    double ky = sqrt(kx * kx + kz * kz);
+    double cx = OCOS(kx * x[0]), sx = OSIN(kx * x[0]);
+    double shy = OSINH(ky * x[2]), chy = OCOSH(ky * x[2]);
+    double sz = OSIN(kz * z + phiz0);
+    *AxoBrho = (b0 / kz) * cx * chy * sz;
+    if (kx == 0.)
+        *IntpAxpydx = (b0 * ky / kz) * x[0] * shy * sz;
+    else
+        *IntpAxpydx = (b0 * ky / kx / kz) * sx * shy * sz;
+}
+static void get_Ay_Wu(double *AyoBrho, double *IntpAypxdy, double z, const double x[], double b0, double kx,
+                      double kz, double phiz0)
+{
+    double ky = sqrt(kx * kx + kz * kz);
+    double cx = OCOS(kx * x[0]), sx = OSIN(kx * x[0]);
+    double chy = OCOSH(ky * x[2]), shy = OSINH(ky * x[2]);
+    double sz = OSIN(kz * z + phiz0);
+    *AyoBrho = (b0 * kx / ky / kz) * sx * shy * sz;
+    *IntpAypxdy = (b0 * kx * kx / ky / ky / kz) * cx * chy * sz;
+}
+
+/* get_wiggler_B, simtrack.cpp:4497-4514 */
+static void get_wiggler_B(const double x[], double B[], double b0, double kx, double kz, double phiz0)
+{
+    double ky = sqrt(kx * kx + kz * kz);
+    double cx = OCOS(kx * x[0]), sx = OSIN(kx * x[0]);
+    double chy = OCOSH(ky * x[2]), shy = OSINH(ky * x[2]);
+    double cz = OCOS(kz * x[4] + phiz0), sz = OSIN(kz * x[4] + phiz0);
+    B[0] = (kx / ky) * b0 * sx * shy * cz;
+    B[1] = b0 * cx * chy * cz;
+    B[2] = -(kz * b0 / ky) * cx * shy * sz;
+}
+
+/* Wiggler_Pass_Wu, simtrack.cpp:4516-4572: second-order splitting per slice
+   (WIGGLER::sPass; the spin is not rotated) */
+static void Wiggler_Pass_Wu(double x[], double l, int nint, double b0, double kx, double kz, double phiz0)
+{
+    double h = l / nint, z = 0.0, hd, AxoBrho, AyoBrho, IntpAxpydx, IntpAypxdy, B[3];
+    for (int i = 0; i < nint; i++) {
+        hd = h / (1.0 + x[5]);
+        z += 0.5 * h;
+        get_Ay_Wu(&AyoBrho, &IntpAypxdy, z, x, b0, kx, kz, phiz0);
+        x[1] = x[1] - IntpAypxdy / GP.brho;
+        x[3] = x[3] - AyoBrho / GP.brho;
+        x[2] = x[2] + 0.5 * hd * x[3];
+        x[4] = x[4] - hd * x[3] * x[3] / (1.0 + x[5]) / (1.0 + x[5]) / 4;
+        get_Ay_Wu(&AyoBrho, &IntpAypxdy, z, x, b0, kx, kz, phiz0);
+        x[1] = x[1] + IntpAypxdy / GP.brho;
+        x[3] = x[3] + AyoBrho / GP.brho;
+        get_Ax_Wu(&AxoBrho, &IntpAxpydx, z, x, b0, kx, kz, phiz0);
+        x[1] = x[1] - AxoBrho / GP.brho;
+        x[3] = x[3] - IntpAxpydx / GP.brho;
+        x[0] = x[0] + hd * x[1];
+        x[4] = x[4] - 0.5 * hd * x[1] * x[1] / (1.0 + x[5]);
+        get_Ax_Wu(&AxoBrho, &IntpAxpydx, z, x, b0, kx, kz, phiz0);
+        x[1] = x[1] + AxoBrho / GP.brho;
+        x[3] = x[3] + IntpAxpydx / GP.brho;
+        get_Ay_Wu(&AyoBrho, &IntpAypxdy, z, x, b0, kx, kz, phiz0);
+        x[1] = x[1] - IntpAypxdy / GP.brho;
+        x[3] = x[3] - AyoBrho / GP.brho;
+        x[2] = x[2] + 0.5 * hd * x[3];
+        x[4] = x[4] - hd * x[3] * x[3] / (1.0 + x[5]) / (1.0 + x[5]) / 4;
+        get_Ay_Wu(&AyoBrho, &IntpAypxdy, z, x, b0, kx, kz, phiz0);
+        x[1] = x[1] + IntpAypxdy / GP.brho;
+        x[3] = x[3] + AyoBrho / GP.brho;
+        z += 0.5 * h;
+        if (GP.radiate) {
+            get_wiggler_B(x, B, b0, kx, kz, phiz0);
+            for (int j = 0; j < 3; j++) B[j] = B[j] * h;
+            radiate(x, h, 0.0, B);
+        }
+    }
 }
 
 /* bend_kick_pass, simtrack.cpp:3790-3794 */
@@ -2364,6 +2508,12 @@ static int elem_pass(const st_element *e, double x[], orc_rng *rng)
         LtoG(x, e->l, 0., MIS(e));
         return 0;
     }
+    case ST_WIGGLER:                        /* WIGGLER::Pass, cpp:9755-9759 */
+        if (e->nint < 1) return -1;
+        GtoL(x, e->l, 0., MIS(e));
+        Wiggler_Pass_Forest(x, e->l, e->nint, e->b0, e->kx, e->kz, e->phiz0);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
     default:
         return -1;
     }
@@ -2438,6 +2588,12 @@ static int elem_spass(const st_element *e, double x[])
         return elem_pass(e, x, NULL);       /* sPass == Pass (orbit only) */
     case ST_PATCH:                          /* PATCH::sPass, cpp:12331-12333: the orbit map */
         return elem_pass(e, x, NULL);
+    case ST_WIGGLER:                        /* WIGGLER::sPass, cpp:9767-9771 (no spin rotation) */
+        if (e->nint < 1) return -1;
+        GtoL(x, e->l, 0., MIS(e));
+        Wiggler_Pass_Wu(x, e->l, e->nint, e->b0, e->kx, e->kz, e->phiz0);
+        LtoG(x, e->l, 0., MIS(e));
+        return 0;
     case ST_SOLEN:                          /* SOLEN::sPass, cpp:9700-9704 */
         if (e->nint < 1) return -1;
         GtoL(x, e->l, 0., MIS(e));
@@ -2666,6 +2822,8 @@ EXPORT double orc_cos(double x) { return OCOS(x); }
 EXPORT double orc_tan(double x) { return OTAN(x); }
 EXPORT double orc_asin(double x) { return OASIN(x); }
 EXPORT double orc_exp(double x) { return OEXP(x); }
+EXPORT double orc_cosh(double x) { return OCOSH(x); }
+EXPORT double orc_sinh(double x) { return OSINH(x); }
 #if !defined(ORACLE_GLIBC) && !defined(ORC_SCALAR_BUILD)
 /* the branch-free common paths the GPU kernels evaluate first */
 EXPORT double orc_exp_phase(double x, int phase) { return st_exp_phase(x, phase); }

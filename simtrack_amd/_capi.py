@@ -34,6 +34,7 @@ ELEMENT_DTYPE = np.dtype([
     ("ne", "<f8"), ("betae", "<f8"), ("rout", "<f8"), ("rin", "<f8"),
     ("nud", "<f8"), ("phid", "<f8"), ("turns", "<f8"), ("turne", "<f8"),
     ("klmax", "<f8"), ("tturns", "<f8"), ("phi0", "<f8"),
+    ("b0", "<f8"), ("kx", "<f8"), ("kz", "<f8"), ("phiz0", "<f8"),
 ])
 
 
@@ -59,7 +60,7 @@ TYPES = {
     "VBPM": 19, "RFCAV": 20, "CRABRF": 21, "CCMULT": 22, "LBT": 23, "ILBT": 24,
     "BEAMBEAM": 25, "CRBB": 26, "MATRIX": 27, "ROTAT": 28, "SNAKE": 29, "PATCH": 30, "SOLEN": 31,
     "TRANS": 32, "SROTAT": 33, "YROTAT": 34, "XROTAT": 35, "COOLING": 36, "LRBB": 37, "ELENS": 38,
-    "HELENS": 39, "HACDIP": 40, "VACDIP": 41, "HACMULT": 42, "VACMULT": 43,
+    "HELENS": 39, "HACDIP": 40, "VACDIP": 41, "HACMULT": 42, "VACMULT": 43, "WIGGLER": 44,
 }
 TRACK_SPIN = 1
 TRACK_RADIAL = 2

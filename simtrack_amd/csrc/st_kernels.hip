@@ -688,6 +688,83 @@ ST_DEV void apply_op(St &s, Ptc &c, const st_op &op, st_pp par, st_gref g,
             drift<HEXP>(s, P[0], P[1], c);
         }
         break;
+    case K_WIGGLER: { /* Wiggler_Pass_Forest cpp:4400-4452 (Pass), Wiggler_Pass_Wu cpp:4516-4572 (sPass) */
+        const double h = P[0], kx = P[9], ky = P[10], kz = P[11], phiz0 = P[12], brho = P[13];
+        if (!SPIN) {
+            for (int i = 0; i < op.nint; i++) {
+                const double z = h * i;
+                double cz, sz, cx, sx, chy, shy;
+                st_sincos(kz * z + phiz0, &sz, &cz);
+                st_sincos(kx * x[0], &sx, &cx);
+                st_coshsinh(ky * x[2], &chy, &shy);
+                /* get_Axy_wiggler (cpp:4371-4397), then divided by brho */
+                const double Ax0 = P[1] * cx * chy * sz / brho, Ay0 = P[2] * sx * shy * sz / brho;
+                const double Ax1 = P[3] * sx * chy * sz / brho, Ay1 = P[4] * cx * shy * sz / brho;
+                const double Ax2 = P[5] * cx * shy * sz / brho, Ay2 = P[6] * sx * chy * sz / brho;
+                const double x00 = x[0], x10 = x[1], x20 = x[2], x30 = x[3];
+                const double psi = 1.0 + x[5], hodp = h / psi;
+                const double det = ((1 - Ax1 * hodp) * (1 - Ay2 * hodp) - Ax2 * Ay1 * hodp * hodp);
+                const double x01 = Ax0 * Ax1, x02 = Ax0 * Ax2, y01 = Ay0 * Ay1, y02 = Ay0 * Ay2;
+                x[1] = (x10 - (x01 + y01) * hodp) * (1 - Ay2 * hodp) + (x30 - (x02 + y02) * hodp) * Ay1 * hodp;
+                x[1] = x[1] / det;
+                x[3] = (x30 - (x02 + y02) * hodp) * (1 - Ax1 * hodp) + (x10 - (x01 + y01) * hodp) * Ax2 * hodp;
+                x[3] = x[3] / det;
+                x[0] = x00 + hodp * (x[1] - Ax0);
+                x[2] = x20 + hodp * (x[3] - Ay0);
+                const double t1 = (x[1] - Ax0) / psi, t2 = (x[3] - Ay0) / psi;
+                x[4] = x[4] - h * (t1 * t1 + t2 * t2) / 2.0;
+                if (RAD) {
+                    const double Ax3 = P[7] * cx * chy * cz / brho, Ay3 = P[8] * sx * shy * cz / brho;
+                    radiate(s, h, 0, -Ay3 * brho * h, Ax3 * brho * h, (Ay1 - Ax2) * brho * h, c, g);
+                }
+            }
+        } else {
+            /* Wu's splitting: six half kicks (y, y, x, x, y, y) around three
+               drifts; one field evaluation in a loop over the kicks */
+            double z = 0.0;
+            for (int i = 0; i < op.nint; i++) {
+                const double hd = h / (1.0 + x[5]);
+                z += 0.5 * h;
+                const double sz = st_sin(kz * z + phiz0);
+#pragma unroll 1
+                for (int k = 0; k < 6; k++) {
+                    const bool ykick = k < 2 || k > 3;
+                    const double sgn = (k & 1) ? 1.0 : -1.0;
+                    double cx, sx, chy, shy;
+                    st_sincos(kx * x[0], &sx, &cx);
+                    st_coshsinh(ky * x[2], &chy, &shy);
+                    double d1, d3;
+                    if (ykick) { /* get_Ay_Wu: AyoBrho, IntpAypxdy */
+                        d3 = P[2] * sx * shy * sz;
+                        d1 = P[15] * cx * chy * sz;
+                    } else {     /* get_Ax_Wu: AxoBrho, IntpAxpydx */
+                        d1 = P[1] * cx * chy * sz;
+                        d3 = P[14] * (kx == 0. ? x[0] : sx) * shy * sz;
+                    }
+                    /* x -+ d/brho written as x + (+-1)(d/brho): the same bits */
+                    x[1] = x[1] + sgn * (d1 / brho);
+                    x[3] = x[3] + sgn * (d3 / brho);
+                    if (k == 0 || k == 4) {
+                        x[2] = x[2] + 0.5 * hd * x[3];
+                        x[4] = x[4] - hd * x[3] * x[3] / (1.0 + x[5]) / (1.0 + x[5]) / 4;
+                    } else if (k == 2) {
+                        x[0] = x[0] + hd * x[1];
+                        x[4] = x[4] - 0.5 * hd * x[1] * x[1] / (1.0 + x[5]);
+                    }
+                }
+                z += 0.5 * h;
+                if (RAD) { /* get_wiggler_B (cpp:4497-4514) times the step */
+                    double cx, sx, chy, shy, cz, szp;
+                    st_sincos(kx * x[0], &sx, &cx);
+                    st_coshsinh(ky * x[2], &chy, &shy);
+                    st_sincos(kz * x[4] + phiz0, &szp, &cz);
+                    radiate(s, h, 0.0, P[16] * sx * shy * cz * h, P[7] * cx * chy * cz * h,
+                            P[17] * cx * shy * szp * h, c, g);
+                }
+            }
+        }
+        break;
+    }
     case K_ACDIP:  /* ACDIP_Pass cpp:5930-5950, ACDIP_sPass cpp:8251-8284; GP.turn = turn */
     case K_ACMULT: /* ACMULT_Pass cpp:5895-5928, ACMULT_sPass cpp:8210-8249 */
     case K_SOLEN:  /* SOLEN_Pass cpp:5804-5857, SOLEN_sPass cpp:8118-8177 */

@@ -920,6 +920,40 @@ struct Builder {
             t.ops.push_back(op);
             return 0;
         }
+        case ST_WIGGLER: {
+            /* WIGGLER::Pass (cpp:9755): GtoL, Wiggler_Pass_Forest (cpp:4400), LtoG;
+               sPass (cpp:9767): Wiggler_Pass_Wu (cpp:4516), no spin rotation.
+               Coefficients as get_Axy_wiggler (cpp:4371), get_Ax_Wu/get_Ay_Wu
+               (cpp:4456-4494) and get_wiggler_B (cpp:4497) group them */
+            if (e.nint < 1) {
+                err = "WIGGLER Nint must be at least 1";
+                return -3;
+            }
+            st_op op = new_op(K_WIGGLER);
+            op.nint = e.nint;
+            const double b0 = e.b0, kx = e.kx, kz = e.kz, ky = std::sqrt(kx * kx + kz * kz);
+            put(e.l / e.nint);
+            put(b0 / kz);
+            put(b0 * kx / ky / kz);
+            put(-(b0 * kx / kz));
+            put(b0 * kx * kx / ky / kz);
+            put(b0 * ky / kz);
+            put(b0 * kx / kz);
+            put(b0);
+            put(b0 * kx / ky);
+            put(kx);
+            put(ky);
+            put(kz);
+            put(e.phiz0);
+            put(gp.brho);
+            put(kx == 0. ? b0 * ky / kz : b0 * ky / kx / kz);
+            put(b0 * kx * kx / ky / ky / kz);
+            put((kx / ky) * b0);
+            put(-(kz * b0 / ky));
+            misalign(op, e, e.l, 0.);
+            t.ops.push_back(op);
+            return 0;
+        }
         case ST_MATRIX: {
             st_op op = new_op(K_MATRIX);
             for (int i = 0; i < 6; i++) put(e.xco_in[i]);

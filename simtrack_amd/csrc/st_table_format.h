@@ -74,6 +74,10 @@ enum st_kind {
                          (norder = Norder)                                        */
     K_SOLEN,          /* 0 L, 1 L/Nint, 2 q*Bz/2, 3 m0, 4 p, 5 L/beta/c,
                          6 Bz*Lint/brho (nint = Nint)                             */
+    K_WIGGLER,        /* 0 h = L/Nint, 1..8 the field expansion's coefficients,
+                         9 kx, 10 ky, 11 kz, 12 phiz0, 13 brho, 14 Wu's Intp
+                         Ax coefficient, 15 Wu's Intp Ay coefficient,
+                         16 (kx/ky)*b0, 17 -(kz*b0/ky) (nint = Nint)              */
     K_COUNT
 };
 

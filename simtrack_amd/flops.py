@@ -205,6 +205,15 @@ def element_ops(r, gp, spin=False, turn=0):
     if name == "PATCH":
         # TRANS_Pass 3; YROTAT_Pass: pz 5 (sqrt), temp 3, x 2, px 3, y 5, z 7
         return 3 + (0 if abs(r["theta"]) < 1e-9 else 24)
+    if name == "WIGGLER":
+        # per step: Forest's implicit step 85 (Pass) or Wu's six field
+        # evaluations 116 (112 when KX = 0: x itself replaces sin(KX x)) (sPass);
+        # with radiation the step's field and radiate()
+        if spin:
+            per = (116 if r["kx"] != 0. else 112) + (71 + PT_UPDATE if rad else 0)
+        else:
+            per = 85 + (60 + PT_UPDATE if rad else 0)
+        return nint * per + mis
     if name == "SOLEN":
         if r["ks"] == 0.:
             return drift_l(L) + mis

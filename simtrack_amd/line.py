@@ -658,6 +658,26 @@ class VACMULT(HACMULT):
     _k = "KSLMAX"
 
 
+class WIGGLER(Element):
+    """WIGGLER (simtrack.cpp:9707-9771): a sinusoidal wiggler field of peak
+    B0 [T], wave numbers KX, KZ and phase PHIZ0; Pass integrates with
+    Forest's implicit step (cpp:4400), sPass with Wu's splitting (cpp:4516)
+    and leaves the spin unrotated."""
+    TYPE = "WIGGLER"
+    _params = {"Nint": "Nint", "B0": "B0", "KX": "KX", "KZ": "KZ", "PHIZ0": "PHIZ0"}
+
+    def __init__(self, name, l, nint, b0, kx, kz, phiz0):
+        super().__init__(name)
+        if not l > 0:
+            raise ValueError("Wiggler: L must be non-zero.")
+        self.L, self.Nint = float(l), int(nint)
+        self.B0, self.KX, self.KZ, self.PHIZ0 = float(b0), float(kx), float(kz), float(phiz0)
+
+    def _fill(self, r):
+        r["nint"] = self.Nint
+        r["b0"], r["kx"], r["kz"], r["phiz0"] = self.B0, self.KX, self.KZ, self.PHIZ0
+
+
 # -------------------------------------------------------------------- line
 
 class Line:

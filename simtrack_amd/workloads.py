@@ -79,7 +79,9 @@ def all_types_line(seed=7, misalign=True, with_bb=True, with_rf=True, with_gsben
                 L.HELENS("HE", 2.0, 5e11, 1.0, 2, 0.5, 1e-2, 1e-6)]
     # AC elements: at turn 0 HAC is inside its ramp and VAC past it
     els += [L.HACDIP("HAC", 0.3, 2e-5, 0.31, 0.2, -3, 5), L.VACDIP("VAC", 0.0, 1e-5, 0.27, 0.0, -10, -2),
-            L.HACMULT("HAM", 0.2, 2, 0.05, 100, 0.3), L.VACMULT("VAM", 0.0, 0, 1e-5, 50, 0.1)]
+            L.HACMULT("HAM", 0.2, 2, 0.05, 100, 0.3), L.VACMULT("VAM", 0.0, 0, 1e-5, 50, 0.1),
+            L.WIGGLER("WIG", 0.8, 8, 0.5, 2.0, 2 * math.pi / 0.2, 0.1),
+            L.WIGGLER("WIP", 0.6, 6, 0.3, 0.0, 2 * math.pi / 0.15, 0.0)]
     for e in els:
         if hasattr(e, "Nint") and e.L > 0:
             e.Nint = max(1, min(e.Nint, 5))

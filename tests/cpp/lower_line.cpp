@@ -70,6 +70,7 @@ int main()
     l.Append(new VACDIP("VAC", 0.0, 1e-5, 0.27, 0.0, -10, -2));
     l.Append(new HACMULT("HAM", 0.2, 2, 0.05, 100, 0.3));
     l.Append(new VACMULT("VAM", 0.0, 0, 1e-5, 50, 0.1));
+    l.Append(new WIGGLER("WIG", 0.8, 8, 0.5, 2.0, 31.4, 0.1));
     /* edits through the reference's parameter names */
     l.Cell[7]->SetP("K1L", 0.06);
     l.Cell[7]->SetP("Nint", 7.9);

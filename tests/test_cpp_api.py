@@ -39,6 +39,7 @@ def python_line():
         st.ELENS("EL", 2.0, 1e11, 1.0, 3, 0.5, 3e-4, 2e-4), st.HELENS("HE", 2.0, 5e11, 1.0, 2, 0.5, 1e-2, 1e-6),
         st.HACDIP("HAC", 0.3, 2e-5, 0.31, 0.2, -3, 5), st.VACDIP("VAC", 0.0, 1e-5, 0.27, 0.0, -10, -2),
         st.HACMULT("HAM", 0.2, 2, 0.05, 100, 0.3), st.VACMULT("VAM", 0.0, 0, 1e-5, 50, 0.1),
+        st.WIGGLER("WIG", 0.8, 8, 0.5, 2.0, 31.4, 0.1),
     ]
     ln = st.Line(els)
     ln.Cell[7].SetP("K1L", 0.06)
@@ -68,7 +69,7 @@ def test_cpp_lowering_matches_python(tmp_path):
     got = np.frombuffer(r.stdout, dtype=st.ELEMENT_DTYPE)
     ln = python_line()
     ref = ln.records()
-    assert len(got) == len(ref) == 41
+    assert len(got) == len(ref) == 42
     for i in range(len(ref)):
         assert got[i].tobytes() == ref[i].tobytes(), ln.Cell[i].TYPE
     length = float(r.stderr.split()[1])

@@ -1,8 +1,10 @@
-"""GPU parity of the frame-change, cooling, long-range beam-beam, e-lens, AC
-and solenoid elements inside a ring: multi-turn Track and Track_spin bit for bit
+"""GPU parity of the frame-change, cooling, long-range beam-beam, e-lens, AC,
+solenoid and wiggler elements inside a ring: multi-turn Track and Track_spin bit for bit
 against the oracle (the AC elements read the turn number, so the
 comparison runs over the ramp and past it), and the same result when the
 turns are split over two device calls (the kernels take the absolute turn)."""
+import math
+
 import numpy as np
 import pytest
 import torch
@@ -26,7 +28,7 @@ def _ring():
              st.HELENS("HE", 0.5, 3e11, 1.0, 2, 0.4, 4e-3, 1e-3),
              st.HACDIP("HAC", 0.3, 2e-6, 0.27, 0.1, 3, 12), st.VACDIP("VAC", 0.2, 1.5e-6, 0.31, 0.0, 5, 9),
              st.HACMULT("HAM", 0.2, 2, 0.02, 7, 0.2), st.VACMULT("VAM", 0.1, 0, 1e-6, 5, 0.3),
-             st.SOLEN("SOL", 0.5, 0.02)]
+             st.SOLEN("SOL", 0.5, 0.02), st.WIGGLER("WIG", 0.4, 8, 0.2, 3.0, 2 * math.pi / 0.1, 0.1)]
     for i, e in enumerate(extra):
         e.APx = e.APy = 0.05
         if e.TYPE in ("LRBB", "ELENS", "HACDIP", "HACMULT") and i % 2:

@@ -44,7 +44,22 @@ def test_example_lattice_bitwise(ctx, which, spin):
     assert_bitwise(xg, xo, f"{which} {'spin' if spin else 'orbit'}")
 
 
-def test_solenoid_with_field_is_rejected(ctx):
+@pytest.mark.parametrize("spin", [False, True])
+def test_detector_solenoid_bitwise(ctx, spin):
+    """A detector-like solenoid (the HSR energy, KS = 0.1 /m over 2 m):
+    SOLEN_Pass / SOLEN_sPass on the GPU equal the oracle bit for bit."""
     gp = W.proton_globals(275000.)
-    with pytest.raises(st.STError, match="SOLEN"):
-        st.Table(ctx, gp, st.Line([st.SOLEN("S", 2.0, 0.1)]).records())
+    rec = st.Line([st.DRIFT("D", 1.0), st.SOLEN("S", 2.0, 0.1), st.DRIFT("D", 1.0)]).records()
+    x = W.random_particles(256, seed=7, spin=spin, scale=1e-3)
+    xg, sg, ltg, lpg = st.Table(ctx, gp, rec).track(x.copy(), 5, spin=spin)
+    xo, so, lto, lpo = pyoracle.Oracle(gp, rec).track(x.copy(), 5, spin=spin)
+    assert np.array_equal(sg, so)
+    assert_bitwise(xg, xo, f"solenoid {'spin' if spin else 'orbit'}")
+
+
+def test_unsupported_type_is_rejected(ctx):
+    gp = W.proton_globals(275000.)
+    rec = st.Line([st.DRIFT("D", 1.0)]).records()
+    rec["type"] = 99
+    with pytest.raises(st.STError, match="type"):
+        st.Table(ctx, gp, rec)

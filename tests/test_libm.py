@@ -52,7 +52,11 @@ CASES = {
                                                1.1875, np.nextafter(1.1875, 0), 2.4375, np.nextafter(2.4375, 0),
                                                1.0, 0.5, 1.5, 2.0 ** 66, 2.0 ** -27, 1e-300]])),
 }
-BOUND = {"orc_tan": 2.0}   # tan = sin/cos: two roundings plus the quotient
+CASES["orc_cosh"] = (mpmath.cosh, np.concatenate([RNG.uniform(-20, 20, 600), RNG.uniform(-1, 1, 300), [0.0, 1e-300]]))
+CASES["orc_sinh"] = (mpmath.sinh, np.concatenate([RNG.uniform(-20, 20, 500), RNG.uniform(-0.6, 0.6, 400),
+                                                  [0.5, np.nextafter(0.5, 0), 1e-9, -1e-200]]))
+BOUND = {"orc_tan": 2.0,    # tan = sin/cos: two roundings plus the quotient
+         "orc_cosh": 2.0, "orc_sinh": 2.0}   # (e^x +- e^-x)/2 from the correctly rounded exp
 
 
 @pytest.mark.parametrize("name", list(CASES))

@@ -295,3 +295,53 @@ def test_solenoid_spin_precesses_by_one_plus_g_times_ks(nint):
     assert abs(y[0, 7]) == pytest.approx(math.sin(ang), abs=1e-13)
     assert y[0, 8] == 0.
     np.testing.assert_allclose(np.linalg.norm(y[0, 6:9]), 1., atol=1e-15)
+
+
+def test_wiggler_forest_step_is_symplectic():
+    S = np.zeros((6, 6))
+    for i in range(3):
+        S[2 * i, 2 * i + 1], S[2 * i + 1, 2 * i] = 1., -1.
+    for kx in (2.0, 0.0):
+        J = _jacobian(st.WIGGLER("W", 0.4, 8, 1.5, kx, 2 * PI / 0.1, 0.2),
+                      np.array([3e-4, -2e-5, 1e-4, 4e-5, 0.01, 2e-4]))
+        np.testing.assert_allclose(J.T @ S @ J, S, rtol=0, atol=2e-8)
+
+
+def test_wiggler_integrators_agree():
+    """Forest's implicit step (Pass) and Wu's splitting (sPass) integrate the
+    same field: with fine steps they converge to one map, so a wrong field
+    coefficient in either would show; with B0 = 0 and pt = 0 both are the
+    same expanded drift."""
+    x = particles(6, scale=5e-4)
+    x[:, 5] = 0.
+    for kx in (3.0, 0.0):
+        el = st.WIGGLER("W", 0.3, 600, 1.2, kx, 2 * PI / 0.1, 0.3)
+        a = orc(el).element_pass(0, x)
+        xs = np.zeros((len(x), 9))
+        xs[:, :6] = x
+        xs[:, 7] = 1.
+        b = orc(el).element_pass(0, xs, spin=True)
+        np.testing.assert_allclose(b[:, :4], a[:, :4], rtol=0, atol=2e-10)
+        assert np.array_equal(b[:, 6:], xs[:, 6:]), "sPass leaves the spin unrotated"
+    el0 = st.WIGGLER("W0", 0.5, 4, 0., 3.0, 2 * PI / 0.1, 0.)
+    a = orc(el0).element_pass(0, x)
+    np.testing.assert_allclose(a[:, 0], x[:, 0] + 0.5 * x[:, 1], rtol=1e-14, atol=1e-20)
+    np.testing.assert_allclose(a[:, 2], x[:, 2] + 0.5 * x[:, 3], rtol=1e-14, atol=1e-20)
+    np.testing.assert_allclose(a[:, 4], x[:, 4] - 0.5 * (x[:, 1] ** 2 + x[:, 3] ** 2) / 2, rtol=1e-12, atol=1e-22)
+
+
+def test_planar_wiggler_focuses_vertically():
+    """A planar wiggler (KX = 0) focuses vertically with k = (B0/Brho)^2/2
+    on average over its periods (the natural wiggler focusing)."""
+    g = W.electron_globals(18000.0)
+    B0, lam, L = 1.2, 0.1, 2.0
+    el = st.WIGGLER("W", L, 400, B0, 0.0, 2 * PI / lam, 0.0)
+    o = pyoracle.Oracle(g, st.Line([el]).records())
+    h = 1e-6
+    x = np.zeros((2, 6))
+    x[0, 2], x[1, 2] = h, -h
+    y = o.element_pass(0, x)
+    dpy_dy = (y[0, 3] - y[1, 3]) / (2 * h)
+    k = (B0 / g.brho) ** 2 / 2
+    assert dpy_dy == pytest.approx(-k * L, rel=0.05)
+    assert abs((y[0, 1] - y[1, 1]) / (2 * h)) < 1e-3 * k * L, "no x-y coupling in a planar wiggler"
