@@ -620,6 +620,14 @@ int st_zboundary_slices_gaussian(double *zb, int Nslice, double sigmal);
  * st_bb_pic_6d: BeamBeam6D_PIC (cpp:23380): the z-sorted slicing and the
  *   collision order of st_bb_soft_gaussian_6d, st_bb_pic_4d per slice pair
  *   with charges Np/Nslice.
+ * st_bb_pic_4d_nonoffset: BeamBeam4D_PIC_NonOffset (cpp:24226): the grid is
+ *   given, Nx x Ny cells of hx x hy centred on the origin (no moments, no
+ *   centroid offsets); the kick takes the field from the potential by the
+ *   same stencil (Interpolate_Electric_Field_1Potential, cpp:3047); the
+ *   window is 29 cells around the origin.  The reference's caller passes
+ *   the Green function of that grid (grn_c_r, grn_c_i); here it is computed
+ *   from hx, hy (Cal_Effective_Green_Function with zero offsets, cpp:2881).
+ *   Nx = Ny = 64 only (else ST_ERR_PARAM), as the solver's FFT is.
  */
 int st_bb_pic_4d(st_ctx *ctx, double Np1, int Mp1, double *d_x1, double Np2, int Mp2, double *d_x2, double gamma1,
                  double gamma2, double bbscale1, double bbscale2);
@@ -627,6 +635,12 @@ int st_bb_pic_6d(st_ctx *ctx, double Np1, int Mp1, double *d_x1, double Np2, int
                  double gamma2, double bbscale1, double bbscale2, int Nslice1, int Nslice2, double *Z1, double *Z2);
 int st_bb_pic_4d_host(st_ctx *ctx, double Np1, int Mp1, double *x1, double Np2, int Mp2, double *x2, double gamma1,
                       double gamma2, double bbscale1, double bbscale2);
+int st_bb_pic_4d_nonoffset(st_ctx *ctx, double Np1, int Mp1, double *d_x1, double Np2, int Mp2, double *d_x2,
+                           double gamma1, double gamma2, double bbscale1, double bbscale2, int Nx, int Ny, double hx,
+                           double hy);
+int st_bb_pic_4d_nonoffset_host(st_ctx *ctx, double Np1, int Mp1, double *x1, double Np2, int Mp2, double *x2,
+                                double gamma1, double gamma2, double bbscale1, double bbscale2, int Nx, int Ny,
+                                double hx, double hy);
 int st_bb_pic_6d_host(st_ctx *ctx, double Np1, int Mp1, double *x1, double Np2, int Mp2, double *x2, double gamma1,
                       double gamma2, double bbscale1, double bbscale2, int Nslice1, int Nslice2, double *Z1,
                       double *Z2);

@@ -20,6 +20,7 @@
  *   BeamBeam4D_SoftGaussian, BeamBeam6D_SoftGaussian   simtrack.cpp:22846, 22874, 23134
  *   (both forms), Zboundary_slices_Gaussian             simtrack.cpp:3157
  *   BeamBeam4D_PIC, BeamBeam6D_PIC                     simtrack.cpp:23286, 23380
+ *   BeamBeam4D_PIC_NonOffset                           simtrack.cpp:24226
  *   Cal_OneTurnMap (matrix only), Track_tbt_ActionDiff simtrack.cpp:15122, 21309
  *   Cal_Twiss, Cal_Dispersion, Cal_Chrom, Print_Twiss   simtrack.cpp:15540, 17049, 16964, 19846
  *
@@ -2206,6 +2207,23 @@ inline void BeamBeam4D_PIC(double Np1, int Mp1, double xc1[], double Np2, int Mp
 {
     check(st_bb_pic_4d_host(default_context(), Np1, Mp1, xc1, Np2, Mp2, xc2, gamma1, gamma2, bbscale1, bbscale2),
           "BeamBeam4D_PIC");
+}
+
+/* BeamBeam4D_PIC_NonOffset (simtrack.cpp:24226-24280): particle-in-cell
+   kicks on the given grid centred on the origin, in place.  The grid's Green
+   function is computed from hx, hy (Cal_Effective_Green_Function with zero
+   offsets, what the reference's caller passes in grn_c_r, grn_c_i); the two
+   arrays are not read.  Nx = Ny = 64. */
+inline void BeamBeam4D_PIC_NonOffset(double Np1, int Mp1, double xc1[], double Np2, int Mp2, double xc2[],
+                                     double gamma1, double gamma2, double bbscale1, double bbscale2, int Nx, int Ny,
+                                     double hx, double hy, const double *grn_c_r = nullptr,
+                                     const double *grn_c_i = nullptr)
+{
+    (void)grn_c_r;
+    (void)grn_c_i;
+    check(st_bb_pic_4d_nonoffset_host(default_context(), Np1, Mp1, xc1, Np2, Mp2, xc2, gamma1, gamma2, bbscale1,
+                                      bbscale2, Nx, Ny, hx, hy),
+          "BeamBeam4D_PIC_NonOffset");
 }
 
 /* BeamBeam6D_PIC (simtrack.cpp:23380-23615): beams sorted by z in place,

@@ -49,6 +49,8 @@ def load(glibc=False):
     lib.orc_bb_soft_gaussian_6d_zb.argtypes = [D, I, P, D, I, P, D, D, D, D, I, I, P, P]
     lib.orc_zboundary_slices_gaussian.argtypes = [P, I, D]
     lib.orc_bb_pic_4d.argtypes = [D, I, P, D, I, P, D, D, D, D]
+    lib.orc_bb_pic_4d_nonoffset.argtypes = [D, I, P, D, I, P, D, D, D, D, D, D]
+    lib.orc_bb_pic_4d_fixed_gridfield.argtypes = [D, I, P, D, I, P, D, D, D, D, D, D]
     lib.orc_bb_pic_6d.argtypes = [D, I, P, D, I, P, D, D, D, D, I, I]
     lib.orc_fft2d_128.argtypes = [P, P, I]
     lib.orc_bb_slice_centers.argtypes = [I, D, P]
@@ -295,6 +297,22 @@ def bb4d(x, y, gamma, N, sigmax, sigmay):
     a, b = ctypes.c_double(), ctypes.c_double()
     load().orc_bb4d(x, y, gamma, N, sigmax, sigmay, ctypes.byref(a), ctypes.byref(b))
     return a.value, b.value
+
+
+def bb_pic_4d_nonoffset(Np1, x1, Np2, x2, gamma1, gamma2, bbscale1, bbscale2, hx, hy):
+    """BeamBeam4D_PIC_NonOffset (simtrack.cpp:24226) on the 64 x 64 grid: new (x1, x2)."""
+    a, b = _beam(x1), _beam(x2)
+    load().orc_bb_pic_4d_nonoffset(Np1, len(a), _p(a), Np2, len(b), _p(b), gamma1, gamma2, bbscale1, bbscale2,
+                                   hx, hy)
+    return a, b
+
+
+def bb_pic_4d_fixed_gridfield(Np1, x1, Np2, x2, gamma1, gamma2, bbscale1, bbscale2, hx, hy):
+    """The fixed-grid collision by the full-grid field path (a check of bb_pic_4d_nonoffset)."""
+    a, b = _beam(x1), _beam(x2)
+    load().orc_bb_pic_4d_fixed_gridfield(Np1, len(a), _p(a), Np2, len(b), _p(b), gamma1, gamma2, bbscale1,
+                                         bbscale2, hx, hy)
+    return a, b
 
 
 def bb_pic_4d(Np1, x1, Np2, x2, gamma1, gamma2, bbscale1, bbscale2):

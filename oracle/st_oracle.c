@@ -3926,8 +3926,8 @@ EXPORT void orc_fft2d_128(double *x, double *y, int dir) { pic_fft2d(x, y, 128, 
 /* the field of one beam's charge (Charge2Grid, Cal_Electric_PIC with
    Cal_Potential_Convolution and Cal_Electric_Field, cpp:2793-3045) on the
    grid at (x0, y0), as seen at the other beam's offset */
-static void pic_field(int m, const double *xc, const double mean[2], double x0, double y0, double xoff, double yoff,
-                      double hx, double hy, double *Ex, double *Ey)
+static void pic_potential(int m, const double *xc, const double mean[2], double x0, double y0, double xoff,
+                          double yoff, double hx, double hy, double *phi)
 {
     const int Nx = PIC_NX, Ny = PIC_NY, N2 = 4 * PIC_NX * PIC_NY;
     const double eps0 = 8.854187817e-12;
@@ -3935,7 +3935,6 @@ static void pic_field(int m, const double *xc, const double mean[2], double x0, 
     double *gr = (double *)calloc((size_t)N2, sizeof(double)), *gi = (double *)calloc((size_t)N2, sizeof(double));
     double *sr = (double *)calloc((size_t)N2, sizeof(double)), *si = (double *)calloc((size_t)N2, sizeof(double));
     double *pr = (double *)calloc((size_t)N2, sizeof(double)), *pi = (double *)calloc((size_t)N2, sizeof(double));
-    double *phi = (double *)calloc((size_t)Nx * Ny, sizeof(double));
     int xm[3], ym[3];
     double xw[3], yw[3];
     for (int k = 0; k < m; k++) {
@@ -3958,6 +3957,21 @@ static void pic_field(int m, const double *xc, const double mean[2], double x0, 
     pic_fft2d(pr, pi, 2 * Nx, 7, -1);
     for (int i = 0; i < Nx; i++)
         for (int j = 0; j < Ny; j++) phi[i * Nx + j] = pr[i * 2 * Nx + j] * 2 * Nx * 2 * Ny / 2 / 3.14159265 / eps0;
+    free(src);
+    free(gr);
+    free(gi);
+    free(sr);
+    free(si);
+    free(pr);
+    free(pi);
+}
+
+static void pic_field(int m, const double *xc, const double mean[2], double x0, double y0, double xoff, double yoff,
+                      double hx, double hy, double *Ex, double *Ey)
+{
+    const int Nx = PIC_NX, Ny = PIC_NY;
+    double *phi = (double *)calloc((size_t)Nx * Ny, sizeof(double));
+    pic_potential(m, xc, mean, x0, y0, xoff, yoff, hx, hy, phi);
     for (int i = 0; i < Nx * Ny; i++) Ex[i] = Ey[i] = 0.;
     for (int i = 1; i < Nx - 1; i++)
         for (int j = 1; j < Ny - 1; j++) {
@@ -3970,13 +3984,6 @@ static void pic_field(int m, const double *xc, const double mean[2], double x0, 
                                (phi[(i - 1) * Nx + j + 1] - phi[(i - 1) * Nx + j - 1])) /
                              12. / hy;
         }
-    free(src);
-    free(gr);
-    free(gi);
-    free(sr);
-    free(si);
-    free(pr);
-    free(pi);
     free(phi);
 }
 
@@ -4028,6 +4035,96 @@ EXPORT void orc_bb_pic_4d(double Np1, int Mp1, double *x1, double Np2, int Mp2, 
     yoff = m1[1] - m2[1];
     pic_field(Mp2, x2, m2, x0, y0, xoff, yoff, hx, hy, Ex, Ey);
     pic_kick(Mp1, x1, m1, x0, y0, xoff, yoff, hx, hy, Ex, Ey, Np2, Mp2, gamma1, bbscale1);
+    free(Ex);
+    free(Ey);
+}
+
+/* Interpolate_Electric_Field_1Potential (cpp:3047-3100): the field at the
+   3 x 3 weighted nodes by the same 4th-order stencil, taken from a 5 x 5
+   patch of the potential around them, then the weighted sum */
+static void pic_field_1potential(const double *phi, double x0, double y0, double hx, double hy, double x, double y,
+                                 double *ex, double *ey)
+{
+    const int Nx = PIC_NX;
+    int xm[3], ym[3];
+    double xw[3], yw[3], p25[25], Ex25[25], Ey25[25];
+    pic_grid_weight(x, y, x0, y0, hx, hy, xm, ym, xw, yw);
+    int i0 = 10000, j0 = 10000;
+    for (int i = 0; i < 3; i++) {
+        if (xm[i] < i0) i0 = xm[i];
+        if (ym[i] < j0) j0 = ym[i];
+    }
+    for (int i = 0; i < 25; i++) p25[i] = Ex25[i] = Ey25[i] = 0.;
+    for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++) p25[5 * i + j] = phi[Nx * (i0 - 1 + i) + (j0 - 1 + j)];
+    for (int i = 1; i < 4; i++)
+        for (int j = 1; j < 4; j++) {
+            Ex25[i * 5 + j] = -((p25[(i + 1) * 5 + j + 1] - p25[(i - 1) * 5 + j + 1]) +
+                                4 * (p25[(i + 1) * 5 + j] - p25[(i - 1) * 5 + j]) +
+                                (p25[(i + 1) * 5 + j - 1] - p25[(i - 1) * 5 + j - 1])) /
+                              12. / hx;
+            Ey25[i * 5 + j] = -((p25[(i + 1) * 5 + j + 1] - p25[(i + 1) * 5 + j - 1]) +
+                                4 * (p25[i * 5 + j + 1] - p25[i * 5 + j - 1]) +
+                                (p25[(i - 1) * 5 + j + 1] - p25[(i - 1) * 5 + j - 1])) /
+                              12. / hy;
+        }
+    double fx = 0., fy = 0.;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            fx = fx + Ex25[(xm[i] - i0 + 1) * 5 + (ym[j] - j0 + 1)] * xw[i] * yw[j];
+            fy = fy + Ey25[(xm[i] - i0 + 1) * 5 + (ym[j] - j0 + 1)] * xw[i] * yw[j];
+        }
+    *ex = fx;
+    *ey = fy;
+}
+
+/* BeamBeam4D_PIC_NonOffset, simtrack.cpp:24226-24280: a fixed 64 x 64 grid
+   of cells hx x hy centred on the origin (no centroid offsets), the Green
+   function of the unshifted grid (Cal_Effective_Green_Function with zero
+   offsets, which the reference's caller computes once and passes in), the
+   field interpolated from the potential; beam 2 kicked by beam 1, then
+   beam 1 by beam 2; particles deposited or kicked only within 29 cells of
+   the origin */
+EXPORT void orc_bb_pic_4d_nonoffset(double Np1, int Mp1, double *x1, double Np2, int Mp2, double *x2, double gamma1,
+                                    double gamma2, double bbscale1, double bbscale2, double hx, double hy)
+{
+    const int Nx = PIC_NX, Ny = PIC_NY;
+    const double x0 = -Nx * hx / 2, y0 = -Ny * hy / 2, zero[2] = {0., 0.};
+    double *phi = (double *)calloc((size_t)Nx * Ny, sizeof(double));
+    for (int dir = 0; dir < 2; dir++) {
+        const int ms = dir ? Mp2 : Mp1, mt = dir ? Mp1 : Mp2;
+        const double *xs = dir ? x2 : x1, Ns = dir ? Np2 : Np1;
+        double *xt = dir ? x1 : x2;
+        const double gt = dir ? gamma1 : gamma2, kt = dir ? bbscale1 : bbscale2;
+        pic_potential(ms, xs, zero, x0, y0, 0., 0., hx, hy, phi);
+        for (int k = 0; k < mt; k++) {
+            const double xp = xt[6 * k], yp = xt[6 * k + 2];
+            if (fabs(xp) < (Nx / 2 - 3) * hx && fabs(yp) < (Ny / 2 - 3) * hy) {
+                double ex, ey;
+                pic_field_1potential(phi, x0, y0, hx, hy, xp, yp, &ex, &ey);
+                xt[6 * k + 1] = xt[6 * k + 1] + ex * 1.60217662e-19 * kt / (gt * 938.27201e6) * (Ns / ms);
+                xt[6 * k + 3] = xt[6 * k + 3] + ey * 1.60217662e-19 * kt / (gt * 938.27201e6) * (Ns / ms);
+            }
+        }
+    }
+    free(phi);
+}
+
+/* The same fixed-grid collision by the full-grid field of BeamBeam4D_PIC
+   (Cal_Electric_Field, then Interpolate_Electric_Field): a check of the
+   potential interpolation above, which must equal it wherever the window
+   keeps the stencil inside the grid */
+EXPORT void orc_bb_pic_4d_fixed_gridfield(double Np1, int Mp1, double *x1, double Np2, int Mp2, double *x2,
+                                          double gamma1, double gamma2, double bbscale1, double bbscale2, double hx,
+                                          double hy)
+{
+    const int Nx = PIC_NX, Ny = PIC_NY;
+    const double x0 = -Nx * hx / 2, y0 = -Ny * hy / 2, zero[2] = {0., 0.};
+    double *Ex = (double *)malloc(sizeof(double) * Nx * Ny), *Ey = (double *)malloc(sizeof(double) * Nx * Ny);
+    pic_field(Mp1, x1, zero, x0, y0, 0., 0., hx, hy, Ex, Ey);
+    pic_kick(Mp2, x2, zero, x0, y0, 0., 0., hx, hy, Ex, Ey, Np1, Mp1, gamma2, bbscale2);
+    pic_field(Mp2, x2, zero, x0, y0, 0., 0., hx, hy, Ex, Ey);
+    pic_kick(Mp1, x1, zero, x0, y0, 0., 0., hx, hy, Ex, Ey, Np2, Mp2, gamma1, bbscale1);
     free(Ex);
     free(Ey);
 }

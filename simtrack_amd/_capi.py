@@ -84,7 +84,8 @@ EXPORTS = [
     "st_one_turn_matrix", "st_element_maps", "st_lyapunov_scan", "st_fine_tune_window_device", "st_action_diff",
     "st_bb_soft_gaussian_4d", "st_bb_soft_gaussian_6d", "st_bb_soft_gaussian_6d_zb", "st_bb_soft_gaussian_4d_host",
     "st_bb_soft_gaussian_6d_host", "st_bb_soft_gaussian_6d_zb_host", "st_zboundary_slices_gaussian",
-    "st_bb_pic_4d", "st_bb_pic_6d", "st_bb_pic_4d_host", "st_bb_pic_6d_host", "st_spin_orbit", "st_spin_tune",
+    "st_bb_pic_4d", "st_bb_pic_6d", "st_bb_pic_4d_host", "st_bb_pic_6d_host", "st_bb_pic_4d_nonoffset",
+    "st_bb_pic_4d_nonoffset_host", "st_spin_orbit", "st_spin_tune",
     "st_set_initial_spin", "st_track_dz",
 ]
 
@@ -178,6 +179,8 @@ def load(path=None):
     for f in ("st_bb_soft_gaussian_6d", "st_bb_soft_gaussian_6d_host", "st_bb_soft_gaussian_6d_zb",
               "st_bb_soft_gaussian_6d_zb_host", "st_bb_pic_6d", "st_bb_pic_6d_host"):
         getattr(lib, f).argtypes = [P, D, I, P, D, I, P, D, D, D, D, I, I, P, P]
+    for f in ("st_bb_pic_4d_nonoffset", "st_bb_pic_4d_nonoffset_host"):
+        getattr(lib, f).argtypes = [P, D, I, P, D, I, P, D, D, D, D, I, I, D, D]
     lib.st_zboundary_slices_gaussian.argtypes = [P, I, D]
     lib.st_fma_scan.argtypes = [P, D, D, D, P, D, D, I, D, D, P, I, ctypes.POINTER(I)]
     lib.st_lyapunov_scan.argtypes = [P, D, D, D, I, D, D, P, I, ctypes.POINTER(I)]

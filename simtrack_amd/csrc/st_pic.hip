@@ -465,6 +465,23 @@ hipError_t st_pic_solve_and_kick(const pic_args &a, int s, hipStream_t stream)
     return hipGetLastError();
 }
 
+/* the fixed grid of BeamBeam4D_PIC_NonOffset (cpp:24236-24237): both
+   centroids at the origin (x0 = -Nx hx/2, no offsets), cells hx x hy; the
+   rms entries are not read once hx, hy are set */
+__global__ void k_pic_fixed_params(double *p, double hx, double hy)
+{
+    if (threadIdx.x != 0) return;
+    for (int q = 0; q < 8; q++) p[q] = 0.;
+    p[8] = hx;
+    p[9] = hy;
+}
+
+hipError_t st_pic_fixed_params(double *params, double hx, double hy, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_pic_fixed_params, dim3(1), dim3(64), 0, stream, params, hx, hy);
+    return hipGetLastError();
+}
+
 hipError_t st_pic_params(const double *part, int nb0, int nb1, int m0, int m1, double *params, hipStream_t stream)
 {
     hipLaunchKernelGGL(k_pic_params, dim3(1), dim3(256), 0, stream, part, nb0, nb1, m0, m1, params);

@@ -512,7 +512,8 @@ extern "C" int st_bb_soft_gaussian_6d_zb_host(st_ctx *c, double Np1, int Mp1, do
    (simtrack.cpp:23286-23378): beam 1's charge kicks beam 2, then beam 2's
    charge kicks beam 1 */
 static int pic_pair(st_ctx *c, ss_work &w, double *d_x1, int s1, int m1, double *d_x2, int s2, int m2, double Np1,
-                    double Np2, double gamma1, double gamma2, double bbscale1, double bbscale2, int lane = 0)
+                    double Np2, double gamma1, double gamma2, double bbscale1, double bbscale2, int lane = 0,
+                    const double *fixed_h = nullptr)
 {
     const ss_work::pic_lane &k = w.pk[lane];
     hipStream_t st = lane == 0 ? c->stream : c->lane[lane];
@@ -530,8 +531,13 @@ static int pic_pair(st_ctx *c, ss_work &w, double *d_x1, int s1, int m1, double 
     a.count[0] = m1;
     a.count[1] = m2;
     a.part = k.part;
-    HIPCHK(st_ss_partials(a, st));
-    HIPCHK(st_pic_params(k.part, a.nb[0], a.nb[1], m1, m2, k.params, st));
+    if (fixed_h) {
+        /* BeamBeam4D_PIC_NonOffset: the grid is given, no moments */
+        HIPCHK(st_pic_fixed_params(k.params, fixed_h[0], fixed_h[1], st));
+    } else {
+        HIPCHK(st_ss_partials(a, st));
+        HIPCHK(st_pic_params(k.part, a.nb[0], a.nb[1], m1, m2, k.params, st));
+    }
     pic_args p{};
     p.x[0] = d_x1;
     p.x[1] = d_x2;
@@ -626,6 +632,26 @@ extern "C" int st_bb_pic_4d(st_ctx *c, double Np1, int Mp1, double *d_x1, double
     return ST_OK;
 }
 
+/* BeamBeam4D_PIC_NonOffset, simtrack.cpp:24226-24280 */
+extern "C" int st_bb_pic_4d_nonoffset(st_ctx *c, double Np1, int Mp1, double *d_x1, double Np2, int Mp2,
+                                      double *d_x2, double gamma1, double gamma2, double bbscale1, double bbscale2,
+                                      int Nx, int Ny, double hx, double hy)
+{
+    int rc = ss_check(c, Mp1, d_x1, Mp2, d_x2, "st_bb_pic_4d_nonoffset");
+    if (rc) return rc;
+    if (Nx != 64 || Ny != 64) return fail(ST_ERR_PARAM, "st_bb_pic_4d_nonoffset: the grid is 64 x 64 (Nx = Ny = 64)");
+    if (!(hx > 0. && hx < 1e300) || !(hy > 0. && hy < 1e300))
+        return fail(ST_ERR_PARAM, "st_bb_pic_4d_nonoffset: hx and hy must be positive and finite");
+    HIPCHK(hipSetDevice(c->device));
+    ss_work w;
+    if ((rc = ss_prepare(c, Mp1, Mp2, 1, 1, false, w, std::max(std::max(Mp1, Mp2), 1)))) return rc;
+    const double h[2] = {hx, hy};
+    if ((rc = pic_pair(c, w, d_x1, 0, Mp1, d_x2, 0, Mp2, Np1, Np2, gamma1, gamma2, bbscale1, bbscale2, 0, h)))
+        return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return ST_OK;
+}
+
 extern "C" int st_bb_pic_6d(st_ctx *c, double Np1, int Mp1, double *d_x1, double Np2, int Mp2, double *d_x2,
                             double gamma1, double gamma2, double bbscale1, double bbscale2, int Nslice1, int Nslice2,
                             double *Z1, double *Z2)
@@ -654,6 +680,26 @@ extern "C" int st_bb_pic_4d_host(st_ctx *c, double Np1, int Mp1, double *x1, dou
     if ((rc = ss_prepare(c, Mp1, Mp2, 1, 1, true, w, std::max(std::max(Mp1, Mp2), 1))) ||
         (rc = ss_host_in(c, w, Mp1, x1, Mp2, x2)) ||
         (rc = pic_pair(c, w, w.hx[0], 0, Mp1, w.hx[1], 0, Mp2, Np1, Np2, gamma1, gamma2, bbscale1, bbscale2)))
+        return rc;
+    return ss_host_out(c, w, Mp1, x1, Mp2, x2);
+}
+
+extern "C" int st_bb_pic_4d_nonoffset_host(st_ctx *c, double Np1, int Mp1, double *x1, double Np2, int Mp2,
+                                           double *x2, double gamma1, double gamma2, double bbscale1,
+                                           double bbscale2, int Nx, int Ny, double hx, double hy)
+{
+    int rc = ss_check(c, Mp1, x1, Mp2, x2, "st_bb_pic_4d_nonoffset_host");
+    if (rc) return rc;
+    if (Nx != 64 || Ny != 64)
+        return fail(ST_ERR_PARAM, "st_bb_pic_4d_nonoffset_host: the grid is 64 x 64 (Nx = Ny = 64)");
+    if (!(hx > 0. && hx < 1e300) || !(hy > 0. && hy < 1e300))
+        return fail(ST_ERR_PARAM, "st_bb_pic_4d_nonoffset_host: hx and hy must be positive and finite");
+    HIPCHK(hipSetDevice(c->device));
+    ss_work w;
+    const double h[2] = {hx, hy};
+    if ((rc = ss_prepare(c, Mp1, Mp2, 1, 1, true, w, std::max(std::max(Mp1, Mp2), 1))) ||
+        (rc = ss_host_in(c, w, Mp1, x1, Mp2, x2)) ||
+        (rc = pic_pair(c, w, w.hx[0], 0, Mp1, w.hx[1], 0, Mp2, Np1, Np2, gamma1, gamma2, bbscale1, bbscale2, 0, h)))
         return rc;
     return ss_host_out(c, w, Mp1, x1, Mp2, x2);
 }

@@ -57,6 +57,8 @@ hipError_t st_sort_cells(void *temp, size_t bytes, const int *kin, int *kout, co
                          hipStream_t stream);
 hipError_t st_pic_params(const double *part, int nb0, int nb1, int m0, int m1, double *params, hipStream_t stream);
 hipError_t st_pic_solve_and_kick(const pic_args &a, int s, hipStream_t stream);
+/* params of the fixed, origin-centred grid (BeamBeam4D_PIC_NonOffset) */
+hipError_t st_pic_fixed_params(double *params, double hx, double hy, hipStream_t stream);
 hipError_t st_pic_drift(double *x, int start, int m, double S, int sgn, hipStream_t stream);
 int st_ss_partial_blocks(int count);
 hipError_t st_ss_zsums(const double *x, const int *idx, const int *seg_start, const int *seg_count,

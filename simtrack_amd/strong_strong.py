@@ -89,6 +89,18 @@ def BeamBeam4D_PIC(ctx, Np1, x1, Np2, x2, gamma1, gamma2, bbscale1, bbscale2):
     check(f(ctx.h, Np1, n1, pa, Np2, n2, pb, gamma1, gamma2, bbscale1, bbscale2), "BeamBeam4D_PIC")
 
 
+def BeamBeam4D_PIC_NonOffset(ctx, Np1, x1, Np2, x2, gamma1, gamma2, bbscale1, bbscale2, Nx, Ny, hx, hy):
+    """BeamBeam4D_PIC_NonOffset (simtrack.cpp:24226): particle-in-cell kicks
+    on the given Nx x Ny grid of hx x hy cells centred on the origin, in
+    place.  The reference takes the grid's Green function from its caller;
+    it is computed here from hx, hy.  Nx = Ny = 64."""
+    lib = load()
+    pa, n1, pb, n2, dev = _pair(x1, x2, "BeamBeam4D_PIC_NonOffset")
+    f = lib.st_bb_pic_4d_nonoffset if dev else lib.st_bb_pic_4d_nonoffset_host
+    check(f(ctx.h, Np1, n1, pa, Np2, n2, pb, gamma1, gamma2, bbscale1, bbscale2, int(Nx), int(Ny), float(hx),
+            float(hy)), "BeamBeam4D_PIC_NonOffset")
+
+
 def BeamBeam6D_PIC(ctx, Np1, x1, Np2, x2, gamma1, gamma2, bbscale1, bbscale2, Nslice1, Nslice2):
     """BeamBeam6D_PIC (simtrack.cpp:23380): beams sorted by z in place, slice
     pairs collided by BeamBeam4D_PIC; returns the slice centres (Z1, Z2)."""

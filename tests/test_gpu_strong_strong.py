@@ -223,6 +223,27 @@ def test_pic_4d_matches_oracle(ctx):
     assert np.array_equal(d1.cpu().numpy(), h1) and np.array_equal(d2.cpu().numpy(), h2)
 
 
+def test_pic_4d_nonoffset_matches_oracle(ctx):
+    """BeamBeam4D_PIC_NonOffset: no moments, so every step follows the
+    oracle's operations in order: bit for bit, device and host entry, with
+    particles outside the 29-cell window left alone."""
+    b1 = _beam(20000, 41, sx=1e-4, sy=6e-5, off=(2e-5, -1e-5))
+    b2 = _beam(15000, 42, sx=8e-5, sy=5e-5, off=(-3e-5, 2e-5))
+    hx, hy = 5 * 8e-5 / 64, 5 * 5e-5 / 64  # a 29-cell window of 2.3 sigma: some particles outside
+    Np1, Np2, g1, g2, k1, k2 = ARGS
+    o1, o2 = pyoracle.bb_pic_4d_nonoffset(Np1, b1, Np2, b2, g1, g2, k1, k2, hx, hy)
+    h1, h2 = b1.copy(), b2.copy()
+    SS.BeamBeam4D_PIC_NonOffset(ctx, Np1, h1, Np2, h2, g1, g2, k1, k2, 64, 64, hx, hy)
+    assert_bitwise(h1, o1, "PIC 4-D non-offset beam 1")
+    assert_bitwise(h2, o2, "PIC 4-D non-offset beam 2")
+    d1 = torch.tensor(b1, device="cuda:0")
+    d2 = torch.tensor(b2, device="cuda:0")
+    SS.BeamBeam4D_PIC_NonOffset(ctx, Np1, d1, Np2, d2, g1, g2, k1, k2, 64, 64, hx, hy)
+    assert np.array_equal(d1.cpu().numpy(), h1) and np.array_equal(d2.cpu().numpy(), h2)
+    with pytest.raises(st.STError):
+        SS.BeamBeam4D_PIC_NonOffset(ctx, Np1, b1.copy(), Np2, b2.copy(), g1, g2, k1, k2, 128, 128, hx, hy)
+
+
 def test_pic_6d_matches_oracle(ctx):
     b1, b2 = _beam(9001, 25), _beam(8003, 26, off=(2e-5, 1e-5))
     Np1, Np2, g1, g2, k1, k2 = ARGS

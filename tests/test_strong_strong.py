@@ -239,6 +239,47 @@ def test_pic_kick_is_the_gaussian_field():
         assert err < 0.01, (col, err)
 
 
+def test_pic_nonoffset_potential_interpolation_is_the_grid_field():
+    """BeamBeam4D_PIC_NonOffset's field from a 5 x 5 patch of the potential
+    (Interpolate_Electric_Field_1Potential, cpp:3047) equals the full-grid
+    field interpolated (Cal_Electric_Field + Interpolate_Electric_Field) for
+    every particle the 29-cell window admits: bit for bit, so a shifted or
+    transposed patch index fails.  Beams off the origin put some particles
+    outside the window (left unkicked)."""
+    b1 = _beam(20000, 31, sx=1e-4, sy=6e-5, off=(2e-5, -1e-5))
+    b2 = _beam(15000, 32, sx=8e-5, sy=5e-5, off=(-3e-5, 2e-5))
+    hx, hy = 5 * 8e-5 / 64, 5 * 5e-5 / 64  # a 29-cell window of 2.3 sigma: some particles outside
+    args = (1e11, b1, 1.2e11, b2, 200., 180., 1., 0.9, hx, hy)
+    n1, n2 = pyoracle.bb_pic_4d_nonoffset(*args)
+    f1, f2 = pyoracle.bb_pic_4d_fixed_gridfield(*args)
+    assert np.array_equal(n1, f1) and np.array_equal(n2, f2)
+    out = (np.abs(b2[:, 0]) >= 29 * hx) | (np.abs(b2[:, 2]) >= 29 * hy)
+    assert out.sum() > 10 and np.array_equal(n2[out], b2[out])
+    assert (n2[~out][:, [1, 3]] != b2[~out][:, [1, 3]]).all(axis=1).mean() > 0.99
+
+
+def test_pic_nonoffset_kick_is_the_gaussian_field():
+    """On its fixed grid (h = 15 x 0.8 rms / 64, centred on the origin) the
+    non-offset kick of a centred Gaussian source is its Gaussian field
+    smoothed by 5 h^2 / 6, as for BeamBeam4D_PIC (above); the Np/Mp charge
+    per macroparticle and the kick's constants are pinned by the same
+    comparison."""
+    src = _beam(200000, 33, sx=1e-4, sy=6e-5)
+    probe = _beam(2000, 34, sx=1e-4, sy=6e-5)
+    N, g = 1e11, 200.
+    hx, hy = 15 * 0.8e-4 / 64, 15 * 0.8 * 6e-5 / 64
+    _, p2 = pyoracle.bb_pic_4d_nonoffset(N, src, N, probe, g, g, 1., 1., hx, hy)
+    sx, sy = src[:, 0].std(), src[:, 2].std()
+    sm = 5. / 6.
+    ex = np.array([pyoracle.bb4d(x, y, g, N, np.sqrt(sx ** 2 + sm * hx ** 2), np.sqrt(sy ** 2 + sm * hy ** 2))
+                   for x, y in probe[:, [0, 2]]])
+    inside = (np.abs(probe[:, 0]) < 29 * hx) & (np.abs(probe[:, 2]) < 29 * hy)
+    for col, k in ((1, 0), (3, 1)):
+        kick = (p2[:, col] - probe[:, col])[inside]
+        err = np.sqrt(np.mean((kick - ex[inside, k]) ** 2)) / np.sqrt(np.mean(ex[inside, k] ** 2))
+        assert err < 0.01, (col, err)
+
+
 def test_pic_6d_head_on_single_slice_is_4d():
     b1, b2 = _beam(3000, 8), _beam(2500, 9, off=(1e-5, 2e-5))
     b1[:, 4] = 0.
